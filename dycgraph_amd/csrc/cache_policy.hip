@@ -1,0 +1,261 @@
+// cache_policy.hip -- device chunk cache over the pinned host chunk store.
+//
+// The cache holds whole 65,536-edge chunks in HBM frames.  Kernels resolve a
+// slot through chunk_frame[]: a resident chunk is read from its frame, any
+// other chunk zero-copy from pinned host memory.  Writes to a resident chunk
+// mark it dirty; eviction writes it back.
+//
+// HOTNESS policy (the reference's plain policy, driver.cuh:486-579 and
+// framework.cuh:1264-1567, at chunk granularity): score = edges read over the
+// last four snapshots, ties by most recent use (LRU), then by chunk id; the
+// top `frames` chunks are admitted.  A refresh happens only when the admitted
+// set differs from the resident set (cache_refresh_gate.h).
+#include <algorithm>
+#include <numeric>
+
+#include "engine.hpp"
+#include "mlp.hpp"
+
+namespace dcg {
+
+// Copy whole chunks between HBM frames and pinned host memory: one workgroup
+// per chunk, 16-byte vector accesses.
+__global__ __launch_bounds__(256) void k_move_chunks(uint32_t *frames, uint32_t *host, const int64_t *list,
+                                                     uint32_t count, int to_host) {
+    for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
+        const int64_t frame = list[2 * i], chunk = list[2 * i + 1];
+        uint4 *f = reinterpret_cast<uint4 *>(frames + ((uint64_t)frame << kChunkLog2));
+        uint4 *h = reinterpret_cast<uint4 *>(host + ((uint64_t)chunk << kChunkLog2));
+        constexpr uint32_t kVec = kChunkEdges / 4;
+        if (to_host) {
+            for (uint32_t j = threadIdx.x; j < kVec; j += blockDim.x) h[j] = f[j];
+        } else {
+            for (uint32_t j = threadIdx.x; j < kVec; j += blockDim.x) f[j] = h[j];
+        }
+    }
+}
+
+// hist[c] = (reads this snapshot, previous three), reads reset afterwards.
+__global__ void k_shift_hist(float *hist, float *reads, uint64_t chunks, float *score) {
+    for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < chunks; c += (uint64_t)gridDim.x * blockDim.x) {
+        float4 h = reinterpret_cast<float4 *>(hist)[c];
+        h.w = h.z;
+        h.z = h.y;
+        h.y = h.x;
+        h.x = reads[c];
+        reinterpret_cast<float4 *>(hist)[c] = h;
+        score[c] = h.x + h.y + h.z + h.w;
+    }
+}
+
+void Engine::cache_init() {
+    const uint64_t C = cache.chunks;
+    cache.d_chunk_frame.alloc(C);
+    cache.chunk_frame.assign(C, -1);
+    cache.frame_chunk.assign(cache.frames, -1);
+    cache.last_use.assign(C, 0);
+    cache.d_dirty.alloc(C);
+    fill<uint8_t>(stream, cache.d_dirty.p, C, (uint8_t)0);
+    cache.d_reads.alloc(2 * C);
+    fill<float>(stream, cache.d_reads.p, 2 * C, 0.f);
+    cache.d_hist.alloc(4 * C);
+    fill<float>(stream, cache.d_hist.p, 4 * C, 0.f);
+    cache.d_touch.alloc(C);
+    fill<float>(stream, cache.d_touch.p, C, 0.f);
+    cache.d_score.alloc(C);
+    if (cfg.placement == DCG_PLACE_DEVICE) {
+        for (uint64_t c = 0; c < C; ++c) cache.chunk_frame[c] = (int32_t)c;
+        cache.frame_chunk.resize(C);
+        for (uint64_t c = 0; c < C; ++c) cache.frame_chunk[c] = (int64_t)c;
+    } else if (cache.frames) {
+        // Initial admission: the lowest chunks.  The load places rows by
+        // descending capacity, so these hold the highest-degree rows (the
+        // reference's initial LoadCache admits by hotness after Start; the
+        // first maintenance call replaces this with the measured ranking).
+        std::vector<int64_t> list;
+        for (uint64_t f = 0; f < cache.frames; ++f) {
+            cache.chunk_frame[f] = (int32_t)f;
+            cache.frame_chunk[f] = (int64_t)f;
+            list.push_back((int64_t)f);
+            list.push_back((int64_t)f);
+        }
+        DevBuf<int64_t> dl;
+        dl.alloc(list.size());
+        DCG_HIP(hipMemcpyAsync(dl.p, list.data(), list.size() * 8, hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL(k_move_chunks, dim3(grid_for(cache.frames, 1, 4096)), dim3(256), 0, stream,
+                           cache.d_frames.p, h_slab, dl.p, (uint32_t)cache.frames, 0);
+        DCG_HIP(hipGetLastError());
+        cache.h2d += cache.frames * kChunkEdges * 4;
+        sync();
+    }
+    DCG_HIP(hipMemcpyAsync(cache.d_chunk_frame.p, cache.chunk_frame.data(), C * 4, hipMemcpyHostToDevice, stream));
+    sync();
+}
+
+// Make the resident set equal to `desired` (chunk ids, at most `frames`).
+void Engine::cache_apply_plan(const std::vector<uint64_t> &desired, dcg_cache_stats *st) {
+    const uint64_t C = cache.chunks;
+    std::vector<uint8_t> want(C, 0);
+    for (uint64_t c : desired) want[c] = 1;
+    std::vector<int64_t> evict_list, free_frames;
+    for (uint64_t f = 0; f < cache.frames; ++f) {
+        const int64_t c = cache.frame_chunk[f];
+        if (c < 0) {
+            free_frames.push_back((int64_t)f);
+        } else if (!want[c]) {
+            evict_list.push_back((int64_t)f);
+            evict_list.push_back(c);
+        }
+    }
+    std::vector<int64_t> load_list;
+    std::vector<uint64_t> to_load;
+    for (uint64_t c : desired)
+        if (cache.chunk_frame[c] < 0) to_load.push_back(c);
+    if (evict_list.empty() && to_load.empty()) return;
+    // dirty evicted frames go back to host memory
+    std::vector<uint8_t> dirty(C, 0);
+    DCG_HIP(hipMemcpyAsync(dirty.data(), cache.d_dirty.p, C, hipMemcpyDeviceToHost, stream));
+    sync();
+    std::vector<int64_t> wb;
+    for (size_t i = 0; i < evict_list.size(); i += 2) {
+        const int64_t f = evict_list[i], c = evict_list[i + 1];
+        if (dirty[c]) {
+            wb.push_back(f);
+            wb.push_back(c);
+            dirty[c] = 0;
+        }
+        cache.chunk_frame[c] = -1;
+        cache.frame_chunk[f] = -1;
+        free_frames.push_back(f);
+    }
+    size_t k = 0;
+    for (uint64_t c : to_load) {
+        if (k >= free_frames.size()) break;
+        const int64_t f = free_frames[k++];
+        cache.chunk_frame[c] = (int32_t)f;
+        cache.frame_chunk[f] = (int64_t)c;
+        load_list.push_back(f);
+        load_list.push_back((int64_t)c);
+    }
+    DevBuf<int64_t> dl;
+    dl.alloc(std::max<size_t>(wb.size() + load_list.size(), 2));
+    if (!wb.empty()) {
+        DCG_HIP(hipMemcpyAsync(dl.p, wb.data(), wb.size() * 8, hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL(k_move_chunks, dim3(grid_for(wb.size() / 2, 1, 4096)), dim3(256), 0, stream,
+                           cache.d_frames.p, h_slab, dl.p, (uint32_t)(wb.size() / 2), 1);
+        DCG_HIP(hipGetLastError());
+    }
+    if (!load_list.empty()) {
+        DCG_HIP(hipMemcpyAsync(dl.p + wb.size(), load_list.data(), load_list.size() * 8, hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL(k_move_chunks, dim3(grid_for(load_list.size() / 2, 1, 4096)), dim3(256), 0, stream,
+                           cache.d_frames.p, h_slab, dl.p + wb.size(), (uint32_t)(load_list.size() / 2), 0);
+        DCG_HIP(hipGetLastError());
+    }
+    DCG_HIP(hipMemcpyAsync(cache.d_dirty.p, dirty.data(), C, hipMemcpyHostToDevice, stream));
+    DCG_HIP(hipMemcpyAsync(cache.d_chunk_frame.p, cache.chunk_frame.data(), C * 4, hipMemcpyHostToDevice, stream));
+    sync();
+    const uint64_t wbn = wb.size() / 2, ln = load_list.size() / 2;
+    cache.written_back += wbn;
+    cache.evicted += evict_list.size() / 2;
+    cache.loaded += ln;
+    cache.d2h += wbn * kChunkEdges * 4;
+    cache.h2d += ln * kChunkEdges * 4;
+    if (st) {
+        st->evicted_chunks += evict_list.size() / 2;
+        st->loaded_chunks += ln;
+        st->written_back_chunks += wbn;
+        st->h2d_bytes += ln * kChunkEdges * 4;
+        st->d2h_bytes += wbn * kChunkEdges * 4;
+    }
+}
+
+static void fill_common(Engine &e, dcg_cache_stats *st) {
+    if (!st) return;
+    st->frames = e.cache.frames;
+    st->total_chunks = e.cache.chunks;
+    uint64_t r = 0;
+    for (uint64_t c = 0; c < e.cache.chunks; ++c) r += e.cache.chunk_frame[c] >= 0;
+    st->resident_chunks = r;
+}
+
+// Ranking used by the plain policy: score desc, last use desc, chunk asc.
+static std::vector<uint64_t> rank_top(const std::vector<float> &score, const std::vector<uint32_t> &last_use,
+                                      uint64_t k) {
+    std::vector<uint64_t> idx(score.size());
+    std::iota(idx.begin(), idx.end(), 0ull);
+    auto better = [&](uint64_t a, uint64_t b) {
+        if (score[a] != score[b]) return score[a] > score[b];
+        if (last_use[a] != last_use[b]) return last_use[a] > last_use[b];
+        return a < b;
+    };
+    if (k < idx.size()) {
+        std::nth_element(idx.begin(), idx.begin() + (ptrdiff_t)k, idx.end(), better);
+        idx.resize(k);
+    }
+    std::sort(idx.begin(), idx.end(), better);
+    return idx;
+}
+
+void Engine::cache_maintain(dcg_cache_stats *st) {
+    const double t0 = now_ms();
+    if (st) *st = dcg_cache_stats{};
+    snapshots++;
+    if (cfg.placement == DCG_PLACE_DEVICE || cache.frames == 0) {
+        fill<float>(stream, cache.d_reads.p, 2 * cache.chunks, 0.f);
+        fill_common(*this, st);
+        if (st) st->ms = now_ms() - t0;
+        return;
+    }
+    const uint64_t C = cache.chunks;
+    std::vector<float> reads(C);
+    DCG_HIP(hipMemcpyAsync(reads.data(), cache.d_reads.p, C * 4, hipMemcpyDeviceToHost, stream));
+    hipLaunchKernelGGL(k_shift_hist, dim3(grid_for(C)), dim3(256), 0, stream, cache.d_hist.p, cache.d_reads.p, C,
+                       cache.d_score.p);
+    std::vector<float> score(C);
+    DCG_HIP(hipMemcpyAsync(score.data(), cache.d_score.p, C * 4, hipMemcpyDeviceToHost, stream));
+    sync();
+    for (uint64_t c = 0; c < C; ++c)
+        if (reads[c] > 0) cache.last_use[c] = (uint32_t)snapshots;
+    if (cfg.cache_policy == DCG_CACHE_LEARNED && cache.mlp) {
+        mlp_observe(*this, reads);   // online training on this snapshot's labels
+    }
+    if (cfg.cache_policy != DCG_CACHE_NONE) {
+        std::vector<uint64_t> desired = rank_top(score, cache.last_use, cache.frames);
+        // chunks never read keep their frames rather than displacing nothing
+        std::vector<uint64_t> keep;
+        keep.reserve(desired.size());
+        for (uint64_t c : desired)
+            if (score[c] > 0) keep.push_back(c);
+        if (keep.size() < cache.frames) {
+            std::vector<uint8_t> in(C, 0);
+            for (uint64_t c : keep) in[c] = 1;
+            for (uint64_t f = 0; f < cache.frames && keep.size() < cache.frames; ++f) {
+                const int64_t c = cache.frame_chunk[f];
+                if (c >= 0 && !in[c]) {
+                    keep.push_back((uint64_t)c);
+                    in[c] = 1;
+                }
+            }
+        }
+        cache_apply_plan(keep, st);
+    }
+    fill<float>(stream, cache.d_reads.p, 2 * C, 0.f);
+    fill_common(*this, st);
+    if (st) st->ms = now_ms() - t0;
+}
+
+void Engine::cache_prefetch(uint64_t ab, uint64_t ac, uint64_t db, uint64_t dc, dcg_cache_stats *st) {
+    const double t0 = now_ms();
+    if (st) *st = dcg_cache_stats{};
+    if (cfg.placement == DCG_PLACE_DEVICE || cache.frames == 0 || cfg.cache_policy != DCG_CACHE_LEARNED) {
+        fill_common(*this, st);
+        return;
+    }
+    if (!cache.mlp) mlp_create(*this);
+    std::vector<uint64_t> desired = mlp_plan(*this, ab, ac, db, dc);
+    cache_apply_plan(desired, st);
+    fill_common(*this, st);
+    if (st) st->ms = now_ms() - t0;
+}
+
+}  // namespace dcg

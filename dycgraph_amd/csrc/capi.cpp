@@ -1,0 +1,346 @@
+// capi.cpp -- C ABI of the engine (include/dcg.h).  Every entry point catches
+// exceptions and returns a status; nothing aborts across the ABI.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "engine.hpp"
+
+using namespace dcg;
+
+struct dcg_engine {
+    Engine *e;
+    std::string err;
+};
+
+namespace dcg {
+
+Engine::Engine(const dcg_config &c) : cfg(c) {
+    device = c.device;
+    DCG_HIP(hipSetDevice(device));
+    DCG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    hipDeviceProp_t prop;
+    DCG_HIP(hipGetDeviceProperties(&prop, device));
+    cu_count = prop.multiProcessorCount;
+    DCG_HIP(hipHostMalloc((void **)&h_ctl, 8 * sizeof(Ctl), hipHostMallocDefault));
+    DCG_HIP(hipHostMalloc((void **)&h_counters, 16 * sizeof(unsigned long long), hipHostMallocDefault));
+    app = c.app;
+    Q = (app == DCG_APP_CC || app == DCG_APP_PR) ? 1u : std::max(1u, c.num_queries);
+    if (c.sources && app != DCG_APP_CC && app != DCG_APP_PR) sources.assign(c.sources, c.sources + Q);
+    else sources.assign(Q, 0u);
+    wp.seed = c.weight_seed;
+    wp.wmax = c.weight_max ? c.weight_max : 1;
+    if (app == DCG_APP_BFS) wp.kind = W_UNIT;
+    else if (app == DCG_APP_CC) wp.kind = W_ZERO;
+    else wp.kind = c.weight_mode == DCG_W_SEEDED ? W_SEEDED : W_REFERENCE;
+}
+
+Engine::~Engine() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (h_slab) (void)hipHostFree(h_slab);
+    if (h_ctl) (void)hipHostFree(h_ctl);
+    if (h_counters) (void)hipHostFree(h_counters);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+}  // namespace dcg
+
+template <typename F>
+static int guarded(dcg_engine *h, F &&f) {
+    if (!h || !h->e) return DCG_ERR_ARG;
+    try {
+        f(*h->e);
+        h->err.clear();
+        return DCG_OK;
+    } catch (const Error &x) {
+        h->err = x.what();
+        return x.status;
+    } catch (const std::bad_alloc &) {
+        h->err = "host allocation failed";
+        return DCG_ERR_CAPACITY;
+    } catch (const std::exception &x) {
+        h->err = x.what();
+        return DCG_ERR_STATE;
+    }
+}
+
+extern "C" {
+
+int dcg_config_init(dcg_config *cfg) {
+    if (!cfg) return DCG_ERR_ARG;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->app = DCG_APP_SSSP;
+    cfg->num_queries = 1;
+    cfg->weight_mode = DCG_W_REFERENCE;
+    cfg->weight_max = 100;
+    cfg->weight_seed = 1;
+    cfg->placement = DCG_PLACE_DEVICE;
+    cfg->cache_policy = DCG_CACHE_HOTNESS;
+    cfg->cache_bytes = 4000000000ull;
+    cfg->store_slack = 0.125;
+    cfg->pr_base = 0.15;
+    cfg->pr_epsilon = 1e-6;
+    cfg->pr_max_rounds = 100;
+    cfg->block_size = 256;
+    cfg->device = 0;
+    return DCG_OK;
+}
+
+const char *dcg_status_string(int status) {
+    switch (status) {
+    case DCG_OK: return "ok";
+    case DCG_ERR_ARG: return "invalid argument";
+    case DCG_ERR_HIP: return "HIP runtime error";
+    case DCG_ERR_STATE: return "invalid engine state";
+    case DCG_ERR_CAPACITY: return "capacity exhausted";
+    case DCG_ERR_NUMERIC: return "non-finite PageRank state";
+    case DCG_ERR_UNSUPPORTED: return "unsupported";
+    default: return "unknown status";
+    }
+}
+
+int dcg_create(const dcg_config *cfg, dcg_engine **out) {
+    if (!cfg || !out) return DCG_ERR_ARG;
+    *out = nullptr;
+    if (cfg->app > DCG_APP_PR || cfg->placement > DCG_PLACE_HOST || cfg->cache_policy > DCG_CACHE_LEARNED)
+        return DCG_ERR_ARG;
+    if (cfg->app == DCG_APP_PR) {
+        // samples/hybrid_pr/hybrid_pr.cu:133-137
+        if (!std::isfinite(cfg->pr_epsilon) || cfg->pr_epsilon <= 0 || (float)cfg->pr_epsilon == 0.f ||
+            cfg->pr_max_rounds == 0 || cfg->pr_max_rounds > 100 || !(cfg->pr_base > 0))
+            return DCG_ERR_ARG;
+    }
+    if (cfg->weight_mode == DCG_W_SEEDED && cfg->weight_max == 0) return DCG_ERR_ARG;
+    if ((cfg->app == DCG_APP_SSSP || cfg->app == DCG_APP_BFS) && cfg->num_queries > 1 && !cfg->sources)
+        return DCG_ERR_ARG;
+    dcg_engine *h = new (std::nothrow) dcg_engine();
+    if (!h) return DCG_ERR_CAPACITY;
+    try {
+        h->e = new Engine(*cfg);
+    } catch (const Error &x) {
+        delete h;
+        return x.status;
+    }
+    *out = h;
+    return DCG_OK;
+}
+
+void dcg_destroy(dcg_engine *h) {
+    if (!h) return;
+    delete h->e;
+    delete h;
+}
+
+const char *dcg_last_error(const dcg_engine *h) { return h ? h->err.c_str() : "null engine"; }
+
+int dcg_load_edges_device(dcg_engine *h, uint32_t num_nodes, const uint32_t *d_src, const uint32_t *d_dst,
+                          uint64_t num_edges) {
+    return guarded(h, [&](Engine &e) {
+        if (e.loaded) throw Error(DCG_ERR_STATE, "graph already loaded");
+        if (num_nodes == 0 || (num_edges && (!d_src || !d_dst))) throw Error(DCG_ERR_ARG, "bad graph arguments");
+        e.load_edges_device(num_nodes, d_src, d_dst, num_edges);
+        e.alloc_state();
+    });
+}
+
+int dcg_load_edges(dcg_engine *h, uint32_t num_nodes, const uint32_t *src, const uint32_t *dst, uint64_t num_edges) {
+    return guarded(h, [&](Engine &e) {
+        if (e.loaded) throw Error(DCG_ERR_STATE, "graph already loaded");
+        if (num_nodes == 0 || (num_edges && (!src || !dst))) throw Error(DCG_ERR_ARG, "bad graph arguments");
+        DevBuf<uint32_t> s, d;
+        s.alloc(num_edges ? num_edges : 1);
+        d.alloc(num_edges ? num_edges : 1);
+        if (num_edges) {
+            DCG_HIP(hipMemcpyAsync(s.p, src, num_edges * 4, hipMemcpyHostToDevice, e.stream));
+            DCG_HIP(hipMemcpyAsync(d.p, dst, num_edges * 4, hipMemcpyHostToDevice, e.stream));
+        }
+        e.load_edges_device(num_nodes, s.p, d.p, num_edges);
+        e.alloc_state();
+    });
+}
+
+int dcg_load_csr(dcg_engine *h, uint32_t num_nodes, const uint64_t *row_ptr, const uint32_t *col) {
+    if (!row_ptr) return DCG_ERR_ARG;
+    const uint64_t m = row_ptr[num_nodes];
+    std::vector<uint32_t> src(m);
+    for (uint32_t u = 0; u < num_nodes; ++u)
+        for (uint64_t k = row_ptr[u]; k < row_ptr[u + 1]; ++k) src[k] = u;
+    return dcg_load_edges(h, num_nodes, src.data(), col, m);
+}
+
+int dcg_start(dcg_engine *h) {
+    return guarded(h, [&](Engine &e) {
+        if (!e.loaded) throw Error(DCG_ERR_STATE, "dcg_start before a graph load");
+        // source clamp of hybrid_sssp.cu:391
+        for (auto &s : e.sources) s = std::min(s, e.n - 1);
+        e.d_sources.alloc(e.sources.size());
+        DCG_HIP(hipMemcpyAsync(e.d_sources.p, e.sources.data(), e.sources.size() * 4, hipMemcpyHostToDevice, e.stream));
+        if (e.app == DCG_APP_PR) e.start_pagerank();
+        else e.start_min_lattice();
+        e.started = true;
+    });
+}
+
+int dcg_load_updates(dcg_engine *h, const dcg_edge *adds, uint64_t num_adds, const dcg_edge *dels, uint64_t num_dels) {
+    return guarded(h, [&](Engine &e) {
+        if ((num_adds && !adds) || (num_dels && !dels)) throw Error(DCG_ERR_ARG, "null update array");
+        e.d_adds.alloc(num_adds ? num_adds : 1);
+        e.d_dels.alloc(num_dels ? num_dels : 1);
+        if (num_adds)
+            DCG_HIP(hipMemcpyAsync(e.d_adds.p, adds, num_adds * sizeof(dcg_edge), hipMemcpyHostToDevice, e.stream));
+        if (num_dels)
+            DCG_HIP(hipMemcpyAsync(e.d_dels.p, dels, num_dels * sizeof(dcg_edge), hipMemcpyHostToDevice, e.stream));
+        e.n_adds = num_adds;
+        e.n_dels = num_dels;
+        e.sync();
+    });
+}
+
+static void check_slice(uint64_t begin, uint64_t count, uint64_t total) {
+    if (begin > total || count > total - begin) throw Error(DCG_ERR_ARG, "update slice out of range");
+}
+
+int dcg_del_edge(dcg_engine *h, uint64_t begin, uint64_t count, dcg_phase_stats *st) {
+    return guarded(h, [&](Engine &e) {
+        if (!e.started) throw Error(DCG_ERR_STATE, "dcg_del_edge before dcg_start");
+        if (e.app == DCG_APP_PR) throw Error(DCG_ERR_STATE, "PageRank batches use dcg_update_pagerank");
+        check_slice(begin, count, e.n_dels);
+        if (st) *st = dcg_phase_stats{};
+        const double t0 = now_ms();
+        e.delete_phase(begin, count, st);
+        e.fold_reads(st);
+        if (st) {
+            st->requests = count;
+            st->ms = now_ms() - t0;
+        }
+    });
+}
+
+int dcg_add_edge(dcg_engine *h, uint64_t begin, uint64_t count, dcg_phase_stats *st) {
+    return guarded(h, [&](Engine &e) {
+        if (!e.started) throw Error(DCG_ERR_STATE, "dcg_add_edge before dcg_start");
+        if (e.app == DCG_APP_PR) throw Error(DCG_ERR_STATE, "PageRank batches use dcg_update_pagerank");
+        check_slice(begin, count, e.n_adds);
+        if (st) *st = dcg_phase_stats{};
+        const double t0 = now_ms();
+        e.add_phase(begin, count, st);
+        e.fold_reads(st);
+        if (st) {
+            st->requests = count;
+            st->ms = now_ms() - t0;
+        }
+    });
+}
+
+int dcg_update_pagerank(dcg_engine *h, uint64_t add_begin, uint64_t add_count, uint64_t del_begin, uint64_t del_count,
+                        dcg_phase_stats *st) {
+    return guarded(h, [&](Engine &e) {
+        if (!e.started) throw Error(DCG_ERR_STATE, "dcg_update_pagerank before dcg_start");
+        if (e.app != DCG_APP_PR) throw Error(DCG_ERR_STATE, "dcg_update_pagerank needs the PR app");
+        check_slice(add_begin, add_count, e.n_adds);
+        check_slice(del_begin, del_count, e.n_dels);
+        if (st) *st = dcg_phase_stats{};
+        const double t0 = now_ms();
+        e.pagerank_batch(add_begin, add_count, del_begin, del_count, st);
+        e.fold_reads(st);
+        if (st) st->ms = now_ms() - t0;
+    });
+}
+
+int dcg_cache_maintain(dcg_engine *h, dcg_cache_stats *st) {
+    return guarded(h, [&](Engine &e) {
+        if (!e.started) throw Error(DCG_ERR_STATE, "cache maintenance before dcg_start");
+        e.cache_maintain(st);
+    });
+}
+
+int dcg_cache_prefetch(dcg_engine *h, uint64_t ab, uint64_t ac, uint64_t db, uint64_t dc, dcg_cache_stats *st) {
+    return guarded(h, [&](Engine &e) {
+        if (!e.started) throw Error(DCG_ERR_STATE, "cache prefetch before dcg_start");
+        check_slice(ab, ac, e.n_adds);
+        check_slice(db, dc, e.n_dels);
+        e.cache_prefetch(ab, ac, db, dc, st);
+    });
+}
+
+int dcg_cache_policy_set(dcg_engine *h, uint32_t policy) {
+    return guarded(h, [&](Engine &e) {
+        if (policy > DCG_CACHE_LEARNED) throw Error(DCG_ERR_ARG, "unknown cache policy");
+        e.cfg.cache_policy = policy;
+    });
+}
+
+int dcg_gather_values(dcg_engine *h, uint32_t query, void *out, size_t n) {
+    return guarded(h, [&](Engine &e) {
+        if (!e.started) throw Error(DCG_ERR_STATE, "gather before dcg_start");
+        if (!out || n != e.n || query >= e.Q) throw Error(DCG_ERR_ARG, "gather size/query mismatch");
+        if (e.app == DCG_APP_PR) {
+            DCG_HIP(hipMemcpyAsync(out, e.d_rank.p, n * 4, hipMemcpyDeviceToHost, e.stream));
+            e.sync();
+            return;
+        }
+        std::vector<uint64_t> tmp(n);
+        DCG_HIP(hipMemcpyAsync(tmp.data(), e.d_st.p + (uint64_t)query * e.n, n * 8, hipMemcpyDeviceToHost, e.stream));
+        e.sync();
+        uint32_t *o = static_cast<uint32_t *>(out);
+        for (size_t i = 0; i < n; ++i) o[i] = (uint32_t)(tmp[i] >> 32);
+    });
+}
+
+int dcg_gather_parents(dcg_engine *h, uint32_t query, uint32_t *out, size_t n) {
+    return guarded(h, [&](Engine &e) {
+        if (!e.started) throw Error(DCG_ERR_STATE, "gather before dcg_start");
+        if (!out || n != e.n || query >= e.Q) throw Error(DCG_ERR_ARG, "gather size/query mismatch");
+        if (e.app == DCG_APP_PR) throw Error(DCG_ERR_UNSUPPORTED, "PageRank has no parents");
+        std::vector<uint64_t> tmp(n);
+        DCG_HIP(hipMemcpyAsync(tmp.data(), e.d_st.p + (uint64_t)query * e.n, n * 8, hipMemcpyDeviceToHost, e.stream));
+        e.sync();
+        for (size_t i = 0; i < n; ++i) out[i] = (uint32_t)tmp[i];
+    });
+}
+
+int dcg_gather_buffers(dcg_engine *h, uint32_t query, void *out, size_t n) {
+    if (h && h->e && h->e->app != DCG_APP_PR) return dcg_gather_values(h, query, out, n);
+    return guarded(h, [&](Engine &e) {
+        if (!e.started) throw Error(DCG_ERR_STATE, "gather before dcg_start");
+        if (!out || n != e.n || query >= e.Q) throw Error(DCG_ERR_ARG, "gather size/query mismatch");
+        DCG_HIP(hipMemcpyAsync(out, e.d_res.p, n * 4, hipMemcpyDeviceToHost, e.stream));
+        e.sync();
+    });
+}
+
+int dcg_get_csr(dcg_engine *h, uint64_t *row_ptr, uint32_t *col, uint64_t *num_edges) {
+    return guarded(h, [&](Engine &e) {
+        if (!e.loaded) throw Error(DCG_ERR_STATE, "no graph loaded");
+        e.export_csr(row_ptr, col, num_edges);
+    });
+}
+
+int dcg_num_nodes(const dcg_engine *h, uint32_t *n) {
+    if (!h || !h->e || !n) return DCG_ERR_ARG;
+    *n = h->e->n;
+    return DCG_OK;
+}
+
+int dcg_get_info(dcg_engine *h, dcg_engine_info *info) {
+    return guarded(h, [&](Engine &e) {
+        if (!info) throw Error(DCG_ERR_ARG, "null info");
+        std::memset(info, 0, sizeof(*info));
+        info->snapshots = e.snapshots;
+        info->store_slots = e.total_slots;
+        info->store_used_slots = e.used_slots;
+        info->chunks = e.cache.chunks;
+        info->frames = e.cache.frames;
+        info->host_bytes = e.h_slab ? e.total_slots * 4 : 0;
+        info->device_bytes = e.cache.d_frames.bytes() + e.d_out.bytes() + e.d_in.bytes() + e.d_st.bytes() +
+                             e.d_tick.bytes() + e.d_qa.bytes() * 3 + e.d_aff.bytes() + e.d_big.bytes();
+        info->cached_edge_reads = e.cum_cached_reads;
+        info->host_edge_reads = e.cum_host_reads;
+    });
+}
+
+int dcg_sync(dcg_engine *h) {
+    return guarded(h, [&](Engine &e) { e.sync(); });
+}
+
+}  // extern "C"

@@ -1,0 +1,167 @@
+// dcg_internal.hpp -- shared device structures and wave64 primitives of the
+// MI355X incremental streaming-graph engine (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/dcg.h"
+
+namespace dcg {
+
+constexpr uint32_t kInf = 0xFFFFFFFFu;
+constexpr uint64_t kInfState = 0xFFFFFFFFFFFFFFFFull;   // (value INF, parent INF)
+constexpr uint32_t kChunkLog2 = 16;                       // 65,536-edge chunks
+constexpr uint64_t kChunkEdges = 1ull << kChunkLog2;
+constexpr uint32_t kMinRowCap = 4;                        // source_local_chunk_store.h:133-145
+constexpr uint32_t kWave = 64;
+constexpr uint32_t kBlock = 256;                          // 4 waves per workgroup
+constexpr uint32_t kBigRow = 256;                         // rows above this go to the piece list
+constexpr uint32_t kPiece = 512;                          // edges per big-row piece
+constexpr int kClasses = 40;                              // power-of-two capacity classes
+
+// One adjacency row: a power-of-two block of slots (cap == 0 for an empty row).
+// A block of <= 65,536 slots is aligned to its capacity, so it never crosses a
+// chunk; larger blocks are whole, aligned runs of chunks.
+struct RowDesc {
+    uint64_t off;
+    uint32_t deg;
+    uint32_t cap;
+};
+
+// Read/write view of the chunk store passed to kernels by value.
+struct StoreView {
+    RowDesc *out;              // n out-rows
+    RowDesc *in;               // n in-rows
+    uint32_t *host;            // pinned host slots (device-visible), null when device-placed
+    uint32_t *frames;          // device chunk frames (frame f holds slots [f<<16, (f+1)<<16) of its chunk)
+    const int32_t *chunk_frame;// chunk -> frame, -1 when not resident
+    uint8_t *dirty;            // chunk written while resident (write-back on eviction)
+    uint32_t *hot_out;         // per-vertex out-row expansions this snapshot
+    uint32_t *hot_in;          // per-vertex in-row expansions this snapshot
+};
+
+__device__ __forceinline__ const uint32_t *row_base(const StoreView &s, uint64_t off) {
+    const uint32_t c = (uint32_t)(off >> kChunkLog2);
+    const int32_t f = s.chunk_frame[c];
+    return f >= 0 ? s.frames + (((uint64_t)f) << kChunkLog2) + (off & (kChunkEdges - 1))
+                  : s.host + off;
+}
+
+__device__ __forceinline__ uint32_t *row_base_w(const StoreView &s, uint64_t off) {
+    const uint32_t c = (uint32_t)(off >> kChunkLog2);
+    const int32_t f = s.chunk_frame[c];
+    if (f >= 0) {
+        if (s.host) s.dirty[c] = 1;
+        return s.frames + (((uint64_t)f) << kChunkLog2) + (off & (kChunkEdges - 1));
+    }
+    return s.host + off;
+}
+
+// Weight policy: REFERENCE (u+v)%128+1 (api.cuh:26-28), SEEDED splitmix64,
+// UNIT (BFS), ZERO (CC label forwarding).
+enum WeightKind : uint32_t { W_REFERENCE = 0, W_SEEDED = 1, W_UNIT = 2, W_ZERO = 3 };
+struct WeightParams {
+    uint32_t kind;
+    uint32_t wmax;
+    uint64_t seed;
+};
+
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
+    x += 0x9e3779b97f4a7c15ull;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+    return x ^ (x >> 31);
+}
+
+__host__ __device__ __forceinline__ uint32_t edge_weight(const WeightParams &w, uint32_t u, uint32_t v) {
+    switch (w.kind) {
+    case W_REFERENCE: return (uint32_t)(((uint64_t)u + v) % 128u + 1u);
+    case W_SEEDED: return 1u + (uint32_t)(mix64(w.seed ^ (((uint64_t)u << 32) | v)) % w.wmax);
+    case W_UNIT: return 1u;
+    default: return 0u;
+    }
+}
+
+// ---- wave64 primitives -------------------------------------------------------
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+    // popcount of mask bits below this lane (v_mbcnt)
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_shfl(T v, int src) { return __shfl(v, src, kWave); }
+
+// Inclusive prefix sum over the wave (Hillis-Steele with DPP-backed shuffles).
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+    const uint32_t l = lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, kWave);
+        if (l >= (uint32_t)d) x += y;
+    }
+    return x;
+}
+
+// Wave-aggregated append: lanes with `want` get consecutive slots of `counter`.
+// Returns the slot for this lane (undefined where !want).
+__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool want) {
+    const uint64_t mask = __ballot(want);
+    if (!mask) return 0;
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if ((int)lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = wave_shfl(base, leader);
+    return base + lanes_below(mask);
+}
+
+// Control block of one persistent launch (frontier counters ring of 3).
+struct Ctl {
+    uint32_t count[3];        // entries in the queue consumed in round r (index r%3)
+    uint32_t cursor[3];       // small-row work cursor of round r
+    unsigned long long big[3];// (entries << 40) | pieces for round r
+    uint32_t big_cursor[3];   // piece cursor of round r
+    uint32_t rounds;          // rounds executed
+    uint32_t overflow;        // a queue overflowed its capacity
+    uint32_t aff_count;       // affected-list length (invalidation)
+    uint32_t pad;
+    unsigned long long edges; // adjacency entries scanned
+    unsigned long long bad;   // PR: non-finite values seen
+};
+
+struct BigEntry {
+    uint64_t off;      // first slot of the row
+    uint32_t g;        // global entry id (q*n + v)
+    uint32_t deg;
+    uint32_t val;      // owner value (min-lattice) or float bits (PR delta/deg)
+    uint32_t piece0;   // first piece index
+};
+
+template <typename T>
+__global__ void k_fill(T *p, uint64_t n, T v) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Fill kernel (used instead of hipMemsetAsync so the sequence stays capturable
+// in a hipGraph).
+template <typename T>
+inline void fill(hipStream_t s, T *p, uint64_t n, T v) {
+    if (!n) return;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_fill<T>, dim3((uint32_t)blocks), dim3(256), 0, s, p, n, v);
+}
+
+inline uint32_t grid_for(uint64_t work, uint32_t block = 256, uint32_t cap = 65536) {
+    uint64_t g = (work + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (uint32_t)g;
+}
+
+}  // namespace dcg

@@ -1,0 +1,180 @@
+// engine.hpp -- host-side engine object behind the C ABI (include/dcg.h).
+#pragma once
+#include <chrono>
+#include <cstdio>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "dcg_internal.hpp"
+
+namespace dcg {
+
+struct Error : std::runtime_error {
+    int status;
+    Error(int st, const std::string &m) : std::runtime_error(m), status(st) {}
+};
+
+#define DCG_HIP(call)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            throw ::dcg::Error(DCG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_) + \
+                                                " (" + __FILE__ + ":" + std::to_string(__LINE__) + ")"); \
+    } while (0)
+
+// Device buffer owned by the engine.
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        release();
+        if (count) DCG_HIP(hipMalloc(&p, count * sizeof(T)));
+        n = count;
+    }
+    void ensure(size_t count) {
+        if (count > n) alloc(count);
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+struct MlpState;   // learned cache policy (mlp.hip)
+struct MlpDeleter {
+    void operator()(MlpState *m) const;
+};
+
+// Per-snapshot chunk access statistics and cache residency.
+struct CacheState {
+    uint64_t chunks = 0, frames = 0;
+    std::vector<int32_t> chunk_frame;   // host mirror
+    std::vector<int64_t> frame_chunk;   // frame -> chunk (-1 free)
+    DevBuf<int32_t> d_chunk_frame;
+    DevBuf<uint8_t> d_dirty;
+    DevBuf<uint32_t> d_frames;          // frames * 65536 slots (device placement: all chunks)
+    DevBuf<float> d_reads;              // edges read per chunk, this snapshot
+    DevBuf<float> d_hist;               // 4-snapshot history of reads (chunk-major, 4 floats)
+    DevBuf<float> d_touch;              // update records touching each chunk, this batch
+    DevBuf<float> d_score;
+    std::vector<uint32_t> last_use;     // snapshot of last access (LRU tie-break)
+    uint64_t loaded = 0, evicted = 0, written_back = 0, h2d = 0, d2h = 0;
+    std::unique_ptr<MlpState, MlpDeleter> mlp;
+};
+
+struct Engine {
+    dcg_config cfg{};
+    std::string err;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cu_count = 0;
+    uint32_t coop_grid[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+    uint32_t n = 0;
+    uint32_t Q = 1;
+    uint32_t app = DCG_APP_SSSP;
+    std::vector<uint32_t> sources;
+    DevBuf<uint32_t> d_sources;
+    WeightParams wp{};
+    bool loaded = false, started = false;
+    uint64_t snapshots = 0;
+
+    // ---- chunk store ----
+    uint64_t total_slots = 0;
+    uint64_t used_slots = 0;           // host copy of the bump tail
+    uint64_t num_edges = 0;
+    DevBuf<RowDesc> d_out, d_in;
+    uint32_t *h_slab = nullptr;        // pinned host slots (DCG_PLACE_HOST)
+    DevBuf<uint64_t> d_tail;           // [0] bump tail, [1] overflow flag, [2..] class cursors
+    DevBuf<uint32_t> d_hot_out, d_hot_in;
+    CacheState cache;
+
+    // ---- vertex state ----
+    DevBuf<uint64_t> d_st;             // Q*n packed (value<<32 | parent)
+    DevBuf<uint32_t> d_tick;           // Q*n dedup stamps
+    uint32_t stamp = 1;
+    DevBuf<uint32_t> d_qa, d_qb, d_qc, d_aff;
+    DevBuf<BigEntry> d_big;
+    DevBuf<Ctl> d_ctl;
+    Ctl *h_ctl = nullptr;              // pinned readback
+    // PageRank
+    DevBuf<float> d_rank, d_res, d_delta;
+    DevBuf<uint32_t> d_queued;
+    uint32_t *pr_pending = nullptr;    // points at one of d_qa/d_qb/d_qc
+    uint64_t pr_pending_count = 0;
+
+    // ---- staged update stream ----
+    DevBuf<dcg_edge> d_adds, d_dels;
+    uint64_t n_adds = 0, n_dels = 0;
+
+    // ---- phase scratch ----
+    DevBuf<uint8_t> d_temp;            // hipcub temporary storage
+    DevBuf<uint64_t> d_k64a, d_k64b, d_k64c, d_k64d;   // delete keys (sorted a/b, unsorted c/d)
+    DevBuf<uint32_t> d_k32a, d_k32b, d_v32a, d_v32b;   // add keys/payloads (sorted)
+    DevBuf<uint32_t> d_k32c, d_k32d, d_v32c, d_v32d;   // add keys/payloads (unsorted)
+    DevBuf<uint32_t> d_gstart, d_gstart2, d_uniq;
+    DevBuf<uint32_t> d_used;           // per-record occurrence counters
+    DevBuf<unsigned long long> d_counters;  // phase counters (see store.hip)
+    unsigned long long *h_counters = nullptr;
+
+    // cumulative counters
+    uint64_t cum_cached_reads = 0, cum_host_reads = 0;
+
+    explicit Engine(const dcg_config &c);
+    ~Engine();
+
+    // store.hip
+    void load_edges_device(uint32_t nn, const uint32_t *d_src, const uint32_t *d_dst, uint64_t m);
+    void mutate_delete(uint64_t begin, uint64_t count, dcg_phase_stats *st);
+    void mutate_add(uint64_t begin, uint64_t count, dcg_phase_stats *st);
+    void export_csr(uint64_t *row_ptr, uint32_t *col, uint64_t *m);
+    StoreView view();
+    void fold_reads(dcg_phase_stats *st);
+
+    // traverse.hip
+    void alloc_state();
+    void start_min_lattice();
+    void delete_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st);
+    void add_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st);
+    uint32_t coop_blocks(const void *kernel);
+
+    // pagerank.hip
+    void start_pagerank();
+    void pagerank_batch(uint64_t ab, uint64_t ac, uint64_t db, uint64_t dc, dcg_phase_stats *st);
+
+    // cache_policy.hip
+    void cache_init();
+    void cache_maintain(dcg_cache_stats *st);
+    void cache_prefetch(uint64_t ab, uint64_t ac, uint64_t db, uint64_t dc, dcg_cache_stats *st);
+    void cache_apply_plan(const std::vector<uint64_t> &desired, dcg_cache_stats *st);
+
+    void sync() { DCG_HIP(hipStreamSynchronize(stream)); }
+    void *temp(size_t bytes) {
+        d_temp.ensure(bytes);
+        return d_temp.p;
+    }
+};
+
+inline double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+inline uint32_t ceil_pow2_cap(uint64_t deg) {
+    if (deg == 0) return 0;
+    uint64_t c = deg < kMinRowCap ? kMinRowCap : deg;
+    c--;
+    c |= c >> 1; c |= c >> 2; c |= c >> 4; c |= c >> 8; c |= c >> 16; c |= c >> 32;
+    return (uint32_t)(c + 1);
+}
+
+}  // namespace dcg

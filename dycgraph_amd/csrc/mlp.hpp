@@ -1,0 +1,15 @@
+// mlp.hpp -- learned chunk-cache admission/prefetch policy (MLP 8->64->64->2).
+#pragma once
+#include <vector>
+
+#include "engine.hpp"
+
+namespace dcg {
+
+void mlp_create(Engine &e);
+// Online training step on the reads observed in the snapshot that just ended.
+void mlp_observe(Engine &e, const std::vector<float> &reads);
+// Chunks to keep resident for the staged batch about to run.
+std::vector<uint64_t> mlp_plan(Engine &e, uint64_t ab, uint64_t ac, uint64_t db, uint64_t dc);
+
+}  // namespace dcg

@@ -1,0 +1,671 @@
+// store.hip -- chunked-CSR topology store: load, GPU mutation, export.
+//
+// Rows are power-of-two blocks (minimum 4 slots) like the reference's
+// source-local chunk store (include/groute/graphs/source_local_chunk_store.h:133-164),
+// but the mutation itself runs on the GPU: one wave64 per touched row.
+//  * delete: remove the first occurrence per request, survivors keep their order
+//    (source_local_chunk_store.h:685-821); requests with no remaining occurrence
+//    are counted as missing_deletes.
+//  * add: append in request order (:809-815); a row that outgrows its block moves
+//    to a fresh block of the next power of two, carved from the store tail by a
+//    per-class bump allocator so blocks stay aligned to their capacity.
+// Both the out-rows and the in-rows (reverse index, the role of
+// include/framework/dynamic_reverse_index.h) live in the same slot space.
+#include <hipcub/hipcub.hpp>
+
+#include "engine.hpp"
+
+namespace dcg {
+
+enum Counter : int {
+    C_MISSING = 0,
+    C_INVALID = 1,
+    C_GROUPS_OUT = 2,
+    C_GROUPS_IN = 3,
+    C_CHANGED = 4,
+    C_RELOCATED = 5,
+    C_BAD_ID = 6,
+    C_CACHED_READS = 7,
+    C_HOST_READS = 8,
+    C_NUM = 16
+};
+
+// d_tail layout
+constexpr int T_TAIL = 0, T_OVERFLOW = 1, T_COUNT = 2, T_CURSOR = 2 + kClasses, T_NUM = 2 + 2 * kClasses;
+
+StoreView Engine::view() {
+    StoreView v;
+    v.out = d_out.p;
+    v.in = d_in.p;
+    v.host = h_slab;
+    v.frames = cache.d_frames.p;
+    v.chunk_frame = cache.d_chunk_frame.p;
+    v.dirty = cache.d_dirty.p;
+    v.hot_out = d_hot_out.p;
+    v.hot_in = d_hot_in.p;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t log2_u64(uint64_t x) { return 63u - (uint32_t)__clzll((long long)x); }
+
+// ---------------------------------------------------------------- load kernels
+__global__ void k_check_ids(const uint32_t *src, const uint32_t *dst, uint64_t m, uint32_t n,
+                            unsigned long long *counters, uint32_t *deg) {
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < m;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t u = src[e], v = dst[e];
+        if (u >= n || v >= n) {
+            atomicAdd(&counters[C_BAD_ID], 1ull);
+            continue;
+        }
+        atomicAdd(&deg[u], 1u);
+        atomicAdd(&deg[(uint64_t)n + v], 1u);
+    }
+}
+
+// Placement key: larger classes first, then row id (out rows 0..n-1, in rows n..2n-1).
+__global__ void k_place_keys(const uint32_t *deg, uint64_t rows, uint64_t *keys) {
+    for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < rows;
+         r += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t d = deg[r];
+        if (!d) {
+            keys[r] = ~0ull;
+            continue;
+        }
+        uint64_t c = d < kMinRowCap ? kMinRowCap : d;
+        const uint32_t cls = 64u - (uint32_t)__clzll((long long)(c - 1));  // ceil log2
+        keys[r] = ((uint64_t)(63u - cls) << 36) | r;
+    }
+}
+
+__global__ void k_place_caps(const uint64_t *keys, uint64_t rows, uint64_t *caps) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < rows;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = keys[i];
+        caps[i] = k == ~0ull ? 0ull : (1ull << (63u - (uint32_t)(k >> 36)));
+    }
+}
+
+__global__ void k_place_desc(const uint64_t *keys, const uint64_t *offs, const uint32_t *deg, uint64_t rows,
+                             uint32_t n, RowDesc *out, RowDesc *in) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < rows;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = keys[i];
+        if (k == ~0ull) continue;
+        const uint64_t r = k & ((1ull << 36) - 1);
+        RowDesc d;
+        d.off = offs[i];
+        d.deg = deg[r];
+        d.cap = (uint32_t)(1ull << (63u - (uint32_t)(k >> 36)));
+        if (r < n) out[r] = d;
+        else in[r - n] = d;
+    }
+}
+
+__global__ void k_widen(const uint32_t *a, uint64_t *b, uint64_t k) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < k; i += (uint64_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
+__global__ void k_zero_desc(RowDesc *d, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        d[i] = RowDesc{0, 0, 0};
+}
+
+// Scatter a row-sorted edge list into the slot space (row order = input order
+// because the sort is stable).
+__global__ void k_scatter_rows(const uint32_t *keys, const uint32_t *vals, uint64_t m, const uint64_t *rp,
+                               const RowDesc *desc, uint32_t *slots) {
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < m;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t r = keys[e];
+        slots[desc[r].off + (e - rp[r])] = vals[e];
+    }
+}
+
+static int bits_for(uint64_t maxval) {
+    int b = 1;
+    while (b < 64 && (maxval >> b)) b++;
+    return b;
+}
+
+void Engine::load_edges_device(uint32_t nn, const uint32_t *d_src, const uint32_t *d_dst, uint64_t m) {
+    if (m >= (1ull << 31)) throw Error(DCG_ERR_UNSUPPORTED, "edge count must be below 2^31 per load");
+    n = nn;
+    num_edges = m;
+    const uint64_t rows = 2ull * n;
+    d_counters.alloc(C_NUM);
+    fill<unsigned long long>(stream, d_counters.p, C_NUM, 0ull);
+    DevBuf<uint32_t> deg;
+    deg.alloc(rows ? rows : 1);
+    fill<uint32_t>(stream, deg.p, rows, 0u);
+    hipLaunchKernelGGL(k_check_ids, dim3(grid_for(m)), dim3(256), 0, stream, d_src, d_dst, m, n, d_counters.p, deg.p);
+    DCG_HIP(hipGetLastError());
+    unsigned long long bad = 0;
+    DCG_HIP(hipMemcpyAsync(&bad, d_counters.p + C_BAD_ID, sizeof(bad), hipMemcpyDeviceToHost, stream));
+    sync();
+    if (bad) throw Error(DCG_ERR_ARG, "graph has " + std::to_string(bad) + " edges with an id >= num_nodes");
+
+    // Row pointers of the out- and in-CSR (64-bit exclusive scans of degrees).
+    DevBuf<uint64_t> rp;
+    rp.alloc(rows + 2);
+    DevBuf<uint64_t> deg64;
+    deg64.alloc(rows + 1);
+    hipLaunchKernelGGL(k_widen, dim3(grid_for(rows)), dim3(256), 0, stream, deg.p, deg64.p, rows);
+    size_t tb = 0;
+    // out rows: rp[0..n), in rows: rp[n+1 .. 2n+1) (each half scanned separately)
+    DCG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, deg64.p, rp.p, (int)(n ? n : 1), stream));
+    void *t = temp(tb);
+    DCG_HIP(hipcub::DeviceScan::ExclusiveSum(t, tb, deg64.p, rp.p, (int)n, stream));
+    DCG_HIP(hipcub::DeviceScan::ExclusiveSum(t, tb, deg64.p + n, rp.p + n + 1, (int)n, stream));
+
+    // Stable sorts: out-CSR by src (payload dst), in-CSR by dst (payload src).
+    DevBuf<uint32_t> ks, vs;
+    ks.alloc(m ? m : 1);
+    vs.alloc(m ? m : 1);
+    const int kb = bits_for(n ? n - 1 : 0);
+
+    // placement first (needs only degrees)
+    DevBuf<uint64_t> pkeys, pkeys2, caps, offs;
+    pkeys.alloc(rows ? rows : 1);
+    pkeys2.alloc(rows ? rows : 1);
+    caps.alloc(rows ? rows : 1);
+    offs.alloc(rows ? rows : 1);
+    hipLaunchKernelGGL(k_place_keys, dim3(grid_for(rows)), dim3(256), 0, stream, deg.p, rows, pkeys.p);
+    tb = 0;
+    DCG_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, pkeys.p, pkeys2.p, (int)rows, 0, 64, stream));
+    t = temp(tb);
+    DCG_HIP(hipcub::DeviceRadixSort::SortKeys(t, tb, pkeys.p, pkeys2.p, (int)rows, 0, 64, stream));
+    hipLaunchKernelGGL(k_place_caps, dim3(grid_for(rows)), dim3(256), 0, stream, pkeys2.p, rows, caps.p);
+    tb = 0;
+    DCG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, caps.p, offs.p, (int)rows, stream));
+    t = temp(tb);
+    DCG_HIP(hipcub::DeviceScan::ExclusiveSum(t, tb, caps.p, offs.p, (int)rows, stream));
+    uint64_t last_off = 0, last_cap = 0;
+    if (rows) {
+        DCG_HIP(hipMemcpyAsync(&last_off, offs.p + rows - 1, 8, hipMemcpyDeviceToHost, stream));
+        DCG_HIP(hipMemcpyAsync(&last_cap, caps.p + rows - 1, 8, hipMemcpyDeviceToHost, stream));
+    }
+    sync();
+    used_slots = last_off + last_cap;
+    double slack = cfg.store_slack < 0.05 ? 0.05 : cfg.store_slack;
+    uint64_t want = used_slots + (uint64_t)((double)used_slots * slack) + 8 * kChunkEdges;
+    total_slots = (want + kChunkEdges - 1) / kChunkEdges * kChunkEdges;
+
+    d_out.alloc(n ? n : 1);
+    d_in.alloc(n ? n : 1);
+    hipLaunchKernelGGL(k_zero_desc, dim3(grid_for(n)), dim3(256), 0, stream, d_out.p, (uint64_t)n);
+    hipLaunchKernelGGL(k_zero_desc, dim3(grid_for(n)), dim3(256), 0, stream, d_in.p, (uint64_t)n);
+    hipLaunchKernelGGL(k_place_desc, dim3(grid_for(rows)), dim3(256), 0, stream, pkeys2.p, offs.p, deg.p, rows, n,
+                       d_out.p, d_in.p);
+    pkeys.release();
+    pkeys2.release();
+    caps.release();
+    offs.release();
+
+    DevBuf<uint32_t> slots;
+    slots.alloc(total_slots);
+    // out rows
+    tb = 0;
+    DCG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, d_src, ks.p, d_dst, vs.p, (int)m, 0, kb, stream));
+    t = temp(tb);
+    DCG_HIP(hipcub::DeviceRadixSort::SortPairs(t, tb, d_src, ks.p, d_dst, vs.p, (int)m, 0, kb, stream));
+    hipLaunchKernelGGL(k_scatter_rows, dim3(grid_for(m)), dim3(256), 0, stream, ks.p, vs.p, m, rp.p, d_out.p, slots.p);
+    // in rows
+    DCG_HIP(hipcub::DeviceRadixSort::SortPairs(t, tb, d_dst, ks.p, d_src, vs.p, (int)m, 0, kb, stream));
+    hipLaunchKernelGGL(k_scatter_rows, dim3(grid_for(m)), dim3(256), 0, stream, ks.p, vs.p, m, rp.p + n + 1, d_in.p,
+                       slots.p);
+    DCG_HIP(hipGetLastError());
+    ks.release();
+    vs.release();
+    rp.release();
+    deg64.release();
+    deg.release();
+
+    d_tail.alloc(T_NUM);
+    fill<uint64_t>(stream, d_tail.p, T_NUM, 0ull);
+    DCG_HIP(hipMemcpyAsync(d_tail.p + T_TAIL, &used_slots, 8, hipMemcpyHostToDevice, stream));
+    d_hot_out.alloc(n ? n : 1);
+    d_hot_in.alloc(n ? n : 1);
+    fill<uint32_t>(stream, d_hot_out.p, n, 0u);
+    fill<uint32_t>(stream, d_hot_in.p, n, 0u);
+
+    cache.chunks = total_slots / kChunkEdges;
+    if (cfg.placement == DCG_PLACE_DEVICE) {
+        // every chunk is resident: the slot space itself is the frame array
+        cache.frames = cache.chunks;
+        std::swap(cache.d_frames.p, slots.p);
+        std::swap(cache.d_frames.n, slots.n);
+    } else {
+        DCG_HIP(hipHostMalloc((void **)&h_slab, total_slots * sizeof(uint32_t), hipHostMallocDefault));
+        DCG_HIP(hipMemcpyAsync(h_slab, slots.p, used_slots * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        sync();
+        slots.release();
+        const uint64_t frame_bytes = kChunkEdges * sizeof(uint32_t);
+        cache.frames = cfg.cache_bytes / frame_bytes;
+        if (cache.frames > cache.chunks) cache.frames = cache.chunks;
+        if (cache.frames) cache.d_frames.alloc(cache.frames * kChunkEdges);
+    }
+    sync();
+    cache_init();
+    loaded = true;
+}
+
+// ------------------------------------------------------------- mutation kernels
+__global__ void k_del_keys(const dcg_edge *recs, uint64_t count, uint32_t n, uint64_t *kout, uint64_t *kin,
+                           unsigned long long *counters) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const dcg_edge e = recs[i];
+        if (e.u >= n || e.v >= n) {
+            kout[i] = ~0ull;
+            kin[i] = ~0ull;
+            atomicAdd(&counters[C_MISSING], 1ull);
+        } else {
+            kout[i] = ((uint64_t)e.u << 32) | e.v;
+            kin[i] = ((uint64_t)e.v << 32) | e.u;
+        }
+    }
+}
+
+__global__ void k_add_keys(const dcg_edge *recs, uint64_t count, uint32_t n, uint32_t *kout, uint32_t *vout,
+                           uint32_t *kin, uint32_t *vin, unsigned long long *counters) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const dcg_edge e = recs[i];
+        const bool ok = e.u < n && e.v < n;
+        kout[i] = ok ? e.u : ~0u;
+        vout[i] = e.v;
+        kin[i] = ok ? e.v : ~0u;
+        vin[i] = e.u;
+        if (!ok) atomicAdd(&counters[C_INVALID], 1ull);
+    }
+}
+
+// Group heads of a sorted key array (high 32 bits for 64-bit keys).
+template <typename K>
+__device__ __forceinline__ uint32_t row_of(K k);
+template <>
+__device__ __forceinline__ uint32_t row_of<uint64_t>(uint64_t k) { return (uint32_t)(k >> 32); }
+template <>
+__device__ __forceinline__ uint32_t row_of<uint32_t>(uint32_t k) { return k; }
+
+template <typename K>
+__global__ void k_group_heads(const K *keys, uint64_t count, uint32_t *gstart, unsigned long long *ngroups) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t base = blockIdx.x * (uint64_t)blockDim.x; base < count; base += stride) {
+        const uint64_t i = base + threadIdx.x;
+        bool head = false;
+        if (i < count) {
+            const uint32_t r = row_of<K>(keys[i]);
+            head = r != ~0u && (i == 0 || row_of<K>(keys[i - 1]) != r);
+        }
+        const uint64_t mask = __ballot(head);
+        if (mask) {
+            const int leader = __ffsll((unsigned long long)mask) - 1;
+            unsigned long long b = 0;
+            if ((int)lane_id() == leader) b = atomicAdd(ngroups, (unsigned long long)__popcll(mask));
+            b = __shfl(b, leader, kWave);
+            if (head) gstart[b + lanes_below(mask)] = (uint32_t)i;
+        }
+    }
+}
+
+template <typename K>
+__device__ __forceinline__ uint64_t group_end(const K *keys, uint64_t s, uint64_t count, uint32_t row) {
+    uint64_t lo = s + 1, hi = count;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (row_of<K>(keys[mid]) <= row) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t load_slot(const StoreView &s, uint64_t slot) { return *row_base(s, slot); }
+__device__ __forceinline__ void store_slot(const StoreView &s, uint64_t slot, uint32_t v) { *row_base_w(s, slot) = v; }
+
+// One wave per touched row: remove the first `requested` occurrences of every
+// target, compact survivors in order.
+__global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, const uint64_t *keys, uint64_t count,
+                                                  const uint32_t *gstart, const unsigned long long *ngroups_p,
+                                                  uint32_t *used, unsigned long long *counters, int count_stats) {
+    const uint32_t lane = lane_id();
+    const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kWave;
+    const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / kWave;
+    const uint64_t ngroups = *ngroups_p;
+    unsigned long long missing = 0, changed = 0;
+    for (uint64_t g = wave; g < ngroups; g += nwaves) {
+        const uint64_t gs = gstart[g];
+        const uint32_t row = (uint32_t)(keys[gs] >> 32);
+        const uint64_t ge = group_end<uint64_t>(keys, gs, count, row);
+        const RowDesc d = desc[row];
+        uint32_t write = 0, removed = 0;
+        for (uint32_t t0 = 0; t0 < d.deg; t0 += kWave) {
+            const uint32_t i = t0 + lane;
+            const bool valid = i < d.deg;
+            const uint32_t x = valid ? load_slot(s, d.off + i) : 0u;
+            // lower bound of x among the group's targets (sorted low halves)
+            uint64_t lo = gs, hi = ge;
+            while (lo < hi) {
+                const uint64_t mid = (lo + hi) >> 1;
+                if ((uint32_t)keys[mid] < x) lo = mid + 1;
+                else hi = mid;
+            }
+            const bool match = valid && lo < ge && (uint32_t)keys[lo] == x;
+            bool rem = false;
+            uint64_t mm = __ballot(match);
+            while (mm) {
+                const int l = __ffsll((unsigned long long)mm) - 1;
+                if ((int)lane == l) {
+                    uint64_t a = lo + 1, b = ge;   // end of this target's run
+                    while (a < b) {
+                        const uint64_t mid = (a + b) >> 1;
+                        if ((uint32_t)keys[mid] <= x) a = mid + 1;
+                        else b = mid;
+                    }
+                    const uint32_t requested = (uint32_t)(a - lo);
+                    const uint32_t old = atomicAdd(&used[lo], 1u);
+                    rem = old < requested;
+                }
+                mm &= mm - 1;
+            }
+            const bool keep = valid && !rem;
+            const uint64_t km = __ballot(keep);
+            const uint32_t rtile = (uint32_t)__popcll(__ballot(rem));
+            if (removed + rtile) {
+                const uint32_t pos = write + lanes_below(km);
+                if (keep && pos != i) store_slot(s, d.off + pos, x);
+            }
+            write += (uint32_t)__popcll(km);
+            removed += rtile;
+        }
+        if (lane == 0) {
+            if (removed) desc[row].deg = write;
+            missing += (ge - gs) - removed;
+            changed += removed ? 1 : 0;
+        }
+    }
+    if (count_stats && lane == 0) {
+        if (missing) atomicAdd(&counters[C_MISSING], missing);
+        if (changed) atomicAdd(&counters[C_CHANGED], changed);
+    }
+}
+
+__global__ void k_add_plan(const RowDesc *desc, const uint32_t *keys, uint64_t count, const uint32_t *gstart,
+                           const unsigned long long *ngroups_p, uint64_t *tail) {
+    const uint64_t ngroups = *ngroups_p;
+    for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < ngroups;
+         g += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t gs = gstart[g];
+        const uint32_t row = keys[gs];
+        const uint64_t ge = group_end<uint32_t>(keys, gs, count, row);
+        const RowDesc d = desc[row];
+        const uint64_t nd = (uint64_t)d.deg + (ge - gs);
+        if (nd > d.cap) {
+            uint64_t c = nd < kMinRowCap ? kMinRowCap : nd;
+            const uint32_t cls = 64u - (uint32_t)__clzll((long long)(c - 1));
+            atomicAdd((unsigned long long *)&tail[T_COUNT + cls], 1ull);
+        }
+    }
+}
+
+__global__ void k_alloc_classes(uint64_t *tail, uint64_t total) {
+    if (threadIdx.x || blockIdx.x) return;
+    uint64_t t = tail[T_TAIL];
+    for (int c = kClasses - 1; c >= 2; --c) {
+        const uint64_t cnt = tail[T_COUNT + c];
+        if (!cnt) continue;
+        const uint64_t a = 1ull << c;
+        t = (t + a - 1) & ~(a - 1);
+        tail[T_CURSOR + c] = t;
+        t += cnt << c;
+        tail[T_COUNT + c] = 0;
+    }
+    if (t > total) tail[T_OVERFLOW] = 1;
+    else tail[T_TAIL] = t;
+}
+
+// One wave per touched row: relocate if needed, then append in request order.
+__global__ __launch_bounds__(256) void k_add_rows(StoreView s, RowDesc *desc, const uint32_t *keys, const uint32_t *vals,
+                                                  uint64_t count, const uint32_t *gstart,
+                                                  const unsigned long long *ngroups_p, uint64_t *tail,
+                                                  unsigned long long *counters, int count_stats) {
+    const uint32_t lane = lane_id();
+    const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kWave;
+    const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / kWave;
+    const uint64_t ngroups = *ngroups_p;
+    const bool overflow = tail[T_OVERFLOW] != 0;
+    unsigned long long relocated = 0;
+    for (uint64_t g = wave; g < ngroups; g += nwaves) {
+        const uint64_t gs = gstart[g];
+        const uint32_t row = keys[gs];
+        const uint64_t ge = group_end<uint32_t>(keys, gs, count, row);
+        const uint32_t k = (uint32_t)(ge - gs);
+        RowDesc d = desc[row];
+        const uint64_t nd = (uint64_t)d.deg + k;
+        if (nd > d.cap) {
+            if (overflow) continue;
+            uint64_t c = nd < kMinRowCap ? kMinRowCap : nd;
+            const uint32_t cls = 64u - (uint32_t)__clzll((long long)(c - 1));
+            unsigned long long noff = 0;
+            if (lane == 0) noff = atomicAdd((unsigned long long *)&tail[T_CURSOR + cls], 1ull << cls);
+            noff = __shfl(noff, 0, kWave);
+            for (uint32_t i = lane; i < d.deg; i += kWave) store_slot(s, noff + i, load_slot(s, d.off + i));
+            d.off = noff;
+            d.cap = (uint32_t)(1ull << cls);
+            relocated++;
+        }
+        for (uint32_t i = lane; i < k; i += kWave) store_slot(s, d.off + d.deg + i, vals[gs + i]);
+        d.deg = (uint32_t)nd;
+        if (lane == 0) desc[row] = d;
+    }
+    if (count_stats && lane == 0 && relocated) atomicAdd(&counters[C_RELOCATED], relocated);
+}
+
+static uint32_t wave_grid(uint64_t groups) {
+    uint64_t waves = groups ? groups : 1;
+    uint64_t blocks = (waves * kWave + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    return (uint32_t)blocks;
+}
+
+void Engine::mutate_delete(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
+    fill<unsigned long long>(stream, d_counters.p, C_NUM, 0ull);
+    if (!count) return;
+    d_k64a.ensure(count);
+    d_k64b.ensure(count);
+    d_gstart.ensure(count);
+    d_gstart2.ensure(count);
+    d_used.ensure(count);
+    d_k64c.ensure(count);
+    d_k64d.ensure(count);
+    const dcg_edge *recs = d_dels.p + begin;
+    // unsorted (u,v) and (v,u) keys, then sorted copies in kout / kin
+    hipLaunchKernelGGL(k_del_keys, dim3(grid_for(count)), dim3(256), 0, stream, recs, count, n, d_k64c.p, d_k64d.p,
+                       d_counters.p);
+    uint64_t *kout = d_k64a.p, *kin = d_k64b.p;
+    size_t tb = 0;
+    DCG_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, d_k64c.p, kout, (int)count, 0, 64, stream));
+    void *t = temp(tb);
+    DCG_HIP(hipcub::DeviceRadixSort::SortKeys(t, tb, d_k64c.p, kout, (int)count, 0, 64, stream));
+    DCG_HIP(hipcub::DeviceRadixSort::SortKeys(t, tb, d_k64d.p, kin, (int)count, 0, 64, stream));
+    hipLaunchKernelGGL(k_group_heads<uint64_t>, dim3(grid_for(count)), dim3(256), 0, stream, kout, count, d_gstart.p,
+                       d_counters.p + C_GROUPS_OUT);
+    hipLaunchKernelGGL(k_group_heads<uint64_t>, dim3(grid_for(count)), dim3(256), 0, stream, kin, count, d_gstart2.p,
+                       d_counters.p + C_GROUPS_IN);
+    StoreView v = view();
+    const uint32_t wg = wave_grid(count);
+    fill<uint32_t>(stream, d_used.p, count, 0u);
+    hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream, v, d_out.p, kout, count, d_gstart.p,
+                       d_counters.p + C_GROUPS_OUT, d_used.p, d_counters.p, 1);
+    fill<uint32_t>(stream, d_used.p, count, 0u);
+    hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream, v, d_in.p, kin, count, d_gstart2.p,
+                       d_counters.p + C_GROUPS_IN, d_used.p, d_counters.p, 0);
+    DCG_HIP(hipGetLastError());
+    DCG_HIP(hipMemcpyAsync(h_counters, d_counters.p, C_NUM * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+    sync();
+    if (st) {
+        st->touched_sources = h_counters[C_GROUPS_OUT];
+        st->changed_sources = h_counters[C_CHANGED];
+        st->missing_deletes = h_counters[C_MISSING];
+    }
+    num_edges -= count - h_counters[C_MISSING];
+}
+
+void Engine::mutate_add(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
+    fill<unsigned long long>(stream, d_counters.p, C_NUM, 0ull);
+    if (!count) return;
+    d_k32a.ensure(count);
+    d_k32b.ensure(count);
+    d_v32a.ensure(count);
+    d_v32b.ensure(count);
+    d_gstart.ensure(count);
+    d_gstart2.ensure(count);
+    d_k32c.ensure(count);
+    d_k32d.ensure(count);
+    d_v32c.ensure(count);
+    d_v32d.ensure(count);
+    const dcg_edge *recs = d_adds.p + begin;
+    // unsorted (key, payload) pairs in the c/d buffers, stable-sorted into a/b
+    uint32_t *tk = d_k32c.p, *tv = d_v32c.p, *tk2 = d_k32d.p, *tv2 = d_v32d.p;
+    hipLaunchKernelGGL(k_add_keys, dim3(grid_for(count)), dim3(256), 0, stream, recs, count, n, tk, tv, tk2,
+                       tv2, d_counters.p);
+    size_t tb = 0;
+    DCG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, tk, d_k32a.p, tv, d_v32a.p, (int)count, 0, 32, stream));
+    void *t = temp(tb);
+    DCG_HIP(hipcub::DeviceRadixSort::SortPairs(t, tb, tk, d_k32a.p, tv, d_v32a.p, (int)count, 0, 32, stream));
+    DCG_HIP(hipcub::DeviceRadixSort::SortPairs(t, tb, tk2, d_k32b.p, tv2, d_v32b.p, (int)count, 0, 32, stream));
+    hipLaunchKernelGGL(k_group_heads<uint32_t>, dim3(grid_for(count)), dim3(256), 0, stream, d_k32a.p, count,
+                       d_gstart.p, d_counters.p + C_GROUPS_OUT);
+    hipLaunchKernelGGL(k_group_heads<uint32_t>, dim3(grid_for(count)), dim3(256), 0, stream, d_k32b.p, count,
+                       d_gstart2.p, d_counters.p + C_GROUPS_IN);
+    hipLaunchKernelGGL(k_add_plan, dim3(grid_for(count)), dim3(256), 0, stream, d_out.p, d_k32a.p, count, d_gstart.p,
+                       d_counters.p + C_GROUPS_OUT, d_tail.p);
+    hipLaunchKernelGGL(k_add_plan, dim3(grid_for(count)), dim3(256), 0, stream, d_in.p, d_k32b.p, count, d_gstart2.p,
+                       d_counters.p + C_GROUPS_IN, d_tail.p);
+    hipLaunchKernelGGL(k_alloc_classes, dim3(1), dim3(64), 0, stream, d_tail.p, total_slots);
+    StoreView v = view();
+    const uint32_t wg = wave_grid(count);
+    hipLaunchKernelGGL(k_add_rows, dim3(wg), dim3(256), 0, stream, v, d_out.p, d_k32a.p, d_v32a.p, count, d_gstart.p,
+                       d_counters.p + C_GROUPS_OUT, d_tail.p, d_counters.p, 1);
+    hipLaunchKernelGGL(k_add_rows, dim3(wg), dim3(256), 0, stream, v, d_in.p, d_k32b.p, d_v32b.p, count, d_gstart2.p,
+                       d_counters.p + C_GROUPS_IN, d_tail.p, d_counters.p, 1);
+    DCG_HIP(hipGetLastError());
+    DCG_HIP(hipMemcpyAsync(h_counters, d_counters.p, C_NUM * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+    uint64_t tl[2];
+    DCG_HIP(hipMemcpyAsync(tl, d_tail.p, 16, hipMemcpyDeviceToHost, stream));
+    sync();
+    if (tl[T_OVERFLOW]) throw Error(DCG_ERR_CAPACITY, "chunk store slot space exhausted (raise store_slack)");
+    used_slots = tl[T_TAIL];
+    if (st) {
+        st->touched_sources = h_counters[C_GROUPS_OUT];
+        st->changed_sources = h_counters[C_GROUPS_OUT];
+        st->invalid_additions = h_counters[C_INVALID];
+        st->relocated_rows = h_counters[C_RELOCATED];
+    }
+    num_edges += count - h_counters[C_INVALID];
+}
+
+// ---------------------------------------------------------------------- export
+__global__ void k_export_deg(const RowDesc *out, uint32_t n, uint64_t *deg) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x)
+        deg[v] = out[v].deg;
+}
+
+__global__ void k_export_rows(StoreView s, uint32_t n, const uint64_t *rp, uint32_t *col) {
+    const uint32_t lane = lane_id();
+    const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kWave;
+    const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / kWave;
+    for (uint64_t v = wave; v < n; v += nwaves) {
+        const RowDesc d = s.out[v];
+        for (uint32_t i = lane; i < d.deg; i += kWave) col[rp[v] + i] = load_slot(s, d.off + i);
+    }
+}
+
+void Engine::export_csr(uint64_t *row_ptr, uint32_t *col, uint64_t *m) {
+    DevBuf<uint64_t> deg, rp;
+    deg.alloc(n + 1);
+    rp.alloc(n + 1);
+    fill<uint64_t>(stream, deg.p, n + 1, 0ull);
+    hipLaunchKernelGGL(k_export_deg, dim3(grid_for(n)), dim3(256), 0, stream, d_out.p, n, deg.p);
+    size_t tb = 0;
+    DCG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, deg.p, rp.p, (int)(n + 1), stream));
+    void *t = temp(tb);
+    DCG_HIP(hipcub::DeviceScan::ExclusiveSum(t, tb, deg.p, rp.p, (int)(n + 1), stream));
+    uint64_t total = 0;
+    DCG_HIP(hipMemcpyAsync(&total, rp.p + n, 8, hipMemcpyDeviceToHost, stream));
+    sync();
+    if (m) *m = total;
+    if (!row_ptr) return;
+    DCG_HIP(hipMemcpyAsync(row_ptr, rp.p, (n + 1) * 8, hipMemcpyDeviceToHost, stream));
+    if (col && total) {
+        DevBuf<uint32_t> dc;
+        dc.alloc(total);
+        hipLaunchKernelGGL(k_export_rows, dim3(wave_grid(n)), dim3(256), 0, stream, view(), n, rp.p, dc.p);
+        DCG_HIP(hipMemcpyAsync(col, dc.p, total * 4, hipMemcpyDeviceToHost, stream));
+        sync();
+    }
+    sync();
+}
+
+// --------------------------------------------------- per-phase read accounting
+// Folds per-vertex expansion counters into per-chunk edge reads (the cache
+// policy's signal) and splits reads into HBM-cache and host-link reads.
+__global__ void k_fold_reads(StoreView s, uint32_t n, float *reads, float *rows, unsigned long long *counters) {
+    unsigned long long cached = 0, host = 0;
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t ho = s.hot_out[v], hi = s.hot_in[v];
+        if (!(ho | hi)) continue;
+        for (int dir = 0; dir < 2; ++dir) {
+            const uint32_t h = dir ? hi : ho;
+            if (!h) continue;
+            const RowDesc d = dir ? s.in[v] : s.out[v];
+            uint64_t a = d.off, e = d.off + d.deg;
+            while (a < e) {
+                const uint64_t c = a >> kChunkLog2;
+                const uint64_t ce = (c + 1) << kChunkLog2;
+                const uint64_t b = e < ce ? e : ce;
+                const float edges = (float)h * (float)(b - a);
+                if (reads) {
+                    atomicAdd(&reads[c], edges);
+                    atomicAdd(&rows[c], (float)h);
+                }
+                if (s.chunk_frame[c] >= 0) cached += (unsigned long long)h * (b - a);
+                else host += (unsigned long long)h * (b - a);
+                a = b;
+            }
+        }
+        s.hot_out[v] = 0;
+        s.hot_in[v] = 0;
+    }
+    // wave-reduce the two counters before the global atomics
+    for (int d = 32; d >= 1; d >>= 1) {
+        cached += __shfl_xor(cached, d, kWave);
+        host += __shfl_xor(host, d, kWave);
+    }
+    if (lane_id() == 0) {
+        if (cached) atomicAdd(&counters[C_CACHED_READS], cached);
+        if (host) atomicAdd(&counters[C_HOST_READS], host);
+    }
+}
+
+void Engine::fold_reads(dcg_phase_stats *st) {
+    fill<unsigned long long>(stream, d_counters.p + C_CACHED_READS, 2, 0ull);
+    float *reads = cache.d_reads.n ? cache.d_reads.p : nullptr;
+    float *rows = cache.d_reads.n ? cache.d_reads.p + cache.chunks : nullptr;
+    hipLaunchKernelGGL(k_fold_reads, dim3(grid_for(n, 256, 16384)), dim3(256), 0, stream, view(), n, reads, rows,
+                       d_counters.p);
+    DCG_HIP(hipGetLastError());
+    unsigned long long c[2];
+    DCG_HIP(hipMemcpyAsync(c, d_counters.p + C_CACHED_READS, 16, hipMemcpyDeviceToHost, stream));
+    sync();
+    cum_cached_reads += c[0];
+    cum_host_reads += c[1];
+    if (st) {
+        st->cached_edge_reads += c[0];
+        st->host_edge_reads += c[1];
+    }
+}
+
+}  // namespace dcg

@@ -1,0 +1,808 @@
+// traverse.hip -- persistent frontier kernels for the min-lattice vertex
+// programs (SSSP, BFS, CC) and the PageRank residual push, plus the host
+// orchestration of one snapshot's delete and add stages.
+//
+// Vertex state is one 64-bit word per (query, vertex): value in the high half,
+// parent in the low half.  A relaxation is one 64-bit atomicMin, so the value
+// lattice (the reference's atomicMin + parent CAS loop, hybrid_sssp.cu:126-145)
+// and the smallest-source tie-break of the pull repair (framework.cuh:229-244)
+// come from the same instruction.
+//
+// One cooperative launch runs rounds until the frontier is empty:
+//   phase A: waves take 64 frontier entries at a time; rows of <= 256 edges are
+//            expanded edge-balanced inside the wave (wave64 prefix sum + binary
+//            search of the owner); larger rows are queued as 512-edge pieces.
+//   phase B: all waves share the pieces of the large rows.
+// New frontier entries are appended with one atomic per wave (ballot + mbcnt).
+#include <hip/hip_cooperative_groups.h>
+
+#include "engine.hpp"
+
+namespace cg = cooperative_groups;
+
+namespace dcg {
+
+enum Mode : uint32_t { M_PUSH = 0, M_PULL = 1, M_INVALIDATE = 2, M_PR = 3, M_PR_AMEND = 4 };
+
+struct TravArgs {
+    StoreView s;
+    uint32_t n, Q;
+    uint64_t *st;
+    uint32_t *tick;
+    const uint32_t *sources;
+    WeightParams w;
+    uint32_t app;
+    uint32_t max_rounds;
+    const uint32_t *in0;
+    uint32_t *qa, *qb;
+    uint64_t qcap;
+    uint32_t *aff;
+    uint64_t affcap;
+    BigEntry *big;
+    uint64_t bigcap;
+    Ctl *ctl;
+    uint32_t stamp0;
+    // PageRank
+    float *rank, *res, *delta;
+    uint32_t *queued;
+    float eps;
+    float sign;
+};
+
+constexpr uint64_t kBigShift = 36;
+constexpr uint64_t kBigMask = (1ull << kBigShift) - 1;
+
+__device__ __forceinline__ void decode_g(const TravArgs &a, uint32_t g, uint32_t &q, uint32_t &v) {
+    if (a.Q == 1) {
+        q = 0;
+        v = g;
+    } else {
+        q = g / a.n;
+        v = g - q * a.n;
+    }
+}
+
+// Dependency predicate of the delete stage: api.cuh:43-49 for SSSP/BFS (parent
+// edge, or a tight witness while the current parent is not tight) and
+// hybrid_cc.cu:60-64 for CC (equal label, label smaller than the id).  A
+// dependent vertex whose parent half is still set is claimed (parent := INF).
+__device__ __forceinline__ bool invalidate_op(const TravArgs &a, uint32_t q, uint32_t s, uint32_t vs, uint32_t d,
+                                              uint32_t &out_g) {
+    const uint64_t qoff = (uint64_t)q * a.n;
+    const uint64_t gd = qoff + d;
+    const uint64_t sd = a.st[gd];
+    const uint32_t vd = (uint32_t)(sd >> 32), pd = (uint32_t)sd;
+    bool dep;
+    if (a.app == DCG_APP_CC) {
+        dep = s != d && vs == vd && vd < d;
+    } else {
+        if (d == a.sources[q] || pd == kInf) return false;
+        if (pd == s) {
+            dep = true;
+        } else if (vs != kInf && (uint64_t)vs + edge_weight(a.w, s, d) == (uint64_t)vd) {
+            const uint32_t vp = (uint32_t)(a.st[qoff + pd] >> 32);
+            const bool ptight = vp != kInf && (uint64_t)vp + edge_weight(a.w, pd, d) == (uint64_t)vd;
+            dep = !ptight;
+        } else {
+            dep = false;
+        }
+    }
+    if (!dep) return false;
+    if (atomicExch(reinterpret_cast<uint32_t *>(&a.st[gd]), kInf) == kInf) return false;
+    out_g = (uint32_t)gd;
+    return true;
+}
+
+__device__ __forceinline__ bool push_op(const TravArgs &a, uint32_t q, uint32_t s, uint32_t vs, uint32_t d,
+                                        uint32_t stamp, uint32_t &out_g) {
+    const uint64_t cand = (uint64_t)vs + edge_weight(a.w, s, d);
+    if (cand >= kInf) return false;
+    const uint64_t gd = (uint64_t)q * a.n + d;
+    const uint64_t cur = a.st[gd];
+    if (cand >= (cur >> 32)) return false;
+    const uint64_t old = atomicMin((unsigned long long *)&a.st[gd], (unsigned long long)((cand << 32) | s));
+    if (cand >= (old >> 32)) return false;
+    if (atomicExch(&a.tick[gd], stamp) == stamp) return false;
+    out_g = (uint32_t)gd;
+    return true;
+}
+
+// Pull: the row owner `d` is the destination, `x` an in-neighbour.
+__device__ __forceinline__ uint64_t pull_cand(const TravArgs &a, uint32_t q, uint32_t d, uint32_t x) {
+    const uint64_t qoff = (uint64_t)q * a.n;
+    const uint32_t vx = (uint32_t)(a.st[qoff + x] >> 32);
+    if (vx == kInf) return ~0ull;
+    const uint64_t cand = (uint64_t)vx + edge_weight(a.w, x, d);
+    if (cand >= kInf) return ~0ull;
+    return (cand << 32) | x;
+}
+
+__device__ __forceinline__ void pull_apply(const TravArgs &a, uint32_t q, uint32_t d, uint64_t packed) {
+    if (packed == ~0ull) return;
+    const uint64_t gd = (uint64_t)q * a.n + d;
+    if (packed >= a.st[gd]) return;
+    atomicMin((unsigned long long *)&a.st[gd], (unsigned long long)packed);
+}
+
+__device__ __forceinline__ bool pr_add(const TravArgs &a, uint32_t d, float c, uint32_t &out_g) {
+    // Frontier::Add (include/framework/pr_residual.cuh:21-30)
+    const float old = atomicAdd(&a.res[d], c);
+    const float val = old + c;
+    if (!isfinite(val)) atomicExch((unsigned int *)&a.ctl->bad, 1u);
+    if (fabsf(old) <= a.eps && fabsf(val) > a.eps && atomicCAS(&a.queued[d], 0u, 1u) == 0u) {
+        out_g = d;
+        return true;
+    }
+    return false;
+}
+
+template <uint32_t MODE>
+__device__ __forceinline__ bool edge_op(const TravArgs &a, uint32_t q, uint32_t owner, uint32_t val, uint32_t x,
+                                        uint32_t stamp, uint32_t &out_g) {
+    if (MODE == M_PUSH) return push_op(a, q, owner, val, x, stamp, out_g);
+    if (MODE == M_INVALIDATE) return invalidate_op(a, q, owner, val, x, out_g);
+    if (MODE == M_PR || MODE == M_PR_AMEND) return pr_add(a, x, __uint_as_float(val), out_g);
+    pull_apply(a, q, owner, pull_cand(a, q, owner, x));
+    return false;
+}
+
+// Append the lanes' results to the next frontier (and, while invalidating, to
+// the affected list) with one atomic per wave.
+template <uint32_t MODE>
+__device__ __forceinline__ void emit(const TravArgs &a, uint32_t *outq, uint32_t *count, bool want, uint32_t g) {
+    const uint64_t mask = __ballot(want);
+    if (!mask) return;
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    const uint32_t lane = lane_id();
+    uint32_t base = 0, abase = 0;
+    if ((int)lane == leader) {
+        base = atomicAdd(count, (uint32_t)__popcll(mask));
+        if (MODE == M_INVALIDATE) abase = atomicAdd(&a.ctl->aff_count, (uint32_t)__popcll(mask));
+    }
+    base = __shfl(base, leader, kWave);
+    const uint32_t r = lanes_below(mask);
+    if (want) {
+        if (base + r < a.qcap) outq[base + r] = g;
+        else a.ctl->overflow = 1;
+    }
+    if (MODE == M_INVALIDATE) {
+        abase = __shfl(abase, leader, kWave);
+        if (want) {
+            if (abase + r < a.affcap) a.aff[abase + r] = g;
+            else a.ctl->overflow = 1;
+        }
+    }
+}
+
+struct WaveLds {
+    uint32_t pref[65];
+    uint32_t val[64];
+    uint32_t owner[64];
+    uint32_t q[64];
+    const uint32_t *base[64];
+};
+
+template <uint32_t MODE>
+__global__ __launch_bounds__(256) void k_persistent(TravArgs a) {
+    cg::grid_group grid = cg::this_grid();
+    __shared__ WaveLds lds_all[kBlock / kWave];
+    const uint32_t lane = lane_id();
+    const uint32_t wib = threadIdx.x / kWave;
+    WaveLds &L = lds_all[wib];
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t gthreads = gridDim.x * blockDim.x;
+    uint64_t edges = 0;
+
+    for (uint32_t r = 0; r < a.max_rounds; ++r) {
+        const uint32_t cur = r % 3, nxt = (r + 1) % 3, nn = (r + 2) % 3;
+        const uint32_t cnt = *(volatile uint32_t *)&a.ctl->count[cur];
+        if (cnt == 0) break;
+        const uint32_t *in = r == 0 ? a.in0 : ((r & 1) ? a.qa : a.qb);
+        uint32_t *out = (r & 1) ? a.qb : a.qa;
+        const uint32_t stamp = a.stamp0 + r;
+        if (gtid == 0) {
+            a.ctl->count[nn] = 0;
+            a.ctl->cursor[nxt] = 0;
+            a.ctl->big[nxt] = 0;
+            a.ctl->big_cursor[nxt] = 0;
+        }
+        if (MODE == M_PR) {
+            // Consume (pr_residual.cuh:53-69): fold residuals above eps into ranks.
+            for (uint32_t i = gtid; i < cnt; i += gthreads) {
+                const uint32_t v = in[i];
+                const float res = a.res[v];
+                a.queued[v] = 0;
+                float dl = 0.f;
+                if (!isfinite(res) || !isfinite(a.rank[v])) atomicExch((unsigned int *)&a.ctl->bad, 1u);
+                if (fabsf(res) > a.eps) {
+                    a.rank[v] += res;
+                    a.res[v] = 0.f;
+                    dl = res;
+                }
+                a.delta[i] = dl;
+            }
+            grid.sync();
+        }
+
+        // ---------------- phase A
+        for (;;) {
+            uint32_t b = 0;
+            if (lane == 0) b = atomicAdd(&a.ctl->cursor[cur], kWave);
+            b = __shfl(b, 0, kWave);
+            if (b >= cnt) break;
+            const uint32_t i = b + lane;
+            (void)gthreads;
+            const bool has = i < cnt;
+            uint32_t g = has ? in[i] : 0u;
+            uint32_t q = 0, v = 0;
+            uint32_t deg = 0, val = 0;
+            uint64_t off = 0;
+            if (has) {
+                if (MODE == M_PR || MODE == M_PR_AMEND) {
+                    v = g;
+                    const RowDesc d = a.s.out[v];
+                    const float x = MODE == M_PR ? a.delta[i] : a.rank[v] * a.sign;
+                    if (d.deg && x != 0.f) {
+                        deg = d.deg;
+                        off = d.off;
+                        val = __float_as_uint(0.85f * x / (float)d.deg);
+                        a.s.hot_out[v] += 1;
+                    }
+                } else {
+                    decode_g(a, g, q, v);
+                    const RowDesc d = MODE == M_PULL ? a.s.in[v] : a.s.out[v];
+                    val = MODE == M_PULL ? 0u : (uint32_t)(a.st[g] >> 32);
+                    if (d.deg && (MODE == M_PULL || val != kInf)) {
+                        deg = d.deg;
+                        off = d.off;
+                        if (MODE == M_PULL) a.s.hot_in[v] += 1;
+                        else a.s.hot_out[v] += 1;
+                    }
+                }
+            }
+            if (deg > kBigRow) {
+                const uint64_t pieces = (deg + kPiece - 1) / kPiece;
+                const unsigned long long p = atomicAdd(&a.ctl->big[cur], (1ull << kBigShift) | pieces);
+                const uint64_t e = p >> kBigShift;
+                if (e < a.bigcap) {
+                    BigEntry be;
+                    be.off = off;
+                    be.g = g;
+                    be.deg = deg;
+                    be.val = val;
+                    be.piece0 = (uint32_t)(p & kBigMask);
+                    a.big[e] = be;
+                } else {
+                    a.ctl->overflow = 1;
+                }
+                deg = 0;
+            }
+            const uint32_t incl = wave_incl_scan(deg);
+            const uint32_t total = __shfl(incl, 63, kWave);
+            L.pref[lane] = incl - deg;
+            L.val[lane] = val;
+            L.owner[lane] = v;
+            L.q[lane] = q;
+            L.base[lane] = deg ? row_base(a.s, off) : nullptr;
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            edges += total;
+            for (uint32_t k0 = 0; k0 < total; k0 += kWave) {
+                const uint32_t k = k0 + lane;
+                bool want = false;
+                uint32_t og = 0;
+                if (k < total) {
+                    uint32_t lo = 0, hi = 64;   // last owner with pref <= k
+                    while (hi - lo > 1) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (L.pref[mid] <= k) lo = mid;
+                        else hi = mid;
+                    }
+                    const uint32_t x = L.base[lo][k - L.pref[lo]];
+                    want = edge_op<MODE>(a, L.q[lo], L.owner[lo], L.val[lo], x, stamp, og);
+                }
+                if (MODE != M_PULL) emit<MODE>(a, out, &a.ctl->count[nxt], want, og);
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        grid.sync();
+
+        // ---------------- phase B: pieces of large rows
+        {
+            const unsigned long long bp = *(volatile unsigned long long *)&a.ctl->big[cur];
+            uint64_t nb = bp >> kBigShift;
+            if (nb > a.bigcap) nb = a.bigcap;
+            const uint64_t np = bp & kBigMask;
+            for (;;) {
+                uint32_t p = 0;
+                if (lane == 0) p = atomicAdd(&a.ctl->big_cursor[cur], 1u);
+                p = __shfl(p, 0, kWave);
+                if (p >= np || nb == 0) break;
+                uint64_t lo = 0, hi = nb;
+                while (hi - lo > 1) {
+                    const uint64_t mid = (lo + hi) >> 1;
+                    if (a.big[mid].piece0 <= p) lo = mid;
+                    else hi = mid;
+                }
+                const BigEntry be = a.big[lo];
+                const uint32_t start = (p - be.piece0) * kPiece;
+                if (start >= be.deg) continue;
+                const uint32_t len = be.deg - start < kPiece ? be.deg - start : kPiece;
+                const uint32_t *base = row_base(a.s, be.off + start);
+                uint32_t q = 0, v = 0;
+                if (MODE == M_PR || MODE == M_PR_AMEND) v = be.g;
+                else decode_g(a, be.g, q, v);
+                edges += lane == 0 ? len : 0;
+                if (MODE == M_PULL) {
+                    uint64_t best = ~0ull;
+                    for (uint32_t k = lane; k < len; k += kWave) {
+                        const uint64_t c = pull_cand(a, q, v, base[k]);
+                        best = c < best ? c : best;
+                    }
+                    for (int d = 32; d >= 1; d >>= 1) {
+                        const uint64_t o = __shfl_xor(best, d, kWave);
+                        best = o < best ? o : best;
+                    }
+                    if (lane == 0) pull_apply(a, q, v, best);
+                } else {
+                    for (uint32_t k0 = 0; k0 < len; k0 += kWave) {
+                        const uint32_t k = k0 + lane;
+                        bool want = false;
+                        uint32_t og = 0;
+                        if (k < len) want = edge_op<MODE>(a, q, v, be.val, base[k], stamp, og);
+                        emit<MODE>(a, out, &a.ctl->count[nxt], want, og);
+                    }
+                }
+            }
+        }
+        grid.sync();
+        if (gtid == 0) a.ctl->rounds = r + 1;
+    }
+    // every lane of a wave holds the same phase-A total; lane 0 reports it
+    if (lane == 0 && edges) atomicAdd(&a.ctl->edges, edges);
+}
+
+// ------------------------------------------------------------- plain kernels
+__global__ void k_ctl_setup(Ctl *c, const uint32_t *src0, uint32_t add0, uint32_t init1, const uint32_t *aff_src) {
+    if (threadIdx.x || blockIdx.x) return;
+    c->count[0] = (src0 ? *src0 : 0u) + add0;
+    c->count[1] = init1;
+    c->count[2] = 0;
+    for (int i = 0; i < 3; ++i) {
+        c->cursor[i] = 0;
+        c->big[i] = 0;
+        c->big_cursor[i] = 0;
+    }
+    c->rounds = 0;
+    c->overflow = 0;
+    c->aff_count = aff_src ? *aff_src : 0u;
+    c->edges = 0;
+    c->bad = 0;
+}
+
+__global__ void k_init_state(uint64_t *st, uint32_t *tick, uint32_t n, uint32_t Q, uint32_t app,
+                             const uint32_t *sources) {
+    const uint64_t total = (uint64_t)n * Q;
+    for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < total;
+         g += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t q = (uint32_t)(g / n), v = (uint32_t)(g - (uint64_t)q * n);
+        uint64_t s = kInfState;
+        if (app == DCG_APP_CC) s = ((uint64_t)v << 32) | kInf;
+        else if (v == sources[q]) s = (uint64_t)kInf;   // value 0, parent INF
+        st[g] = s;
+        tick[g] = 0;
+    }
+}
+
+__global__ void k_iota_sources(uint32_t *q0, uint32_t n, uint32_t Q, uint32_t app, const uint32_t *sources) {
+    if (app == DCG_APP_CC) {
+        for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x)
+            q0[v] = (uint32_t)v;
+    } else {
+        for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < Q; q += (uint64_t)gridDim.x * blockDim.x)
+            q0[q] = (uint32_t)(q * n + sources[q]);
+    }
+}
+
+// Delete seeds (driver.cuh:394-441): test every deleted edge against every query.
+__global__ void k_seed_dels(TravArgs a, const dcg_edge *recs, uint64_t count) {
+    const uint64_t total = count * a.Q;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t b = blockIdx.x * (uint64_t)blockDim.x; b < total; b += stride) {
+        const uint64_t i = b + threadIdx.x;
+        bool want = false;
+        uint32_t og = 0;
+        if (i < total) {
+            const uint32_t q = (uint32_t)(i / count);
+            const dcg_edge e = recs[i - (uint64_t)q * count];
+            if (e.u < a.n && e.v < a.n) {
+                const uint32_t vs = (uint32_t)(a.st[(uint64_t)q * a.n + e.u] >> 32);
+                want = invalidate_op(a, q, e.u, vs, e.v, og);
+            }
+        }
+        const uint64_t mask = __ballot(want);
+        if (mask) {
+            const int leader = __ffsll((unsigned long long)mask) - 1;
+            uint32_t base = 0;
+            if ((int)lane_id() == leader) base = atomicAdd(&a.ctl->aff_count, (uint32_t)__popcll(mask));
+            base = __shfl(base, leader, kWave);
+            if (want) {
+                const uint32_t pos = base + lanes_below(mask);
+                if (pos < a.affcap) a.aff[pos] = og;
+                else a.ctl->overflow = 1;
+            }
+        }
+    }
+}
+
+// Reset affected values (driver.cuh:443-463): INF, or the own id for CC.
+__global__ void k_reset_affected(uint64_t *st, const uint32_t *aff, const Ctl *c, uint32_t n, uint32_t Q, uint32_t app) {
+    const uint32_t count = c->aff_count;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t g = aff[i];
+        const uint32_t v = Q == 1 ? g : g % n;
+        st[g] = app == DCG_APP_CC ? (((uint64_t)v << 32) | kInf) : kInfState;
+    }
+}
+
+// Insertion seeds (framework.cuh:297-370): relax every added edge for every query.
+__global__ void k_seed_adds(TravArgs a, const dcg_edge *recs, uint64_t count, uint32_t *outq) {
+    const uint64_t total = count * a.Q;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t b = blockIdx.x * (uint64_t)blockDim.x; b < total; b += stride) {
+        const uint64_t i = b + threadIdx.x;
+        bool want = false;
+        uint32_t og = 0;
+        if (i < total) {
+            const uint32_t q = (uint32_t)(i / count);
+            const dcg_edge e = recs[i - (uint64_t)q * count];
+            if (e.u < a.n && e.v < a.n) {
+                const uint32_t vs = (uint32_t)(a.st[(uint64_t)q * a.n + e.u] >> 32);
+                if (vs != kInf) want = push_op(a, q, e.u, vs, e.v, a.stamp0, og);
+            }
+        }
+        const uint64_t mask = __ballot(want);
+        if (mask) {
+            const int leader = __ffsll((unsigned long long)mask) - 1;
+            uint32_t base = 0;
+            if ((int)lane_id() == leader) base = atomicAdd(&a.ctl->aff_count, (uint32_t)__popcll(mask));
+            base = __shfl(base, leader, kWave);
+            if (want) {
+                const uint32_t pos = base + lanes_below(mask);
+                if (pos < a.qcap) outq[pos] = og;
+                else a.ctl->overflow = 1;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------- host side
+uint32_t Engine::coop_blocks(const void *kernel) {
+    int per_cu = 0;
+    DCG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0));
+    if (per_cu < 1) per_cu = 1;
+    if (per_cu > 4) per_cu = 4;
+    return (uint32_t)(per_cu * cu_count);
+}
+
+static const void *persistent_fn(uint32_t mode) {
+    switch (mode) {
+    case M_PUSH: return (const void *)&k_persistent<M_PUSH>;
+    case M_PULL: return (const void *)&k_persistent<M_PULL>;
+    case M_INVALIDATE: return (const void *)&k_persistent<M_INVALIDATE>;
+    case M_PR: return (const void *)&k_persistent<M_PR>;
+    default: return (const void *)&k_persistent<M_PR_AMEND>;
+    }
+}
+
+void launch_persistent(Engine &e, uint32_t mode, TravArgs &a) {
+    const void *fn = persistent_fn(mode);
+    if (!e.coop_grid[mode]) e.coop_grid[mode] = e.coop_blocks(fn);
+    void *args[] = {&a};
+    DCG_HIP(hipLaunchCooperativeKernel(fn, dim3(e.coop_grid[mode]), dim3(kBlock), args, 0, e.stream));
+}
+
+// Stamps: each launch owns a window of 2^20 round stamps; the tick array is
+// cleared when the 32-bit stamp space would wrap.
+static uint32_t take_stamps(Engine &e) {
+    constexpr uint32_t kWindow = 1u << 20;
+    if (e.stamp > 0xFFFFFFFFu - 2 * kWindow) {
+        fill<uint32_t>(e.stream, e.d_tick.p, (uint64_t)e.n * e.Q, 0u);
+        e.stamp = 1;
+    }
+    const uint32_t s = e.stamp;
+    e.stamp += kWindow;
+    return s;
+}
+
+static TravArgs base_args(Engine &e) {
+    TravArgs a{};
+    a.s = e.view();
+    a.n = e.n;
+    a.Q = e.Q;
+    a.st = e.d_st.p;
+    a.tick = e.d_tick.p;
+    a.sources = e.d_sources.p;
+    a.w = e.wp;
+    a.app = e.app;
+    a.max_rounds = 1u << 20;
+    a.qa = e.d_qa.p;
+    a.qb = e.d_qb.p;
+    a.qcap = e.d_qa.n;
+    a.aff = e.d_aff.p;
+    a.affcap = e.d_aff.n;
+    a.big = e.d_big.p;
+    a.bigcap = e.d_big.n;
+    a.rank = e.d_rank.p;
+    a.res = e.d_res.p;
+    a.delta = e.d_delta.p;
+    a.queued = e.d_queued.p;
+    a.eps = (float)e.cfg.pr_epsilon;
+    a.sign = 1.f;
+    return a;
+}
+
+void Engine::alloc_state() {
+    const uint64_t qn = (uint64_t)n * Q;
+    if (qn >= 0xFFFFFFFFull) throw Error(DCG_ERR_UNSUPPORTED, "num_queries * num_nodes must be below 2^32");
+    const uint64_t cap = qn ? qn : 1;
+    d_qa.alloc(cap);
+    d_qb.alloc(cap);
+    d_qc.alloc(cap);
+    d_big.alloc(cap < (1ull << 26) ? cap : (1ull << 26));
+    d_ctl.alloc(8);
+    d_tick.alloc(cap);
+    fill<uint32_t>(stream, d_tick.p, cap, 0u);
+    if (app == DCG_APP_PR) {
+        d_rank.alloc(cap);
+        d_res.alloc(cap);
+        d_delta.alloc(cap);
+        d_queued.alloc(cap);
+    } else {
+        d_st.alloc(cap);
+        d_aff.alloc(cap);
+    }
+}
+
+void Engine::start_min_lattice() {
+    const uint64_t qn = (uint64_t)n * Q;
+    hipLaunchKernelGGL(k_init_state, dim3(grid_for(qn)), dim3(256), 0, stream, d_st.p, d_tick.p, n, Q, app, d_sources.p);
+    stamp = 1;
+    const uint32_t count0 = app == DCG_APP_CC ? n : Q;
+    hipLaunchKernelGGL(k_iota_sources, dim3(grid_for(count0)), dim3(256), 0, stream, d_qc.p, n, Q, app, d_sources.p);
+    Ctl *c = d_ctl.p;
+    hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c, (const uint32_t *)nullptr, count0, 0u,
+                       (const uint32_t *)nullptr);
+    TravArgs a = base_args(*this);
+    a.in0 = d_qc.p;
+    a.ctl = c;
+    a.stamp0 = take_stamps(*this);
+    launch_persistent(*this, M_PUSH, a);
+    DCG_HIP(hipGetLastError());
+    DCG_HIP(hipMemcpyAsync(h_ctl, c, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
+    sync();
+    if (h_ctl->overflow) throw Error(DCG_ERR_CAPACITY, "frontier overflow during static convergence");
+    fold_reads(nullptr);
+}
+
+// Delete stage of one snapshot (framework.cuh:4811-4957, re-ordered: the
+// physical deletion runs first; the invalidation then scans post-delete rows,
+// which only drops deleted edges that the seed step already examined).
+void Engine::delete_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
+    mutate_delete(begin, count, st);
+    if (!count) return;
+    Ctl *c = d_ctl.p;
+    const dcg_edge *recs = d_dels.p + begin;
+    TravArgs a = base_args(*this);
+    // 1. seeds -> affected list (ctl[0].aff_count)
+    hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 0, (const uint32_t *)nullptr, 0u, 0u,
+                       (const uint32_t *)nullptr);
+    a.ctl = c + 0;
+    hipLaunchKernelGGL(k_seed_dels, dim3(grid_for(count * Q)), dim3(256), 0, stream, a, recs, count);
+    // 2. invalidation closure over out-rows (frontier = newly affected)
+    hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 1, &c[0].aff_count, 0u, 0u, &c[0].aff_count);
+    a.ctl = c + 1;
+    a.in0 = d_aff.p;
+    a.stamp0 = take_stamps(*this);
+    launch_persistent(*this, M_INVALIDATE, a);
+    // 3. reset affected values
+    hipLaunchKernelGGL(k_reset_affected, dim3(grid_for((uint64_t)n * Q, 256, 8192)), dim3(256), 0, stream, d_st.p,
+                       d_aff.p, c + 1, n, Q, app);
+    // 4. one pull pass over the in-rows of the affected vertices
+    hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 2, &c[1].aff_count, 0u, 0u,
+                       (const uint32_t *)nullptr);
+    a.ctl = c + 2;
+    a.max_rounds = 1;
+    launch_persistent(*this, M_PULL, a);
+    // 5. push closure from every affected vertex
+    hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 3, &c[1].aff_count, 0u, 0u,
+                       (const uint32_t *)nullptr);
+    a.ctl = c + 3;
+    a.max_rounds = 1u << 20;
+    a.stamp0 = take_stamps(*this);
+    launch_persistent(*this, M_PUSH, a);
+    DCG_HIP(hipGetLastError());
+    DCG_HIP(hipMemcpyAsync(h_ctl, c, 4 * sizeof(Ctl), hipMemcpyDeviceToHost, stream));
+    sync();
+    for (int i = 0; i < 4; ++i)
+        if (h_ctl[i].overflow) throw Error(DCG_ERR_CAPACITY, "frontier overflow in the delete stage");
+    if (st) {
+        st->affected = h_ctl[1].aff_count;
+        st->rounds = h_ctl[1].rounds + h_ctl[3].rounds;
+        st->edges_scanned = h_ctl[1].edges + h_ctl[2].edges + h_ctl[3].edges;
+    }
+}
+
+// Add stage (framework.cuh:3525-3756): append, seed from the added edges, close.
+void Engine::add_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
+    mutate_add(begin, count, st);
+    if (!count) return;
+    Ctl *c = d_ctl.p;
+    const dcg_edge *recs = d_adds.p + begin;
+    TravArgs a = base_args(*this);
+    hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 4, (const uint32_t *)nullptr, 0u, 0u,
+                       (const uint32_t *)nullptr);
+    a.ctl = c + 4;
+    a.stamp0 = take_stamps(*this);
+    hipLaunchKernelGGL(k_seed_adds, dim3(grid_for(count * Q)), dim3(256), 0, stream, a, recs, count, d_qc.p);
+    hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 5, &c[4].aff_count, 0u, 0u,
+                       (const uint32_t *)nullptr);
+    a.ctl = c + 5;
+    a.in0 = d_qc.p;
+    a.stamp0 = take_stamps(*this);
+    launch_persistent(*this, M_PUSH, a);
+    DCG_HIP(hipGetLastError());
+    DCG_HIP(hipMemcpyAsync(h_ctl + 4, c + 4, 2 * sizeof(Ctl), hipMemcpyDeviceToHost, stream));
+    sync();
+    if (h_ctl[4].overflow || h_ctl[5].overflow) throw Error(DCG_ERR_CAPACITY, "frontier overflow in the add stage");
+    if (st) {
+        st->rounds = h_ctl[5].rounds;
+        st->edges_scanned = h_ctl[5].edges;
+    }
+}
+
+// ------------------------------------------------------------------ PageRank
+__global__ void k_pr_init(float *rank, float *res, uint32_t *queued, uint32_t *q0, uint32_t n, float base) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+        rank[v] = 0.f;
+        res[v] = base;
+        queued[v] = 1;
+        q0[v] = (uint32_t)v;
+    }
+}
+
+// Unique sources of one PR batch (adds and deletes), deduplicated with stamps.
+__global__ void k_pr_sources(const dcg_edge *adds, uint64_t na, const dcg_edge *dels, uint64_t nd, uint32_t n,
+                             uint32_t *tick, uint32_t stamp, uint32_t *out, Ctl *c) {
+    const uint64_t total = na + nd;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t b = blockIdx.x * (uint64_t)blockDim.x; b < total; b += stride) {
+        const uint64_t i = b + threadIdx.x;
+        bool want = false;
+        uint32_t u = 0;
+        if (i < total) {
+            const dcg_edge e = i < na ? adds[i] : dels[i - na];
+            u = e.u;
+            if (e.u < n && e.v < n) want = atomicExch(&tick[u], stamp) != stamp;
+        }
+        const uint64_t mask = __ballot(want);
+        if (mask) {
+            const int leader = __ffsll((unsigned long long)mask) - 1;
+            uint32_t base = 0;
+            if ((int)lane_id() == leader) base = atomicAdd(&c->aff_count, (uint32_t)__popcll(mask));
+            base = __shfl(base, leader, kWave);
+            if (want) out[base + lanes_below(mask)] = u;
+        }
+    }
+}
+
+void Engine::start_pagerank() {
+    const float base = (float)cfg.pr_base;
+    hipLaunchKernelGGL(k_pr_init, dim3(grid_for(n)), dim3(256), 0, stream, d_rank.p, d_res.p, d_queued.p, d_qc.p, n,
+                       base);
+    pr_pending = d_qc.p;
+    pr_pending_count = n;
+    Ctl *c = d_ctl.p;
+    hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 6, (const uint32_t *)nullptr,
+                       (uint32_t)pr_pending_count, 0u, (const uint32_t *)nullptr);
+    TravArgs a = base_args(*this);
+    // converge: input = pending (qc), rounds alternate qa/qb
+    a.ctl = c + 6;
+    a.in0 = pr_pending;
+    a.max_rounds = cfg.pr_max_rounds;
+    launch_persistent(*this, M_PR, a);
+    DCG_HIP(hipGetLastError());
+    DCG_HIP(hipMemcpyAsync(h_ctl + 6, c + 6, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
+    sync();
+    if (h_ctl[6].overflow) throw Error(DCG_ERR_CAPACITY, "PR frontier overflow");
+    if (h_ctl[6].bad) throw Error(DCG_ERR_NUMERIC, "PR non-finite rank/residual");
+    const uint32_t r = h_ctl[6].rounds;
+    // the unfinished frontier (if the round cap was hit) is the output of round r-1
+    if (r == 0) {
+        pr_pending_count = h_ctl[6].count[0];
+    } else {
+        pr_pending = ((r - 1) & 1) ? d_qb.p : d_qa.p;
+        pr_pending_count = h_ctl[6].count[r % 3];
+    }
+    fold_reads(nullptr);
+}
+
+// One PR batch (framework.cuh:2692-2736): Amend(-1) over the old rows of every
+// touched source, delete+add mutation, Amend(+1) over the new rows, then the
+// residual solve capped at pr_max_rounds; an unfinished frontier carries over.
+void Engine::pagerank_batch(uint64_t ab, uint64_t ac, uint64_t db, uint64_t dc, dcg_phase_stats *st) {
+    Ctl *c = d_ctl.p;
+    // Pending frontier must live in a buffer that neither amend nor converge
+    // overwrites before it is consumed: move it to qc when needed.
+    if (pr_pending != d_qc.p && pr_pending_count) {
+        DCG_HIP(hipMemcpyAsync(d_qc.p, pr_pending, pr_pending_count * 4, hipMemcpyDeviceToDevice, stream));
+    }
+    pr_pending = d_qc.p;
+    // unique touched sources (Runtime::BeginBatch sources) -> qa
+    uint32_t *srcs = d_qa.p;
+    hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 0, (const uint32_t *)nullptr, 0u, 0u,
+                       (const uint32_t *)nullptr);
+    const uint32_t stmp = take_stamps(*this);
+    hipLaunchKernelGGL(k_pr_sources, dim3(grid_for(ac + dc)), dim3(256), 0, stream, d_adds.p + ab, ac, d_dels.p + db,
+                       dc, n, d_tick.p, stmp, srcs, c + 0);
+    // Amend(-1): appends after the carried entries in qc (count[1] starts there)
+    hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 1, &c[0].aff_count, 0u,
+                       (uint32_t)pr_pending_count, (const uint32_t *)nullptr);
+    TravArgs a = base_args(*this);
+    a.ctl = c + 1;
+    a.in0 = srcs;
+    a.qa = d_qc.p;   // round 0 output buffer
+    a.max_rounds = 1;
+    a.sign = -1.f;
+    launch_persistent(*this, M_PR_AMEND, a);
+    DCG_HIP(hipGetLastError());
+    // topology mutation of the whole batch
+    dcg_phase_stats dst{}, ast{};
+    mutate_delete(db, dc, &dst);
+    mutate_add(ab, ac, &ast);
+    // Amend(+1) over the new rows; count[1] continues from ctl[1].count[1]
+    hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 2, &c[0].aff_count, 0u, 0u,
+                       (const uint32_t *)nullptr);
+    // copy the running count into ctl[2].count[1]
+    DCG_HIP(hipMemcpyAsync(&c[2].count[1], &c[1].count[1], 4, hipMemcpyDeviceToDevice, stream));
+    a.ctl = c + 2;
+    a.sign = 1.f;
+    launch_persistent(*this, M_PR_AMEND, a);
+    DCG_HIP(hipGetLastError());
+    // converge from qc (count = ctl[2].count[1])
+    hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 3, &c[2].count[1], 0u, 0u,
+                       (const uint32_t *)nullptr);
+    TravArgs b = base_args(*this);
+    b.ctl = c + 3;
+    b.in0 = d_qc.p;
+    b.max_rounds = cfg.pr_max_rounds;
+    launch_persistent(*this, M_PR, b);
+    DCG_HIP(hipGetLastError());
+    DCG_HIP(hipMemcpyAsync(h_ctl, c, 4 * sizeof(Ctl), hipMemcpyDeviceToHost, stream));
+    sync();
+    for (int i = 0; i < 4; ++i) {
+        if (h_ctl[i].overflow) throw Error(DCG_ERR_CAPACITY, "PR frontier overflow");
+        if (h_ctl[i].bad) throw Error(DCG_ERR_NUMERIC, "PR non-finite rank/residual");
+    }
+    const uint32_t r = h_ctl[3].rounds;
+    if (r == 0) {
+        pr_pending = d_qc.p;
+        pr_pending_count = h_ctl[3].count[0];
+    } else {
+        pr_pending = ((r - 1) & 1) ? d_qb.p : d_qa.p;
+        pr_pending_count = h_ctl[3].count[r % 3];
+    }
+    if (st) {
+        st->requests = ac + dc;
+        st->touched_sources = h_ctl[0].aff_count;
+        st->missing_deletes = dst.missing_deletes;
+        st->invalid_additions = ast.invalid_additions;
+        st->relocated_rows = ast.relocated_rows;
+        st->changed_sources = dst.changed_sources + ast.changed_sources;
+        st->rounds = r;
+        st->edges_scanned = h_ctl[1].edges + h_ctl[2].edges + h_ctl[3].edges;
+        st->affected = pr_pending_count;
+    }
+}
+
+}  // namespace dcg

@@ -1,0 +1,302 @@
+"""Host-side mirror of the reference engine surface over the C ABI (include/dcg.h).
+
+Method names follow sepgraph::engine::Engine (include/framework/framework.cuh):
+LoadGraph -> load_graph, InitGraph+Start -> start, get_update_file -> load_updates,
+del_edge / add_edge / UpdatePageRank, Gather{Value,Parent,Buffer}, and the cache
+maintenance sequence of samples/hybrid_sssp/hybrid_sssp.cu:494-521.
+
+There is no fallback: if libdcg.so is missing or a HIP call fails, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdcg.so")
+
+APP_SSSP, APP_BFS, APP_CC, APP_PR = 0, 1, 2, 3
+W_REFERENCE, W_SEEDED = 0, 1
+PLACE_DEVICE, PLACE_HOST = 0, 1
+CACHE_NONE, CACHE_HOTNESS, CACHE_LEARNED = 0, 1, 2
+INF = 0xFFFFFFFF
+APPS = {"sssp": APP_SSSP, "bfs": APP_BFS, "cc": APP_CC, "pr": APP_PR}
+
+
+class DcgError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"[{status}] {message}")
+        self.status = status
+
+
+class Edge(C.Structure):
+    _fields_ = [("u", C.c_uint32), ("v", C.c_uint32)]
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("app", C.c_uint32), ("num_queries", C.c_uint32), ("sources", C.POINTER(C.c_uint32)),
+        ("weight_mode", C.c_uint32), ("weight_max", C.c_uint32), ("weight_seed", C.c_uint64),
+        ("placement", C.c_uint32), ("cache_policy", C.c_uint32), ("cache_bytes", C.c_uint64),
+        ("store_slack", C.c_double), ("pr_base", C.c_double), ("pr_epsilon", C.c_double),
+        ("pr_max_rounds", C.c_uint32), ("block_size", C.c_uint32), ("device", C.c_int32),
+        ("reserved", C.c_uint32 * 7),
+    ]
+
+
+class PhaseStats(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in (
+        "requests", "touched_sources", "changed_sources", "missing_deletes", "invalid_additions",
+        "relocated_rows", "affected", "rounds", "edges_scanned", "cached_edge_reads",
+        "host_edge_reads")] + [("ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class CacheStats(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in (
+        "frames", "resident_chunks", "total_chunks", "loaded_chunks", "evicted_chunks",
+        "written_back_chunks", "h2d_bytes", "d2h_bytes")] + [("ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class EngineInfo(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in (
+        "snapshots", "store_slots", "store_used_slots", "chunks", "frames", "host_bytes",
+        "device_bytes", "cached_edge_reads", "host_edge_reads")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libdcg.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `make -C dycgraph_amd`")
+    L = C.CDLL(LIB_PATH)
+    P, U32, U64, I = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int
+    sig = {
+        "dcg_config_init": (I, [C.POINTER(Config)]),
+        "dcg_create": (I, [C.POINTER(Config), C.POINTER(P)]),
+        "dcg_destroy": (None, [P]),
+        "dcg_last_error": (C.c_char_p, [P]),
+        "dcg_status_string": (C.c_char_p, [I]),
+        "dcg_load_edges": (I, [P, U32, P, P, U64]),
+        "dcg_load_edges_device": (I, [P, U32, P, P, U64]),
+        "dcg_load_csr": (I, [P, U32, P, P]),
+        "dcg_start": (I, [P]),
+        "dcg_load_updates": (I, [P, P, U64, P, U64]),
+        "dcg_del_edge": (I, [P, U64, U64, C.POINTER(PhaseStats)]),
+        "dcg_add_edge": (I, [P, U64, U64, C.POINTER(PhaseStats)]),
+        "dcg_update_pagerank": (I, [P, U64, U64, U64, U64, C.POINTER(PhaseStats)]),
+        "dcg_cache_maintain": (I, [P, C.POINTER(CacheStats)]),
+        "dcg_cache_prefetch": (I, [P, U64, U64, U64, U64, C.POINTER(CacheStats)]),
+        "dcg_cache_policy_set": (I, [P, U32]),
+        "dcg_gather_values": (I, [P, U32, P, C.c_size_t]),
+        "dcg_gather_parents": (I, [P, U32, P, C.c_size_t]),
+        "dcg_gather_buffers": (I, [P, U32, P, C.c_size_t]),
+        "dcg_get_csr": (I, [P, P, P, C.POINTER(U64)]),
+        "dcg_num_nodes": (I, [P, C.POINTER(U32)]),
+        "dcg_get_info": (I, [P, C.POINTER(EngineInfo)]),
+        "dcg_sync": (I, [P]),
+        "dcg_gen_rmat": (I, [U32, U64, C.c_double, C.c_double, C.c_double, U64, P, P]),
+        "dcg_gen_stream": (I, [U32, P, P, U64, U32, P, P, U64, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def edges_array(pairs) -> np.ndarray:
+    """(k,2) uint32 array of (u, v) update records (the dcg_edge layout)."""
+    a = np.ascontiguousarray(np.asarray(pairs, dtype=np.uint32).reshape(-1, 2))
+    return a
+
+
+def gen_rmat(scale: int, num_edges: int, a=0.57, b=0.19, c=0.19, seed=1):
+    src = np.empty(num_edges, np.uint32)
+    dst = np.empty(num_edges, np.uint32)
+    st = lib().dcg_gen_rmat(scale, num_edges, a, b, c, seed, _ptr(src), _ptr(dst))
+    if st:
+        raise DcgError(st, "dcg_gen_rmat failed")
+    return src, dst
+
+
+def gen_stream(n, src, dst, adds_per, dels_per, seed=7):
+    adds_per = np.ascontiguousarray(adds_per, dtype=np.uint64)
+    dels_per = np.ascontiguousarray(dels_per, dtype=np.uint64)
+    adds = np.empty((int(adds_per.sum()), 2), np.uint32)
+    dels = np.empty((int(dels_per.sum()), 2), np.uint32)
+    src = np.ascontiguousarray(src, dtype=np.uint32)
+    dst = np.ascontiguousarray(dst, dtype=np.uint32)
+    st = lib().dcg_gen_stream(n, _ptr(src), _ptr(dst), len(src), len(adds_per), _ptr(adds_per),
+                              _ptr(dels_per), seed, _ptr(adds), _ptr(dels))
+    if st:
+        raise DcgError(st, "dcg_gen_stream failed")
+    return adds, dels
+
+
+@dataclass
+class EngineConfig:
+    app: str = "sssp"
+    sources: tuple = (0,)
+    weight_mode: int = W_REFERENCE
+    weight_max: int = 100
+    weight_seed: int = 1
+    placement: int = PLACE_DEVICE
+    cache_policy: int = CACHE_HOTNESS
+    cache_bytes: int = 4_000_000_000
+    store_slack: float = 0.125
+    pr_base: float = 0.15
+    pr_epsilon: float = 1e-6
+    pr_max_rounds: int = 100
+    device: int = 0
+
+
+class Engine:
+    """One engine handle (not re-entrant), owning device memory and the chunk store."""
+
+    def __init__(self, config: EngineConfig | None = None, **kw):
+        cfg = config or EngineConfig(**kw)
+        self.cfg = cfg
+        L = lib()
+        c = Config()
+        L.dcg_config_init(C.byref(c))
+        c.app = APPS[cfg.app] if isinstance(cfg.app, str) else int(cfg.app)
+        self.app = c.app
+        srcs = np.ascontiguousarray(np.asarray(cfg.sources, dtype=np.uint32).reshape(-1))
+        self._srcs = srcs
+        c.num_queries = len(srcs)
+        c.sources = srcs.ctypes.data_as(C.POINTER(C.c_uint32))
+        c.weight_mode, c.weight_max, c.weight_seed = cfg.weight_mode, cfg.weight_max, cfg.weight_seed
+        c.placement, c.cache_policy, c.cache_bytes = cfg.placement, cfg.cache_policy, cfg.cache_bytes
+        c.store_slack = cfg.store_slack
+        c.pr_base, c.pr_epsilon, c.pr_max_rounds = cfg.pr_base, cfg.pr_epsilon, cfg.pr_max_rounds
+        c.device = cfg.device
+        h = C.c_void_p()
+        self._h = None
+        self._check(L.dcg_create(C.byref(c), C.byref(h)), "dcg_create")
+        self._h = h
+        self.n = 0
+        self.queries = 1 if self.app in (APP_CC, APP_PR) else len(srcs)
+
+    def _check(self, st, what):
+        if st:
+            msg = lib().dcg_last_error(self._h).decode() if self._h else lib().dcg_status_string(st).decode()
+            raise DcgError(st, f"{what}: {msg}")
+
+    def close(self):
+        if self._h:
+            lib().dcg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # --- LoadGraph / InitGraph / Start
+    def load_graph(self, n, src, dst):
+        src = np.ascontiguousarray(src, dtype=np.uint32)
+        dst = np.ascontiguousarray(dst, dtype=np.uint32)
+        self._check(lib().dcg_load_edges(self._h, n, _ptr(src), _ptr(dst), len(src)), "dcg_load_edges")
+        self.n = n
+
+    def start(self):
+        self._check(lib().dcg_start(self._h), "dcg_start")
+
+    # --- update stream
+    def load_updates(self, adds, dels):
+        adds = edges_array(adds if len(adds) else np.zeros((0, 2)))
+        dels = edges_array(dels if len(dels) else np.zeros((0, 2)))
+        self._adds, self._dels = adds, dels
+        self._check(lib().dcg_load_updates(self._h, _ptr(adds), len(adds), _ptr(dels), len(dels)),
+                    "dcg_load_updates")
+
+    def del_edge(self, begin, count):
+        st = PhaseStats()
+        self._check(lib().dcg_del_edge(self._h, begin, count, C.byref(st)), "dcg_del_edge")
+        return st.as_dict()
+
+    def add_edge(self, begin, count):
+        st = PhaseStats()
+        self._check(lib().dcg_add_edge(self._h, begin, count, C.byref(st)), "dcg_add_edge")
+        return st.as_dict()
+
+    def update_pagerank(self, add_begin, add_count, del_begin, del_count):
+        st = PhaseStats()
+        self._check(lib().dcg_update_pagerank(self._h, add_begin, add_count, del_begin, del_count,
+                                              C.byref(st)), "dcg_update_pagerank")
+        return st.as_dict()
+
+    def cache_maintain(self):
+        st = CacheStats()
+        self._check(lib().dcg_cache_maintain(self._h, C.byref(st)), "dcg_cache_maintain")
+        return st.as_dict()
+
+    def cache_prefetch(self, add_begin, add_count, del_begin, del_count):
+        st = CacheStats()
+        self._check(lib().dcg_cache_prefetch(self._h, add_begin, add_count, del_begin, del_count,
+                                             C.byref(st)), "dcg_cache_prefetch")
+        return st.as_dict()
+
+    def set_cache_policy(self, policy):
+        self._check(lib().dcg_cache_policy_set(self._h, policy), "dcg_cache_policy_set")
+
+    # --- Gather*
+    def gather_value(self, query=0):
+        out = np.empty(self.n, np.float32 if self.app == APP_PR else np.uint32)
+        self._check(lib().dcg_gather_values(self._h, query, _ptr(out), self.n), "dcg_gather_values")
+        return out
+
+    def gather_parent(self, query=0):
+        out = np.empty(self.n, np.uint32)
+        self._check(lib().dcg_gather_parents(self._h, query, _ptr(out), self.n), "dcg_gather_parents")
+        return out
+
+    def gather_buffer(self, query=0):
+        out = np.empty(self.n, np.float32 if self.app == APP_PR else np.uint32)
+        self._check(lib().dcg_gather_buffers(self._h, query, _ptr(out), self.n), "dcg_gather_buffers")
+        return out
+
+    def csr(self):
+        m = C.c_uint64()
+        self._check(lib().dcg_get_csr(self._h, None, None, C.byref(m)), "dcg_get_csr")
+        rp = np.empty(self.n + 1, np.uint64)
+        col = np.empty(max(m.value, 1), np.uint32)
+        self._check(lib().dcg_get_csr(self._h, _ptr(rp), _ptr(col), C.byref(m)), "dcg_get_csr")
+        return rp, col[: m.value]
+
+    def info(self):
+        st = EngineInfo()
+        self._check(lib().dcg_get_info(self._h, C.byref(st)), "dcg_get_info")
+        return st.as_dict()
+
+    def sync(self):
+        self._check(lib().dcg_sync(self._h), "dcg_sync")

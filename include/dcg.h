@@ -1,0 +1,200 @@
+/*
+ * dcg.h -- C ABI of the MI355X incremental streaming-graph engine.
+ *
+ * Plain C types only (pointers, sizes, integers).  Every entry point returns 0 on
+ * success or a negative dcg_status; dcg_last_error() gives the message.  An
+ * engine handle is single-threaded and not re-entrant; distinct handles are
+ * independent.  The engine owns all device memory, the pinned host chunk store
+ * and its mirrors; input arrays are borrowed for the duration of a call.
+ *
+ * Each entry point cites the reference interface it replaces
+ * (sepgraph::engine::Engine in include/framework/framework.cuh and the sample
+ * drivers under samples/hybrid_*).
+ */
+#ifndef DCG_H
+#define DCG_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DCG_INF 0xFFFFFFFFu
+#define DCG_ABI_VERSION 1
+
+typedef struct dcg_engine dcg_engine;
+
+/* One update record: an edge u->v (weights are pure functions of (u,v)).
+ * Replaces WEdge{u,v,w} of include/framework/Loader.h:21-22 (the w column of
+ * the update file is ignored there too, framework.cuh:2611-2616). */
+typedef struct { uint32_t u, v; } dcg_edge;
+
+typedef enum {
+    DCG_OK = 0,
+    DCG_ERR_ARG = -1,       /* bad argument (null pointer, out-of-range id, bad size) */
+    DCG_ERR_HIP = -2,       /* a HIP runtime call failed                               */
+    DCG_ERR_STATE = -3,     /* call out of order (e.g. add before load)               */
+    DCG_ERR_CAPACITY = -4,  /* chunk store or frontier capacity exhausted             */
+    DCG_ERR_NUMERIC = -5,   /* PageRank produced a non-finite rank/residual           */
+    DCG_ERR_UNSUPPORTED = -6
+} dcg_status;
+
+typedef enum { DCG_APP_SSSP = 0, DCG_APP_BFS = 1, DCG_APP_CC = 2, DCG_APP_PR = 3 } dcg_app;
+
+/* Edge weights used by SSSP (BFS uses 1, CC uses 0).  REFERENCE is
+ * (u+v)%128+1 (include/framework/variants/api.cuh:26-28). SEEDED is
+ * 1 + splitmix64(seed ^ (u<<32|v)) % weight_max. */
+typedef enum { DCG_W_REFERENCE = 0, DCG_W_SEEDED = 1 } dcg_weight_mode;
+
+/* Where the chunk store lives.  DEVICE keeps every chunk in HBM; HOST keeps the
+ * authoritative chunks in pinned host memory read zero-copy, with a device
+ * chunk cache capped at cache_bytes (the reference's -cache flag,
+ * include/framework/graph_datum.cuh:164-186, graph-wide instead of per-row). */
+typedef enum { DCG_PLACE_DEVICE = 0, DCG_PLACE_HOST = 1 } dcg_placement;
+
+/* Chunk-cache admission policy.  HOTNESS is the reference's 4-snapshot
+ * activity window with an LRU tie-break (driver.cuh:486-579,
+ * framework.cuh:1264-1567); LEARNED is the online-trained MLP with HOTNESS as
+ * fallback; NONE never admits (all-zero-copy). */
+typedef enum { DCG_CACHE_NONE = 0, DCG_CACHE_HOTNESS = 1, DCG_CACHE_LEARNED = 2 } dcg_cache_policy;
+
+typedef struct {
+    uint32_t app;             /* dcg_app                                            */
+    uint32_t num_queries;     /* concurrent sources (SSSP/BFS); CC and PR use 1     */
+    const uint32_t *sources;  /* num_queries source ids (clamped to [0,n-1] at init) */
+    uint32_t weight_mode;     /* dcg_weight_mode                                    */
+    uint32_t weight_max;      /* W for DCG_W_SEEDED                                 */
+    uint64_t weight_seed;
+    uint32_t placement;       /* dcg_placement                                      */
+    uint32_t cache_policy;    /* dcg_cache_policy                                   */
+    uint64_t cache_bytes;     /* device chunk-cache cap for DCG_PLACE_HOST          */
+    double store_slack;       /* spare slot fraction for relocated rows (>= 0.05)   */
+    double pr_base;           /* PR base term per vertex (reference: 0.15)          */
+    double pr_epsilon;        /* PR residual threshold (reference --error)          */
+    uint32_t pr_max_rounds;   /* 1..100 (reference --pr_max_rounds)                 */
+    uint32_t block_size;      /* threads per workgroup for traversal kernels (256)  */
+    int32_t device;           /* HIP device ordinal                                 */
+    uint32_t reserved[7];
+} dcg_config;
+
+/* Per-phase counters (the reference's [P0-DELETE-ATTR]/[C3-*] log fields). */
+typedef struct {
+    uint64_t requests;            /* update records in this phase                      */
+    uint64_t touched_sources;     /* distinct valid sources touched                    */
+    uint64_t changed_sources;     /* sources whose row changed                         */
+    uint64_t missing_deletes;     /* deletions with no remaining occurrence            */
+    uint64_t invalid_additions;   /* additions with an endpoint id >= n                */
+    uint64_t relocated_rows;      /* rows moved to a larger power-of-two block         */
+    uint64_t affected;            /* vertices invalidated by the delete phase          */
+    uint64_t rounds;              /* closure rounds (device-side)                      */
+    uint64_t edges_scanned;       /* adjacency entries read by repair kernels          */
+    uint64_t cached_edge_reads;   /* of which served from the device chunk cache       */
+    uint64_t host_edge_reads;     /* of which served zero-copy from host memory        */
+    double   ms;                  /* wall time of the phase                            */
+} dcg_phase_stats;
+
+/* Cache maintenance counters. */
+typedef struct {
+    uint64_t frames, resident_chunks, total_chunks;
+    uint64_t loaded_chunks, evicted_chunks, written_back_chunks;
+    uint64_t h2d_bytes, d2h_bytes;
+    double   ms;
+} dcg_cache_stats;
+
+/* Engine construction.  Replaces Engine::Engine + Engine::SetOptions
+ * (framework.cuh:1158-1176, 1245-1262). */
+int dcg_config_init(dcg_config *cfg);
+int dcg_create(const dcg_config *cfg, dcg_engine **out);
+void dcg_destroy(dcg_engine *eng);
+const char *dcg_last_error(const dcg_engine *eng);
+const char *dcg_status_string(int status);
+
+/* Graph load.  Replaces Engine::LoadGraph (framework.cuh:1569-1798): rows keep
+ * file order (src/utils/parser.cpp:379-400), n = num_nodes.  src/dst may be
+ * host pointers.  dcg_load_csr takes a host CSR (row_ptr has n+1 entries). */
+int dcg_load_edges(dcg_engine *eng, uint32_t num_nodes, const uint32_t *src,
+                   const uint32_t *dst, uint64_t num_edges);
+int dcg_load_csr(dcg_engine *eng, uint32_t num_nodes, const uint64_t *row_ptr,
+                 const uint32_t *col);
+/* Same, but src/dst already in device memory (sorted or not). */
+int dcg_load_edges_device(dcg_engine *eng, uint32_t num_nodes, const uint32_t *d_src,
+                          const uint32_t *d_dst, uint64_t num_edges);
+
+/* Static convergence.  Replaces Engine::InitGraph + Engine::Start
+ * (framework.cuh:1803-1853, 1963-2005) and Engine::StartPageRank
+ * (framework.cuh:2681-2690). */
+int dcg_start(dcg_engine *eng);
+
+/* Update stream.  Replaces Loader + Engine::get_update_file
+ * (include/framework/Loader.h:30-101, framework.cuh:2387-2418): the whole
+ * stream is staged in device memory once; phases address slices of it. */
+int dcg_load_updates(dcg_engine *eng, const dcg_edge *adds, uint64_t num_adds,
+                     const dcg_edge *dels, uint64_t num_dels);
+
+/* Delete stage of one snapshot: mutation, invalidation, repair.  Replaces
+ * Engine::del_edge / update_tree_del (framework.cuh:2595-2618, 4811-4957).
+ * [begin, begin+count) indexes the staged deletions. */
+int dcg_del_edge(dcg_engine *eng, uint64_t begin, uint64_t count, dcg_phase_stats *stats);
+/* Add stage: mutation, seeding, closure.  Replaces Engine::add_edge /
+ * update_tree_add (framework.cuh:3525-3756). */
+int dcg_add_edge(dcg_engine *eng, uint64_t begin, uint64_t count, dcg_phase_stats *stats);
+/* PageRank batch (delete+add mutation between the two amend passes, then the
+ * residual solve).  Replaces Engine::UpdatePageRank (framework.cuh:2692-2736). */
+int dcg_update_pagerank(dcg_engine *eng, uint64_t add_begin, uint64_t add_count,
+                        uint64_t del_begin, uint64_t del_count, dcg_phase_stats *stats);
+
+/* Cache maintenance.  Replaces compute_hot_vertices_* + confirm_candidate_batch
+ * + CacheRefreshRequired + evication_cache + compact_cache + LoadCache
+ * (hybrid_sssp.cu:494-521, framework.cuh:1264-1567, 2978-3061). */
+int dcg_cache_maintain(dcg_engine *eng, dcg_cache_stats *stats);
+/* Prefetch for the staged batch about to be applied (learned policy only; a
+ * no-op for HOTNESS/NONE).  No reference counterpart: the reference admits
+ * only after a batch. */
+int dcg_cache_prefetch(dcg_engine *eng, uint64_t add_begin, uint64_t add_count,
+                       uint64_t del_begin, uint64_t del_count, dcg_cache_stats *stats);
+int dcg_cache_policy_set(dcg_engine *eng, uint32_t policy);
+
+/* Result gather.  Replaces Engine::GatherValue/GatherParent/GatherBuffer
+ * (framework.cuh:4959-4984, graph_datum.cuh:346-398).  values: u32 (SSSP/BFS
+ * distances, CC labels) or f32 PR ranks; buffers: PR residuals; parents: u32.
+ * n must equal the vertex count. */
+int dcg_gather_values(dcg_engine *eng, uint32_t query, void *out, size_t n);
+int dcg_gather_parents(dcg_engine *eng, uint32_t query, uint32_t *out, size_t n);
+int dcg_gather_buffers(dcg_engine *eng, uint32_t query, void *out, size_t n);
+
+/* Topology export (for the in-binary checkers; replaces ChunkStore()
+ * Descriptor/SlabData reads, hybrid_sssp.cu:228-333).  Pass row_ptr == NULL
+ * to query the edge count. */
+int dcg_get_csr(dcg_engine *eng, uint64_t *row_ptr, uint32_t *col, uint64_t *num_edges);
+int dcg_num_nodes(const dcg_engine *eng, uint32_t *n);
+
+/* Counters of the whole run so far. */
+typedef struct {
+    uint64_t snapshots, store_slots, store_used_slots, chunks, frames;
+    uint64_t host_bytes, device_bytes;
+    uint64_t cached_edge_reads, host_edge_reads;
+} dcg_engine_info;
+int dcg_get_info(dcg_engine *eng, dcg_engine_info *info);
+
+/* Synchronise the engine's stream (timing boundaries). */
+int dcg_sync(dcg_engine *eng);
+
+/* Seeded input generators (no reference counterpart: the reference reads
+ * files produced by its scripts/ tools).  RMAT: the first num_edges distinct
+ * non-self-loop pairs in draw order of a counter-based RMAT(a,b,c) stream,
+ * returned sorted by (src,dst).  Buffers are caller-allocated host memory. */
+int dcg_gen_rmat(uint32_t scale, uint64_t num_edges, double a, double b, double c,
+                 uint64_t seed, uint32_t *src, uint32_t *dst);
+/* Update stream: per snapshot `dels` deletions uniform over the edges present
+ * at the snapshot start (without replacement) and `adds` insertions uniform
+ * over absent non-loop pairs (distinct within the snapshot). The graph must be
+ * given sorted by (src,dst) without duplicates. */
+int dcg_gen_stream(uint32_t num_nodes, const uint32_t *src, const uint32_t *dst,
+                   uint64_t num_edges, uint32_t snapshots, const uint64_t *adds_per,
+                   const uint64_t *dels_per, uint64_t seed, dcg_edge *adds, dcg_edge *dels);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
