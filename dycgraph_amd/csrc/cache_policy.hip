@@ -63,6 +63,8 @@ void Engine::cache_init() {
     cache.d_touch.alloc(C);
     fill<float>(stream, cache.d_touch.p, C, 0.f);
     cache.d_score.alloc(C);
+    cache.d_rows_last.alloc(C);
+    fill<float>(stream, cache.d_rows_last.p, C, 0.f);
     if (cfg.placement == DCG_PLACE_DEVICE) {
         for (uint64_t c = 0; c < C; ++c) cache.chunk_frame[c] = (int32_t)c;
         cache.frame_chunk.resize(C);
@@ -219,7 +221,9 @@ void Engine::cache_maintain(dcg_cache_stats *st) {
     if (cfg.cache_policy == DCG_CACHE_LEARNED && cache.mlp) {
         mlp_observe(*this, reads);   // online training on this snapshot's labels
     }
-    if (cfg.cache_policy != DCG_CACHE_NONE) {
+    DCG_HIP(hipMemcpyAsync(cache.d_rows_last.p, cache.d_reads.p + C, C * 4, hipMemcpyDeviceToDevice, stream));
+    // the learned policy admits at dcg_cache_prefetch, before the next batch
+    if (cfg.cache_policy == DCG_CACHE_HOTNESS) {
         std::vector<uint64_t> desired = rank_top(score, cache.last_use, cache.frames);
         // chunks never read keep their frames rather than displacing nothing
         std::vector<uint64_t> keep;

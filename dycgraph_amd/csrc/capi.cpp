@@ -35,8 +35,51 @@ Engine::Engine(const dcg_config &c) : cfg(c) {
     else wp.kind = c.weight_mode == DCG_W_SEEDED ? W_SEEDED : W_REFERENCE;
 }
 
+int Engine::time_begin(int kernel, int ctl, double bytes) {
+    const size_t slot = timed.size();
+    if (2 * slot + 2 > events.size()) {
+        for (int i = 0; i < 32; ++i) {
+            hipEvent_t ev;
+            DCG_HIP(hipEventCreate(&ev));
+            events.push_back(ev);
+        }
+    }
+    timed.push_back(Timed{kernel, (int)(2 * slot), ctl, bytes});
+    DCG_HIP(hipEventRecord(events[2 * slot], stream));
+    return (int)slot;
+}
+
+void Engine::time_end(int slot) { DCG_HIP(hipEventRecord(events[timed[slot].ev + 1], stream)); }
+
+// Fold recorded launches into kstat (call after the stream has synchronised).
+void Engine::collect_timing() {
+    if (timed.empty()) return;
+    Ctl ctl[8];
+    if (d_ctl.p) {
+        DCG_HIP(hipMemcpyAsync(ctl, d_ctl.p, sizeof(ctl), hipMemcpyDeviceToHost, stream));
+        sync();
+    }
+    for (const Timed &t : timed) {
+        float ms = 0.f;
+        DCG_HIP(hipEventElapsedTime(&ms, events[t.ev], events[t.ev + 1]));
+        dcg_kernel_stats &k = kstat[t.kernel];
+        k.launches++;
+        k.ms += ms;
+        if (t.ctl >= 0) {
+            const Ctl &c = ctl[t.ctl];
+            k.edges += c.edges;
+            k.expansions += c.expansions;
+            k.algo_bytes += 24.0 * (double)c.expansions + 12.0 * (double)c.edges;
+        } else {
+            k.algo_bytes += t.bytes;
+        }
+    }
+    timed.clear();
+}
+
 Engine::~Engine() {
     if (stream) (void)hipStreamSynchronize(stream);
+    for (hipEvent_t ev : events) (void)hipEventDestroy(ev);
     if (h_slab) (void)hipHostFree(h_slab);
     if (h_ctl) (void)hipHostFree(h_ctl);
     if (h_counters) (void)hipHostFree(h_counters);
@@ -177,6 +220,7 @@ int dcg_start(dcg_engine *h) {
         DCG_HIP(hipMemcpyAsync(e.d_sources.p, e.sources.data(), e.sources.size() * 4, hipMemcpyHostToDevice, e.stream));
         if (e.app == DCG_APP_PR) e.start_pagerank();
         else e.start_min_lattice();
+        e.collect_timing();
         e.started = true;
     });
 }
@@ -208,6 +252,7 @@ int dcg_del_edge(dcg_engine *h, uint64_t begin, uint64_t count, dcg_phase_stats 
         if (st) *st = dcg_phase_stats{};
         const double t0 = now_ms();
         e.delete_phase(begin, count, st);
+        e.collect_timing();
         e.fold_reads(st);
         if (st) {
             st->requests = count;
@@ -224,6 +269,7 @@ int dcg_add_edge(dcg_engine *h, uint64_t begin, uint64_t count, dcg_phase_stats 
         if (st) *st = dcg_phase_stats{};
         const double t0 = now_ms();
         e.add_phase(begin, count, st);
+        e.collect_timing();
         e.fold_reads(st);
         if (st) {
             st->requests = count;
@@ -242,6 +288,7 @@ int dcg_update_pagerank(dcg_engine *h, uint64_t add_begin, uint64_t add_count, u
         if (st) *st = dcg_phase_stats{};
         const double t0 = now_ms();
         e.pagerank_batch(add_begin, add_count, del_begin, del_count, st);
+        e.collect_timing();
         e.fold_reads(st);
         if (st) st->ms = now_ms() - t0;
     });
@@ -336,6 +383,23 @@ int dcg_get_info(dcg_engine *h, dcg_engine_info *info) {
                              e.d_tick.bytes() + e.d_qa.bytes() * 3 + e.d_aff.bytes() + e.d_big.bytes();
         info->cached_edge_reads = e.cum_cached_reads;
         info->host_edge_reads = e.cum_host_reads;
+    });
+}
+
+int dcg_get_kernel_stats(dcg_engine *h, uint32_t kernel, dcg_kernel_stats *out) {
+    return guarded(h, [&](Engine &e) {
+        if (!out || kernel >= DCG_K_NUM) throw Error(DCG_ERR_ARG, "bad kernel id");
+        e.sync();
+        e.collect_timing();
+        *out = e.kstat[kernel];
+    });
+}
+
+int dcg_reset_kernel_stats(dcg_engine *h) {
+    return guarded(h, [&](Engine &e) {
+        e.sync();
+        e.collect_timing();
+        for (auto &k : e.kstat) k = dcg_kernel_stats{};
     });
 }
 

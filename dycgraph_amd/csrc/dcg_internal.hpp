@@ -130,6 +130,7 @@ struct Ctl {
     uint32_t aff_count;       // affected-list length (invalidation)
     uint32_t pad;
     unsigned long long edges; // adjacency entries scanned
+    unsigned long long expansions; // rows expanded
     unsigned long long bad;   // PR: non-finite values seen
 };
 

@@ -65,6 +65,7 @@ struct CacheState {
     DevBuf<float> d_reads;              // edges read per chunk, this snapshot
     DevBuf<float> d_hist;               // 4-snapshot history of reads (chunk-major, 4 floats)
     DevBuf<float> d_touch;              // update records touching each chunk, this batch
+    DevBuf<float> d_rows_last;          // rows expanded per chunk in the last snapshot
     DevBuf<float> d_score;
     std::vector<uint32_t> last_use;     // snapshot of last access (LRU tie-break)
     uint64_t loaded = 0, evicted = 0, written_back = 0, h2d = 0, d2h = 0;
@@ -128,6 +129,20 @@ struct Engine {
 
     // cumulative counters
     uint64_t cum_cached_reads = 0, cum_host_reads = 0;
+
+    // per-kernel measurement: HIP events around launches on `stream`
+    struct Timed {
+        int kernel;
+        int ev;
+        int ctl;       // Ctl block index whose counters price the launch (-1: mutation)
+        double bytes;  // preset bytes (mutation kernels)
+    };
+    dcg_kernel_stats kstat[DCG_K_NUM] = {};
+    std::vector<hipEvent_t> events;   // pairs (start, end)
+    std::vector<Timed> timed;
+    int time_begin(int kernel, int ctl, double bytes = 0);
+    void time_end(int slot);
+    void collect_timing();
 
     explicit Engine(const dcg_config &c);
     ~Engine();

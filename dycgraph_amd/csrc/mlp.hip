@@ -1,21 +1,453 @@
-// mlp.hip -- learned cache policy (placeholder until the MLP kernels land).
+// mlp.hip -- learned chunk-cache admission/prefetch policy.
+//
+// An MLP 8 -> 64 -> 64 -> 2 scores every chunk before a batch runs:
+//   features  log1p of the chunk's edge reads in each of the last 4 snapshots,
+//             log1p of rows expanded in the last snapshot (frontier overlap),
+//             log1p of the chunk's adjacency entries (out-degree sum),
+//             log1p of update records of the staged batch touching the chunk,
+//             recency exp(-(snapshots since last read)/4)
+//   outputs   o0 = logit(chunk is read in the coming snapshot),
+//             o1 = log1p(edge reads in the coming snapshot)
+//   score     sigmoid(o0) * max(o1, 0)
+// Weights are kept in FP16 for the device passes (FP32 masters for Adam);
+// every product accumulates in FP32.  The 64x64 layer and both of its
+// backward products run on MFMA (v_mfma_f32_32x32x8_f16), one wave per
+// 32-row tile.  After each snapshot the observed reads label the features the
+// decision used, and a few Adam steps train the network online.  The
+// hotness/LRU ranking stays as the fallback and the judge: when the network's
+// top-`frames` set captured fewer of the observed reads than the hotness set
+// would have, the next decision uses hotness.  The policy decides placement
+// only; values never depend on it.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
 #include "mlp.hpp"
 
 namespace dcg {
 
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef float float16_t __attribute__((ext_vector_type(16)));
+
+constexpr int kIn = 8, kHid = 64, kOut = 2, kTile = 32;
+constexpr int kW1 = kIn * kHid, kW2 = kHid * kHid, kW3 = kHid * kOut;
+// parameter block layout (FP32 masters and gradients use the same layout)
+constexpr int P_W1 = 0, P_B1 = P_W1 + kW1, P_W2 = P_B1 + kHid, P_B2 = P_W2 + kW2, P_W3 = P_B2 + kHid,
+              P_B3 = P_W3 + kW3, P_NUM = P_B3 + kOut;
+
+constexpr int kReplay = 4;   // decisions kept for training (ring)
+
 struct MlpState {
-    int dummy = 0;
+    DevBuf<float> master, grad, m1, m2;   // FP32 masters, gradients, Adam moments
+    DevBuf<_Float16> w16;                 // FP16 weights used by the device passes
+    DevBuf<float> X, Y, out, degsum, lastuse;   // X/Y hold kReplay decisions of C rows each
+    uint64_t rows = 0;                    // C (rows per decision)
+    int slot = 0, filled = 0;             // ring position of the current decision
+    int steps = 0;
+    bool have_decision = false, fallback = true;
+    double mlp_capture = 0, hot_capture = 0;
+    std::vector<float> hot_score_at_decision;
+    int decisions = 0, fallbacks = 0;
 };
 
 void MlpDeleter::operator()(MlpState *m) const { delete m; }
 
-void mlp_create(Engine &e) { e.cache.mlp.reset(new MlpState()); }
-void mlp_observe(Engine &, const std::vector<float> &) {}
-std::vector<uint64_t> mlp_plan(Engine &e, uint64_t, uint64_t, uint64_t, uint64_t) {
-    std::vector<uint64_t> keep;
-    for (uint64_t f = 0; f < e.cache.frames; ++f)
-        if (e.cache.frame_chunk[f] >= 0) keep.push_back((uint64_t)e.cache.frame_chunk[f]);
-    return keep;
+// ---------------------------------------------------------------- device part
+// C(32x32 tile) += A(32xK) * B(Kx32) on one wave.  Operand layout of
+// v_mfma_f32_32x32x8_f16: lane l supplies A[l%32][k0+4*(l/32)+j] and
+// B[k0+4*(l/32)+j][l%32], j=0..3; accumulator register r of lane l holds
+// C[8*(r/4) + 4*(l/32) + r%4][l%32].
+template <typename FA, typename FB>
+__device__ __forceinline__ float16_t mfma_tile(FA a, FB b, int K, float16_t acc) {
+    const int l = (int)lane_id();
+    const int i = l & 31, kq = 4 * (l >> 5);
+    for (int k0 = 0; k0 < K; k0 += 8) {
+        half4_t av, bv;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            av[j] = a(i, k0 + kq + j);
+            bv[j] = b(k0 + kq + j, i);
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x8f16(av, bv, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+__device__ __forceinline__ int acc_row(int r) { return 8 * (r >> 2) + 4 * ((int)lane_id() >> 5) + (r & 3); }
+
+struct TileLds {
+    _Float16 w2[kHid][kHid];      // W2[i][j] (in i, out j)
+    _Float16 h1[kTile][kHid];     // layer-1 activations
+    _Float16 dz2[kTile][kHid];    // layer-2 pre-activation gradient
+    float z2[kTile][kHid];        // layer-2 pre-activations (+bias)
+    float x[kTile][kIn];
+    float dout[kTile][kOut];
+    float w1[kIn * kHid];
+    float b1[kHid], b2[kHid], w3[kHid * kOut], b3[kOut];
+};
+
+// Forward (and, with y != null, backward) of one 32-row tile per workgroup
+// (one wave).  Gradients are summed into grad[] (mean over `rows`).
+__global__ __launch_bounds__(64) void k_mlp_tile(const float *X, const float *Y, uint64_t rows, const _Float16 *w16,
+                                                 const float *master, float *out, float *grad) {
+    __shared__ TileLds L;
+    const int l = (int)lane_id();
+    const uint64_t r0 = (uint64_t)blockIdx.x * kTile;
+    // stage parameters: W2 in FP16, the small layers in FP32 from the FP16 copy
+    for (int t = l; t < kW2; t += 64) L.w2[t / kHid][t % kHid] = w16[P_W2 + t];
+    for (int t = l; t < kW1; t += 64) L.w1[t] = (float)w16[P_W1 + t];
+    for (int t = l; t < kW3; t += 64) L.w3[t] = (float)w16[P_W3 + t];
+    L.b1[l] = master[P_B1 + l];
+    L.b2[l] = master[P_B2 + l];
+    if (l < kOut) L.b3[l] = master[P_B3 + l];
+    for (int t = l; t < kTile * kIn; t += 64) {
+        const uint64_t r = r0 + t / kIn;
+        L.x[t / kIn][t % kIn] = r < rows ? X[r * kIn + t % kIn] : 0.f;
+    }
+    __syncthreads();
+    // layer 1 (FP32 FMA): lane owns hidden unit l for all 32 rows
+    for (int i = 0; i < kTile; ++i) {
+        float a = L.b1[l];
+#pragma unroll
+        for (int k = 0; k < kIn; ++k) a += L.x[i][k] * L.w1[k * kHid + l];
+        L.h1[i][l] = (_Float16)(a > 0.f ? a : 0.f);
+    }
+    __syncthreads();
+    // layer 2 on MFMA: Z2 = H1 * W2 (two 32x32 output tiles)
+    for (int nt = 0; nt < 2; ++nt) {
+        float16_t acc = {};
+        acc = mfma_tile([&](int i, int k) { return L.h1[i][k]; },
+                        [&](int k, int j) { return L.w2[k][nt * 32 + j]; }, kHid, acc);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) L.z2[acc_row(r)][nt * 32 + (l & 31)] = acc[r] + L.b2[nt * 32 + (l & 31)];
+    }
+    __syncthreads();
+    // layer 3 (FP32): lanes 0..63 -> (row = l/2, output = l%2)
+    {
+        const int i = l >> 1, o = l & 1;
+        float a = L.b3[o];
+        for (int k = 0; k < kHid; ++k) {
+            const float h = L.z2[i][k] > 0.f ? L.z2[i][k] : 0.f;
+            a += h * L.w3[k * kOut + o];
+        }
+        const uint64_t r = r0 + i;
+        if (r < rows && out) out[r * kOut + o] = a;
+        if (Y) {
+            float g = 0.f;
+            if (r < rows) {
+                const float t = Y[r * kOut + o];
+                // unscaled per-row gradient (keeps the FP16 MFMA operands
+                // in range); the 1/rows of the mean is applied at the sums
+                g = o == 0 ? (1.f / (1.f + __expf(-a)) - t) : (a - t);
+            }
+            L.dout[i][o] = g;
+        }
+    }
+    if (!Y) return;
+    const float inv = 1.f / (float)rows;
+    __syncthreads();
+    // dW3, db3, dZ2 = (dout * W3^T) .* relu'(Z2)
+    {
+        // lane owns hidden unit l: dW3[l][o] = sum_i relu(z2[i][l]) * dout[i][o]
+        float g0 = 0.f, g1 = 0.f;
+        for (int i = 0; i < kTile; ++i) {
+            const float h = L.z2[i][l] > 0.f ? L.z2[i][l] : 0.f;
+            g0 += h * L.dout[i][0];
+            g1 += h * L.dout[i][1];
+            const float dh = L.dout[i][0] * L.w3[l * kOut + 0] + L.dout[i][1] * L.w3[l * kOut + 1];
+            L.dz2[i][l] = (_Float16)(L.z2[i][l] > 0.f ? dh : 0.f);
+        }
+        atomicAdd(&grad[P_W3 + l * kOut + 0], g0 * inv);
+        atomicAdd(&grad[P_W3 + l * kOut + 1], g1 * inv);
+        if (l < kOut) {
+            float b = 0.f;
+            for (int i = 0; i < kTile; ++i) b += L.dout[i][l];
+            atomicAdd(&grad[P_B3 + l], b * inv);
+        }
+    }
+    __syncthreads();
+    // db2
+    {
+        float b = 0.f;
+        for (int i = 0; i < kTile; ++i) b += (float)L.dz2[i][l];
+        atomicAdd(&grad[P_B2 + l], b * inv);
+    }
+    // dW2 = H1^T * dZ2 on MFMA (M=64 in two tiles, N=64 in two tiles, K=32 rows)
+    for (int mt = 0; mt < 2; ++mt)
+        for (int nt = 0; nt < 2; ++nt) {
+            float16_t acc = {};
+            acc = mfma_tile([&](int i, int k) { return L.h1[k][mt * 32 + i]; },
+                            [&](int k, int j) { return L.dz2[k][nt * 32 + j]; }, kTile, acc);
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                atomicAdd(&grad[P_W2 + (mt * 32 + acc_row(r)) * kHid + nt * 32 + (l & 31)], acc[r] * inv);
+        }
+    // dH1 = dZ2 * W2^T on MFMA, then dZ1 = dH1 .* relu'(Z1); reuse z2 as dH1 storage
+    __syncthreads();
+    for (int nt = 0; nt < 2; ++nt) {
+        float16_t acc = {};
+        acc = mfma_tile([&](int i, int k) { return L.dz2[i][k]; },
+                        [&](int k, int j) { return L.w2[nt * 32 + j][k]; }, kHid, acc);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) L.z2[acc_row(r)][nt * 32 + (l & 31)] = acc[r];
+    }
+    __syncthreads();
+    {
+        // lane owns hidden unit l of layer 1
+        float gb = 0.f, gw[kIn] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int i = 0; i < kTile; ++i) {
+            const float dz = (float)L.h1[i][l] > 0.f ? L.z2[i][l] : 0.f;
+            gb += dz;
+#pragma unroll
+            for (int k = 0; k < kIn; ++k) gw[k] += L.x[i][k] * dz;
+        }
+        atomicAdd(&grad[P_B1 + l], gb * inv);
+#pragma unroll
+        for (int k = 0; k < kIn; ++k) atomicAdd(&grad[P_W1 + k * kHid + l], gw[k] * inv);
+    }
+}
+
+__global__ void k_adam(float *master, const float *grad, float *m1, float *m2, _Float16 *w16, int n, float lr,
+                       float b1c, float b2c) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float g = grad[i];
+        const float a = 0.9f * m1[i] + 0.1f * g;
+        const float b = 0.999f * m2[i] + 0.001f * g * g;
+        m1[i] = a;
+        m2[i] = b;
+        const float w = master[i] - lr * (a / b1c) / (sqrtf(b / b2c) + 1e-8f);
+        master[i] = w;
+        w16[i] = (_Float16)w;
+    }
+}
+
+__global__ void k_to_half(const float *a, _Float16 *b, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = (_Float16)a[i];
+}
+
+// Update records of the staged batch per chunk (rows of the touched out- and in-rows).
+__global__ void k_touch(const RowDesc *out, const RowDesc *in, const dcg_edge *adds, uint64_t na, const dcg_edge *dels,
+                        uint64_t nd, uint32_t n, float *touch) {
+    const uint64_t total = na + nd;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x) {
+        const dcg_edge e = i < na ? adds[i] : dels[i - na];
+        if (e.u >= n || e.v >= n) continue;
+        const RowDesc a = out[e.u], b = in[e.v];
+        if (a.cap) atomicAdd(&touch[a.off >> kChunkLog2], 1.f);
+        if (b.cap) atomicAdd(&touch[b.off >> kChunkLog2], 1.f);
+    }
+}
+
+__global__ void k_degsum(const RowDesc *out, const RowDesc *in, uint32_t n, float *degsum) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < 2ull * n; v += (uint64_t)gridDim.x * blockDim.x) {
+        const RowDesc d = v < n ? out[v] : in[v - n];
+        if (!d.deg) continue;
+        uint64_t a = d.off, e = d.off + d.deg;
+        while (a < e) {
+            const uint64_t c = a >> kChunkLog2, ce = (c + 1) << kChunkLog2, b = e < ce ? e : ce;
+            atomicAdd(&degsum[c], (float)(b - a));
+            a = b;
+        }
+    }
+}
+
+__global__ void k_features(const float *hist, const float *rows_last, const float *degsum, const float *touch,
+                           const float *lastuse, float now, uint64_t C, float *X) {
+    for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < C; c += (uint64_t)gridDim.x * blockDim.x) {
+        const float4 h = reinterpret_cast<const float4 *>(hist)[c];
+        float *x = X + c * kIn;
+        x[0] = log1pf(h.x);
+        x[1] = log1pf(h.y);
+        x[2] = log1pf(h.z);
+        x[3] = log1pf(h.w);
+        x[4] = log1pf(rows_last[c]);
+        x[5] = log1pf(degsum[c]) * 0.25f;
+        x[6] = log1pf(touch[c]);
+        x[7] = lastuse[c] > 0.f ? __expf(-(now - lastuse[c]) * 0.25f) : 0.f;
+    }
+}
+
+__global__ void k_labels(const float *reads, uint64_t C, float *Y) {
+    for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < C; c += (uint64_t)gridDim.x * blockDim.x) {
+        Y[2 * c] = reads[c] > 0.f ? 1.f : 0.f;
+        Y[2 * c + 1] = log1pf(reads[c]);
+    }
+}
+
+// ------------------------------------------------------------------ host part
+static void mlp_train_steps(Engine &e, MlpState &m, int steps) {
+    const uint64_t rows = m.rows * (uint64_t)m.filled;   // every labelled decision in the ring
+    for (int s = 0; s < steps; ++s) {
+        fill<float>(e.stream, m.grad.p, P_NUM, 0.f);
+        const uint32_t tiles = (uint32_t)((rows + kTile - 1) / kTile);
+        hipLaunchKernelGGL(k_mlp_tile, dim3(tiles), dim3(64), 0, e.stream, m.X.p, m.Y.p, rows, m.w16.p, m.master.p,
+                           (float *)nullptr, m.grad.p);
+        m.steps++;
+        const float b1c = 1.f - std::pow(0.9f, (float)m.steps), b2c = 1.f - std::pow(0.999f, (float)m.steps);
+        hipLaunchKernelGGL(k_adam, dim3(grid_for(P_NUM)), dim3(256), 0, e.stream, m.master.p, m.grad.p, m.m1.p,
+                           m.m2.p, m.w16.p, P_NUM, 3e-3f, b1c, b2c);
+    }
+    DCG_HIP(hipGetLastError());
+}
+
+void mlp_create(Engine &e) {
+    MlpState *m = new MlpState();
+    e.cache.mlp.reset(m);
+    const uint64_t C = e.cache.chunks;
+    m->master.alloc(P_NUM);
+    m->grad.alloc(P_NUM);
+    m->m1.alloc(P_NUM);
+    m->m2.alloc(P_NUM);
+    m->w16.alloc(P_NUM);
+    m->X.alloc(kReplay * C * kIn);
+    m->Y.alloc(kReplay * C * kOut);
+    m->out.alloc(C * kOut);
+    m->degsum.alloc(C);
+    m->lastuse.alloc(C);
+    m->rows = C;
+    // He-style init from a fixed seed (splitmix64 -> uniform)
+    std::vector<float> w(P_NUM, 0.f);
+    uint64_t st = 0x5eed5eedull;
+    auto uni = [&]() { return (float)((mix64(st++) >> 11) * (1.0 / 9007199254740992.0)) * 2.f - 1.f; };
+    for (int i = 0; i < kW1; ++i) w[P_W1 + i] = uni() * std::sqrt(6.f / kIn);
+    for (int i = 0; i < kW2; ++i) w[P_W2 + i] = uni() * std::sqrt(6.f / kHid);
+    for (int i = 0; i < kW3; ++i) w[P_W3 + i] = uni() * std::sqrt(6.f / kHid);
+    DCG_HIP(hipMemcpyAsync(m->master.p, w.data(), P_NUM * 4, hipMemcpyHostToDevice, e.stream));
+    fill<float>(e.stream, m->m1.p, P_NUM, 0.f);
+    fill<float>(e.stream, m->m2.p, P_NUM, 0.f);
+    hipLaunchKernelGGL(k_to_half, dim3(grid_for(P_NUM)), dim3(256), 0, e.stream, m->master.p, m->w16.p, P_NUM);
+    fill<float>(e.stream, m->degsum.p, C, 0.f);
+    hipLaunchKernelGGL(k_degsum, dim3(grid_for(2ull * e.n, 256, 8192)), dim3(256), 0, e.stream, e.d_out.p, e.d_in.p,
+                       e.n, m->degsum.p);
+    DCG_HIP(hipGetLastError());
+    e.sync();
+}
+
+// Fraction of `reads` captured by the top-k chunks of `score`.
+static double capture(const std::vector<float> &score, const std::vector<float> &reads, uint64_t k) {
+    std::vector<uint64_t> idx(score.size());
+    std::iota(idx.begin(), idx.end(), 0ull);
+    if (k < idx.size())
+        std::nth_element(idx.begin(), idx.begin() + (ptrdiff_t)k, idx.end(),
+                         [&](uint64_t a, uint64_t b) { return score[a] > score[b] || (score[a] == score[b] && a < b); });
+    double got = 0, total = 0;
+    for (float r : reads) total += r;
+    for (uint64_t i = 0; i < std::min<uint64_t>(k, idx.size()); ++i) got += reads[idx[i]];
+    return total > 0 ? got / total : 1.0;
+}
+
+void mlp_observe(Engine &e, const std::vector<float> &reads) {
+    MlpState &m = *e.cache.mlp;
+    if (!m.have_decision) return;
+    const uint64_t C = e.cache.chunks;
+    // label the decision's features with what the snapshot actually read
+    hipLaunchKernelGGL(k_labels, dim3(grid_for(C)), dim3(256), 0, e.stream, e.cache.d_reads.p, C,
+                       m.Y.p + (uint64_t)m.slot * C * kOut);
+    std::vector<float> out(C * kOut), mscore(C);
+    DCG_HIP(hipMemcpyAsync(out.data(), m.out.p, C * kOut * 4, hipMemcpyDeviceToHost, e.stream));
+    e.sync();
+    for (uint64_t c = 0; c < C; ++c) {
+        const float p = 1.f / (1.f + std::exp(-out[2 * c]));
+        mscore[c] = p * std::max(out[2 * c + 1], 0.f);
+    }
+    // judge: which ranking would have captured more of the observed reads
+    m.mlp_capture = capture(mscore, reads, e.cache.frames);
+    m.hot_capture = capture(m.hot_score_at_decision, reads, e.cache.frames);
+    m.filled = std::min(kReplay, m.filled + 1);
+    m.slot = (m.slot + 1) % kReplay;
+    mlp_train_steps(e, m, 16);
+    m.fallback = m.steps < 32 || m.mlp_capture < m.hot_capture;
+    m.have_decision = false;
+}
+
+std::vector<uint64_t> mlp_plan(Engine &e, uint64_t ab, uint64_t ac, uint64_t db, uint64_t dc) {
+    MlpState &m = *e.cache.mlp;
+    const uint64_t C = e.cache.chunks;
+    if ((e.snapshots & 7) == 0) {
+        fill<float>(e.stream, m.degsum.p, C, 0.f);
+        hipLaunchKernelGGL(k_degsum, dim3(grid_for(2ull * e.n, 256, 8192)), dim3(256), 0, e.stream, e.d_out.p,
+                           e.d_in.p, e.n, m.degsum.p);
+    }
+    fill<float>(e.stream, e.cache.d_touch.p, C, 0.f);
+    hipLaunchKernelGGL(k_touch, dim3(grid_for(ac + dc, 256, 8192)), dim3(256), 0, e.stream, e.d_out.p, e.d_in.p,
+                       e.d_adds.p + ab, ac, e.d_dels.p + db, dc, e.n, e.cache.d_touch.p);
+    std::vector<float> lu(C);
+    for (uint64_t c = 0; c < C; ++c) lu[c] = (float)e.cache.last_use[c];
+    DCG_HIP(hipMemcpyAsync(m.lastuse.p, lu.data(), C * 4, hipMemcpyHostToDevice, e.stream));
+    float *X = m.X.p + (uint64_t)m.slot * C * kIn;
+    hipLaunchKernelGGL(k_features, dim3(grid_for(C)), dim3(256), 0, e.stream, e.cache.d_hist.p,
+                       e.cache.d_rows_last.p, m.degsum.p, e.cache.d_touch.p, m.lastuse.p, (float)e.snapshots, C, X);
+    const uint32_t tiles = (uint32_t)((C + kTile - 1) / kTile);
+    hipLaunchKernelGGL(k_mlp_tile, dim3(tiles), dim3(64), 0, e.stream, X, (const float *)nullptr, C, m.w16.p,
+                       m.master.p, m.out.p, (float *)nullptr);
+    DCG_HIP(hipGetLastError());
+    std::vector<float> out(C * kOut), hist(4 * C);
+    DCG_HIP(hipMemcpyAsync(out.data(), m.out.p, C * kOut * 4, hipMemcpyDeviceToHost, e.stream));
+    DCG_HIP(hipMemcpyAsync(hist.data(), e.cache.d_hist.p, 4 * C * 4, hipMemcpyDeviceToHost, e.stream));
+    e.sync();
+    std::vector<float> score(C);
+    m.hot_score_at_decision.assign(C, 0.f);
+    for (uint64_t c = 0; c < C; ++c) {
+        m.hot_score_at_decision[c] = hist[4 * c] + hist[4 * c + 1] + hist[4 * c + 2] + hist[4 * c + 3];
+        const float p = 1.f / (1.f + std::exp(-out[2 * c]));
+        score[c] = p * std::max(out[2 * c + 1], 0.f);
+    }
+    const std::vector<float> &use = m.fallback ? m.hot_score_at_decision : score;
+    m.decisions++;
+    m.fallbacks += m.fallback ? 1 : 0;
+    m.have_decision = true;
+    std::vector<uint64_t> idx(C);
+    std::iota(idx.begin(), idx.end(), 0ull);
+    const uint64_t k = std::min<uint64_t>(e.cache.frames, C);
+    auto better = [&](uint64_t a, uint64_t b) {
+        if (use[a] != use[b]) return use[a] > use[b];
+        if (e.cache.last_use[a] != e.cache.last_use[b]) return e.cache.last_use[a] > e.cache.last_use[b];
+        return a < b;
+    };
+    if (k < C) std::nth_element(idx.begin(), idx.begin() + (ptrdiff_t)k, idx.end(), better);
+    idx.resize(k);
+    return idx;
 }
 
 }  // namespace dcg
+
+// Evaluate the predictor on caller-provided rows and FP32 parameters (rounded to
+// FP16 exactly as the policy does): outputs, and with y != NULL the gradient of
+// the mean loss.  Host pointers.  Exposed so the device passes can be checked
+// against an FP64 reference; the policy itself never calls it.
+extern "C" int dcg_mlp_eval(int32_t device, uint32_t rows, const float *x, const float *y, const float *params,
+                            float *out, float *grad) {
+    using namespace dcg;
+    if (!x || !params || !out || rows == 0) return DCG_ERR_ARG;
+    try {
+        DCG_HIP(hipSetDevice(device));
+        hipStream_t s;
+        DCG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        DevBuf<float> dx, dy, dp, dout, dg;
+        DevBuf<_Float16> w16;
+        dx.alloc((uint64_t)rows * kIn);
+        dp.alloc(P_NUM);
+        dout.alloc((uint64_t)rows * kOut);
+        dg.alloc(P_NUM);
+        w16.alloc(P_NUM);
+        DCG_HIP(hipMemcpyAsync(dx.p, x, (uint64_t)rows * kIn * 4, hipMemcpyHostToDevice, s));
+        DCG_HIP(hipMemcpyAsync(dp.p, params, P_NUM * 4, hipMemcpyHostToDevice, s));
+        if (y) {
+            dy.alloc((uint64_t)rows * kOut);
+            DCG_HIP(hipMemcpyAsync(dy.p, y, (uint64_t)rows * kOut * 4, hipMemcpyHostToDevice, s));
+        }
+        hipLaunchKernelGGL(k_to_half, dim3(grid_for(P_NUM)), dim3(256), 0, s, dp.p, w16.p, P_NUM);
+        fill<float>(s, dg.p, P_NUM, 0.f);
+        const uint32_t tiles = (rows + kTile - 1) / kTile;
+        hipLaunchKernelGGL(k_mlp_tile, dim3(tiles), dim3(64), 0, s, dx.p, y ? dy.p : (const float *)nullptr,
+                           (uint64_t)rows, w16.p, dp.p, dout.p, y ? dg.p : (float *)nullptr);
+        DCG_HIP(hipGetLastError());
+        DCG_HIP(hipMemcpyAsync(out, dout.p, (uint64_t)rows * kOut * 4, hipMemcpyDeviceToHost, s));
+        if (y && grad) DCG_HIP(hipMemcpyAsync(grad, dg.p, P_NUM * 4, hipMemcpyDeviceToHost, s));
+        DCG_HIP(hipStreamSynchronize(s));
+        DCG_HIP(hipStreamDestroy(s));
+    } catch (const Error &e) {
+        return e.status;
+    }
+    return DCG_OK;
+}

@@ -27,6 +27,8 @@ enum Counter : int {
     C_BAD_ID = 6,
     C_CACHED_READS = 7,
     C_HOST_READS = 8,
+    C_SLOTS_READ = 9,
+    C_SLOTS_WRITTEN = 10,
     C_NUM = 16
 };
 
@@ -334,13 +336,13 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, co
     const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kWave;
     const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / kWave;
     const uint64_t ngroups = *ngroups_p;
-    unsigned long long missing = 0, changed = 0;
+    unsigned long long missing = 0, changed = 0, rd = 0, wr = 0;
     for (uint64_t g = wave; g < ngroups; g += nwaves) {
         const uint64_t gs = gstart[g];
         const uint32_t row = (uint32_t)(keys[gs] >> 32);
         const uint64_t ge = group_end<uint64_t>(keys, gs, count, row);
         const RowDesc d = desc[row];
-        uint32_t write = 0, removed = 0;
+        uint32_t write = 0, removed = 0, moved = 0;
         for (uint32_t t0 = 0; t0 < d.deg; t0 += kWave) {
             const uint32_t i = t0 + lane;
             const bool valid = i < d.deg;
@@ -375,7 +377,9 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, co
             const uint32_t rtile = (uint32_t)__popcll(__ballot(rem));
             if (removed + rtile) {
                 const uint32_t pos = write + lanes_below(km);
-                if (keep && pos != i) store_slot(s, d.off + pos, x);
+                const bool mv = keep && pos != i;
+                if (mv) store_slot(s, d.off + pos, x);
+                moved += (uint32_t)__popcll(__ballot(mv));
             }
             write += (uint32_t)__popcll(km);
             removed += rtile;
@@ -384,11 +388,15 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, co
             if (removed) desc[row].deg = write;
             missing += (ge - gs) - removed;
             changed += removed ? 1 : 0;
+            rd += d.deg;
+            wr += moved;
         }
     }
-    if (count_stats && lane == 0) {
-        if (missing) atomicAdd(&counters[C_MISSING], missing);
-        if (changed) atomicAdd(&counters[C_CHANGED], changed);
+    if (lane == 0) {
+        if (count_stats && missing) atomicAdd(&counters[C_MISSING], missing);
+        if (count_stats && changed) atomicAdd(&counters[C_CHANGED], changed);
+        if (rd) atomicAdd(&counters[C_SLOTS_READ], rd);
+        if (wr) atomicAdd(&counters[C_SLOTS_WRITTEN], wr);
     }
 }
 
@@ -436,7 +444,7 @@ __global__ __launch_bounds__(256) void k_add_rows(StoreView s, RowDesc *desc, co
     const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / kWave;
     const uint64_t ngroups = *ngroups_p;
     const bool overflow = tail[T_OVERFLOW] != 0;
-    unsigned long long relocated = 0;
+    unsigned long long relocated = 0, rd = 0, wr = 0;
     for (uint64_t g = wave; g < ngroups; g += nwaves) {
         const uint64_t gs = gstart[g];
         const uint32_t row = keys[gs];
@@ -455,12 +463,19 @@ __global__ __launch_bounds__(256) void k_add_rows(StoreView s, RowDesc *desc, co
             d.off = noff;
             d.cap = (uint32_t)(1ull << cls);
             relocated++;
+            rd += d.deg;
+            wr += d.deg;
         }
+        wr += k;
         for (uint32_t i = lane; i < k; i += kWave) store_slot(s, d.off + d.deg + i, vals[gs + i]);
         d.deg = (uint32_t)nd;
         if (lane == 0) desc[row] = d;
     }
-    if (count_stats && lane == 0 && relocated) atomicAdd(&counters[C_RELOCATED], relocated);
+    if (lane == 0) {
+        if (count_stats && relocated) atomicAdd(&counters[C_RELOCATED], relocated);
+        if (rd) atomicAdd(&counters[C_SLOTS_READ], rd);
+        if (wr) atomicAdd(&counters[C_SLOTS_WRITTEN], wr);
+    }
 }
 
 static uint32_t wave_grid(uint64_t groups) {
@@ -497,14 +512,18 @@ void Engine::mutate_delete(uint64_t begin, uint64_t count, dcg_phase_stats *st) 
     StoreView v = view();
     const uint32_t wg = wave_grid(count);
     fill<uint32_t>(stream, d_used.p, count, 0u);
+    const int slot = time_begin(DCG_K_DELETE_ROWS, -1);
     hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream, v, d_out.p, kout, count, d_gstart.p,
                        d_counters.p + C_GROUPS_OUT, d_used.p, d_counters.p, 1);
     fill<uint32_t>(stream, d_used.p, count, 0u);
     hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream, v, d_in.p, kin, count, d_gstart2.p,
                        d_counters.p + C_GROUPS_IN, d_used.p, d_counters.p, 0);
+    time_end(slot);
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipMemcpyAsync(h_counters, d_counters.p, C_NUM * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
     sync();
+    timed[slot].bytes = 4.0 * (double)(h_counters[C_SLOTS_READ] + h_counters[C_SLOTS_WRITTEN]) +
+                        16.0 * (double)(h_counters[C_GROUPS_OUT] + h_counters[C_GROUPS_IN]);
     if (st) {
         st->touched_sources = h_counters[C_GROUPS_OUT];
         st->changed_sources = h_counters[C_CHANGED];
@@ -547,15 +566,19 @@ void Engine::mutate_add(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
     hipLaunchKernelGGL(k_alloc_classes, dim3(1), dim3(64), 0, stream, d_tail.p, total_slots);
     StoreView v = view();
     const uint32_t wg = wave_grid(count);
+    const int slot = time_begin(DCG_K_APPEND_ROWS, -1);
     hipLaunchKernelGGL(k_add_rows, dim3(wg), dim3(256), 0, stream, v, d_out.p, d_k32a.p, d_v32a.p, count, d_gstart.p,
                        d_counters.p + C_GROUPS_OUT, d_tail.p, d_counters.p, 1);
     hipLaunchKernelGGL(k_add_rows, dim3(wg), dim3(256), 0, stream, v, d_in.p, d_k32b.p, d_v32b.p, count, d_gstart2.p,
                        d_counters.p + C_GROUPS_IN, d_tail.p, d_counters.p, 1);
+    time_end(slot);
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipMemcpyAsync(h_counters, d_counters.p, C_NUM * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
     uint64_t tl[2];
     DCG_HIP(hipMemcpyAsync(tl, d_tail.p, 16, hipMemcpyDeviceToHost, stream));
     sync();
+    timed[slot].bytes = 4.0 * (double)(h_counters[C_SLOTS_READ] + h_counters[C_SLOTS_WRITTEN]) +
+                        16.0 * (double)(h_counters[C_GROUPS_OUT] + h_counters[C_GROUPS_IN]);
     if (tl[T_OVERFLOW]) throw Error(DCG_ERR_CAPACITY, "chunk store slot space exhausted (raise store_slack)");
     used_slots = tl[T_TAIL];
     if (st) {

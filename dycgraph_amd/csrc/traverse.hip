@@ -191,7 +191,7 @@ __global__ __launch_bounds__(256) void k_persistent(TravArgs a) {
     WaveLds &L = lds_all[wib];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t gthreads = gridDim.x * blockDim.x;
-    uint64_t edges = 0;
+    uint64_t edges = 0, expansions = 0;
 
     for (uint32_t r = 0; r < a.max_rounds; ++r) {
         const uint32_t cur = r % 3, nxt = (r + 1) % 3, nn = (r + 2) % 3;
@@ -260,6 +260,7 @@ __global__ __launch_bounds__(256) void k_persistent(TravArgs a) {
                     }
                 }
             }
+            expansions += (uint64_t)__popcll(__ballot(deg > 0));
             if (deg > kBigRow) {
                 const uint64_t pieces = (deg + kPiece - 1) / kPiece;
                 const unsigned long long p = atomicAdd(&a.ctl->big[cur], (1ull << kBigShift) | pieces);
@@ -360,6 +361,7 @@ __global__ __launch_bounds__(256) void k_persistent(TravArgs a) {
     }
     // every lane of a wave holds the same phase-A total; lane 0 reports it
     if (lane == 0 && edges) atomicAdd(&a.ctl->edges, edges);
+    if (lane == 0 && expansions) atomicAdd(&a.ctl->expansions, expansions);
 }
 
 // ------------------------------------------------------------- plain kernels
@@ -377,6 +379,7 @@ __global__ void k_ctl_setup(Ctl *c, const uint32_t *src0, uint32_t add0, uint32_
     c->overflow = 0;
     c->aff_count = aff_src ? *aff_src : 0u;
     c->edges = 0;
+    c->expansions = 0;
     c->bad = 0;
 }
 
@@ -499,7 +502,10 @@ void launch_persistent(Engine &e, uint32_t mode, TravArgs &a) {
     const void *fn = persistent_fn(mode);
     if (!e.coop_grid[mode]) e.coop_grid[mode] = e.coop_blocks(fn);
     void *args[] = {&a};
+    static const int kid[5] = {DCG_K_PUSH, DCG_K_PULL, DCG_K_INVALIDATE, DCG_K_PR_SOLVE, DCG_K_PR_AMEND};
+    const int slot = e.time_begin(kid[mode], (int)(a.ctl - e.d_ctl.p));
     DCG_HIP(hipLaunchCooperativeKernel(fn, dim3(e.coop_grid[mode]), dim3(kBlock), args, 0, e.stream));
+    e.time_end(slot);
 }
 
 // Stamps: each launch owns a window of 2^20 round stamps; the tick array is

@@ -66,6 +66,18 @@ class CacheStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class KernelStats(C.Structure):
+    _fields_ = [("launches", C.c_uint64), ("ms", C.c_double), ("algo_bytes", C.c_double),
+                ("edges", C.c_uint64), ("expansions", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+KERNELS = {"push": 0, "pull": 1, "invalidate": 2, "pr_solve": 3, "pr_amend": 4, "delete_rows": 5,
+           "append_rows": 6}
+
+
 class EngineInfo(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in (
         "snapshots", "store_slots", "store_used_slots", "chunks", "frames", "host_bytes",
@@ -111,6 +123,9 @@ def lib() -> C.CDLL:
         "dcg_num_nodes": (I, [P, C.POINTER(U32)]),
         "dcg_get_info": (I, [P, C.POINTER(EngineInfo)]),
         "dcg_sync": (I, [P]),
+        "dcg_get_kernel_stats": (I, [P, U32, C.POINTER(KernelStats)]),
+        "dcg_reset_kernel_stats": (I, [P]),
+        "dcg_mlp_eval": (I, [C.c_int32, U32, P, P, P, P, P]),
         "dcg_gen_rmat": (I, [U32, U64, C.c_double, C.c_double, C.c_double, U64, P, P]),
         "dcg_gen_stream": (I, [U32, P, P, U64, U32, P, P, U64, P, P]),
     }
@@ -153,6 +168,30 @@ def gen_stream(n, src, dst, adds_per, dels_per, seed=7):
     if st:
         raise DcgError(st, "dcg_gen_stream failed")
     return adds, dels
+
+
+MLP_SHAPES = [(8, 64), (64,), (64, 64), (64,), (64, 2), (2,)]
+
+
+def mlp_eval(x, params, y=None, device=0):
+    """Run the policy's predictor on the GPU: returns (out, grads or None)."""
+    x = np.ascontiguousarray(x, np.float32)
+    flat = np.ascontiguousarray(np.concatenate([np.asarray(p, np.float32).reshape(-1) for p in params]))
+    out = np.empty((x.shape[0], 2), np.float32)
+    grad = np.empty(flat.size, np.float32)
+    yy = None if y is None else np.ascontiguousarray(y, np.float32)
+    st = lib().dcg_mlp_eval(device, x.shape[0], _ptr(x), None if yy is None else _ptr(yy), _ptr(flat), _ptr(out),
+                            _ptr(grad))
+    if st:
+        raise DcgError(st, "dcg_mlp_eval failed")
+    if y is None:
+        return out, None
+    gs, k = [], 0
+    for shp in MLP_SHAPES:
+        sz = int(np.prod(shp))
+        gs.append(grad[k:k + sz].reshape(shp))
+        k += sz
+    return out, gs
 
 
 @dataclass
@@ -297,6 +336,17 @@ class Engine:
         st = EngineInfo()
         self._check(lib().dcg_get_info(self._h, C.byref(st)), "dcg_get_info")
         return st.as_dict()
+
+    def kernel_stats(self):
+        out = {}
+        for name, kid in KERNELS.items():
+            st = KernelStats()
+            self._check(lib().dcg_get_kernel_stats(self._h, kid, C.byref(st)), "dcg_get_kernel_stats")
+            out[name] = st.as_dict()
+        return out
+
+    def reset_kernel_stats(self):
+        self._check(lib().dcg_reset_kernel_stats(self._h), "dcg_reset_kernel_stats")
 
     def sync(self):
         self._check(lib().dcg_sync(self._h), "dcg_sync")

@@ -177,6 +177,35 @@ typedef struct {
 } dcg_engine_info;
 int dcg_get_info(dcg_engine *eng, dcg_engine_info *info);
 
+/* Per-kernel measurement (the role of the reference's communication meter,
+ * include/utils/communication_meter.h, for device work): launches, summed
+ * launch time from HIP events on the engine stream, and algorithmic bytes
+ * (24 B per expanded row: 16 B row descriptor + 8 B owner state; 12 B per
+ * scanned adjacency entry: 4 B id + 8 B destination state; mutation: 4 B per
+ * slot read or written + 16 B per touched row descriptor). */
+typedef enum {
+    DCG_K_PUSH = 0, DCG_K_PULL = 1, DCG_K_INVALIDATE = 2, DCG_K_PR_SOLVE = 3, DCG_K_PR_AMEND = 4,
+    DCG_K_DELETE_ROWS = 5, DCG_K_APPEND_ROWS = 6, DCG_K_NUM = 8
+} dcg_kernel_id;
+typedef struct {
+    uint64_t launches;
+    double   ms;
+    double   algo_bytes;
+    uint64_t edges, expansions;
+} dcg_kernel_stats;
+int dcg_get_kernel_stats(dcg_engine *eng, uint32_t kernel, dcg_kernel_stats *out);
+int dcg_reset_kernel_stats(dcg_engine *eng);
+
+/* The learned policy's predictor (MLP 8->64->64->2, FP16 weights, FP32
+ * accumulation) evaluated on caller rows: x is rows*8, y (optional) rows*2
+ * targets, params the 4,868 FP32 parameters (W1 8x64, b1, W2 64x64, b2, W3 64x2,
+ * b3; row-major, input index first), out rows*2 outputs, grad (with y) the
+ * gradient of mean(BCE(sigmoid(o0), y0) + 0.5*(o1-y1)^2).  No reference
+ * counterpart: exposed so the device passes can be checked against an FP64
+ * reference. */
+int dcg_mlp_eval(int32_t device, uint32_t rows, const float *x, const float *y, const float *params,
+                 float *out, float *grad);
+
 /* Synchronise the engine's stream (timing boundaries). */
 int dcg_sync(dcg_engine *eng);
 
