@@ -25,14 +25,21 @@ Engine::Engine(const dcg_config &c) : cfg(c) {
     DCG_HIP(hipHostMalloc((void **)&h_ctl, 8 * sizeof(Ctl), hipHostMallocDefault));
     DCG_HIP(hipHostMalloc((void **)&h_counters, 16 * sizeof(unsigned long long), hipHostMallocDefault));
     app = c.app;
-    Q = (app == DCG_APP_CC || app == DCG_APP_PR) ? 1u : std::max(1u, c.num_queries);
-    if (c.sources && app != DCG_APP_CC && app != DCG_APP_PR) sources.assign(c.sources, c.sources + Q);
+    const bool mixed = c.query_apps && app != DCG_APP_PR;
+    Q = app == DCG_APP_PR || (app == DCG_APP_CC && !mixed) ? 1u : std::max(1u, c.num_queries);
+    if (c.sources && app != DCG_APP_PR) sources.assign(c.sources, c.sources + Q);
     else sources.assign(Q, 0u);
+    qapps.assign(Q, app);
+    if (mixed) {
+        for (uint32_t q = 0; q < Q; ++q) {
+            if (c.query_apps[q] > DCG_APP_CC) throw Error(DCG_ERR_ARG, "query_apps entries must be SSSP, BFS or CC");
+            qapps[q] = c.query_apps[q];
+        }
+    }
     wp.seed = c.weight_seed;
     wp.wmax = c.weight_max ? c.weight_max : 1;
-    if (app == DCG_APP_BFS) wp.kind = W_UNIT;
-    else if (app == DCG_APP_CC) wp.kind = W_ZERO;
-    else wp.kind = c.weight_mode == DCG_W_SEEDED ? W_SEEDED : W_REFERENCE;
+    // SSSP weights; BFS (1) and CC (0) queries override per query in the kernels
+    wp.kind = c.weight_mode == DCG_W_SEEDED ? W_SEEDED : W_REFERENCE;
 }
 
 int Engine::time_begin(int kernel, int ctl, double bytes) {
@@ -154,8 +161,7 @@ int dcg_create(const dcg_config *cfg, dcg_engine **out) {
             return DCG_ERR_ARG;
     }
     if (cfg->weight_mode == DCG_W_SEEDED && cfg->weight_max == 0) return DCG_ERR_ARG;
-    if ((cfg->app == DCG_APP_SSSP || cfg->app == DCG_APP_BFS) && cfg->num_queries > 1 && !cfg->sources)
-        return DCG_ERR_ARG;
+    if (cfg->app != DCG_APP_PR && cfg->num_queries > 1 && !cfg->sources) return DCG_ERR_ARG;
     dcg_engine *h = new (std::nothrow) dcg_engine();
     if (!h) return DCG_ERR_CAPACITY;
     try {
@@ -163,6 +169,9 @@ int dcg_create(const dcg_config *cfg, dcg_engine **out) {
     } catch (const Error &x) {
         delete h;
         return x.status;
+    } catch (const std::exception &) {
+        delete h;
+        return DCG_ERR_STATE;
     }
     *out = h;
     return DCG_OK;
@@ -218,6 +227,8 @@ int dcg_start(dcg_engine *h) {
         for (auto &s : e.sources) s = std::min(s, e.n - 1);
         e.d_sources.alloc(e.sources.size());
         DCG_HIP(hipMemcpyAsync(e.d_sources.p, e.sources.data(), e.sources.size() * 4, hipMemcpyHostToDevice, e.stream));
+        e.d_qapps.alloc(e.qapps.size());
+        DCG_HIP(hipMemcpyAsync(e.d_qapps.p, e.qapps.data(), e.qapps.size() * 4, hipMemcpyHostToDevice, e.stream));
         if (e.app == DCG_APP_PR) e.start_pagerank();
         else e.start_min_lattice();
         e.collect_timing();

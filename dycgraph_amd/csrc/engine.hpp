@@ -84,7 +84,8 @@ struct Engine {
     uint32_t Q = 1;
     uint32_t app = DCG_APP_SSSP;
     std::vector<uint32_t> sources;
-    DevBuf<uint32_t> d_sources;
+    std::vector<uint32_t> qapps;        // per-query program (SSSP/BFS/CC)
+    DevBuf<uint32_t> d_sources, d_qapps;
     WeightParams wp{};
     bool loaded = false, started = false;
     uint64_t snapshots = 0;
@@ -124,6 +125,7 @@ struct Engine {
     DevBuf<uint32_t> d_k32c, d_k32d, d_v32c, d_v32d;   // add keys/payloads (unsorted)
     DevBuf<uint32_t> d_gstart, d_gstart2, d_uniq;
     DevBuf<uint32_t> d_used;           // per-record occurrence counters
+    DevBuf<uint32_t> d_occ, d_big_groups;  // large-row delete path
     DevBuf<unsigned long long> d_counters;  // phase counters (see store.hip)
     unsigned long long *h_counters = nullptr;
 

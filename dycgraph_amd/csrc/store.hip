@@ -29,6 +29,7 @@ enum Counter : int {
     C_HOST_READS = 8,
     C_SLOTS_READ = 9,
     C_SLOTS_WRITTEN = 10,
+    C_NBIG = 11,
     C_NUM = 16
 };
 
@@ -327,11 +328,81 @@ __device__ __forceinline__ uint64_t group_end(const K *keys, uint64_t s, uint64_
 __device__ __forceinline__ uint32_t load_slot(const StoreView &s, uint64_t slot) { return *row_base(s, slot); }
 __device__ __forceinline__ void store_slot(const StoreView &s, uint64_t slot, uint32_t v) { *row_base_w(s, slot) = v; }
 
-// One wave per touched row: remove the first `requested` occurrences of every
-// target, compact survivors in order.
+constexpr uint32_t kBigDelete = 8192;   // rows above this use one 1024-thread workgroup
+
+// Index of the first record of x's run among the group's sorted targets
+// [gs, ge) (lower bound on the low halves).
+__device__ __forceinline__ uint64_t target_lb(const uint64_t *keys, uint64_t gs, uint64_t ge, uint32_t x) {
+    uint64_t lo = gs, hi = ge;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if ((uint32_t)keys[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint64_t target_ub(const uint64_t *keys, uint64_t from, uint64_t ge, uint32_t x) {
+    uint64_t a = from, b = ge;
+    while (a < b) {
+        const uint64_t mid = (a + b) >> 1;
+        if ((uint32_t)keys[mid] <= x) a = mid + 1;
+        else b = mid;
+    }
+    return a;
+}
+
+struct DelResult {
+    uint32_t removed, moved;
+};
+
+// One wave: remove the first `requested` occurrences of every target of the
+// group from the row, survivors compacted in order.  Matches are resolved in
+// lane order through per-run counters, so the occurrences removed are exactly
+// the first ones (source_local_chunk_store.h:685-739).
+__device__ DelResult wave_delete_row(const StoreView &s, RowDesc *desc, uint32_t row, const RowDesc d,
+                                     const uint64_t *keys, uint64_t gs, uint64_t ge, uint32_t *used) {
+    const uint32_t lane = lane_id();
+    uint32_t write = 0, removed = 0, moved = 0;
+    for (uint32_t t0 = 0; t0 < d.deg; t0 += kWave) {
+        const uint32_t i = t0 + lane;
+        const bool valid = i < d.deg;
+        const uint32_t x = valid ? load_slot(s, d.off + i) : 0u;
+        const uint64_t lo = target_lb(keys, gs, ge, x);
+        const bool match = valid && lo < ge && (uint32_t)keys[lo] == x;
+        bool rem = false;
+        uint64_t mm = __ballot(match);
+        while (mm) {
+            const int l = __ffsll((unsigned long long)mm) - 1;
+            if ((int)lane == l) {
+                const uint32_t requested = (uint32_t)(target_ub(keys, lo + 1, ge, x) - lo);
+                const uint32_t old = atomicAdd(&used[lo], 1u);
+                rem = old < requested;
+            }
+            mm &= mm - 1;
+        }
+        const bool keep = valid && !rem;
+        const uint64_t km = __ballot(keep);
+        const uint32_t rtile = (uint32_t)__popcll(__ballot(rem));
+        if (removed + rtile) {
+            const uint32_t pos = write + lanes_below(km);
+            const bool mv = keep && pos != i;
+            if (mv) store_slot(s, d.off + pos, x);
+            moved += (uint32_t)__popcll(__ballot(mv));
+        }
+        write += (uint32_t)__popcll(km);
+        removed += rtile;
+    }
+    if (lane == 0 && removed) desc[row].deg = write;
+    return DelResult{removed, moved};
+}
+
+// One wave per touched row of at most kBigDelete entries; larger rows are
+// listed for k_del_big.
 __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, const uint64_t *keys, uint64_t count,
                                                   const uint32_t *gstart, const unsigned long long *ngroups_p,
-                                                  uint32_t *used, unsigned long long *counters, int count_stats) {
+                                                  uint32_t *used, unsigned long long *counters, int count_stats,
+                                                  uint32_t *big_groups, unsigned long long *nbig) {
     const uint32_t lane = lane_id();
     const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kWave;
     const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / kWave;
@@ -340,56 +411,18 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, co
     for (uint64_t g = wave; g < ngroups; g += nwaves) {
         const uint64_t gs = gstart[g];
         const uint32_t row = (uint32_t)(keys[gs] >> 32);
-        const uint64_t ge = group_end<uint64_t>(keys, gs, count, row);
         const RowDesc d = desc[row];
-        uint32_t write = 0, removed = 0, moved = 0;
-        for (uint32_t t0 = 0; t0 < d.deg; t0 += kWave) {
-            const uint32_t i = t0 + lane;
-            const bool valid = i < d.deg;
-            const uint32_t x = valid ? load_slot(s, d.off + i) : 0u;
-            // lower bound of x among the group's targets (sorted low halves)
-            uint64_t lo = gs, hi = ge;
-            while (lo < hi) {
-                const uint64_t mid = (lo + hi) >> 1;
-                if ((uint32_t)keys[mid] < x) lo = mid + 1;
-                else hi = mid;
-            }
-            const bool match = valid && lo < ge && (uint32_t)keys[lo] == x;
-            bool rem = false;
-            uint64_t mm = __ballot(match);
-            while (mm) {
-                const int l = __ffsll((unsigned long long)mm) - 1;
-                if ((int)lane == l) {
-                    uint64_t a = lo + 1, b = ge;   // end of this target's run
-                    while (a < b) {
-                        const uint64_t mid = (a + b) >> 1;
-                        if ((uint32_t)keys[mid] <= x) a = mid + 1;
-                        else b = mid;
-                    }
-                    const uint32_t requested = (uint32_t)(a - lo);
-                    const uint32_t old = atomicAdd(&used[lo], 1u);
-                    rem = old < requested;
-                }
-                mm &= mm - 1;
-            }
-            const bool keep = valid && !rem;
-            const uint64_t km = __ballot(keep);
-            const uint32_t rtile = (uint32_t)__popcll(__ballot(rem));
-            if (removed + rtile) {
-                const uint32_t pos = write + lanes_below(km);
-                const bool mv = keep && pos != i;
-                if (mv) store_slot(s, d.off + pos, x);
-                moved += (uint32_t)__popcll(__ballot(mv));
-            }
-            write += (uint32_t)__popcll(km);
-            removed += rtile;
+        if (d.deg > kBigDelete) {
+            if (lane == 0) big_groups[atomicAdd(nbig, 1ull)] = (uint32_t)gs;
+            continue;
         }
+        const uint64_t ge = group_end<uint64_t>(keys, gs, count, row);
+        const DelResult r = wave_delete_row(s, desc, row, d, keys, gs, ge, used);
         if (lane == 0) {
-            if (removed) desc[row].deg = write;
-            missing += (ge - gs) - removed;
-            changed += removed ? 1 : 0;
+            missing += (ge - gs) - r.removed;
+            changed += r.removed ? 1 : 0;
             rd += d.deg;
-            wr += moved;
+            wr += r.moved;
         }
     }
     if (lane == 0) {
@@ -397,6 +430,103 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, co
         if (count_stats && changed) atomicAdd(&counters[C_CHANGED], changed);
         if (rd) atomicAdd(&counters[C_SLOTS_READ], rd);
         if (wr) atomicAdd(&counters[C_SLOTS_WRITTEN], wr);
+    }
+}
+
+// One 1024-thread workgroup per large row.  Pass 1 counts the occurrences of
+// every target run; if no run has more occurrences than requests, every
+// occurrence of every target goes, and pass 2 compacts the row in 4096-entry
+// tiles with a workgroup prefix sum (all reads of a tile precede its writes).
+// Otherwise wave 0 takes the exact first-occurrence path.
+__global__ __launch_bounds__(1024) void k_del_big(StoreView s, RowDesc *desc, const uint64_t *keys, uint64_t count,
+                                                  const uint32_t *big_groups, const unsigned long long *nbig_p,
+                                                  uint32_t *occ, uint32_t *used, unsigned long long *counters,
+                                                  int count_stats) {
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t flag;
+    const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
+    const uint64_t nbig = *nbig_p;
+    for (uint64_t b = blockIdx.x; b < nbig; b += gridDim.x) {
+        const uint64_t gs = big_groups[b];
+        const uint32_t row = (uint32_t)(keys[gs] >> 32);
+        const uint64_t ge = group_end<uint64_t>(keys, gs, count, row);
+        const RowDesc d = desc[row];
+        for (uint32_t i = tid; i < d.deg; i += blockDim.x) {
+            const uint32_t x = load_slot(s, d.off + i);
+            const uint64_t lo = target_lb(keys, gs, ge, x);
+            if (lo < ge && (uint32_t)keys[lo] == x) atomicAdd(&occ[lo], 1u);
+        }
+        if (tid == 0) flag = 0;
+        __syncthreads();
+        for (uint64_t r = gs + tid; r < ge; r += blockDim.x) {
+            const uint32_t x = (uint32_t)keys[r];
+            if (r > gs && (uint32_t)keys[r - 1] == x) continue;
+            if (occ[r] > target_ub(keys, r + 1, ge, x) - r) flag = 1;
+        }
+        __syncthreads();
+        uint32_t removed = 0, moved = 0;
+        if (flag) {
+            if (w == 0) {
+                const DelResult rr = wave_delete_row(s, desc, row, d, keys, gs, ge, used);
+                if (tid == 0) {
+                    removed = rr.removed;
+                    moved = rr.moved;
+                }
+            }
+        } else {
+            uint32_t write = 0;
+            for (uint32_t t0 = 0; t0 < d.deg; t0 += 4 * 1024) {
+                uint32_t xv[4];
+                bool keep[4];
+                for (int j = 0; j < 4; ++j) {
+                    const uint32_t i = t0 + j * 1024 + tid;
+                    keep[j] = false;
+                    if (i < d.deg) {
+                        xv[j] = load_slot(s, d.off + i);
+                        const uint64_t lo = target_lb(keys, gs, ge, xv[j]);
+                        keep[j] = !(lo < ge && (uint32_t)keys[lo] == xv[j]);
+                    }
+                }
+                uint32_t pos[4];
+                uint32_t base = write;
+                for (int j = 0; j < 4; ++j) {
+                    const uint64_t km = __ballot(keep[j]);
+                    if (lane == 0) wsum[w] = (uint32_t)__popcll(km);
+                    __syncthreads();
+                    uint32_t before = 0, tot = 0;
+                    for (uint32_t q = 0; q < 16; ++q) {
+                        before += q < w ? wsum[q] : 0u;
+                        tot += wsum[q];
+                    }
+                    pos[j] = base + before + lanes_below(km);
+                    base += tot;
+                    __syncthreads();
+                }
+                for (int j = 0; j < 4; ++j) {
+                    const uint32_t i = t0 + j * 1024 + tid;
+                    if (i < d.deg && keep[j] && pos[j] != i) {
+                        store_slot(s, d.off + pos[j], xv[j]);
+                        moved++;
+                    }
+                }
+                write = base;
+                __syncthreads();
+            }
+            if (tid == 0) {
+                removed = d.deg - write;
+                if (removed) desc[row].deg = write;
+            }
+        }
+        // block totals: `removed` lives in thread 0
+        if (tid == 0) {
+            if (count_stats) {
+                atomicAdd(&counters[C_MISSING], (unsigned long long)((ge - gs) - removed));
+                if (removed) atomicAdd(&counters[C_CHANGED], 1ull);
+            }
+            atomicAdd(&counters[C_SLOTS_READ], (unsigned long long)(2ull * d.deg));
+        }
+        if (moved) atomicAdd(&counters[C_SLOTS_WRITTEN], (unsigned long long)moved);
+        __syncthreads();
     }
 }
 
@@ -511,13 +641,24 @@ void Engine::mutate_delete(uint64_t begin, uint64_t count, dcg_phase_stats *st) 
                        d_counters.p + C_GROUPS_IN);
     StoreView v = view();
     const uint32_t wg = wave_grid(count);
+    d_big_groups.ensure(count);
+    d_occ.ensure(count);
     fill<uint32_t>(stream, d_used.p, count, 0u);
+    fill<uint32_t>(stream, d_occ.p, count, 0u);
     const int slot = time_begin(DCG_K_DELETE_ROWS, -1);
     hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream, v, d_out.p, kout, count, d_gstart.p,
-                       d_counters.p + C_GROUPS_OUT, d_used.p, d_counters.p, 1);
+                       d_counters.p + C_GROUPS_OUT, d_used.p, d_counters.p, 1, d_big_groups.p,
+                       d_counters.p + C_NBIG);
+    hipLaunchKernelGGL(k_del_big, dim3(256), dim3(1024), 0, stream, v, d_out.p, kout, count, d_big_groups.p,
+                       d_counters.p + C_NBIG, d_occ.p, d_used.p, d_counters.p, 1);
     fill<uint32_t>(stream, d_used.p, count, 0u);
+    fill<uint32_t>(stream, d_occ.p, count, 0u);
+    fill<unsigned long long>(stream, d_counters.p + C_NBIG, 1, 0ull);
     hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream, v, d_in.p, kin, count, d_gstart2.p,
-                       d_counters.p + C_GROUPS_IN, d_used.p, d_counters.p, 0);
+                       d_counters.p + C_GROUPS_IN, d_used.p, d_counters.p, 0, d_big_groups.p,
+                       d_counters.p + C_NBIG);
+    hipLaunchKernelGGL(k_del_big, dim3(256), dim3(1024), 0, stream, v, d_in.p, kin, count, d_big_groups.p,
+                       d_counters.p + C_NBIG, d_occ.p, d_used.p, d_counters.p, 0);
     time_end(slot);
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipMemcpyAsync(h_counters, d_counters.p, C_NUM * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));

@@ -30,7 +30,8 @@ struct TravArgs {
     uint64_t *st;
     uint32_t *tick;
     const uint32_t *sources;
-    WeightParams w;
+    const uint32_t *qapp;      // per-query program
+    WeightParams w;            // SSSP weights (BFS: 1, CC: 0)
     uint32_t app;
     uint32_t max_rounds;
     const uint32_t *in0;
@@ -62,6 +63,12 @@ __device__ __forceinline__ void decode_g(const TravArgs &a, uint32_t g, uint32_t
     }
 }
 
+__device__ __forceinline__ uint32_t qweight(const TravArgs &a, uint32_t app, uint32_t u, uint32_t v) {
+    if (app == DCG_APP_BFS) return 1u;
+    if (app == DCG_APP_CC) return 0u;
+    return edge_weight(a.w, u, v);
+}
+
 // Dependency predicate of the delete stage: api.cuh:43-49 for SSSP/BFS (parent
 // edge, or a tight witness while the current parent is not tight) and
 // hybrid_cc.cu:60-64 for CC (equal label, label smaller than the id).  A
@@ -72,16 +79,17 @@ __device__ __forceinline__ bool invalidate_op(const TravArgs &a, uint32_t q, uin
     const uint64_t gd = qoff + d;
     const uint64_t sd = a.st[gd];
     const uint32_t vd = (uint32_t)(sd >> 32), pd = (uint32_t)sd;
+    const uint32_t app = a.qapp[q];
     bool dep;
-    if (a.app == DCG_APP_CC) {
+    if (app == DCG_APP_CC) {
         dep = s != d && vs == vd && vd < d;
     } else {
         if (d == a.sources[q] || pd == kInf) return false;
         if (pd == s) {
             dep = true;
-        } else if (vs != kInf && (uint64_t)vs + edge_weight(a.w, s, d) == (uint64_t)vd) {
+        } else if (vs != kInf && (uint64_t)vs + qweight(a, app, s, d) == (uint64_t)vd) {
             const uint32_t vp = (uint32_t)(a.st[qoff + pd] >> 32);
-            const bool ptight = vp != kInf && (uint64_t)vp + edge_weight(a.w, pd, d) == (uint64_t)vd;
+            const bool ptight = vp != kInf && (uint64_t)vp + qweight(a, app, pd, d) == (uint64_t)vd;
             dep = !ptight;
         } else {
             dep = false;
@@ -95,7 +103,7 @@ __device__ __forceinline__ bool invalidate_op(const TravArgs &a, uint32_t q, uin
 
 __device__ __forceinline__ bool push_op(const TravArgs &a, uint32_t q, uint32_t s, uint32_t vs, uint32_t d,
                                         uint32_t stamp, uint32_t &out_g) {
-    const uint64_t cand = (uint64_t)vs + edge_weight(a.w, s, d);
+    const uint64_t cand = (uint64_t)vs + qweight(a, a.qapp[q], s, d);
     if (cand >= kInf) return false;
     const uint64_t gd = (uint64_t)q * a.n + d;
     const uint64_t cur = a.st[gd];
@@ -112,7 +120,7 @@ __device__ __forceinline__ uint64_t pull_cand(const TravArgs &a, uint32_t q, uin
     const uint64_t qoff = (uint64_t)q * a.n;
     const uint32_t vx = (uint32_t)(a.st[qoff + x] >> 32);
     if (vx == kInf) return ~0ull;
-    const uint64_t cand = (uint64_t)vx + edge_weight(a.w, x, d);
+    const uint64_t cand = (uint64_t)vx + qweight(a, a.qapp[q], x, d);
     if (cand >= kInf) return ~0ull;
     return (cand << 32) | x;
 }
@@ -383,27 +391,32 @@ __global__ void k_ctl_setup(Ctl *c, const uint32_t *src0, uint32_t add0, uint32_
     c->bad = 0;
 }
 
-__global__ void k_init_state(uint64_t *st, uint32_t *tick, uint32_t n, uint32_t Q, uint32_t app,
+__global__ void k_init_state(uint64_t *st, uint32_t *tick, uint32_t n, uint32_t Q, const uint32_t *qapp,
                              const uint32_t *sources) {
     const uint64_t total = (uint64_t)n * Q;
     for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < total;
          g += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t q = (uint32_t)(g / n), v = (uint32_t)(g - (uint64_t)q * n);
         uint64_t s = kInfState;
-        if (app == DCG_APP_CC) s = ((uint64_t)v << 32) | kInf;
+        if (qapp[q] == DCG_APP_CC) s = ((uint64_t)v << 32) | kInf;
         else if (v == sources[q]) s = (uint64_t)kInf;   // value 0, parent INF
         st[g] = s;
         tick[g] = 0;
     }
 }
 
-__global__ void k_iota_sources(uint32_t *q0, uint32_t n, uint32_t Q, uint32_t app, const uint32_t *sources) {
-    if (app == DCG_APP_CC) {
-        for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x)
-            q0[v] = (uint32_t)v;
-    } else {
-        for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < Q; q += (uint64_t)gridDim.x * blockDim.x)
-            q0[q] = (uint32_t)(q * n + sources[q]);
+// Initial frontier: a traversal query starts at its source, a CC query at every
+// vertex.  `slot0[q]` is the query's first position in q0 (host prefix sum).
+__global__ void k_iota_sources(uint32_t *q0, uint32_t n, uint32_t Q, const uint32_t *qapp, const uint32_t *sources,
+                               const uint64_t *slot0) {
+    for (uint32_t q = 0; q < Q; ++q) {
+        if (qapp[q] == DCG_APP_CC) {
+            for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n;
+                 v += (uint64_t)gridDim.x * blockDim.x)
+                q0[slot0[q] + v] = (uint32_t)((uint64_t)q * n + v);
+        } else if (blockIdx.x == 0 && threadIdx.x == 0) {
+            q0[slot0[q]] = (uint32_t)((uint64_t)q * n + sources[q]);
+        }
     }
 }
 
@@ -439,12 +452,14 @@ __global__ void k_seed_dels(TravArgs a, const dcg_edge *recs, uint64_t count) {
 }
 
 // Reset affected values (driver.cuh:443-463): INF, or the own id for CC.
-__global__ void k_reset_affected(uint64_t *st, const uint32_t *aff, const Ctl *c, uint32_t n, uint32_t Q, uint32_t app) {
+__global__ void k_reset_affected(uint64_t *st, const uint32_t *aff, const Ctl *c, uint32_t n, uint32_t Q,
+                                 const uint32_t *qapp) {
     const uint32_t count = c->aff_count;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t g = aff[i];
-        const uint32_t v = Q == 1 ? g : g % n;
-        st[g] = app == DCG_APP_CC ? (((uint64_t)v << 32) | kInf) : kInfState;
+        const uint32_t q = Q == 1 ? 0u : g / n;
+        const uint32_t v = g - q * n;
+        st[g] = qapp[q] == DCG_APP_CC ? (((uint64_t)v << 32) | kInf) : kInfState;
     }
 }
 
@@ -529,6 +544,7 @@ static TravArgs base_args(Engine &e) {
     a.st = e.d_st.p;
     a.tick = e.d_tick.p;
     a.sources = e.d_sources.p;
+    a.qapp = e.d_qapps.p;
     a.w = e.wp;
     a.app = e.app;
     a.max_rounds = 1u << 20;
@@ -572,12 +588,22 @@ void Engine::alloc_state() {
 
 void Engine::start_min_lattice() {
     const uint64_t qn = (uint64_t)n * Q;
-    hipLaunchKernelGGL(k_init_state, dim3(grid_for(qn)), dim3(256), 0, stream, d_st.p, d_tick.p, n, Q, app, d_sources.p);
+    hipLaunchKernelGGL(k_init_state, dim3(grid_for(qn)), dim3(256), 0, stream, d_st.p, d_tick.p, n, Q, d_qapps.p,
+                       d_sources.p);
     stamp = 1;
-    const uint32_t count0 = app == DCG_APP_CC ? n : Q;
-    hipLaunchKernelGGL(k_iota_sources, dim3(grid_for(count0)), dim3(256), 0, stream, d_qc.p, n, Q, app, d_sources.p);
+    std::vector<uint64_t> slot0(Q);
+    uint64_t count0 = 0;
+    for (uint32_t q = 0; q < Q; ++q) {
+        slot0[q] = count0;
+        count0 += qapps[q] == DCG_APP_CC ? n : 1;
+    }
+    DevBuf<uint64_t> d_slot0;
+    d_slot0.alloc(Q);
+    DCG_HIP(hipMemcpyAsync(d_slot0.p, slot0.data(), Q * 8, hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(k_iota_sources, dim3(grid_for(n)), dim3(256), 0, stream, d_qc.p, n, Q, d_qapps.p, d_sources.p,
+                       d_slot0.p);
     Ctl *c = d_ctl.p;
-    hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c, (const uint32_t *)nullptr, count0, 0u,
+    hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c, (const uint32_t *)nullptr, (uint32_t)count0, 0u,
                        (const uint32_t *)nullptr);
     TravArgs a = base_args(*this);
     a.in0 = d_qc.p;
@@ -613,7 +639,7 @@ void Engine::delete_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
     launch_persistent(*this, M_INVALIDATE, a);
     // 3. reset affected values
     hipLaunchKernelGGL(k_reset_affected, dim3(grid_for((uint64_t)n * Q, 256, 8192)), dim3(256), 0, stream, d_st.p,
-                       d_aff.p, c + 1, n, Q, app);
+                       d_aff.p, c + 1, n, Q, d_qapps.p);
     // 4. one pull pass over the in-rows of the affected vertices
     hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 2, &c[1].aff_count, 0u, 0u,
                        (const uint32_t *)nullptr);

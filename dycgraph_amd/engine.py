@@ -43,7 +43,7 @@ class Config(C.Structure):
         ("placement", C.c_uint32), ("cache_policy", C.c_uint32), ("cache_bytes", C.c_uint64),
         ("store_slack", C.c_double), ("pr_base", C.c_double), ("pr_epsilon", C.c_double),
         ("pr_max_rounds", C.c_uint32), ("block_size", C.c_uint32), ("device", C.c_int32),
-        ("reserved", C.c_uint32 * 7),
+        ("query_apps", C.POINTER(C.c_uint32)), ("reserved", C.c_uint32 * 4),
     ]
 
 
@@ -127,6 +127,7 @@ def lib() -> C.CDLL:
         "dcg_reset_kernel_stats": (I, [P]),
         "dcg_mlp_eval": (I, [C.c_int32, U32, P, P, P, P, P]),
         "dcg_gen_rmat": (I, [U32, U64, C.c_double, C.c_double, C.c_double, U64, P, P]),
+        "dcg_gen_rmat_gpu": (I, [C.c_int32, U32, U64, C.c_double, C.c_double, C.c_double, U64, P, P]),
         "dcg_gen_stream": (I, [U32, P, P, U64, U32, P, P, U64, P, P]),
     }
     for name, (res, args) in sig.items():
@@ -147,10 +148,17 @@ def edges_array(pairs) -> np.ndarray:
     return a
 
 
-def gen_rmat(scale: int, num_edges: int, a=0.57, b=0.19, c=0.19, seed=1):
+def gen_rmat(scale: int, num_edges: int, a=0.57, b=0.19, c=0.19, seed=1, gpu=None, device=0):
+    """Seeded RMAT edge set sorted by (src, dst).  gpu=None picks the GPU path for
+    more than 2^22 edges; both paths return the same edges."""
     src = np.empty(num_edges, np.uint32)
     dst = np.empty(num_edges, np.uint32)
-    st = lib().dcg_gen_rmat(scale, num_edges, a, b, c, seed, _ptr(src), _ptr(dst))
+    if gpu is None:
+        gpu = num_edges > (1 << 22)
+    if gpu:
+        st = lib().dcg_gen_rmat_gpu(device, scale, num_edges, a, b, c, seed, _ptr(src), _ptr(dst))
+    else:
+        st = lib().dcg_gen_rmat(scale, num_edges, a, b, c, seed, _ptr(src), _ptr(dst))
     if st:
         raise DcgError(st, "dcg_gen_rmat failed")
     return src, dst
@@ -209,6 +217,7 @@ class EngineConfig:
     pr_epsilon: float = 1e-6
     pr_max_rounds: int = 100
     device: int = 0
+    query_apps: tuple = ()      # per-query program ("sssp"/"bfs"/"cc"); empty: all `app`
 
 
 class Engine:
@@ -231,6 +240,12 @@ class Engine:
         c.store_slack = cfg.store_slack
         c.pr_base, c.pr_epsilon, c.pr_max_rounds = cfg.pr_base, cfg.pr_epsilon, cfg.pr_max_rounds
         c.device = cfg.device
+        if cfg.query_apps:
+            qa = np.ascontiguousarray([APPS[a] if isinstance(a, str) else int(a) for a in cfg.query_apps],
+                                      dtype=np.uint32)
+            assert len(qa) == len(srcs), "query_apps needs one entry per source"
+            self._qapps = qa
+            c.query_apps = qa.ctypes.data_as(C.POINTER(C.c_uint32))
         h = C.c_void_p()
         self._h = None
         self._check(L.dcg_create(C.byref(c), C.byref(h)), "dcg_create")

@@ -75,7 +75,11 @@ typedef struct {
     uint32_t pr_max_rounds;   /* 1..100 (reference --pr_max_rounds)                 */
     uint32_t block_size;      /* threads per workgroup for traversal kernels (256)  */
     int32_t device;           /* HIP device ordinal                                 */
-    uint32_t reserved[7];
+    /* Optional per-query program (DCG_APP_SSSP/BFS/CC, num_queries entries): one
+     * engine then repairs SSSP, BFS and CC queries over one chunk store in the
+     * same launches.  NULL: every query runs `app`.  Ignored for PR. */
+    const uint32_t *query_apps;
+    uint32_t reserved[4];
 } dcg_config;
 
 /* Per-phase counters (the reference's [P0-DELETE-ATTR]/[C3-*] log fields). */
@@ -215,6 +219,10 @@ int dcg_sync(dcg_engine *eng);
  * returned sorted by (src,dst).  Buffers are caller-allocated host memory. */
 int dcg_gen_rmat(uint32_t scale, uint64_t num_edges, double a, double b, double c,
                  uint64_t seed, uint32_t *src, uint32_t *dst);
+/* Same edge set as dcg_gen_rmat, generated on the GPU (stable radix sort of the
+ * draws plus a first-occurrence scan); src/dst are host buffers. */
+int dcg_gen_rmat_gpu(int32_t device, uint32_t scale, uint64_t num_edges, double a, double b, double c,
+                     uint64_t seed, uint32_t *src, uint32_t *dst);
 /* Update stream: per snapshot `dels` deletions uniform over the edges present
  * at the snapshot start (without replacement) and `adds` insertions uniform
  * over absent non-loop pairs (distinct within the snapshot). The graph must be

@@ -90,3 +90,79 @@ def test_rmat16_sssp_seeded_weights():
     n, src, dst = rmat_graph(16, 1 << 20)
     _run_stream("sssp", n, src, dst, [1000] * 3, [1000] * 3, placement=0, wmode=1, seed=11, wmax=100,
                 check_rows=False)
+
+
+@pytest.mark.parametrize("dups", [False, True])
+@pytest.mark.parametrize("placement", [0, 1])
+def test_large_row_deletes_exact_rows(dups, placement):
+    """Rows above 8,192 entries take the workgroup delete path; with duplicate
+    targets (more occurrences than requests) the exact first-occurrence path."""
+    announce(f"large-row deletes dups={dups} placement={placement}")
+    import dycgraph_amd as d
+    n = 30000
+    rng = np.random.default_rng(5)
+    hub = list(range(1, 20001))
+    if dups:
+        hub += [5, 9, 5, 77]                     # value 5 three times, 9 and 77 twice
+    rng.shuffle(hub)
+    src = [0] * len(hub) + list(range(1, 1000))
+    dst = hub + [int(x) for x in rng.integers(1, n, 999)]
+    # the in-row of vertex 1 is large too: many sources point at 1
+    src += list(range(100, 12100))
+    dst += [1] * 12000
+    src, dst = np.array(src, np.uint32), np.array(dst, np.uint32)
+    dels = [(0, int(v)) for v in rng.choice(np.arange(10, 20000), 300, replace=False)]
+    dels += [(0, 5), (0, 9), (0, 9), (0, 9), (0, 123456 % n)] + [(u, 1) for u in range(100, 400)]
+    adds = [(0, 5), (3, 4)]
+    eng = _engine(app="bfs", placement=placement, cache_bytes=1 << 18)
+    eng.load_graph(n, src, dst)
+    eng.start()
+    eng.load_updates(adds, dels)
+    store = oracle.Store(n, src, dst)
+    st = eng.del_edge(0, len(dels))
+    m = store.apply(dels, [])
+    assert st["missing_deletes"] == m["missing_deletes"]
+    erp, ecol = eng.csr()
+    assert same_rows(erp, ecol, store)
+    rp, col = store.csr()
+    assert np.array_equal(eng.gather_value(), oracle.bfs(n, rp, col, 0))
+    eng.add_edge(0, len(adds))
+    store.apply([], adds)
+    erp, ecol = eng.csr()
+    assert same_rows(erp, ecol, store)
+    eng.close()
+
+
+def test_mixed_programs_one_engine():
+    """SSSP, BFS and CC queries (several sources) repaired together over one chunk
+    store: every query bit-exact after every stage."""
+    announce("mixed programs: sssp x2, bfs x2, cc on rmat12 host-cache")
+    import dycgraph_amd as d
+    n, src, dst = rmat_graph(12, 40000, seed=9)
+    adds, dels = d.gen_stream(n, src, dst, [400] * 3, [400] * 3, seed=3)
+    apps = ("sssp", "bfs", "cc", "sssp", "bfs")
+    sources = (0, 0, 0, 17, 1234)
+    eng = _engine(app="sssp", sources=sources, query_apps=apps, weight_mode=1, weight_seed=5, weight_max=50,
+                  placement=1, cache_bytes=1 << 18)
+    eng.load_graph(n, src, dst)
+    eng.start()
+    eng.load_updates(adds, dels)
+    store = oracle.Store(n, src, dst)
+
+    def check(stage):
+        rp, col = store.csr()
+        for q, (app, s) in enumerate(zip(apps, sources)):
+            want = oracle_values(app, n, rp, col, s, oracle.W_SEEDED, 5, 50)
+            got = eng.gather_value(q)
+            assert np.array_equal(got, want), (stage, q, app, int(np.sum(got != want)))
+
+    check("static")
+    for s in range(3):
+        eng.del_edge(400 * s, 400)
+        store.apply(dels[400 * s:400 * (s + 1)], [])
+        check(f"delete {s}")
+        eng.add_edge(400 * s, 400)
+        store.apply([], adds[400 * s:400 * (s + 1)])
+        check(f"add {s}")
+        eng.cache_maintain()
+    eng.close()
