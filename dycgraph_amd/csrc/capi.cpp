@@ -414,6 +414,25 @@ int dcg_reset_kernel_stats(dcg_engine *h) {
     });
 }
 
+int dcg_autotune(dcg_engine *h, dcg_tune_result *out) {
+    return guarded(h, [&](Engine &e) {
+        if (!e.started) throw Error(DCG_ERR_STATE, "dcg_autotune before dcg_start");
+        e.autotune(out);
+    });
+}
+
+int dcg_set_tuning(dcg_engine *h, uint32_t blocks_per_cu, uint32_t big_row, uint32_t piece) {
+    return guarded(h, [&](Engine &e) {
+        auto pow2 = [](uint32_t x) { return x && !(x & (x - 1)); };
+        if (blocks_per_cu > 16 || (big_row && big_row < 1) || (piece && (!pow2(piece) || piece > 65536)))
+            throw Error(DCG_ERR_ARG, "bad tuning values");
+        if (blocks_per_cu) e.tune.blocks_per_cu = blocks_per_cu;
+        if (big_row) e.tune.big_row = big_row;
+        if (piece) e.tune.piece = piece;
+        for (auto &g : e.coop_grid) g = 0;
+    });
+}
+
 int dcg_sync(dcg_engine *h) {
     return guarded(h, [&](Engine &e) { e.sync(); });
 }

@@ -79,6 +79,11 @@ struct Engine {
     hipStream_t stream = nullptr;
     int cu_count = 0;
     uint32_t coop_grid[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    struct Tune {
+        uint32_t blocks_per_cu = 2;   // persistent workgroups per CU
+        uint32_t big_row = kBigRow;   // rows above this are split into pieces
+        uint32_t piece = kPiece;
+    } tune;
 
     uint32_t n = 0;
     uint32_t Q = 1;
@@ -163,6 +168,7 @@ struct Engine {
     void delete_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st);
     void add_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st);
     uint32_t coop_blocks(const void *kernel);
+    void autotune(dcg_tune_result *res);
 
     // pagerank.hip
     void start_pagerank();

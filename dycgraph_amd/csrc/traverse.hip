@@ -43,6 +43,8 @@ struct TravArgs {
     uint64_t bigcap;
     Ctl *ctl;
     uint32_t stamp0;
+    uint32_t big_row;          // rows above this many entries are split into pieces
+    uint32_t piece;            // piece size (power of two dividing 65,536)
     // PageRank
     float *rank, *res, *delta;
     uint32_t *queued;
@@ -269,8 +271,8 @@ __global__ __launch_bounds__(256) void k_persistent(TravArgs a) {
                 }
             }
             expansions += (uint64_t)__popcll(__ballot(deg > 0));
-            if (deg > kBigRow) {
-                const uint64_t pieces = (deg + kPiece - 1) / kPiece;
+            if (deg > a.big_row) {
+                const uint64_t pieces = (deg + a.piece - 1) / a.piece;
                 const unsigned long long p = atomicAdd(&a.ctl->big[cur], (1ull << kBigShift) | pieces);
                 const uint64_t e = p >> kBigShift;
                 if (e < a.bigcap) {
@@ -334,9 +336,9 @@ __global__ __launch_bounds__(256) void k_persistent(TravArgs a) {
                     else hi = mid;
                 }
                 const BigEntry be = a.big[lo];
-                const uint32_t start = (p - be.piece0) * kPiece;
+                const uint32_t start = (p - be.piece0) * a.piece;
                 if (start >= be.deg) continue;
-                const uint32_t len = be.deg - start < kPiece ? be.deg - start : kPiece;
+                const uint32_t len = be.deg - start < a.piece ? be.deg - start : a.piece;
                 const uint32_t *base = row_base(a.s, be.off + start);
                 uint32_t q = 0, v = 0;
                 if (MODE == M_PR || MODE == M_PR_AMEND) v = be.g;
@@ -499,7 +501,8 @@ uint32_t Engine::coop_blocks(const void *kernel) {
     int per_cu = 0;
     DCG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0));
     if (per_cu < 1) per_cu = 1;
-    if (per_cu > 4) per_cu = 4;
+    const int want = tune.blocks_per_cu ? (int)tune.blocks_per_cu : 4;
+    if (per_cu > want) per_cu = want;
     return (uint32_t)(per_cu * cu_count);
 }
 
@@ -561,6 +564,8 @@ static TravArgs base_args(Engine &e) {
     a.queued = e.d_queued.p;
     a.eps = (float)e.cfg.pr_epsilon;
     a.sign = 1.f;
+    a.big_row = e.tune.big_row;
+    a.piece = e.tune.piece;
     return a;
 }
 
@@ -583,6 +588,70 @@ void Engine::alloc_state() {
     } else {
         d_st.alloc(cap);
         d_aff.alloc(cap);
+    }
+}
+
+// Autotuner: coordinate descent over (workgroups per CU, large-row threshold,
+// piece size), each candidate timed on a from-scratch closure of the loaded
+// graph (HIP events around the launch).  Every candidate yields the same values.
+void Engine::autotune(dcg_tune_result *res) {
+    if (app == DCG_APP_PR) throw Error(DCG_ERR_UNSUPPORTED, "the autotuner tunes the min-lattice closure");
+    auto measure = [&]() {
+        for (auto &g : coop_grid) g = 0;
+        start_min_lattice();
+        sync();
+        collect_timing();
+        const double ms = kstat[DCG_K_PUSH].ms;
+        kstat[DCG_K_PUSH] = dcg_kernel_stats{};
+        return ms;
+    };
+    const dcg_kernel_stats keep = kstat[DCG_K_PUSH];
+    kstat[DCG_K_PUSH] = dcg_kernel_stats{};
+    double best = measure();
+    uint32_t evals = 1;
+    const uint32_t bpc[] = {1, 2, 3, 4};
+    const uint32_t rows[] = {64, 128, 256, 512, 1024, 4096};
+    const uint32_t pieces[] = {256, 512, 1024, 2048, 4096};
+    for (int pass = 0; pass < 2; ++pass) {
+        for (uint32_t v : bpc) {
+            if (v == tune.blocks_per_cu) continue;
+            const uint32_t old = tune.blocks_per_cu;
+            tune.blocks_per_cu = v;
+            const double t = measure();
+            evals++;
+            if (t < best) best = t;
+            else tune.blocks_per_cu = old;
+        }
+        for (uint32_t v : rows) {
+            if (v == tune.big_row) continue;
+            const uint32_t old = tune.big_row;
+            tune.big_row = v;
+            const double t = measure();
+            evals++;
+            if (t < best) best = t;
+            else tune.big_row = old;
+        }
+        for (uint32_t v : pieces) {
+            if (v == tune.piece) continue;
+            const uint32_t old = tune.piece;
+            tune.piece = v;
+            const double t = measure();
+            evals++;
+            if (t < best) best = t;
+            else tune.piece = old;
+        }
+    }
+    for (auto &g : coop_grid) g = 0;
+    start_min_lattice();   // leave the converged state behind
+    sync();
+    collect_timing();
+    kstat[DCG_K_PUSH] = keep;
+    if (res) {
+        res->blocks_per_cu = tune.blocks_per_cu;
+        res->big_row = tune.big_row;
+        res->piece = tune.piece;
+        res->evaluations = evals;
+        res->best_ms = best;
     }
 }
 

@@ -78,6 +78,14 @@ KERNELS = {"push": 0, "pull": 1, "invalidate": 2, "pr_solve": 3, "pr_amend": 4, 
            "append_rows": 6}
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("blocks_per_cu", C.c_uint32), ("big_row", C.c_uint32), ("piece", C.c_uint32),
+                ("evaluations", C.c_uint32), ("best_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class EngineInfo(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in (
         "snapshots", "store_slots", "store_used_slots", "chunks", "frames", "host_bytes",
@@ -124,6 +132,8 @@ def lib() -> C.CDLL:
         "dcg_get_info": (I, [P, C.POINTER(EngineInfo)]),
         "dcg_sync": (I, [P]),
         "dcg_get_kernel_stats": (I, [P, U32, C.POINTER(KernelStats)]),
+        "dcg_autotune": (I, [P, C.POINTER(TuneResult)]),
+        "dcg_set_tuning": (I, [P, U32, U32, U32]),
         "dcg_reset_kernel_stats": (I, [P]),
         "dcg_mlp_eval": (I, [C.c_int32, U32, P, P, P, P, P]),
         "dcg_gen_rmat": (I, [U32, U64, C.c_double, C.c_double, C.c_double, U64, P, P]),
@@ -359,6 +369,14 @@ class Engine:
             self._check(lib().dcg_get_kernel_stats(self._h, kid, C.byref(st)), "dcg_get_kernel_stats")
             out[name] = st.as_dict()
         return out
+
+    def autotune(self):
+        st = TuneResult()
+        self._check(lib().dcg_autotune(self._h, C.byref(st)), "dcg_autotune")
+        return st.as_dict()
+
+    def set_tuning(self, blocks_per_cu=0, big_row=0, piece=0):
+        self._check(lib().dcg_set_tuning(self._h, blocks_per_cu, big_row, piece), "dcg_set_tuning")
 
     def reset_kernel_stats(self):
         self._check(lib().dcg_reset_kernel_stats(self._h), "dcg_reset_kernel_stats")

@@ -210,6 +210,19 @@ int dcg_reset_kernel_stats(dcg_engine *eng);
 int dcg_mlp_eval(int32_t device, uint32_t rows, const float *x, const float *y, const float *params,
                  float *out, float *grad);
 
+/* Autotuner (no reference counterpart; the reference fixes block 512 and its
+ * CTA/warp thresholds, include/utils/graphs/traversal.h:80-90): times the
+ * persistent closure on the loaded graph for candidate workgroups per CU,
+ * large-row thresholds and piece sizes (coordinate descent) and keeps the
+ * fastest.  Call after dcg_start; leaves the converged state unchanged. */
+typedef struct {
+    uint32_t blocks_per_cu, big_row, piece, evaluations;
+    double best_ms;
+} dcg_tune_result;
+int dcg_autotune(dcg_engine *eng, dcg_tune_result *out);
+/* Set the tuned parameters directly (0 keeps a value). */
+int dcg_set_tuning(dcg_engine *eng, uint32_t blocks_per_cu, uint32_t big_row, uint32_t piece);
+
 /* Synchronise the engine's stream (timing boundaries). */
 int dcg_sync(dcg_engine *eng);
 

@@ -166,3 +166,30 @@ def test_mixed_programs_one_engine():
         check(f"add {s}")
         eng.cache_maintain()
     eng.close()
+
+
+def test_autotune_keeps_values_and_picks_candidates():
+    announce("autotune rmat14 sssp")
+    n, src, dst = rmat_graph(14, 200000, seed=2)
+    eng = _engine(app="sssp", weight_mode=1, weight_seed=3, weight_max=100)
+    eng.load_graph(n, src, dst)
+    eng.start()
+    before = eng.gather_value()
+    res = eng.autotune()
+    assert res["blocks_per_cu"] in (1, 2, 3, 4) and res["big_row"] in (64, 128, 256, 512, 1024, 4096)
+    assert res["piece"] in (256, 512, 1024, 2048, 4096) and res["evaluations"] >= 10
+    assert np.array_equal(eng.gather_value(), before)
+    rp, col = oracle.csr_from_edges(n, src, dst)
+    assert np.array_equal(before, oracle.sssp(n, rp, col, 0, oracle.W_SEEDED, 3, 100))
+    # an odd but legal setting still gives exact results
+    import dycgraph_amd as d
+    adds, dels = d.gen_stream(n, src, dst, [2000], [2000], seed=1)
+    eng.set_tuning(1, 64, 256)
+    eng.load_updates(adds, dels)
+    eng.del_edge(0, 2000)
+    eng.add_edge(0, 2000)
+    store = oracle.Store(n, src, dst)
+    store.apply(dels, adds)
+    rp, col = store.csr()
+    assert np.array_equal(eng.gather_value(), oracle.sssp(n, rp, col, 0, oracle.W_SEEDED, 3, 100))
+    eng.close()
