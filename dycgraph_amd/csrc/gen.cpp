@@ -174,3 +174,57 @@ extern "C" int dcg_gen_stream(uint32_t num_nodes, const uint32_t *src, const uin
     }
     return DCG_OK;
 }
+
+// Road-like grid: side x side vertices (id = r*side + c), 4-neighbour arcs in
+// both directions, plus for each cell with hash < diag_frac a diagonal to
+// (r+1, c+1) in both directions.  Returned sorted by (src, dst).  Pass
+// src == NULL to get the edge count.
+extern "C" int dcg_gen_grid(uint32_t side, double diag_frac, uint64_t seed, uint32_t *src, uint32_t *dst,
+                            uint64_t *num_edges) {
+    if (side < 2 || (uint64_t)side * side > 0xFFFFFFFFull || !num_edges || diag_frac < 0 || diag_frac > 1)
+        return DCG_ERR_ARG;
+    const uint64_t thresh = (uint64_t)(diag_frac * 18446744073709551615.0);
+    auto diag = [&](uint32_t r, uint32_t c) {   // cell (r, c) has a diagonal to (r+1, c+1)
+        return r + 1 < side && c + 1 < side && dcg::mix64(seed ^ ((uint64_t)r * side + c)) < thresh;
+    };
+    uint64_t k = 0;
+    for (uint32_t r = 0; r < side; ++r) {
+        for (uint32_t c = 0; c < side; ++c) {
+            const uint32_t u = r * side + c;
+            uint32_t nb[6];
+            int m = 0;
+            if (r > 0 && c > 0 && diag(r - 1, c - 1)) nb[m++] = u - side - 1;
+            if (r > 0) nb[m++] = u - side;
+            if (c > 0) nb[m++] = u - 1;
+            if (c + 1 < side) nb[m++] = u + 1;
+            if (r + 1 < side) nb[m++] = u + side;
+            if (diag(r, c)) nb[m++] = u + side + 1;
+            for (int j = 0; j < m; ++j) {
+                if (src) {
+                    src[k] = u;
+                    dst[k] = nb[j];
+                }
+                k++;
+            }
+        }
+    }
+    *num_edges = k;
+    return DCG_OK;
+}
+
+// Batch sizes lognormal(mu, sigma) clipped to [lo, hi] (Box-Muller on
+// splitmix64 draws).
+extern "C" int dcg_gen_lognormal_sizes(uint32_t count, double mu, double sigma, uint64_t lo, uint64_t hi,
+                                       uint64_t seed, uint64_t *out) {
+    if (!out || lo > hi) return DCG_ERR_ARG;
+    for (uint32_t i = 0; i < count; ++i) {
+        const double u1 = ((dcg::mix64(seed ^ (2ull * i)) >> 11) + 1) * (1.0 / 9007199254740993.0);
+        const double u2 = (dcg::mix64(seed ^ (2ull * i + 1)) >> 11) * (1.0 / 9007199254740992.0);
+        const double z = std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+        double x = std::exp(mu + sigma * z);
+        if (x < (double)lo) x = (double)lo;
+        if (x > (double)hi) x = (double)hi;
+        out[i] = (uint64_t)std::llround(x);
+    }
+    return DCG_OK;
+}

@@ -139,6 +139,8 @@ def lib() -> C.CDLL:
         "dcg_gen_rmat": (I, [U32, U64, C.c_double, C.c_double, C.c_double, U64, P, P]),
         "dcg_gen_rmat_gpu": (I, [C.c_int32, U32, U64, C.c_double, C.c_double, C.c_double, U64, P, P]),
         "dcg_gen_stream": (I, [U32, P, P, U64, U32, P, P, U64, P, P]),
+        "dcg_gen_grid": (I, [U32, C.c_double, U64, P, P, C.POINTER(U64)]),
+        "dcg_gen_lognormal_sizes": (I, [U32, C.c_double, C.c_double, U64, U64, U64, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -172,6 +174,27 @@ def gen_rmat(scale: int, num_edges: int, a=0.57, b=0.19, c=0.19, seed=1, gpu=Non
     if st:
         raise DcgError(st, "dcg_gen_rmat failed")
     return src, dst
+
+
+def gen_grid(side, diag_frac=0.05, seed=1):
+    m = C.c_uint64()
+    st = lib().dcg_gen_grid(side, diag_frac, seed, None, None, C.byref(m))
+    if st:
+        raise DcgError(st, "dcg_gen_grid failed")
+    src = np.empty(m.value, np.uint32)
+    dst = np.empty(m.value, np.uint32)
+    st = lib().dcg_gen_grid(side, diag_frac, seed, _ptr(src), _ptr(dst), C.byref(m))
+    if st:
+        raise DcgError(st, "dcg_gen_grid failed")
+    return src, dst
+
+
+def gen_lognormal_sizes(count, mu=8.0, sigma=1.5, lo=100, hi=100000, seed=1):
+    out = np.empty(count, np.uint64)
+    st = lib().dcg_gen_lognormal_sizes(count, mu, sigma, lo, hi, seed, _ptr(out))
+    if st:
+        raise DcgError(st, "dcg_gen_lognormal_sizes failed")
+    return out
 
 
 def gen_stream(n, src, dst, adds_per, dels_per, seed=7):

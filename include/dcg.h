@@ -236,6 +236,14 @@ int dcg_gen_rmat(uint32_t scale, uint64_t num_edges, double a, double b, double 
  * draws plus a first-occurrence scan); src/dst are host buffers. */
 int dcg_gen_rmat_gpu(int32_t device, uint32_t scale, uint64_t num_edges, double a, double b, double c,
                      uint64_t seed, uint32_t *src, uint32_t *dst);
+/* Road-like grid (side x side, ids r*side+c): 4-neighbour arcs both ways plus a
+ * seeded diag_frac of cells with a diagonal to (r+1,c+1), both ways; sorted by
+ * (src,dst).  With src == NULL only *num_edges is set. */
+int dcg_gen_grid(uint32_t side, double diag_frac, uint64_t seed, uint32_t *src, uint32_t *dst,
+                 uint64_t *num_edges);
+/* `count` batch sizes lognormal(mu, sigma) clipped to [lo, hi]. */
+int dcg_gen_lognormal_sizes(uint32_t count, double mu, double sigma, uint64_t lo, uint64_t hi, uint64_t seed,
+                            uint64_t *out);
 /* Update stream: per snapshot `dels` deletions uniform over the edges present
  * at the snapshot start (without replacement) and `adds` insertions uniform
  * over absent non-loop pairs (distinct within the snapshot). The graph must be
