@@ -2,6 +2,7 @@
 #pragma once
 #include <chrono>
 #include <cstdio>
+#include <cstring>
 #include <memory>
 #include <stdexcept>
 #include <string>

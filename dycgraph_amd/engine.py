@@ -139,6 +139,7 @@ def lib() -> C.CDLL:
         "dcg_gen_rmat": (I, [U32, U64, C.c_double, C.c_double, C.c_double, U64, P, P]),
         "dcg_gen_rmat_gpu": (I, [C.c_int32, U32, U64, C.c_double, C.c_double, C.c_double, U64, P, P]),
         "dcg_gen_stream": (I, [U32, P, P, U64, U32, P, P, U64, P, P]),
+        "dcg_calibrate": (I, [C.c_int32, U64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "dcg_gen_grid": (I, [U32, C.c_double, U64, P, P, C.POINTER(U64)]),
         "dcg_gen_lognormal_sizes": (I, [U32, C.c_double, C.c_double, U64, U64, U64, P]),
     }
@@ -174,6 +175,15 @@ def gen_rmat(scale: int, num_edges: int, a=0.57, b=0.19, c=0.19, seed=1, gpu=Non
     if st:
         raise DcgError(st, "dcg_gen_rmat failed")
     return src, dst
+
+
+def calibrate(nbytes=1 << 30, device=0):
+    """(HBM copy GB/s, pinned-host zero-copy read GB/s) measured on the GPU."""
+    h, z = C.c_double(), C.c_double()
+    st = lib().dcg_calibrate(device, nbytes, C.byref(h), C.byref(z))
+    if st:
+        raise DcgError(st, "dcg_calibrate failed")
+    return h.value, z.value
 
 
 def gen_grid(side, diag_frac=0.05, seed=1):

@@ -223,6 +223,10 @@ int dcg_autotune(dcg_engine *eng, dcg_tune_result *out);
 /* Set the tuned parameters directly (0 keeps a value). */
 int dcg_set_tuning(dcg_engine *eng, uint32_t blocks_per_cu, uint32_t big_row, uint32_t piece);
 
+/* Link microbenchmarks on `device`: HBM copy rate (read+write bytes / time of a
+ * `bytes`-sized copy kernel) and zero-copy read rate of pinned host memory. */
+int dcg_calibrate(int32_t device, uint64_t bytes, double *hbm_copy_gbs, double *host_read_gbs);
+
 /* Synchronise the engine's stream (timing boundaries). */
 int dcg_sync(dcg_engine *eng);
 
