@@ -131,6 +131,8 @@ struct Ctl {
     uint32_t pad;
     unsigned long long edges; // adjacency entries scanned
     unsigned long long expansions; // rows expanded
+    uint32_t solo_round;      // round reached by a workgroup-0 stretch
+    uint32_t solo_resume_b;   // that stretch stopped before the round's pieces
     unsigned long long bad;   // PR: non-finite values seen
 };
 

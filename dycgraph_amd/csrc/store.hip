@@ -328,7 +328,7 @@ __device__ __forceinline__ uint64_t group_end(const K *keys, uint64_t s, uint64_
 __device__ __forceinline__ uint32_t load_slot(const StoreView &s, uint64_t slot) { return *row_base(s, slot); }
 __device__ __forceinline__ void store_slot(const StoreView &s, uint64_t slot, uint32_t v) { *row_base_w(s, slot) = v; }
 
-constexpr uint32_t kBigDelete = 8192;   // rows above this use one 1024-thread workgroup
+constexpr uint32_t kBigDelete = 1024;   // rows above this use one 1024-thread workgroup
 
 // Index of the first record of x's run among the group's sorted targets
 // [gs, ge) (lower bound on the low halves).
@@ -815,6 +815,12 @@ __global__ void k_fold_reads(StoreView s, uint32_t n, float *reads, float *rows,
 }
 
 void Engine::fold_reads(dcg_phase_stats *st) {
+    if (cfg.placement == DCG_PLACE_DEVICE) {
+        // every chunk is resident: all adjacency reads are HBM reads
+        if (st) st->cached_edge_reads += st->edges_scanned;
+        cum_cached_reads += st ? st->edges_scanned : 0;
+        return;
+    }
     fill<unsigned long long>(stream, d_counters.p + C_CACHED_READS, 2, 0ull);
     float *reads = cache.d_reads.n ? cache.d_reads.p : nullptr;
     float *rows = cache.d_reads.n ? cache.d_reads.p + cache.chunks : nullptr;

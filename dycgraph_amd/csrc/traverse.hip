@@ -192,184 +192,270 @@ struct WaveLds {
     const uint32_t *base[64];
 };
 
+// Work split of one group of waves (the whole grid, or workgroup 0 alone).
+struct Team {
+    uint32_t wave, nwaves;   // this wave's index and the team's wave count
+    uint32_t tid, nthreads;  // thread index and count (for plain loops)
+};
+
+// Phase A of round r: frontier entries in batches of 64 (the first batch of
+// every wave is static; further batches come from an atomic cursor), rows up to
+// `big_row` expanded edge-balanced in the wave, larger rows listed as pieces.
+template <uint32_t MODE>
+__device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L, uint64_t &edges,
+                        uint64_t &expansions) {
+    const uint32_t cur = r % 3, nxt = (r + 1) % 3;
+    const uint32_t lane = lane_id();
+    const uint32_t cnt = *(volatile uint32_t *)&a.ctl->count[cur];
+    const uint32_t *in = r == 0 ? a.in0 : ((r & 1) ? a.qa : a.qb);
+    uint32_t *out = (r & 1) ? a.qb : a.qa;
+    const uint32_t stamp = a.stamp0 + r;
+    uint32_t batch = t.wave;
+    for (;;) {
+        const uint32_t b = batch * kWave;
+        if (b >= cnt) break;
+        const uint32_t i = b + lane;
+        const bool has = i < cnt;
+        const uint32_t g = has ? in[i] : 0u;
+        uint32_t q = 0, v = 0;
+        uint32_t deg = 0, val = 0;
+        uint64_t off = 0;
+        if (has) {
+            if (MODE == M_PR || MODE == M_PR_AMEND) {
+                v = g;
+                const RowDesc d = a.s.out[v];
+                const float x = MODE == M_PR ? a.delta[i] : a.rank[v] * a.sign;
+                if (d.deg && x != 0.f) {
+                    deg = d.deg;
+                    off = d.off;
+                    val = __float_as_uint(0.85f * x / (float)d.deg);
+                    a.s.hot_out[v] += 1;
+                }
+            } else {
+                decode_g(a, g, q, v);
+                const RowDesc d = MODE == M_PULL ? a.s.in[v] : a.s.out[v];
+                val = MODE == M_PULL ? 0u : (uint32_t)(a.st[g] >> 32);
+                if (d.deg && (MODE == M_PULL || val != kInf)) {
+                    deg = d.deg;
+                    off = d.off;
+                    if (MODE == M_PULL) a.s.hot_in[v] += 1;
+                    else a.s.hot_out[v] += 1;
+                }
+            }
+        }
+        expansions += (uint64_t)__popcll(__ballot(deg > 0));
+        if (deg > a.big_row) {
+            const uint64_t pieces = (deg + a.piece - 1) / a.piece;
+            const unsigned long long p = atomicAdd(&a.ctl->big[cur], (1ull << kBigShift) | pieces);
+            const uint64_t e = p >> kBigShift;
+            if (e < a.bigcap) {
+                BigEntry be;
+                be.off = off;
+                be.g = g;
+                be.deg = deg;
+                be.val = val;
+                be.piece0 = (uint32_t)(p & kBigMask);
+                a.big[e] = be;
+            } else {
+                a.ctl->overflow = 1;
+            }
+            deg = 0;
+        }
+        const uint32_t incl = wave_incl_scan(deg);
+        const uint32_t total = __shfl(incl, 63, kWave);
+        L.pref[lane] = incl - deg;
+        L.val[lane] = val;
+        L.owner[lane] = v;
+        L.q[lane] = q;
+        L.base[lane] = deg ? row_base(a.s, off) : nullptr;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        edges += total;
+        for (uint32_t k0 = 0; k0 < total; k0 += kWave) {
+            const uint32_t k = k0 + lane;
+            bool want = false;
+            uint32_t og = 0;
+            if (k < total) {
+                uint32_t lo = 0, hi = 64;   // last owner with pref <= k
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (L.pref[mid] <= k) lo = mid;
+                    else hi = mid;
+                }
+                const uint32_t x = L.base[lo][k - L.pref[lo]];
+                want = edge_op<MODE>(a, L.q[lo], L.owner[lo], L.val[lo], x, stamp, og);
+            }
+            if (MODE != M_PULL) emit<MODE>(a, out, &a.ctl->count[nxt], want, og);
+        }
+        __builtin_amdgcn_wave_barrier();
+        uint32_t nb = 0;
+        if (lane == 0) nb = t.nwaves + atomicAdd(&a.ctl->cursor[cur], 1u);
+        batch = __shfl(nb, 0, kWave);
+    }
+}
+
+// Phase B of round r: the pieces of the large rows (first piece of every wave
+// static, then an atomic cursor).
+template <uint32_t MODE>
+__device__ void phase_b(const TravArgs &a, uint32_t r, const Team &t, uint64_t &edges) {
+    const uint32_t cur = r % 3, nxt = (r + 1) % 3;
+    const uint32_t lane = lane_id();
+    uint32_t *out = (r & 1) ? a.qb : a.qa;
+    const uint32_t stamp = a.stamp0 + r;
+    const unsigned long long bp = *(volatile unsigned long long *)&a.ctl->big[cur];
+    uint64_t nb = bp >> kBigShift;
+    if (nb > a.bigcap) nb = a.bigcap;
+    const uint64_t np = bp & kBigMask;
+    if (nb == 0) return;
+    uint32_t p = t.wave;
+    while (p < np) {
+        uint64_t lo = 0, hi = nb;
+        while (hi - lo > 1) {
+            const uint64_t mid = (lo + hi) >> 1;
+            if (a.big[mid].piece0 <= p) lo = mid;
+            else hi = mid;
+        }
+        const BigEntry be = a.big[lo];
+        const uint32_t start = (p - be.piece0) * a.piece;
+        if (start < be.deg) {
+            const uint32_t len = be.deg - start < a.piece ? be.deg - start : a.piece;
+            const uint32_t *base = row_base(a.s, be.off + start);
+            uint32_t q = 0, v = 0;
+            if (MODE == M_PR || MODE == M_PR_AMEND) v = be.g;
+            else decode_g(a, be.g, q, v);
+            edges += lane == 0 ? len : 0;
+            if (MODE == M_PULL) {
+                uint64_t best = ~0ull;
+                for (uint32_t k = lane; k < len; k += kWave) {
+                    const uint64_t c = pull_cand(a, q, v, base[k]);
+                    best = c < best ? c : best;
+                }
+                for (int d = 32; d >= 1; d >>= 1) {
+                    const uint64_t o = __shfl_xor(best, d, kWave);
+                    best = o < best ? o : best;
+                }
+                if (lane == 0) pull_apply(a, q, v, best);
+            } else {
+                for (uint32_t k0 = 0; k0 < len; k0 += kWave) {
+                    const uint32_t k = k0 + lane;
+                    bool want = false;
+                    uint32_t og = 0;
+                    if (k < len) want = edge_op<MODE>(a, q, v, be.val, base[k], stamp, og);
+                    emit<MODE>(a, out, &a.ctl->count[nxt], want, og);
+                }
+            }
+        }
+        uint32_t np2 = 0;
+        if (lane == 0) np2 = t.nwaves + atomicAdd(&a.ctl->big_cursor[cur], 1u);
+        p = __shfl(np2, 0, kWave);
+    }
+}
+
+// Start of round r: ring counters for rounds r+1 / r+2, and PR's consume step.
+template <uint32_t MODE>
+__device__ void round_open(const TravArgs &a, uint32_t r, const Team &t) {
+    const uint32_t cur = r % 3, nxt = (r + 1) % 3, nn = (r + 2) % 3;
+    if (t.tid == 0) {
+        a.ctl->count[nn] = 0;
+        a.ctl->cursor[nxt] = 0;
+        a.ctl->big[nxt] = 0;
+        a.ctl->big_cursor[nxt] = 0;
+        a.ctl->rounds = r + 1;
+    }
+    if (MODE == M_PR) {
+        // Consume (pr_residual.cuh:53-69): fold residuals above eps into ranks.
+        const uint32_t cnt = *(volatile uint32_t *)&a.ctl->count[cur];
+        const uint32_t *in = r == 0 ? a.in0 : ((r & 1) ? a.qa : a.qb);
+        for (uint32_t i = t.tid; i < cnt; i += t.nthreads) {
+            const uint32_t v = in[i];
+            const float res = a.res[v];
+            a.queued[v] = 0;
+            float dl = 0.f;
+            if (!isfinite(res) || !isfinite(a.rank[v])) atomicExch((unsigned int *)&a.ctl->bad, 1u);
+            if (fabsf(res) > a.eps) {
+                a.rank[v] += res;
+                a.res[v] = 0.f;
+                dl = res;
+            }
+            a.delta[i] = dl;
+        }
+    }
+}
+
+__device__ __forceinline__ bool pieces_pending(const TravArgs &a, uint32_t r) {
+    return (*(volatile unsigned long long *)&a.ctl->big[r % 3] & kBigMask) != 0;
+}
+
+constexpr uint32_t kSoloMax = 1024;      // frontier entries workgroup 0 handles alone
+constexpr uint32_t kSoloPieces = 64;     // more pending pieces than this end a solo stretch
+
+// One cooperative launch runs rounds until the frontier is empty.  While the
+// frontier is small, workgroup 0 runs rounds by itself with workgroup barriers
+// (the rest of the grid waits at one grid barrier); otherwise all workgroups
+// share each round, separated by grid barriers (a second one only when the
+// round queued pieces of large rows).
 template <uint32_t MODE>
 __global__ __launch_bounds__(256) void k_persistent(TravArgs a) {
     cg::grid_group grid = cg::this_grid();
     __shared__ WaveLds lds_all[kBlock / kWave];
     const uint32_t lane = lane_id();
-    const uint32_t wib = threadIdx.x / kWave;
-    WaveLds &L = lds_all[wib];
-    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t gthreads = gridDim.x * blockDim.x;
+    WaveLds &L = lds_all[threadIdx.x / kWave];
+    const Team gt{(blockIdx.x * blockDim.x + threadIdx.x) / kWave, gridDim.x * blockDim.x / kWave,
+                  blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x};
+    const Team bt{threadIdx.x / kWave, blockDim.x / kWave, threadIdx.x, blockDim.x};
     uint64_t edges = 0, expansions = 0;
-
-    for (uint32_t r = 0; r < a.max_rounds; ++r) {
-        const uint32_t cur = r % 3, nxt = (r + 1) % 3, nn = (r + 2) % 3;
-        const uint32_t cnt = *(volatile uint32_t *)&a.ctl->count[cur];
+    uint32_t r = 0;
+    while (r < a.max_rounds) {
+        const uint32_t cnt = *(volatile uint32_t *)&a.ctl->count[r % 3];
         if (cnt == 0) break;
-        const uint32_t *in = r == 0 ? a.in0 : ((r & 1) ? a.qa : a.qb);
-        uint32_t *out = (r & 1) ? a.qb : a.qa;
-        const uint32_t stamp = a.stamp0 + r;
-        if (gtid == 0) {
-            a.ctl->count[nn] = 0;
-            a.ctl->cursor[nxt] = 0;
-            a.ctl->big[nxt] = 0;
-            a.ctl->big_cursor[nxt] = 0;
-        }
-        if (MODE == M_PR) {
-            // Consume (pr_residual.cuh:53-69): fold residuals above eps into ranks.
-            for (uint32_t i = gtid; i < cnt; i += gthreads) {
-                const uint32_t v = in[i];
-                const float res = a.res[v];
-                a.queued[v] = 0;
-                float dl = 0.f;
-                if (!isfinite(res) || !isfinite(a.rank[v])) atomicExch((unsigned int *)&a.ctl->bad, 1u);
-                if (fabsf(res) > a.eps) {
-                    a.rank[v] += res;
-                    a.res[v] = 0.f;
-                    dl = res;
+        if (cnt <= kSoloMax && gridDim.x > 1) {
+            if (blockIdx.x == 0) {
+                // solo stretch: rounds on workgroup 0 until the frontier grows,
+                // empties, or a round queues many large-row pieces
+                uint32_t resume_b = 0;
+                for (;;) {
+                    round_open<MODE>(a, r, bt);
+                    __syncthreads();
+                    phase_a<MODE>(a, r, bt, L, edges, expansions);
+                    __syncthreads();
+                    const uint64_t np = *(volatile unsigned long long *)&a.ctl->big[r % 3] & kBigMask;
+                    if (np > kSoloPieces) {
+                        resume_b = 1;   // the whole grid takes this round's pieces
+                        break;
+                    }
+                    if (np) {
+                        phase_b<MODE>(a, r, bt, edges);
+                        __syncthreads();
+                    }
+                    ++r;
+                    const uint32_t c2 = *(volatile uint32_t *)&a.ctl->count[r % 3];
+                    if (c2 == 0 || c2 > kSoloMax || r >= a.max_rounds) break;
                 }
-                a.delta[i] = dl;
+                if (threadIdx.x == 0) {
+                    a.ctl->solo_round = r;
+                    a.ctl->solo_resume_b = resume_b;
+                }
             }
             grid.sync();
+            r = *(volatile uint32_t *)&a.ctl->solo_round;
+            if (*(volatile uint32_t *)&a.ctl->solo_resume_b) {
+                phase_b<MODE>(a, r, gt, edges);
+                grid.sync();
+                ++r;
+            }
+            continue;
         }
-
-        // ---------------- phase A
-        for (;;) {
-            uint32_t b = 0;
-            if (lane == 0) b = atomicAdd(&a.ctl->cursor[cur], kWave);
-            b = __shfl(b, 0, kWave);
-            if (b >= cnt) break;
-            const uint32_t i = b + lane;
-            (void)gthreads;
-            const bool has = i < cnt;
-            uint32_t g = has ? in[i] : 0u;
-            uint32_t q = 0, v = 0;
-            uint32_t deg = 0, val = 0;
-            uint64_t off = 0;
-            if (has) {
-                if (MODE == M_PR || MODE == M_PR_AMEND) {
-                    v = g;
-                    const RowDesc d = a.s.out[v];
-                    const float x = MODE == M_PR ? a.delta[i] : a.rank[v] * a.sign;
-                    if (d.deg && x != 0.f) {
-                        deg = d.deg;
-                        off = d.off;
-                        val = __float_as_uint(0.85f * x / (float)d.deg);
-                        a.s.hot_out[v] += 1;
-                    }
-                } else {
-                    decode_g(a, g, q, v);
-                    const RowDesc d = MODE == M_PULL ? a.s.in[v] : a.s.out[v];
-                    val = MODE == M_PULL ? 0u : (uint32_t)(a.st[g] >> 32);
-                    if (d.deg && (MODE == M_PULL || val != kInf)) {
-                        deg = d.deg;
-                        off = d.off;
-                        if (MODE == M_PULL) a.s.hot_in[v] += 1;
-                        else a.s.hot_out[v] += 1;
-                    }
-                }
-            }
-            expansions += (uint64_t)__popcll(__ballot(deg > 0));
-            if (deg > a.big_row) {
-                const uint64_t pieces = (deg + a.piece - 1) / a.piece;
-                const unsigned long long p = atomicAdd(&a.ctl->big[cur], (1ull << kBigShift) | pieces);
-                const uint64_t e = p >> kBigShift;
-                if (e < a.bigcap) {
-                    BigEntry be;
-                    be.off = off;
-                    be.g = g;
-                    be.deg = deg;
-                    be.val = val;
-                    be.piece0 = (uint32_t)(p & kBigMask);
-                    a.big[e] = be;
-                } else {
-                    a.ctl->overflow = 1;
-                }
-                deg = 0;
-            }
-            const uint32_t incl = wave_incl_scan(deg);
-            const uint32_t total = __shfl(incl, 63, kWave);
-            L.pref[lane] = incl - deg;
-            L.val[lane] = val;
-            L.owner[lane] = v;
-            L.q[lane] = q;
-            L.base[lane] = deg ? row_base(a.s, off) : nullptr;
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            edges += total;
-            for (uint32_t k0 = 0; k0 < total; k0 += kWave) {
-                const uint32_t k = k0 + lane;
-                bool want = false;
-                uint32_t og = 0;
-                if (k < total) {
-                    uint32_t lo = 0, hi = 64;   // last owner with pref <= k
-                    while (hi - lo > 1) {
-                        const uint32_t mid = (lo + hi) >> 1;
-                        if (L.pref[mid] <= k) lo = mid;
-                        else hi = mid;
-                    }
-                    const uint32_t x = L.base[lo][k - L.pref[lo]];
-                    want = edge_op<MODE>(a, L.q[lo], L.owner[lo], L.val[lo], x, stamp, og);
-                }
-                if (MODE != M_PULL) emit<MODE>(a, out, &a.ctl->count[nxt], want, og);
-            }
-            __builtin_amdgcn_wave_barrier();
-        }
+        round_open<MODE>(a, r, gt);
+        if (MODE == M_PR) grid.sync();
+        phase_a<MODE>(a, r, gt, L, edges, expansions);
         grid.sync();
-
-        // ---------------- phase B: pieces of large rows
-        {
-            const unsigned long long bp = *(volatile unsigned long long *)&a.ctl->big[cur];
-            uint64_t nb = bp >> kBigShift;
-            if (nb > a.bigcap) nb = a.bigcap;
-            const uint64_t np = bp & kBigMask;
-            for (;;) {
-                uint32_t p = 0;
-                if (lane == 0) p = atomicAdd(&a.ctl->big_cursor[cur], 1u);
-                p = __shfl(p, 0, kWave);
-                if (p >= np || nb == 0) break;
-                uint64_t lo = 0, hi = nb;
-                while (hi - lo > 1) {
-                    const uint64_t mid = (lo + hi) >> 1;
-                    if (a.big[mid].piece0 <= p) lo = mid;
-                    else hi = mid;
-                }
-                const BigEntry be = a.big[lo];
-                const uint32_t start = (p - be.piece0) * a.piece;
-                if (start >= be.deg) continue;
-                const uint32_t len = be.deg - start < a.piece ? be.deg - start : a.piece;
-                const uint32_t *base = row_base(a.s, be.off + start);
-                uint32_t q = 0, v = 0;
-                if (MODE == M_PR || MODE == M_PR_AMEND) v = be.g;
-                else decode_g(a, be.g, q, v);
-                edges += lane == 0 ? len : 0;
-                if (MODE == M_PULL) {
-                    uint64_t best = ~0ull;
-                    for (uint32_t k = lane; k < len; k += kWave) {
-                        const uint64_t c = pull_cand(a, q, v, base[k]);
-                        best = c < best ? c : best;
-                    }
-                    for (int d = 32; d >= 1; d >>= 1) {
-                        const uint64_t o = __shfl_xor(best, d, kWave);
-                        best = o < best ? o : best;
-                    }
-                    if (lane == 0) pull_apply(a, q, v, best);
-                } else {
-                    for (uint32_t k0 = 0; k0 < len; k0 += kWave) {
-                        const uint32_t k = k0 + lane;
-                        bool want = false;
-                        uint32_t og = 0;
-                        if (k < len) want = edge_op<MODE>(a, q, v, be.val, base[k], stamp, og);
-                        emit<MODE>(a, out, &a.ctl->count[nxt], want, og);
-                    }
-                }
-            }
+        if (pieces_pending(a, r)) {
+            phase_b<MODE>(a, r, gt, edges);
+            grid.sync();
         }
-        grid.sync();
-        if (gtid == 0) a.ctl->rounds = r + 1;
+        ++r;
     }
-    // every lane of a wave holds the same phase-A total; lane 0 reports it
+    // every lane of a wave holds the same totals; lane 0 reports them
     if (lane == 0 && edges) atomicAdd(&a.ctl->edges, edges);
     if (lane == 0 && expansions) atomicAdd(&a.ctl->expansions, expansions);
 }
@@ -391,6 +477,8 @@ __global__ void k_ctl_setup(Ctl *c, const uint32_t *src0, uint32_t add0, uint32_
     c->edges = 0;
     c->expansions = 0;
     c->bad = 0;
+    c->solo_round = 0;
+    c->solo_resume_b = 0;
 }
 
 __global__ void k_init_state(uint64_t *st, uint32_t *tick, uint32_t n, uint32_t Q, const uint32_t *qapp,
