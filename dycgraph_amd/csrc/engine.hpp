@@ -131,7 +131,7 @@ struct Engine {
     DevBuf<uint32_t> d_k32c, d_k32d, d_v32c, d_v32d;   // add keys/payloads (unsorted)
     DevBuf<uint32_t> d_gstart, d_gstart2, d_uniq;
     DevBuf<uint32_t> d_used;           // per-record occurrence counters
-    DevBuf<uint32_t> d_occ, d_big_groups;  // large-row delete path
+    DevBuf<uint32_t> d_big_groups;     // large-row mutation paths
     DevBuf<unsigned long long> d_counters;  // phase counters (see store.hip)
     unsigned long long *h_counters = nullptr;
 

@@ -399,7 +399,8 @@ __device__ DelResult wave_delete_row(const StoreView &s, RowDesc *desc, uint32_t
 
 // One wave per touched row of at most kBigDelete entries; larger rows are
 // listed for k_del_big.
-__global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, const uint64_t *keys, uint64_t count,
+__global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, uint32_t *hot, const uint64_t *keys,
+                                                  uint64_t count,
                                                   const uint32_t *gstart, const unsigned long long *ngroups_p,
                                                   uint32_t *used, unsigned long long *counters, int count_stats,
                                                   uint32_t *big_groups, unsigned long long *nbig) {
@@ -419,6 +420,7 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, co
         const uint64_t ge = group_end<uint64_t>(keys, gs, count, row);
         const DelResult r = wave_delete_row(s, desc, row, d, keys, gs, ge, used);
         if (lane == 0) {
+            hot[row] += 1;   // the scan reads the whole row (cache-policy signal)
             missing += (ge - gs) - r.removed;
             changed += r.removed ? 1 : 0;
             rd += d.deg;
@@ -433,17 +435,21 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, co
     }
 }
 
-// One 1024-thread workgroup per large row.  Pass 1 counts the occurrences of
-// every target run; if no run has more occurrences than requests, every
-// occurrence of every target goes, and pass 2 compacts the row in 4096-entry
-// tiles with a workgroup prefix sum (all reads of a tile precede its writes).
-// Otherwise wave 0 takes the exact first-occurrence path.
-__global__ __launch_bounds__(1024) void k_del_big(StoreView s, RowDesc *desc, const uint64_t *keys, uint64_t count,
-                                                  const uint32_t *big_groups, const unsigned long long *nbig_p,
-                                                  uint32_t *occ, uint32_t *used, unsigned long long *counters,
-                                                  int count_stats) {
+// One 1024-thread workgroup per large row, one pass over the row in tiles of
+// 4,096 entries: each thread matches its 4 entries against the sorted targets
+// and flags matches in LDS; one wave then resolves the tile's matches in row
+// order through the per-run counters (exact first-occurrence rule); survivors
+// are compacted with a workgroup prefix sum.  All reads of a tile precede its
+// writes, and writes only move entries towards the row start.
+__global__ __launch_bounds__(1024) void k_del_big(StoreView s, RowDesc *desc, uint32_t *hot, const uint64_t *keys,
+                                                  uint64_t count, const uint32_t *big_groups,
+                                                  const unsigned long long *nbig_p, uint32_t *used,
+                                                  unsigned long long *counters, int count_stats) {
+    constexpr uint32_t kTileE = 4096;
+    __shared__ uint32_t runs[kTileE];    // run index of a matched entry, ~0u otherwise
+    __shared__ uint8_t gone[kTileE];
     __shared__ uint32_t wsum[16];
-    __shared__ uint32_t flag;
+    __shared__ uint32_t nmatch;
     const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
     const uint64_t nbig = *nbig_p;
     for (uint64_t b = blockIdx.x; b < nbig; b += gridDim.x) {
@@ -451,79 +457,83 @@ __global__ __launch_bounds__(1024) void k_del_big(StoreView s, RowDesc *desc, co
         const uint32_t row = (uint32_t)(keys[gs] >> 32);
         const uint64_t ge = group_end<uint64_t>(keys, gs, count, row);
         const RowDesc d = desc[row];
-        for (uint32_t i = tid; i < d.deg; i += blockDim.x) {
-            const uint32_t x = load_slot(s, d.off + i);
-            const uint64_t lo = target_lb(keys, gs, ge, x);
-            if (lo < ge && (uint32_t)keys[lo] == x) atomicAdd(&occ[lo], 1u);
-        }
-        if (tid == 0) flag = 0;
-        __syncthreads();
-        for (uint64_t r = gs + tid; r < ge; r += blockDim.x) {
-            const uint32_t x = (uint32_t)keys[r];
-            if (r > gs && (uint32_t)keys[r - 1] == x) continue;
-            if (occ[r] > target_ub(keys, r + 1, ge, x) - r) flag = 1;
-        }
-        __syncthreads();
-        uint32_t removed = 0, moved = 0;
-        if (flag) {
-            if (w == 0) {
-                const DelResult rr = wave_delete_row(s, desc, row, d, keys, gs, ge, used);
-                if (tid == 0) {
-                    removed = rr.removed;
-                    moved = rr.moved;
+        uint32_t write = 0, moved = 0;
+        for (uint32_t t0 = 0; t0 < d.deg; t0 += kTileE) {
+            if (tid == 0) nmatch = 0;
+            __syncthreads();
+            uint32_t xv[4];
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t e = j * 1024 + tid, i = t0 + e;
+                uint32_t run = ~0u;
+                xv[j] = 0;
+                if (i < d.deg) {
+                    xv[j] = load_slot(s, d.off + i);
+                    const uint64_t lo = target_lb(keys, gs, ge, xv[j]);
+                    if (lo < ge && (uint32_t)keys[lo] == xv[j]) {
+                        run = (uint32_t)(lo - gs);
+                        atomicAdd(&nmatch, 1u);
+                    }
+                }
+                runs[e] = run;
+                gone[e] = 0;
+            }
+            __syncthreads();
+            if (nmatch && w == 0) {
+                // resolve matches in row order
+                for (uint32_t c0 = 0; c0 < kTileE; c0 += kWave) {
+                    const uint32_t run = runs[c0 + lane];
+                    uint64_t mm = __ballot(run != ~0u);
+                    bool rem = false;
+                    while (mm) {
+                        const int l = __ffsll((unsigned long long)mm) - 1;
+                        if ((int)lane == l) {
+                            const uint64_t lo = gs + run;
+                            const uint32_t x = (uint32_t)keys[lo];
+                            const uint32_t requested = (uint32_t)(target_ub(keys, lo + 1, ge, x) - lo);
+                            rem = atomicAdd(&used[lo], 1u) < requested;
+                        }
+                        mm &= mm - 1;
+                    }
+                    if (rem) gone[c0 + lane] = 1;
                 }
             }
-        } else {
-            uint32_t write = 0;
-            for (uint32_t t0 = 0; t0 < d.deg; t0 += 4 * 1024) {
-                uint32_t xv[4];
-                bool keep[4];
-                for (int j = 0; j < 4; ++j) {
-                    const uint32_t i = t0 + j * 1024 + tid;
-                    keep[j] = false;
-                    if (i < d.deg) {
-                        xv[j] = load_slot(s, d.off + i);
-                        const uint64_t lo = target_lb(keys, gs, ge, xv[j]);
-                        keep[j] = !(lo < ge && (uint32_t)keys[lo] == xv[j]);
-                    }
+            __syncthreads();
+            bool keep[4];
+            for (int j = 0; j < 4; ++j) keep[j] = t0 + j * 1024 + tid < d.deg && !gone[j * 1024 + tid];
+            uint32_t pos[4];
+            uint32_t base = write;
+            for (int j = 0; j < 4; ++j) {
+                const uint64_t km = __ballot(keep[j]);
+                if (lane == 0) wsum[w] = (uint32_t)__popcll(km);
+                __syncthreads();
+                uint32_t before = 0, tot = 0;
+                for (uint32_t q = 0; q < 16; ++q) {
+                    before += q < w ? wsum[q] : 0u;
+                    tot += wsum[q];
                 }
-                uint32_t pos[4];
-                uint32_t base = write;
-                for (int j = 0; j < 4; ++j) {
-                    const uint64_t km = __ballot(keep[j]);
-                    if (lane == 0) wsum[w] = (uint32_t)__popcll(km);
-                    __syncthreads();
-                    uint32_t before = 0, tot = 0;
-                    for (uint32_t q = 0; q < 16; ++q) {
-                        before += q < w ? wsum[q] : 0u;
-                        tot += wsum[q];
-                    }
-                    pos[j] = base + before + lanes_below(km);
-                    base += tot;
-                    __syncthreads();
-                }
-                for (int j = 0; j < 4; ++j) {
-                    const uint32_t i = t0 + j * 1024 + tid;
-                    if (i < d.deg && keep[j] && pos[j] != i) {
-                        store_slot(s, d.off + pos[j], xv[j]);
-                        moved++;
-                    }
-                }
-                write = base;
+                pos[j] = base + before + lanes_below(km);
+                base += tot;
                 __syncthreads();
             }
-            if (tid == 0) {
-                removed = d.deg - write;
-                if (removed) desc[row].deg = write;
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t i = t0 + j * 1024 + tid;
+                if (keep[j] && pos[j] != i) {
+                    store_slot(s, d.off + pos[j], xv[j]);
+                    moved++;
+                }
             }
+            write = base;
+            __syncthreads();
         }
-        // block totals: `removed` lives in thread 0
         if (tid == 0) {
+            const uint32_t removed = d.deg - write;
+            if (removed) desc[row].deg = write;
             if (count_stats) {
                 atomicAdd(&counters[C_MISSING], (unsigned long long)((ge - gs) - removed));
                 if (removed) atomicAdd(&counters[C_CHANGED], 1ull);
             }
-            atomicAdd(&counters[C_SLOTS_READ], (unsigned long long)(2ull * d.deg));
+            atomicAdd(&counters[C_SLOTS_READ], (unsigned long long)d.deg);
+            hot[row] += 1;
         }
         if (moved) atomicAdd(&counters[C_SLOTS_WRITTEN], (unsigned long long)moved);
         __syncthreads();
@@ -564,11 +574,20 @@ __global__ void k_alloc_classes(uint64_t *tail, uint64_t total) {
     else tail[T_TAIL] = t;
 }
 
+constexpr uint32_t kBigAppend = 4096;   // relocations of larger rows use one 1024-thread workgroup
+
+__device__ __forceinline__ uint32_t class_of(uint64_t nd) {
+    const uint64_t c = nd < kMinRowCap ? kMinRowCap : nd;
+    return 64u - (uint32_t)__clzll((long long)(c - 1));
+}
+
 // One wave per touched row: relocate if needed, then append in request order.
-__global__ __launch_bounds__(256) void k_add_rows(StoreView s, RowDesc *desc, const uint32_t *keys, const uint32_t *vals,
-                                                  uint64_t count, const uint32_t *gstart,
+// Rows above kBigAppend entries that must relocate are listed for k_add_big.
+__global__ __launch_bounds__(256) void k_add_rows(StoreView s, RowDesc *desc, uint32_t *hot, const uint32_t *keys,
+                                                  const uint32_t *vals, uint64_t count, const uint32_t *gstart,
                                                   const unsigned long long *ngroups_p, uint64_t *tail,
-                                                  unsigned long long *counters, int count_stats) {
+                                                  unsigned long long *counters, int count_stats,
+                                                  uint32_t *big_groups, unsigned long long *nbig) {
     const uint32_t lane = lane_id();
     const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kWave;
     const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / kWave;
@@ -584,20 +603,24 @@ __global__ __launch_bounds__(256) void k_add_rows(StoreView s, RowDesc *desc, co
         const uint64_t nd = (uint64_t)d.deg + k;
         if (nd > d.cap) {
             if (overflow) continue;
-            uint64_t c = nd < kMinRowCap ? kMinRowCap : nd;
-            const uint32_t cls = 64u - (uint32_t)__clzll((long long)(c - 1));
+            if (d.deg > kBigAppend) {
+                if (lane == 0) big_groups[atomicAdd(nbig, 1ull)] = (uint32_t)gs;
+                continue;
+            }
+            const uint32_t cls = class_of(nd);
             unsigned long long noff = 0;
             if (lane == 0) noff = atomicAdd((unsigned long long *)&tail[T_CURSOR + cls], 1ull << cls);
             noff = __shfl(noff, 0, kWave);
             for (uint32_t i = lane; i < d.deg; i += kWave) store_slot(s, noff + i, load_slot(s, d.off + i));
+            if (lane == 0 && d.deg) hot[row] += 1;
             d.off = noff;
             d.cap = (uint32_t)(1ull << cls);
             relocated++;
             rd += d.deg;
             wr += d.deg;
         }
-        wr += k;
         for (uint32_t i = lane; i < k; i += kWave) store_slot(s, d.off + d.deg + i, vals[gs + i]);
+        wr += k;
         d.deg = (uint32_t)nd;
         if (lane == 0) desc[row] = d;
     }
@@ -605,6 +628,45 @@ __global__ __launch_bounds__(256) void k_add_rows(StoreView s, RowDesc *desc, co
         if (count_stats && relocated) atomicAdd(&counters[C_RELOCATED], relocated);
         if (rd) atomicAdd(&counters[C_SLOTS_READ], rd);
         if (wr) atomicAdd(&counters[C_SLOTS_WRITTEN], wr);
+    }
+}
+
+// Relocation of a large row: one 1024-thread workgroup copies the row into its
+// new block (16-byte accesses where the row is aligned) and appends the batch.
+__global__ __launch_bounds__(1024) void k_add_big(StoreView s, RowDesc *desc, uint32_t *hot, const uint32_t *keys,
+                                                  const uint32_t *vals, uint64_t count, const uint32_t *big_groups,
+                                                  const unsigned long long *nbig_p, uint64_t *tail,
+                                                  unsigned long long *counters, int count_stats) {
+    __shared__ unsigned long long noff_s;
+    const uint64_t nbig = *nbig_p;
+    for (uint64_t b = blockIdx.x; b < nbig; b += gridDim.x) {
+        const uint64_t gs = big_groups[b];
+        const uint32_t row = keys[gs];
+        const uint64_t ge = group_end<uint32_t>(keys, gs, count, row);
+        const uint32_t k = (uint32_t)(ge - gs);
+        RowDesc d = desc[row];
+        const uint64_t nd = (uint64_t)d.deg + k;
+        const uint32_t cls = class_of(nd);
+        if (threadIdx.x == 0) noff_s = atomicAdd((unsigned long long *)&tail[T_CURSOR + cls], 1ull << cls);
+        __syncthreads();
+        const uint64_t noff = noff_s;
+        // blocks above 65,536 slots are whole chunks; copy chunk segment by segment
+        for (uint64_t seg = 0; seg < d.deg; seg += kChunkEdges) {
+            const uint32_t len = (uint32_t)min((uint64_t)kChunkEdges, (uint64_t)d.deg - seg);
+            const uint32_t *src = row_base(s, d.off + seg);
+            uint32_t *dst = row_base_w(s, noff + seg);
+            for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) dst[i] = src[i];
+        }
+        for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) store_slot(s, noff + d.deg + i, vals[gs + i]);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            desc[row] = RowDesc{noff, (uint32_t)nd, (uint32_t)(1ull << cls)};
+            hot[row] += 1;
+            if (count_stats) atomicAdd(&counters[C_RELOCATED], 1ull);
+            atomicAdd(&counters[C_SLOTS_READ], (unsigned long long)d.deg);
+            atomicAdd(&counters[C_SLOTS_WRITTEN], (unsigned long long)nd);
+        }
+        __syncthreads();
     }
 }
 
@@ -642,23 +704,20 @@ void Engine::mutate_delete(uint64_t begin, uint64_t count, dcg_phase_stats *st) 
     StoreView v = view();
     const uint32_t wg = wave_grid(count);
     d_big_groups.ensure(count);
-    d_occ.ensure(count);
     fill<uint32_t>(stream, d_used.p, count, 0u);
-    fill<uint32_t>(stream, d_occ.p, count, 0u);
     const int slot = time_begin(DCG_K_DELETE_ROWS, -1);
-    hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream, v, d_out.p, kout, count, d_gstart.p,
+    hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream, v, d_out.p, d_hot_out.p, kout, count, d_gstart.p,
                        d_counters.p + C_GROUPS_OUT, d_used.p, d_counters.p, 1, d_big_groups.p,
                        d_counters.p + C_NBIG);
-    hipLaunchKernelGGL(k_del_big, dim3(256), dim3(1024), 0, stream, v, d_out.p, kout, count, d_big_groups.p,
-                       d_counters.p + C_NBIG, d_occ.p, d_used.p, d_counters.p, 1);
+    hipLaunchKernelGGL(k_del_big, dim3(256), dim3(1024), 0, stream, v, d_out.p, d_hot_out.p, kout, count,
+                       d_big_groups.p, d_counters.p + C_NBIG, d_used.p, d_counters.p, 1);
     fill<uint32_t>(stream, d_used.p, count, 0u);
-    fill<uint32_t>(stream, d_occ.p, count, 0u);
     fill<unsigned long long>(stream, d_counters.p + C_NBIG, 1, 0ull);
-    hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream, v, d_in.p, kin, count, d_gstart2.p,
+    hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream, v, d_in.p, d_hot_in.p, kin, count, d_gstart2.p,
                        d_counters.p + C_GROUPS_IN, d_used.p, d_counters.p, 0, d_big_groups.p,
                        d_counters.p + C_NBIG);
-    hipLaunchKernelGGL(k_del_big, dim3(256), dim3(1024), 0, stream, v, d_in.p, kin, count, d_big_groups.p,
-                       d_counters.p + C_NBIG, d_occ.p, d_used.p, d_counters.p, 0);
+    hipLaunchKernelGGL(k_del_big, dim3(256), dim3(1024), 0, stream, v, d_in.p, d_hot_in.p, kin, count,
+                       d_big_groups.p, d_counters.p + C_NBIG, d_used.p, d_counters.p, 0);
     time_end(slot);
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipMemcpyAsync(h_counters, d_counters.p, C_NUM * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
@@ -707,11 +766,19 @@ void Engine::mutate_add(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
     hipLaunchKernelGGL(k_alloc_classes, dim3(1), dim3(64), 0, stream, d_tail.p, total_slots);
     StoreView v = view();
     const uint32_t wg = wave_grid(count);
+    d_big_groups.ensure(count);
     const int slot = time_begin(DCG_K_APPEND_ROWS, -1);
-    hipLaunchKernelGGL(k_add_rows, dim3(wg), dim3(256), 0, stream, v, d_out.p, d_k32a.p, d_v32a.p, count, d_gstart.p,
-                       d_counters.p + C_GROUPS_OUT, d_tail.p, d_counters.p, 1);
-    hipLaunchKernelGGL(k_add_rows, dim3(wg), dim3(256), 0, stream, v, d_in.p, d_k32b.p, d_v32b.p, count, d_gstart2.p,
-                       d_counters.p + C_GROUPS_IN, d_tail.p, d_counters.p, 1);
+    hipLaunchKernelGGL(k_add_rows, dim3(wg), dim3(256), 0, stream, v, d_out.p, d_hot_out.p, d_k32a.p, d_v32a.p, count,
+                       d_gstart.p, d_counters.p + C_GROUPS_OUT, d_tail.p, d_counters.p, 1, d_big_groups.p,
+                       d_counters.p + C_NBIG);
+    hipLaunchKernelGGL(k_add_big, dim3(64), dim3(1024), 0, stream, v, d_out.p, d_hot_out.p, d_k32a.p, d_v32a.p, count,
+                       d_big_groups.p, d_counters.p + C_NBIG, d_tail.p, d_counters.p, 1);
+    fill<unsigned long long>(stream, d_counters.p + C_NBIG, 1, 0ull);
+    hipLaunchKernelGGL(k_add_rows, dim3(wg), dim3(256), 0, stream, v, d_in.p, d_hot_in.p, d_k32b.p, d_v32b.p, count,
+                       d_gstart2.p, d_counters.p + C_GROUPS_IN, d_tail.p, d_counters.p, 1, d_big_groups.p,
+                       d_counters.p + C_NBIG);
+    hipLaunchKernelGGL(k_add_big, dim3(64), dim3(1024), 0, stream, v, d_in.p, d_hot_in.p, d_k32b.p, d_v32b.p, count,
+                       d_big_groups.p, d_counters.p + C_NBIG, d_tail.p, d_counters.p, 1);
     time_end(slot);
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipMemcpyAsync(h_counters, d_counters.p, C_NUM * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));

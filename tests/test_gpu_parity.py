@@ -193,3 +193,32 @@ def test_autotune_keeps_values_and_picks_candidates():
     rp, col = store.csr()
     assert np.array_equal(eng.gather_value(), oracle.sssp(n, rp, col, 0, oracle.W_SEEDED, 3, 100))
     eng.close()
+
+
+@pytest.mark.parametrize("placement", [0, 1])
+def test_large_row_relocation_exact_rows(placement):
+    """A row above 4,096 entries that outgrows its block moves on the workgroup path."""
+    announce(f"large-row relocation placement={placement}")
+    n = 20000
+    hub = list(range(1, 8193))                       # exactly 8,192 entries: cap 8,192
+    src = np.array([0] * len(hub) + list(range(10, 8202)), np.uint32)
+    dst = np.array(hub + [5] * 8192, np.uint32)      # in-row of 5 holds 8,192 entries too
+    adds = [(0, 9000), (0, 9001), (0, 1), (9, 5), (8300, 5)]
+    dels = [(0, 77), (10, 5)]
+    eng = _engine(app="bfs", placement=placement, cache_bytes=1 << 18)
+    eng.load_graph(n, src, dst)
+    eng.start()
+    eng.load_updates(adds, dels)
+    store = oracle.Store(n, src, dst)
+    st = eng.add_edge(0, len(adds))
+    store.apply([], adds)
+    assert st["relocated_rows"] >= 1
+    erp, ecol = eng.csr()
+    assert same_rows(erp, ecol, store)
+    eng.del_edge(0, len(dels))
+    store.apply(dels, [])
+    erp, ecol = eng.csr()
+    assert same_rows(erp, ecol, store)
+    rp, col = store.csr()
+    assert np.array_equal(eng.gather_value(), oracle.bfs(n, rp, col, 0))
+    eng.close()
