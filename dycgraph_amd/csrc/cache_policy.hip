@@ -180,22 +180,49 @@ static void fill_common(Engine &e, dcg_cache_stats *st) {
     st->resident_chunks = r;
 }
 
-// Ranking used by the plain policy: score desc, last use desc, chunk asc.
-static std::vector<uint64_t> rank_top(const std::vector<float> &score, const std::vector<uint32_t> &last_use,
-                                      uint64_t k) {
-    std::vector<uint64_t> idx(score.size());
-    std::iota(idx.begin(), idx.end(), 0ull);
+// Admission plan shared by both policies.  Candidates are ranked by score
+// (ties: most recent use, then chunk id).  Loading a chunk moves 256 KiB over
+// the host link, the same as reading all of it zero-copy once, so a chunk is
+// admitted only if its score reaches `admit_min` edge reads, and it displaces a
+// resident chunk only if its score exceeds the resident's by `margin`.
+std::vector<uint64_t> Engine::plan_admission(const std::vector<float> &score, double admit_min, double margin) {
+    const uint64_t C = cache.chunks, F = cache.frames;
     auto better = [&](uint64_t a, uint64_t b) {
         if (score[a] != score[b]) return score[a] > score[b];
-        if (last_use[a] != last_use[b]) return last_use[a] > last_use[b];
+        if (cache.last_use[a] != cache.last_use[b]) return cache.last_use[a] > cache.last_use[b];
         return a < b;
     };
-    if (k < idx.size()) {
-        std::nth_element(idx.begin(), idx.begin() + (ptrdiff_t)k, idx.end(), better);
-        idx.resize(k);
+    // resident chunks, weakest first
+    std::vector<uint64_t> res;
+    for (uint64_t f = 0; f < F; ++f)
+        if (cache.frame_chunk[f] >= 0) res.push_back((uint64_t)cache.frame_chunk[f]);
+    std::sort(res.begin(), res.end(), [&](uint64_t a, uint64_t b) { return better(b, a); });
+    // candidates, strongest first
+    std::vector<uint64_t> cand;
+    for (uint64_t c = 0; c < C; ++c)
+        if (cache.chunk_frame[c] < 0 && score[c] >= admit_min) cand.push_back(c);
+    const uint64_t free_frames = F - res.size();
+    const uint64_t want = std::min<uint64_t>(cand.size(), free_frames + res.size());
+    if (want < cand.size()) {
+        std::nth_element(cand.begin(), cand.begin() + (ptrdiff_t)want, cand.end(), better);
+        cand.resize(want);
     }
-    std::sort(idx.begin(), idx.end(), better);
-    return idx;
+    std::sort(cand.begin(), cand.end(), better);
+    std::vector<uint8_t> drop(res.size(), 0);
+    std::vector<uint64_t> admit;
+    size_t ci = 0, ri = 0;
+    for (; ci < cand.size() && admit.size() < free_frames; ++ci) admit.push_back(cand[ci]);
+    for (; ci < cand.size() && ri < res.size(); ++ci, ++ri) {
+        if (!(score[cand[ci]] > margin * std::max(score[res[ri]], 1.0f))) break;
+        drop[ri] = 1;
+        admit.push_back(cand[ci]);
+    }
+    std::vector<uint64_t> keep;
+    keep.reserve(F);
+    for (size_t i = 0; i < res.size(); ++i)
+        if (!drop[i]) keep.push_back(res[i]);
+    for (uint64_t c : admit) keep.push_back(c);
+    return keep;
 }
 
 void Engine::cache_maintain(dcg_cache_stats *st) {
@@ -224,24 +251,8 @@ void Engine::cache_maintain(dcg_cache_stats *st) {
     DCG_HIP(hipMemcpyAsync(cache.d_rows_last.p, cache.d_reads.p + C, C * 4, hipMemcpyDeviceToDevice, stream));
     // the learned policy admits at dcg_cache_prefetch, before the next batch
     if (cfg.cache_policy == DCG_CACHE_HOTNESS) {
-        std::vector<uint64_t> desired = rank_top(score, cache.last_use, cache.frames);
-        // chunks never read keep their frames rather than displacing nothing
-        std::vector<uint64_t> keep;
-        keep.reserve(desired.size());
-        for (uint64_t c : desired)
-            if (score[c] > 0) keep.push_back(c);
-        if (keep.size() < cache.frames) {
-            std::vector<uint8_t> in(C, 0);
-            for (uint64_t c : keep) in[c] = 1;
-            for (uint64_t f = 0; f < cache.frames && keep.size() < cache.frames; ++f) {
-                const int64_t c = cache.frame_chunk[f];
-                if (c >= 0 && !in[c]) {
-                    keep.push_back((uint64_t)c);
-                    in[c] = 1;
-                }
-            }
-        }
-        cache_apply_plan(keep, st);
+        // window of 4 snapshots: admit chunks read more than once per snapshot on average
+        cache_apply_plan(plan_admission(score, 4.0 * (double)kChunkEdges, 1.5), st);
     }
     fill<float>(stream, cache.d_reads.p, 2 * C, 0.f);
     fill_common(*this, st);

@@ -86,13 +86,14 @@ struct TileLds {
     float b1[kHid], b2[kHid], w3[kHid * kOut], b3[kOut];
 };
 
-// Forward (and, with y != null, backward) of one 32-row tile per workgroup
-// (one wave).  Gradients are summed into grad[] (mean over `rows`).
+// Forward (and, with Y != null, backward) over 32-row tiles; one wave per
+// workgroup walks tiles blockIdx.x, +gridDim.x, ...  Gradient contributions stay
+// in registers across the workgroup's tiles (the dW2 tiles in the MFMA
+// accumulators) and are added to grad[] once per workgroup (mean over `rows`).
 __global__ __launch_bounds__(64) void k_mlp_tile(const float *X, const float *Y, uint64_t rows, const _Float16 *w16,
                                                  const float *master, float *out, float *grad) {
     __shared__ TileLds L;
     const int l = (int)lane_id();
-    const uint64_t r0 = (uint64_t)blockIdx.x * kTile;
     // stage parameters: W2 in FP16, the small layers in FP32 from the FP16 copy
     for (int t = l; t < kW2; t += 64) L.w2[t / kHid][t % kHid] = w16[P_W2 + t];
     for (int t = l; t < kW1; t += 64) L.w1[t] = (float)w16[P_W1 + t];
@@ -100,111 +101,114 @@ __global__ __launch_bounds__(64) void k_mlp_tile(const float *X, const float *Y,
     L.b1[l] = master[P_B1 + l];
     L.b2[l] = master[P_B2 + l];
     if (l < kOut) L.b3[l] = master[P_B3 + l];
-    for (int t = l; t < kTile * kIn; t += 64) {
-        const uint64_t r = r0 + t / kIn;
-        L.x[t / kIn][t % kIn] = r < rows ? X[r * kIn + t % kIn] : 0.f;
-    }
-    __syncthreads();
-    // layer 1 (FP32 FMA): lane owns hidden unit l for all 32 rows
-    for (int i = 0; i < kTile; ++i) {
-        float a = L.b1[l];
-#pragma unroll
-        for (int k = 0; k < kIn; ++k) a += L.x[i][k] * L.w1[k * kHid + l];
-        L.h1[i][l] = (_Float16)(a > 0.f ? a : 0.f);
-    }
-    __syncthreads();
-    // layer 2 on MFMA: Z2 = H1 * W2 (two 32x32 output tiles)
-    for (int nt = 0; nt < 2; ++nt) {
-        float16_t acc = {};
-        acc = mfma_tile([&](int i, int k) { return L.h1[i][k]; },
-                        [&](int k, int j) { return L.w2[k][nt * 32 + j]; }, kHid, acc);
-#pragma unroll
-        for (int r = 0; r < 16; ++r) L.z2[acc_row(r)][nt * 32 + (l & 31)] = acc[r] + L.b2[nt * 32 + (l & 31)];
-    }
-    __syncthreads();
-    // layer 3 (FP32): lanes 0..63 -> (row = l/2, output = l%2)
-    {
-        const int i = l >> 1, o = l & 1;
-        float a = L.b3[o];
-        for (int k = 0; k < kHid; ++k) {
-            const float h = L.z2[i][k] > 0.f ? L.z2[i][k] : 0.f;
-            a += h * L.w3[k * kOut + o];
+    // gradient accumulators (per lane): dW2 as four 32x32 MFMA tiles, lane-owned rest
+    float16_t gw2[2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) gw2[a][b] = float16_t{};
+    float gw1[kIn] = {0, 0, 0, 0, 0, 0, 0, 0}, gb1 = 0.f, gb2 = 0.f, gw3a = 0.f, gw3b = 0.f, gb3 = 0.f;
+    const uint64_t tiles = (rows + kTile - 1) / kTile;
+    for (uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const uint64_t r0 = tile * kTile;
+        __syncthreads();
+        for (int t = l; t < kTile * kIn; t += 64) {
+            const uint64_t r = r0 + t / kIn;
+            L.x[t / kIn][t % kIn] = r < rows ? X[r * kIn + t % kIn] : 0.f;
         }
-        const uint64_t r = r0 + i;
-        if (r < rows && out) out[r * kOut + o] = a;
-        if (Y) {
-            float g = 0.f;
-            if (r < rows) {
-                const float t = Y[r * kOut + o];
-                // unscaled per-row gradient (keeps the FP16 MFMA operands
-                // in range); the 1/rows of the mean is applied at the sums
-                g = o == 0 ? (1.f / (1.f + __expf(-a)) - t) : (a - t);
+        __syncthreads();
+        // layer 1 (FP32 FMA): lane owns hidden unit l for all 32 rows
+        for (int i = 0; i < kTile; ++i) {
+            float a = L.b1[l];
+#pragma unroll
+            for (int k = 0; k < kIn; ++k) a += L.x[i][k] * L.w1[k * kHid + l];
+            L.h1[i][l] = (_Float16)(a > 0.f ? a : 0.f);
+        }
+        __syncthreads();
+        // layer 2 on MFMA: Z2 = H1 * W2 (two 32x32 output tiles)
+        for (int nt = 0; nt < 2; ++nt) {
+            float16_t acc = {};
+            acc = mfma_tile([&](int i, int k) { return L.h1[i][k]; },
+                            [&](int k, int j) { return L.w2[k][nt * 32 + j]; }, kHid, acc);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) L.z2[acc_row(r)][nt * 32 + (l & 31)] = acc[r] + L.b2[nt * 32 + (l & 31)];
+        }
+        __syncthreads();
+        // layer 3 (FP32): lanes 0..63 -> (row = l/2, output = l%2)
+        {
+            const int i = l >> 1, o = l & 1;
+            float a = L.b3[o];
+            for (int k = 0; k < kHid; ++k) {
+                const float h = L.z2[i][k] > 0.f ? L.z2[i][k] : 0.f;
+                a += h * L.w3[k * kOut + o];
             }
-            L.dout[i][o] = g;
+            const uint64_t r = r0 + i;
+            if (r < rows && out) out[r * kOut + o] = a;
+            if (Y) {
+                // unscaled per-row gradient (keeps the FP16 MFMA operands in
+                // range); the 1/rows of the mean is applied at the flush
+                float g = 0.f;
+                if (r < rows) {
+                    const float t = Y[r * kOut + o];
+                    g = o == 0 ? (1.f / (1.f + __expf(-a)) - t) : (a - t);
+                }
+                L.dout[i][o] = g;
+            }
         }
-    }
-    if (!Y) return;
-    const float inv = 1.f / (float)rows;
-    __syncthreads();
-    // dW3, db3, dZ2 = (dout * W3^T) .* relu'(Z2)
-    {
-        // lane owns hidden unit l: dW3[l][o] = sum_i relu(z2[i][l]) * dout[i][o]
-        float g0 = 0.f, g1 = 0.f;
+        if (!Y) continue;
+        __syncthreads();
+        // dW3, db3, dZ2 = (dout * W3^T) .* relu'(Z2)
         for (int i = 0; i < kTile; ++i) {
             const float h = L.z2[i][l] > 0.f ? L.z2[i][l] : 0.f;
-            g0 += h * L.dout[i][0];
-            g1 += h * L.dout[i][1];
+            gw3a += h * L.dout[i][0];
+            gw3b += h * L.dout[i][1];
             const float dh = L.dout[i][0] * L.w3[l * kOut + 0] + L.dout[i][1] * L.w3[l * kOut + 1];
             L.dz2[i][l] = (_Float16)(L.z2[i][l] > 0.f ? dh : 0.f);
         }
-        atomicAdd(&grad[P_W3 + l * kOut + 0], g0 * inv);
-        atomicAdd(&grad[P_W3 + l * kOut + 1], g1 * inv);
-        if (l < kOut) {
-            float b = 0.f;
-            for (int i = 0; i < kTile; ++i) b += L.dout[i][l];
-            atomicAdd(&grad[P_B3 + l], b * inv);
-        }
-    }
-    __syncthreads();
-    // db2
-    {
-        float b = 0.f;
-        for (int i = 0; i < kTile; ++i) b += (float)L.dz2[i][l];
-        atomicAdd(&grad[P_B2 + l], b * inv);
-    }
-    // dW2 = H1^T * dZ2 on MFMA (M=64 in two tiles, N=64 in two tiles, K=32 rows)
-    for (int mt = 0; mt < 2; ++mt)
+        if (l < kOut)
+            for (int i = 0; i < kTile; ++i) gb3 += L.dout[i][l];
+        __syncthreads();
+        for (int i = 0; i < kTile; ++i) gb2 += (float)L.dz2[i][l];
+        // dW2 += H1^T * dZ2 on MFMA (M=64 in two tiles, N=64 in two tiles, K=32 rows)
+        for (int mt = 0; mt < 2; ++mt)
+            for (int nt = 0; nt < 2; ++nt)
+                gw2[mt][nt] = mfma_tile([&](int i, int k) { return L.h1[k][mt * 32 + i]; },
+                                        [&](int k, int j) { return L.dz2[k][nt * 32 + j]; }, kTile, gw2[mt][nt]);
+        // dH1 = dZ2 * W2^T on MFMA; reuse z2 as dH1 storage
+        __syncthreads();
         for (int nt = 0; nt < 2; ++nt) {
             float16_t acc = {};
-            acc = mfma_tile([&](int i, int k) { return L.h1[k][mt * 32 + i]; },
-                            [&](int k, int j) { return L.dz2[k][nt * 32 + j]; }, kTile, acc);
+            acc = mfma_tile([&](int i, int k) { return L.dz2[i][k]; },
+                            [&](int k, int j) { return L.w2[nt * 32 + j][k]; }, kHid, acc);
 #pragma unroll
-            for (int r = 0; r < 16; ++r)
-                atomicAdd(&grad[P_W2 + (mt * 32 + acc_row(r)) * kHid + nt * 32 + (l & 31)], acc[r] * inv);
+            for (int r = 0; r < 16; ++r) L.z2[acc_row(r)][nt * 32 + (l & 31)] = acc[r];
         }
-    // dH1 = dZ2 * W2^T on MFMA, then dZ1 = dH1 .* relu'(Z1); reuse z2 as dH1 storage
-    __syncthreads();
-    for (int nt = 0; nt < 2; ++nt) {
-        float16_t acc = {};
-        acc = mfma_tile([&](int i, int k) { return L.dz2[i][k]; },
-                        [&](int k, int j) { return L.w2[nt * 32 + j][k]; }, kHid, acc);
-#pragma unroll
-        for (int r = 0; r < 16; ++r) L.z2[acc_row(r)][nt * 32 + (l & 31)] = acc[r];
-    }
-    __syncthreads();
-    {
-        // lane owns hidden unit l of layer 1
-        float gb = 0.f, gw[kIn] = {0, 0, 0, 0, 0, 0, 0, 0};
+        __syncthreads();
+        // dZ1 = dH1 .* relu'(Z1); lane owns hidden unit l of layer 1
         for (int i = 0; i < kTile; ++i) {
             const float dz = (float)L.h1[i][l] > 0.f ? L.z2[i][l] : 0.f;
-            gb += dz;
+            gb1 += dz;
 #pragma unroll
-            for (int k = 0; k < kIn; ++k) gw[k] += L.x[i][k] * dz;
+            for (int k = 0; k < kIn; ++k) gw1[k] += L.x[i][k] * dz;
         }
-        atomicAdd(&grad[P_B1 + l], gb * inv);
-#pragma unroll
-        for (int k = 0; k < kIn; ++k) atomicAdd(&grad[P_W1 + k * kHid + l], gw[k] * inv);
     }
+    if (!Y || blockIdx.x >= tiles) return;
+    const float inv = 1.f / (float)rows;
+    atomicAdd(&grad[P_W3 + l * kOut + 0], gw3a * inv);
+    atomicAdd(&grad[P_W3 + l * kOut + 1], gw3b * inv);
+    if (l < kOut) atomicAdd(&grad[P_B3 + l], gb3 * inv);
+    atomicAdd(&grad[P_B2 + l], gb2 * inv);
+    for (int mt = 0; mt < 2; ++mt)
+        for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                atomicAdd(&grad[P_W2 + (mt * 32 + acc_row(r)) * kHid + nt * 32 + (l & 31)], gw2[mt][nt][r] * inv);
+    atomicAdd(&grad[P_B1 + l], gb1 * inv);
+#pragma unroll
+    for (int k = 0; k < kIn; ++k) atomicAdd(&grad[P_W1 + k * kHid + l], gw1[k] * inv);
+}
+
+static uint32_t mlp_grid(uint64_t rows) {
+    const uint64_t tiles = (rows + kTile - 1) / kTile;
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(tiles, 512));
 }
 
 __global__ void k_adam(float *master, const float *grad, float *m1, float *m2, _Float16 *w16, int n, float lr,
@@ -279,9 +283,8 @@ static void mlp_train_steps(Engine &e, MlpState &m, int steps) {
     const uint64_t rows = m.rows * (uint64_t)m.filled;   // every labelled decision in the ring
     for (int s = 0; s < steps; ++s) {
         fill<float>(e.stream, m.grad.p, P_NUM, 0.f);
-        const uint32_t tiles = (uint32_t)((rows + kTile - 1) / kTile);
-        hipLaunchKernelGGL(k_mlp_tile, dim3(tiles), dim3(64), 0, e.stream, m.X.p, m.Y.p, rows, m.w16.p, m.master.p,
-                           (float *)nullptr, m.grad.p);
+        hipLaunchKernelGGL(k_mlp_tile, dim3(mlp_grid(rows)), dim3(64), 0, e.stream, m.X.p, m.Y.p, rows, m.w16.p,
+                           m.master.p, (float *)nullptr, m.grad.p);
         m.steps++;
         const float b1c = 1.f - std::pow(0.9f, (float)m.steps), b2c = 1.f - std::pow(0.999f, (float)m.steps);
         hipLaunchKernelGGL(k_adam, dim3(grid_for(P_NUM)), dim3(256), 0, e.stream, m.master.p, m.grad.p, m.m1.p,
@@ -355,19 +358,14 @@ void mlp_observe(Engine &e, const std::vector<float> &reads) {
     m.hot_capture = capture(m.hot_score_at_decision, reads, e.cache.frames);
     m.filled = std::min(kReplay, m.filled + 1);
     m.slot = (m.slot + 1) % kReplay;
-    mlp_train_steps(e, m, 16);
-    m.fallback = m.steps < 32 || m.mlp_capture < m.hot_capture;
+    mlp_train_steps(e, m, 8);
+    m.fallback = m.steps < 16 || m.mlp_capture < m.hot_capture;
     m.have_decision = false;
 }
 
 std::vector<uint64_t> mlp_plan(Engine &e, uint64_t ab, uint64_t ac, uint64_t db, uint64_t dc) {
     MlpState &m = *e.cache.mlp;
     const uint64_t C = e.cache.chunks;
-    if ((e.snapshots & 7) == 0) {
-        fill<float>(e.stream, m.degsum.p, C, 0.f);
-        hipLaunchKernelGGL(k_degsum, dim3(grid_for(2ull * e.n, 256, 8192)), dim3(256), 0, e.stream, e.d_out.p,
-                           e.d_in.p, e.n, m.degsum.p);
-    }
     fill<float>(e.stream, e.cache.d_touch.p, C, 0.f);
     hipLaunchKernelGGL(k_touch, dim3(grid_for(ac + dc, 256, 8192)), dim3(256), 0, e.stream, e.d_out.p, e.d_in.p,
                        e.d_adds.p + ab, ac, e.d_dels.p + db, dc, e.n, e.cache.d_touch.p);
@@ -377,8 +375,7 @@ std::vector<uint64_t> mlp_plan(Engine &e, uint64_t ab, uint64_t ac, uint64_t db,
     float *X = m.X.p + (uint64_t)m.slot * C * kIn;
     hipLaunchKernelGGL(k_features, dim3(grid_for(C)), dim3(256), 0, e.stream, e.cache.d_hist.p,
                        e.cache.d_rows_last.p, m.degsum.p, e.cache.d_touch.p, m.lastuse.p, (float)e.snapshots, C, X);
-    const uint32_t tiles = (uint32_t)((C + kTile - 1) / kTile);
-    hipLaunchKernelGGL(k_mlp_tile, dim3(tiles), dim3(64), 0, e.stream, X, (const float *)nullptr, C, m.w16.p,
+    hipLaunchKernelGGL(k_mlp_tile, dim3(mlp_grid(C)), dim3(64), 0, e.stream, X, (const float *)nullptr, C, m.w16.p,
                        m.master.p, m.out.p, (float *)nullptr);
     DCG_HIP(hipGetLastError());
     std::vector<float> out(C * kOut), hist(4 * C);
@@ -392,21 +389,13 @@ std::vector<uint64_t> mlp_plan(Engine &e, uint64_t ab, uint64_t ac, uint64_t db,
         const float p = 1.f / (1.f + std::exp(-out[2 * c]));
         score[c] = p * std::max(out[2 * c + 1], 0.f);
     }
-    const std::vector<float> &use = m.fallback ? m.hot_score_at_decision : score;
     m.decisions++;
     m.fallbacks += m.fallback ? 1 : 0;
     m.have_decision = true;
-    std::vector<uint64_t> idx(C);
-    std::iota(idx.begin(), idx.end(), 0ull);
-    const uint64_t k = std::min<uint64_t>(e.cache.frames, C);
-    auto better = [&](uint64_t a, uint64_t b) {
-        if (use[a] != use[b]) return use[a] > use[b];
-        if (e.cache.last_use[a] != e.cache.last_use[b]) return e.cache.last_use[a] > e.cache.last_use[b];
-        return a < b;
-    };
-    if (k < C) std::nth_element(idx.begin(), idx.begin() + (ptrdiff_t)k, idx.end(), better);
-    idx.resize(k);
-    return idx;
+    if (m.fallback) return e.plan_admission(m.hot_score_at_decision, 4.0 * (double)kChunkEdges, 1.5);
+    // the network predicts one snapshot's reads: admit above one chunk's worth
+    for (auto &x : score) x = std::expm1(std::min(x, 30.f));
+    return e.plan_admission(score, (double)kChunkEdges, 1.5);
 }
 
 }  // namespace dcg
@@ -438,8 +427,7 @@ extern "C" int dcg_mlp_eval(int32_t device, uint32_t rows, const float *x, const
         }
         hipLaunchKernelGGL(k_to_half, dim3(grid_for(P_NUM)), dim3(256), 0, s, dp.p, w16.p, P_NUM);
         fill<float>(s, dg.p, P_NUM, 0.f);
-        const uint32_t tiles = (rows + kTile - 1) / kTile;
-        hipLaunchKernelGGL(k_mlp_tile, dim3(tiles), dim3(64), 0, s, dx.p, y ? dy.p : (const float *)nullptr,
+        hipLaunchKernelGGL(k_mlp_tile, dim3(mlp_grid(rows)), dim3(64), 0, s, dx.p, y ? dy.p : (const float *)nullptr,
                            (uint64_t)rows, w16.p, dp.p, dout.p, y ? dg.p : (float *)nullptr);
         DCG_HIP(hipGetLastError());
         DCG_HIP(hipMemcpyAsync(out, dout.p, (uint64_t)rows * kOut * 4, hipMemcpyDeviceToHost, s));

@@ -540,8 +540,14 @@ __global__ __launch_bounds__(1024) void k_del_big(StoreView s, RowDesc *desc, ui
     }
 }
 
-__global__ void k_add_plan(const RowDesc *desc, const uint32_t *keys, uint64_t count, const uint32_t *gstart,
-                           const unsigned long long *ngroups_p, uint64_t *tail) {
+// Relocation demand per capacity class: a workgroup histogram in LDS, then one
+// global atomic per class per workgroup.
+__global__ __launch_bounds__(256) void k_add_plan(const RowDesc *desc, const uint32_t *keys, uint64_t count,
+                                                  const uint32_t *gstart, const unsigned long long *ngroups_p,
+                                                  uint64_t *tail) {
+    __shared__ uint32_t hist[kClasses];
+    for (uint32_t i = threadIdx.x; i < kClasses; i += blockDim.x) hist[i] = 0;
+    __syncthreads();
     const uint64_t ngroups = *ngroups_p;
     for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < ngroups;
          g += (uint64_t)gridDim.x * blockDim.x) {
@@ -553,9 +559,12 @@ __global__ void k_add_plan(const RowDesc *desc, const uint32_t *keys, uint64_t c
         if (nd > d.cap) {
             uint64_t c = nd < kMinRowCap ? kMinRowCap : nd;
             const uint32_t cls = 64u - (uint32_t)__clzll((long long)(c - 1));
-            atomicAdd((unsigned long long *)&tail[T_COUNT + cls], 1ull);
+            atomicAdd(&hist[cls], 1u);
         }
     }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kClasses; i += blockDim.x)
+        if (hist[i]) atomicAdd((unsigned long long *)&tail[T_COUNT + i], (unsigned long long)hist[i]);
 }
 
 __global__ void k_alloc_classes(uint64_t *tail, uint64_t total) {
@@ -759,10 +768,10 @@ void Engine::mutate_add(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
                        d_gstart.p, d_counters.p + C_GROUPS_OUT);
     hipLaunchKernelGGL(k_group_heads<uint32_t>, dim3(grid_for(count)), dim3(256), 0, stream, d_k32b.p, count,
                        d_gstart2.p, d_counters.p + C_GROUPS_IN);
-    hipLaunchKernelGGL(k_add_plan, dim3(grid_for(count)), dim3(256), 0, stream, d_out.p, d_k32a.p, count, d_gstart.p,
-                       d_counters.p + C_GROUPS_OUT, d_tail.p);
-    hipLaunchKernelGGL(k_add_plan, dim3(grid_for(count)), dim3(256), 0, stream, d_in.p, d_k32b.p, count, d_gstart2.p,
-                       d_counters.p + C_GROUPS_IN, d_tail.p);
+    hipLaunchKernelGGL(k_add_plan, dim3(grid_for(count, 256, 1024)), dim3(256), 0, stream, d_out.p, d_k32a.p, count,
+                       d_gstart.p, d_counters.p + C_GROUPS_OUT, d_tail.p);
+    hipLaunchKernelGGL(k_add_plan, dim3(grid_for(count, 256, 1024)), dim3(256), 0, stream, d_in.p, d_k32b.p, count,
+                       d_gstart2.p, d_counters.p + C_GROUPS_IN, d_tail.p);
     hipLaunchKernelGGL(k_alloc_classes, dim3(1), dim3(64), 0, stream, d_tail.p, total_slots);
     StoreView v = view();
     const uint32_t wg = wave_grid(count);
