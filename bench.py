@@ -9,9 +9,13 @@ Workloads (seeded, generated in-process):
   c1  RMAT-16 (65,536 v, 1,048,576 e), SSSP, weights 1+mix64 % 100, source 0,
       1,000 deletes + 1,000 inserts per snapshot, topology in HBM.
   c2  RMAT-22 (4,194,304 v, 67,108,864 e), PageRank (probability scale), 50k+50k per snapshot.
-  c3  RMAT-26, SSSP, 4e9-byte device chunk cache over pinned host memory, hotness/LRU policy,
+  c3  RMAT-26, SSSP + BFS + CC batched in one engine (sources drawn from the edge list with a
+      fixed seed), 4e9-byte device chunk cache over pinned host memory, hotness/LRU policy,
       500k+500k per snapshot.
-  c4  c3 with the learned cache policy (c3 measured beside it for the comparison).
+  c4  c3 with the learned cache policy.  Measured beside it on the same graph and stream:
+      the batched engine with hotness/LRU, and the plain form (one single-query engine per
+      program, hotness/LRU, run one after the other; its snapshot time is the sum).
+  c5  4096x4096 grid, 32 SSSP + 32 BFS + CC in one engine, lognormal batch sizes.
 """
 from __future__ import annotations
 
@@ -34,12 +38,12 @@ CONFIGS = {
                app="sssp", adds=1000, dels=1000, placement=0, policy=1, cache_bytes=0, snapshots=10),
     "c2": dict(workload="RMAT-22 incremental PageRank (device-resident topology)", scale=22, edges=1 << 26,
                app="pr", adds=50000, dels=50000, placement=0, policy=1, cache_bytes=0, snapshots=20),
-    "c3": dict(workload="RMAT-26 incremental SSSP, 4e9-byte chunk cache, hotness/LRU", scale=26,
-               edges=1 << 30, app="sssp", adds=500000, dels=500000, placement=1, policy=1,
-               cache_bytes=4_000_000_000, snapshots=10),
-    "c4": dict(workload="RMAT-26 incremental SSSP, 4e9-byte chunk cache, learned policy", scale=26,
-               edges=1 << 30, app="sssp", adds=500000, dels=500000, placement=1, policy=2,
-               cache_bytes=4_000_000_000, snapshots=10),
+    "c3": dict(workload="RMAT-26 incremental SSSP+BFS+CC batched, 4e9-byte chunk cache, hotness/LRU", scale=26,
+               edges=1 << 30, app="sssp", programs=("sssp", "bfs", "cc"), adds=500000, dels=500000, placement=1,
+               policy=1, cache_bytes=4_000_000_000, snapshots=10),
+    "c4": dict(workload="RMAT-26 incremental SSSP+BFS+CC batched, 4e9-byte chunk cache, learned policy",
+               scale=26, edges=1 << 30, app="sssp", programs=("sssp", "bfs", "cc"), adds=500000, dels=500000,
+               placement=1, policy=2, cache_bytes=4_000_000_000, snapshots=10),
     "c5": dict(workload="4096x4096 grid + 5% diagonals, 32 SSSP + 32 BFS sources + CC in one engine, "
                         "lognormal(8,1.5) batches", grid=4096, app="sssp", placement=0, policy=1, cache_bytes=0,
                snapshots=30, queries=64),
@@ -75,26 +79,37 @@ def make_inputs(cfg, snapshots, seed=1):
     return n, src, dst, adds, dels, ap, dp
 
 
-def queries_of(cfg, n, seed=1):
-    """(sources, programs): one SSSP query from vertex 0, or for the grid 32 SSSP +
-    32 BFS seeded sources plus one CC query."""
+def queries_of(cfg, n, src=None, seed=1):
+    """(sources, programs): one SSSP query from vertex 0; for `programs` one query per
+    program whose source is the tail of a seeded random edge (so it has out-edges; CC
+    ignores its source); for the grid 32 SSSP + 32 BFS seeded sources plus one CC query."""
+    if "programs" in cfg:
+        rng = np.random.default_rng(seed + 23)
+        picks = [int(src[i]) for i in rng.integers(0, len(src), len(cfg["programs"]))]
+        return tuple(0 if p == "cc" else v for p, v in zip(cfg["programs"], picks)), tuple(cfg["programs"])
     if "queries" not in cfg:
         return (0,), ()
-    import dycgraph_amd as d
     rng = np.random.default_rng(seed + 17)
     src = [int(x) for x in rng.integers(0, n, cfg["queries"])]
     half = cfg["queries"] // 2
     return tuple(src + [0]), tuple(["sssp"] * half + ["bfs"] * (cfg["queries"] - half) + ["cc"])
 
 
-def run_gpu(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, policy):
+def engine_config(cfg, n, policy, sources, programs, placement=None):
+    import dycgraph_amd as d
+    wmax = 1000 if "grid" in cfg else 100
+    app = cfg["app"] if not programs or len(programs) > 1 else programs[0]
+    return d.EngineConfig(app=app, sources=sources, query_apps=programs if len(programs) > 1 else (),
+                          weight_mode=d.W_SEEDED, weight_max=wmax, weight_seed=11,
+                          placement=cfg["placement"] if placement is None else placement, cache_policy=policy,
+                          cache_bytes=cfg["cache_bytes"], pr_base=0.15 / n, pr_epsilon=1e-7 / n, pr_max_rounds=100)
+
+
+def run_gpu(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, policy, queries=None):
     import dycgraph_amd as d
     pr = cfg["app"] == "pr"
-    sources, programs = queries_of(cfg, n)
-    wmax = 1000 if "grid" in cfg else 100
-    ec = d.EngineConfig(app=cfg["app"], sources=sources, query_apps=programs, weight_mode=d.W_SEEDED,
-                        weight_max=wmax, weight_seed=11, placement=cfg["placement"], cache_policy=policy,
-                        cache_bytes=cfg["cache_bytes"], pr_base=0.15 / n, pr_epsilon=1e-7 / n, pr_max_rounds=100)
+    sources, programs = queries if queries is not None else queries_of(cfg, n, src)
+    ec = engine_config(cfg, n, policy, sources, programs)
     eng = d.Engine(ec)
     t0 = time.time()
     eng.load_graph(n, src, dst)
@@ -134,47 +149,88 @@ def run_gpu(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, policy):
     total = time.perf_counter() - t_all
     ks = eng.kernel_stats()
     info = eng.info()
-    values = eng.gather_value()
-    return dict(total_s=total, lat=lat, ks=ks, info=info, hits=hits, host=host, eng=eng, values=values)
+    values = [eng.gather_value(q) for q in range(eng.queries)]
+    return dict(total_s=total, lat=lat, ks=ks, info=info, hits=hits, host=host, eng=eng, values=values,
+                queries=(sources, programs))
+
+
+def run_plain(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, queries):
+    """The plain GPU form: one single-query engine per program, hotness/LRU cache, each
+    applying the whole stream; engines run one after the other and a snapshot of the
+    plain form costs the sum of their snapshot times."""
+    import dycgraph_amd as d
+    total, lat, hits, host, values = 0.0, np.zeros(steps), 0, 0, []
+    for s_q, p_q in zip(*queries):
+        r = run_gpu(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, d.CACHE_HOTNESS, ((s_q,), (p_q,)))
+        r["eng"].close()
+        total += r["total_s"]
+        lat += np.array(r["lat"])
+        hits += r["hits"]
+        host += r["host"]
+        values += r["values"]
+        log(f"plain {p_q} from {s_q}: {r['total_s'] * 1e3:.1f} ms")
+    return dict(total_s=total, lat=list(lat), hits=hits, host=host, values=values)
+
+
+def verify_from_scratch(cfg, n, eng, queries, values):
+    """Exactness at full size: rebuild an engine from the incremental engine's final
+    topology (device placement), converge it from scratch and compare every query's
+    values bit for bit."""
+    import dycgraph_amd as d
+    rp, col = eng.csr()
+    srcs = np.repeat(np.arange(n, dtype=np.uint32), np.diff(rp).astype(np.int64))
+    ec = engine_config(cfg, n, d.CACHE_HOTNESS, queries[0], queries[1], placement=d.PLACE_DEVICE)
+    with d.Engine(ec) as fresh:
+        fresh.load_graph(n, srcs, col)
+        fresh.start()
+        fresh.sync()
+        return all(np.array_equal(fresh.gather_value(q), values[q]) for q in range(fresh.queries))
 
 
 def cpu_baseline(cfg, n, src, dst, adds, dels, warmup, budget_s=20.0, note=""):
     """The CPU port of the reference's incremental path (oracle/), 1 thread, on the
-    same stream: snapshots after the warmup ones until ~budget_s of CPU work."""
+    same stream: one incremental state per query (for several programs, each applies
+    the snapshot in turn and the snapshot costs the sum), snapshots after the warmup
+    ones until ~budget_s of CPU work."""
     import oracle
     rp, col = oracle.csr_from_edges(n, src, dst)
-    app = {"sssp": oracle.APP_SSSP, "bfs": oracle.APP_BFS, "cc": oracle.APP_CC, "pr": oracle.APP_PR}[cfg["app"]]
-    inc = oracle.Incr(n, rp, col, app, source=0, wmode=oracle.W_SEEDED, seed=11, wmax=100, pr_base=0.15 / n,
-                      pr_eps=1e-7 / n, pr_max_rounds=100)
-    inc.converge()
+    codes = {"sssp": oracle.APP_SSSP, "bfs": oracle.APP_BFS, "cc": oracle.APP_CC, "pr": oracle.APP_PR}
+    sources, programs = queries_of(cfg, n, src)
+    programs = programs or (cfg["app"],) * len(sources)
+    states = [oracle.Incr(n, rp, col, codes[p], source=s, wmode=oracle.W_SEEDED, seed=11, wmax=100,
+                          pr_base=0.15 / n, pr_eps=1e-7 / n, pr_max_rounds=100) for s, p in zip(sources, programs)]
+    for inc in states:
+        inc.converge()
     na, nd = cfg["adds"], cfg["dels"]
-    for s in range(warmup):
-        if app == oracle.APP_PR:
+
+    def snapshot(inc, s):
+        if inc.app == oracle.APP_PR:
             inc.pr_batch(dels[s * nd:(s + 1) * nd], adds[s * na:(s + 1) * na])
         else:
             inc.delete_phase(dels[s * nd:(s + 1) * nd])
             inc.add_phase(adds[s * na:(s + 1) * na])
+
+    for s in range(warmup):
+        for inc in states:
+            snapshot(inc, s)
     done, spent = 0, 0.0
     s = warmup
     while spent < budget_s and (s + 1) * max(na, nd) <= len(adds):
         t = time.perf_counter()
-        if app == oracle.APP_PR:
-            inc.pr_batch(dels[s * nd:(s + 1) * nd], adds[s * na:(s + 1) * na])
-        else:
-            inc.delete_phase(dels[s * nd:(s + 1) * nd])
-            inc.add_phase(adds[s * na:(s + 1) * na])
+        for inc in states:
+            snapshot(inc, s)
         spent += time.perf_counter() - t
         done += 1
         s += 1
     return dict(value=done * (na + nd) / spent if spent else 0.0, unit="updates/s", cores=1, kind="port",
                 sample=f"{note}{done} snapshots after {warmup} warm-up snapshots, single-threaded oracle port "
-                       f"(delete stage + add stage), {spent:.2f}s", inc=inc)
+                       f"(delete stage + add stage) of {'+'.join(programs)}, {spent:.2f}s")
 
 
 # The CPU port needs minutes just to build its rows and converge on 2^30 edges,
 # so the RMAT-26 workloads are sampled on RMAT-22 with the same update/edge ratio.
-CPU_SAMPLE = {"c3": dict(scale=22, edges=1 << 26, app="sssp", adds=31250, dels=31250),
-              "c4": dict(scale=22, edges=1 << 26, app="sssp", adds=31250, dels=31250)}
+CPU_SAMPLE = {c: dict(scale=22, edges=1 << 26, app="sssp", programs=("sssp", "bfs", "cc"), adds=31250, dels=31250)
+              for c in ("c3", "c4")}
 
 
 def main():
@@ -231,23 +287,51 @@ def main():
                            "GBps": round(v["algo_bytes"] / (v["ms"] / 1e3) / 1e9, 1) if v["ms"] else 0.0}
                     for name, v in ks.items() if v["launches"]},
     }
+    queries = res["queries"]
+    if "programs" in cfg or "queries" in cfg:
+        out["config"]["queries"] = [f"{p}@{s}" for s, p in zip(*queries)]
+        out["config"]["app"] = "+".join(queries[1])
+    if args.check or "programs" in cfg:
+        if cfg["app"] == "pr":
+            pass
+        elif "programs" in cfg or "queries" in cfg:
+            t = time.time()
+            out["exact"] = verify_from_scratch(cfg, n, res["eng"], queries, res["values"])
+            out["exact_check"] = "final values of every query == from-scratch GPU convergence on the final topology"
+            log(f"from-scratch check: {out['exact']} ({time.time() - t:.1f}s)")
+        else:
+            import oracle
+            store = oracle.Store(n, src, dst)
+            aoff = np.concatenate([[0], np.cumsum(ap)]).astype(np.int64)
+            doff = np.concatenate([[0], np.cumsum(dp)]).astype(np.int64)
+            for s in range(snapshots):
+                store.apply(dels[doff[s]:doff[s + 1]], adds[aoff[s]:aoff[s + 1]])
+            rp, col = store.csr()
+            want = oracle.sssp(n, rp, col, 0, oracle.W_SEEDED, 11, 100)
+            out["exact"] = bool(np.array_equal(res["values"][0], want))
+            out["exact_check"] = "final SSSP distances == oracle Dijkstra on the oracle's final multiset"
+    vals_main = res["values"]
+    res["eng"].close()
+
+    def summary(r, label):
+        reads = r["hits"] + r["host"]
+        v = upd / r["total_s"]
+        return {"form": label, "value": round(v, 1),
+                "p50_ms": round(float(np.percentile(r["lat"], 50)), 3),
+                "p90_ms": round(float(np.percentile(r["lat"], 90)), 3),
+                "cache_hit_rate": round(r["hits"] / reads, 4) if reads else None,
+                "host_link_edge_bytes": int(r["host"] * 4),
+                "headline_over_this": round(value / v, 4) if v else None,
+                "identical_values": all(np.array_equal(a, b) for a, b in zip(vals_main, r["values"]))}
+
     if cfg["policy"] == 2 and not args.no_compare:
-        # the plain policy on the identical graph and stream (the judge of the learned one)
-        vals_learned = res["values"]
-        res["eng"].close()
-        lru = run_gpu(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, 1)
-        lreads = lru["hits"] + lru["host"]
-        lval = upd / lru["total_s"]
-        out["compare_lru"] = {
-            "policy": "hotness/LRU", "value": round(lval, 1),
-            "p50_ms": round(float(np.percentile(lru["lat"], 50)), 3),
-            "p90_ms": round(float(np.percentile(lru["lat"], 90)), 3),
-            "cache_hit_rate": round(lru["hits"] / lreads, 4) if lreads else None,
-            "host_link_edge_bytes": int(lru["host"] * 4),
-            "learned_over_lru": round(value / lval, 4) if lval else None,
-            "identical_values": bool(np.array_equal(vals_learned, lru["values"])),
-        }
-        res = lru
+        # the hotness/LRU policy on the identical graph, stream and batched queries
+        lru = run_gpu(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, 1, queries)
+        lru["eng"].close()
+        out["compare_lru"] = summary(lru, "batched engine, hotness/LRU cache")
+    if "programs" in cfg and not args.no_compare:
+        plain = run_plain(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, queries)
+        out["compare_plain"] = summary(plain, "plain: one single-query engine per program, hotness/LRU cache")
     cpu = None
     if not args.no_cpu and cfg["app"] != "pr" and "grid" not in cfg:
         if args.config in CPU_SAMPLE:
@@ -258,18 +342,7 @@ def main():
                                     f"updates/snapshot, same generator and update/edge ratio): ")
         else:
             cpu = cpu_baseline(cfg, n, src, dst, adds, dels, warmup)
-        out["cpu_baseline"] = {k: (round(v, 1) if isinstance(v, float) else v) for k, v in cpu.items() if k != "inc"}
-    if args.check and cfg["app"] != "pr" and "queries" not in cfg:
-        import oracle
-        store = oracle.Store(n, src, dst)
-        aoff = np.concatenate([[0], np.cumsum(ap)]).astype(np.int64)
-        doff = np.concatenate([[0], np.cumsum(dp)]).astype(np.int64)
-        for s in range(snapshots):
-            store.apply(dels[doff[s]:doff[s + 1]], adds[aoff[s]:aoff[s + 1]])
-        rp, col = store.csr()
-        want = oracle.sssp(n, rp, col, 0, oracle.W_SEEDED, 11, 100)
-        out["exact"] = bool(np.array_equal(res["values"], want))
-    res["eng"].close()
+        out["cpu_baseline"] = {k: (round(v, 1) if isinstance(v, float) else v) for k, v in cpu.items()}
     print(json.dumps(out), flush=True)
 
 
