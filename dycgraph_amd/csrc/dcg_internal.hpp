@@ -21,14 +21,30 @@ constexpr uint32_t kBigRow = 256;                         // rows above this go 
 constexpr uint32_t kPiece = 512;                          // edges per big-row piece
 constexpr int kClasses = 40;                              // power-of-two capacity classes
 
-// One adjacency row: a power-of-two block of slots (cap == 0 for an empty row).
+// One adjacency row: a power-of-two block of slots (meta == 0 for an empty row).
 // A block of <= 65,536 slots is aligned to its capacity, so it never crosses a
 // chunk; larger blocks are whole, aligned runs of chunks.
+//   ext   slots in use: live entries plus tombstones
+//   meta  log2(capacity) in bits 0..5, tombstone count in bits 6..31
+// Only rows mutated by the large-row delete path hold tombstones: a deleted
+// entry of their sorted prefix keeps its id in bits 0..30 and gets bit 31
+// (kTomb), so the prefix stays binary-searchable; readers skip such entries.
 struct RowDesc {
     uint64_t off;
-    uint32_t deg;
-    uint32_t cap;
+    uint32_t ext;
+    uint32_t meta;
 };
+
+constexpr uint32_t kTomb = 0x80000000u;
+constexpr uint32_t kTombShift = 6;
+constexpr uint32_t kMaxTombs = (1u << (32 - kTombShift)) - 1;
+
+__host__ __device__ __forceinline__ uint64_t row_cap(const RowDesc &d) { return d.meta ? 1ull << (d.meta & 63u) : 0ull; }
+__host__ __device__ __forceinline__ uint32_t row_tombs(const RowDesc &d) { return d.meta >> kTombShift; }
+__host__ __device__ __forceinline__ uint32_t row_live(const RowDesc &d) { return d.ext - row_tombs(d); }
+__host__ __device__ __forceinline__ uint32_t make_meta(uint32_t log2cap, uint32_t tombs) {
+    return log2cap | (tombs << kTombShift);
+}
 
 // Read/write view of the chunk store passed to kernels by value.
 struct StoreView {

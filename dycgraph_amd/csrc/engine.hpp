@@ -104,6 +104,7 @@ struct Engine {
     uint32_t *h_slab = nullptr;        // pinned host slots (DCG_PLACE_HOST)
     DevBuf<uint64_t> d_tail;           // [0] bump tail, [1] overflow flag, [2..] class cursors
     DevBuf<uint32_t> d_hot_out, d_hot_in;
+    DevBuf<uint32_t> d_sorted_out, d_sorted_in;   // per row: length of the sorted prefix
     CacheState cache;
 
     // ---- vertex state ----

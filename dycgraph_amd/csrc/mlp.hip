@@ -237,16 +237,16 @@ __global__ void k_touch(const RowDesc *out, const RowDesc *in, const dcg_edge *a
         const dcg_edge e = i < na ? adds[i] : dels[i - na];
         if (e.u >= n || e.v >= n) continue;
         const RowDesc a = out[e.u], b = in[e.v];
-        if (a.cap) atomicAdd(&touch[a.off >> kChunkLog2], 1.f);
-        if (b.cap) atomicAdd(&touch[b.off >> kChunkLog2], 1.f);
+        if (a.meta) atomicAdd(&touch[a.off >> kChunkLog2], 1.f);
+        if (b.meta) atomicAdd(&touch[b.off >> kChunkLog2], 1.f);
     }
 }
 
 __global__ void k_degsum(const RowDesc *out, const RowDesc *in, uint32_t n, float *degsum) {
     for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < 2ull * n; v += (uint64_t)gridDim.x * blockDim.x) {
         const RowDesc d = v < n ? out[v] : in[v - n];
-        if (!d.deg) continue;
-        uint64_t a = d.off, e = d.off + d.deg;
+        if (!d.ext) continue;
+        uint64_t a = d.off, e = d.off + d.ext;
         while (a < e) {
             const uint64_t c = a >> kChunkLog2, ce = (c + 1) << kChunkLog2, b = e < ce ? e : ce;
             atomicAdd(&degsum[c], (float)(b - a));

@@ -98,8 +98,8 @@ __global__ void k_place_desc(const uint64_t *keys, const uint64_t *offs, const u
         const uint64_t r = k & ((1ull << 36) - 1);
         RowDesc d;
         d.off = offs[i];
-        d.deg = deg[r];
-        d.cap = (uint32_t)(1ull << (63u - (uint32_t)(k >> 36)));
+        d.ext = deg[r];
+        d.meta = make_meta(63u - (uint32_t)(k >> 36), 0);
         if (r < n) out[r] = d;
         else in[r - n] = d;
     }
@@ -126,6 +126,22 @@ __global__ void k_scatter_rows(const uint32_t *keys, const uint32_t *vals, uint6
     }
 }
 
+// Length of each row's longest non-decreasing prefix (initialised to the degree):
+// the first descent position of a row-sorted edge list.
+__global__ void k_sorted_prefix(const uint32_t *keys, const uint32_t *vals, uint64_t m, const uint64_t *rp,
+                                uint32_t *sorted) {
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < m;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t r = keys[e];
+        if (e > rp[r] && vals[e] < vals[e - 1]) atomicMin(&sorted[r], (uint32_t)(e - rp[r]));
+    }
+}
+
+__global__ void k_copy_u32(const uint32_t *a, uint32_t *b, uint64_t k) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < k; i += (uint64_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
 static int bits_for(uint64_t maxval) {
     int b = 1;
     while (b < 64 && (maxval >> b)) b++;
@@ -134,6 +150,7 @@ static int bits_for(uint64_t maxval) {
 
 void Engine::load_edges_device(uint32_t nn, const uint32_t *d_src, const uint32_t *d_dst, uint64_t m) {
     if (m >= (1ull << 31)) throw Error(DCG_ERR_UNSUPPORTED, "edge count must be below 2^31 per load");
+    if (nn > 0x7FFFFFFFu) throw Error(DCG_ERR_UNSUPPORTED, "vertex ids must be below 2^31");
     n = nn;
     num_edges = m;
     const uint64_t rows = 2ull * n;
@@ -214,10 +231,18 @@ void Engine::load_edges_device(uint32_t nn, const uint32_t *d_src, const uint32_
     t = temp(tb);
     DCG_HIP(hipcub::DeviceRadixSort::SortPairs(t, tb, d_src, ks.p, d_dst, vs.p, (int)m, 0, kb, stream));
     hipLaunchKernelGGL(k_scatter_rows, dim3(grid_for(m)), dim3(256), 0, stream, ks.p, vs.p, m, rp.p, d_out.p, slots.p);
+    d_sorted_out.alloc(n ? n : 1);
+    d_sorted_in.alloc(n ? n : 1);
+    hipLaunchKernelGGL(k_copy_u32, dim3(grid_for(n)), dim3(256), 0, stream, deg.p, d_sorted_out.p, (uint64_t)n);
+    hipLaunchKernelGGL(k_copy_u32, dim3(grid_for(n)), dim3(256), 0, stream, deg.p + n, d_sorted_in.p, (uint64_t)n);
+    hipLaunchKernelGGL(k_sorted_prefix, dim3(grid_for(m)), dim3(256), 0, stream, ks.p, vs.p, m, rp.p,
+                       d_sorted_out.p);
     // in rows
     DCG_HIP(hipcub::DeviceRadixSort::SortPairs(t, tb, d_dst, ks.p, d_src, vs.p, (int)m, 0, kb, stream));
     hipLaunchKernelGGL(k_scatter_rows, dim3(grid_for(m)), dim3(256), 0, stream, ks.p, vs.p, m, rp.p + n + 1, d_in.p,
                        slots.p);
+    hipLaunchKernelGGL(k_sorted_prefix, dim3(grid_for(m)), dim3(256), 0, stream, ks.p, vs.p, m, rp.p + n + 1,
+                       d_sorted_in.p);
     DCG_HIP(hipGetLastError());
     ks.release();
     vs.release();
@@ -353,20 +378,22 @@ __device__ __forceinline__ uint64_t target_ub(const uint64_t *keys, uint64_t fro
 }
 
 struct DelResult {
-    uint32_t removed, moved;
+    uint32_t removed, moved, removed_in_prefix;
 };
 
 // One wave: remove the first `requested` occurrences of every target of the
 // group from the row, survivors compacted in order.  Matches are resolved in
 // lane order through per-run counters, so the occurrences removed are exactly
-// the first ones (source_local_chunk_store.h:685-739).
+// the first ones (source_local_chunk_store.h:685-739).  The row holds no
+// tombstones (rows with tombstones take the large-row path).
 __device__ DelResult wave_delete_row(const StoreView &s, RowDesc *desc, uint32_t row, const RowDesc d,
-                                     const uint64_t *keys, uint64_t gs, uint64_t ge, uint32_t *used) {
+                                     uint32_t sorted_len, const uint64_t *keys, uint64_t gs, uint64_t ge,
+                                     uint32_t *used) {
     const uint32_t lane = lane_id();
-    uint32_t write = 0, removed = 0, moved = 0;
-    for (uint32_t t0 = 0; t0 < d.deg; t0 += kWave) {
+    uint32_t write = 0, removed = 0, moved = 0, before = 0;
+    for (uint32_t t0 = 0; t0 < d.ext; t0 += kWave) {
         const uint32_t i = t0 + lane;
-        const bool valid = i < d.deg;
+        const bool valid = i < d.ext;
         const uint32_t x = valid ? load_slot(s, d.off + i) : 0u;
         const uint64_t lo = target_lb(keys, gs, ge, x);
         const bool match = valid && lo < ge && (uint32_t)keys[lo] == x;
@@ -384,6 +411,7 @@ __device__ DelResult wave_delete_row(const StoreView &s, RowDesc *desc, uint32_t
         const bool keep = valid && !rem;
         const uint64_t km = __ballot(keep);
         const uint32_t rtile = (uint32_t)__popcll(__ballot(rem));
+        before += (uint32_t)__popcll(__ballot(rem && i < sorted_len));
         if (removed + rtile) {
             const uint32_t pos = write + lanes_below(km);
             const bool mv = keep && pos != i;
@@ -393,14 +421,16 @@ __device__ DelResult wave_delete_row(const StoreView &s, RowDesc *desc, uint32_t
         write += (uint32_t)__popcll(km);
         removed += rtile;
     }
-    if (lane == 0 && removed) desc[row].deg = write;
-    return DelResult{removed, moved};
+    if (lane == 0 && removed) desc[row].ext = write;
+    return DelResult{removed, moved, before};
 }
 
-// One wave per touched row of at most kBigDelete entries; larger rows are
-// listed for k_del_big.
-__global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, uint32_t *hot, const uint64_t *keys,
-                                                  uint64_t count,
+// One wave per touched row of at most kBigDelete slots without tombstones;
+// other rows are listed for k_del_big.  `sorted` holds each row's sorted
+// prefix length; removing entries keeps a prefix sorted, so it shrinks by the
+// number of entries removed from it.
+__global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, uint32_t *sorted, uint32_t *hot,
+                                                  const uint64_t *keys, uint64_t count,
                                                   const uint32_t *gstart, const unsigned long long *ngroups_p,
                                                   uint32_t *used, unsigned long long *counters, int count_stats,
                                                   uint32_t *big_groups, unsigned long long *nbig) {
@@ -413,17 +443,19 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
         const uint64_t gs = gstart[g];
         const uint32_t row = (uint32_t)(keys[gs] >> 32);
         const RowDesc d = desc[row];
-        if (d.deg > kBigDelete) {
+        if (d.ext > kBigDelete || row_tombs(d)) {
             if (lane == 0) big_groups[atomicAdd(nbig, 1ull)] = (uint32_t)gs;
             continue;
         }
         const uint64_t ge = group_end<uint64_t>(keys, gs, count, row);
-        const DelResult r = wave_delete_row(s, desc, row, d, keys, gs, ge, used);
+        const uint32_t sl = sorted[row];
+        const DelResult r = wave_delete_row(s, desc, row, d, sl, keys, gs, ge, used);
         if (lane == 0) {
             hot[row] += 1;   // the scan reads the whole row (cache-policy signal)
+            if (r.removed_in_prefix) sorted[row] = sl - r.removed_in_prefix;
             missing += (ge - gs) - r.removed;
             changed += r.removed ? 1 : 0;
-            rd += d.deg;
+            rd += d.ext;
             wr += r.moved;
         }
     }
@@ -435,53 +467,59 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
     }
 }
 
-// One 1024-thread workgroup per large row, one pass over the row in tiles of
-// 4,096 entries: each thread matches its 4 entries against the sorted targets
-// and flags matches in LDS; one wave then resolves the tile's matches in row
-// order through the per-run counters (exact first-occurrence rule); survivors
-// are compacted with a workgroup prefix sum.  All reads of a tile precede its
-// writes, and writes only move entries towards the row start.
-__global__ __launch_bounds__(1024) void k_del_big(StoreView s, RowDesc *desc, uint32_t *hot, const uint64_t *keys,
-                                                  uint64_t count, const uint32_t *big_groups,
-                                                  const unsigned long long *nbig_p, uint32_t *used,
-                                                  unsigned long long *counters, int count_stats) {
-    constexpr uint32_t kTileE = 4096;
-    __shared__ uint32_t runs[kTileE];    // run index of a matched entry, ~0u otherwise
-    __shared__ uint8_t gone[kTileE];
-    __shared__ uint32_t wsum[16];
-    __shared__ uint32_t nmatch;
+constexpr uint32_t kTileE = 4096;   // entries per workgroup tile of the large-row passes
+
+struct BigLds {
+    uint32_t runs[kTileE];   // run index of a matched entry, ~0u otherwise
+    uint8_t gone[kTileE];
+    uint32_t wsum[16];
+    uint32_t nmatch;
+    uint32_t tombs, probes, need_tail;
+};
+
+// Compact slots [from, to) of a row in place, in order, by one 1024-thread
+// workgroup in tiles of 4,096 entries.  TOMBS: drop tombstones.  Otherwise:
+// drop the first still-requested occurrences of the group's targets; each
+// thread matches its 4 entries against the sorted targets and flags matches in
+// LDS, then one wave resolves the tile's matches in row order through the
+// per-run counters (exact first-occurrence rule).  Survivors are placed with a
+// workgroup prefix sum; all reads of a tile precede its writes, and writes only
+// move entries towards the row start.  Returns the new end; `kept_below` counts
+// survivors at positions below `mark`.
+template <bool TOMBS>
+__device__ uint32_t compact_range(const StoreView &s, uint64_t off, uint32_t from, uint32_t to, uint32_t mark,
+                                  const uint64_t *keys, uint64_t gs, uint64_t ge, uint32_t *used, BigLds &L,
+                                  uint32_t &moved, uint32_t &kept_below) {
     const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
-    const uint64_t nbig = *nbig_p;
-    for (uint64_t b = blockIdx.x; b < nbig; b += gridDim.x) {
-        const uint64_t gs = big_groups[b];
-        const uint32_t row = (uint32_t)(keys[gs] >> 32);
-        const uint64_t ge = group_end<uint64_t>(keys, gs, count, row);
-        const RowDesc d = desc[row];
-        uint32_t write = 0, moved = 0;
-        for (uint32_t t0 = 0; t0 < d.deg; t0 += kTileE) {
-            if (tid == 0) nmatch = 0;
-            __syncthreads();
-            uint32_t xv[4];
-            for (int j = 0; j < 4; ++j) {
-                const uint32_t e = j * 1024 + tid, i = t0 + e;
+    uint32_t write = from;
+    for (uint32_t t0 = from; t0 < to; t0 += kTileE) {
+        if (tid == 0) L.nmatch = 0;
+        __syncthreads();
+        uint32_t xv[4];
+        bool keep[4];
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t e = j * 1024 + tid, i = t0 + e;
+            xv[j] = i < to ? load_slot(s, off + i) : 0u;
+            if (TOMBS) {
+                keep[j] = i < to && !(xv[j] & kTomb);
+            } else {
                 uint32_t run = ~0u;
-                xv[j] = 0;
-                if (i < d.deg) {
-                    xv[j] = load_slot(s, d.off + i);
+                if (i < to) {
                     const uint64_t lo = target_lb(keys, gs, ge, xv[j]);
                     if (lo < ge && (uint32_t)keys[lo] == xv[j]) {
                         run = (uint32_t)(lo - gs);
-                        atomicAdd(&nmatch, 1u);
+                        atomicAdd(&L.nmatch, 1u);
                     }
                 }
-                runs[e] = run;
-                gone[e] = 0;
+                L.runs[e] = run;
+                L.gone[e] = 0;
             }
+        }
+        if (!TOMBS) {
             __syncthreads();
-            if (nmatch && w == 0) {
-                // resolve matches in row order
+            if (L.nmatch && w == 0) {
                 for (uint32_t c0 = 0; c0 < kTileE; c0 += kWave) {
-                    const uint32_t run = runs[c0 + lane];
+                    const uint32_t run = L.runs[c0 + lane];
                     uint64_t mm = __ballot(run != ~0u);
                     bool rem = false;
                     while (mm) {
@@ -494,46 +532,146 @@ __global__ __launch_bounds__(1024) void k_del_big(StoreView s, RowDesc *desc, ui
                         }
                         mm &= mm - 1;
                     }
-                    if (rem) gone[c0 + lane] = 1;
+                    if (rem) L.gone[c0 + lane] = 1;
                 }
             }
             __syncthreads();
-            bool keep[4];
-            for (int j = 0; j < 4; ++j) keep[j] = t0 + j * 1024 + tid < d.deg && !gone[j * 1024 + tid];
-            uint32_t pos[4];
-            uint32_t base = write;
-            for (int j = 0; j < 4; ++j) {
-                const uint64_t km = __ballot(keep[j]);
-                if (lane == 0) wsum[w] = (uint32_t)__popcll(km);
-                __syncthreads();
-                uint32_t before = 0, tot = 0;
-                for (uint32_t q = 0; q < 16; ++q) {
-                    before += q < w ? wsum[q] : 0u;
-                    tot += wsum[q];
-                }
-                pos[j] = base + before + lanes_below(km);
-                base += tot;
-                __syncthreads();
+            for (int j = 0; j < 4; ++j) keep[j] = t0 + j * 1024 + tid < to && !L.gone[j * 1024 + tid];
+        }
+        uint32_t pos[4];
+        uint32_t base = write;
+        for (int j = 0; j < 4; ++j) {
+            const uint64_t km = __ballot(keep[j]);
+            if (lane == 0) L.wsum[w] = (uint32_t)__popcll(km);
+            __syncthreads();
+            uint32_t before = 0, tot = 0;
+            for (uint32_t q = 0; q < 16; ++q) {
+                before += q < w ? L.wsum[q] : 0u;
+                tot += L.wsum[q];
             }
-            for (int j = 0; j < 4; ++j) {
-                const uint32_t i = t0 + j * 1024 + tid;
-                if (keep[j] && pos[j] != i) {
-                    store_slot(s, d.off + pos[j], xv[j]);
-                    moved++;
-                }
-            }
-            write = base;
+            pos[j] = base + before + lanes_below(km);
+            base += tot;
             __syncthreads();
         }
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t i = t0 + j * 1024 + tid;
+            if (keep[j] && i < mark) kept_below++;
+            if (keep[j] && pos[j] != i) {
+                store_slot(s, off + pos[j], xv[j]);
+                moved++;
+            }
+        }
+        write = base;
+        __syncthreads();
+    }
+    return write;
+}
+
+constexpr uint32_t kSearchPrefix = 8192;
+constexpr uint32_t kDelBigGrid = 512;      // two 1024-thread workgroups per CU   // sorted prefixes at least this long are binary-searched
+
+// Sum of a per-thread count over the 1024-thread workgroup (via LDS).
+__device__ __forceinline__ uint32_t block_sum(uint32_t x, uint32_t &slot) {
+    if (threadIdx.x == 0) slot = 0;
+    __syncthreads();
+    if (x) atomicAdd(&slot, x);
+    __syncthreads();
+    const uint32_t r = slot;
+    __syncthreads();
+    return r;
+}
+
+// Large rows (more than kBigDelete slots, or holding tombstones): one
+// 1024-thread workgroup per row.
+//  * Sorted prefix [0, S) of at least kSearchPrefix entries: the head of every
+//    run of equal targets binary-searches the run's value and tombstones its
+//    first live occurrences (bit 31 set, id bits kept, so the prefix stays
+//    ordered): O(log S) slot reads per target instead of a scan of the row.
+//    Then, only when some run still wants occurrences, compact_range over the
+//    tail [S, ext) (appended entries, never tombstoned) continues the per-run
+//    counters.  The prefix precedes the tail in the row's sequence, so exactly
+//    the first occurrences go.
+//  * Otherwise one compact_range pass over the whole row (tombstones never
+//    match a target and stay); S shrinks by the entries removed below it.
+//  * When tombstones pass a quarter of the row, a last pass compacts the whole
+//    row; S becomes the number of live prefix entries.
+__global__ __launch_bounds__(1024) void k_del_big(StoreView s, RowDesc *desc, uint32_t *sorted, uint32_t *hot,
+                                                  const uint64_t *keys, uint64_t count, const uint32_t *big_groups,
+                                                  const unsigned long long *nbig_p, uint32_t *used,
+                                                  unsigned long long *counters, int count_stats) {
+    __shared__ BigLds L;
+    const uint32_t tid = threadIdx.x;
+    const uint64_t nbig = *nbig_p;
+    for (uint64_t b = blockIdx.x; b < nbig; b += gridDim.x) {
+        const uint64_t gs = big_groups[b];
+        const uint32_t row = (uint32_t)(keys[gs] >> 32);
+        const uint64_t ge = group_end<uint64_t>(keys, gs, count, row);
+        const RowDesc d = desc[row];
+        const uint32_t S = sorted[row] < d.ext ? sorted[row] : d.ext;
+        uint32_t moved = 0, kept = 0, ext = d.ext, S2 = S, new_tombs = 0;
+        uint64_t scanned = 0;
+        bool full_read = false;
+        if (S >= kSearchPrefix) {
+            uint32_t tombs = 0, probes = 0, need_tail = 0;
+            for (uint64_t q = gs + tid; q < ge; q += blockDim.x) {
+                const uint32_t v = (uint32_t)keys[q];
+                if (q > gs && (uint32_t)keys[q - 1] == v) continue;
+                const uint32_t c = (uint32_t)(target_ub(keys, q + 1, ge, v) - q);
+                uint32_t lo = 0, hi = S;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    probes++;
+                    if ((load_slot(s, d.off + mid) & ~kTomb) < v) lo = mid + 1;
+                    else hi = mid;
+                }
+                uint32_t taken = 0;
+                for (uint32_t j = lo; j < S && taken < c; ++j) {
+                    const uint32_t x = load_slot(s, d.off + j);
+                    probes++;
+                    if ((x & ~kTomb) != v) break;
+                    if (!(x & kTomb)) {
+                        store_slot(s, d.off + j, x | kTomb);
+                        taken++;
+                    }
+                }
+                used[q] = taken;
+                tombs += taken;
+                need_tail |= taken < c ? 1u : 0u;
+            }
+            new_tombs = block_sum(tombs, L.tombs);
+            scanned += block_sum(probes, L.probes);
+            if (block_sum(need_tail, L.need_tail) && S < d.ext) {
+                ext = compact_range<false>(s, d.off, S, d.ext, 0, keys, gs, ge, used, L, moved, kept);
+                scanned += d.ext - S;
+            }
+        } else {
+            ext = compact_range<false>(s, d.off, 0, d.ext, S, keys, gs, ge, used, L, moved, kept);
+            S2 = block_sum(kept, L.tombs);
+            scanned += d.ext;
+            full_read = true;
+        }
+        const uint32_t removed = new_tombs + (d.ext - ext);
+        uint32_t tombs_left = row_tombs(d) + new_tombs;
+        if (tombs_left && ((uint64_t)tombs_left * 4 > ext || tombs_left > kMaxTombs / 2)) {
+            kept = 0;
+            const uint32_t ext2 = compact_range<true>(s, d.off, 0, ext, S2, keys, gs, ge, used, L, moved, kept);
+            S2 = block_sum(kept, L.tombs);
+            scanned += ext;
+            ext = ext2;
+            tombs_left = 0;
+            full_read = true;
+        }
         if (tid == 0) {
-            const uint32_t removed = d.deg - write;
-            if (removed) desc[row].deg = write;
+            desc[row].ext = ext;
+            desc[row].meta = make_meta(d.meta & 63u, tombs_left);
+            sorted[row] = S2;
             if (count_stats) {
                 atomicAdd(&counters[C_MISSING], (unsigned long long)((ge - gs) - removed));
                 if (removed) atomicAdd(&counters[C_CHANGED], 1ull);
             }
-            atomicAdd(&counters[C_SLOTS_READ], (unsigned long long)d.deg);
-            hot[row] += 1;
+            atomicAdd(&counters[C_SLOTS_READ], scanned);
+            if (new_tombs) atomicAdd(&counters[C_SLOTS_WRITTEN], (unsigned long long)new_tombs);
+            if (full_read) hot[row] += 1;   // a pass read the whole row (cache-policy signal)
         }
         if (moved) atomicAdd(&counters[C_SLOTS_WRITTEN], (unsigned long long)moved);
         __syncthreads();
@@ -555,8 +693,8 @@ __global__ __launch_bounds__(256) void k_add_plan(const RowDesc *desc, const uin
         const uint32_t row = keys[gs];
         const uint64_t ge = group_end<uint32_t>(keys, gs, count, row);
         const RowDesc d = desc[row];
-        const uint64_t nd = (uint64_t)d.deg + (ge - gs);
-        if (nd > d.cap) {
+        const uint64_t nd = (uint64_t)d.ext + (ge - gs);
+        if (nd > row_cap(d)) {
             uint64_t c = nd < kMinRowCap ? kMinRowCap : nd;
             const uint32_t cls = 64u - (uint32_t)__clzll((long long)(c - 1));
             atomicAdd(&hist[cls], 1u);
@@ -609,10 +747,10 @@ __global__ __launch_bounds__(256) void k_add_rows(StoreView s, RowDesc *desc, ui
         const uint64_t ge = group_end<uint32_t>(keys, gs, count, row);
         const uint32_t k = (uint32_t)(ge - gs);
         RowDesc d = desc[row];
-        const uint64_t nd = (uint64_t)d.deg + k;
-        if (nd > d.cap) {
+        const uint64_t nd = (uint64_t)d.ext + k;
+        if (nd > row_cap(d)) {
             if (overflow) continue;
-            if (d.deg > kBigAppend) {
+            if (d.ext > kBigAppend) {
                 if (lane == 0) big_groups[atomicAdd(nbig, 1ull)] = (uint32_t)gs;
                 continue;
             }
@@ -620,17 +758,19 @@ __global__ __launch_bounds__(256) void k_add_rows(StoreView s, RowDesc *desc, ui
             unsigned long long noff = 0;
             if (lane == 0) noff = atomicAdd((unsigned long long *)&tail[T_CURSOR + cls], 1ull << cls);
             noff = __shfl(noff, 0, kWave);
-            for (uint32_t i = lane; i < d.deg; i += kWave) store_slot(s, noff + i, load_slot(s, d.off + i));
-            if (lane == 0 && d.deg) hot[row] += 1;
+            // entries (tombstones included) keep their positions, so the
+            // sorted prefix length of the row stays valid
+            for (uint32_t i = lane; i < d.ext; i += kWave) store_slot(s, noff + i, load_slot(s, d.off + i));
+            if (lane == 0 && d.ext) hot[row] += 1;
             d.off = noff;
-            d.cap = (uint32_t)(1ull << cls);
+            d.meta = make_meta(cls, row_tombs(d));
             relocated++;
-            rd += d.deg;
-            wr += d.deg;
+            rd += d.ext;
+            wr += d.ext;
         }
-        for (uint32_t i = lane; i < k; i += kWave) store_slot(s, d.off + d.deg + i, vals[gs + i]);
+        for (uint32_t i = lane; i < k; i += kWave) store_slot(s, d.off + d.ext + i, vals[gs + i]);
         wr += k;
-        d.deg = (uint32_t)nd;
+        d.ext = (uint32_t)nd;
         if (lane == 0) desc[row] = d;
     }
     if (lane == 0) {
@@ -654,25 +794,25 @@ __global__ __launch_bounds__(1024) void k_add_big(StoreView s, RowDesc *desc, ui
         const uint64_t ge = group_end<uint32_t>(keys, gs, count, row);
         const uint32_t k = (uint32_t)(ge - gs);
         RowDesc d = desc[row];
-        const uint64_t nd = (uint64_t)d.deg + k;
+        const uint64_t nd = (uint64_t)d.ext + k;
         const uint32_t cls = class_of(nd);
         if (threadIdx.x == 0) noff_s = atomicAdd((unsigned long long *)&tail[T_CURSOR + cls], 1ull << cls);
         __syncthreads();
         const uint64_t noff = noff_s;
         // blocks above 65,536 slots are whole chunks; copy chunk segment by segment
-        for (uint64_t seg = 0; seg < d.deg; seg += kChunkEdges) {
-            const uint32_t len = (uint32_t)min((uint64_t)kChunkEdges, (uint64_t)d.deg - seg);
+        for (uint64_t seg = 0; seg < d.ext; seg += kChunkEdges) {
+            const uint32_t len = (uint32_t)min((uint64_t)kChunkEdges, (uint64_t)d.ext - seg);
             const uint32_t *src = row_base(s, d.off + seg);
             uint32_t *dst = row_base_w(s, noff + seg);
             for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) dst[i] = src[i];
         }
-        for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) store_slot(s, noff + d.deg + i, vals[gs + i]);
+        for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) store_slot(s, noff + d.ext + i, vals[gs + i]);
         __syncthreads();
         if (threadIdx.x == 0) {
-            desc[row] = RowDesc{noff, (uint32_t)nd, (uint32_t)(1ull << cls)};
+            desc[row] = RowDesc{noff, (uint32_t)nd, make_meta(cls, row_tombs(d))};
             hot[row] += 1;
             if (count_stats) atomicAdd(&counters[C_RELOCATED], 1ull);
-            atomicAdd(&counters[C_SLOTS_READ], (unsigned long long)d.deg);
+            atomicAdd(&counters[C_SLOTS_READ], (unsigned long long)d.ext);
             atomicAdd(&counters[C_SLOTS_WRITTEN], (unsigned long long)nd);
         }
         __syncthreads();
@@ -715,17 +855,21 @@ void Engine::mutate_delete(uint64_t begin, uint64_t count, dcg_phase_stats *st) 
     d_big_groups.ensure(count);
     fill<uint32_t>(stream, d_used.p, count, 0u);
     const int slot = time_begin(DCG_K_DELETE_ROWS, -1);
-    hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream, v, d_out.p, d_hot_out.p, kout, count, d_gstart.p,
+    hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream, v, d_out.p, d_sorted_out.p, d_hot_out.p, kout,
+                       count, d_gstart.p,
                        d_counters.p + C_GROUPS_OUT, d_used.p, d_counters.p, 1, d_big_groups.p,
                        d_counters.p + C_NBIG);
-    hipLaunchKernelGGL(k_del_big, dim3(256), dim3(1024), 0, stream, v, d_out.p, d_hot_out.p, kout, count,
+    hipLaunchKernelGGL(k_del_big, dim3(kDelBigGrid), dim3(1024), 0, stream, v, d_out.p, d_sorted_out.p, d_hot_out.p,
+                       kout, count,
                        d_big_groups.p, d_counters.p + C_NBIG, d_used.p, d_counters.p, 1);
     fill<uint32_t>(stream, d_used.p, count, 0u);
     fill<unsigned long long>(stream, d_counters.p + C_NBIG, 1, 0ull);
-    hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream, v, d_in.p, d_hot_in.p, kin, count, d_gstart2.p,
+    hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream, v, d_in.p, d_sorted_in.p, d_hot_in.p, kin,
+                       count, d_gstart2.p,
                        d_counters.p + C_GROUPS_IN, d_used.p, d_counters.p, 0, d_big_groups.p,
                        d_counters.p + C_NBIG);
-    hipLaunchKernelGGL(k_del_big, dim3(256), dim3(1024), 0, stream, v, d_in.p, d_hot_in.p, kin, count,
+    hipLaunchKernelGGL(k_del_big, dim3(kDelBigGrid), dim3(1024), 0, stream, v, d_in.p, d_sorted_in.p, d_hot_in.p,
+                       kin, count,
                        d_big_groups.p, d_counters.p + C_NBIG, d_used.p, d_counters.p, 0);
     time_end(slot);
     DCG_HIP(hipGetLastError());
@@ -810,7 +954,7 @@ void Engine::mutate_add(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
 // ---------------------------------------------------------------------- export
 __global__ void k_export_deg(const RowDesc *out, uint32_t n, uint64_t *deg) {
     for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x)
-        deg[v] = out[v].deg;
+        deg[v] = row_live(out[v]);
 }
 
 __global__ void k_export_rows(StoreView s, uint32_t n, const uint64_t *rp, uint32_t *col) {
@@ -819,7 +963,18 @@ __global__ void k_export_rows(StoreView s, uint32_t n, const uint64_t *rp, uint3
     const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / kWave;
     for (uint64_t v = wave; v < n; v += nwaves) {
         const RowDesc d = s.out[v];
-        for (uint32_t i = lane; i < d.deg; i += kWave) col[rp[v] + i] = load_slot(s, d.off + i);
+        if (!row_tombs(d)) {
+            for (uint32_t i = lane; i < d.ext; i += kWave) col[rp[v] + i] = load_slot(s, d.off + i);
+            continue;
+        }
+        uint64_t w = rp[v];   // skip tombstones, order kept
+        for (uint32_t t0 = 0; t0 < d.ext; t0 += kWave) {
+            const uint32_t i = t0 + lane;
+            const uint32_t x = i < d.ext ? load_slot(s, d.off + i) : kTomb;
+            const uint64_t km = __ballot(!(x & kTomb));
+            if (!(x & kTomb)) col[w + lanes_below(km)] = x;
+            w += (uint64_t)__popcll(km);
+        }
     }
 }
 
@@ -861,7 +1016,7 @@ __global__ void k_fold_reads(StoreView s, uint32_t n, float *reads, float *rows,
             const uint32_t h = dir ? hi : ho;
             if (!h) continue;
             const RowDesc d = dir ? s.in[v] : s.out[v];
-            uint64_t a = d.off, e = d.off + d.deg;
+            uint64_t a = d.off, e = d.off + d.ext;
             while (a < e) {
                 const uint64_t c = a >> kChunkLog2;
                 const uint64_t ce = (c + 1) << kChunkLog2;

@@ -149,6 +149,7 @@ __device__ __forceinline__ bool pr_add(const TravArgs &a, uint32_t d, float c, u
 template <uint32_t MODE>
 __device__ __forceinline__ bool edge_op(const TravArgs &a, uint32_t q, uint32_t owner, uint32_t val, uint32_t x,
                                         uint32_t stamp, uint32_t &out_g) {
+    if (x & kTomb) return false;   // deleted entry of a large row
     if (MODE == M_PUSH) return push_op(a, q, owner, val, x, stamp, out_g);
     if (MODE == M_INVALIDATE) return invalidate_op(a, q, owner, val, x, out_g);
     if (MODE == M_PR || MODE == M_PR_AMEND) return pr_add(a, x, __uint_as_float(val), out_g);
@@ -225,18 +226,19 @@ __device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L
                 v = g;
                 const RowDesc d = a.s.out[v];
                 const float x = MODE == M_PR ? a.delta[i] : a.rank[v] * a.sign;
-                if (d.deg && x != 0.f) {
-                    deg = d.deg;
+                const uint32_t live = row_live(d);
+                if (live && x != 0.f) {
+                    deg = d.ext;
                     off = d.off;
-                    val = __float_as_uint(0.85f * x / (float)d.deg);
+                    val = __float_as_uint(0.85f * x / (float)live);
                     a.s.hot_out[v] += 1;
                 }
             } else {
                 decode_g(a, g, q, v);
                 const RowDesc d = MODE == M_PULL ? a.s.in[v] : a.s.out[v];
                 val = MODE == M_PULL ? 0u : (uint32_t)(a.st[g] >> 32);
-                if (d.deg && (MODE == M_PULL || val != kInf)) {
-                    deg = d.deg;
+                if (d.ext && (MODE == M_PULL || val != kInf)) {
+                    deg = d.ext;
                     off = d.off;
                     if (MODE == M_PULL) a.s.hot_in[v] += 1;
                     else a.s.hot_out[v] += 1;
@@ -327,7 +329,8 @@ __device__ void phase_b(const TravArgs &a, uint32_t r, const Team &t, uint64_t &
             if (MODE == M_PULL) {
                 uint64_t best = ~0ull;
                 for (uint32_t k = lane; k < len; k += kWave) {
-                    const uint64_t c = pull_cand(a, q, v, base[k]);
+                    const uint32_t x = base[k];
+                    const uint64_t c = (x & kTomb) ? ~0ull : pull_cand(a, q, v, x);
                     best = c < best ? c : best;
                 }
                 for (int d = 32; d >= 1; d >>= 1) {

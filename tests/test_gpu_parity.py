@@ -222,3 +222,66 @@ def test_large_row_relocation_exact_rows(placement):
     rp, col = store.csr()
     assert np.array_equal(eng.gather_value(), oracle.bfs(n, rp, col, 0))
     eng.close()
+
+
+@pytest.mark.parametrize("placement", [0, 1])
+def test_sorted_large_rows_tombstones(placement):
+    """Rows whose sorted prefix holds at least 8,192 entries delete by binary
+    search and tombstones: first occurrences across the prefix and the appended
+    tail, requests beyond the occurrences, tombstones carried through a
+    relocation, and the compaction once tombstones pass a quarter of the row.
+    Rows (tombstones skipped) and BFS levels must match the multiset model after
+    every stage."""
+    announce(f"sorted large rows with tombstones placement={placement}")
+    import dycgraph_amd as d
+    n = 60000
+    hub = sorted(list(range(1, 20001)) + [5, 5, 9, 20000])    # 5 x3, 9 x2, 20000 x2, sorted
+    src = [0] * len(hub) + list(range(100, 20100)) + list(range(20100, 20600))
+    dst = hub + [1] * 20000 + [int(x) for x in np.random.default_rng(3).integers(2, n, 500)]
+    src, dst = np.array(src, np.uint32), np.array(dst, np.uint32)
+    rng = np.random.default_rng(9)
+    stages = []
+    # 1: prefix deletes (duplicates, over-requests, a missing value), tail appends incl. repeats
+    stages.append(([(0, 5), (0, 5), (0, 9), (0, 9), (0, 9), (0, 123), (0, 50001)] +
+                   [(int(u), 1) for u in rng.choice(np.arange(100, 20100), 400, replace=False)],
+                   [(0, 5), (0, 123), (0, 40000), (0, 40000), (0, 9), (150, 1), (40000, 1)]))
+    # 2: deletes reaching past the prefix into the tail, tail-only values
+    stages.append(([(0, 5), (0, 5), (0, 40000), (0, 9), (0, 9), (0, 123), (0, 123), (40000, 1), (150, 1)],
+                   [(0, 41000 + i) for i in range(300)]))
+    # 3: relocation of the tombstoned row (cap 32,768 overflows)
+    stages.append(([(0, int(v)) for v in rng.choice(np.arange(10, 20000), 200, replace=False)],
+                   [(0, 45000 + i) for i in range(13000)]))
+    # 4: enough deletes to pass a quarter tombstones -> compaction; in-row too
+    big = [int(v) for v in rng.choice(np.arange(10, 20000), 9000, replace=False)]
+    stages.append(([(0, v) for v in big] + [(int(u), 1) for u in range(100, 8000)],
+                   [(0, 7), (0, 7), (3, 1)]))
+    # 5: after the compaction: prefix (recomputed length) and tail deletes
+    stages.append(([(0, 7), (0, 45001), (0, 20000), (0, 20000), (0, 20000)] +
+                   [(0, v) for v in big[:50]], [(0, 2), (0, 1)]))
+    adds = [e for _, a in stages for e in a]
+    dels = [e for dl, _ in stages for e in dl]
+    eng = _engine(app="bfs", placement=placement, cache_bytes=1 << 18)
+    eng.load_graph(n, src, dst)
+    eng.start()
+    eng.load_updates(adds, dels)
+    store = oracle.Store(n, src, dst)
+    ab = db = 0
+    for s, (dl, al) in enumerate(stages):
+        st = eng.del_edge(db, len(dl))
+        m = store.apply(dl, [])
+        assert st["missing_deletes"] == m["missing_deletes"], f"stage {s}"
+        erp, ecol = eng.csr()
+        assert same_rows(erp, ecol, store), f"rows differ after delete stage {s}"
+        rp, col = store.csr()
+        assert np.array_equal(eng.gather_value(), oracle.bfs(n, rp, col, 0)), f"levels, delete stage {s}"
+        st = eng.add_edge(ab, len(al))
+        store.apply([], al)
+        erp, ecol = eng.csr()
+        assert same_rows(erp, ecol, store), f"rows differ after add stage {s}"
+        rp, col = store.csr()
+        assert np.array_equal(eng.gather_value(), oracle.bfs(n, rp, col, 0)), f"levels, add stage {s}"
+        if s == 2:
+            assert st["relocated_rows"] >= 1
+        ab += len(al)
+        db += len(dl)
+    eng.close()
