@@ -350,6 +350,12 @@ int dcg_gather_parents(dcg_engine *h, uint32_t query, uint32_t *out, size_t n) {
         if (!e.started) throw Error(DCG_ERR_STATE, "gather before dcg_start");
         if (!out || n != e.n || query >= e.Q) throw Error(DCG_ERR_ARG, "gather size/query mismatch");
         if (e.app == DCG_APP_PR) throw Error(DCG_ERR_UNSUPPORTED, "PageRank has no parents");
+        if (e.qapps[query] == DCG_APP_CC) {
+            // CC keeps (label, depth) in the state word and parents apart
+            DCG_HIP(hipMemcpyAsync(out, e.d_par.p + (uint64_t)query * e.n, n * 4, hipMemcpyDeviceToHost, e.stream));
+            e.sync();
+            return;
+        }
         std::vector<uint64_t> tmp(n);
         DCG_HIP(hipMemcpyAsync(tmp.data(), e.d_st.p + (uint64_t)query * e.n, n * 8, hipMemcpyDeviceToHost, e.stream));
         e.sync();

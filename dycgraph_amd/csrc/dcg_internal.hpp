@@ -154,10 +154,11 @@ struct Ctl {
 
 struct BigEntry {
     uint64_t off;      // first slot of the row
+    uint64_t val;      // owner state word (min-lattice) or float bits (PR delta/deg)
     uint32_t g;        // global entry id (q*n + v)
     uint32_t deg;
-    uint32_t val;      // owner value (min-lattice) or float bits (PR delta/deg)
     uint32_t piece0;   // first piece index
+    uint32_t pad;
 };
 
 template <typename T>

@@ -108,7 +108,8 @@ struct Engine {
     CacheState cache;
 
     // ---- vertex state ----
-    DevBuf<uint64_t> d_st;             // Q*n packed (value<<32 | parent)
+    DevBuf<uint64_t> d_st;             // Q*n packed (value<<32 | parent); CC: (label<<32 | depth)
+    DevBuf<uint32_t> d_par;            // Q*n CC parents (allocated when a query runs CC)
     DevBuf<uint32_t> d_tick;           // Q*n dedup stamps
     uint32_t stamp = 1;
     DevBuf<uint32_t> d_qa, d_qb, d_qc, d_aff;

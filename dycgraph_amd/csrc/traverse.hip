@@ -2,11 +2,17 @@
 // programs (SSSP, BFS, CC) and the PageRank residual push, plus the host
 // orchestration of one snapshot's delete and add stages.
 //
-// Vertex state is one 64-bit word per (query, vertex): value in the high half,
-// parent in the low half.  A relaxation is one 64-bit atomicMin, so the value
-// lattice (the reference's atomicMin + parent CAS loop, hybrid_sssp.cu:126-145)
-// and the smallest-source tie-break of the pull repair (framework.cuh:229-244)
-// come from the same instruction.
+// Vertex state is one 64-bit word per (query, vertex).  SSSP/BFS: value in the
+// high half, parent in the low half; a relaxation is one 64-bit atomicMin, so
+// the value lattice (the reference's atomicMin + parent CAS loop,
+// hybrid_sssp.cu:126-145) and the smallest-source tie-break of the pull repair
+// (framework.cuh:229-244) come from the same instruction.  CC: label in the high
+// half, hop depth from the label's root in the low half, parent in a separate
+// array.  The label lattice is unchanged (label = smallest id that reaches the
+// vertex), and the (label, depth) order makes parents a shortest-hop tree
+// inside each label class, so a deletion invalidates only the part of that tree
+// that lost its path to the root (a rooted witness, the exact counterpart of
+// directed_label_witness.h) instead of every vertex with an equal label.
 //
 // One cooperative launch runs rounds until the frontier is empty:
 //   phase A: waves take 64 frontier entries at a time; rows of <= 256 edges are
@@ -28,6 +34,7 @@ struct TravArgs {
     StoreView s;
     uint32_t n, Q;
     uint64_t *st;
+    uint32_t *par;             // CC parents (q*n + v), null without CC queries
     uint32_t *tick;
     const uint32_t *sources;
     const uint32_t *qapp;      // per-query program
@@ -71,20 +78,31 @@ __device__ __forceinline__ uint32_t qweight(const TravArgs &a, uint32_t app, uin
     return edge_weight(a.w, u, v);
 }
 
-// Dependency predicate of the delete stage: api.cuh:43-49 for SSSP/BFS (parent
-// edge, or a tight witness while the current parent is not tight) and
-// hybrid_cc.cu:60-64 for CC (equal label, label smaller than the id).  A
-// dependent vertex whose parent half is still set is claimed (parent := INF).
-__device__ __forceinline__ bool invalidate_op(const TravArgs &a, uint32_t q, uint32_t s, uint32_t vs, uint32_t d,
+// Dependency predicate of the delete stage (api.cuh:43-49): the parent edge,
+// or a tight witness while the current parent is not tight.  For CC "tight"
+// means equal label and depth + 1 (the reference's hybrid_cc.cu:60-64 marks
+// every equal-label successor instead).  A dependent vertex is claimed once:
+// SSSP/BFS set the parent half to INF, CC its parent entry.  `sv` is the
+// state word of the edge's source.
+__device__ __forceinline__ bool invalidate_op(const TravArgs &a, uint32_t q, uint32_t s, uint64_t sv, uint32_t d,
                                               uint32_t &out_g) {
     const uint64_t qoff = (uint64_t)q * a.n;
     const uint64_t gd = qoff + d;
     const uint64_t sd = a.st[gd];
     const uint32_t vd = (uint32_t)(sd >> 32), pd = (uint32_t)sd;
+    const uint32_t vs = (uint32_t)(sv >> 32);
     const uint32_t app = a.qapp[q];
     bool dep;
     if (app == DCG_APP_CC) {
-        dep = s != d && vs == vd && vd < d;
+        const uint32_t pc = a.par[gd];
+        if (pc == kInf) return false;   // a root (own label) or already claimed
+        if (pc == s) dep = true;
+        else if (sv + 1 == sd) dep = a.st[qoff + pc] + 1 != sd;
+        else dep = false;
+        if (!dep) return false;
+        if (atomicExch(&a.par[gd], kInf) == kInf) return false;
+        out_g = (uint32_t)gd;
+        return true;
     } else {
         if (d == a.sources[q] || pd == kInf) return false;
         if (pd == s) {
@@ -103,9 +121,23 @@ __device__ __forceinline__ bool invalidate_op(const TravArgs &a, uint32_t q, uin
     return true;
 }
 
-__device__ __forceinline__ bool push_op(const TravArgs &a, uint32_t q, uint32_t s, uint32_t vs, uint32_t d,
+__device__ __forceinline__ bool push_op(const TravArgs &a, uint32_t q, uint32_t s, uint64_t sv, uint32_t d,
                                         uint32_t stamp, uint32_t &out_g) {
-    const uint64_t cand = (uint64_t)vs + qweight(a, a.qapp[q], s, d);
+    const uint32_t app = a.qapp[q];
+    if (app == DCG_APP_CC) {
+        // (label, depth + 1); the parent entry follows the winning word
+        const uint64_t gd = (uint64_t)q * a.n + d;
+        const uint64_t cand = sv + 1;
+        if (cand >= a.st[gd]) return false;
+        const uint64_t old = atomicMin((unsigned long long *)&a.st[gd], (unsigned long long)cand);
+        if (cand >= old) return false;
+        a.par[gd] = s;
+        if (atomicExch(&a.tick[gd], stamp) == stamp) return false;
+        out_g = (uint32_t)gd;
+        return true;
+    }
+    const uint32_t vs = (uint32_t)(sv >> 32);
+    const uint64_t cand = (uint64_t)vs + qweight(a, app, s, d);
     if (cand >= kInf) return false;
     const uint64_t gd = (uint64_t)q * a.n + d;
     const uint64_t cur = a.st[gd];
@@ -134,6 +166,17 @@ __device__ __forceinline__ void pull_apply(const TravArgs &a, uint32_t q, uint32
     atomicMin((unsigned long long *)&a.st[gd], (unsigned long long)packed);
 }
 
+// CC pull: (label, depth + 1) of in-neighbour x; the parent entry follows the
+// winning word.
+__device__ __forceinline__ void pull_cc(const TravArgs &a, uint32_t q, uint32_t d, uint32_t x) {
+    const uint64_t qoff = (uint64_t)q * a.n;
+    const uint64_t cand = a.st[qoff + x] + 1;
+    const uint64_t gd = qoff + d;
+    if (cand >= a.st[gd]) return;
+    const uint64_t old = atomicMin((unsigned long long *)&a.st[gd], (unsigned long long)cand);
+    if (cand < old) a.par[gd] = x;
+}
+
 __device__ __forceinline__ bool pr_add(const TravArgs &a, uint32_t d, float c, uint32_t &out_g) {
     // Frontier::Add (include/framework/pr_residual.cuh:21-30)
     const float old = atomicAdd(&a.res[d], c);
@@ -147,13 +190,14 @@ __device__ __forceinline__ bool pr_add(const TravArgs &a, uint32_t d, float c, u
 }
 
 template <uint32_t MODE>
-__device__ __forceinline__ bool edge_op(const TravArgs &a, uint32_t q, uint32_t owner, uint32_t val, uint32_t x,
+__device__ __forceinline__ bool edge_op(const TravArgs &a, uint32_t q, uint32_t owner, uint64_t val, uint32_t x,
                                         uint32_t stamp, uint32_t &out_g) {
     if (x & kTomb) return false;   // deleted entry of a large row
     if (MODE == M_PUSH) return push_op(a, q, owner, val, x, stamp, out_g);
     if (MODE == M_INVALIDATE) return invalidate_op(a, q, owner, val, x, out_g);
-    if (MODE == M_PR || MODE == M_PR_AMEND) return pr_add(a, x, __uint_as_float(val), out_g);
-    pull_apply(a, q, owner, pull_cand(a, q, owner, x));
+    if (MODE == M_PR || MODE == M_PR_AMEND) return pr_add(a, x, __uint_as_float((uint32_t)val), out_g);
+    if (a.qapp[q] == DCG_APP_CC) pull_cc(a, q, owner, x);
+    else pull_apply(a, q, owner, pull_cand(a, q, owner, x));
     return false;
 }
 
@@ -187,7 +231,7 @@ __device__ __forceinline__ void emit(const TravArgs &a, uint32_t *outq, uint32_t
 
 struct WaveLds {
     uint32_t pref[65];
-    uint32_t val[64];
+    uint64_t val[64];
     uint32_t owner[64];
     uint32_t q[64];
     const uint32_t *base[64];
@@ -219,7 +263,8 @@ __device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L
         const bool has = i < cnt;
         const uint32_t g = has ? in[i] : 0u;
         uint32_t q = 0, v = 0;
-        uint32_t deg = 0, val = 0;
+        uint32_t deg = 0;
+        uint64_t val = 0;
         uint64_t off = 0;
         if (has) {
             if (MODE == M_PR || MODE == M_PR_AMEND) {
@@ -236,8 +281,8 @@ __device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L
             } else {
                 decode_g(a, g, q, v);
                 const RowDesc d = MODE == M_PULL ? a.s.in[v] : a.s.out[v];
-                val = MODE == M_PULL ? 0u : (uint32_t)(a.st[g] >> 32);
-                if (d.ext && (MODE == M_PULL || val != kInf)) {
+                val = MODE == M_PULL ? 0ull : a.st[g];
+                if (d.ext && (MODE == M_PULL || (uint32_t)(val >> 32) != kInf)) {
                     deg = d.ext;
                     off = d.off;
                     if (MODE == M_PULL) a.s.hot_in[v] += 1;
@@ -326,7 +371,12 @@ __device__ void phase_b(const TravArgs &a, uint32_t r, const Team &t, uint64_t &
             if (MODE == M_PR || MODE == M_PR_AMEND) v = be.g;
             else decode_g(a, be.g, q, v);
             edges += lane == 0 ? len : 0;
-            if (MODE == M_PULL) {
+            if (MODE == M_PULL && a.qapp[q] == DCG_APP_CC) {
+                for (uint32_t k = lane; k < len; k += kWave) {
+                    const uint32_t x = base[k];
+                    if (!(x & kTomb)) pull_cc(a, q, v, x);
+                }
+            } else if (MODE == M_PULL) {
                 uint64_t best = ~0ull;
                 for (uint32_t k = lane; k < len; k += kWave) {
                     const uint32_t x = base[k];
@@ -484,15 +534,19 @@ __global__ void k_ctl_setup(Ctl *c, const uint32_t *src0, uint32_t add0, uint32_
     c->solo_resume_b = 0;
 }
 
-__global__ void k_init_state(uint64_t *st, uint32_t *tick, uint32_t n, uint32_t Q, const uint32_t *qapp,
-                             const uint32_t *sources) {
+__global__ void k_init_state(uint64_t *st, uint32_t *par, uint32_t *tick, uint32_t n, uint32_t Q,
+                             const uint32_t *qapp, const uint32_t *sources) {
     const uint64_t total = (uint64_t)n * Q;
     for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < total;
          g += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t q = (uint32_t)(g / n), v = (uint32_t)(g - (uint64_t)q * n);
         uint64_t s = kInfState;
-        if (qapp[q] == DCG_APP_CC) s = ((uint64_t)v << 32) | kInf;
-        else if (v == sources[q]) s = (uint64_t)kInf;   // value 0, parent INF
+        if (qapp[q] == DCG_APP_CC) {
+            s = (uint64_t)v << 32;   // own label at depth 0, no parent
+            par[g] = kInf;
+        } else if (v == sources[q]) {
+            s = (uint64_t)kInf;   // value 0, parent INF
+        }
         st[g] = s;
         tick[g] = 0;
     }
@@ -524,10 +578,7 @@ __global__ void k_seed_dels(TravArgs a, const dcg_edge *recs, uint64_t count) {
         if (i < total) {
             const uint32_t q = (uint32_t)(i / count);
             const dcg_edge e = recs[i - (uint64_t)q * count];
-            if (e.u < a.n && e.v < a.n) {
-                const uint32_t vs = (uint32_t)(a.st[(uint64_t)q * a.n + e.u] >> 32);
-                want = invalidate_op(a, q, e.u, vs, e.v, og);
-            }
+            if (e.u < a.n && e.v < a.n) want = invalidate_op(a, q, e.u, a.st[(uint64_t)q * a.n + e.u], e.v, og);
         }
         const uint64_t mask = __ballot(want);
         if (mask) {
@@ -552,7 +603,7 @@ __global__ void k_reset_affected(uint64_t *st, const uint32_t *aff, const Ctl *c
         const uint32_t g = aff[i];
         const uint32_t q = Q == 1 ? 0u : g / n;
         const uint32_t v = g - q * n;
-        st[g] = qapp[q] == DCG_APP_CC ? (((uint64_t)v << 32) | kInf) : kInfState;
+        st[g] = qapp[q] == DCG_APP_CC ? ((uint64_t)v << 32) : kInfState;   // CC: own root (parent already INF)
     }
 }
 
@@ -568,8 +619,8 @@ __global__ void k_seed_adds(TravArgs a, const dcg_edge *recs, uint64_t count, ui
             const uint32_t q = (uint32_t)(i / count);
             const dcg_edge e = recs[i - (uint64_t)q * count];
             if (e.u < a.n && e.v < a.n) {
-                const uint32_t vs = (uint32_t)(a.st[(uint64_t)q * a.n + e.u] >> 32);
-                if (vs != kInf) want = push_op(a, q, e.u, vs, e.v, a.stamp0, og);
+                const uint64_t sv = a.st[(uint64_t)q * a.n + e.u];
+                if ((uint32_t)(sv >> 32) != kInf) want = push_op(a, q, e.u, sv, e.v, a.stamp0, og);
             }
         }
         const uint64_t mask = __ballot(want);
@@ -636,6 +687,7 @@ static TravArgs base_args(Engine &e) {
     a.n = e.n;
     a.Q = e.Q;
     a.st = e.d_st.p;
+    a.par = e.d_par.p;
     a.tick = e.d_tick.p;
     a.sources = e.d_sources.p;
     a.qapp = e.d_qapps.p;
@@ -679,6 +731,11 @@ void Engine::alloc_state() {
     } else {
         d_st.alloc(cap);
         d_aff.alloc(cap);
+        for (uint32_t q = 0; q < Q; ++q)
+            if (qapps[q] == DCG_APP_CC) {
+                d_par.alloc(cap);
+                break;
+            }
     }
 }
 
@@ -748,7 +805,8 @@ void Engine::autotune(dcg_tune_result *res) {
 
 void Engine::start_min_lattice() {
     const uint64_t qn = (uint64_t)n * Q;
-    hipLaunchKernelGGL(k_init_state, dim3(grid_for(qn)), dim3(256), 0, stream, d_st.p, d_tick.p, n, Q, d_qapps.p,
+    hipLaunchKernelGGL(k_init_state, dim3(grid_for(qn)), dim3(256), 0, stream, d_st.p, d_par.p, d_tick.p, n, Q,
+                       d_qapps.p,
                        d_sources.p);
     stamp = 1;
     std::vector<uint64_t> slot0(Q);

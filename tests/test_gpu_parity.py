@@ -10,7 +10,7 @@ import pytest
 
 import oracle
 from tests.conftest import announce
-from tests.helpers import oracle_values, rmat_graph, same_rows
+from tests.helpers import cc_parents_ok, oracle_values, rmat_graph, same_rows
 
 pytestmark = pytest.mark.gpu
 
@@ -50,6 +50,8 @@ def _run_stream(app, n, src, dst, adds_per, dels_per, placement, cache_bytes=0, 
             chk = oracle.check_sssp(n, rp, col, source, got, eng.gather_parent(),
                                     wmode if app == "sssp" else oracle.W_UNIT, seed, wmax)
             assert chk["invalid_parents"] == 0 and chk["relaxable_edges"] == 0, chk
+        else:
+            assert cc_parents_ok(n, rp, col, got, eng.gather_parent()), f"CC parents, delete stage {s}"
         st = eng.add_edge(ab, na)
         m = store.apply([], adds[ab:ab + na])
         assert st["invalid_additions"] == m["invalid_additions"]
@@ -64,6 +66,8 @@ def _run_stream(app, n, src, dst, adds_per, dels_per, placement, cache_bytes=0, 
             chk = oracle.check_sssp(n, rp, col, source, got, eng.gather_parent(),
                                     wmode if app == "sssp" else oracle.W_UNIT, seed, wmax)
             assert chk["invalid_parents"] == 0 and chk["relaxable_edges"] == 0, chk
+        else:
+            assert cc_parents_ok(n, rp, col, got, eng.gather_parent()), f"CC parents, add stage {s}"
         eng.cache_maintain()
         ab += na
         db += nd
