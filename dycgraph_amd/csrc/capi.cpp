@@ -439,6 +439,14 @@ int dcg_set_tuning(dcg_engine *h, uint32_t blocks_per_cu, uint32_t big_row, uint
     });
 }
 
+int dcg_set_buckets(dcg_engine *h, uint32_t quarters) {
+    return guarded(h, [&](Engine &e) {
+        if (quarters > 4096) throw Error(DCG_ERR_ARG, "bucket width out of range");
+        e.tune.bucket_quarters = quarters;
+        e.update_buckets();
+    });
+}
+
 int dcg_sync(dcg_engine *h) {
     return guarded(h, [&](Engine &e) { e.sync(); });
 }

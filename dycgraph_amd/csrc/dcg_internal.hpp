@@ -150,6 +150,9 @@ struct Ctl {
     uint32_t solo_round;      // round reached by a workgroup-0 stretch
     uint32_t solo_resume_b;   // that stretch stopped before the round's pieces
     unsigned long long bad;   // PR: non-finite values seen
+    uint32_t far_count[2];    // entries of the two far piles (bucketed push closure)
+    uint32_t far_par;         // pile that collects this epoch's far entries
+    uint32_t advances;        // bucket advances taken
 };
 
 struct BigEntry {

@@ -84,6 +84,7 @@ struct Engine {
         uint32_t blocks_per_cu = 2;   // persistent workgroups per CU
         uint32_t big_row = kBigRow;   // rows above this are split into pieces
         uint32_t piece = kPiece;
+        uint32_t bucket_quarters = 4;   // push-closure bucket width: SSSP w_max*q/4, BFS q; 0 = off
     } tune;
 
     uint32_t n = 0;
@@ -110,6 +111,9 @@ struct Engine {
     // ---- vertex state ----
     DevBuf<uint64_t> d_st;             // Q*n packed (value<<32 | parent); CC: (label<<32 | depth)
     DevBuf<uint32_t> d_par;            // Q*n CC parents (allocated when a query runs CC)
+    DevBuf<uint32_t> d_far0, d_far1, d_fstamp;   // bucketed push closure: far piles, pile epochs
+    DevBuf<uint32_t> d_thr, d_minfar, d_fdelta, d_epoch;
+    void update_buckets();
     DevBuf<uint32_t> d_tick;           // Q*n dedup stamps
     uint32_t stamp = 1;
     DevBuf<uint32_t> d_qa, d_qb, d_qc, d_aff;

@@ -52,6 +52,14 @@ struct TravArgs {
     uint32_t stamp0;
     uint32_t big_row;          // rows above this many entries are split into pieces
     uint32_t piece;            // piece size (power of two dividing 65,536)
+    // bucketed push closure: values at or above thr[q] wait in a far pile until
+    // the near frontier drains; null far0 turns it off
+    uint32_t *far0, *far1;
+    uint32_t *fstamp;          // q*n: epoch in which the entry entered a far pile
+    uint32_t *thr;             // per query: values below this are near
+    uint32_t *minfar;          // 2*Q per-query minima of the two far piles
+    const uint32_t *fdelta;    // per query bucket width (kInf: never far)
+    uint32_t *epoch;           // epoch counter (one word, monotone across launches)
     // PageRank
     float *rank, *res, *delta;
     uint32_t *queued;
@@ -121,32 +129,40 @@ __device__ __forceinline__ bool invalidate_op(const TravArgs &a, uint32_t q, uin
     return true;
 }
 
-__device__ __forceinline__ bool push_op(const TravArgs &a, uint32_t q, uint32_t s, uint64_t sv, uint32_t d,
-                                        uint32_t stamp, uint32_t &out_g) {
+// Relax s -> d.  Returns 0 (no change to publish), 1 (d joins the next
+// frontier) or 2 (d joins the far pile: bucketed closures only).
+__device__ __forceinline__ uint32_t push_op(const TravArgs &a, uint32_t q, uint32_t s, uint64_t sv, uint32_t d,
+                                            uint32_t stamp, uint32_t &out_g, bool bucket = true) {
     const uint32_t app = a.qapp[q];
     if (app == DCG_APP_CC) {
         // (label, depth + 1); the parent entry follows the winning word
         const uint64_t gd = (uint64_t)q * a.n + d;
         const uint64_t cand = sv + 1;
-        if (cand >= a.st[gd]) return false;
+        if (cand >= a.st[gd]) return 0;
         const uint64_t old = atomicMin((unsigned long long *)&a.st[gd], (unsigned long long)cand);
-        if (cand >= old) return false;
+        if (cand >= old) return 0;
         a.par[gd] = s;
-        if (atomicExch(&a.tick[gd], stamp) == stamp) return false;
+        if (atomicExch(&a.tick[gd], stamp) == stamp) return 0;
         out_g = (uint32_t)gd;
-        return true;
+        return 1;
     }
     const uint32_t vs = (uint32_t)(sv >> 32);
     const uint64_t cand = (uint64_t)vs + qweight(a, app, s, d);
-    if (cand >= kInf) return false;
+    if (cand >= kInf) return 0;
     const uint64_t gd = (uint64_t)q * a.n + d;
     const uint64_t cur = a.st[gd];
-    if (cand >= (cur >> 32)) return false;
+    if (cand >= (cur >> 32)) return 0;
     const uint64_t old = atomicMin((unsigned long long *)&a.st[gd], (unsigned long long)((cand << 32) | s));
-    if (cand >= (old >> 32)) return false;
-    if (atomicExch(&a.tick[gd], stamp) == stamp) return false;
+    if (cand >= (old >> 32)) return 0;
+    if (bucket && a.far0 && (uint32_t)cand >= a.thr[q]) {
+        const uint32_t ep = *(volatile uint32_t *)a.epoch;
+        if (atomicExch(&a.fstamp[gd], ep) == ep) return 0;   // already in this epoch's pile
+        out_g = (uint32_t)gd;
+        return 2;
+    }
+    if (atomicExch(&a.tick[gd], stamp) == stamp) return 0;
     out_g = (uint32_t)gd;
-    return true;
+    return 1;
 }
 
 // Pull: the row owner `d` is the destination, `x` an in-neighbour.
@@ -190,15 +206,16 @@ __device__ __forceinline__ bool pr_add(const TravArgs &a, uint32_t d, float c, u
 }
 
 template <uint32_t MODE>
-__device__ __forceinline__ bool edge_op(const TravArgs &a, uint32_t q, uint32_t owner, uint64_t val, uint32_t x,
-                                        uint32_t stamp, uint32_t &out_g) {
-    if (x & kTomb) return false;   // deleted entry of a large row
+// 0: nothing to publish, 1: next frontier, 2: far pile (push only).
+__device__ __forceinline__ uint32_t edge_op(const TravArgs &a, uint32_t q, uint32_t owner, uint64_t val, uint32_t x,
+                                            uint32_t stamp, uint32_t &out_g) {
+    if (x & kTomb) return 0;   // deleted entry of a large row
     if (MODE == M_PUSH) return push_op(a, q, owner, val, x, stamp, out_g);
-    if (MODE == M_INVALIDATE) return invalidate_op(a, q, owner, val, x, out_g);
-    if (MODE == M_PR || MODE == M_PR_AMEND) return pr_add(a, x, __uint_as_float((uint32_t)val), out_g);
+    if (MODE == M_INVALIDATE) return invalidate_op(a, q, owner, val, x, out_g) ? 1u : 0u;
+    if (MODE == M_PR || MODE == M_PR_AMEND) return pr_add(a, x, __uint_as_float((uint32_t)val), out_g) ? 1u : 0u;
     if (a.qapp[q] == DCG_APP_CC) pull_cc(a, q, owner, x);
     else pull_apply(a, q, owner, pull_cand(a, q, owner, x));
-    return false;
+    return 0;
 }
 
 // Append the lanes' results to the next frontier (and, while invalidating, to
@@ -227,6 +244,27 @@ __device__ __forceinline__ void emit(const TravArgs &a, uint32_t *outq, uint32_t
             else a.ctl->overflow = 1;
         }
     }
+}
+
+// Append to one of the far piles with one atomic per wave.
+__device__ __forceinline__ void emit_to(const TravArgs &a, uint32_t *pile, uint32_t *count, bool want, uint32_t g) {
+    const uint64_t mask = __ballot(want);
+    if (!mask) return;
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if ((int)lane_id() == leader) base = atomicAdd(count, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader, kWave);
+    if (want) {
+        const uint32_t pos = base + lanes_below(mask);
+        if (pos < a.qcap) pile[pos] = g;
+        else a.ctl->overflow = 1;
+    }
+}
+
+__device__ __forceinline__ void emit_far(const TravArgs &a, bool want, uint32_t g) {
+    if (!__ballot(want)) return;
+    const uint32_t fp = *(volatile uint32_t *)&a.ctl->far_par;
+    emit_to(a, fp ? a.far1 : a.far0, &a.ctl->far_count[fp], want, g);
 }
 
 struct WaveLds {
@@ -320,7 +358,7 @@ __device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L
         edges += total;
         for (uint32_t k0 = 0; k0 < total; k0 += kWave) {
             const uint32_t k = k0 + lane;
-            bool want = false;
+            uint32_t kind = 0;
             uint32_t og = 0;
             if (k < total) {
                 uint32_t lo = 0, hi = 64;   // last owner with pref <= k
@@ -330,9 +368,10 @@ __device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L
                     else hi = mid;
                 }
                 const uint32_t x = L.base[lo][k - L.pref[lo]];
-                want = edge_op<MODE>(a, L.q[lo], L.owner[lo], L.val[lo], x, stamp, og);
+                kind = edge_op<MODE>(a, L.q[lo], L.owner[lo], L.val[lo], x, stamp, og);
             }
-            if (MODE != M_PULL) emit<MODE>(a, out, &a.ctl->count[nxt], want, og);
+            if (MODE != M_PULL) emit<MODE>(a, out, &a.ctl->count[nxt], kind == 1, og);
+            if (MODE == M_PUSH && a.far0) emit_far(a, kind == 2, og);
         }
         __builtin_amdgcn_wave_barrier();
         uint32_t nb = 0;
@@ -391,10 +430,11 @@ __device__ void phase_b(const TravArgs &a, uint32_t r, const Team &t, uint64_t &
             } else {
                 for (uint32_t k0 = 0; k0 < len; k0 += kWave) {
                     const uint32_t k = k0 + lane;
-                    bool want = false;
+                    uint32_t kind = 0;
                     uint32_t og = 0;
-                    if (k < len) want = edge_op<MODE>(a, q, v, be.val, base[k], stamp, og);
-                    emit<MODE>(a, out, &a.ctl->count[nxt], want, og);
+                    if (k < len) kind = edge_op<MODE>(a, q, v, be.val, base[k], stamp, og);
+                    emit<MODE>(a, out, &a.ctl->count[nxt], kind == 1, og);
+                    if (MODE == M_PUSH && a.far0) emit_far(a, kind == 2, og);
                 }
             }
         }
@@ -439,6 +479,64 @@ __device__ __forceinline__ bool pieces_pending(const TravArgs &a, uint32_t r) {
     return (*(volatile unsigned long long *)&a.ctl->big[r % 3] & kBigMask) != 0;
 }
 
+__device__ __forceinline__ uint32_t thr_of(uint32_t minv, uint32_t delta) {
+    if (delta == kInf) return kInf;
+    const uint64_t t = (uint64_t)minv + delta;
+    return t >= kInf ? kInf : (uint32_t)t;
+}
+
+// Bucket advance of a push closure whose near frontier drained (round r found
+// no entries) while the far pile holds some: every query's threshold moves to
+// its smallest far value + its bucket width, entries now below it become round
+// r's frontier and the rest move to the other pile for the next epoch.  Called
+// by the whole grid; returns false when the far pile is empty (closure done).
+__device__ bool far_advance(const TravArgs &a, uint32_t r, const Team &t, cg::grid_group &grid) {
+    const uint32_t fp = *(volatile uint32_t *)&a.ctl->far_par;
+    uint32_t fc = *(volatile uint32_t *)&a.ctl->far_count[fp];
+    if (fc == 0 || r == 0) return false;
+    if (fc > a.qcap) fc = (uint32_t)a.qcap;
+    const uint32_t *src = fp ? a.far1 : a.far0;
+    uint32_t *dst = fp ? a.far0 : a.far1;
+    uint32_t *mf = a.minfar + (uint64_t)fp * a.Q;
+    for (uint32_t i = t.tid; i < fc; i += t.nthreads) {
+        uint32_t q, v;
+        const uint32_t g = src[i];
+        decode_g(a, g, q, v);
+        atomicMin(&mf[q], (uint32_t)(a.st[g] >> 32));
+    }
+    grid.sync();
+    const uint32_t ep = *(volatile uint32_t *)a.epoch + 1;
+    if (t.tid == 0) {
+        a.ctl->far_count[fp] = 0;
+        a.ctl->far_par = fp ^ 1;
+        a.ctl->advances += 1;
+    }
+    for (uint32_t q = t.tid; q < a.Q; q += t.nthreads) {
+        a.thr[q] = thr_of(mf[q], a.fdelta[q]);
+        a.minfar[(uint64_t)(fp ^ 1) * a.Q + q] = kInf;
+    }
+    uint32_t *in = (r & 1) ? a.qa : a.qb;   // round r's input queue
+    const uint32_t lane = lane_id();
+    for (uint32_t b = t.wave * kWave; b < fc; b += t.nwaves * kWave) {
+        const uint32_t i = b + lane;
+        bool near = false, far = false;
+        uint32_t g = 0;
+        if (i < fc) {
+            uint32_t q, v;
+            g = src[i];
+            decode_g(a, g, q, v);
+            near = (uint32_t)(a.st[g] >> 32) < thr_of(mf[q], a.fdelta[q]);
+            far = !near;
+            if (far) a.fstamp[g] = ep;
+        }
+        emit<M_PUSH>(a, in, &a.ctl->count[r % 3], near, g);
+        emit_to(a, dst, &a.ctl->far_count[fp ^ 1], far, g);
+    }
+    if (t.tid == 0) *(volatile uint32_t *)a.epoch = ep;
+    grid.sync();
+    return true;
+}
+
 constexpr uint32_t kSoloMax = 1024;      // frontier entries workgroup 0 handles alone
 constexpr uint32_t kSoloPieces = 64;     // more pending pieces than this end a solo stretch
 
@@ -457,10 +555,20 @@ __global__ __launch_bounds__(256) void k_persistent(TravArgs a) {
                   blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x};
     const Team bt{threadIdx.x / kWave, blockDim.x / kWave, threadIdx.x, blockDim.x};
     uint64_t edges = 0, expansions = 0;
-    uint32_t r = 0;
+    uint32_t r = 0, idle_advances = 0;
     while (r < a.max_rounds) {
         const uint32_t cnt = *(volatile uint32_t *)&a.ctl->count[r % 3];
-        if (cnt == 0) break;
+        if (cnt == 0) {
+            // an advance always moves the smallest far entry below the new
+            // threshold; two in a row without a round would mean none moved
+            if (MODE == M_PUSH && a.far0 && idle_advances < 2 && far_advance(a, r, gt, grid)) {
+                ++idle_advances;
+                continue;
+            }
+            if (idle_advances >= 2 && threadIdx.x == 0 && blockIdx.x == 0) a.ctl->overflow = 1;
+            break;
+        }
+        idle_advances = 0;
         if (cnt <= kSoloMax && gridDim.x > 1) {
             if (blockIdx.x == 0) {
                 // solo stretch: rounds on workgroup 0 until the frontier grows,
@@ -532,6 +640,19 @@ __global__ void k_ctl_setup(Ctl *c, const uint32_t *src0, uint32_t add0, uint32_
     c->bad = 0;
     c->solo_round = 0;
     c->solo_resume_b = 0;
+    c->far_count[0] = c->far_count[1] = 0;
+    c->far_par = 0;
+    c->advances = 0;
+}
+
+// Bucket thresholds of a push closure: 0 (everything relaxed waits in the far
+// pile until the first advance) or INF for queries without buckets.
+__global__ void k_far_setup(uint32_t *thr, uint32_t *minfar, const uint32_t *fdelta, uint32_t Q) {
+    for (uint32_t q = threadIdx.x; q < Q; q += blockDim.x) {
+        thr[q] = fdelta[q] == kInf ? kInf : 0u;
+        minfar[q] = kInf;
+        minfar[Q + q] = kInf;
+    }
 }
 
 __global__ void k_init_state(uint64_t *st, uint32_t *par, uint32_t *tick, uint32_t n, uint32_t Q,
@@ -620,7 +741,7 @@ __global__ void k_seed_adds(TravArgs a, const dcg_edge *recs, uint64_t count, ui
             const dcg_edge e = recs[i - (uint64_t)q * count];
             if (e.u < a.n && e.v < a.n) {
                 const uint64_t sv = a.st[(uint64_t)q * a.n + e.u];
-                if ((uint32_t)(sv >> 32) != kInf) want = push_op(a, q, e.u, sv, e.v, a.stamp0, og);
+                if ((uint32_t)(sv >> 32) != kInf) want = push_op(a, q, e.u, sv, e.v, a.stamp0, og, false) == 1;
             }
         }
         const uint64_t mask = __ballot(want);
@@ -663,6 +784,8 @@ void launch_persistent(Engine &e, uint32_t mode, TravArgs &a) {
     if (!e.coop_grid[mode]) e.coop_grid[mode] = e.coop_blocks(fn);
     void *args[] = {&a};
     static const int kid[5] = {DCG_K_PUSH, DCG_K_PULL, DCG_K_INVALIDATE, DCG_K_PR_SOLVE, DCG_K_PR_AMEND};
+    if (mode == M_PUSH && a.far0)
+        hipLaunchKernelGGL(k_far_setup, dim3(1), dim3(64), 0, e.stream, a.thr, a.minfar, a.fdelta, a.Q);
     const int slot = e.time_begin(kid[mode], (int)(a.ctl - e.d_ctl.p));
     DCG_HIP(hipLaunchCooperativeKernel(fn, dim3(e.coop_grid[mode]), dim3(kBlock), args, 0, e.stream));
     e.time_end(slot);
@@ -709,7 +832,33 @@ static TravArgs base_args(Engine &e) {
     a.sign = 1.f;
     a.big_row = e.tune.big_row;
     a.piece = e.tune.piece;
+    if (e.tune.bucket_quarters && e.d_far0.p) {
+        a.far0 = e.d_far0.p;
+        a.far1 = e.d_far1.p;
+        a.fstamp = e.d_fstamp.p;
+        a.thr = e.d_thr.p;
+        a.minfar = e.d_minfar.p;
+        a.fdelta = e.d_fdelta.p;
+        a.epoch = e.d_epoch.p;
+    }
     return a;
+}
+
+// Per-query bucket widths of the push closure: SSSP w_max * q / 4 (the
+// reference's ordered repair uses 128 for its (u+v)%128+1 weights,
+// ordered_gpu_repair.cuh:116-160), BFS q levels, CC none.
+void Engine::update_buckets() {
+    if (!d_fdelta.p) return;
+    const uint32_t wmax = wp.kind == W_SEEDED ? wp.wmax : 128u;
+    const uint32_t qt = tune.bucket_quarters;
+    std::vector<uint32_t> h(Q);
+    for (uint32_t q = 0; q < Q; ++q) {
+        if (qapps[q] == DCG_APP_CC || !qt) h[q] = kInf;
+        else if (qapps[q] == DCG_APP_BFS) h[q] = qt;
+        else h[q] = std::max<uint32_t>(1u, (uint32_t)(((uint64_t)wmax * qt) / 4));
+    }
+    DCG_HIP(hipMemcpyAsync(d_fdelta.p, h.data(), Q * 4, hipMemcpyHostToDevice, stream));
+    sync();
 }
 
 void Engine::alloc_state() {
@@ -731,11 +880,21 @@ void Engine::alloc_state() {
     } else {
         d_st.alloc(cap);
         d_aff.alloc(cap);
-        for (uint32_t q = 0; q < Q; ++q)
-            if (qapps[q] == DCG_APP_CC) {
-                d_par.alloc(cap);
-                break;
-            }
+        bool cc = false, ordered = false;
+        for (uint32_t q = 0; q < Q; ++q) (qapps[q] == DCG_APP_CC ? cc : ordered) = true;
+        if (cc) d_par.alloc(cap);
+        if (ordered) {
+            d_far0.alloc(cap);
+            d_far1.alloc(cap);
+            d_fstamp.alloc(cap);
+            fill<uint32_t>(stream, d_fstamp.p, cap, 0u);
+            d_thr.alloc(Q);
+            d_minfar.alloc(2ull * Q);
+            d_fdelta.alloc(Q);
+            d_epoch.alloc(1);
+            fill<uint32_t>(stream, d_epoch.p, 1, 1u);
+            update_buckets();
+        }
     }
 }
 
@@ -760,7 +919,23 @@ void Engine::autotune(dcg_tune_result *res) {
     const uint32_t bpc[] = {1, 2, 3, 4};
     const uint32_t rows[] = {64, 128, 256, 512, 1024, 4096};
     const uint32_t pieces[] = {256, 512, 1024, 2048, 4096};
+    const uint32_t quarters[] = {0, 1, 4, 16, 64};
     for (int pass = 0; pass < 2; ++pass) {
+        if (d_fdelta.p)
+            for (uint32_t v : quarters) {
+                if (v == tune.bucket_quarters) continue;
+                const uint32_t old = tune.bucket_quarters;
+                tune.bucket_quarters = v;
+                update_buckets();
+                const double t = measure();
+                evals++;
+                if (t < best) {
+                    best = t;
+                } else {
+                    tune.bucket_quarters = old;
+                    update_buckets();
+                }
+            }
         for (uint32_t v : bpc) {
             if (v == tune.blocks_per_cu) continue;
             const uint32_t old = tune.blocks_per_cu;
@@ -800,6 +975,7 @@ void Engine::autotune(dcg_tune_result *res) {
         res->piece = tune.piece;
         res->evaluations = evals;
         res->best_ms = best;
+        res->bucket_quarters = tune.bucket_quarters;
     }
 }
 

@@ -80,7 +80,7 @@ KERNELS = {"push": 0, "pull": 1, "invalidate": 2, "pr_solve": 3, "pr_amend": 4, 
 
 class TuneResult(C.Structure):
     _fields_ = [("blocks_per_cu", C.c_uint32), ("big_row", C.c_uint32), ("piece", C.c_uint32),
-                ("evaluations", C.c_uint32), ("best_ms", C.c_double)]
+                ("evaluations", C.c_uint32), ("best_ms", C.c_double), ("bucket_quarters", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -134,6 +134,7 @@ def lib() -> C.CDLL:
         "dcg_get_kernel_stats": (I, [P, U32, C.POINTER(KernelStats)]),
         "dcg_autotune": (I, [P, C.POINTER(TuneResult)]),
         "dcg_set_tuning": (I, [P, U32, U32, U32]),
+        "dcg_set_buckets": (I, [P, U32]),
         "dcg_reset_kernel_stats": (I, [P]),
         "dcg_mlp_eval": (I, [C.c_int32, U32, P, P, P, P, P]),
         "dcg_gen_rmat": (I, [U32, U64, C.c_double, C.c_double, C.c_double, U64, P, P]),
@@ -408,8 +409,10 @@ class Engine:
         self._check(lib().dcg_autotune(self._h, C.byref(st)), "dcg_autotune")
         return st.as_dict()
 
-    def set_tuning(self, blocks_per_cu=0, big_row=0, piece=0):
+    def set_tuning(self, blocks_per_cu=0, big_row=0, piece=0, bucket_quarters=None):
         self._check(lib().dcg_set_tuning(self._h, blocks_per_cu, big_row, piece), "dcg_set_tuning")
+        if bucket_quarters is not None:
+            self._check(lib().dcg_set_buckets(self._h, bucket_quarters), "dcg_set_buckets")
 
     def reset_kernel_stats(self):
         self._check(lib().dcg_reset_kernel_stats(self._h), "dcg_reset_kernel_stats")

@@ -211,17 +211,25 @@ int dcg_mlp_eval(int32_t device, uint32_t rows, const float *x, const float *y, 
                  float *out, float *grad);
 
 /* Autotuner (no reference counterpart; the reference fixes block 512 and its
- * CTA/warp thresholds, include/utils/graphs/traversal.h:80-90): times the
- * persistent closure on the loaded graph for candidate workgroups per CU,
- * large-row thresholds and piece sizes (coordinate descent) and keeps the
- * fastest.  Call after dcg_start; leaves the converged state unchanged. */
+ * CTA/warp thresholds, include/utils/graphs/traversal.h:80-90, and its ordered
+ * repair's bucket width 128, include/framework/ordered_gpu_repair.cuh:116-160):
+ * times the persistent closure on the loaded graph for candidate workgroups per
+ * CU, large-row thresholds, piece sizes and bucket widths (coordinate descent)
+ * and keeps the fastest.  Call after dcg_start; leaves the converged state
+ * unchanged. */
 typedef struct {
     uint32_t blocks_per_cu, big_row, piece, evaluations;
     double best_ms;
+    uint32_t bucket_quarters;
 } dcg_tune_result;
 int dcg_autotune(dcg_engine *eng, dcg_tune_result *out);
 /* Set the tuned parameters directly (0 keeps a value). */
 int dcg_set_tuning(dcg_engine *eng, uint32_t blocks_per_cu, uint32_t big_row, uint32_t piece);
+/* Bucket width of the push closure (ordered repair, ordered_gpu_repair.cuh:116-160,
+ * used by every SSSP/BFS closure here): SSSP queries w_max*quarters/4, BFS
+ * queries `quarters` levels; 0 turns the buckets off (plain frontier order).
+ * Values never depend on it. */
+int dcg_set_buckets(dcg_engine *eng, uint32_t quarters);
 
 /* Link microbenchmarks on `device`: HBM copy rate (read+write bytes / time of a
  * `bytes`-sized copy kernel) and zero-copy read rate of pinned host memory. */

@@ -289,3 +289,55 @@ def test_sorted_large_rows_tombstones(placement):
         ab += len(al)
         db += len(dl)
     eng.close()
+
+
+@pytest.mark.parametrize("quarters", [0, 1, 4, 64])
+def test_grid_mixed_queries_bucket_widths(quarters):
+    """Road-like grid (4-neighbour + diagonal shortcuts, weights 1..1000), 3 SSSP
+    + 3 BFS sources + CC in one engine, lognormal batch sizes: every query
+    bit-exact after every stage for every bucket width of the push closure
+    (0 = plain frontier order) and CC parents rooted witnesses."""
+    announce(f"grid 96x96 mixed queries, bucket quarters {quarters}")
+    import dycgraph_amd as d
+    from tests.helpers import cc_parents_ok
+    side = 96
+    src, dst = d.gen_grid(side, 0.05, seed=4)
+    n = side * side
+    sizes = d.gen_lognormal_sizes(4, 5.0, 1.0, 20, 600, seed=2).astype(np.int64)
+    ap, dp = sizes - sizes // 2, sizes // 2
+    adds, dels = d.gen_stream(n, src, dst, ap, dp, seed=8)
+    apps = ("sssp", "bfs", "sssp", "bfs", "sssp", "bfs", "cc")
+    sources = (0, 0, 4000, 4000, 9215, 123, 0)
+    eng = _engine(app="sssp", sources=sources, query_apps=apps, weight_mode=1, weight_seed=3, weight_max=1000,
+                  placement=0)
+    eng.set_tuning(bucket_quarters=quarters)
+    eng.load_graph(n, src, dst)
+    eng.start()
+    eng.load_updates(adds, dels)
+    store = oracle.Store(n, src, dst)
+
+    def check(stage):
+        rp, col = store.csr()
+        for q, (app, s) in enumerate(zip(apps, sources)):
+            want = oracle_values(app, n, rp, col, s, oracle.W_SEEDED, 3, 1000)
+            got = eng.gather_value(q)
+            assert np.array_equal(got, want), (stage, q, app, int(np.sum(got != want)))
+            if app == "cc":
+                assert cc_parents_ok(n, rp, col, got, eng.gather_parent(q)), (stage, "cc parents")
+            else:
+                chk = oracle.check_sssp(n, rp, col, s, got, eng.gather_parent(q),
+                                        oracle.W_SEEDED if app == "sssp" else oracle.W_UNIT, 3, 1000)
+                assert chk["invalid_parents"] == 0 and chk["relaxable_edges"] == 0, (stage, q, chk)
+
+    check("static")
+    ab = db = 0
+    for s in range(len(sizes)):
+        eng.del_edge(db, int(dp[s]))
+        store.apply(dels[db:db + dp[s]], [])
+        check(f"delete {s}")
+        eng.add_edge(ab, int(ap[s]))
+        store.apply([], adds[ab:ab + ap[s]])
+        check(f"add {s}")
+        ab += int(ap[s])
+        db += int(dp[s])
+    eng.close()
