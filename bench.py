@@ -111,6 +111,8 @@ def run_gpu(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, policy, queries
     sources, programs = queries if queries is not None else queries_of(cfg, n, src)
     ec = engine_config(cfg, n, policy, sources, programs)
     eng = d.Engine(ec)
+    if cfg.get("buckets") is not None:
+        eng.set_tuning(bucket_quarters=cfg["buckets"])
     t0 = time.time()
     eng.load_graph(n, src, dst)
     eng.start()
@@ -243,12 +245,14 @@ def main():
     ap.add_argument("--check", action="store_true", help="verify the final state against the oracle")
     ap.add_argument("--no-compare", action="store_true",
                     help="with the learned policy, skip the hotness/LRU run on the same stream")
+    ap.add_argument("--buckets", type=int, default=None,
+                    help="push-closure bucket width in quarters (0 off; default: the engine's automatic choice)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
                           "n_gpus": args.gpus}))
         sys.exit(2)
-    cfg = CONFIGS[args.config]
+    cfg = dict(CONFIGS[args.config], buckets=args.buckets)
     warmup = args.warmup
     steps = args.steps if args.steps is not None else cfg["snapshots"] - warmup
     snapshots = warmup + steps

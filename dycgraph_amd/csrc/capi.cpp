@@ -441,7 +441,7 @@ int dcg_set_tuning(dcg_engine *h, uint32_t blocks_per_cu, uint32_t big_row, uint
 
 int dcg_set_buckets(dcg_engine *h, uint32_t quarters) {
     return guarded(h, [&](Engine &e) {
-        if (quarters > 4096) throw Error(DCG_ERR_ARG, "bucket width out of range");
+        if (quarters > 4096 && quarters != ~0u) throw Error(DCG_ERR_ARG, "bucket width out of range");
         e.tune.bucket_quarters = quarters;
         e.update_buckets();
     });

@@ -84,7 +84,7 @@ struct Engine {
         uint32_t blocks_per_cu = 2;   // persistent workgroups per CU
         uint32_t big_row = kBigRow;   // rows above this are split into pieces
         uint32_t piece = kPiece;
-        uint32_t bucket_quarters = 4;   // push-closure bucket width: SSSP w_max*q/4, BFS q; 0 = off
+        uint32_t bucket_quarters = ~0u;   // push-closure bucket width: SSSP w_max*q/4, BFS q; 0 off, ~0 auto
     } tune;
 
     uint32_t n = 0;
@@ -114,6 +114,14 @@ struct Engine {
     DevBuf<uint32_t> d_far0, d_far1, d_fstamp;   // bucketed push closure: far piles, pile epochs
     DevBuf<uint32_t> d_thr, d_minfar, d_fdelta, d_epoch;
     void update_buckets();
+    uint32_t max_out_deg = 0;          // largest out-degree at load
+    // Bucket width in use: the set one, or (auto) 4 quarters on graphs whose
+    // largest out-degree is at most 64 (grids, road networks: high diameter,
+    // where the ordered closure pays) and none otherwise.
+    uint32_t bucket_width() const {
+        if (tune.bucket_quarters != ~0u) return tune.bucket_quarters;
+        return max_out_deg <= 64 ? 4u : 0u;
+    }
     DevBuf<uint32_t> d_tick;           // Q*n dedup stamps
     uint32_t stamp = 1;
     DevBuf<uint32_t> d_qa, d_qb, d_qc, d_aff;

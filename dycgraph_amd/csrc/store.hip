@@ -137,6 +137,17 @@ __global__ void k_sorted_prefix(const uint32_t *keys, const uint32_t *vals, uint
     }
 }
 
+__global__ void k_max_u32(const uint32_t *a, uint64_t k, uint32_t *out) {
+    uint32_t m = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < k; i += (uint64_t)gridDim.x * blockDim.x)
+        m = a[i] > m ? a[i] : m;
+    for (int d = 32; d >= 1; d >>= 1) {
+        const uint32_t o = __shfl_xor(m, d, kWave);
+        m = o > m ? o : m;
+    }
+    if (lane_id() == 0 && m) atomicMax(out, m);
+}
+
 __global__ void k_copy_u32(const uint32_t *a, uint32_t *b, uint64_t k) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < k; i += (uint64_t)gridDim.x * blockDim.x)
         b[i] = a[i];
@@ -160,11 +171,17 @@ void Engine::load_edges_device(uint32_t nn, const uint32_t *d_src, const uint32_
     deg.alloc(rows ? rows : 1);
     fill<uint32_t>(stream, deg.p, rows, 0u);
     hipLaunchKernelGGL(k_check_ids, dim3(grid_for(m)), dim3(256), 0, stream, d_src, d_dst, m, n, d_counters.p, deg.p);
+    DevBuf<uint32_t> dmax;
+    dmax.alloc(1);
+    fill<uint32_t>(stream, dmax.p, 1, 0u);
+    hipLaunchKernelGGL(k_max_u32, dim3(grid_for(n)), dim3(256), 0, stream, deg.p, (uint64_t)n, dmax.p);
     DCG_HIP(hipGetLastError());
     unsigned long long bad = 0;
     DCG_HIP(hipMemcpyAsync(&bad, d_counters.p + C_BAD_ID, sizeof(bad), hipMemcpyDeviceToHost, stream));
     sync();
     if (bad) throw Error(DCG_ERR_ARG, "graph has " + std::to_string(bad) + " edges with an id >= num_nodes");
+    DCG_HIP(hipMemcpyAsync(&max_out_deg, dmax.p, 4, hipMemcpyDeviceToHost, stream));
+    sync();
 
     // Row pointers of the out- and in-CSR (64-bit exclusive scans of degrees).
     DevBuf<uint64_t> rp;

@@ -832,7 +832,7 @@ static TravArgs base_args(Engine &e) {
     a.sign = 1.f;
     a.big_row = e.tune.big_row;
     a.piece = e.tune.piece;
-    if (e.tune.bucket_quarters && e.d_far0.p) {
+    if (e.bucket_width() && e.d_far0.p) {
         a.far0 = e.d_far0.p;
         a.far1 = e.d_far1.p;
         a.fstamp = e.d_fstamp.p;
@@ -850,7 +850,7 @@ static TravArgs base_args(Engine &e) {
 void Engine::update_buckets() {
     if (!d_fdelta.p) return;
     const uint32_t wmax = wp.kind == W_SEEDED ? wp.wmax : 128u;
-    const uint32_t qt = tune.bucket_quarters;
+    const uint32_t qt = bucket_width();
     std::vector<uint32_t> h(Q);
     for (uint32_t q = 0; q < Q; ++q) {
         if (qapps[q] == DCG_APP_CC || !qt) h[q] = kInf;
@@ -920,6 +920,7 @@ void Engine::autotune(dcg_tune_result *res) {
     const uint32_t rows[] = {64, 128, 256, 512, 1024, 4096};
     const uint32_t pieces[] = {256, 512, 1024, 2048, 4096};
     const uint32_t quarters[] = {0, 1, 4, 16, 64};
+    tune.bucket_quarters = bucket_width();   // tune from the width in use
     for (int pass = 0; pass < 2; ++pass) {
         if (d_fdelta.p)
             for (uint32_t v : quarters) {

@@ -227,8 +227,9 @@ int dcg_autotune(dcg_engine *eng, dcg_tune_result *out);
 int dcg_set_tuning(dcg_engine *eng, uint32_t blocks_per_cu, uint32_t big_row, uint32_t piece);
 /* Bucket width of the push closure (ordered repair, ordered_gpu_repair.cuh:116-160,
  * used by every SSSP/BFS closure here): SSSP queries w_max*quarters/4, BFS
- * queries `quarters` levels; 0 turns the buckets off (plain frontier order).
- * Values never depend on it. */
+ * queries `quarters` levels; 0 turns the buckets off (plain frontier order);
+ * 0xFFFFFFFF (the default) picks 4 on graphs whose largest out-degree is at
+ * most 64 (grids, road networks) and 0 otherwise.  Values never depend on it. */
 int dcg_set_buckets(dcg_engine *eng, uint32_t quarters);
 
 /* Link microbenchmarks on `device`: HBM copy rate (read+write bytes / time of a

@@ -230,7 +230,7 @@ def test_large_row_relocation_exact_rows(placement):
 
 @pytest.mark.parametrize("placement", [0, 1])
 def test_sorted_large_rows_tombstones(placement):
-    """Rows whose sorted prefix holds at least 8,192 entries delete by binary
+    """Rows whose sorted prefix holds at least 1,024 entries delete by binary
     search and tombstones: first occurrences across the prefix and the appended
     tail, requests beyond the occurrences, tombstones carried through a
     relocation, and the compaction once tombstones pass a quarter of the row.
