@@ -584,7 +584,7 @@ __device__ uint32_t compact_range(const StoreView &s, uint64_t off, uint32_t fro
     return write;
 }
 
-constexpr uint32_t kSearchPrefix = 8192;
+constexpr uint32_t kSearchPrefix = 1024;   // sorted prefixes at least this long are binary-searched
 constexpr uint32_t kDelBigGrid = 512;      // two 1024-thread workgroups per CU   // sorted prefixes at least this long are binary-searched
 
 // Sum of a per-thread count over the 1024-thread workgroup (via LDS).
