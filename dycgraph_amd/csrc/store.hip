@@ -585,7 +585,7 @@ __device__ uint32_t compact_range(const StoreView &s, uint64_t off, uint32_t fro
 }
 
 constexpr uint32_t kSearchPrefix = 1024;   // sorted prefixes at least this long are binary-searched
-constexpr uint32_t kDelBigGrid = 512;      // two 1024-thread workgroups per CU   // sorted prefixes at least this long are binary-searched
+constexpr uint32_t kDelBigGrid = 512;      // two 1024-thread workgroups per CU
 
 // Sum of a per-thread count over the 1024-thread workgroup (via LDS).
 __device__ __forceinline__ uint32_t block_sum(uint32_t x, uint32_t &slot) {

@@ -99,7 +99,7 @@ def test_rmat16_sssp_seeded_weights():
 @pytest.mark.parametrize("dups", [False, True])
 @pytest.mark.parametrize("placement", [0, 1])
 def test_large_row_deletes_exact_rows(dups, placement):
-    """Rows above 8,192 entries take the workgroup delete path; with duplicate
+    """Rows above 1,024 entries take the workgroup delete path; with duplicate
     targets (more occurrences than requests) the exact first-occurrence path."""
     announce(f"large-row deletes dups={dups} placement={placement}")
     import dycgraph_amd as d
