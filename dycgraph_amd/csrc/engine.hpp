@@ -115,12 +115,12 @@ struct Engine {
     DevBuf<uint32_t> d_thr, d_minfar, d_fdelta, d_epoch;
     void update_buckets();
     uint32_t max_out_deg = 0;          // largest out-degree at load
-    // Bucket width in use: the set one, or (auto) 4 quarters on graphs whose
+    // Bucket width in use: the set one, or (auto) 64 quarters on graphs whose
     // largest out-degree is at most 64 (grids, road networks: high diameter,
     // where the ordered closure pays) and none otherwise.
     uint32_t bucket_width() const {
         if (tune.bucket_quarters != ~0u) return tune.bucket_quarters;
-        return max_out_deg <= 64 ? 4u : 0u;
+        return max_out_deg <= 64 ? 64u : 0u;
     }
     DevBuf<uint32_t> d_tick;           // Q*n dedup stamps
     uint32_t stamp = 1;

@@ -738,6 +738,44 @@ __global__ void k_alloc_classes(uint64_t *tail, uint64_t total) {
     else tail[T_TAIL] = t;
 }
 
+// New blocks of the rows that relocate: one pass per 256 groups reserves each
+// class's blocks with one global atomic per class (LDS ranks inside the
+// workgroup), so relocations do not serialise on the class cursors.  Same
+// relocation test as k_add_plan.
+__global__ __launch_bounds__(256) void k_add_assign(const RowDesc *desc, const uint32_t *keys, uint64_t count,
+                                                    const uint32_t *gstart, const unsigned long long *ngroups_p,
+                                                    uint64_t *tail, uint64_t *reloc) {
+    __shared__ uint32_t hist[kClasses];
+    __shared__ unsigned long long base[kClasses];
+    if (tail[T_OVERFLOW]) return;
+    const uint64_t ngroups = *ngroups_p;
+    for (uint64_t g0 = (uint64_t)blockIdx.x * blockDim.x; g0 < ngroups; g0 += (uint64_t)gridDim.x * blockDim.x) {
+        for (uint32_t i = threadIdx.x; i < kClasses; i += blockDim.x) hist[i] = 0;
+        __syncthreads();
+        const uint64_t g = g0 + threadIdx.x;
+        uint32_t cls = 0, rank = 0;
+        if (g < ngroups) {
+            const uint64_t gs = gstart[g];
+            const uint32_t row = keys[gs];
+            const uint64_t ge = group_end<uint32_t>(keys, gs, count, row);
+            const RowDesc d = desc[row];
+            const uint64_t nd = (uint64_t)d.ext + (ge - gs);
+            if (nd > row_cap(d)) {
+                uint64_t c = nd < kMinRowCap ? kMinRowCap : nd;
+                cls = 64u - (uint32_t)__clzll((long long)(c - 1));
+                rank = atomicAdd(&hist[cls], 1u);
+            }
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < kClasses; i += blockDim.x)
+            base[i] = hist[i] ? atomicAdd((unsigned long long *)&tail[T_CURSOR + i], (unsigned long long)hist[i] << i)
+                              : 0ull;
+        __syncthreads();
+        if (cls) reloc[g] = base[cls] + ((uint64_t)rank << cls);
+        __syncthreads();
+    }
+}
+
 constexpr uint32_t kBigAppend = 4096;   // relocations of larger rows use one 1024-thread workgroup
 
 __device__ __forceinline__ uint32_t class_of(uint64_t nd) {
@@ -749,9 +787,9 @@ __device__ __forceinline__ uint32_t class_of(uint64_t nd) {
 // Rows above kBigAppend entries that must relocate are listed for k_add_big.
 __global__ __launch_bounds__(256) void k_add_rows(StoreView s, RowDesc *desc, uint32_t *hot, const uint32_t *keys,
                                                   const uint32_t *vals, uint64_t count, const uint32_t *gstart,
-                                                  const unsigned long long *ngroups_p, uint64_t *tail,
-                                                  unsigned long long *counters, int count_stats,
-                                                  uint32_t *big_groups, unsigned long long *nbig) {
+                                                  const unsigned long long *ngroups_p, const uint64_t *tail,
+                                                  const uint64_t *reloc, unsigned long long *counters,
+                                                  int count_stats, uint32_t *big_groups, unsigned long long *nbig) {
     const uint32_t lane = lane_id();
     const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kWave;
     const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / kWave;
@@ -768,13 +806,11 @@ __global__ __launch_bounds__(256) void k_add_rows(StoreView s, RowDesc *desc, ui
         if (nd > row_cap(d)) {
             if (overflow) continue;
             if (d.ext > kBigAppend) {
-                if (lane == 0) big_groups[atomicAdd(nbig, 1ull)] = (uint32_t)gs;
+                if (lane == 0) big_groups[atomicAdd(nbig, 1ull)] = (uint32_t)g;
                 continue;
             }
             const uint32_t cls = class_of(nd);
-            unsigned long long noff = 0;
-            if (lane == 0) noff = atomicAdd((unsigned long long *)&tail[T_CURSOR + cls], 1ull << cls);
-            noff = __shfl(noff, 0, kWave);
+            const uint64_t noff = reloc[g];
             // entries (tombstones included) keep their positions, so the
             // sorted prefix length of the row stays valid
             for (uint32_t i = lane; i < d.ext; i += kWave) store_slot(s, noff + i, load_slot(s, d.off + i));
@@ -800,22 +836,21 @@ __global__ __launch_bounds__(256) void k_add_rows(StoreView s, RowDesc *desc, ui
 // Relocation of a large row: one 1024-thread workgroup copies the row into its
 // new block (16-byte accesses where the row is aligned) and appends the batch.
 __global__ __launch_bounds__(1024) void k_add_big(StoreView s, RowDesc *desc, uint32_t *hot, const uint32_t *keys,
-                                                  const uint32_t *vals, uint64_t count, const uint32_t *big_groups,
-                                                  const unsigned long long *nbig_p, uint64_t *tail,
-                                                  unsigned long long *counters, int count_stats) {
-    __shared__ unsigned long long noff_s;
+                                                  const uint32_t *vals, uint64_t count, const uint32_t *gstart,
+                                                  const uint32_t *big_groups, const unsigned long long *nbig_p,
+                                                  const uint64_t *reloc, unsigned long long *counters,
+                                                  int count_stats) {
     const uint64_t nbig = *nbig_p;
     for (uint64_t b = blockIdx.x; b < nbig; b += gridDim.x) {
-        const uint64_t gs = big_groups[b];
+        const uint64_t g = big_groups[b];
+        const uint64_t gs = gstart[g];
         const uint32_t row = keys[gs];
         const uint64_t ge = group_end<uint32_t>(keys, gs, count, row);
         const uint32_t k = (uint32_t)(ge - gs);
         RowDesc d = desc[row];
         const uint64_t nd = (uint64_t)d.ext + k;
         const uint32_t cls = class_of(nd);
-        if (threadIdx.x == 0) noff_s = atomicAdd((unsigned long long *)&tail[T_CURSOR + cls], 1ull << cls);
-        __syncthreads();
-        const uint64_t noff = noff_s;
+        const uint64_t noff = reloc[g];
         // blocks above 65,536 slots are whole chunks; copy chunk segment by segment
         for (uint64_t seg = 0; seg < d.ext; seg += kChunkEdges) {
             const uint32_t len = (uint32_t)min((uint64_t)kChunkEdges, (uint64_t)d.ext - seg);
@@ -934,21 +969,28 @@ void Engine::mutate_add(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
     hipLaunchKernelGGL(k_add_plan, dim3(grid_for(count, 256, 1024)), dim3(256), 0, stream, d_in.p, d_k32b.p, count,
                        d_gstart2.p, d_counters.p + C_GROUPS_IN, d_tail.p);
     hipLaunchKernelGGL(k_alloc_classes, dim3(1), dim3(64), 0, stream, d_tail.p, total_slots);
+    d_k64a.ensure(count);
+    d_k64b.ensure(count);
+    uint64_t *reloc_out = d_k64a.p, *reloc_in = d_k64b.p;
+    hipLaunchKernelGGL(k_add_assign, dim3(grid_for(count, 256, 1024)), dim3(256), 0, stream, d_out.p, d_k32a.p, count,
+                       d_gstart.p, d_counters.p + C_GROUPS_OUT, d_tail.p, reloc_out);
+    hipLaunchKernelGGL(k_add_assign, dim3(grid_for(count, 256, 1024)), dim3(256), 0, stream, d_in.p, d_k32b.p, count,
+                       d_gstart2.p, d_counters.p + C_GROUPS_IN, d_tail.p, reloc_in);
     StoreView v = view();
     const uint32_t wg = wave_grid(count);
     d_big_groups.ensure(count);
     const int slot = time_begin(DCG_K_APPEND_ROWS, -1);
     hipLaunchKernelGGL(k_add_rows, dim3(wg), dim3(256), 0, stream, v, d_out.p, d_hot_out.p, d_k32a.p, d_v32a.p, count,
-                       d_gstart.p, d_counters.p + C_GROUPS_OUT, d_tail.p, d_counters.p, 1, d_big_groups.p,
+                       d_gstart.p, d_counters.p + C_GROUPS_OUT, d_tail.p, reloc_out, d_counters.p, 1, d_big_groups.p,
                        d_counters.p + C_NBIG);
     hipLaunchKernelGGL(k_add_big, dim3(64), dim3(1024), 0, stream, v, d_out.p, d_hot_out.p, d_k32a.p, d_v32a.p, count,
-                       d_big_groups.p, d_counters.p + C_NBIG, d_tail.p, d_counters.p, 1);
+                       d_gstart.p, d_big_groups.p, d_counters.p + C_NBIG, reloc_out, d_counters.p, 1);
     fill<unsigned long long>(stream, d_counters.p + C_NBIG, 1, 0ull);
     hipLaunchKernelGGL(k_add_rows, dim3(wg), dim3(256), 0, stream, v, d_in.p, d_hot_in.p, d_k32b.p, d_v32b.p, count,
-                       d_gstart2.p, d_counters.p + C_GROUPS_IN, d_tail.p, d_counters.p, 1, d_big_groups.p,
+                       d_gstart2.p, d_counters.p + C_GROUPS_IN, d_tail.p, reloc_in, d_counters.p, 1, d_big_groups.p,
                        d_counters.p + C_NBIG);
     hipLaunchKernelGGL(k_add_big, dim3(64), dim3(1024), 0, stream, v, d_in.p, d_hot_in.p, d_k32b.p, d_v32b.p, count,
-                       d_big_groups.p, d_counters.p + C_NBIG, d_tail.p, d_counters.p, 1);
+                       d_gstart2.p, d_big_groups.p, d_counters.p + C_NBIG, reloc_in, d_counters.p, 1);
     time_end(slot);
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipMemcpyAsync(h_counters, d_counters.p, C_NUM * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));

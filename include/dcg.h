@@ -228,7 +228,7 @@ int dcg_set_tuning(dcg_engine *eng, uint32_t blocks_per_cu, uint32_t big_row, ui
 /* Bucket width of the push closure (ordered repair, ordered_gpu_repair.cuh:116-160,
  * used by every SSSP/BFS closure here): SSSP queries w_max*quarters/4, BFS
  * queries `quarters` levels; 0 turns the buckets off (plain frontier order);
- * 0xFFFFFFFF (the default) picks 4 on graphs whose largest out-degree is at
+ * 0xFFFFFFFF (the default) picks 64 on graphs whose largest out-degree is at
  * most 64 (grids, road networks) and 0 otherwise.  Values never depend on it. */
 int dcg_set_buckets(dcg_engine *eng, uint32_t quarters);
 
