@@ -395,23 +395,23 @@ __device__ __forceinline__ uint64_t target_ub(const uint64_t *keys, uint64_t fro
 }
 
 struct DelResult {
-    uint32_t removed, moved, removed_in_prefix;
+    uint32_t end, removed, moved, removed_below;
 };
 
-// One wave: remove the first `requested` occurrences of every target of the
-// group from the row, survivors compacted in order.  Matches are resolved in
-// lane order through per-run counters, so the occurrences removed are exactly
-// the first ones (source_local_chunk_store.h:685-739).  The row holds no
-// tombstones (rows with tombstones take the large-row path).
-__device__ DelResult wave_delete_row(const StoreView &s, RowDesc *desc, uint32_t row, const RowDesc d,
-                                     uint32_t sorted_len, const uint64_t *keys, uint64_t gs, uint64_t ge,
-                                     uint32_t *used) {
+// One wave over slots [from, to) of a row: remove the first still-requested
+// occurrences of the group's targets, survivors compacted in order.  Matches
+// are resolved in lane order through per-run counters, so the occurrences
+// removed are exactly the first ones (source_local_chunk_store.h:685-739).
+// Tombstones never match a target and stay in place.  `removed_below` counts
+// removals at positions below `mark`.
+__device__ DelResult wave_delete_range(const StoreView &s, uint64_t off, uint32_t from, uint32_t to, uint32_t mark,
+                                       const uint64_t *keys, uint64_t gs, uint64_t ge, uint32_t *used) {
     const uint32_t lane = lane_id();
-    uint32_t write = 0, removed = 0, moved = 0, before = 0;
-    for (uint32_t t0 = 0; t0 < d.ext; t0 += kWave) {
+    uint32_t write = from, removed = 0, moved = 0, below = 0;
+    for (uint32_t t0 = from; t0 < to; t0 += kWave) {
         const uint32_t i = t0 + lane;
-        const bool valid = i < d.ext;
-        const uint32_t x = valid ? load_slot(s, d.off + i) : 0u;
+        const bool valid = i < to;
+        const uint32_t x = valid ? load_slot(s, off + i) : 0u;
         const uint64_t lo = target_lb(keys, gs, ge, x);
         const bool match = valid && lo < ge && (uint32_t)keys[lo] == x;
         bool rem = false;
@@ -428,24 +428,83 @@ __device__ DelResult wave_delete_row(const StoreView &s, RowDesc *desc, uint32_t
         const bool keep = valid && !rem;
         const uint64_t km = __ballot(keep);
         const uint32_t rtile = (uint32_t)__popcll(__ballot(rem));
-        before += (uint32_t)__popcll(__ballot(rem && i < sorted_len));
+        below += (uint32_t)__popcll(__ballot(rem && i < mark));
         if (removed + rtile) {
             const uint32_t pos = write + lanes_below(km);
             const bool mv = keep && pos != i;
-            if (mv) store_slot(s, d.off + pos, x);
+            if (mv) store_slot(s, off + pos, x);
             moved += (uint32_t)__popcll(__ballot(mv));
         }
         write += (uint32_t)__popcll(km);
         removed += rtile;
     }
-    if (lane == 0 && removed) desc[row].ext = write;
-    return DelResult{removed, moved, before};
+    return DelResult{write, removed, moved, below};
 }
 
-// One wave per touched row of at most kBigDelete slots without tombstones;
-// other rows are listed for k_del_big.  `sorted` holds each row's sorted
-// prefix length; removing entries keeps a prefix sorted, so it shrinks by the
-// number of entries removed from it.
+// One wave: drop the tombstones of slots [0, to), order kept.  Returns the new
+// end in .end and the live entries below `mark` in .removed_below.
+__device__ DelResult wave_compact_tombs(const StoreView &s, uint64_t off, uint32_t to, uint32_t mark) {
+    const uint32_t lane = lane_id();
+    uint32_t write = 0, moved = 0, live_below = 0;
+    for (uint32_t t0 = 0; t0 < to; t0 += kWave) {
+        const uint32_t i = t0 + lane;
+        const uint32_t x = i < to ? load_slot(s, off + i) : kTomb;
+        const bool keep = !(x & kTomb);
+        const uint64_t km = __ballot(keep);
+        live_below += (uint32_t)__popcll(__ballot(keep && i < mark));
+        const uint32_t pos = write + lanes_below(km);
+        const bool mv = keep && pos != i;
+        if (mv) store_slot(s, off + pos, x);
+        moved += (uint32_t)__popcll(__ballot(mv));
+        write += (uint32_t)__popcll(km);
+    }
+    return DelResult{write, 0, moved, live_below};
+}
+
+// Sorted prefix of the row [0, S): the head of every run of equal targets
+// (one lane each) binary-searches the run's value and tombstones its first
+// live occurrences (bit 31 set, id bits kept, so the prefix stays ordered).
+// used[head] records what each run took; returns the tombstones written and
+// (in probes) the slots read; need_tail tells whether a run wants more.
+__device__ __forceinline__ uint32_t search_prefix(const StoreView &s, uint64_t off, uint32_t S, const uint64_t *keys,
+                                                  uint64_t q, uint64_t gs, uint64_t ge, uint32_t *used,
+                                                  uint32_t &probes, uint32_t &need_tail) {
+    const uint32_t v = (uint32_t)keys[q];
+    if (q > gs && (uint32_t)keys[q - 1] == v) return 0;
+    const uint32_t c = (uint32_t)(target_ub(keys, q + 1, ge, v) - q);
+    uint32_t lo = 0, hi = S;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        probes++;
+        if ((load_slot(s, off + mid) & ~kTomb) < v) lo = mid + 1;
+        else hi = mid;
+    }
+    uint32_t taken = 0;
+    for (uint32_t j = lo; j < S && taken < c; ++j) {
+        const uint32_t x = load_slot(s, off + j);
+        probes++;
+        if ((x & ~kTomb) != v) break;
+        if (!(x & kTomb)) {
+            store_slot(s, off + j, x | kTomb);
+            taken++;
+        }
+    }
+    used[q] = taken;
+    need_tail |= taken < c ? 1u : 0u;
+    return taken;
+}
+
+constexpr uint32_t kWaveSearch = 64;   // sorted prefixes at least this long are binary-searched by the wave
+
+// One wave per touched row of at most kBigDelete slots; larger rows are listed
+// for k_del_big.  `sorted` holds each row's sorted prefix length S.
+//  * S >= kWaveSearch: search_prefix, one lane per run, then (only when a run
+//    still wants occurrences) wave_delete_range over the tail [S, ext), which
+//    holds appended entries and never tombstones.  The prefix precedes the
+//    tail in the row's sequence, so exactly the first occurrences go.
+//  * otherwise one wave_delete_range pass over the row; S shrinks by the
+//    entries removed below it (removal keeps a prefix sorted).
+//  * tombstones above a quarter of the row: wave_compact_tombs.
 __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, uint32_t *sorted, uint32_t *hot,
                                                   const uint64_t *keys, uint64_t count,
                                                   const uint32_t *gstart, const unsigned long long *ngroups_p,
@@ -460,20 +519,62 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
         const uint64_t gs = gstart[g];
         const uint32_t row = (uint32_t)(keys[gs] >> 32);
         const RowDesc d = desc[row];
-        if (d.ext > kBigDelete || row_tombs(d)) {
+        if (d.ext > kBigDelete) {
             if (lane == 0) big_groups[atomicAdd(nbig, 1ull)] = (uint32_t)gs;
             continue;
         }
         const uint64_t ge = group_end<uint64_t>(keys, gs, count, row);
-        const uint32_t sl = sorted[row];
-        const DelResult r = wave_delete_row(s, desc, row, d, sl, keys, gs, ge, used);
+        const uint32_t S = sorted[row] < d.ext ? sorted[row] : d.ext;
+        uint32_t ext = d.ext, S2 = S, removed = 0, moved = 0, new_tombs = 0, read = 0;
+        bool full = false;
+        if (S >= kWaveSearch) {
+            uint32_t probes = 0, need = 0;
+            for (uint64_t q0 = gs; q0 < ge; q0 += kWave)
+                if (q0 + lane < ge) new_tombs += search_prefix(s, d.off, S, keys, q0 + lane, gs, ge, used, probes, need);
+            for (int o = 32; o >= 1; o >>= 1) {
+                new_tombs += __shfl_xor(new_tombs, o, kWave);
+                probes += __shfl_xor(probes, o, kWave);
+            }
+            read += probes;
+            removed += new_tombs;
+            if (__ballot(need != 0) && S < ext) {
+                const DelResult r = wave_delete_range(s, d.off, S, ext, 0, keys, gs, ge, used);
+                read += ext - S;
+                ext = r.end;
+                removed += r.removed;
+                moved += r.moved;
+            }
+        } else {
+            const DelResult r = wave_delete_range(s, d.off, 0, ext, S, keys, gs, ge, used);
+            read += ext;
+            ext = r.end;
+            removed = r.removed;
+            moved = r.moved;
+            S2 = S - r.removed_below;
+            full = true;
+        }
+        uint32_t tombs = row_tombs(d) + new_tombs;
+        if (tombs && (uint64_t)tombs * 4 > ext) {
+            __threadfence_block();   // the lanes' tombstone stores complete before the pass reads them
+            const DelResult c = wave_compact_tombs(s, d.off, ext, S2);
+            read += ext;
+            ext = c.end;
+            S2 = c.removed_below;
+            moved += c.moved;
+            tombs = 0;
+            full = true;
+        }
         if (lane == 0) {
-            hot[row] += 1;   // the scan reads the whole row (cache-policy signal)
-            if (r.removed_in_prefix) sorted[row] = sl - r.removed_in_prefix;
-            missing += (ge - gs) - r.removed;
-            changed += r.removed ? 1 : 0;
-            rd += d.ext;
-            wr += r.moved;
+            if (ext != d.ext || tombs != row_tombs(d)) {
+                desc[row].ext = ext;
+                desc[row].meta = make_meta(d.meta & 63u, tombs);
+            }
+            if (S2 != S || S != sorted[row]) sorted[row] = S2;
+            if (full) hot[row] += 1;   // a pass read the whole row (cache-policy signal)
+            missing += (ge - gs) - removed;
+            changed += removed ? 1 : 0;
+            rd += read;
+            wr += moved + new_tombs;
         }
     }
     if (lane == 0) {

@@ -341,3 +341,63 @@ def test_grid_mixed_queries_bucket_widths(quarters):
         ab += int(ap[s])
         db += int(dp[s])
     eng.close()
+
+
+@pytest.mark.parametrize("placement", [0, 1])
+def test_sorted_medium_rows_wave_search(placement):
+    """Rows of 64..1,024 entries with sorted prefixes take the wave path's
+    binary search: first occurrences across prefix and tail (duplicates,
+    over-requests), tombstones, the wave compaction past a quarter of the row,
+    and rows that shrink below the search length.  Rows (tombstones skipped)
+    and BFS levels match the multiset model after every stage."""
+    announce(f"sorted medium rows, wave search, placement={placement}")
+    rng = np.random.default_rng(21)
+    n = 5000
+    src, dst = [], []
+    for u in range(60):
+        deg = int(rng.integers(70, 1000))
+        row = sorted(int(x) for x in rng.integers(0, n, deg))      # sorted, with repeats
+        src += [u] * len(row)
+        dst += row
+    src += [int(x) for x in rng.integers(60, n, 3000)]
+    dst += [int(x) for x in rng.integers(0, n, 3000)]
+    src, dst = np.array(src, np.uint32), np.array(dst, np.uint32)
+    store = oracle.Store(n, src, dst)
+    rp0, col0 = store.csr()
+    stages = []
+    for s in range(5):
+        dl, al = [], []
+        frac = 0.4 if s == 3 else 0.05
+        for u in range(60):
+            row = col0[rp0[u]:rp0[u + 1]]
+            k = int(len(row) * frac)
+            pick = [int(x) for x in rng.choice(row, k)] if k else []
+            dl += [(u, v) for v in pick]                             # repeats -> over-requests
+            al += [(u, int(x)) for x in rng.integers(0, n, 3)]
+            al += [(u, int(row[0]))] if len(row) else []             # a prefix value again, in the tail
+        dl += [(u, v) for u, v in al[:40]] if s else []              # tail hits of earlier appends
+        stages.append((dl, al))
+    adds = [e for _, a in stages for e in a]
+    dels = [e for d_, _ in stages for e in d_]
+    eng = _engine(app="bfs", placement=placement, cache_bytes=1 << 18)
+    eng.load_graph(n, src, dst)
+    eng.start()
+    eng.load_updates(adds, dels)
+    ab = db = 0
+    for s, (dl, al) in enumerate(stages):
+        st = eng.del_edge(db, len(dl))
+        m = store.apply(dl, [])
+        assert st["missing_deletes"] == m["missing_deletes"], f"stage {s}"
+        erp, ecol = eng.csr()
+        assert same_rows(erp, ecol, store), f"rows differ after delete stage {s}"
+        rp, col = store.csr()
+        assert np.array_equal(eng.gather_value(), oracle.bfs(n, rp, col, 0)), f"levels, delete stage {s}"
+        eng.add_edge(ab, len(al))
+        store.apply([], al)
+        erp, ecol = eng.csr()
+        assert same_rows(erp, ecol, store), f"rows differ after add stage {s}"
+        rp, col = store.csr()
+        assert np.array_equal(eng.gather_value(), oracle.bfs(n, rp, col, 0)), f"levels, add stage {s}"
+        ab += len(al)
+        db += len(dl)
+    eng.close()
