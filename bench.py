@@ -32,6 +32,25 @@ sys.path.insert(0, ROOT)
 
 METRIC = "incremental updates/s + p50 repair latency/batch; learned vs LRU cache, 1 GPU"
 HBM_PEAK_GBS = 8000.0
+# timing category -> device kernel (rocprofv3 names)
+KERNEL_SYMBOL = {"push": "k_persistent<0u>", "pull": "k_persistent<1u>", "invalidate": "k_persistent<2u>",
+                 "pr_solve": "k_persistent<3u>", "pr_amend": "k_persistent<4u>", "delete_rows": "k_del_rows",
+                 "delete_big": "k_del_big", "append_rows": "k_add_rows + k_add_big"}
+# HBM traffic per launch measured with rocprofv3 PMC counters (FETCH_SIZE and
+# WRITE_SIZE runs, scaled by the counters of a copy kernel of known size)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+
+
+def measured_traffic(config, kernel):
+    try:
+        with open(TRAFFIC_FILE) as f:
+            entries = json.load(f)
+    except (OSError, ValueError):
+        return None
+    for e in entries:
+        if e.get("config") == config and e.get("kernel") == kernel:
+            return e.get("bytes_per_launch")
+    return None
 
 CONFIGS = {
     "c1": dict(workload="RMAT-16 incremental SSSP (device-resident topology)", scale=16, edges=1 << 20,
@@ -283,8 +302,10 @@ def main():
         "cache_hit_rate": round(res["hits"] / reads, 4) if reads else None,
         "host_link_edge_bytes": int(res["host"] * 4),
         "calibration": {"hbm_copy_GBps": round(hbm_gbs, 1), "host_zero_copy_read_GBps": round(host_gbs, 2)},
-        "roofline": {"kernel": kname, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "roofline": {"kernel": kname, "hip_kernel": KERNEL_SYMBOL.get(kname, kname), "bound": "hbm",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(args.config, kname),
                      "launches": k["launches"], "avg_launch_ms": round(k["ms"] / max(k["launches"], 1), 4),
                      "algo_bytes_per_launch": round(k["algo_bytes"] / max(k["launches"], 1), 1)},
         "kernels": {name: {"ms": round(v["ms"], 3), "launches": v["launches"],

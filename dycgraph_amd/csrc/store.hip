@@ -30,6 +30,10 @@ enum Counter : int {
     C_SLOTS_READ = 9,
     C_SLOTS_WRITTEN = 10,
     C_NBIG = 11,
+    C_W_READ = 12,       // wave-path delete: slots read, slots written, rows
+    C_W_WRITTEN = 13,
+    C_W_ROWS = 14,
+    C_B_ROWS = 15,       // workgroup-path delete rows
     C_NUM = 16
 };
 
@@ -514,7 +518,7 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
     const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kWave;
     const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / kWave;
     const uint64_t ngroups = *ngroups_p;
-    unsigned long long missing = 0, changed = 0, rd = 0, wr = 0;
+    unsigned long long missing = 0, changed = 0, rd = 0, wr = 0, rows = 0;
     for (uint64_t g = wave; g < ngroups; g += nwaves) {
         const uint64_t gs = gstart[g];
         const uint32_t row = (uint32_t)(keys[gs] >> 32);
@@ -575,13 +579,15 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
             changed += removed ? 1 : 0;
             rd += read;
             wr += moved + new_tombs;
+            rows++;
         }
     }
     if (lane == 0) {
         if (count_stats && missing) atomicAdd(&counters[C_MISSING], missing);
         if (count_stats && changed) atomicAdd(&counters[C_CHANGED], changed);
-        if (rd) atomicAdd(&counters[C_SLOTS_READ], rd);
-        if (wr) atomicAdd(&counters[C_SLOTS_WRITTEN], wr);
+        if (rd) atomicAdd(&counters[C_W_READ], rd);
+        if (wr) atomicAdd(&counters[C_W_WRITTEN], wr);
+        if (rows) atomicAdd(&counters[C_W_ROWS], rows);
     }
 }
 
@@ -788,6 +794,7 @@ __global__ __launch_bounds__(1024) void k_del_big(StoreView s, RowDesc *desc, ui
                 if (removed) atomicAdd(&counters[C_CHANGED], 1ull);
             }
             atomicAdd(&counters[C_SLOTS_READ], scanned);
+            atomicAdd(&counters[C_B_ROWS], 1ull);
             if (new_tombs) atomicAdd(&counters[C_SLOTS_WRITTEN], (unsigned long long)new_tombs);
             if (full_read) hot[row] += 1;   // a pass read the whole row (cache-policy signal)
         }
@@ -1007,29 +1014,36 @@ void Engine::mutate_delete(uint64_t begin, uint64_t count, dcg_phase_stats *st) 
     const uint32_t wg = wave_grid(count);
     d_big_groups.ensure(count);
     fill<uint32_t>(stream, d_used.p, count, 0u);
-    const int slot = time_begin(DCG_K_DELETE_ROWS, -1);
+    // each launch timed on its own (HIP events); the bytes of both directions
+    // go to the first launch of each kernel, so sums and averages hold
+    const int w0 = time_begin(DCG_K_DELETE_ROWS, -1);
     hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream, v, d_out.p, d_sorted_out.p, d_hot_out.p, kout,
-                       count, d_gstart.p,
-                       d_counters.p + C_GROUPS_OUT, d_used.p, d_counters.p, 1, d_big_groups.p,
+                       count, d_gstart.p, d_counters.p + C_GROUPS_OUT, d_used.p, d_counters.p, 1, d_big_groups.p,
                        d_counters.p + C_NBIG);
+    time_end(w0);
+    const int b0 = time_begin(DCG_K_DELETE_BIG, -1);
     hipLaunchKernelGGL(k_del_big, dim3(kDelBigGrid), dim3(1024), 0, stream, v, d_out.p, d_sorted_out.p, d_hot_out.p,
-                       kout, count,
-                       d_big_groups.p, d_counters.p + C_NBIG, d_used.p, d_counters.p, 1);
+                       kout, count, d_big_groups.p, d_counters.p + C_NBIG, d_used.p, d_counters.p, 1);
+    time_end(b0);
     fill<uint32_t>(stream, d_used.p, count, 0u);
     fill<unsigned long long>(stream, d_counters.p + C_NBIG, 1, 0ull);
+    const int w1 = time_begin(DCG_K_DELETE_ROWS, -1);
     hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream, v, d_in.p, d_sorted_in.p, d_hot_in.p, kin,
-                       count, d_gstart2.p,
-                       d_counters.p + C_GROUPS_IN, d_used.p, d_counters.p, 0, d_big_groups.p,
+                       count, d_gstart2.p, d_counters.p + C_GROUPS_IN, d_used.p, d_counters.p, 0, d_big_groups.p,
                        d_counters.p + C_NBIG);
+    time_end(w1);
+    const int b1 = time_begin(DCG_K_DELETE_BIG, -1);
     hipLaunchKernelGGL(k_del_big, dim3(kDelBigGrid), dim3(1024), 0, stream, v, d_in.p, d_sorted_in.p, d_hot_in.p,
-                       kin, count,
-                       d_big_groups.p, d_counters.p + C_NBIG, d_used.p, d_counters.p, 0);
-    time_end(slot);
+                       kin, count, d_big_groups.p, d_counters.p + C_NBIG, d_used.p, d_counters.p, 0);
+    time_end(b1);
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipMemcpyAsync(h_counters, d_counters.p, C_NUM * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
     sync();
-    timed[slot].bytes = 4.0 * (double)(h_counters[C_SLOTS_READ] + h_counters[C_SLOTS_WRITTEN]) +
-                        16.0 * (double)(h_counters[C_GROUPS_OUT] + h_counters[C_GROUPS_IN]);
+    timed[w0].bytes = 4.0 * (double)(h_counters[C_W_READ] + h_counters[C_W_WRITTEN]) + 16.0 * (double)h_counters[C_W_ROWS];
+    timed[w1].bytes = 0.0;
+    timed[b0].bytes = 4.0 * (double)(h_counters[C_SLOTS_READ] + h_counters[C_SLOTS_WRITTEN]) +
+                      16.0 * (double)h_counters[C_B_ROWS];
+    timed[b1].bytes = 0.0;
     if (st) {
         st->touched_sources = h_counters[C_GROUPS_OUT];
         st->changed_sources = h_counters[C_CHANGED];

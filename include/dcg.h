@@ -189,7 +189,10 @@ int dcg_get_info(dcg_engine *eng, dcg_engine_info *info);
  * slot read or written + 16 B per touched row descriptor). */
 typedef enum {
     DCG_K_PUSH = 0, DCG_K_PULL = 1, DCG_K_INVALIDATE = 2, DCG_K_PR_SOLVE = 3, DCG_K_PR_AMEND = 4,
-    DCG_K_DELETE_ROWS = 5, DCG_K_APPEND_ROWS = 6, DCG_K_NUM = 8
+    DCG_K_DELETE_ROWS = 5,   /* delete mutation, wave path (rows of at most 1,024 slots) */
+    DCG_K_APPEND_ROWS = 6,   /* append / relocation, both paths */
+    DCG_K_DELETE_BIG = 7,    /* delete mutation, workgroup path (larger rows) */
+    DCG_K_NUM = 8
 } dcg_kernel_id;
 typedef struct {
     uint64_t launches;
