@@ -5,7 +5,8 @@ maintenance (the reference's P0-TIMER window, samples/hybrid_sssp/hybrid_sssp.cu
 with the update stream already resident in HBM.  `value` is updates/s over the
 timed snapshots; p50/p90 are per-snapshot repair latencies.
 
-Workloads (seeded, generated in-process):
+Default: c4, the configuration the headline metric (learned vs LRU cache) is
+quoted on.  Workloads (seeded, generated in-process):
   c1  RMAT-16 (65,536 v, 1,048,576 e), SSSP, weights 1+mix64 % 100, source 0,
       1,000 deletes + 1,000 inserts per snapshot, topology in HBM.
   c2  RMAT-22 (4,194,304 v, 67,108,864 e), PageRank (probability scale), 50k+50k per snapshot.
@@ -259,7 +260,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default=os.environ.get("DCG_BENCH_CONFIG", "c1"), choices=sorted(CONFIGS))
+    ap.add_argument("--config", default=os.environ.get("DCG_BENCH_CONFIG", "c4"), choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", action="store_true", help="verify the final state against the oracle")
     ap.add_argument("--no-compare", action="store_true",

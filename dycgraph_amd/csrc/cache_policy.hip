@@ -173,6 +173,7 @@ void Engine::cache_apply_plan(const std::vector<uint64_t> &desired, dcg_cache_st
 
 static void fill_common(Engine &e, dcg_cache_stats *st) {
     if (!st) return;
+    if (e.cache.mlp) mlp_stats(e, st);
     st->frames = e.cache.frames;
     st->total_chunks = e.cache.chunks;
     uint64_t r = 0;
@@ -180,11 +181,17 @@ static void fill_common(Engine &e, dcg_cache_stats *st) {
     st->resident_chunks = r;
 }
 
+// Admissions per decision: 1/32 of the frames (at least 4), so a decision
+// moves at most ~3% of the cache over the host link and a changed ranking is
+// followed over several snapshots instead of in one stall.
+uint64_t Engine::move_budget() const { return std::max<uint64_t>(4, cache.frames / 32); }
+
 // Admission plan shared by both policies.  Candidates are ranked by score
 // (ties: most recent use, then chunk id).  Loading a chunk moves 256 KiB over
 // the host link, the same as reading all of it zero-copy once, so a chunk is
-// admitted only if its score reaches `admit_min` edge reads, and it displaces a
-// resident chunk only if its score exceeds the resident's by `margin`.
+// admitted only if its score reaches `admit_min` edge reads, it displaces a
+// resident chunk only if its score exceeds the resident's by `margin`, and at
+// most move_budget() chunks are admitted.
 std::vector<uint64_t> Engine::plan_admission(const std::vector<float> &score, double admit_min, double margin) {
     const uint64_t C = cache.chunks, F = cache.frames;
     auto better = [&](uint64_t a, uint64_t b) {
@@ -211,8 +218,9 @@ std::vector<uint64_t> Engine::plan_admission(const std::vector<float> &score, do
     std::vector<uint8_t> drop(res.size(), 0);
     std::vector<uint64_t> admit;
     size_t ci = 0, ri = 0;
-    for (; ci < cand.size() && admit.size() < free_frames; ++ci) admit.push_back(cand[ci]);
-    for (; ci < cand.size() && ri < res.size(); ++ci, ++ri) {
+    const uint64_t budget = move_budget();
+    for (; ci < cand.size() && admit.size() < free_frames && admit.size() < budget; ++ci) admit.push_back(cand[ci]);
+    for (; ci < cand.size() && ri < res.size() && admit.size() < budget; ++ci, ++ri) {
         if (!(score[cand[ci]] > margin * std::max(score[res[ri]], 1.0f))) break;
         drop[ri] = 1;
         admit.push_back(cand[ci]);

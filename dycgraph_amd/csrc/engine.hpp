@@ -195,6 +195,7 @@ struct Engine {
     void cache_prefetch(uint64_t ab, uint64_t ac, uint64_t db, uint64_t dc, dcg_cache_stats *st);
     void cache_apply_plan(const std::vector<uint64_t> &desired, dcg_cache_stats *st);
     std::vector<uint64_t> plan_admission(const std::vector<float> &score, double admit_min, double margin);
+    uint64_t move_budget() const;   // admissions per decision
 
     void sync() { DCG_HIP(hipStreamSynchronize(stream)); }
     void *temp(size_t bytes) {

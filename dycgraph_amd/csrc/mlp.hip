@@ -358,9 +358,18 @@ void mlp_observe(Engine &e, const std::vector<float> &reads) {
     m.hot_capture = capture(m.hot_score_at_decision, reads, e.cache.frames);
     m.filled = std::min(kReplay, m.filled + 1);
     m.slot = (m.slot + 1) % kReplay;
-    mlp_train_steps(e, m, 8);
-    m.fallback = m.steps < 16 || m.mlp_capture < m.hot_capture;
+    mlp_train_steps(e, m, 4);
+    // the network's ranking is used only once trained and clearly ahead
+    m.fallback = m.steps < 8 || !(m.mlp_capture > 1.02 * m.hot_capture);
     m.have_decision = false;
+}
+
+void mlp_stats(const Engine &e, dcg_cache_stats *st) {
+    const MlpState &m = *e.cache.mlp;
+    st->learned_decisions = (uint64_t)(m.decisions - m.fallbacks);
+    st->fallback_decisions = (uint64_t)m.fallbacks;
+    st->mlp_capture = m.mlp_capture;
+    st->hot_capture = m.hot_capture;
 }
 
 std::vector<uint64_t> mlp_plan(Engine &e, uint64_t ab, uint64_t ac, uint64_t db, uint64_t dc) {

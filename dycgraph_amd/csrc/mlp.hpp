@@ -11,5 +11,7 @@ void mlp_create(Engine &e);
 void mlp_observe(Engine &e, const std::vector<float> &reads);
 // Chunks to keep resident for the staged batch about to run.
 std::vector<uint64_t> mlp_plan(Engine &e, uint64_t ab, uint64_t ac, uint64_t db, uint64_t dc);
+// Decision counters and the judge's last comparison (dcg_cache_stats fields).
+void mlp_stats(const Engine &e, dcg_cache_stats *st);
 
 }  // namespace dcg

@@ -499,9 +499,21 @@ __device__ __forceinline__ uint32_t search_prefix(const StoreView &s, uint64_t o
 }
 
 constexpr uint32_t kWaveSearch = 64;   // sorted prefixes at least this long are binary-searched by the wave
+constexpr uint32_t kWaveTail = 4096;   // longest tail a wave scans for a large row
+constexpr uint32_t kWaveRuns = 256;    // most targets a wave takes for a large row
 
-// One wave per touched row of at most kBigDelete slots; larger rows are listed
-// for k_del_big.  `sorted` holds each row's sorted prefix length S.
+// A row above kBigDelete slots still takes the wave path when the wave only
+// binary-searches its sorted prefix and scans a short tail: few targets, and
+// tombstones below the compaction point (k_del_big compacts).
+__device__ __forceinline__ bool wave_searchable(const RowDesc &d, uint32_t S, uint64_t targets) {
+    const uint32_t tombs = row_tombs(d);
+    return S >= kWaveSearch && d.ext - S <= kWaveTail && targets <= kWaveRuns && (uint64_t)tombs * 4 <= d.ext &&
+           tombs <= kMaxTombs / 2;
+}
+
+// One wave per touched row of at most kBigDelete slots, or a larger row that
+// wave_searchable admits; the rest are listed for k_del_big.  `sorted` holds
+// each row's sorted prefix length S.
 //  * S >= kWaveSearch: search_prefix, one lane per run, then (only when a run
 //    still wants occurrences) wave_delete_range over the tail [S, ext), which
 //    holds appended entries and never tombstones.  The prefix precedes the
@@ -523,12 +535,12 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
         const uint64_t gs = gstart[g];
         const uint32_t row = (uint32_t)(keys[gs] >> 32);
         const RowDesc d = desc[row];
-        if (d.ext > kBigDelete) {
+        const uint64_t ge = group_end<uint64_t>(keys, gs, count, row);
+        const uint32_t S = sorted[row] < d.ext ? sorted[row] : d.ext;
+        if (d.ext > kBigDelete && !wave_searchable(d, S, ge - gs)) {
             if (lane == 0) big_groups[atomicAdd(nbig, 1ull)] = (uint32_t)gs;
             continue;
         }
-        const uint64_t ge = group_end<uint64_t>(keys, gs, count, row);
-        const uint32_t S = sorted[row] < d.ext ? sorted[row] : d.ext;
         uint32_t ext = d.ext, S2 = S, removed = 0, moved = 0, new_tombs = 0, read = 0;
         bool full = false;
         if (S >= kWaveSearch) {
@@ -558,7 +570,7 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
             full = true;
         }
         uint32_t tombs = row_tombs(d) + new_tombs;
-        if (tombs && (uint64_t)tombs * 4 > ext) {
+        if (tombs && (uint64_t)tombs * 4 > ext && ext <= kBigDelete) {   // larger rows: k_del_big compacts
             __threadfence_block();   // the lanes' tombstone stores complete before the pass reads them
             const DelResult c = wave_compact_tombs(s, d.off, ext, S2);
             read += ext;

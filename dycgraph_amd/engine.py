@@ -60,7 +60,9 @@ class PhaseStats(C.Structure):
 class CacheStats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in (
         "frames", "resident_chunks", "total_chunks", "loaded_chunks", "evicted_chunks",
-        "written_back_chunks", "h2d_bytes", "d2h_bytes")] + [("ms", C.c_double)]
+        "written_back_chunks", "h2d_bytes", "d2h_bytes")] + [("ms", C.c_double)] + [
+        ("learned_decisions", C.c_uint64), ("fallback_decisions", C.c_uint64), ("mlp_capture", C.c_double),
+        ("hot_capture", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

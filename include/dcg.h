@@ -98,12 +98,18 @@ typedef struct {
     double   ms;                  /* wall time of the phase                            */
 } dcg_phase_stats;
 
-/* Cache maintenance counters. */
+/* Cache maintenance counters.  With the learned policy, learned_decisions and
+ * fallback_decisions count the admission decisions taken so far from the
+ * network's ranking and from the hotness ranking, and mlp_capture /
+ * hot_capture are the judge's last comparison: the share of the last
+ * snapshot's edge reads each ranking's top `frames` chunks would have held. */
 typedef struct {
     uint64_t frames, resident_chunks, total_chunks;
     uint64_t loaded_chunks, evicted_chunks, written_back_chunks;
     uint64_t h2d_bytes, d2h_bytes;
     double   ms;
+    uint64_t learned_decisions, fallback_decisions;
+    double   mlp_capture, hot_capture;
 } dcg_cache_stats;
 
 /* Engine construction.  Replaces Engine::Engine + Engine::SetOptions
