@@ -30,12 +30,19 @@ def _region_graph(seed=5):
     return n, src[keep].astype(np.uint32), dst[keep].astype(np.uint32)
 
 
-def _stream(n, src, dst, snapshots, per, seed=9):
+def _stream(n, src, dst, snapshots, per, hub_churn=32, seed=9):
+    """Per snapshot, inside one region: `per` random deletes and inserts of
+    region edges, and `hub_churn` of the source's edges into the region
+    replaced by new ones (the region's BFS levels move, so its rows are read
+    by the repair)."""
     rng = np.random.default_rng(seed)
     adds, dels = [], []
-    live = {}
+    live, hubs = {}, {}
     for u, v in zip(src.tolist(), dst.tolist()):
-        live.setdefault(u // SIDE, []).append((u, v))
+        if u == 0:
+            hubs.setdefault(v // SIDE, []).append(v)
+        else:
+            live.setdefault(u // SIDE, []).append((u, v))
     for s in range(snapshots):
         r = (s * 5) % REGIONS                     # jump between regions
         pool = live[r]
@@ -46,6 +53,12 @@ def _stream(n, src, dst, snapshots, per, seed=9):
             pool.pop()
         al = [(int(r * SIDE + a), int(r * SIDE + b)) for a, b in rng.integers(1, SIDE, (per, 2)) if a != b]
         pool.extend(al)
+        hp = hubs[r]
+        gone = [hp.pop(int(rng.integers(len(hp)))) for _ in range(hub_churn)]
+        new = [int(r * SIDE + x) for x in rng.integers(1, SIDE, hub_churn)]
+        hp.extend(new)
+        dl += [(0, v) for v in gone]
+        al += [(0, v) for v in new]
         adds.append(al)
         dels.append(dl)
     return adds, dels
@@ -86,7 +99,7 @@ def test_learned_policy_gains_on_moving_batches():
     import dycgraph_amd as d
     announce("learned vs hotness on region-hopping batches")
     n, src, dst = _region_graph()
-    adds, dels = _stream(n, src, dst, 40, 1500)
+    adds, dels = _stream(n, src, dst, 40, 5000, hub_churn=48)
     hot_rate, hot_vals, _ = _run(d.CACHE_HOTNESS, n, src, dst, adds, dels)
     mlp_rate, mlp_vals, st = _run(d.CACHE_LEARNED, n, src, dst, adds, dels)
     print(f"hit rate second half: hotness {hot_rate:.4f} learned {mlp_rate:.4f} "
