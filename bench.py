@@ -142,6 +142,7 @@ def run_gpu(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, policy, queries
     aoff = np.concatenate([[0], np.cumsum(ap)]).astype(np.int64)
     doff = np.concatenate([[0], np.cumsum(dp)]).astype(np.int64)
     lat, hits, host = [], 0, 0
+    closure = {"affected": 0, "rounds": 0, "edges_scanned": 0, "bucket_advances": 0, "far_entries": 0}
 
     def snapshot(s):
         ab, db, na, nd = int(aoff[s]), int(doff[s]), int(ap[s]), int(dp[s])
@@ -168,12 +169,14 @@ def run_gpu(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, policy, queries
         for p in phases:
             hits += p["cached_edge_reads"]
             host += p["host_edge_reads"]
+            for k in closure:
+                closure[k] += p.get(k, 0)
     total = time.perf_counter() - t_all
     ks = eng.kernel_stats()
     info = eng.info()
     values = [eng.gather_value(q) for q in range(eng.queries)]
     return dict(total_s=total, lat=lat, ks=ks, info=info, hits=hits, host=host, eng=eng, values=values,
-                queries=(sources, programs))
+                queries=(sources, programs), closure={k: round(v / steps, 1) for k, v in closure.items()})
 
 
 def run_plain(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, queries):
@@ -309,6 +312,7 @@ def main():
                      "traffic": measured_traffic(args.config, kname),
                      "launches": k["launches"], "avg_launch_ms": round(k["ms"] / max(k["launches"], 1), 4),
                      "algo_bytes_per_launch": round(k["algo_bytes"] / max(k["launches"], 1), 1)},
+        "per_snapshot": res["closure"],
         "kernels": {name: {"ms": round(v["ms"], 3), "launches": v["launches"],
                            "GBps": round(v["algo_bytes"] / (v["ms"] / 1e3) / 1e9, 1) if v["ms"] else 0.0}
                     for name, v in ks.items() if v["launches"]},

@@ -153,6 +153,7 @@ struct Ctl {
     uint32_t far_count[2];    // entries of the two far piles (bucketed push closure)
     uint32_t far_par;         // pile that collects this epoch's far entries
     uint32_t advances;        // bucket advances taken
+    unsigned long long far_scanned;   // far-pile entries examined by the advances
 };
 
 struct BigEntry {

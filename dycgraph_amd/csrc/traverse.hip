@@ -510,6 +510,7 @@ __device__ bool far_advance(const TravArgs &a, uint32_t r, const Team &t, cg::gr
         a.ctl->far_count[fp] = 0;
         a.ctl->far_par = fp ^ 1;
         a.ctl->advances += 1;
+        a.ctl->far_scanned += fc;
     }
     for (uint32_t q = t.tid; q < a.Q; q += t.nthreads) {
         a.thr[q] = thr_of(mf[q], a.fdelta[q]);
@@ -643,6 +644,7 @@ __global__ void k_ctl_setup(Ctl *c, const uint32_t *src0, uint32_t add0, uint32_
     c->far_count[0] = c->far_count[1] = 0;
     c->far_par = 0;
     c->advances = 0;
+    c->far_scanned = 0;
 }
 
 // Bucket thresholds of a push closure: 0 (everything relaxed waits in the far
@@ -728,6 +730,47 @@ __global__ void k_reset_affected(uint64_t *st, const uint32_t *aff, const Ctl *c
     }
 }
 
+// CC parents are written after the winning atomicMin of the state word, so two
+// improvers of one vertex can leave the earlier one's parent behind (the
+// invalidation rule treats such a parent as not tight, which only widens it).
+// After a closure every CC vertex it touched (tick >= stamp_lo, or on the
+// affected list) has its parent checked: kept if tight (equal label, depth - 1),
+// else replaced by the first tight in-neighbour; depth 0 is a root (none).
+__global__ void k_cc_fix_parents(TravArgs a, uint32_t stamp_lo, const uint32_t *list, const Ctl *list_ctl) {
+    const uint64_t total = list ? (uint64_t)list_ctl->aff_count : (uint64_t)a.n * a.Q;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t g;
+        if (list) {
+            g = list[i];
+        } else {
+            g = (uint32_t)i;
+            if (a.tick[g] < stamp_lo) continue;
+        }
+        uint32_t q, v;
+        decode_g(a, g, q, v);
+        if (a.qapp[q] != DCG_APP_CC) continue;
+        const uint64_t qoff = (uint64_t)q * a.n;
+        const uint64_t sv = a.st[g];
+        const uint32_t p = a.par[g];
+        if ((uint32_t)sv == 0) {
+            if (p != kInf) a.par[g] = kInf;
+            continue;
+        }
+        if (p != kInf && a.st[qoff + p] + 1 == sv) continue;
+        const RowDesc d = a.s.in[v];
+        uint32_t best = kInf;
+        for (uint32_t j = 0; j < d.ext; ++j) {
+            const uint32_t x = *row_base(a.s, d.off + j);
+            if (!(x & kTomb) && a.st[qoff + x] + 1 == sv) {
+                best = x;
+                break;
+            }
+        }
+        a.par[g] = best;
+    }
+}
+
 // Insertion seeds (framework.cuh:297-370): relax every added edge for every query.
 __global__ void k_seed_adds(TravArgs a, const dcg_edge *recs, uint64_t count, uint32_t *outq) {
     const uint64_t total = count * a.Q;
@@ -760,6 +803,13 @@ __global__ void k_seed_adds(TravArgs a, const dcg_edge *recs, uint64_t count, ui
 }
 
 // ------------------------------------------------------------------- host side
+static void cc_fix_parents(Engine &e, const TravArgs &a, uint32_t stamp_lo, const uint32_t *list, const Ctl *lc) {
+    if (!e.d_par.p) return;
+    const uint64_t work = (uint64_t)e.n * e.Q;
+    hipLaunchKernelGGL(k_cc_fix_parents, dim3(grid_for(work, 256, 16384)), dim3(256), 0, e.stream, a, stamp_lo, list,
+                       lc);
+}
+
 uint32_t Engine::coop_blocks(const void *kernel) {
     int per_cu = 0;
     DCG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0));
@@ -793,6 +843,17 @@ void launch_persistent(Engine &e, uint32_t mode, TravArgs &a) {
 
 // Stamps: each launch owns a window of 2^20 round stamps; the tick array is
 // cleared when the 32-bit stamp space would wrap.
+// Before a phase: keep `windows` stamp windows ahead without a wrap, so every
+// stamp the phase takes is above the first one (the CC parent check below
+// selects the phase's vertices by stamp).
+static void reserve_stamps(Engine &e, uint32_t windows) {
+    constexpr uint64_t kWindow = 1ull << 20;
+    if ((uint64_t)e.stamp + (uint64_t)(windows + 2) * kWindow > 0xFFFFFFFFull) {
+        fill<uint32_t>(e.stream, e.d_tick.p, (uint64_t)e.n * e.Q, 0u);
+        e.stamp = 1;
+    }
+}
+
 static uint32_t take_stamps(Engine &e) {
     constexpr uint32_t kWindow = 1u << 20;
     if (e.stamp > 0xFFFFFFFFu - 2 * kWindow) {
@@ -1005,6 +1066,7 @@ void Engine::start_min_lattice() {
     a.ctl = c;
     a.stamp0 = take_stamps(*this);
     launch_persistent(*this, M_PUSH, a);
+    cc_fix_parents(*this, a, a.stamp0, nullptr, nullptr);
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipMemcpyAsync(h_ctl, c, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
     sync();
@@ -1018,6 +1080,7 @@ void Engine::start_min_lattice() {
 void Engine::delete_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
     mutate_delete(begin, count, st);
     if (!count) return;
+    reserve_stamps(*this, 2);
     Ctl *c = d_ctl.p;
     const dcg_edge *recs = d_dels.p + begin;
     TravArgs a = base_args(*this);
@@ -1048,6 +1111,8 @@ void Engine::delete_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
     a.max_rounds = 1u << 20;
     a.stamp0 = take_stamps(*this);
     launch_persistent(*this, M_PUSH, a);
+    cc_fix_parents(*this, a, a.stamp0, d_aff.p, c + 1);
+    cc_fix_parents(*this, a, a.stamp0, nullptr, nullptr);
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipMemcpyAsync(h_ctl, c, 4 * sizeof(Ctl), hipMemcpyDeviceToHost, stream));
     sync();
@@ -1056,6 +1121,8 @@ void Engine::delete_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
     if (st) {
         st->affected = h_ctl[1].aff_count;
         st->rounds = h_ctl[1].rounds + h_ctl[3].rounds;
+        st->bucket_advances = h_ctl[3].advances;
+        st->far_entries = h_ctl[3].far_scanned;
         st->edges_scanned = h_ctl[1].edges + h_ctl[2].edges + h_ctl[3].edges;
     }
 }
@@ -1064,6 +1131,7 @@ void Engine::delete_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
 void Engine::add_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
     mutate_add(begin, count, st);
     if (!count) return;
+    reserve_stamps(*this, 2);
     Ctl *c = d_ctl.p;
     const dcg_edge *recs = d_adds.p + begin;
     TravArgs a = base_args(*this);
@@ -1071,6 +1139,7 @@ void Engine::add_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
                        (const uint32_t *)nullptr);
     a.ctl = c + 4;
     a.stamp0 = take_stamps(*this);
+    const uint32_t stamp_lo = a.stamp0;
     hipLaunchKernelGGL(k_seed_adds, dim3(grid_for(count * Q)), dim3(256), 0, stream, a, recs, count, d_qc.p);
     hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 5, &c[4].aff_count, 0u, 0u,
                        (const uint32_t *)nullptr);
@@ -1078,12 +1147,15 @@ void Engine::add_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
     a.in0 = d_qc.p;
     a.stamp0 = take_stamps(*this);
     launch_persistent(*this, M_PUSH, a);
+    cc_fix_parents(*this, a, stamp_lo, nullptr, nullptr);
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipMemcpyAsync(h_ctl + 4, c + 4, 2 * sizeof(Ctl), hipMemcpyDeviceToHost, stream));
     sync();
     if (h_ctl[4].overflow || h_ctl[5].overflow) throw Error(DCG_ERR_CAPACITY, "frontier overflow in the add stage");
     if (st) {
         st->rounds = h_ctl[5].rounds;
+        st->bucket_advances = h_ctl[5].advances;
+        st->far_entries = h_ctl[5].far_scanned;
         st->edges_scanned = h_ctl[5].edges;
     }
 }

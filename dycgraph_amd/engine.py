@@ -51,7 +51,7 @@ class PhaseStats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in (
         "requests", "touched_sources", "changed_sources", "missing_deletes", "invalid_additions",
         "relocated_rows", "affected", "rounds", "edges_scanned", "cached_edge_reads",
-        "host_edge_reads")] + [("ms", C.c_double)]
+        "host_edge_reads")] + [("ms", C.c_double), ("bucket_advances", C.c_uint64), ("far_entries", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -96,6 +96,8 @@ typedef struct {
     uint64_t cached_edge_reads;   /* of which served from the device chunk cache       */
     uint64_t host_edge_reads;     /* of which served zero-copy from host memory        */
     double   ms;                  /* wall time of the phase                            */
+    uint64_t bucket_advances;     /* bucket advances of the ordered push closure       */
+    uint64_t far_entries;         /* far-pile entries those advances examined          */
 } dcg_phase_stats;
 
 /* Cache maintenance counters.  With the learned policy, learned_decisions and
