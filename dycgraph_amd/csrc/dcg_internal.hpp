@@ -154,6 +154,7 @@ struct Ctl {
     uint32_t far_par;         // pile that collects this epoch's far entries
     uint32_t advances;        // bucket advances taken
     unsigned long long far_scanned;   // far-pile entries examined by the advances
+    uint32_t rmax[2];         // dense PageRank sweeps: largest |residual| (float bits) by round parity
 };
 
 struct BigEntry {

@@ -188,6 +188,10 @@ struct Engine {
     // pagerank.hip
     void start_pagerank();
     void pagerank_batch(uint64_t ab, uint64_t ac, uint64_t db, uint64_t dc, dcg_phase_stats *st);
+    void pagerank_dense(Ctl *c, dcg_phase_stats *st);
+    // dense sweeps pay once the frontier holds more than 1/32 of the vertices
+    bool dense_pr(uint64_t frontier) const { return frontier * 32 > (uint64_t)n; }
+    DevBuf<uint32_t> d_iota;
 
     // cache_policy.hip
     void cache_init();

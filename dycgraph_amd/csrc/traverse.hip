@@ -28,7 +28,7 @@ namespace cg = cooperative_groups;
 
 namespace dcg {
 
-enum Mode : uint32_t { M_PUSH = 0, M_PULL = 1, M_INVALIDATE = 2, M_PR = 3, M_PR_AMEND = 4 };
+enum Mode : uint32_t { M_PUSH = 0, M_PULL = 1, M_INVALIDATE = 2, M_PR = 3, M_PR_AMEND = 4, M_PR_DENSE = 5 };
 
 struct TravArgs {
     StoreView s;
@@ -316,6 +316,14 @@ __device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L
                     val = __float_as_uint(0.85f * x / (float)live);
                     a.s.hot_out[v] += 1;
                 }
+            } else if (MODE == M_PR_DENSE) {
+                v = g;   // every vertex pulls over its in-row
+                const RowDesc d = a.s.in[v];
+                if (d.ext) {
+                    deg = d.ext;
+                    off = d.off;
+                    a.s.hot_in[v] += 1;
+                }
             } else {
                 decode_g(a, g, q, v);
                 const RowDesc d = MODE == M_PULL ? a.s.in[v] : a.s.out[v];
@@ -356,7 +364,31 @@ __device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         edges += total;
-        for (uint32_t k0 = 0; k0 < total; k0 += kWave) {
+        for (uint32_t k0 = 0; MODE == M_PR_DENSE && k0 < total; k0 += kWave) {
+            // owners are contiguous along the lanes: segmented sum, one atomic per owner
+            const uint32_t k = k0 + lane;
+            uint32_t o = 64;
+            float c = 0.f;
+            if (k < total) {
+                uint32_t lo = 0, hi = 64;
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (L.pref[mid] <= k) lo = mid;
+                    else hi = mid;
+                }
+                o = lo;
+                const uint32_t x = L.base[lo][k - L.pref[lo]];
+                c = (x & kTomb) ? 0.f : a.delta[x];
+            }
+            for (uint32_t dd = 1; dd < kWave; dd <<= 1) {
+                const float y = __shfl_up(c, dd, kWave);
+                const uint32_t oy = __shfl_up(o, dd, kWave);
+                if (lane >= dd && oy == o) c += y;
+            }
+            const uint32_t onext = __shfl_down(o, 1, kWave);
+            if (o < 64 && (lane == kWave - 1 || onext != o) && c != 0.f) atomicAdd(&a.res[L.owner[o]], c);
+        }
+        for (uint32_t k0 = 0; MODE != M_PR_DENSE && k0 < total; k0 += kWave) {
             const uint32_t k = k0 + lane;
             uint32_t kind = 0;
             uint32_t og = 0;
@@ -407,10 +439,18 @@ __device__ void phase_b(const TravArgs &a, uint32_t r, const Team &t, uint64_t &
             const uint32_t len = be.deg - start < a.piece ? be.deg - start : a.piece;
             const uint32_t *base = row_base(a.s, be.off + start);
             uint32_t q = 0, v = 0;
-            if (MODE == M_PR || MODE == M_PR_AMEND) v = be.g;
+            if (MODE == M_PR || MODE == M_PR_AMEND || MODE == M_PR_DENSE) v = be.g;
             else decode_g(a, be.g, q, v);
             edges += lane == 0 ? len : 0;
-            if (MODE == M_PULL && a.qapp[q] == DCG_APP_CC) {
+            if (MODE == M_PR_DENSE) {
+                float c = 0.f;
+                for (uint32_t k = lane; k < len; k += kWave) {
+                    const uint32_t x = base[k];
+                    if (!(x & kTomb)) c += a.delta[x];
+                }
+                for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, kWave);
+                if (lane == 0 && c != 0.f) atomicAdd(&a.res[v], c);
+            } else if (MODE == M_PULL && a.qapp[q] == DCG_APP_CC) {
                 for (uint32_t k = lane; k < len; k += kWave) {
                     const uint32_t x = base[k];
                     if (!(x & kTomb)) pull_cc(a, q, v, x);
@@ -454,6 +494,21 @@ __device__ void round_open(const TravArgs &a, uint32_t r, const Team &t) {
         a.ctl->big[nxt] = 0;
         a.ctl->big_cursor[nxt] = 0;
         a.ctl->rounds = r + 1;
+    }
+    if (MODE == M_PR_DENSE) {
+        // every round is a full sweep: the next one again takes all n vertices
+        if (t.tid == 0) a.ctl->count[nxt] = a.n;
+        // Consume every residual (pr_residual.cuh:53-69 without the eps filter):
+        // rank += r, and the share 0.85 r / outdeg each out-neighbour will pull.
+        for (uint32_t v = t.tid; v < a.n; v += t.nthreads) {
+            const float res = a.res[v];
+            if (!isfinite(res) || !isfinite(a.rank[v])) atomicExch((unsigned int *)&a.ctl->bad, 1u);
+            a.queued[v] = 0;
+            a.rank[v] += res;
+            a.res[v] = 0.f;
+            const uint32_t live = row_live(a.s.out[v]);
+            a.delta[v] = live ? 0.85f * res / (float)live : 0.f;
+        }
     }
     if (MODE == M_PR) {
         // Consume (pr_residual.cuh:53-69): fold residuals above eps into ranks.
@@ -570,7 +625,7 @@ __global__ __launch_bounds__(256) void k_persistent(TravArgs a) {
             break;
         }
         idle_advances = 0;
-        if (cnt <= kSoloMax && gridDim.x > 1) {
+        if (MODE != M_PR_DENSE && cnt <= kSoloMax && gridDim.x > 1) {
             if (blockIdx.x == 0) {
                 // solo stretch: rounds on workgroup 0 until the frontier grows,
                 // empties, or a round queues many large-row pieces
@@ -608,12 +663,25 @@ __global__ __launch_bounds__(256) void k_persistent(TravArgs a) {
             continue;
         }
         round_open<MODE>(a, r, gt);
-        if (MODE == M_PR) grid.sync();
+        if (MODE == M_PR || MODE == M_PR_DENSE) grid.sync();
+        if (MODE == M_PR_DENSE && gt.tid == 0) a.ctl->rmax[(r + 1) & 1] = 0;   // all read it last round
         phase_a<MODE>(a, r, gt, L, edges, expansions);
         grid.sync();
         if (pieces_pending(a, r)) {
             phase_b<MODE>(a, r, gt, edges);
             grid.sync();
+        }
+        if (MODE == M_PR_DENSE) {
+            // converged when no residual is above eps
+            float m = 0.f;
+            for (uint32_t v = gt.tid; v < a.n; v += gt.nthreads) m = fmaxf(m, fabsf(a.res[v]));
+            for (int d = 32; d >= 1; d >>= 1) m = fmaxf(m, __shfl_xor(m, d, kWave));
+            if (lane == 0 && m > 0.f) atomicMax(&a.ctl->rmax[r & 1], __float_as_uint(m));
+            grid.sync();
+            const float rm = __uint_as_float(*(volatile uint32_t *)&a.ctl->rmax[r & 1]);
+            ++r;
+            if (!(rm > a.eps)) break;
+            continue;
         }
         ++r;
     }
@@ -645,6 +713,7 @@ __global__ void k_ctl_setup(Ctl *c, const uint32_t *src0, uint32_t add0, uint32_
     c->far_par = 0;
     c->advances = 0;
     c->far_scanned = 0;
+    c->rmax[0] = c->rmax[1] = 0;
 }
 
 // Bucket thresholds of a push closure: 0 (everything relaxed waits in the far
@@ -825,6 +894,7 @@ static const void *persistent_fn(uint32_t mode) {
     case M_PULL: return (const void *)&k_persistent<M_PULL>;
     case M_INVALIDATE: return (const void *)&k_persistent<M_INVALIDATE>;
     case M_PR: return (const void *)&k_persistent<M_PR>;
+    case M_PR_DENSE: return (const void *)&k_persistent<M_PR_DENSE>;
     default: return (const void *)&k_persistent<M_PR_AMEND>;
     }
 }
@@ -833,7 +903,8 @@ void launch_persistent(Engine &e, uint32_t mode, TravArgs &a) {
     const void *fn = persistent_fn(mode);
     if (!e.coop_grid[mode]) e.coop_grid[mode] = e.coop_blocks(fn);
     void *args[] = {&a};
-    static const int kid[5] = {DCG_K_PUSH, DCG_K_PULL, DCG_K_INVALIDATE, DCG_K_PR_SOLVE, DCG_K_PR_AMEND};
+    static const int kid[6] = {DCG_K_PUSH, DCG_K_PULL, DCG_K_INVALIDATE, DCG_K_PR_SOLVE, DCG_K_PR_AMEND,
+                               DCG_K_PR_SOLVE};
     if (mode == M_PUSH && a.far0)
         hipLaunchKernelGGL(k_far_setup, dim3(1), dim3(64), 0, e.stream, a.thr, a.minfar, a.fdelta, a.Q);
     const int slot = e.time_begin(kid[mode], (int)(a.ctl - e.d_ctl.p));
@@ -1202,6 +1273,11 @@ void Engine::start_pagerank() {
     pr_pending = d_qc.p;
     pr_pending_count = n;
     Ctl *c = d_ctl.p;
+    if (dense_pr(pr_pending_count)) {
+        pagerank_dense(c + 6, nullptr);
+        fold_reads(nullptr);
+        return;
+    }
     hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 6, (const uint32_t *)nullptr,
                        (uint32_t)pr_pending_count, 0u, (const uint32_t *)nullptr);
     TravArgs a = base_args(*this);
@@ -1224,6 +1300,61 @@ void Engine::start_pagerank() {
         pr_pending_count = h_ctl[6].count[r % 3];
     }
     fold_reads(nullptr);
+}
+
+// After dense sweeps stopped at the round cap: the vertices still above eps
+// become the pending frontier (queued, appended to `out`, count in ctl->aff_count).
+__global__ void k_pr_pending(const float *res, uint32_t *queued, uint32_t n, float eps, uint32_t *out, Ctl *c) {
+    for (uint32_t b = blockIdx.x * blockDim.x; b < n; b += gridDim.x * blockDim.x) {
+        const uint32_t v = b + threadIdx.x;
+        const bool want = v < n && fabsf(res[v]) > eps;
+        if (want) queued[v] = 1;
+        const uint64_t mask = __ballot(want);
+        if (!mask) continue;
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        uint32_t base = 0;
+        if ((int)lane_id() == leader) base = atomicAdd(&c->aff_count, (uint32_t)__popcll(mask));
+        base = __shfl(base, leader, kWave);
+        if (want) out[base + lanes_below(mask)] = v;
+    }
+}
+
+__global__ void k_iota(uint32_t *q, uint32_t n) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) q[v] = v;
+}
+
+// Dense solve (no reference counterpart; the reference always pushes,
+// pr_residual.cuh:53-188): when the frontier holds more than 1/32 of the
+// vertices, every round consumes all residuals and each vertex pulls its new
+// residual over its in-row (segmented wave sums, one atomic per row segment
+// instead of one per edge), until no residual exceeds eps or the round cap.
+// Same fixed point and the same invariant x + r as the push solve.
+void Engine::pagerank_dense(Ctl *c, dcg_phase_stats *st) {
+    if (!d_iota.p) {
+        d_iota.alloc(n ? n : 1);
+        hipLaunchKernelGGL(k_iota, dim3(grid_for(n)), dim3(256), 0, stream, d_iota.p, n);
+    }
+    hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c, (const uint32_t *)nullptr, n, 0u,
+                       (const uint32_t *)nullptr);
+    TravArgs a = base_args(*this);
+    a.ctl = c;
+    a.in0 = d_iota.p;
+    a.max_rounds = cfg.pr_max_rounds;
+    launch_persistent(*this, M_PR_DENSE, a);
+    hipLaunchKernelGGL(k_pr_pending, dim3(grid_for(n)), dim3(256), 0, stream, d_res.p, d_queued.p, n,
+                       (float)cfg.pr_epsilon, d_qc.p, c);
+    DCG_HIP(hipGetLastError());
+    DCG_HIP(hipMemcpyAsync(h_ctl + (c - d_ctl.p), c, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
+    sync();
+    const Ctl &h = h_ctl[c - d_ctl.p];
+    if (h.bad) throw Error(DCG_ERR_NUMERIC, "PR non-finite rank/residual");
+    pr_pending = d_qc.p;
+    pr_pending_count = h.aff_count;
+    if (st) {
+        st->rounds = h.rounds;
+        st->edges_scanned = h.edges;
+        st->affected = pr_pending_count;
+    }
 }
 
 // One PR batch (framework.cuh:2692-2736): Amend(-1) over the old rows of every
@@ -1268,6 +1399,28 @@ void Engine::pagerank_batch(uint64_t ab, uint64_t ac, uint64_t db, uint64_t dc, 
     a.sign = 1.f;
     launch_persistent(*this, M_PR_AMEND, a);
     DCG_HIP(hipGetLastError());
+    uint32_t frontier = 0;
+    DCG_HIP(hipMemcpyAsync(&frontier, &c[2].count[1], 4, hipMemcpyDeviceToHost, stream));
+    sync();
+    if (dense_pr(frontier)) {
+        DCG_HIP(hipMemcpyAsync(h_ctl, c, 3 * sizeof(Ctl), hipMemcpyDeviceToHost, stream));
+        sync();
+        for (int i = 0; i < 3; ++i)
+            if (h_ctl[i].overflow) throw Error(DCG_ERR_CAPACITY, "PR frontier overflow");
+        const uint64_t amend_edges = h_ctl[1].edges + h_ctl[2].edges;
+        const uint64_t touched = h_ctl[0].aff_count;
+        pagerank_dense(c + 3, st);
+        if (st) {
+            st->requests = ac + dc;
+            st->touched_sources = touched;
+            st->missing_deletes = dst.missing_deletes;
+            st->invalid_additions = ast.invalid_additions;
+            st->relocated_rows = ast.relocated_rows;
+            st->changed_sources = dst.changed_sources + ast.changed_sources;
+            st->edges_scanned += amend_edges;
+        }
+        return;
+    }
     // converge from qc (count = ctl[2].count[1])
     hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 3, &c[2].count[1], 0u, 0u,
                        (const uint32_t *)nullptr);
