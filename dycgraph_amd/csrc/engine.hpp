@@ -106,6 +106,8 @@ struct Engine {
     DevBuf<uint64_t> d_tail;           // [0] bump tail, [1] overflow flag, [2..] class cursors
     DevBuf<uint32_t> d_hot_out, d_hot_in;
     DevBuf<uint32_t> d_sorted_out, d_sorted_in;   // per row: length of the sorted prefix
+    DevBuf<uint64_t> d_pool;           // free blocks (class << 40 | offset) of relocated rows
+    uint64_t pool_len = 0, pool_taken = 0, reused_blocks = 0;
     CacheState cache;
 
     // ---- vertex state ----

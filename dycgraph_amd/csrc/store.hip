@@ -38,7 +38,13 @@ enum Counter : int {
 };
 
 // d_tail layout
-constexpr int T_TAIL = 0, T_OVERFLOW = 1, T_COUNT = 2, T_CURSOR = 2 + kClasses, T_NUM = 2 + 2 * kClasses;
+// Store tail block (u64 words): bump pointer, overflow flag, and per capacity
+// class: relocation demand, bump cursor, rank counter, blocks taken from the
+// free pool, pool segment start, pool blocks; then the retired-block cursor.
+constexpr int T_TAIL = 0, T_OVERFLOW = 1, T_COUNT = 2, T_CURSOR = 2 + kClasses, T_RANK = 2 + 2 * kClasses,
+              T_TAKE = 2 + 3 * kClasses, T_PSTART = 2 + 4 * kClasses, T_FREE = 2 + 5 * kClasses,
+              T_RETIRED = 2 + 6 * kClasses, T_NUM = 3 + 6 * kClasses;
+constexpr uint64_t kPoolOff = (1ull << 40) - 1;   // pool key = class << 40 | slot offset
 
 StoreView Engine::view() {
     StoreView v;
@@ -229,6 +235,7 @@ void Engine::load_edges_device(uint32_t nn, const uint32_t *d_src, const uint32_
     }
     sync();
     used_slots = last_off + last_cap;
+    pool_len = pool_taken = 0;
     double slack = cfg.store_slack < 0.05 ? 0.05 : cfg.store_slack;
     uint64_t want = used_slots + (uint64_t)((double)used_slots * slack) + 8 * kChunkEdges;
     total_slots = (want + kChunkEdges - 1) / kChunkEdges * kChunkEdges;
@@ -842,29 +849,46 @@ __global__ __launch_bounds__(256) void k_add_plan(const RowDesc *desc, const uin
         if (hist[i]) atomicAdd((unsigned long long *)&tail[T_COUNT + i], (unsigned long long)hist[i]);
 }
 
+// Per class: blocks retired by earlier batches are taken first, the rest come
+// from the bump tail (largest classes first, each aligned to its size).
 __global__ void k_alloc_classes(uint64_t *tail, uint64_t total) {
     if (threadIdx.x || blockIdx.x) return;
     uint64_t t = tail[T_TAIL];
-    for (int c = kClasses - 1; c >= 2; --c) {
+    for (int c = kClasses - 1; c >= 0; --c) {
         const uint64_t cnt = tail[T_COUNT + c];
-        if (!cnt) continue;
+        const uint64_t take = cnt < tail[T_FREE + c] ? cnt : tail[T_FREE + c];
+        tail[T_TAKE + c] = take;
+        tail[T_RANK + c] = 0;
+        tail[T_COUNT + c] = 0;
+        if (cnt == take) continue;
         const uint64_t a = 1ull << c;
         t = (t + a - 1) & ~(a - 1);
         tail[T_CURSOR + c] = t;
-        t += cnt << c;
-        tail[T_COUNT + c] = 0;
+        t += (cnt - take) << c;
     }
     if (t > total) tail[T_OVERFLOW] = 1;
     else tail[T_TAIL] = t;
 }
 
-// New blocks of the rows that relocate: one pass per 256 groups reserves each
-// class's blocks with one global atomic per class (LDS ranks inside the
-// workgroup), so relocations do not serialise on the class cursors.  Same
+// Free-pool segments of the class-sorted pool keys (taken entries, ~0, sort last).
+__global__ void k_pool_segments(const uint64_t *pool, uint64_t len, uint64_t *tail) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < len; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = pool[i];
+        if (k == ~0ull) continue;
+        const uint32_t c = (uint32_t)(k >> 40);
+        if (i == 0 || (uint32_t)(pool[i - 1] >> 40) != c) tail[T_PSTART + c] = i;
+        atomicAdd((unsigned long long *)&tail[T_FREE + c], 1ull);
+    }
+}
+
+// New blocks of the rows that relocate: one pass per 256 groups reserves ranks
+// in each class with one global atomic per class (LDS ranks inside the
+// workgroup), so relocations do not serialise on a cursor.  Rank i of class c
+// takes the i-th free-pool block of c, or a bump block past them.  Same
 // relocation test as k_add_plan.
 __global__ __launch_bounds__(256) void k_add_assign(const RowDesc *desc, const uint32_t *keys, uint64_t count,
                                                     const uint32_t *gstart, const unsigned long long *ngroups_p,
-                                                    uint64_t *tail, uint64_t *reloc) {
+                                                    uint64_t *tail, uint64_t *pool, uint64_t *reloc) {
     __shared__ uint32_t hist[kClasses];
     __shared__ unsigned long long base[kClasses];
     if (tail[T_OVERFLOW]) return;
@@ -888,12 +912,28 @@ __global__ __launch_bounds__(256) void k_add_assign(const RowDesc *desc, const u
         }
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < kClasses; i += blockDim.x)
-            base[i] = hist[i] ? atomicAdd((unsigned long long *)&tail[T_CURSOR + i], (unsigned long long)hist[i] << i)
-                              : 0ull;
+            base[i] = hist[i] ? atomicAdd((unsigned long long *)&tail[T_RANK + i], (unsigned long long)hist[i]) : 0ull;
         __syncthreads();
-        if (cls) reloc[g] = base[cls] + ((uint64_t)rank << cls);
+        if (cls) {
+            const uint64_t i = base[cls] + rank, take = tail[T_TAKE + cls];
+            if (i < take) {
+                const uint64_t slot = tail[T_PSTART + cls] + i;
+                reloc[g] = pool[slot] & kPoolOff;
+                pool[slot] = ~0ull;   // taken
+            } else {
+                reloc[g] = tail[T_CURSOR + cls] + ((i - take) << cls);
+            }
+        }
         __syncthreads();
     }
+}
+
+// A relocated row's old block joins the free pool (appended past the pool's
+// live entries); the next batch may reuse it (source_local_chunk_store.h:410-433
+// retires a block until the next batch in the same way).
+__device__ __forceinline__ void retire_block(uint64_t *tail, uint64_t *pool, const RowDesc &old) {
+    const uint64_t i = atomicAdd((unsigned long long *)&tail[T_RETIRED], 1ull);
+    pool[i] = ((uint64_t)(old.meta & 63u) << 40) | old.off;
 }
 
 constexpr uint32_t kBigAppend = 4096;   // relocations of larger rows use one 1024-thread workgroup
@@ -907,8 +947,8 @@ __device__ __forceinline__ uint32_t class_of(uint64_t nd) {
 // Rows above kBigAppend entries that must relocate are listed for k_add_big.
 __global__ __launch_bounds__(256) void k_add_rows(StoreView s, RowDesc *desc, uint32_t *hot, const uint32_t *keys,
                                                   const uint32_t *vals, uint64_t count, const uint32_t *gstart,
-                                                  const unsigned long long *ngroups_p, const uint64_t *tail,
-                                                  const uint64_t *reloc, unsigned long long *counters,
+                                                  const unsigned long long *ngroups_p, uint64_t *tail,
+                                                  uint64_t *pool, const uint64_t *reloc, unsigned long long *counters,
                                                   int count_stats, uint32_t *big_groups, unsigned long long *nbig) {
     const uint32_t lane = lane_id();
     const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kWave;
@@ -935,6 +975,7 @@ __global__ __launch_bounds__(256) void k_add_rows(StoreView s, RowDesc *desc, ui
             // sorted prefix length of the row stays valid
             for (uint32_t i = lane; i < d.ext; i += kWave) store_slot(s, noff + i, load_slot(s, d.off + i));
             if (lane == 0 && d.ext) hot[row] += 1;
+            if (lane == 0 && d.meta) retire_block(tail, pool, d);
             d.off = noff;
             d.meta = make_meta(cls, row_tombs(d));
             relocated++;
@@ -958,8 +999,8 @@ __global__ __launch_bounds__(256) void k_add_rows(StoreView s, RowDesc *desc, ui
 __global__ __launch_bounds__(1024) void k_add_big(StoreView s, RowDesc *desc, uint32_t *hot, const uint32_t *keys,
                                                   const uint32_t *vals, uint64_t count, const uint32_t *gstart,
                                                   const uint32_t *big_groups, const unsigned long long *nbig_p,
-                                                  const uint64_t *reloc, unsigned long long *counters,
-                                                  int count_stats) {
+                                                  const uint64_t *reloc, uint64_t *tail, uint64_t *pool,
+                                                  unsigned long long *counters, int count_stats) {
     const uint64_t nbig = *nbig_p;
     for (uint64_t b = blockIdx.x; b < nbig; b += gridDim.x) {
         const uint64_t g = big_groups[b];
@@ -982,6 +1023,7 @@ __global__ __launch_bounds__(1024) void k_add_big(StoreView s, RowDesc *desc, ui
         __syncthreads();
         if (threadIdx.x == 0) {
             desc[row] = RowDesc{noff, (uint32_t)nd, make_meta(cls, row_tombs(d))};
+            retire_block(tail, pool, d);
             hot[row] += 1;
             if (count_stats) atomicAdd(&counters[C_RELOCATED], 1ull);
             atomicAdd(&counters[C_SLOTS_READ], (unsigned long long)d.ext);
@@ -1095,35 +1137,63 @@ void Engine::mutate_add(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
                        d_gstart.p, d_counters.p + C_GROUPS_OUT, d_tail.p);
     hipLaunchKernelGGL(k_add_plan, dim3(grid_for(count, 256, 1024)), dim3(256), 0, stream, d_in.p, d_k32b.p, count,
                        d_gstart2.p, d_counters.p + C_GROUPS_IN, d_tail.p);
+    // free pool: live entries and the blocks the previous batch retired, sorted
+    // by class (taken entries, ~0, sort last); this batch retires past them
+    fill<uint64_t>(stream, d_tail.p + T_FREE, kClasses, 0ull);
+    const uint64_t pool_live = pool_len - pool_taken;
+    if (pool_len) {
+        d_k64c.ensure(pool_len);
+        size_t pb = 0;
+        DCG_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, pb, d_pool.p, d_k64c.p, (int)pool_len, 0, 64, stream));
+        void *pt = temp(pb);
+        DCG_HIP(hipcub::DeviceRadixSort::SortKeys(pt, pb, d_pool.p, d_k64c.p, (int)pool_len, 0, 64, stream));
+        hipLaunchKernelGGL(k_pool_segments, dim3(grid_for(pool_live, 256, 4096)), dim3(256), 0, stream, d_k64c.p,
+                           pool_live, d_tail.p);
+    }
+    if (d_pool.n < pool_live + 2 * count + 1) {
+        DevBuf<uint64_t> grown;
+        grown.alloc(pool_live + 2 * count + 1 + pool_live / 2);
+        std::swap(grown.p, d_pool.p);
+        std::swap(grown.n, d_pool.n);
+    }
+    if (pool_live)
+        DCG_HIP(hipMemcpyAsync(d_pool.p, d_k64c.p, pool_live * 8, hipMemcpyDeviceToDevice, stream));
+    fill<uint64_t>(stream, d_tail.p + T_RETIRED, 1, pool_live);
     hipLaunchKernelGGL(k_alloc_classes, dim3(1), dim3(64), 0, stream, d_tail.p, total_slots);
     d_k64a.ensure(count);
     d_k64b.ensure(count);
     uint64_t *reloc_out = d_k64a.p, *reloc_in = d_k64b.p;
     hipLaunchKernelGGL(k_add_assign, dim3(grid_for(count, 256, 1024)), dim3(256), 0, stream, d_out.p, d_k32a.p, count,
-                       d_gstart.p, d_counters.p + C_GROUPS_OUT, d_tail.p, reloc_out);
+                       d_gstart.p, d_counters.p + C_GROUPS_OUT, d_tail.p, d_pool.p, reloc_out);
     hipLaunchKernelGGL(k_add_assign, dim3(grid_for(count, 256, 1024)), dim3(256), 0, stream, d_in.p, d_k32b.p, count,
-                       d_gstart2.p, d_counters.p + C_GROUPS_IN, d_tail.p, reloc_in);
+                       d_gstart2.p, d_counters.p + C_GROUPS_IN, d_tail.p, d_pool.p, reloc_in);
     StoreView v = view();
     const uint32_t wg = wave_grid(count);
     d_big_groups.ensure(count);
     const int slot = time_begin(DCG_K_APPEND_ROWS, -1);
     hipLaunchKernelGGL(k_add_rows, dim3(wg), dim3(256), 0, stream, v, d_out.p, d_hot_out.p, d_k32a.p, d_v32a.p, count,
-                       d_gstart.p, d_counters.p + C_GROUPS_OUT, d_tail.p, reloc_out, d_counters.p, 1, d_big_groups.p,
-                       d_counters.p + C_NBIG);
+                       d_gstart.p, d_counters.p + C_GROUPS_OUT, d_tail.p, d_pool.p, reloc_out, d_counters.p, 1,
+                       d_big_groups.p, d_counters.p + C_NBIG);
     hipLaunchKernelGGL(k_add_big, dim3(64), dim3(1024), 0, stream, v, d_out.p, d_hot_out.p, d_k32a.p, d_v32a.p, count,
-                       d_gstart.p, d_big_groups.p, d_counters.p + C_NBIG, reloc_out, d_counters.p, 1);
+                       d_gstart.p, d_big_groups.p, d_counters.p + C_NBIG, reloc_out, d_tail.p, d_pool.p, d_counters.p,
+                       1);
     fill<unsigned long long>(stream, d_counters.p + C_NBIG, 1, 0ull);
     hipLaunchKernelGGL(k_add_rows, dim3(wg), dim3(256), 0, stream, v, d_in.p, d_hot_in.p, d_k32b.p, d_v32b.p, count,
-                       d_gstart2.p, d_counters.p + C_GROUPS_IN, d_tail.p, reloc_in, d_counters.p, 1, d_big_groups.p,
-                       d_counters.p + C_NBIG);
+                       d_gstart2.p, d_counters.p + C_GROUPS_IN, d_tail.p, d_pool.p, reloc_in, d_counters.p, 1,
+                       d_big_groups.p, d_counters.p + C_NBIG);
     hipLaunchKernelGGL(k_add_big, dim3(64), dim3(1024), 0, stream, v, d_in.p, d_hot_in.p, d_k32b.p, d_v32b.p, count,
-                       d_gstart2.p, d_big_groups.p, d_counters.p + C_NBIG, reloc_in, d_counters.p, 1);
+                       d_gstart2.p, d_big_groups.p, d_counters.p + C_NBIG, reloc_in, d_tail.p, d_pool.p, d_counters.p,
+                       1);
     time_end(slot);
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipMemcpyAsync(h_counters, d_counters.p, C_NUM * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
-    uint64_t tl[2];
-    DCG_HIP(hipMemcpyAsync(tl, d_tail.p, 16, hipMemcpyDeviceToHost, stream));
+    uint64_t tl[T_NUM];
+    DCG_HIP(hipMemcpyAsync(tl, d_tail.p, sizeof(tl), hipMemcpyDeviceToHost, stream));
     sync();
+    pool_len = tl[T_RETIRED];
+    pool_taken = 0;
+    for (int c = 0; c < kClasses; ++c) pool_taken += tl[T_TAKE + c];
+    reused_blocks += pool_taken;
     timed[slot].bytes = 4.0 * (double)(h_counters[C_SLOTS_READ] + h_counters[C_SLOTS_WRITTEN]) +
                         16.0 * (double)(h_counters[C_GROUPS_OUT] + h_counters[C_GROUPS_IN]);
     if (tl[T_OVERFLOW]) throw Error(DCG_ERR_CAPACITY, "chunk store slot space exhausted (raise store_slack)");

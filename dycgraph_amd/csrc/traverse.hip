@@ -290,7 +290,8 @@ __device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L
     const uint32_t cur = r % 3, nxt = (r + 1) % 3;
     const uint32_t lane = lane_id();
     const uint32_t cnt = *(volatile uint32_t *)&a.ctl->count[cur];
-    const uint32_t *in = r == 0 ? a.in0 : ((r & 1) ? a.qa : a.qb);
+    // dense PageRank sweeps take every vertex each round (the fixed list in0)
+    const uint32_t *in = (r == 0 || MODE == M_PR_DENSE) ? a.in0 : ((r & 1) ? a.qa : a.qb);
     uint32_t *out = (r & 1) ? a.qb : a.qa;
     const uint32_t stamp = a.stamp0 + r;
     uint32_t batch = t.wave;

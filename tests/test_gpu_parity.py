@@ -401,3 +401,39 @@ def test_sorted_medium_rows_wave_search(placement):
         ab += len(al)
         db += len(dl)
     eng.close()
+
+
+def test_relocated_blocks_are_reused():
+    """A long stream of inserts and deletes: rows grow past their blocks again
+    and again.  Blocks of relocated rows are retired and reused from the next
+    batch on, so the store's bump tail stays bounded (without reuse it would
+    grow by every relocation), and rows and levels stay exact."""
+    announce("block reuse over 150 snapshots")
+    import dycgraph_amd as d
+    n, src, dst = rmat_graph(11, 12000, seed=4)
+    per = [600] * 150
+    adds, dels = d.gen_stream(n, src, dst, per, per, seed=12)
+    eng = _engine(app="bfs", placement=0)
+    eng.load_graph(n, src, dst)
+    eng.start()
+    eng.load_updates(adds, dels)
+    store = oracle.Store(n, src, dst)
+    used0 = eng.info()["store_used_slots"]
+    relocated = 0
+    ab = db = 0
+    for s, na in enumerate(per):
+        eng.del_edge(db, na)
+        st = eng.add_edge(ab, na)
+        relocated += st["relocated_rows"]
+        store.apply(dels[db:db + na], adds[ab:ab + na])
+        ab += na
+        db += na
+    used = eng.info()["store_used_slots"]
+    erp, ecol = eng.csr()
+    assert same_rows(erp, ecol, store)
+    rp, col = store.csr()
+    assert np.array_equal(eng.gather_value(), oracle.bfs(n, rp, col, 0))
+    # every relocation takes at least 4 fresh slots without reuse; with it the tail grows far less
+    assert relocated > 5000
+    assert used - used0 < 2 * relocated, (used0, used, relocated)
+    eng.close()
