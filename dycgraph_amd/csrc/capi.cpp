@@ -400,6 +400,8 @@ int dcg_get_info(dcg_engine *h, dcg_engine_info *info) {
                              e.d_tick.bytes() + e.d_qa.bytes() * 3 + e.d_aff.bytes() + e.d_big.bytes();
         info->cached_edge_reads = e.cum_cached_reads;
         info->host_edge_reads = e.cum_host_reads;
+        info->free_blocks = e.pool_len - e.pool_taken;
+        info->reused_blocks = e.reused_blocks;
     });
 }
 

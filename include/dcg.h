@@ -186,6 +186,7 @@ typedef struct {
     uint64_t snapshots, store_slots, store_used_slots, chunks, frames;
     uint64_t host_bytes, device_bytes;
     uint64_t cached_edge_reads, host_edge_reads;
+    uint64_t free_blocks, reused_blocks;   /* relocation blocks in the free pool / reused from it */
 } dcg_engine_info;
 int dcg_get_info(dcg_engine *eng, dcg_engine_info *info);
 
