@@ -66,8 +66,9 @@ def _stream(n, src, dst, snapshots, per, hub_churn=32, seed=9):
 
 def _run(policy, n, src, dst, adds, dels):
     import dycgraph_amd as d
+    # spare slots for the relocations of 200k inserts into a few regions
     eng = d.Engine(d.EngineConfig(app="bfs", sources=(0,), placement=d.PLACE_HOST, cache_policy=policy,
-                                  cache_bytes=8 * 65536 * 4))
+                                  cache_bytes=8 * 65536 * 4, store_slack=1.0))
     eng.load_graph(n, src, dst)
     eng.start()
     flat_a = [e for a in adds for e in a]
