@@ -17,6 +17,7 @@ constexpr uint64_t kChunkEdges = 1ull << kChunkLog2;
 constexpr uint32_t kMinRowCap = 4;                        // source_local_chunk_store.h:133-145
 constexpr uint32_t kWave = 64;
 constexpr uint32_t kBlock = 256;                          // 4 waves per workgroup
+constexpr uint32_t kPBlock = 1024;                        // persistent closures: 16 waves per workgroup
 constexpr uint32_t kBigRow = 256;                         // rows above this go to the piece list
 constexpr uint32_t kPiece = 512;                          // edges per big-row piece
 constexpr int kClasses = 40;                              // power-of-two capacity classes
@@ -155,6 +156,9 @@ struct Ctl {
     uint32_t advances;        // bucket advances taken
     unsigned long long far_scanned;   // far-pile entries examined by the advances
     uint32_t rmax[2];         // dense PageRank sweeps: largest |residual| (float bits) by round parity
+    uint32_t bar_arrive;      // grid barrier: workgroups arrived in the current generation
+    uint32_t bar_gen;         // grid barrier generation
+    uint32_t bar_timeout;     // a workgroup waited ~10 s at the barrier (the launch gives up)
 };
 
 struct BigEntry {

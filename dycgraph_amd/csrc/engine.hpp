@@ -81,7 +81,7 @@ struct Engine {
     int cu_count = 0;
     uint32_t coop_grid[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     struct Tune {
-        uint32_t blocks_per_cu = 2;   // persistent workgroups per CU
+        uint32_t blocks_per_cu = 1;   // persistent workgroups (1,024 threads) per CU
         uint32_t big_row = kBigRow;   // rows above this are split into pieces
         uint32_t piece = kPiece;
         uint32_t bucket_quarters = ~0u;   // push-closure bucket width: SSSP w_max*q/4, BFS q; 0 off, ~0 auto

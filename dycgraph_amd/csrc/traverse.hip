@@ -20,11 +20,7 @@
 //            search of the owner); larger rows are queued as 512-edge pieces.
 //   phase B: all waves share the pieces of the large rows.
 // New frontier entries are appended with one atomic per wave (ballot + mbcnt).
-#include <hip/hip_cooperative_groups.h>
-
 #include "engine.hpp"
-
-namespace cg = cooperative_groups;
 
 namespace dcg {
 
@@ -535,6 +531,36 @@ __device__ __forceinline__ bool pieces_pending(const TravArgs &a, uint32_t r) {
     return (*(volatile unsigned long long *)&a.ctl->big[r % 3] & kBigMask) != 0;
 }
 
+// Grid barrier of a persistent launch (co-resident workgroups: the launch is
+// cooperative).  One arrival counter and a generation word per launch in its
+// control block; agent-scope release before arriving and acquire after the
+// generation moves publish every workgroup's writes to all XCDs.  A waiter that
+// sees no progress for ~10 s (100 MHz wall clock) marks the launch timed out
+// and every later barrier returns at once, so the launch always ends.
+__device__ __forceinline__ void gsync(const TravArgs &a) {
+    __syncthreads();
+    if (threadIdx.x == 0 && !__hip_atomic_load(&a.ctl->bar_timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        uint32_t *arrive = &a.ctl->bar_arrive, *gen = &a.ctl->bar_gen;
+        const uint32_t g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (__hip_atomic_fetch_add(arrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1) {
+            __hip_atomic_store(arrive, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(gen, g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            const uint64_t t0 = wall_clock64();
+            while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                __builtin_amdgcn_s_sleep(1);
+                if (wall_clock64() - t0 > 1000000000ull) {
+                    __hip_atomic_store(&a.ctl->bar_timeout, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+}
+
 __device__ __forceinline__ uint32_t thr_of(uint32_t minv, uint32_t delta) {
     if (delta == kInf) return kInf;
     const uint64_t t = (uint64_t)minv + delta;
@@ -546,7 +572,7 @@ __device__ __forceinline__ uint32_t thr_of(uint32_t minv, uint32_t delta) {
 // its smallest far value + its bucket width, entries now below it become round
 // r's frontier and the rest move to the other pile for the next epoch.  Called
 // by the whole grid; returns false when the far pile is empty (closure done).
-__device__ bool far_advance(const TravArgs &a, uint32_t r, const Team &t, cg::grid_group &grid) {
+__device__ bool far_advance(const TravArgs &a, uint32_t r, const Team &t) {
     const uint32_t fp = *(volatile uint32_t *)&a.ctl->far_par;
     uint32_t fc = *(volatile uint32_t *)&a.ctl->far_count[fp];
     if (fc == 0 || r == 0) return false;
@@ -560,7 +586,7 @@ __device__ bool far_advance(const TravArgs &a, uint32_t r, const Team &t, cg::gr
         decode_g(a, g, q, v);
         atomicMin(&mf[q], (uint32_t)(a.st[g] >> 32));
     }
-    grid.sync();
+    gsync(a);
     const uint32_t ep = *(volatile uint32_t *)a.epoch + 1;
     if (t.tid == 0) {
         a.ctl->far_count[fp] = 0;
@@ -590,22 +616,22 @@ __device__ bool far_advance(const TravArgs &a, uint32_t r, const Team &t, cg::gr
         emit_to(a, dst, &a.ctl->far_count[fp ^ 1], far, g);
     }
     if (t.tid == 0) *(volatile uint32_t *)a.epoch = ep;
-    grid.sync();
+    gsync(a);
     return true;
 }
 
-constexpr uint32_t kSoloMax = 1024;      // frontier entries workgroup 0 handles alone
+constexpr uint32_t kSoloMax = 2048;      // frontier entries workgroup 0 handles alone
 constexpr uint32_t kSoloPieces = 64;     // more pending pieces than this end a solo stretch
 
-// One cooperative launch runs rounds until the frontier is empty.  While the
+// One cooperative launch (1,024-thread workgroups) runs rounds until the
+// frontier is empty, with the gsync grid barrier between rounds.  While the
 // frontier is small, workgroup 0 runs rounds by itself with workgroup barriers
 // (the rest of the grid waits at one grid barrier); otherwise all workgroups
 // share each round, separated by grid barriers (a second one only when the
 // round queued pieces of large rows).
 template <uint32_t MODE>
-__global__ __launch_bounds__(256) void k_persistent(TravArgs a) {
-    cg::grid_group grid = cg::this_grid();
-    __shared__ WaveLds lds_all[kBlock / kWave];
+__global__ __launch_bounds__(kPBlock) void k_persistent(TravArgs a) {
+    __shared__ WaveLds lds_all[kPBlock / kWave];
     const uint32_t lane = lane_id();
     WaveLds &L = lds_all[threadIdx.x / kWave];
     const Team gt{(blockIdx.x * blockDim.x + threadIdx.x) / kWave, gridDim.x * blockDim.x / kWave,
@@ -614,11 +640,12 @@ __global__ __launch_bounds__(256) void k_persistent(TravArgs a) {
     uint64_t edges = 0, expansions = 0;
     uint32_t r = 0, idle_advances = 0;
     while (r < a.max_rounds) {
+        if (*(volatile uint32_t *)&a.ctl->bar_timeout) break;
         const uint32_t cnt = *(volatile uint32_t *)&a.ctl->count[r % 3];
         if (cnt == 0) {
             // an advance always moves the smallest far entry below the new
             // threshold; two in a row without a round would mean none moved
-            if (MODE == M_PUSH && a.far0 && idle_advances < 2 && far_advance(a, r, gt, grid)) {
+            if (MODE == M_PUSH && a.far0 && idle_advances < 2 && far_advance(a, r, gt)) {
                 ++idle_advances;
                 continue;
             }
@@ -654,23 +681,23 @@ __global__ __launch_bounds__(256) void k_persistent(TravArgs a) {
                     a.ctl->solo_resume_b = resume_b;
                 }
             }
-            grid.sync();
+            gsync(a);
             r = *(volatile uint32_t *)&a.ctl->solo_round;
             if (*(volatile uint32_t *)&a.ctl->solo_resume_b) {
                 phase_b<MODE>(a, r, gt, edges);
-                grid.sync();
+                gsync(a);
                 ++r;
             }
             continue;
         }
         round_open<MODE>(a, r, gt);
-        if (MODE == M_PR || MODE == M_PR_DENSE) grid.sync();
+        if (MODE == M_PR || MODE == M_PR_DENSE) gsync(a);
         if (MODE == M_PR_DENSE && gt.tid == 0) a.ctl->rmax[(r + 1) & 1] = 0;   // all read it last round
         phase_a<MODE>(a, r, gt, L, edges, expansions);
-        grid.sync();
+        gsync(a);
         if (pieces_pending(a, r)) {
             phase_b<MODE>(a, r, gt, edges);
-            grid.sync();
+            gsync(a);
         }
         if (MODE == M_PR_DENSE) {
             // converged when no residual is above eps
@@ -678,7 +705,7 @@ __global__ __launch_bounds__(256) void k_persistent(TravArgs a) {
             for (uint32_t v = gt.tid; v < a.n; v += gt.nthreads) m = fmaxf(m, fabsf(a.res[v]));
             for (int d = 32; d >= 1; d >>= 1) m = fmaxf(m, __shfl_xor(m, d, kWave));
             if (lane == 0 && m > 0.f) atomicMax(&a.ctl->rmax[r & 1], __float_as_uint(m));
-            grid.sync();
+            gsync(a);
             const float rm = __uint_as_float(*(volatile uint32_t *)&a.ctl->rmax[r & 1]);
             ++r;
             if (!(rm > a.eps)) break;
@@ -715,6 +742,7 @@ __global__ void k_ctl_setup(Ctl *c, const uint32_t *src0, uint32_t add0, uint32_
     c->advances = 0;
     c->far_scanned = 0;
     c->rmax[0] = c->rmax[1] = 0;
+    c->bar_arrive = c->bar_gen = c->bar_timeout = 0;
 }
 
 // Bucket thresholds of a push closure: 0 (everything relaxed waits in the far
@@ -873,6 +901,11 @@ __global__ void k_seed_adds(TravArgs a, const dcg_edge *recs, uint64_t count, ui
 }
 
 // ------------------------------------------------------------------- host side
+static void check_launch(const Ctl &c, const char *stage) {
+    if (c.bar_timeout) throw Error(DCG_ERR_HIP, std::string("a grid barrier timed out in ") + stage);
+    if (c.overflow) throw Error(DCG_ERR_CAPACITY, std::string("frontier overflow in ") + stage);
+}
+
 static void cc_fix_parents(Engine &e, const TravArgs &a, uint32_t stamp_lo, const uint32_t *list, const Ctl *lc) {
     if (!e.d_par.p) return;
     const uint64_t work = (uint64_t)e.n * e.Q;
@@ -882,9 +915,9 @@ static void cc_fix_parents(Engine &e, const TravArgs &a, uint32_t stamp_lo, cons
 
 uint32_t Engine::coop_blocks(const void *kernel) {
     int per_cu = 0;
-    DCG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0));
+    DCG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kPBlock, 0));
     if (per_cu < 1) per_cu = 1;
-    const int want = tune.blocks_per_cu ? (int)tune.blocks_per_cu : 4;
+    const int want = tune.blocks_per_cu ? (int)tune.blocks_per_cu : 1;
     if (per_cu > want) per_cu = want;
     return (uint32_t)(per_cu * cu_count);
 }
@@ -909,7 +942,7 @@ void launch_persistent(Engine &e, uint32_t mode, TravArgs &a) {
     if (mode == M_PUSH && a.far0)
         hipLaunchKernelGGL(k_far_setup, dim3(1), dim3(64), 0, e.stream, a.thr, a.minfar, a.fdelta, a.Q);
     const int slot = e.time_begin(kid[mode], (int)(a.ctl - e.d_ctl.p));
-    DCG_HIP(hipLaunchCooperativeKernel(fn, dim3(e.coop_grid[mode]), dim3(kBlock), args, 0, e.stream));
+    DCG_HIP(hipLaunchCooperativeKernel(fn, dim3(e.coop_grid[mode]), dim3(kPBlock), args, 0, e.stream));
     e.time_end(slot);
 }
 
@@ -1049,7 +1082,7 @@ void Engine::autotune(dcg_tune_result *res) {
     kstat[DCG_K_PUSH] = dcg_kernel_stats{};
     double best = measure();
     uint32_t evals = 1;
-    const uint32_t bpc[] = {1, 2, 3, 4};
+    const uint32_t bpc[] = {1, 2};
     const uint32_t rows[] = {64, 128, 256, 512, 1024, 4096};
     const uint32_t pieces[] = {256, 512, 1024, 2048, 4096};
     const uint32_t quarters[] = {0, 1, 4, 16, 64};
@@ -1142,7 +1175,7 @@ void Engine::start_min_lattice() {
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipMemcpyAsync(h_ctl, c, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
     sync();
-    if (h_ctl->overflow) throw Error(DCG_ERR_CAPACITY, "frontier overflow during static convergence");
+    check_launch(h_ctl[0], "static convergence");
     fold_reads(nullptr);
 }
 
@@ -1189,7 +1222,7 @@ void Engine::delete_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
     DCG_HIP(hipMemcpyAsync(h_ctl, c, 4 * sizeof(Ctl), hipMemcpyDeviceToHost, stream));
     sync();
     for (int i = 0; i < 4; ++i)
-        if (h_ctl[i].overflow) throw Error(DCG_ERR_CAPACITY, "frontier overflow in the delete stage");
+        check_launch(h_ctl[i], "the delete stage");
     if (st) {
         st->affected = h_ctl[1].aff_count;
         st->rounds = h_ctl[1].rounds + h_ctl[3].rounds;
@@ -1223,7 +1256,8 @@ void Engine::add_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipMemcpyAsync(h_ctl + 4, c + 4, 2 * sizeof(Ctl), hipMemcpyDeviceToHost, stream));
     sync();
-    if (h_ctl[4].overflow || h_ctl[5].overflow) throw Error(DCG_ERR_CAPACITY, "frontier overflow in the add stage");
+    check_launch(h_ctl[4], "the add stage");
+    check_launch(h_ctl[5], "the add stage");
     if (st) {
         st->rounds = h_ctl[5].rounds;
         st->bucket_advances = h_ctl[5].advances;
@@ -1290,7 +1324,7 @@ void Engine::start_pagerank() {
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipMemcpyAsync(h_ctl + 6, c + 6, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
     sync();
-    if (h_ctl[6].overflow) throw Error(DCG_ERR_CAPACITY, "PR frontier overflow");
+    check_launch(h_ctl[6], "PageRank");
     if (h_ctl[6].bad) throw Error(DCG_ERR_NUMERIC, "PR non-finite rank/residual");
     const uint32_t r = h_ctl[6].rounds;
     // the unfinished frontier (if the round cap was hit) is the output of round r-1
@@ -1348,6 +1382,7 @@ void Engine::pagerank_dense(Ctl *c, dcg_phase_stats *st) {
     DCG_HIP(hipMemcpyAsync(h_ctl + (c - d_ctl.p), c, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
     sync();
     const Ctl &h = h_ctl[c - d_ctl.p];
+    check_launch(h, "PageRank");
     if (h.bad) throw Error(DCG_ERR_NUMERIC, "PR non-finite rank/residual");
     pr_pending = d_qc.p;
     pr_pending_count = h.aff_count;
@@ -1407,7 +1442,7 @@ void Engine::pagerank_batch(uint64_t ab, uint64_t ac, uint64_t db, uint64_t dc, 
         DCG_HIP(hipMemcpyAsync(h_ctl, c, 3 * sizeof(Ctl), hipMemcpyDeviceToHost, stream));
         sync();
         for (int i = 0; i < 3; ++i)
-            if (h_ctl[i].overflow) throw Error(DCG_ERR_CAPACITY, "PR frontier overflow");
+            check_launch(h_ctl[i], "PageRank");
         const uint64_t amend_edges = h_ctl[1].edges + h_ctl[2].edges;
         const uint64_t touched = h_ctl[0].aff_count;
         pagerank_dense(c + 3, st);
@@ -1434,7 +1469,7 @@ void Engine::pagerank_batch(uint64_t ab, uint64_t ac, uint64_t db, uint64_t dc, 
     DCG_HIP(hipMemcpyAsync(h_ctl, c, 4 * sizeof(Ctl), hipMemcpyDeviceToHost, stream));
     sync();
     for (int i = 0; i < 4; ++i) {
-        if (h_ctl[i].overflow) throw Error(DCG_ERR_CAPACITY, "PR frontier overflow");
+        check_launch(h_ctl[i], "PageRank");
         if (h_ctl[i].bad) throw Error(DCG_ERR_NUMERIC, "PR non-finite rank/residual");
     }
     const uint32_t r = h_ctl[3].rounds;
