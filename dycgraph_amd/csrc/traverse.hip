@@ -263,6 +263,110 @@ __device__ __forceinline__ void emit_far(const TravArgs &a, bool want, uint32_t 
     emit_to(a, fp ? a.far1 : a.far0, &a.ctl->far_count[fp], want, g);
 }
 
+// Workgroup output buffer of a persistent launch: waves append new frontier
+// (and far-pile) entries with LDS atomics and the workgroup publishes them
+// with one global atomic per phase (flush_out), instead of one global atomic
+// per wave and 64-edge chunk on a single counter.  A group that finds the
+// buffer full goes to global memory directly and leaves sentinels behind,
+// which the flush compacts away.
+constexpr uint32_t kOutCap = 4096, kFarCap = 2048;
+struct OutLds {
+    uint32_t n, nf, cn, cf, base, fbase, abase;
+    uint32_t near[kOutCap];
+    uint32_t far[kFarCap];
+};
+
+template <uint32_t MODE>
+__device__ __forceinline__ void emit_b(OutLds &O, const TravArgs &a, uint32_t *outq, uint32_t *count, bool want,
+                                       uint32_t g) {
+    const uint64_t mask = __ballot(want);
+    if (!mask) return;
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    const uint32_t pc = (uint32_t)__popcll(mask), r = lanes_below(mask);
+    uint32_t base = 0;
+    if ((int)lane_id() == leader) base = atomicAdd(&O.n, pc);
+    base = __shfl(base, leader, kWave);
+    if (base + pc <= kOutCap) {
+        if (want) O.near[base + r] = g;
+        return;
+    }
+    if (want && base + r < kOutCap) O.near[base + r] = kInf;
+    emit<MODE>(a, outq, count, want, g);
+}
+
+__device__ __forceinline__ void emit_far_b(OutLds &O, const TravArgs &a, bool want, uint32_t g) {
+    const uint64_t mask = __ballot(want);
+    if (!mask) return;
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    const uint32_t pc = (uint32_t)__popcll(mask), r = lanes_below(mask);
+    uint32_t base = 0;
+    if ((int)lane_id() == leader) base = atomicAdd(&O.nf, pc);
+    base = __shfl(base, leader, kWave);
+    if (base + pc <= kFarCap) {
+        if (want) O.far[base + r] = g;
+        return;
+    }
+    if (want && base + r < kFarCap) O.far[base + r] = kInf;
+    emit_far(a, want, g);
+}
+
+// Wave 0 drops the sentinels of an overflowed buffer in place; returns the count.
+__device__ __forceinline__ uint32_t compact_lds(uint32_t *buf, uint32_t n) {
+    uint32_t w = 0;
+    for (uint32_t i0 = 0; i0 < n; i0 += kWave) {
+        const uint32_t i = i0 + lane_id();
+        const uint32_t x = i < n ? buf[i] : kInf;
+        const uint64_t m = __ballot(x != kInf);
+        __builtin_amdgcn_wave_barrier();
+        if (x != kInf) buf[w + lanes_below(m)] = x;
+        __builtin_amdgcn_wave_barrier();
+        w += (uint32_t)__popcll(m);
+    }
+    return w;
+}
+
+// Publish the workgroup's buffered entries (all threads of the workgroup).
+template <uint32_t MODE>
+__device__ void flush_out(OutLds &O, const TravArgs &a, uint32_t *outq, uint32_t *count) {
+    __syncthreads();
+    if (threadIdx.x < kWave) {
+        const uint32_t cn = O.n <= kOutCap ? O.n : compact_lds(O.near, kOutCap);
+        const uint32_t cf = O.nf <= kFarCap ? O.nf : compact_lds(O.far, kFarCap);
+        if (threadIdx.x == 0) {
+            O.cn = cn;
+            O.cf = cf;
+            O.base = cn ? atomicAdd(count, cn) : 0u;
+            if (MODE == M_INVALIDATE) O.abase = cn ? atomicAdd(&a.ctl->aff_count, cn) : 0u;
+            if (cf) {
+                const uint32_t fp = *(volatile uint32_t *)&a.ctl->far_par;
+                O.fbase = atomicAdd(&a.ctl->far_count[fp], cf) | (fp << 31);
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t cn = O.cn, cf = O.cf;
+    for (uint32_t i = threadIdx.x; i < cn; i += blockDim.x) {
+        const uint32_t g = O.near[i];
+        if (O.base + i < a.qcap) outq[O.base + i] = g;
+        else a.ctl->overflow = 1;
+        if (MODE == M_INVALIDATE) {
+            if (O.abase + i < a.affcap) a.aff[O.abase + i] = g;
+            else a.ctl->overflow = 1;
+        }
+    }
+    if (cf) {
+        uint32_t *pile = (O.fbase >> 31) ? a.far1 : a.far0;
+        const uint32_t fb = O.fbase & 0x7FFFFFFFu;
+        for (uint32_t i = threadIdx.x; i < cf; i += blockDim.x) {
+            if (fb + i < a.qcap) pile[fb + i] = O.far[i];
+            else a.ctl->overflow = 1;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) O.n = O.nf = 0;
+    __syncthreads();
+}
+
 struct WaveLds {
     uint32_t pref[65];
     uint64_t val[64];
@@ -280,8 +384,10 @@ struct Team {
 // Phase A of round r: frontier entries in batches of 64 (the first batch of
 // every wave is static; further batches come from an atomic cursor), rows up to
 // `big_row` expanded edge-balanced in the wave, larger rows listed as pieces.
+constexpr uint32_t kClaim = 4;   // batches a wave claims per cursor atomic
+
 template <uint32_t MODE>
-__device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L, uint64_t &edges,
+__device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L, OutLds &O, uint64_t &edges,
                         uint64_t &expansions) {
     const uint32_t cur = r % 3, nxt = (r + 1) % 3;
     const uint32_t lane = lane_id();
@@ -290,7 +396,7 @@ __device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L
     const uint32_t *in = (r == 0 || MODE == M_PR_DENSE) ? a.in0 : ((r & 1) ? a.qa : a.qb);
     uint32_t *out = (r & 1) ? a.qb : a.qa;
     const uint32_t stamp = a.stamp0 + r;
-    uint32_t batch = t.wave;
+    uint32_t batch = t.wave, spare = 0;   // the first batch is static; then kClaim per atomic
     for (;;) {
         const uint32_t b = batch * kWave;
         if (b >= cnt) break;
@@ -399,20 +505,26 @@ __device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L
                 const uint32_t x = L.base[lo][k - L.pref[lo]];
                 kind = edge_op<MODE>(a, L.q[lo], L.owner[lo], L.val[lo], x, stamp, og);
             }
-            if (MODE != M_PULL) emit<MODE>(a, out, &a.ctl->count[nxt], kind == 1, og);
-            if (MODE == M_PUSH && a.far0) emit_far(a, kind == 2, og);
+            if (MODE != M_PULL) emit_b<MODE>(O, a, out, &a.ctl->count[nxt], kind == 1, og);
+            if (MODE == M_PUSH && a.far0) emit_far_b(O, a, kind == 2, og);
         }
         __builtin_amdgcn_wave_barrier();
-        uint32_t nb = 0;
-        if (lane == 0) nb = t.nwaves + atomicAdd(&a.ctl->cursor[cur], 1u);
-        batch = __shfl(nb, 0, kWave);
+        if (spare) {
+            ++batch;
+            --spare;
+        } else {
+            uint32_t nb = 0;
+            if (lane == 0) nb = t.nwaves + atomicAdd(&a.ctl->cursor[cur], kClaim);
+            batch = __shfl(nb, 0, kWave);
+            spare = kClaim - 1;
+        }
     }
 }
 
 // Phase B of round r: the pieces of the large rows (first piece of every wave
 // static, then an atomic cursor).
 template <uint32_t MODE>
-__device__ void phase_b(const TravArgs &a, uint32_t r, const Team &t, uint64_t &edges) {
+__device__ void phase_b(const TravArgs &a, uint32_t r, const Team &t, OutLds &O, uint64_t &edges) {
     const uint32_t cur = r % 3, nxt = (r + 1) % 3;
     const uint32_t lane = lane_id();
     uint32_t *out = (r & 1) ? a.qb : a.qa;
@@ -470,8 +582,8 @@ __device__ void phase_b(const TravArgs &a, uint32_t r, const Team &t, uint64_t &
                     uint32_t kind = 0;
                     uint32_t og = 0;
                     if (k < len) kind = edge_op<MODE>(a, q, v, be.val, base[k], stamp, og);
-                    emit<MODE>(a, out, &a.ctl->count[nxt], kind == 1, og);
-                    if (MODE == M_PUSH && a.far0) emit_far(a, kind == 2, og);
+                    emit_b<MODE>(O, a, out, &a.ctl->count[nxt], kind == 1, og);
+                    if (MODE == M_PUSH && a.far0) emit_far_b(O, a, kind == 2, og);
                 }
             }
         }
@@ -632,6 +744,14 @@ constexpr uint32_t kSoloPieces = 64;     // more pending pieces than this end a 
 template <uint32_t MODE>
 __global__ __launch_bounds__(kPBlock) void k_persistent(TravArgs a) {
     __shared__ WaveLds lds_all[kPBlock / kWave];
+    __shared__ OutLds O;
+    if (threadIdx.x == 0) O.n = O.nf = 0;
+    __syncthreads();
+    // publish round rr's buffered entries (modes that emit)
+    auto flush = [&](uint32_t rr) {
+        if (MODE != M_PULL && MODE != M_PR_DENSE)
+            flush_out<MODE>(O, a, (rr & 1) ? a.qb : a.qa, &a.ctl->count[(rr + 1) % 3]);
+    };
     const uint32_t lane = lane_id();
     WaveLds &L = lds_all[threadIdx.x / kWave];
     const Team gt{(blockIdx.x * blockDim.x + threadIdx.x) / kWave, gridDim.x * blockDim.x / kWave,
@@ -661,16 +781,16 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(TravArgs a) {
                 for (;;) {
                     round_open<MODE>(a, r, bt);
                     __syncthreads();
-                    phase_a<MODE>(a, r, bt, L, edges, expansions);
-                    __syncthreads();
+                    phase_a<MODE>(a, r, bt, L, O, edges, expansions);
+                    flush(r);
                     const uint64_t np = *(volatile unsigned long long *)&a.ctl->big[r % 3] & kBigMask;
                     if (np > kSoloPieces) {
                         resume_b = 1;   // the whole grid takes this round's pieces
                         break;
                     }
                     if (np) {
-                        phase_b<MODE>(a, r, bt, edges);
-                        __syncthreads();
+                        phase_b<MODE>(a, r, bt, O, edges);
+                        flush(r);
                     }
                     ++r;
                     const uint32_t c2 = *(volatile uint32_t *)&a.ctl->count[r % 3];
@@ -684,7 +804,8 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(TravArgs a) {
             gsync(a);
             r = *(volatile uint32_t *)&a.ctl->solo_round;
             if (*(volatile uint32_t *)&a.ctl->solo_resume_b) {
-                phase_b<MODE>(a, r, gt, edges);
+                phase_b<MODE>(a, r, gt, O, edges);
+                flush(r);
                 gsync(a);
                 ++r;
             }
@@ -693,10 +814,12 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(TravArgs a) {
         round_open<MODE>(a, r, gt);
         if (MODE == M_PR || MODE == M_PR_DENSE) gsync(a);
         if (MODE == M_PR_DENSE && gt.tid == 0) a.ctl->rmax[(r + 1) & 1] = 0;   // all read it last round
-        phase_a<MODE>(a, r, gt, L, edges, expansions);
+        phase_a<MODE>(a, r, gt, L, O, edges, expansions);
+        flush(r);
         gsync(a);
         if (pieces_pending(a, r)) {
-            phase_b<MODE>(a, r, gt, edges);
+            phase_b<MODE>(a, r, gt, O, edges);
+            flush(r);
             gsync(a);
         }
         if (MODE == M_PR_DENSE) {
