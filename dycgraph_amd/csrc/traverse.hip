@@ -747,10 +747,13 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(TravArgs a) {
     __shared__ OutLds O;
     if (threadIdx.x == 0) O.n = O.nf = 0;
     __syncthreads();
-    // publish round rr's buffered entries (modes that emit)
+    // publish round rr's buffered entries (modes that emit); a workgroup
+    // barrier either way, so the phase's counters are complete afterwards
     auto flush = [&](uint32_t rr) {
         if (MODE != M_PULL && MODE != M_PR_DENSE)
             flush_out<MODE>(O, a, (rr & 1) ? a.qb : a.qa, &a.ctl->count[(rr + 1) % 3]);
+        else
+            __syncthreads();
     };
     const uint32_t lane = lane_id();
     WaveLds &L = lds_all[threadIdx.x / kWave];
