@@ -277,8 +277,13 @@ struct OutLds {
 };
 
 template <uint32_t MODE>
-__device__ __forceinline__ void emit_b(OutLds &O, const TravArgs &a, uint32_t *outq, uint32_t *count, bool want,
+__device__ __forceinline__ void emit_b(OutLds *Op, const TravArgs &a, uint32_t *outq, uint32_t *count, bool want,
                                        uint32_t g) {
+    if (!Op) {   // workgroup 0 alone: direct appends
+        emit<MODE>(a, outq, count, want, g);
+        return;
+    }
+    OutLds &O = *Op;
     const uint64_t mask = __ballot(want);
     if (!mask) return;
     const int leader = __ffsll((unsigned long long)mask) - 1;
@@ -294,7 +299,12 @@ __device__ __forceinline__ void emit_b(OutLds &O, const TravArgs &a, uint32_t *o
     emit<MODE>(a, outq, count, want, g);
 }
 
-__device__ __forceinline__ void emit_far_b(OutLds &O, const TravArgs &a, bool want, uint32_t g) {
+__device__ __forceinline__ void emit_far_b(OutLds *Op, const TravArgs &a, bool want, uint32_t g) {
+    if (!Op) {
+        emit_far(a, want, g);
+        return;
+    }
+    OutLds &O = *Op;
     const uint64_t mask = __ballot(want);
     if (!mask) return;
     const int leader = __ffsll((unsigned long long)mask) - 1;
@@ -387,7 +397,7 @@ struct Team {
 constexpr uint32_t kClaim = 4;   // batches a wave claims per cursor atomic
 
 template <uint32_t MODE>
-__device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L, OutLds &O, uint64_t &edges,
+__device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L, OutLds *O, uint64_t &edges,
                         uint64_t &expansions) {
     const uint32_t cur = r % 3, nxt = (r + 1) % 3;
     const uint32_t lane = lane_id();
@@ -524,7 +534,7 @@ __device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L
 // Phase B of round r: the pieces of the large rows (first piece of every wave
 // static, then an atomic cursor).
 template <uint32_t MODE>
-__device__ void phase_b(const TravArgs &a, uint32_t r, const Team &t, OutLds &O, uint64_t &edges) {
+__device__ void phase_b(const TravArgs &a, uint32_t r, const Team &t, OutLds *O, uint64_t &edges) {
     const uint32_t cur = r % 3, nxt = (r + 1) % 3;
     const uint32_t lane = lane_id();
     uint32_t *out = (r & 1) ? a.qb : a.qa;
@@ -747,8 +757,8 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(TravArgs a) {
     __shared__ OutLds O;
     if (threadIdx.x == 0) O.n = O.nf = 0;
     __syncthreads();
-    // publish round rr's buffered entries (modes that emit); a workgroup
-    // barrier either way, so the phase's counters are complete afterwards
+    // publish round rr's buffered entries (modes that emit; whole-grid rounds
+    // only: workgroup 0 alone appends directly); a workgroup barrier either way
     auto flush = [&](uint32_t rr) {
         if (MODE != M_PULL && MODE != M_PR_DENSE)
             flush_out<MODE>(O, a, (rr & 1) ? a.qb : a.qa, &a.ctl->count[(rr + 1) % 3]);
@@ -784,16 +794,16 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(TravArgs a) {
                 for (;;) {
                     round_open<MODE>(a, r, bt);
                     __syncthreads();
-                    phase_a<MODE>(a, r, bt, L, O, edges, expansions);
-                    flush(r);
+                    phase_a<MODE>(a, r, bt, L, nullptr, edges, expansions);
+                    __syncthreads();
                     const uint64_t np = *(volatile unsigned long long *)&a.ctl->big[r % 3] & kBigMask;
                     if (np > kSoloPieces) {
                         resume_b = 1;   // the whole grid takes this round's pieces
                         break;
                     }
                     if (np) {
-                        phase_b<MODE>(a, r, bt, O, edges);
-                        flush(r);
+                        phase_b<MODE>(a, r, bt, nullptr, edges);
+                        __syncthreads();
                     }
                     ++r;
                     const uint32_t c2 = *(volatile uint32_t *)&a.ctl->count[r % 3];
@@ -807,7 +817,7 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(TravArgs a) {
             gsync(a);
             r = *(volatile uint32_t *)&a.ctl->solo_round;
             if (*(volatile uint32_t *)&a.ctl->solo_resume_b) {
-                phase_b<MODE>(a, r, gt, O, edges);
+                phase_b<MODE>(a, r, gt, &O, edges);
                 flush(r);
                 gsync(a);
                 ++r;
@@ -817,11 +827,11 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(TravArgs a) {
         round_open<MODE>(a, r, gt);
         if (MODE == M_PR || MODE == M_PR_DENSE) gsync(a);
         if (MODE == M_PR_DENSE && gt.tid == 0) a.ctl->rmax[(r + 1) & 1] = 0;   // all read it last round
-        phase_a<MODE>(a, r, gt, L, O, edges, expansions);
+        phase_a<MODE>(a, r, gt, L, &O, edges, expansions);
         flush(r);
         gsync(a);
         if (pieces_pending(a, r)) {
-            phase_b<MODE>(a, r, gt, O, edges);
+            phase_b<MODE>(a, r, gt, &O, edges);
             flush(r);
             gsync(a);
         }
