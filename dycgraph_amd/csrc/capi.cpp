@@ -23,6 +23,8 @@ Engine::Engine(const dcg_config &c) : cfg(c) {
     DCG_HIP(hipGetDeviceProperties(&prop, device));
     cu_count = prop.multiProcessorCount;
     DCG_HIP(hipHostMalloc((void **)&h_ctl, 8 * sizeof(Ctl), hipHostMallocDefault));
+    DCG_HIP(hipHostMalloc((void **)&h_step, 2 * sizeof(StepArgs), hipHostMallocDefault));
+    d_step.alloc(2);
     DCG_HIP(hipHostMalloc((void **)&h_counters, 16 * sizeof(unsigned long long), hipHostMallocDefault));
     app = c.app;
     const bool mixed = c.query_apps && app != DCG_APP_PR;
@@ -89,6 +91,8 @@ Engine::~Engine() {
     for (hipEvent_t ev : events) (void)hipEventDestroy(ev);
     if (h_slab) (void)hipHostFree(h_slab);
     if (h_ctl) (void)hipHostFree(h_ctl);
+    if (h_step) (void)hipHostFree(h_step);
+    drop_graphs();
     if (h_counters) (void)hipHostFree(h_counters);
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -438,6 +442,14 @@ int dcg_set_tuning(dcg_engine *h, uint32_t blocks_per_cu, uint32_t big_row, uint
         if (big_row) e.tune.big_row = big_row;
         if (piece) e.tune.piece = piece;
         for (auto &g : e.coop_grid) g = 0;
+        e.drop_graphs();
+    });
+}
+
+int dcg_set_graphs(dcg_engine *h, int on) {
+    return guarded(h, [&](Engine &e) {
+        e.tune.graphs = on ? 1u : 0u;
+        e.drop_graphs();
     });
 }
 
@@ -446,6 +458,7 @@ int dcg_set_buckets(dcg_engine *h, uint32_t quarters) {
         if (quarters > 4096 && quarters != ~0u) throw Error(DCG_ERR_ARG, "bucket width out of range");
         e.tune.bucket_quarters = quarters;
         e.update_buckets();
+        e.drop_graphs();
     });
 }
 

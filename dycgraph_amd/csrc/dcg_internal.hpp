@@ -161,6 +161,14 @@ struct Ctl {
     uint32_t bar_timeout;     // a workgroup waited ~10 s at the barrier (the launch gives up)
 };
 
+// Per-stage arguments read by the kernels from device memory, so one captured
+// graph of a stage's launches serves every snapshot: the stage's records
+// (begin, count) and the round-stamp bases of its closures.
+struct StepArgs {
+    uint64_t begin, count;
+    uint32_t stamp[4];
+};
+
 struct BigEntry {
     uint64_t off;      // first slot of the row
     uint64_t val;      // owner state word (min-lattice) or float bits (PR delta/deg)

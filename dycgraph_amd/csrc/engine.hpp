@@ -84,6 +84,7 @@ struct Engine {
         uint32_t blocks_per_cu = 1;   // persistent workgroups (1,024 threads) per CU
         uint32_t big_row = kBigRow;   // rows above this are split into pieces
         uint32_t piece = kPiece;
+        uint32_t graphs = 1;            // replay each stage's repair launches as a captured hipGraph
         uint32_t bucket_quarters = ~0u;   // push-closure bucket width: SSSP w_max*q/4, BFS q; 0 off, ~0 auto
     } tune;
 
@@ -130,6 +131,13 @@ struct Engine {
     DevBuf<BigEntry> d_big;
     DevBuf<Ctl> d_ctl;
     Ctl *h_ctl = nullptr;              // pinned readback
+    StepArgs *h_step = nullptr;        // pinned: [0] delete stage, [1] add stage
+    DevBuf<StepArgs> d_step;
+    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};   // captured repair launches of each stage
+    bool capturing = false;
+    void run_stage(int stage);
+    void enqueue_repair(int stage);
+    void drop_graphs();
     // PageRank
     DevBuf<float> d_rank, d_res, d_delta;
     DevBuf<uint32_t> d_queued;

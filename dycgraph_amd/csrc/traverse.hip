@@ -46,6 +46,8 @@ struct TravArgs {
     uint64_t bigcap;
     Ctl *ctl;
     uint32_t stamp0;
+    const StepArgs *step;      // when set, the round-stamp base is step->stamp[stamp_slot]
+    uint32_t stamp_slot;
     uint32_t big_row;          // rows above this many entries are split into pieces
     uint32_t piece;            // piece size (power of two dividing 65,536)
     // bucketed push closure: values at or above thr[q] wait in a far pile until
@@ -65,6 +67,10 @@ struct TravArgs {
 
 constexpr uint64_t kBigShift = 36;
 constexpr uint64_t kBigMask = (1ull << kBigShift) - 1;
+
+__device__ __forceinline__ uint32_t stamp_base(const TravArgs &a) {
+    return a.step ? a.step->stamp[a.stamp_slot] : a.stamp0;
+}
 
 __device__ __forceinline__ void decode_g(const TravArgs &a, uint32_t g, uint32_t &q, uint32_t &v) {
     if (a.Q == 1) {
@@ -405,7 +411,7 @@ __device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L
     // dense PageRank sweeps take every vertex each round (the fixed list in0)
     const uint32_t *in = (r == 0 || MODE == M_PR_DENSE) ? a.in0 : ((r & 1) ? a.qa : a.qb);
     uint32_t *out = (r & 1) ? a.qb : a.qa;
-    const uint32_t stamp = a.stamp0 + r;
+    const uint32_t stamp = stamp_base(a) + r;
     uint32_t batch = t.wave, spare = 0;   // the first batch is static; then kClaim per atomic
     for (;;) {
         const uint32_t b = batch * kWave;
@@ -538,7 +544,7 @@ __device__ void phase_b(const TravArgs &a, uint32_t r, const Team &t, OutLds *O,
     const uint32_t cur = r % 3, nxt = (r + 1) % 3;
     const uint32_t lane = lane_id();
     uint32_t *out = (r & 1) ? a.qb : a.qa;
-    const uint32_t stamp = a.stamp0 + r;
+    const uint32_t stamp = stamp_base(a) + r;
     const unsigned long long bp = *(volatile unsigned long long *)&a.ctl->big[cur];
     uint64_t nb = bp >> kBigShift;
     if (nb > a.bigcap) nb = a.bigcap;
@@ -925,7 +931,9 @@ __global__ void k_iota_sources(uint32_t *q0, uint32_t n, uint32_t Q, const uint3
 }
 
 // Delete seeds (driver.cuh:394-441): test every deleted edge against every query.
-__global__ void k_seed_dels(TravArgs a, const dcg_edge *recs, uint64_t count) {
+__global__ void k_seed_dels(TravArgs a, const dcg_edge *base) {
+    const uint64_t count = a.step->count;
+    const dcg_edge *recs = base + a.step->begin;
     const uint64_t total = count * a.Q;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t b = blockIdx.x * (uint64_t)blockDim.x; b < total; b += stride) {
@@ -970,7 +978,8 @@ __global__ void k_reset_affected(uint64_t *st, const uint32_t *aff, const Ctl *c
 // After a closure every CC vertex it touched (tick >= stamp_lo, or on the
 // affected list) has its parent checked: kept if tight (equal label, depth - 1),
 // else replaced by the first tight in-neighbour; depth 0 is a root (none).
-__global__ void k_cc_fix_parents(TravArgs a, uint32_t stamp_lo, const uint32_t *list, const Ctl *list_ctl) {
+__global__ void k_cc_fix_parents(TravArgs a, const uint32_t *list, const Ctl *list_ctl) {
+    const uint32_t stamp_lo = stamp_base(a);
     const uint64_t total = list ? (uint64_t)list_ctl->aff_count : (uint64_t)a.n * a.Q;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
          i += (uint64_t)gridDim.x * blockDim.x) {
@@ -1006,7 +1015,10 @@ __global__ void k_cc_fix_parents(TravArgs a, uint32_t stamp_lo, const uint32_t *
 }
 
 // Insertion seeds (framework.cuh:297-370): relax every added edge for every query.
-__global__ void k_seed_adds(TravArgs a, const dcg_edge *recs, uint64_t count, uint32_t *outq) {
+__global__ void k_seed_adds(TravArgs a, const dcg_edge *base, uint32_t *outq) {
+    const uint64_t count = a.step->count;
+    const dcg_edge *recs = base + a.step->begin;
+    const uint32_t stamp0 = stamp_base(a);
     const uint64_t total = count * a.Q;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t b = blockIdx.x * (uint64_t)blockDim.x; b < total; b += stride) {
@@ -1018,7 +1030,7 @@ __global__ void k_seed_adds(TravArgs a, const dcg_edge *recs, uint64_t count, ui
             const dcg_edge e = recs[i - (uint64_t)q * count];
             if (e.u < a.n && e.v < a.n) {
                 const uint64_t sv = a.st[(uint64_t)q * a.n + e.u];
-                if ((uint32_t)(sv >> 32) != kInf) want = push_op(a, q, e.u, sv, e.v, a.stamp0, og, false) == 1;
+                if ((uint32_t)(sv >> 32) != kInf) want = push_op(a, q, e.u, sv, e.v, stamp0, og, false) == 1;
             }
         }
         const uint64_t mask = __ballot(want);
@@ -1042,11 +1054,11 @@ static void check_launch(const Ctl &c, const char *stage) {
     if (c.overflow) throw Error(DCG_ERR_CAPACITY, std::string("frontier overflow in ") + stage);
 }
 
-static void cc_fix_parents(Engine &e, const TravArgs &a, uint32_t stamp_lo, const uint32_t *list, const Ctl *lc) {
+// `a`'s stamp base (stamp0, or its step slot) is the first stamp of the phase.
+static void cc_fix_parents(Engine &e, const TravArgs &a, const uint32_t *list, const Ctl *lc) {
     if (!e.d_par.p) return;
     const uint64_t work = (uint64_t)e.n * e.Q;
-    hipLaunchKernelGGL(k_cc_fix_parents, dim3(grid_for(work, 256, 16384)), dim3(256), 0, e.stream, a, stamp_lo, list,
-                       lc);
+    hipLaunchKernelGGL(k_cc_fix_parents, dim3(grid_for(work, 256, 16384)), dim3(256), 0, e.stream, a, list, lc);
 }
 
 uint32_t Engine::coop_blocks(const void *kernel) {
@@ -1077,6 +1089,12 @@ void launch_persistent(Engine &e, uint32_t mode, TravArgs &a) {
                                DCG_K_PR_SOLVE};
     if (mode == M_PUSH && a.far0)
         hipLaunchKernelGGL(k_far_setup, dim3(1), dim3(64), 0, e.stream, a.thr, a.minfar, a.fdelta, a.Q);
+    if (e.capturing) {
+        // graph node: a plain launch of the same co-resident grid (the grid
+        // barrier needs only co-residency; it gives up after ~10 s otherwise)
+        DCG_HIP(hipLaunchKernel(fn, dim3(e.coop_grid[mode]), dim3(kPBlock), args, 0, e.stream));
+        return;
+    }
     const int slot = e.time_begin(kid[mode], (int)(a.ctl - e.d_ctl.p));
     DCG_HIP(hipLaunchCooperativeKernel(fn, dim3(e.coop_grid[mode]), dim3(kPBlock), args, 0, e.stream));
     e.time_end(slot);
@@ -1164,6 +1182,7 @@ void Engine::update_buckets() {
 }
 
 void Engine::alloc_state() {
+    drop_graphs();
     const uint64_t qn = (uint64_t)n * Q;
     if (qn >= 0xFFFFFFFFull) throw Error(DCG_ERR_UNSUPPORTED, "num_queries * num_nodes must be below 2^32");
     const uint64_t cap = qn ? qn : 1;
@@ -1207,6 +1226,7 @@ void Engine::autotune(dcg_tune_result *res) {
     if (app == DCG_APP_PR) throw Error(DCG_ERR_UNSUPPORTED, "the autotuner tunes the min-lattice closure");
     auto measure = [&]() {
         for (auto &g : coop_grid) g = 0;
+        drop_graphs();
         start_min_lattice();
         sync();
         collect_timing();
@@ -1307,7 +1327,7 @@ void Engine::start_min_lattice() {
     a.ctl = c;
     a.stamp0 = take_stamps(*this);
     launch_persistent(*this, M_PUSH, a);
-    cc_fix_parents(*this, a, a.stamp0, nullptr, nullptr);
+    cc_fix_parents(*this, a, nullptr, nullptr);
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipMemcpyAsync(h_ctl, c, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
     sync();
@@ -1322,38 +1342,10 @@ void Engine::delete_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
     mutate_delete(begin, count, st);
     if (!count) return;
     reserve_stamps(*this, 2);
+    h_step[0] = StepArgs{begin, count, {take_stamps(*this), take_stamps(*this), 0u, 0u}};
+    DCG_HIP(hipMemcpyAsync(d_step.p, h_step, sizeof(StepArgs), hipMemcpyHostToDevice, stream));
+    run_stage(0);
     Ctl *c = d_ctl.p;
-    const dcg_edge *recs = d_dels.p + begin;
-    TravArgs a = base_args(*this);
-    // 1. seeds -> affected list (ctl[0].aff_count)
-    hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 0, (const uint32_t *)nullptr, 0u, 0u,
-                       (const uint32_t *)nullptr);
-    a.ctl = c + 0;
-    hipLaunchKernelGGL(k_seed_dels, dim3(grid_for(count * Q)), dim3(256), 0, stream, a, recs, count);
-    // 2. invalidation closure over out-rows (frontier = newly affected)
-    hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 1, &c[0].aff_count, 0u, 0u, &c[0].aff_count);
-    a.ctl = c + 1;
-    a.in0 = d_aff.p;
-    a.stamp0 = take_stamps(*this);
-    launch_persistent(*this, M_INVALIDATE, a);
-    // 3. reset affected values
-    hipLaunchKernelGGL(k_reset_affected, dim3(grid_for((uint64_t)n * Q, 256, 8192)), dim3(256), 0, stream, d_st.p,
-                       d_aff.p, c + 1, n, Q, d_qapps.p);
-    // 4. one pull pass over the in-rows of the affected vertices
-    hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 2, &c[1].aff_count, 0u, 0u,
-                       (const uint32_t *)nullptr);
-    a.ctl = c + 2;
-    a.max_rounds = 1;
-    launch_persistent(*this, M_PULL, a);
-    // 5. push closure from every affected vertex
-    hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 3, &c[1].aff_count, 0u, 0u,
-                       (const uint32_t *)nullptr);
-    a.ctl = c + 3;
-    a.max_rounds = 1u << 20;
-    a.stamp0 = take_stamps(*this);
-    launch_persistent(*this, M_PUSH, a);
-    cc_fix_parents(*this, a, a.stamp0, d_aff.p, c + 1);
-    cc_fix_parents(*this, a, a.stamp0, nullptr, nullptr);
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipMemcpyAsync(h_ctl, c, 4 * sizeof(Ctl), hipMemcpyDeviceToHost, stream));
     sync();
@@ -1373,22 +1365,10 @@ void Engine::add_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
     mutate_add(begin, count, st);
     if (!count) return;
     reserve_stamps(*this, 2);
+    h_step[1] = StepArgs{begin, count, {take_stamps(*this), take_stamps(*this), 0u, 0u}};
+    DCG_HIP(hipMemcpyAsync(d_step.p + 1, h_step + 1, sizeof(StepArgs), hipMemcpyHostToDevice, stream));
+    run_stage(1);
     Ctl *c = d_ctl.p;
-    const dcg_edge *recs = d_adds.p + begin;
-    TravArgs a = base_args(*this);
-    hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 4, (const uint32_t *)nullptr, 0u, 0u,
-                       (const uint32_t *)nullptr);
-    a.ctl = c + 4;
-    a.stamp0 = take_stamps(*this);
-    const uint32_t stamp_lo = a.stamp0;
-    hipLaunchKernelGGL(k_seed_adds, dim3(grid_for(count * Q)), dim3(256), 0, stream, a, recs, count, d_qc.p);
-    hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 5, &c[4].aff_count, 0u, 0u,
-                       (const uint32_t *)nullptr);
-    a.ctl = c + 5;
-    a.in0 = d_qc.p;
-    a.stamp0 = take_stamps(*this);
-    launch_persistent(*this, M_PUSH, a);
-    cc_fix_parents(*this, a, stamp_lo, nullptr, nullptr);
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipMemcpyAsync(h_ctl + 4, c + 4, 2 * sizeof(Ctl), hipMemcpyDeviceToHost, stream));
     sync();
@@ -1399,6 +1379,101 @@ void Engine::add_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
         st->bucket_advances = h_ctl[5].advances;
         st->far_entries = h_ctl[5].far_scanned;
         st->edges_scanned = h_ctl[5].edges;
+    }
+}
+
+// The repair launches of a stage, reading the batch and the round stamps from
+// d_step[stage].  Delete (framework.cuh:4811-4957, re-ordered: the physical
+// deletion has run; the invalidation scans post-delete rows, which only drops
+// deleted edges that the seed step already examined): seeds -> invalidation
+// closure -> reset -> one pull pass -> push closure.  Add: seeds -> push closure.
+void Engine::enqueue_repair(int stage) {
+    Ctl *c = d_ctl.p;
+    TravArgs a = base_args(*this);
+    a.step = d_step.p + stage;
+    constexpr uint32_t kSeedBlocks = 2048;   // grid-stride over count * Q records
+    if (stage == 0) {
+        hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 0, (const uint32_t *)nullptr, 0u, 0u,
+                           (const uint32_t *)nullptr);
+        a.ctl = c + 0;
+        hipLaunchKernelGGL(k_seed_dels, dim3(kSeedBlocks), dim3(256), 0, stream, a, (const dcg_edge *)d_dels.p);
+        hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 1, &c[0].aff_count, 0u, 0u,
+                           &c[0].aff_count);
+        a.ctl = c + 1;
+        a.in0 = d_aff.p;
+        a.stamp_slot = 0;
+        launch_persistent(*this, M_INVALIDATE, a);
+        hipLaunchKernelGGL(k_reset_affected, dim3(grid_for((uint64_t)n * Q, 256, 8192)), dim3(256), 0, stream,
+                           d_st.p, d_aff.p, c + 1, n, Q, d_qapps.p);
+        hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 2, &c[1].aff_count, 0u, 0u,
+                           (const uint32_t *)nullptr);
+        a.ctl = c + 2;
+        a.max_rounds = 1;
+        launch_persistent(*this, M_PULL, a);
+        hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 3, &c[1].aff_count, 0u, 0u,
+                           (const uint32_t *)nullptr);
+        a.ctl = c + 3;
+        a.max_rounds = 1u << 20;
+        a.stamp_slot = 1;
+        launch_persistent(*this, M_PUSH, a);
+        cc_fix_parents(*this, a, d_aff.p, c + 1);
+        cc_fix_parents(*this, a, nullptr, nullptr);
+    } else {
+        hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 4, (const uint32_t *)nullptr, 0u, 0u,
+                           (const uint32_t *)nullptr);
+        a.ctl = c + 4;
+        a.stamp_slot = 0;
+        hipLaunchKernelGGL(k_seed_adds, dim3(kSeedBlocks), dim3(256), 0, stream, a, (const dcg_edge *)d_adds.p,
+                           d_qc.p);
+        hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 5, &c[4].aff_count, 0u, 0u,
+                           (const uint32_t *)nullptr);
+        a.ctl = c + 5;
+        a.in0 = d_qc.p;
+        a.stamp_slot = 1;
+        launch_persistent(*this, M_PUSH, a);
+        a.stamp_slot = 0;   // the parent check covers the seeds' round too
+        cc_fix_parents(*this, a, nullptr, nullptr);
+    }
+}
+
+// Run a stage's repair launches: replay its captured graph (captured at the
+// first call; a capture the runtime rejects turns graphs off for the engine and
+// the launches are issued directly), or issue them directly.
+void Engine::run_stage(int stage) {
+    if (!tune.graphs) {
+        enqueue_repair(stage);
+        return;
+    }
+    if (!graph_exec[stage]) {
+        hipGraph_t g = nullptr;
+        bool ok = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        if (ok) {
+            capturing = true;
+            try {
+                enqueue_repair(stage);
+            } catch (...) {
+                ok = false;
+            }
+            capturing = false;
+            ok = hipStreamEndCapture(stream, &g) == hipSuccess && ok;
+        }
+        if (ok) ok = hipGraphInstantiate(&graph_exec[stage], g, nullptr, nullptr, 0) == hipSuccess;
+        if (g) (void)hipGraphDestroy(g);
+        if (!ok) {
+            (void)hipGetLastError();
+            graph_exec[stage] = nullptr;
+            tune.graphs = 0;
+            enqueue_repair(stage);
+            return;
+        }
+    }
+    DCG_HIP(hipGraphLaunch(graph_exec[stage], stream));
+}
+
+void Engine::drop_graphs() {
+    for (auto &g : graph_exec) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
     }
 }
 

@@ -137,6 +137,7 @@ def lib() -> C.CDLL:
         "dcg_autotune": (I, [P, C.POINTER(TuneResult)]),
         "dcg_set_tuning": (I, [P, U32, U32, U32]),
         "dcg_set_buckets": (I, [P, U32]),
+        "dcg_set_graphs": (I, [P, I]),
         "dcg_reset_kernel_stats": (I, [P]),
         "dcg_mlp_eval": (I, [C.c_int32, U32, P, P, P, P, P]),
         "dcg_gen_rmat": (I, [U32, U64, C.c_double, C.c_double, C.c_double, U64, P, P]),
@@ -411,10 +412,12 @@ class Engine:
         self._check(lib().dcg_autotune(self._h, C.byref(st)), "dcg_autotune")
         return st.as_dict()
 
-    def set_tuning(self, blocks_per_cu=0, big_row=0, piece=0, bucket_quarters=None):
+    def set_tuning(self, blocks_per_cu=0, big_row=0, piece=0, bucket_quarters=None, graphs=None):
         self._check(lib().dcg_set_tuning(self._h, blocks_per_cu, big_row, piece), "dcg_set_tuning")
         if bucket_quarters is not None:
             self._check(lib().dcg_set_buckets(self._h, bucket_quarters), "dcg_set_buckets")
+        if graphs is not None:
+            self._check(lib().dcg_set_graphs(self._h, 1 if graphs else 0), "dcg_set_graphs")
 
     def reset_kernel_stats(self):
         self._check(lib().dcg_reset_kernel_stats(self._h), "dcg_reset_kernel_stats")

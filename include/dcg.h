@@ -243,6 +243,13 @@ int dcg_set_tuning(dcg_engine *eng, uint32_t blocks_per_cu, uint32_t big_row, ui
  * 0xFFFFFFFF (the default) picks 64 on graphs whose largest out-degree is at
  * most 64 (grids, road networks) and 0 otherwise.  Values never depend on it. */
 int dcg_set_buckets(dcg_engine *eng, uint32_t quarters);
+/* Replay each stage's repair launches (seeds, invalidation, reset, pull, push
+ * closures, CC parent check) as one captured hipGraph (default on; no reference
+ * counterpart).  Per-snapshot arguments live in device memory, so the graph
+ * captured at the first snapshot serves all later ones.  With graphs on, the
+ * closures are not timed per launch (dcg_get_kernel_stats keeps the mutation
+ * kernels).  Values never depend on it. */
+int dcg_set_graphs(dcg_engine *eng, int on);
 
 /* Link microbenchmarks on `device`: HBM copy rate (read+write bytes / time of a
  * `bytes`-sized copy kernel) and zero-copy read rate of pinned host memory. */

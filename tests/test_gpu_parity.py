@@ -137,10 +137,12 @@ def test_large_row_deletes_exact_rows(dups, placement):
     eng.close()
 
 
-def test_mixed_programs_one_engine():
+@pytest.mark.parametrize("graphs", [True, False])
+def test_mixed_programs_one_engine(graphs):
     """SSSP, BFS and CC queries (several sources) repaired together over one chunk
-    store: every query bit-exact after every stage."""
-    announce("mixed programs: sssp x2, bfs x2, cc on rmat12 host-cache")
+    store: every query bit-exact after every stage, with each stage's repair
+    launches replayed from a captured graph or issued directly."""
+    announce(f"mixed programs: sssp x2, bfs x2, cc on rmat12 host-cache, graphs={graphs}")
     import dycgraph_amd as d
     n, src, dst = rmat_graph(12, 40000, seed=9)
     adds, dels = d.gen_stream(n, src, dst, [400] * 3, [400] * 3, seed=3)
@@ -148,6 +150,7 @@ def test_mixed_programs_one_engine():
     sources = (0, 0, 0, 17, 1234)
     eng = _engine(app="sssp", sources=sources, query_apps=apps, weight_mode=1, weight_seed=5, weight_max=50,
                   placement=1, cache_bytes=1 << 18)
+    eng.set_tuning(graphs=graphs)
     eng.load_graph(n, src, dst)
     eng.start()
     eng.load_updates(adds, dels)
