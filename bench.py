@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # timing category -> device kernel (rocprofv3 names)
 KERNEL_SYMBOL = {"push": "k_persistent<0u>", "pull": "k_persistent<1u>", "invalidate": "k_persistent<2u>",
                  "pr_solve": "k_persistent<3u>", "pr_amend": "k_persistent<4u>", "delete_rows": "k_del_rows",
-                 "delete_big": "k_del_big", "delete_single": "k_del_single", "append_rows": "k_add_rows + k_add_big"}
+                 "delete_big": "k_del_big", "append_rows": "k_add_rows + k_add_big"}
 # HBM traffic per launch measured with rocprofv3 PMC counters (FETCH_SIZE and
 # WRITE_SIZE runs, scaled by the counters of a copy kernel of known size)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic.json")

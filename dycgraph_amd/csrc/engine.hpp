@@ -156,9 +156,7 @@ struct Engine {
     DevBuf<uint32_t> d_gstart, d_gstart2, d_uniq;
     DevBuf<uint32_t> d_used;           // per-record occurrence counters
     DevBuf<uint32_t> d_big_groups;     // large-row mutation paths
-    DevBuf<uint32_t> d_defer;          // delete groups the lane path hands to the wave path
     DevBuf<unsigned long long> d_counters;  // phase counters (see store.hip)
-    static constexpr int kCounterSlots = 32;   // pinned mirror of d_counters
     unsigned long long *h_counters = nullptr;
 
     // cumulative counters

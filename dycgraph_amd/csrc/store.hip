@@ -34,13 +34,8 @@ enum Counter : int {
     C_W_WRITTEN = 13,
     C_W_ROWS = 14,
     C_B_ROWS = 15,       // workgroup-path delete rows
-    C_S_READ = 16,       // lane-path delete: slots read, rows changed (one slot written each)
-    C_S_ROWS = 17,
-    C_DEFER_OUT = 18,    // groups the lane path hands to the wave path, per direction
-    C_DEFER_IN = 19,
-    C_NUM = 20
+    C_NUM = 16
 };
-static_assert(C_NUM <= Engine::kCounterSlots, "pinned counter mirror too small");
 
 // d_tail layout
 // Store tail block (u64 words): bump pointer, overflow flag, and per capacity
@@ -615,78 +610,6 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
     }
 }
 
-// One lane per touched row, for groups with a single target: the lane
-// binary-searches the row's sorted prefix [0, S) and tombstones the first live
-// occurrence (the prefix precedes the tail, so it is the row's first live
-// occurrence).  Everything else goes, unchanged, to the list `defer` for
-// k_del_rows: groups of several targets, a target absent from the prefix, rows
-// without a sorted prefix, and rows where one more tombstone would cross the
-// compaction point.  64 searches per wave are in flight at once instead of one.
-__global__ __launch_bounds__(256) void k_del_single(StoreView s, RowDesc *desc, const uint32_t *sorted,
-                                                    const uint64_t *keys, uint64_t count, const uint32_t *gstart,
-                                                    const unsigned long long *ngroups_p, uint32_t *defer,
-                                                    unsigned long long *ndefer, unsigned long long *counters,
-                                                    int count_stats) {
-    const uint64_t ngroups = *ngroups_p;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    unsigned long long done_n = 0, rd = 0;
-    for (uint64_t base = blockIdx.x * (uint64_t)blockDim.x; base < ngroups; base += stride) {
-        const uint64_t g = base + threadIdx.x;
-        bool valid = g < ngroups, done = false;
-        uint32_t gs = 0;
-        if (valid) {
-            gs = gstart[g];
-            const uint64_t key = keys[gs];
-            const uint32_t row = (uint32_t)(key >> 32), v = (uint32_t)key;
-            const bool single = (uint64_t)gs + 1 >= count || (uint32_t)(keys[gs + 1] >> 32) != row;
-            if (single) {
-                const RowDesc d = desc[row];
-                const uint32_t S = sorted[row] < d.ext ? sorted[row] : d.ext;
-                const uint32_t tombs = row_tombs(d);
-                if (S && (uint64_t)(tombs + 1) * 4 <= d.ext && tombs + 1 <= kMaxTombs / 2) {
-                    uint32_t lo = 0, hi = S;
-                    while (lo < hi) {
-                        const uint32_t mid = (lo + hi) >> 1;
-                        rd++;
-                        if ((load_slot(s, d.off + mid) & ~kTomb) < v) lo = mid + 1;
-                        else hi = mid;
-                    }
-                    for (uint32_t j = lo; j < S; ++j) {
-                        const uint32_t x = load_slot(s, d.off + j);
-                        rd++;
-                        if ((x & ~kTomb) != v) break;
-                        if (!(x & kTomb)) {
-                            store_slot(s, d.off + j, x | kTomb);
-                            desc[row].meta = make_meta(d.meta & 63u, tombs + 1);
-                            done = true;
-                            break;
-                        }
-                    }
-                }
-            }
-        }
-        const bool want = valid && !done;
-        const uint64_t mask = __ballot(want);
-        if (mask) {
-            const int leader = __ffsll((unsigned long long)mask) - 1;
-            unsigned long long b = 0;
-            if ((int)lane_id() == leader) b = atomicAdd(ndefer, (unsigned long long)__popcll(mask));
-            b = __shfl(b, leader, kWave);
-            if (want) defer[b + lanes_below(mask)] = gs;
-        }
-        done_n += done ? 1 : 0;
-    }
-    for (int o = 32; o >= 1; o >>= 1) {
-        done_n += __shfl_xor(done_n, o, kWave);
-        rd += __shfl_xor(rd, o, kWave);
-    }
-    if (lane_id() == 0) {
-        if (count_stats && done_n) atomicAdd(&counters[C_CHANGED], done_n);
-        if (rd) atomicAdd(&counters[C_S_READ], rd);
-        if (done_n) atomicAdd(&counters[C_S_ROWS], done_n);
-    }
-}
-
 constexpr uint32_t kTileE = 4096;   // entries per workgroup tile of the large-row passes
 
 struct BigLds {
@@ -1147,15 +1070,9 @@ void Engine::mutate_delete(uint64_t begin, uint64_t count, dcg_phase_stats *st) 
     fill<uint32_t>(stream, d_used.p, count, 0u);
     // each launch timed on its own (HIP events); the bytes of both directions
     // go to the first launch of each kernel, so sums and averages hold
-    d_defer.ensure(count);
-    const uint32_t sg = grid_for(count);
-    const int s0 = time_begin(DCG_K_DELETE_SINGLE, -1);
-    hipLaunchKernelGGL(k_del_single, dim3(sg), dim3(256), 0, stream, v, d_out.p, d_sorted_out.p, kout, count,
-                       d_gstart.p, d_counters.p + C_GROUPS_OUT, d_defer.p, d_counters.p + C_DEFER_OUT, d_counters.p, 1);
-    time_end(s0);
     const int w0 = time_begin(DCG_K_DELETE_ROWS, -1);
     hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream, v, d_out.p, d_sorted_out.p, d_hot_out.p, kout,
-                       count, d_defer.p, d_counters.p + C_DEFER_OUT, d_used.p, d_counters.p, 1, d_big_groups.p,
+                       count, d_gstart.p, d_counters.p + C_GROUPS_OUT, d_used.p, d_counters.p, 1, d_big_groups.p,
                        d_counters.p + C_NBIG);
     time_end(w0);
     const int b0 = time_begin(DCG_K_DELETE_BIG, -1);
@@ -1164,14 +1081,9 @@ void Engine::mutate_delete(uint64_t begin, uint64_t count, dcg_phase_stats *st) 
     time_end(b0);
     fill<uint32_t>(stream, d_used.p, count, 0u);
     fill<unsigned long long>(stream, d_counters.p + C_NBIG, 1, 0ull);
-    // d_defer is reused: the launches above finish first (one stream)
-    const int s1 = time_begin(DCG_K_DELETE_SINGLE, -1);
-    hipLaunchKernelGGL(k_del_single, dim3(sg), dim3(256), 0, stream, v, d_in.p, d_sorted_in.p, kin, count,
-                       d_gstart2.p, d_counters.p + C_GROUPS_IN, d_defer.p, d_counters.p + C_DEFER_IN, d_counters.p, 0);
-    time_end(s1);
     const int w1 = time_begin(DCG_K_DELETE_ROWS, -1);
     hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream, v, d_in.p, d_sorted_in.p, d_hot_in.p, kin,
-                       count, d_defer.p, d_counters.p + C_DEFER_IN, d_used.p, d_counters.p, 0, d_big_groups.p,
+                       count, d_gstart2.p, d_counters.p + C_GROUPS_IN, d_used.p, d_counters.p, 0, d_big_groups.p,
                        d_counters.p + C_NBIG);
     time_end(w1);
     const int b1 = time_begin(DCG_K_DELETE_BIG, -1);
@@ -1183,8 +1095,6 @@ void Engine::mutate_delete(uint64_t begin, uint64_t count, dcg_phase_stats *st) 
     sync();
     timed[w0].bytes = 4.0 * (double)(h_counters[C_W_READ] + h_counters[C_W_WRITTEN]) + 16.0 * (double)h_counters[C_W_ROWS];
     timed[w1].bytes = 0.0;
-    timed[s0].bytes = 4.0 * (double)(h_counters[C_S_READ] + h_counters[C_S_ROWS]) + 16.0 * (double)h_counters[C_S_ROWS];
-    timed[s1].bytes = 0.0;
     timed[b0].bytes = 4.0 * (double)(h_counters[C_SLOTS_READ] + h_counters[C_SLOTS_WRITTEN]) +
                       16.0 * (double)h_counters[C_B_ROWS];
     timed[b1].bytes = 0.0;

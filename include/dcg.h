@@ -201,8 +201,7 @@ typedef enum {
     DCG_K_DELETE_ROWS = 5,   /* delete mutation, wave path (rows of at most 1,024 slots) */
     DCG_K_APPEND_ROWS = 6,   /* append / relocation, both paths */
     DCG_K_DELETE_BIG = 7,    /* delete mutation, workgroup path (larger rows) */
-    DCG_K_DELETE_SINGLE = 8, /* delete mutation, lane path (rows with one target) */
-    DCG_K_NUM = 9
+    DCG_K_NUM = 8
 } dcg_kernel_id;
 typedef struct {
     uint64_t launches;
