@@ -107,6 +107,10 @@ struct Engine {
     DevBuf<uint64_t> d_tail;           // [0] bump tail, [1] overflow flag, [2..] class cursors
     DevBuf<uint32_t> d_hot_out, d_hot_in;
     DevBuf<uint32_t> d_sorted_out, d_sorted_in;   // per row: length of the sorted prefix
+    // Fences of the sorted prefixes, in HBM: row r owns fence[fbase[r] .. fbase[r+1]),
+    // fence j holding the id at slot 16j of the row (valid for 16j below its
+    // current prefix length); rows whose prefix started below 64 have none.
+    DevBuf<uint32_t> d_fbase_out, d_fbase_in, d_fence_out, d_fence_in;
     DevBuf<uint64_t> d_pool;           // free blocks (class << 40 | offset) of relocated rows
     uint64_t pool_len = 0, pool_taken = 0, reused_blocks = 0;
     CacheState cache;
@@ -180,6 +184,8 @@ struct Engine {
     ~Engine();
 
     // store.hip
+    void build_fences(const uint32_t *keys, const uint32_t *vals, uint64_t m, const uint64_t *rp,
+                      const uint32_t *sorted, DevBuf<uint32_t> &fbase, DevBuf<uint32_t> &fence);
     void load_edges_device(uint32_t nn, const uint32_t *d_src, const uint32_t *d_dst, uint64_t m);
     void mutate_delete(uint64_t begin, uint64_t count, dcg_phase_stats *st);
     void mutate_add(uint64_t begin, uint64_t count, dcg_phase_stats *st);

@@ -136,6 +136,9 @@ __global__ void k_scatter_rows(const uint32_t *keys, const uint32_t *vals, uint6
     }
 }
 
+constexpr uint32_t kFenceStride = 16;   // prefix slots per fence (one 64-byte line)
+constexpr uint32_t kFenceMin = 64;      // prefixes shorter than this get no fences
+
 // Length of each row's longest non-decreasing prefix (initialised to the degree):
 // the first descent position of a row-sorted edge list.
 __global__ void k_sorted_prefix(const uint32_t *keys, const uint32_t *vals, uint64_t m, const uint64_t *rp,
@@ -144,6 +147,26 @@ __global__ void k_sorted_prefix(const uint32_t *keys, const uint32_t *vals, uint
          e += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t r = keys[e];
         if (e > rp[r] && vals[e] < vals[e - 1]) atomicMin(&sorted[r], (uint32_t)(e - rp[r]));
+    }
+}
+
+// Fence capacity of each row: one fence per 16 slots of a sorted prefix of at
+// least kFenceMin entries (shorter prefixes are scanned, not searched).
+__global__ void k_fence_caps(const uint32_t *sorted, uint32_t n, uint32_t *cap) {
+    for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r <= n; r += (uint64_t)gridDim.x * blockDim.x)
+        cap[r] = r < n && sorted[r] >= kFenceMin ? (sorted[r] + kFenceStride - 1) / kFenceStride : 0u;
+}
+
+// Fence j of row r = the id at slot 16j (row-sorted edge list, before placement).
+__global__ void k_fence_fill(const uint32_t *keys, const uint32_t *vals, uint64_t m, const uint64_t *rp,
+                             const uint32_t *fbase, uint32_t *fence) {
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < m;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t r = keys[e];
+        const uint64_t i = e - rp[r];
+        if (i % kFenceStride) continue;
+        const uint64_t j = i / kFenceStride;
+        if (j < fbase[r + 1] - fbase[r]) fence[fbase[r] + j] = vals[e];
     }
 }
 
@@ -169,9 +192,27 @@ static int bits_for(uint64_t maxval) {
     return b;
 }
 
+void Engine::build_fences(const uint32_t *keys, const uint32_t *vals, uint64_t m, const uint64_t *rp,
+                          const uint32_t *sorted, DevBuf<uint32_t> &fbase, DevBuf<uint32_t> &fence) {
+    DevBuf<uint32_t> caps;
+    caps.alloc((uint64_t)n + 1);
+    fbase.alloc((uint64_t)n + 1);
+    hipLaunchKernelGGL(k_fence_caps, dim3(grid_for((uint64_t)n + 1)), dim3(256), 0, stream, sorted, n, caps.p);
+    size_t tb = 0;
+    DCG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, caps.p, fbase.p, (int)n + 1, stream));
+    void *t = temp(tb);
+    DCG_HIP(hipcub::DeviceScan::ExclusiveSum(t, tb, caps.p, fbase.p, (int)n + 1, stream));
+    uint32_t total = 0;
+    DCG_HIP(hipMemcpyAsync(&total, fbase.p + n, 4, hipMemcpyDeviceToHost, stream));
+    sync();
+    fence.alloc(total ? total : 1);
+    hipLaunchKernelGGL(k_fence_fill, dim3(grid_for(m)), dim3(256), 0, stream, keys, vals, m, rp, fbase.p, fence.p);
+    DCG_HIP(hipGetLastError());
+}
+
 void Engine::load_edges_device(uint32_t nn, const uint32_t *d_src, const uint32_t *d_dst, uint64_t m) {
     if (m >= (1ull << 31)) throw Error(DCG_ERR_UNSUPPORTED, "edge count must be below 2^31 per load");
-    if (nn > 0x7FFFFFFFu) throw Error(DCG_ERR_UNSUPPORTED, "vertex ids must be below 2^31");
+    if (nn >= 0x7FFFFFFFu) throw Error(DCG_ERR_UNSUPPORTED, "vertex count must be below 2^31 - 1");
     n = nn;
     num_edges = m;
     const uint64_t rows = 2ull * n;
@@ -265,12 +306,14 @@ void Engine::load_edges_device(uint32_t nn, const uint32_t *d_src, const uint32_
     hipLaunchKernelGGL(k_copy_u32, dim3(grid_for(n)), dim3(256), 0, stream, deg.p + n, d_sorted_in.p, (uint64_t)n);
     hipLaunchKernelGGL(k_sorted_prefix, dim3(grid_for(m)), dim3(256), 0, stream, ks.p, vs.p, m, rp.p,
                        d_sorted_out.p);
+    build_fences(ks.p, vs.p, m, rp.p, d_sorted_out.p, d_fbase_out, d_fence_out);
     // in rows
     DCG_HIP(hipcub::DeviceRadixSort::SortPairs(t, tb, d_dst, ks.p, d_src, vs.p, (int)m, 0, kb, stream));
     hipLaunchKernelGGL(k_scatter_rows, dim3(grid_for(m)), dim3(256), 0, stream, ks.p, vs.p, m, rp.p + n + 1, d_in.p,
                        slots.p);
     hipLaunchKernelGGL(k_sorted_prefix, dim3(grid_for(m)), dim3(256), 0, stream, ks.p, vs.p, m, rp.p + n + 1,
                        d_sorted_in.p);
+    build_fences(ks.p, vs.p, m, rp.p + n + 1, d_sorted_in.p, d_fbase_in, d_fence_in);
     DCG_HIP(hipGetLastError());
     ks.release();
     vs.release();
@@ -409,6 +452,23 @@ struct DelResult {
     uint32_t end, removed, moved, removed_below;
 };
 
+// A row's fences: f[j] is the id at slot 16j of the sorted prefix, for j < cap.
+struct Fences {
+    uint32_t *f;
+    uint32_t cap;
+};
+
+__device__ __forceinline__ Fences fences_of(const uint32_t *fbase, uint32_t *fence, uint32_t row) {
+    const uint32_t b = fbase[row];
+    return Fences{fence + b, fbase[row + 1] - b};
+}
+
+// An entry that lands on a fence position of the prefix refreshes that fence
+// (compaction moves prefix entries towards the row start).
+__device__ __forceinline__ void fence_put(const Fences &F, uint32_t pos, uint32_t x) {
+    if (pos % kFenceStride == 0 && pos / kFenceStride < F.cap) F.f[pos / kFenceStride] = x;
+}
+
 // One wave over slots [from, to) of a row: remove the first still-requested
 // occurrences of the group's targets, survivors compacted in order.  Matches
 // are resolved in lane order through per-run counters, so the occurrences
@@ -416,7 +476,8 @@ struct DelResult {
 // Tombstones never match a target and stay in place.  `removed_below` counts
 // removals at positions below `mark`.
 __device__ DelResult wave_delete_range(const StoreView &s, uint64_t off, uint32_t from, uint32_t to, uint32_t mark,
-                                       const uint64_t *keys, uint64_t gs, uint64_t ge, uint32_t *used) {
+                                       const uint64_t *keys, uint64_t gs, uint64_t ge, uint32_t *used,
+                                       const Fences &F) {
     const uint32_t lane = lane_id();
     uint32_t write = from, removed = 0, moved = 0, below = 0;
     for (uint32_t t0 = from; t0 < to; t0 += kWave) {
@@ -444,6 +505,7 @@ __device__ DelResult wave_delete_range(const StoreView &s, uint64_t off, uint32_
             const uint32_t pos = write + lanes_below(km);
             const bool mv = keep && pos != i;
             if (mv) store_slot(s, off + pos, x);
+            if (mv && i < mark) fence_put(F, pos, x);
             moved += (uint32_t)__popcll(__ballot(mv));
         }
         write += (uint32_t)__popcll(km);
@@ -454,7 +516,8 @@ __device__ DelResult wave_delete_range(const StoreView &s, uint64_t off, uint32_
 
 // One wave: drop the tombstones of slots [0, to), order kept.  Returns the new
 // end in .end and the live entries below `mark` in .removed_below.
-__device__ DelResult wave_compact_tombs(const StoreView &s, uint64_t off, uint32_t to, uint32_t mark) {
+__device__ DelResult wave_compact_tombs(const StoreView &s, uint64_t off, uint32_t to, uint32_t mark,
+                                        const Fences &F) {
     const uint32_t lane = lane_id();
     uint32_t write = 0, moved = 0, live_below = 0;
     for (uint32_t t0 = 0; t0 < to; t0 += kWave) {
@@ -466,32 +529,108 @@ __device__ DelResult wave_compact_tombs(const StoreView &s, uint64_t off, uint32
         const uint32_t pos = write + lanes_below(km);
         const bool mv = keep && pos != i;
         if (mv) store_slot(s, off + pos, x);
+        if (mv && i < mark) fence_put(F, pos, x);
         moved += (uint32_t)__popcll(__ballot(mv));
         write += (uint32_t)__popcll(km);
     }
     return DelResult{write, 0, moved, live_below};
 }
 
-// Sorted prefix of the row [0, S): the head of every run of equal targets
-// (one lane each) binary-searches the run's value and tombstones its first
-// live occurrences (bit 31 set, id bits kept, so the prefix stays ordered).
-// used[head] records what each run took; returns the tombstones written and
-// (in probes) the slots read; need_tail tells whether a run wants more.
-__device__ __forceinline__ uint32_t search_prefix(const StoreView &s, uint64_t off, uint32_t S, const uint64_t *keys,
-                                                  uint64_t q, uint64_t gs, uint64_t ge, uint32_t *used,
-                                                  uint32_t &probes, uint32_t &need_tail) {
+// Fenced search of a sorted prefix [0, S) for the first `c` live occurrences
+// of v, by a subgroup of W lanes (W = 64: the whole wave; W = 16: a quarter at
+// bit `shift` of the wave's ballots); q is the lane's index in the subgroup and
+// every lane of the subgroup takes part with the same v and c.
+//  1. W-ary search over the fences (HBM): k = fences below v, so the lower bound
+//     of v lies in slots (16(k-1), 16k].
+//  2. 16 lanes read the 16-slot window at fence k-1 (one 64-byte line of the
+//     row) and tombstone the first live occurrences of v in it (bit 31 set, id
+//     bits kept, so the prefix stays ordered and its fences valid); while more
+//     are wanted and the window's last slot does not pass v, the next window.
+// Returns the occurrences taken; `probes` (kept by subgroup lane 0) counts the
+// row slots read.
+template <uint32_t W>
+__device__ uint32_t fenced_take(const StoreView &s, uint64_t off, uint32_t S, const Fences &F, uint32_t v,
+                                uint32_t c, uint32_t q, uint32_t shift, uint32_t &probes) {
+    static_assert(W == 16 || W == 64, "subgroup of 16 or 64 lanes");
+    const uint64_t below = (1ull << q) - 1;
+    const uint32_t nfc = (S + kFenceStride - 1) / kFenceStride;
+    uint32_t lo = 0, hi = F.cap < nfc ? F.cap : nfc;
+    while (hi - lo > W) {
+        const uint32_t step = (hi - lo + W - 1) / W;
+        const uint32_t idx = lo + q * step;
+        const bool p = idx < hi && (F.f[idx] & ~kTomb) < v;
+        const uint64_t m = __ballot(p);
+        const uint32_t cnt = (uint32_t)__popcll(W == 64 ? m : (m >> shift) & 0xFFFFull);
+        if (!cnt) {
+            hi = lo;
+            break;
+        }
+        const uint32_t nhi = lo + cnt * step;
+        lo += (cnt - 1) * step + 1;
+        hi = nhi < hi ? nhi : hi;
+    }
+    if (hi > lo) {
+        const uint32_t idx = lo + q;
+        const bool p = idx < hi && (F.f[idx] & ~kTomb) < v;
+        const uint64_t m = __ballot(p);
+        lo += (uint32_t)__popcll(W == 64 ? m : (m >> shift) & 0xFFFFull);
+    }
+    uint32_t pos = lo ? (lo - 1) * kFenceStride : 0u, taken = 0;
+    while (true) {
+        const uint32_t i = pos + q;
+        const bool valid = q < kFenceStride && i < S;
+        const uint32_t x = valid ? load_slot(s, off + i) : 0u;
+        const uint32_t id = x & ~kTomb;
+        const bool live = valid && id == v && !(x & kTomb);
+        uint64_t lm = __ballot(live);
+        if (W != 64) lm = (lm >> shift) & 0xFFFFull;
+        if (live && (uint32_t)__popcll(lm & below) < c - taken) store_slot(s, off + i, x | kTomb);
+        const uint32_t nlive = (uint32_t)__popcll(lm);
+        taken += nlive < c - taken ? nlive : c - taken;
+        if (q == 0) probes += S - pos < kFenceStride ? S - pos : kFenceStride;
+        // the run may go on past the window when its last slot is <= v
+        uint64_t em = __ballot(valid && q == kFenceStride - 1 && id <= v);
+        if (W != 64) em = (em >> shift) & 0xFFFFull;
+        if (taken >= c || !em || pos + kFenceStride >= S) break;
+        pos += kFenceStride;
+    }
+    return taken;
+}
+
+// Lower bound of v in the sorted prefix [0, S) for one thread: binary search
+// over the fences (HBM), then over the at most 16 slots they leave open.
+__device__ __forceinline__ uint32_t fenced_lb(const StoreView &s, uint64_t off, uint32_t S, const Fences &F,
+                                              uint32_t v, uint32_t &probes) {
+    const uint32_t nfc = (S + kFenceStride - 1) / kFenceStride;
+    const uint32_t nf = F.cap < nfc ? F.cap : nfc;
+    uint32_t lo = 0, hi = nf;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if ((F.f[mid] & ~kTomb) < v) lo = mid + 1;
+        else hi = mid;
+    }
+    if (!lo) return 0;
+    uint32_t a = (lo - 1) * kFenceStride + 1, b = lo < nf ? lo * kFenceStride : S;
+    while (a < b) {
+        const uint32_t mid = (a + b) >> 1;
+        probes++;
+        if ((load_slot(s, off + mid) & ~kTomb) < v) a = mid + 1;
+        else b = mid;
+    }
+    return a;
+}
+
+// One lane per run of equal targets (the run's head): fenced_lb, then the live
+// occurrences from there are tombstoned in order.  used[head] records what the
+// run took; need_tail tells whether a run wants more.
+__device__ __forceinline__ uint32_t search_prefix(const StoreView &s, uint64_t off, uint32_t S, const Fences &F,
+                                                  const uint64_t *keys, uint64_t q, uint64_t gs, uint64_t ge,
+                                                  uint32_t *used, uint32_t &probes, uint32_t &need_tail) {
     const uint32_t v = (uint32_t)keys[q];
     if (q > gs && (uint32_t)keys[q - 1] == v) return 0;
     const uint32_t c = (uint32_t)(target_ub(keys, q + 1, ge, v) - q);
-    uint32_t lo = 0, hi = S;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        probes++;
-        if ((load_slot(s, off + mid) & ~kTomb) < v) lo = mid + 1;
-        else hi = mid;
-    }
     uint32_t taken = 0;
-    for (uint32_t j = lo; j < S && taken < c; ++j) {
+    for (uint32_t j = fenced_lb(s, off, S, F, v, probes); j < S && taken < c; ++j) {
         const uint32_t x = load_slot(s, off + j);
         probes++;
         if ((x & ~kTomb) != v) break;
@@ -505,7 +644,9 @@ __device__ __forceinline__ uint32_t search_prefix(const StoreView &s, uint64_t o
     return taken;
 }
 
-constexpr uint32_t kWaveSearch = 64;   // sorted prefixes at least this long are binary-searched by the wave
+constexpr uint32_t kQuarterRuns = 16;   // groups of up to this many targets search a run per quarter-wave
+
+constexpr uint32_t kWaveSearch = kFenceMin;   // sorted prefixes at least this long are searched through fences
 constexpr uint32_t kWaveTail = 4096;   // longest tail a wave scans for a large row
 constexpr uint32_t kWaveRuns = 256;    // most targets a wave takes for a large row
 
@@ -521,18 +662,23 @@ __device__ __forceinline__ bool wave_searchable(const RowDesc &d, uint32_t S, ui
 // One wave per touched row of at most kBigDelete slots, or a larger row that
 // wave_searchable admits; the rest are listed for k_del_big.  `sorted` holds
 // each row's sorted prefix length S.
-//  * S >= kWaveSearch: search_prefix, one lane per run, then (only when a run
-//    still wants occurrences) wave_delete_range over the tail [S, ext), which
-//    holds appended entries and never tombstones.  The prefix precedes the
-//    tail in the row's sequence, so exactly the first occurrences go.
+//  * S >= kWaveSearch: the prefix is searched through the row's fences
+//    (fenced_take: the whole wave for a single run of equal targets, a
+//    quarter-wave per run for up to kQuarterRuns targets; search_prefix, a
+//    lane per run, beyond), then (only when a run still wants occurrences)
+//    wave_delete_range over the tail [S, ext), which holds appended entries
+//    and never tombstones.  The prefix precedes the tail in the row's
+//    sequence, so exactly the first occurrences go.
 //  * otherwise one wave_delete_range pass over the row; S shrinks by the
 //    entries removed below it (removal keeps a prefix sorted).
 //  * tombstones above a quarter of the row: wave_compact_tombs.
+// Compaction refreshes the fences of the prefix entries it moves.
 __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, uint32_t *sorted, uint32_t *hot,
                                                   const uint64_t *keys, uint64_t count,
                                                   const uint32_t *gstart, const unsigned long long *ngroups_p,
                                                   uint32_t *used, unsigned long long *counters, int count_stats,
-                                                  uint32_t *big_groups, unsigned long long *nbig) {
+                                                  uint32_t *big_groups, unsigned long long *nbig,
+                                                  const uint32_t *fbase, uint32_t *fence) {
     const uint32_t lane = lane_id();
     const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kWave;
     const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / kWave;
@@ -550,10 +696,41 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
         }
         uint32_t ext = d.ext, S2 = S, removed = 0, moved = 0, new_tombs = 0, read = 0;
         bool full = false;
+        const Fences F = fences_of(fbase, fence, row);
         if (S >= kWaveSearch) {
             uint32_t probes = 0, need = 0;
-            for (uint64_t q0 = gs; q0 < ge; q0 += kWave)
-                if (q0 + lane < ge) new_tombs += search_prefix(s, d.off, S, keys, q0 + lane, gs, ge, used, probes, need);
+            const uint32_t v0 = (uint32_t)keys[gs];
+            if ((uint32_t)keys[ge - 1] == v0) {
+                // one run (the group's targets are sorted): the whole wave searches it
+                const uint32_t c = (uint32_t)(ge - gs);
+                const uint32_t got = fenced_take<64>(s, d.off, S, F, v0, c, lane, 0, probes);
+                if (lane == 0) {
+                    used[gs] = got;
+                    new_tombs = got;
+                    need = got < c ? 1u : 0u;
+                }
+            } else if (ge - gs <= kQuarterRuns) {
+                // a quarter-wave per run, four runs at a time
+                const uint32_t quarter = lane >> 4, q = lane & 15;
+                for (uint64_t q0 = gs; q0 < ge; q0 += 4) {
+                    const uint64_t t = q0 + quarter;
+                    if (t < ge && (t == gs || (uint32_t)keys[t - 1] != (uint32_t)keys[t])) {
+                        const uint32_t v = (uint32_t)keys[t];
+                        const uint32_t c = (uint32_t)(target_ub(keys, t + 1, ge, v) - t);
+                        const uint32_t got = fenced_take<16>(s, d.off, S, F, v, c, q, quarter * 16, probes);
+                        if (q == 0) {
+                            used[t] = got;
+                            new_tombs += got;
+                            need |= got < c ? 1u : 0u;
+                        }
+                    }
+                }
+            } else {
+                // many runs: one lane per run
+                for (uint64_t q0 = gs; q0 < ge; q0 += kWave)
+                    if (q0 + lane < ge)
+                        new_tombs += search_prefix(s, d.off, S, F, keys, q0 + lane, gs, ge, used, probes, need);
+            }
             for (int o = 32; o >= 1; o >>= 1) {
                 new_tombs += __shfl_xor(new_tombs, o, kWave);
                 probes += __shfl_xor(probes, o, kWave);
@@ -561,14 +738,14 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
             read += probes;
             removed += new_tombs;
             if (__ballot(need != 0) && S < ext) {
-                const DelResult r = wave_delete_range(s, d.off, S, ext, 0, keys, gs, ge, used);
+                const DelResult r = wave_delete_range(s, d.off, S, ext, 0, keys, gs, ge, used, F);
                 read += ext - S;
                 ext = r.end;
                 removed += r.removed;
                 moved += r.moved;
             }
         } else {
-            const DelResult r = wave_delete_range(s, d.off, 0, ext, S, keys, gs, ge, used);
+            const DelResult r = wave_delete_range(s, d.off, 0, ext, S, keys, gs, ge, used, F);
             read += ext;
             ext = r.end;
             removed = r.removed;
@@ -579,7 +756,7 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
         uint32_t tombs = row_tombs(d) + new_tombs;
         if (tombs && (uint64_t)tombs * 4 > ext && ext <= kBigDelete) {   // larger rows: k_del_big compacts
             __threadfence_block();   // the lanes' tombstone stores complete before the pass reads them
-            const DelResult c = wave_compact_tombs(s, d.off, ext, S2);
+            const DelResult c = wave_compact_tombs(s, d.off, ext, S2, F);
             read += ext;
             ext = c.end;
             S2 = c.removed_below;
@@ -632,7 +809,7 @@ struct BigLds {
 template <bool TOMBS>
 __device__ uint32_t compact_range(const StoreView &s, uint64_t off, uint32_t from, uint32_t to, uint32_t mark,
                                   const uint64_t *keys, uint64_t gs, uint64_t ge, uint32_t *used, BigLds &L,
-                                  uint32_t &moved, uint32_t &kept_below) {
+                                  uint32_t &moved, uint32_t &kept_below, const Fences &F) {
     const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
     uint32_t write = from;
     for (uint32_t t0 = from; t0 < to; t0 += kTileE) {
@@ -701,6 +878,7 @@ __device__ uint32_t compact_range(const StoreView &s, uint64_t off, uint32_t fro
             if (keep[j] && i < mark) kept_below++;
             if (keep[j] && pos[j] != i) {
                 store_slot(s, off + pos[j], xv[j]);
+                if (i < mark) fence_put(F, pos[j], xv[j]);
                 moved++;
             }
         }
@@ -741,7 +919,8 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t x, uint32_t &slot) {
 __global__ __launch_bounds__(1024) void k_del_big(StoreView s, RowDesc *desc, uint32_t *sorted, uint32_t *hot,
                                                   const uint64_t *keys, uint64_t count, const uint32_t *big_groups,
                                                   const unsigned long long *nbig_p, uint32_t *used,
-                                                  unsigned long long *counters, int count_stats) {
+                                                  unsigned long long *counters, int count_stats,
+                                                  const uint32_t *fbase, uint32_t *fence) {
     __shared__ BigLds L;
     const uint32_t tid = threadIdx.x;
     const uint64_t nbig = *nbig_p;
@@ -751,6 +930,7 @@ __global__ __launch_bounds__(1024) void k_del_big(StoreView s, RowDesc *desc, ui
         const uint64_t ge = group_end<uint64_t>(keys, gs, count, row);
         const RowDesc d = desc[row];
         const uint32_t S = sorted[row] < d.ext ? sorted[row] : d.ext;
+        const Fences F = fences_of(fbase, fence, row);
         uint32_t moved = 0, kept = 0, ext = d.ext, S2 = S, new_tombs = 0;
         uint64_t scanned = 0;
         bool full_read = false;
@@ -760,15 +940,8 @@ __global__ __launch_bounds__(1024) void k_del_big(StoreView s, RowDesc *desc, ui
                 const uint32_t v = (uint32_t)keys[q];
                 if (q > gs && (uint32_t)keys[q - 1] == v) continue;
                 const uint32_t c = (uint32_t)(target_ub(keys, q + 1, ge, v) - q);
-                uint32_t lo = 0, hi = S;
-                while (lo < hi) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    probes++;
-                    if ((load_slot(s, d.off + mid) & ~kTomb) < v) lo = mid + 1;
-                    else hi = mid;
-                }
                 uint32_t taken = 0;
-                for (uint32_t j = lo; j < S && taken < c; ++j) {
+                for (uint32_t j = fenced_lb(s, d.off, S, F, v, probes); j < S && taken < c; ++j) {
                     const uint32_t x = load_slot(s, d.off + j);
                     probes++;
                     if ((x & ~kTomb) != v) break;
@@ -784,11 +957,11 @@ __global__ __launch_bounds__(1024) void k_del_big(StoreView s, RowDesc *desc, ui
             new_tombs = block_sum(tombs, L.tombs);
             scanned += block_sum(probes, L.probes);
             if (block_sum(need_tail, L.need_tail) && S < d.ext) {
-                ext = compact_range<false>(s, d.off, S, d.ext, 0, keys, gs, ge, used, L, moved, kept);
+                ext = compact_range<false>(s, d.off, S, d.ext, 0, keys, gs, ge, used, L, moved, kept, F);
                 scanned += d.ext - S;
             }
         } else {
-            ext = compact_range<false>(s, d.off, 0, d.ext, S, keys, gs, ge, used, L, moved, kept);
+            ext = compact_range<false>(s, d.off, 0, d.ext, S, keys, gs, ge, used, L, moved, kept, F);
             S2 = block_sum(kept, L.tombs);
             scanned += d.ext;
             full_read = true;
@@ -797,7 +970,7 @@ __global__ __launch_bounds__(1024) void k_del_big(StoreView s, RowDesc *desc, ui
         uint32_t tombs_left = row_tombs(d) + new_tombs;
         if (tombs_left && ((uint64_t)tombs_left * 4 > ext || tombs_left > kMaxTombs / 2)) {
             kept = 0;
-            const uint32_t ext2 = compact_range<true>(s, d.off, 0, ext, S2, keys, gs, ge, used, L, moved, kept);
+            const uint32_t ext2 = compact_range<true>(s, d.off, 0, ext, S2, keys, gs, ge, used, L, moved, kept, F);
             S2 = block_sum(kept, L.tombs);
             scanned += ext;
             ext = ext2;
@@ -1073,22 +1246,24 @@ void Engine::mutate_delete(uint64_t begin, uint64_t count, dcg_phase_stats *st) 
     const int w0 = time_begin(DCG_K_DELETE_ROWS, -1);
     hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream, v, d_out.p, d_sorted_out.p, d_hot_out.p, kout,
                        count, d_gstart.p, d_counters.p + C_GROUPS_OUT, d_used.p, d_counters.p, 1, d_big_groups.p,
-                       d_counters.p + C_NBIG);
+                       d_counters.p + C_NBIG, d_fbase_out.p, d_fence_out.p);
     time_end(w0);
     const int b0 = time_begin(DCG_K_DELETE_BIG, -1);
     hipLaunchKernelGGL(k_del_big, dim3(kDelBigGrid), dim3(1024), 0, stream, v, d_out.p, d_sorted_out.p, d_hot_out.p,
-                       kout, count, d_big_groups.p, d_counters.p + C_NBIG, d_used.p, d_counters.p, 1);
+                       kout, count, d_big_groups.p, d_counters.p + C_NBIG, d_used.p, d_counters.p, 1, d_fbase_out.p,
+                       d_fence_out.p);
     time_end(b0);
     fill<uint32_t>(stream, d_used.p, count, 0u);
     fill<unsigned long long>(stream, d_counters.p + C_NBIG, 1, 0ull);
     const int w1 = time_begin(DCG_K_DELETE_ROWS, -1);
     hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream, v, d_in.p, d_sorted_in.p, d_hot_in.p, kin,
                        count, d_gstart2.p, d_counters.p + C_GROUPS_IN, d_used.p, d_counters.p, 0, d_big_groups.p,
-                       d_counters.p + C_NBIG);
+                       d_counters.p + C_NBIG, d_fbase_in.p, d_fence_in.p);
     time_end(w1);
     const int b1 = time_begin(DCG_K_DELETE_BIG, -1);
     hipLaunchKernelGGL(k_del_big, dim3(kDelBigGrid), dim3(1024), 0, stream, v, d_in.p, d_sorted_in.p, d_hot_in.p,
-                       kin, count, d_big_groups.p, d_counters.p + C_NBIG, d_used.p, d_counters.p, 0);
+                       kin, count, d_big_groups.p, d_counters.p + C_NBIG, d_used.p, d_counters.p, 0, d_fbase_in.p,
+                       d_fence_in.p);
     time_end(b1);
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipMemcpyAsync(h_counters, d_counters.p, C_NUM * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));

@@ -442,3 +442,90 @@ def test_relocated_blocks_are_reused():
     assert info["reused_blocks"] > relocated // 20
     assert info["reused_blocks"] + info["free_blocks"] <= relocated
     eng.close()
+
+
+@pytest.mark.parametrize("app", ["bfs", "cc"])
+@pytest.mark.parametrize("placement", [0, 1])
+def test_fenced_prefix_search_rows(app, placement):
+    """Prefix searches through the fences: single targets (the whole wave),
+    a few targets (a quarter-wave each) and many targets (a lane each); hits
+    in the sorted prefix, a repeated value whose first occurrence is already a
+    tombstone, a value only in the appended tail, a missing value, and small
+    rows where tombstones cross the compaction point (fences refreshed by the
+    compaction, then searched again).  Rows, values and the missing count
+    match the multiset model after every stage."""
+    announce(f"fenced prefix search, app={app}, placement={placement}")
+    rng = np.random.default_rng(33)
+    n = 4000
+    src, dst = [], []
+    for u in range(300):
+        deg = int(rng.integers(4, 12)) if u < 100 else int(rng.integers(12, 400))
+        row = sorted(int(x) for x in rng.integers(0, n, deg))
+        row += [row[len(row) // 2]] * int(rng.integers(0, 3))          # repeats of one value
+        row.sort()
+        src += [u] * len(row)
+        dst += row
+    src += [int(x) for x in rng.integers(300, n, 4000)]
+    dst += [int(x) for x in rng.integers(0, n, 4000)]
+    src, dst = np.array(src, np.uint32), np.array(dst, np.uint32)
+    store = oracle.Store(n, src, dst)
+    stages, last = [], {}
+    appended = {u: [] for u in range(300)}
+    rp, col = store.csr()
+    for s in range(6):
+        dl, al = [], []
+        for u in range(300):
+            row = [int(x) for x in col[rp[u]:rp[u + 1]]]
+            # one target (whole-wave search), a few (quarter-waves), many (a lane each)
+            nd = 1 if u % 3 == 0 else (int(rng.integers(2, 9)) if u % 3 == 1 else int(rng.integers(17, 41)))
+            for _ in range(nd):
+                kind = int(rng.integers(0, 5))
+                if kind == 1 and u in last:
+                    v = last[u]                                       # next occurrence or missing
+                elif kind == 2 and appended[u]:
+                    v = appended[u][int(rng.integers(0, len(appended[u])))]
+                elif kind == 3:
+                    v = int(n - 1 - (u % 7))                          # usually absent
+                elif row:
+                    v = row[int(rng.integers(0, len(row)))]
+                else:
+                    continue
+                dl.append((u, v))
+                last[u] = v
+            if rng.random() < 0.3:
+                w = int(rng.integers(0, n))
+                al.append((u, w))
+                appended[u].append(w)
+        stages.append((dl, al))
+        store.apply(dl, al)
+        rp, col = store.csr()
+    store = oracle.Store(n, src, dst)
+    adds = [e for _, a in stages for e in a]
+    dels = [e for d_, _ in stages for e in d_]
+    eng = _engine(app=app, placement=placement, cache_bytes=1 << 18)
+    eng.load_graph(n, src, dst)
+    eng.start()
+    eng.load_updates(adds, dels)
+    def ref(n_, rp_, col_):
+        return oracle_values(app, n_, rp_, col_)
+    ab = db = 0
+    for s, (dl, al) in enumerate(stages):
+        st = eng.del_edge(db, len(dl))
+        m = store.apply(dl, [])
+        assert st["missing_deletes"] == m["missing_deletes"], f"stage {s}"
+        erp, ecol = eng.csr()
+        assert same_rows(erp, ecol, store), f"rows differ after delete stage {s}"
+        rp, col = store.csr()
+        got = eng.gather_value()
+        assert np.array_equal(got, ref(n, rp, col)), f"{app} values, delete stage {s}"
+        if app == "cc":
+            assert cc_parents_ok(n, rp, col, got, eng.gather_parent()), f"CC parents, delete stage {s}"
+        eng.add_edge(ab, len(al))
+        store.apply([], al)
+        erp, ecol = eng.csr()
+        assert same_rows(erp, ecol, store), f"rows differ after add stage {s}"
+        rp, col = store.csr()
+        assert np.array_equal(eng.gather_value(), ref(n, rp, col)), f"{app} values, add stage {s}"
+        ab += len(al)
+        db += len(dl)
+    eng.close()
