@@ -2,8 +2,9 @@
 //
 // The cache holds whole 65,536-edge chunks in HBM frames.  Kernels resolve a
 // slot through chunk_frame[]: a resident chunk is read from its frame, any
-// other chunk zero-copy from pinned host memory.  Writes to a resident chunk
-// mark it dirty; eviction writes it back.
+// other chunk zero-copy from pinned host memory.  Writes go through to both
+// copies (store_slot), so the host copy is always current and an eviction
+// only unmaps the frame.
 //
 // HOTNESS policy (the reference's plain policy, driver.cuh:486-579 and
 // framework.cuh:1264-1567, at chunk granularity): score = edges read over the
@@ -18,20 +19,16 @@
 
 namespace dcg {
 
-// Copy whole chunks between HBM frames and pinned host memory: one workgroup
+// Load whole chunks from pinned host memory into HBM frames: one workgroup
 // per chunk, 16-byte vector accesses.
-__global__ __launch_bounds__(256) void k_move_chunks(uint32_t *frames, uint32_t *host, const int64_t *list,
-                                                     uint32_t count, int to_host) {
+__global__ __launch_bounds__(256) void k_move_chunks(uint32_t *frames, const uint32_t *host, const int64_t *list,
+                                                     uint32_t count) {
     for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
         const int64_t frame = list[2 * i], chunk = list[2 * i + 1];
         uint4 *f = reinterpret_cast<uint4 *>(frames + ((uint64_t)frame << kChunkLog2));
-        uint4 *h = reinterpret_cast<uint4 *>(host + ((uint64_t)chunk << kChunkLog2));
+        const uint4 *h = reinterpret_cast<const uint4 *>(host + ((uint64_t)chunk << kChunkLog2));
         constexpr uint32_t kVec = kChunkEdges / 4;
-        if (to_host) {
-            for (uint32_t j = threadIdx.x; j < kVec; j += blockDim.x) h[j] = f[j];
-        } else {
-            for (uint32_t j = threadIdx.x; j < kVec; j += blockDim.x) f[j] = h[j];
-        }
+        for (uint32_t j = threadIdx.x; j < kVec; j += blockDim.x) f[j] = h[j];
     }
 }
 
@@ -54,8 +51,6 @@ void Engine::cache_init() {
     cache.chunk_frame.assign(C, -1);
     cache.frame_chunk.assign(cache.frames, -1);
     cache.last_use.assign(C, 0);
-    cache.d_dirty.alloc(C);
-    fill<uint8_t>(stream, cache.d_dirty.p, C, (uint8_t)0);
     cache.d_reads.alloc(2 * C);
     fill<float>(stream, cache.d_reads.p, 2 * C, 0.f);
     cache.d_hist.alloc(4 * C);
@@ -85,7 +80,7 @@ void Engine::cache_init() {
         dl.alloc(list.size());
         DCG_HIP(hipMemcpyAsync(dl.p, list.data(), list.size() * 8, hipMemcpyHostToDevice, stream));
         hipLaunchKernelGGL(k_move_chunks, dim3(grid_for(cache.frames, 1, 4096)), dim3(256), 0, stream,
-                           cache.d_frames.p, h_slab, dl.p, (uint32_t)cache.frames, 0);
+                           cache.d_frames.p, h_slab, dl.p, (uint32_t)cache.frames);
         DCG_HIP(hipGetLastError());
         cache.h2d += cache.frames * kChunkEdges * 4;
         sync();
@@ -114,18 +109,9 @@ void Engine::cache_apply_plan(const std::vector<uint64_t> &desired, dcg_cache_st
     for (uint64_t c : desired)
         if (cache.chunk_frame[c] < 0) to_load.push_back(c);
     if (evict_list.empty() && to_load.empty()) return;
-    // dirty evicted frames go back to host memory
-    std::vector<uint8_t> dirty(C, 0);
-    DCG_HIP(hipMemcpyAsync(dirty.data(), cache.d_dirty.p, C, hipMemcpyDeviceToHost, stream));
-    sync();
-    std::vector<int64_t> wb;
+    // the host copy is current (write-through): evicting only unmaps the frame
     for (size_t i = 0; i < evict_list.size(); i += 2) {
         const int64_t f = evict_list[i], c = evict_list[i + 1];
-        if (dirty[c]) {
-            wb.push_back(f);
-            wb.push_back(c);
-            dirty[c] = 0;
-        }
         cache.chunk_frame[c] = -1;
         cache.frame_chunk[f] = -1;
         free_frames.push_back(f);
@@ -140,34 +126,23 @@ void Engine::cache_apply_plan(const std::vector<uint64_t> &desired, dcg_cache_st
         load_list.push_back((int64_t)c);
     }
     DevBuf<int64_t> dl;
-    dl.alloc(std::max<size_t>(wb.size() + load_list.size(), 2));
-    if (!wb.empty()) {
-        DCG_HIP(hipMemcpyAsync(dl.p, wb.data(), wb.size() * 8, hipMemcpyHostToDevice, stream));
-        hipLaunchKernelGGL(k_move_chunks, dim3(grid_for(wb.size() / 2, 1, 4096)), dim3(256), 0, stream,
-                           cache.d_frames.p, h_slab, dl.p, (uint32_t)(wb.size() / 2), 1);
-        DCG_HIP(hipGetLastError());
-    }
+    dl.alloc(std::max<size_t>(load_list.size(), 2));
     if (!load_list.empty()) {
-        DCG_HIP(hipMemcpyAsync(dl.p + wb.size(), load_list.data(), load_list.size() * 8, hipMemcpyHostToDevice, stream));
+        DCG_HIP(hipMemcpyAsync(dl.p, load_list.data(), load_list.size() * 8, hipMemcpyHostToDevice, stream));
         hipLaunchKernelGGL(k_move_chunks, dim3(grid_for(load_list.size() / 2, 1, 4096)), dim3(256), 0, stream,
-                           cache.d_frames.p, h_slab, dl.p + wb.size(), (uint32_t)(load_list.size() / 2), 0);
+                           cache.d_frames.p, h_slab, dl.p, (uint32_t)(load_list.size() / 2));
         DCG_HIP(hipGetLastError());
     }
-    DCG_HIP(hipMemcpyAsync(cache.d_dirty.p, dirty.data(), C, hipMemcpyHostToDevice, stream));
     DCG_HIP(hipMemcpyAsync(cache.d_chunk_frame.p, cache.chunk_frame.data(), C * 4, hipMemcpyHostToDevice, stream));
     sync();
-    const uint64_t wbn = wb.size() / 2, ln = load_list.size() / 2;
-    cache.written_back += wbn;
+    const uint64_t ln = load_list.size() / 2;
     cache.evicted += evict_list.size() / 2;
     cache.loaded += ln;
-    cache.d2h += wbn * kChunkEdges * 4;
     cache.h2d += ln * kChunkEdges * 4;
     if (st) {
         st->evicted_chunks += evict_list.size() / 2;
         st->loaded_chunks += ln;
-        st->written_back_chunks += wbn;
         st->h2d_bytes += ln * kChunkEdges * 4;
-        st->d2h_bytes += wbn * kChunkEdges * 4;
     }
 }
 

@@ -54,7 +54,6 @@ struct StoreView {
     uint32_t *host;            // pinned host slots (device-visible), null when device-placed
     uint32_t *frames;          // device chunk frames (frame f holds slots [f<<16, (f+1)<<16) of its chunk)
     const int32_t *chunk_frame;// chunk -> frame, -1 when not resident
-    uint8_t *dirty;            // chunk written while resident (write-back on eviction)
     uint32_t *hot_out;         // per-vertex out-row expansions this snapshot
     uint32_t *hot_in;          // per-vertex in-row expansions this snapshot
 };
@@ -66,14 +65,18 @@ __device__ __forceinline__ const uint32_t *row_base(const StoreView &s, uint64_t
                   : s.host + off;
 }
 
-__device__ __forceinline__ uint32_t *row_base_w(const StoreView &s, uint64_t off) {
-    const uint32_t c = (uint32_t)(off >> kChunkLog2);
-    const int32_t f = s.chunk_frame[c];
-    if (f >= 0) {
-        if (s.host) s.dirty[c] = 1;
-        return s.frames + (((uint64_t)f) << kChunkLog2) + (off & (kChunkEdges - 1));
-    }
-    return s.host + off;
+// Frame copy of a slot, or null when its chunk is not resident.
+__device__ __forceinline__ uint32_t *frame_w(const StoreView &s, uint64_t off) {
+    const int32_t f = s.chunk_frame[(uint32_t)(off >> kChunkLog2)];
+    return f >= 0 ? s.frames + (((uint64_t)f) << kChunkLog2) + (off & (kChunkEdges - 1)) : nullptr;
+}
+
+// Write-through: a store reaches the frame of a resident chunk and the pinned
+// host copy, so the host copy is always current and eviction copies nothing.
+__device__ __forceinline__ void store_slot(const StoreView &s, uint64_t off, uint32_t v) {
+    uint32_t *f = frame_w(s, off);
+    if (f) *f = v;
+    if (s.host) s.host[off] = v;
 }
 
 // Weight policy: REFERENCE (u+v)%128+1 (api.cuh:26-28), SEEDED splitmix64,

@@ -61,7 +61,6 @@ struct CacheState {
     std::vector<int32_t> chunk_frame;   // host mirror
     std::vector<int64_t> frame_chunk;   // frame -> chunk (-1 free)
     DevBuf<int32_t> d_chunk_frame;
-    DevBuf<uint8_t> d_dirty;
     DevBuf<uint32_t> d_frames;          // frames * 65536 slots (device placement: all chunks)
     DevBuf<float> d_reads;              // edges read per chunk, this snapshot
     DevBuf<float> d_hist;               // 4-snapshot history of reads (chunk-major, 4 floats)
@@ -69,7 +68,7 @@ struct CacheState {
     DevBuf<float> d_rows_last;          // rows expanded per chunk in the last snapshot
     DevBuf<float> d_score;
     std::vector<uint32_t> last_use;     // snapshot of last access (LRU tie-break)
-    uint64_t loaded = 0, evicted = 0, written_back = 0, h2d = 0, d2h = 0;
+    uint64_t loaded = 0, evicted = 0, h2d = 0;
     std::unique_ptr<MlpState, MlpDeleter> mlp;
 };
 

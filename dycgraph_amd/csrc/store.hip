@@ -53,7 +53,6 @@ StoreView Engine::view() {
     v.host = h_slab;
     v.frames = cache.d_frames.p;
     v.chunk_frame = cache.d_chunk_frame.p;
-    v.dirty = cache.d_dirty.p;
     v.hot_out = d_hot_out.p;
     v.hot_in = d_hot_in.p;
     return v;
@@ -422,7 +421,6 @@ __device__ __forceinline__ uint64_t group_end(const K *keys, uint64_t s, uint64_
 }
 
 __device__ __forceinline__ uint32_t load_slot(const StoreView &s, uint64_t slot) { return *row_base(s, slot); }
-__device__ __forceinline__ void store_slot(const StoreView &s, uint64_t slot, uint32_t v) { *row_base_w(s, slot) = v; }
 
 constexpr uint32_t kBigDelete = 1024;   // rows above this use one 1024-thread workgroup
 
@@ -1189,8 +1187,13 @@ __global__ __launch_bounds__(1024) void k_add_big(StoreView s, RowDesc *desc, ui
         for (uint64_t seg = 0; seg < d.ext; seg += kChunkEdges) {
             const uint32_t len = (uint32_t)min((uint64_t)kChunkEdges, (uint64_t)d.ext - seg);
             const uint32_t *src = row_base(s, d.off + seg);
-            uint32_t *dst = row_base_w(s, noff + seg);
-            for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) dst[i] = src[i];
+            uint32_t *dst = frame_w(s, noff + seg);
+            uint32_t *hdst = s.host ? s.host + noff + seg : nullptr;
+            for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) {
+                const uint32_t x = src[i];
+                if (dst) dst[i] = x;
+                if (hdst) hdst[i] = x;
+            }
         }
         for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) store_slot(s, noff + d.ext + i, vals[gs + i]);
         __syncthreads();

@@ -104,7 +104,9 @@ typedef struct {
  * fallback_decisions count the admission decisions taken so far from the
  * network's ranking and from the hotness ranking, and mlp_capture /
  * hot_capture are the judge's last comparison: the share of the last
- * snapshot's edge reads each ranking's top `frames` chunks would have held. */
+ * snapshot's edge reads each ranking's top `frames` chunks would have held.
+ * Stores reach both a resident chunk's frame and its host copy (write-through),
+ * so evictions copy nothing back: written_back_chunks and d2h_bytes stay 0. */
 typedef struct {
     uint64_t frames, resident_chunks, total_chunks;
     uint64_t loaded_chunks, evicted_chunks, written_back_chunks;
