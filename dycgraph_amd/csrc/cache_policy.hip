@@ -156,46 +156,51 @@ static void fill_common(Engine &e, dcg_cache_stats *st) {
     st->resident_chunks = r;
 }
 
-// Admissions per decision: 1/32 of the frames (at least 4), so a decision
-// moves at most ~3% of the cache over the host link and a changed ranking is
-// followed over several snapshots instead of in one stall.
-uint64_t Engine::move_budget() const { return std::max<uint64_t>(4, cache.frames / 32); }
+// Admissions per decision: 1/128 of the frames (at least 4).  A load moves a
+// whole 256 KiB chunk over the host link while the repair kernels read only
+// what they touch (and the delete search reads one line per target), so churn
+// near the ranking boundary costs more link time than the hits it buys; a
+// changed ranking is followed over several snapshots instead.
+uint64_t Engine::move_budget() const { return std::max<uint64_t>(4, cache.frames / 128); }
 
 // Admission plan shared by both policies.  Candidates are ranked by score
-// (ties: most recent use, then chunk id).  Loading a chunk moves 256 KiB over
-// the host link, the same as reading all of it zero-copy once, so a chunk is
-// admitted only if its score reaches `admit_min` edge reads, it displaces a
-// resident chunk only if its score exceeds the resident's by `margin`, and at
-// most move_budget() chunks are admitted.
+// (ties: most recent use, then chunk id).  A chunk is admitted only if its
+// score reaches `admit_min` edge reads, it displaces a resident chunk only if
+// its score exceeds the resident's by `margin`, and at most move_budget()
+// chunks are admitted, so only the budget strongest candidates and the budget
+// weakest residents are ranked (partial selection, then a sort of those).
 std::vector<uint64_t> Engine::plan_admission(const std::vector<float> &score, double admit_min, double margin) {
     const uint64_t C = cache.chunks, F = cache.frames;
+    const uint64_t budget = move_budget();
     auto better = [&](uint64_t a, uint64_t b) {
         if (score[a] != score[b]) return score[a] > score[b];
         if (cache.last_use[a] != cache.last_use[b]) return cache.last_use[a] > cache.last_use[b];
         return a < b;
     };
-    // resident chunks, weakest first
+    auto weaker = [&](uint64_t a, uint64_t b) { return better(b, a); };
+    // resident chunks: the `budget` weakest, weakest first
     std::vector<uint64_t> res;
+    res.reserve(F);
     for (uint64_t f = 0; f < F; ++f)
         if (cache.frame_chunk[f] >= 0) res.push_back((uint64_t)cache.frame_chunk[f]);
-    std::sort(res.begin(), res.end(), [&](uint64_t a, uint64_t b) { return better(b, a); });
-    // candidates, strongest first
+    const size_t rk = std::min<size_t>(budget, res.size());
+    if (rk < res.size()) std::nth_element(res.begin(), res.begin() + (ptrdiff_t)rk, res.end(), weaker);
+    std::sort(res.begin(), res.begin() + (ptrdiff_t)rk, weaker);
+    // candidates: the `budget` strongest, strongest first
     std::vector<uint64_t> cand;
     for (uint64_t c = 0; c < C; ++c)
         if (cache.chunk_frame[c] < 0 && score[c] >= admit_min) cand.push_back(c);
-    const uint64_t free_frames = F - res.size();
-    const uint64_t want = std::min<uint64_t>(cand.size(), free_frames + res.size());
-    if (want < cand.size()) {
-        std::nth_element(cand.begin(), cand.begin() + (ptrdiff_t)want, cand.end(), better);
-        cand.resize(want);
+    if (budget < cand.size()) {
+        std::nth_element(cand.begin(), cand.begin() + (ptrdiff_t)budget, cand.end(), better);
+        cand.resize(budget);
     }
     std::sort(cand.begin(), cand.end(), better);
-    std::vector<uint8_t> drop(res.size(), 0);
+    const uint64_t free_frames = F - res.size();
+    std::vector<uint8_t> drop(rk, 0);
     std::vector<uint64_t> admit;
     size_t ci = 0, ri = 0;
-    const uint64_t budget = move_budget();
     for (; ci < cand.size() && admit.size() < free_frames && admit.size() < budget; ++ci) admit.push_back(cand[ci]);
-    for (; ci < cand.size() && ri < res.size() && admit.size() < budget; ++ci, ++ri) {
+    for (; ci < cand.size() && ri < rk && admit.size() < budget; ++ci, ++ri) {
         if (!(score[cand[ci]] > margin * std::max(score[res[ri]], 1.0f))) break;
         drop[ri] = 1;
         admit.push_back(cand[ci]);
@@ -203,7 +208,7 @@ std::vector<uint64_t> Engine::plan_admission(const std::vector<float> &score, do
     std::vector<uint64_t> keep;
     keep.reserve(F);
     for (size_t i = 0; i < res.size(); ++i)
-        if (!drop[i]) keep.push_back(res[i]);
+        if (i >= rk || !drop[i]) keep.push_back(res[i]);
     for (uint64_t c : admit) keep.push_back(c);
     return keep;
 }
@@ -229,7 +234,7 @@ void Engine::cache_maintain(dcg_cache_stats *st) {
     for (uint64_t c = 0; c < C; ++c)
         if (reads[c] > 0) cache.last_use[c] = (uint32_t)snapshots;
     if (cfg.cache_policy == DCG_CACHE_LEARNED && cache.mlp) {
-        mlp_observe(*this, reads);   // online training on this snapshot's labels
+        mlp_observe(*this);   // online training on this snapshot's labels
     }
     DCG_HIP(hipMemcpyAsync(cache.d_rows_last.p, cache.d_reads.p + C, C * 4, hipMemcpyDeviceToDevice, stream));
     // the learned policy admits at dcg_cache_prefetch, before the next batch

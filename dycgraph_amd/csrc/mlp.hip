@@ -22,6 +22,8 @@
 #include <cmath>
 #include <numeric>
 
+#include <hipcub/hipcub.hpp>
+
 #include "mlp.hpp"
 
 namespace dcg {
@@ -39,6 +41,7 @@ constexpr int kReplay = 4;   // decisions kept for training (ring)
 
 struct MlpState {
     DevBuf<float> master, grad, m1, m2;   // FP32 masters, gradients, Adam moments
+    DevBuf<float> part;                   // per-workgroup gradient sums of a pass
     DevBuf<_Float16> w16;                 // FP16 weights used by the device passes
     DevBuf<float> X, Y, out, degsum, lastuse;   // X/Y hold kReplay decisions of C rows each
     uint64_t rows = 0;                    // C (rows per decision)
@@ -46,7 +49,10 @@ struct MlpState {
     int steps = 0;
     bool have_decision = false, fallback = true;
     double mlp_capture = 0, hot_capture = 0;
-    std::vector<float> hot_score_at_decision;
+    DevBuf<float> mscore, hot_dec;        // the decision's network and hotness scores (judge)
+    DevBuf<float> plan;                   // the decision's network score as predicted reads
+    DevBuf<float> sort_k, sort_v;         // judge: scores sorted descending, reads carried along
+    DevBuf<double> judge;                 // judge sums: network top-k, hotness top-k, all reads
     int decisions = 0, fallbacks = 0;
 };
 
@@ -89,9 +95,11 @@ struct TileLds {
 // Forward (and, with Y != null, backward) over 32-row tiles; one wave per
 // workgroup walks tiles blockIdx.x, +gridDim.x, ...  Gradient contributions stay
 // in registers across the workgroup's tiles (the dW2 tiles in the MFMA
-// accumulators) and are added to grad[] once per workgroup (mean over `rows`).
+// accumulators) and are stored once per workgroup as its row of `part`
+// (unscaled sums); k_grad_reduce adds the rows in workgroup order, so the
+// gradient is the same on every run.
 __global__ __launch_bounds__(64) void k_mlp_tile(const float *X, const float *Y, uint64_t rows, const _Float16 *w16,
-                                                 const float *master, float *out, float *grad) {
+                                                 const float *master, float *out, float *part) {
     __shared__ TileLds L;
     const int l = (int)lane_id();
     // stage parameters: W2 in FP16, the small layers in FP32 from the FP16 copy
@@ -191,24 +199,43 @@ __global__ __launch_bounds__(64) void k_mlp_tile(const float *X, const float *Y,
         }
     }
     if (!Y || blockIdx.x >= tiles) return;
-    const float inv = 1.f / (float)rows;
-    atomicAdd(&grad[P_W3 + l * kOut + 0], gw3a * inv);
-    atomicAdd(&grad[P_W3 + l * kOut + 1], gw3b * inv);
-    if (l < kOut) atomicAdd(&grad[P_B3 + l], gb3 * inv);
-    atomicAdd(&grad[P_B2 + l], gb2 * inv);
+    float *g = part + (uint64_t)blockIdx.x * P_NUM;
+    g[P_W3 + l * kOut + 0] = gw3a;
+    g[P_W3 + l * kOut + 1] = gw3b;
+    if (l < kOut) g[P_B3 + l] = gb3;
+    g[P_B2 + l] = gb2;
     for (int mt = 0; mt < 2; ++mt)
         for (int nt = 0; nt < 2; ++nt)
 #pragma unroll
-            for (int r = 0; r < 16; ++r)
-                atomicAdd(&grad[P_W2 + (mt * 32 + acc_row(r)) * kHid + nt * 32 + (l & 31)], gw2[mt][nt][r] * inv);
-    atomicAdd(&grad[P_B1 + l], gb1 * inv);
+            for (int r = 0; r < 16; ++r) g[P_W2 + (mt * 32 + acc_row(r)) * kHid + nt * 32 + (l & 31)] = gw2[mt][nt][r];
+    g[P_B1 + l] = gb1;
 #pragma unroll
-    for (int k = 0; k < kIn; ++k) atomicAdd(&grad[P_W1 + k * kHid + l], gw1[k] * inv);
+    for (int k = 0; k < kIn; ++k) g[P_W1 + k * kHid + l] = gw1[k];
 }
+
+// grad = (sum of the first nb workgroup rows of part) / rows (mean loss).
+__global__ void k_grad_reduce(const float *part, uint32_t nb, float inv, float *grad) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < P_NUM; p += gridDim.x * blockDim.x) {
+        float a = 0.f;
+        for (uint32_t b = 0; b < nb; ++b) a += part[(uint64_t)b * P_NUM + p];
+        grad[p] = a * inv;
+    }
+}
+
+constexpr uint32_t kMlpGrid = 1024;   // workgroups (one wave each) of a pass: four per CU
 
 static uint32_t mlp_grid(uint64_t rows) {
     const uint64_t tiles = (rows + kTile - 1) / kTile;
-    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(tiles, 512));
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(tiles, kMlpGrid));
+}
+
+// Gradient of the mean loss over `rows` rows: the tile pass, then the reduction
+// of its per-workgroup rows.
+static void mlp_gradient(hipStream_t st, const float *X, const float *Y, uint64_t rows, const _Float16 *w16,
+                         const float *master, float *part, float *grad) {
+    const uint32_t nb = mlp_grid(rows);
+    hipLaunchKernelGGL(k_mlp_tile, dim3(nb), dim3(64), 0, st, X, Y, rows, w16, master, (float *)nullptr, part);
+    hipLaunchKernelGGL(k_grad_reduce, dim3(grid_for(P_NUM)), dim3(256), 0, st, part, nb, 1.f / (float)rows, grad);
 }
 
 __global__ void k_adam(float *master, const float *grad, float *m1, float *m2, _Float16 *w16, int n, float lr,
@@ -278,13 +305,53 @@ __global__ void k_labels(const float *reads, uint64_t C, float *Y) {
     }
 }
 
+// Decision scores from the network outputs (P(read) * predicted log reads) and
+// the hotness score (reads over the four-snapshot window).  `mlp` holds the
+// network score as predicted reads (expm1) for the admission plan; `raw`, when
+// given, the score itself (the judge ranks by it).
+__global__ void k_plan_scores(const float *out, const float *hist, uint64_t C, float *mlp, float *raw, float *hot) {
+    for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < C; c += (uint64_t)gridDim.x * blockDim.x) {
+        const float p = 1.f / (1.f + __expf(-out[2 * c]));
+        const float sc = p * fmaxf(out[2 * c + 1], 0.f);
+        mlp[c] = expm1f(fminf(sc, 30.f));
+        if (raw) raw[c] = sc;
+        const float4 h = reinterpret_cast<const float4 *>(hist)[c];
+        hot[c] = h.x + h.y + h.z + h.w;
+    }
+}
+
+// One workgroup: sums[slot] = reads held by the first k chunks of a ranking
+// (sorted[0, k)), sums[2] = all reads; FP64 accumulation in a fixed order.
+__global__ __launch_bounds__(1024) void k_judge_sum(const float *sorted, uint64_t C, uint64_t k, double *sums,
+                                                   int slot) {
+    __shared__ double part[2][1024];
+    double top = 0, all = 0;
+    for (uint64_t i = threadIdx.x; i < C; i += blockDim.x) {
+        const double r = sorted[i];
+        all += r;
+        if (i < k) top += r;
+    }
+    part[0][threadIdx.x] = top;
+    part[1][threadIdx.x] = all;
+    __syncthreads();
+    for (uint32_t w = 512; w >= 1; w >>= 1) {
+        if (threadIdx.x < w) {
+            part[0][threadIdx.x] += part[0][threadIdx.x + w];
+            part[1][threadIdx.x] += part[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        sums[slot] = part[0][0];
+        sums[2] = part[1][0];
+    }
+}
+
 // ------------------------------------------------------------------ host part
 static void mlp_train_steps(Engine &e, MlpState &m, int steps) {
     const uint64_t rows = m.rows * (uint64_t)m.filled;   // every labelled decision in the ring
     for (int s = 0; s < steps; ++s) {
-        fill<float>(e.stream, m.grad.p, P_NUM, 0.f);
-        hipLaunchKernelGGL(k_mlp_tile, dim3(mlp_grid(rows)), dim3(64), 0, e.stream, m.X.p, m.Y.p, rows, m.w16.p,
-                           m.master.p, (float *)nullptr, m.grad.p);
+        mlp_gradient(e.stream, m.X.p, m.Y.p, rows, m.w16.p, m.master.p, m.part.p, m.grad.p);
         m.steps++;
         const float b1c = 1.f - std::pow(0.9f, (float)m.steps), b2c = 1.f - std::pow(0.999f, (float)m.steps);
         hipLaunchKernelGGL(k_adam, dim3(grid_for(P_NUM)), dim3(256), 0, e.stream, m.master.p, m.grad.p, m.m1.p,
@@ -299,6 +366,13 @@ void mlp_create(Engine &e) {
     const uint64_t C = e.cache.chunks;
     m->master.alloc(P_NUM);
     m->grad.alloc(P_NUM);
+    m->part.alloc((uint64_t)kMlpGrid * P_NUM);
+    m->mscore.alloc(C);
+    m->plan.alloc(C);
+    m->hot_dec.alloc(C);
+    m->sort_k.alloc(C);
+    m->sort_v.alloc(C);
+    m->judge.alloc(3);
     m->m1.alloc(P_NUM);
     m->m2.alloc(P_NUM);
     m->w16.alloc(P_NUM);
@@ -326,36 +400,41 @@ void mlp_create(Engine &e) {
     e.sync();
 }
 
-// Fraction of `reads` captured by the top-k chunks of `score`.
-static double capture(const std::vector<float> &score, const std::vector<float> &reads, uint64_t k) {
-    std::vector<uint64_t> idx(score.size());
-    std::iota(idx.begin(), idx.end(), 0ull);
-    if (k < idx.size())
-        std::nth_element(idx.begin(), idx.begin() + (ptrdiff_t)k, idx.end(),
-                         [&](uint64_t a, uint64_t b) { return score[a] > score[b] || (score[a] == score[b] && a < b); });
-    double got = 0, total = 0;
-    for (float r : reads) total += r;
-    for (uint64_t i = 0; i < std::min<uint64_t>(k, idx.size()); ++i) got += reads[idx[i]];
-    return total > 0 ? got / total : 1.0;
+// Judge: reads of the snapshot held by each ranking's top `frames` chunks.
+// Rankings sort descending and stably (ties keep chunk order, lowest id
+// first); the reads travel with the scores.  Returns {network, hotness}
+// fractions of all reads (1 when nothing was read).
+static std::pair<double, double> judge(Engine &e, MlpState &m, const float *reads) {
+    const uint64_t C = e.cache.chunks, k = e.cache.frames;
+    size_t tb = 0;
+    DCG_HIP(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tb, m.mscore.p, m.sort_k.p, reads, m.sort_v.p,
+                                                         (int)C, 0, 32, e.stream));
+    void *t = e.temp(tb);
+    const float *ranks[2] = {m.mscore.p, m.hot_dec.p};
+    for (int r = 0; r < 2; ++r) {
+        DCG_HIP(hipcub::DeviceRadixSort::SortPairsDescending(t, tb, ranks[r], m.sort_k.p, reads, m.sort_v.p, (int)C,
+                                                             0, 32, e.stream));
+        hipLaunchKernelGGL(k_judge_sum, dim3(1), dim3(1024), 0, e.stream, m.sort_v.p, C, k, m.judge.p, r);
+    }
+    DCG_HIP(hipGetLastError());
+    double sums[3];
+    DCG_HIP(hipMemcpyAsync(sums, m.judge.p, sizeof(sums), hipMemcpyDeviceToHost, e.stream));
+    e.sync();
+    if (!(sums[2] > 0)) return {1.0, 1.0};
+    return {sums[0] / sums[2], sums[1] / sums[2]};
 }
 
-void mlp_observe(Engine &e, const std::vector<float> &reads) {
+void mlp_observe(Engine &e) {
     MlpState &m = *e.cache.mlp;
     if (!m.have_decision) return;
     const uint64_t C = e.cache.chunks;
     // label the decision's features with what the snapshot actually read
     hipLaunchKernelGGL(k_labels, dim3(grid_for(C)), dim3(256), 0, e.stream, e.cache.d_reads.p, C,
                        m.Y.p + (uint64_t)m.slot * C * kOut);
-    std::vector<float> out(C * kOut), mscore(C);
-    DCG_HIP(hipMemcpyAsync(out.data(), m.out.p, C * kOut * 4, hipMemcpyDeviceToHost, e.stream));
-    e.sync();
-    for (uint64_t c = 0; c < C; ++c) {
-        const float p = 1.f / (1.f + std::exp(-out[2 * c]));
-        mscore[c] = p * std::max(out[2 * c + 1], 0.f);
-    }
     // judge: which ranking would have captured more of the observed reads
-    m.mlp_capture = capture(mscore, reads, e.cache.frames);
-    m.hot_capture = capture(m.hot_score_at_decision, reads, e.cache.frames);
+    const std::pair<double, double> cap = judge(e, m, e.cache.d_reads.p);
+    m.mlp_capture = cap.first;
+    m.hot_capture = cap.second;
     m.filled = std::min(kReplay, m.filled + 1);
     m.slot = (m.slot + 1) % kReplay;
     mlp_train_steps(e, m, 4);
@@ -386,24 +465,20 @@ std::vector<uint64_t> mlp_plan(Engine &e, uint64_t ab, uint64_t ac, uint64_t db,
                        e.cache.d_rows_last.p, m.degsum.p, e.cache.d_touch.p, m.lastuse.p, (float)e.snapshots, C, X);
     hipLaunchKernelGGL(k_mlp_tile, dim3(mlp_grid(C)), dim3(64), 0, e.stream, X, (const float *)nullptr, C, m.w16.p,
                        m.master.p, m.out.p, (float *)nullptr);
+    // both rankings of this decision stay on the device for the judge; the
+    // plan reads back the one it follows
+    hipLaunchKernelGGL(k_plan_scores, dim3(grid_for(C)), dim3(256), 0, e.stream, m.out.p, e.cache.d_hist.p, C,
+                       m.plan.p, m.mscore.p, m.hot_dec.p);
     DCG_HIP(hipGetLastError());
-    std::vector<float> out(C * kOut), hist(4 * C);
-    DCG_HIP(hipMemcpyAsync(out.data(), m.out.p, C * kOut * 4, hipMemcpyDeviceToHost, e.stream));
-    DCG_HIP(hipMemcpyAsync(hist.data(), e.cache.d_hist.p, 4 * C * 4, hipMemcpyDeviceToHost, e.stream));
-    e.sync();
     std::vector<float> score(C);
-    m.hot_score_at_decision.assign(C, 0.f);
-    for (uint64_t c = 0; c < C; ++c) {
-        m.hot_score_at_decision[c] = hist[4 * c] + hist[4 * c + 1] + hist[4 * c + 2] + hist[4 * c + 3];
-        const float p = 1.f / (1.f + std::exp(-out[2 * c]));
-        score[c] = p * std::max(out[2 * c + 1], 0.f);
-    }
+    DCG_HIP(hipMemcpyAsync(score.data(), m.fallback ? m.hot_dec.p : m.plan.p, C * 4, hipMemcpyDeviceToHost,
+                           e.stream));
+    e.sync();
     m.decisions++;
     m.fallbacks += m.fallback ? 1 : 0;
     m.have_decision = true;
-    if (m.fallback) return e.plan_admission(m.hot_score_at_decision, 4.0 * (double)kChunkEdges, 1.5);
+    if (m.fallback) return e.plan_admission(score, 4.0 * (double)kChunkEdges, 1.5);
     // the network predicts one snapshot's reads: admit above one chunk's worth
-    for (auto &x : score) x = std::expm1(std::min(x, 30.f));
     return e.plan_admission(score, (double)kChunkEdges, 1.5);
 }
 
@@ -435,9 +510,14 @@ extern "C" int dcg_mlp_eval(int32_t device, uint32_t rows, const float *x, const
             DCG_HIP(hipMemcpyAsync(dy.p, y, (uint64_t)rows * kOut * 4, hipMemcpyHostToDevice, s));
         }
         hipLaunchKernelGGL(k_to_half, dim3(grid_for(P_NUM)), dim3(256), 0, s, dp.p, w16.p, P_NUM);
-        fill<float>(s, dg.p, P_NUM, 0.f);
-        hipLaunchKernelGGL(k_mlp_tile, dim3(mlp_grid(rows)), dim3(64), 0, s, dx.p, y ? dy.p : (const float *)nullptr,
-                           (uint64_t)rows, w16.p, dp.p, dout.p, y ? dg.p : (float *)nullptr);
+        hipLaunchKernelGGL(k_mlp_tile, dim3(mlp_grid(rows)), dim3(64), 0, s, dx.p, (const float *)nullptr,
+                           (uint64_t)rows, w16.p, dp.p, dout.p, (float *)nullptr);
+        if (y) {
+            DevBuf<float> part;
+            part.alloc((uint64_t)kMlpGrid * P_NUM);
+            mlp_gradient(s, dx.p, dy.p, rows, w16.p, dp.p, part.p, dg.p);
+            DCG_HIP(hipStreamSynchronize(s));
+        }
         DCG_HIP(hipGetLastError());
         DCG_HIP(hipMemcpyAsync(out, dout.p, (uint64_t)rows * kOut * 4, hipMemcpyDeviceToHost, s));
         if (y && grad) DCG_HIP(hipMemcpyAsync(grad, dg.p, P_NUM * 4, hipMemcpyDeviceToHost, s));

@@ -371,7 +371,7 @@ void orc_check_pagerank(uint32_t n, const uint64_t *rp, const uint32_t *col, dou
     orc_pr_check r;
     memset(&r, 0, sizeof(r));
     double *ref = (double *)malloc((size_t)(n ? n : 1) * sizeof(double));
-    double *act = (double *)malloc((size_t)(n ? n : 1) * sizeof(double));
+    double *act = (double *)calloc((size_t)(n ? n : 1), sizeof(double));
     double *next = (double *)malloc((size_t)(n ? n : 1) * sizeof(double));
     /* base*n*1e-12 == 1e-12*n at the reference's base of 0.15 per vertex, scaled. */
     double tol = 1e-12 * (base / 0.15) * (double)(n ? n : 1);
