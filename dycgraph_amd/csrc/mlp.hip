@@ -40,12 +40,26 @@ constexpr int P_W1 = 0, P_B1 = P_W1 + kW1, P_W2 = P_B1 + kHid, P_B2 = P_W2 + kW2
 constexpr int kReplay = 4;   // decisions kept for training (ring)
 
 struct MlpState {
+    // Training runs on its own stream, overlapping the snapshot's mutation and
+    // repair work; `trained` marks the end of the last training launch.  The
+    // engine stream waits on it before a pass reads the weights or a decision
+    // is written into the ring.
+    hipStream_t train = nullptr;
+    hipEvent_t trained = nullptr;
+    ~MlpState() {
+        if (train) {
+            (void)hipStreamSynchronize(train);
+            (void)hipStreamDestroy(train);
+        }
+        if (trained) (void)hipEventDestroy(trained);
+    }
     DevBuf<float> master, grad, m1, m2;   // FP32 masters, gradients, Adam moments
     DevBuf<float> part;                   // per-workgroup gradient sums of a pass
     DevBuf<_Float16> w16;                 // FP16 weights used by the device passes
-    DevBuf<float> X, Y, out, degsum, lastuse;   // X/Y hold kReplay decisions of C rows each
+    DevBuf<float> X, Y, out, degsum, lastuse;   // X/Y hold kReplay labelled decisions of C rows each
+    DevBuf<float> Xcur;                   // features of the decision awaiting its labels
     uint64_t rows = 0;                    // C (rows per decision)
-    int slot = 0, filled = 0;             // ring position of the current decision
+    int slot = 0, filled = 0;             // ring slot the next labelled decision goes to
     int steps = 0;
     bool have_decision = false, fallback = true;
     double mlp_capture = 0, hot_capture = 0;
@@ -348,16 +362,21 @@ __global__ __launch_bounds__(1024) void k_judge_sum(const float *sorted, uint64_
 }
 
 // ------------------------------------------------------------------ host part
-static void mlp_train_steps(Engine &e, MlpState &m, int steps) {
-    const uint64_t rows = m.rows * (uint64_t)m.filled;   // every labelled decision in the ring
+// `steps` Adam steps over every labelled decision in the ring, on the training
+// stream (the caller has made the ring and weights ready on the engine stream
+// and synchronised it).
+static void mlp_train_steps(MlpState &m, int steps) {
+    const uint64_t rows = m.rows * (uint64_t)m.filled;
+    if (!rows) return;
     for (int s = 0; s < steps; ++s) {
-        mlp_gradient(e.stream, m.X.p, m.Y.p, rows, m.w16.p, m.master.p, m.part.p, m.grad.p);
+        mlp_gradient(m.train, m.X.p, m.Y.p, rows, m.w16.p, m.master.p, m.part.p, m.grad.p);
         m.steps++;
         const float b1c = 1.f - std::pow(0.9f, (float)m.steps), b2c = 1.f - std::pow(0.999f, (float)m.steps);
-        hipLaunchKernelGGL(k_adam, dim3(grid_for(P_NUM)), dim3(256), 0, e.stream, m.master.p, m.grad.p, m.m1.p,
+        hipLaunchKernelGGL(k_adam, dim3(grid_for(P_NUM)), dim3(256), 0, m.train, m.master.p, m.grad.p, m.m1.p,
                            m.m2.p, m.w16.p, P_NUM, 3e-3f, b1c, b2c);
     }
     DCG_HIP(hipGetLastError());
+    DCG_HIP(hipEventRecord(m.trained, m.train));
 }
 
 void mlp_create(Engine &e) {
@@ -376,7 +395,10 @@ void mlp_create(Engine &e) {
     m->m1.alloc(P_NUM);
     m->m2.alloc(P_NUM);
     m->w16.alloc(P_NUM);
+    DCG_HIP(hipStreamCreateWithFlags(&m->train, hipStreamNonBlocking));
+    DCG_HIP(hipEventCreateWithFlags(&m->trained, hipEventDisableTiming));
     m->X.alloc(kReplay * C * kIn);
+    m->Xcur.alloc(C * kIn);
     m->Y.alloc(kReplay * C * kOut);
     m->out.alloc(C * kOut);
     m->degsum.alloc(C);
@@ -428,7 +450,11 @@ void mlp_observe(Engine &e) {
     MlpState &m = *e.cache.mlp;
     if (!m.have_decision) return;
     const uint64_t C = e.cache.chunks;
+    // the ring is read by a training launch that may still run
+    DCG_HIP(hipStreamWaitEvent(e.stream, m.trained, 0));
     // label the decision's features with what the snapshot actually read
+    DCG_HIP(hipMemcpyAsync(m.X.p + (uint64_t)m.slot * C * kIn, m.Xcur.p, C * kIn * 4, hipMemcpyDeviceToDevice,
+                           e.stream));
     hipLaunchKernelGGL(k_labels, dim3(grid_for(C)), dim3(256), 0, e.stream, e.cache.d_reads.p, C,
                        m.Y.p + (uint64_t)m.slot * C * kOut);
     // judge: which ranking would have captured more of the observed reads
@@ -437,7 +463,6 @@ void mlp_observe(Engine &e) {
     m.hot_capture = cap.second;
     m.filled = std::min(kReplay, m.filled + 1);
     m.slot = (m.slot + 1) % kReplay;
-    mlp_train_steps(e, m, 4);
     // the network's ranking is used only once trained and clearly ahead
     m.fallback = m.steps < 8 || !(m.mlp_capture > 1.02 * m.hot_capture);
     m.have_decision = false;
@@ -454,13 +479,14 @@ void mlp_stats(const Engine &e, dcg_cache_stats *st) {
 std::vector<uint64_t> mlp_plan(Engine &e, uint64_t ab, uint64_t ac, uint64_t db, uint64_t dc) {
     MlpState &m = *e.cache.mlp;
     const uint64_t C = e.cache.chunks;
+    DCG_HIP(hipStreamWaitEvent(e.stream, m.trained, 0));   // the forward pass reads the trained weights
     fill<float>(e.stream, e.cache.d_touch.p, C, 0.f);
     hipLaunchKernelGGL(k_touch, dim3(grid_for(ac + dc, 256, 8192)), dim3(256), 0, e.stream, e.d_out.p, e.d_in.p,
                        e.d_adds.p + ab, ac, e.d_dels.p + db, dc, e.n, e.cache.d_touch.p);
     std::vector<float> lu(C);
     for (uint64_t c = 0; c < C; ++c) lu[c] = (float)e.cache.last_use[c];
     DCG_HIP(hipMemcpyAsync(m.lastuse.p, lu.data(), C * 4, hipMemcpyHostToDevice, e.stream));
-    float *X = m.X.p + (uint64_t)m.slot * C * kIn;
+    float *X = m.Xcur.p;
     hipLaunchKernelGGL(k_features, dim3(grid_for(C)), dim3(256), 0, e.stream, e.cache.d_hist.p,
                        e.cache.d_rows_last.p, m.degsum.p, e.cache.d_touch.p, m.lastuse.p, (float)e.snapshots, C, X);
     hipLaunchKernelGGL(k_mlp_tile, dim3(mlp_grid(C)), dim3(64), 0, e.stream, X, (const float *)nullptr, C, m.w16.p,
@@ -474,6 +500,8 @@ std::vector<uint64_t> mlp_plan(Engine &e, uint64_t ab, uint64_t ac, uint64_t db,
     DCG_HIP(hipMemcpyAsync(score.data(), m.fallback ? m.hot_dec.p : m.plan.p, C * 4, hipMemcpyDeviceToHost,
                            e.stream));
     e.sync();
+    // online training on the labelled decisions so far overlaps the batch
+    mlp_train_steps(m, 4);
     m.decisions++;
     m.fallbacks += m.fallback ? 1 : 0;
     m.have_decision = true;
