@@ -159,6 +159,7 @@ struct Engine {
     DevBuf<uint32_t> d_gstart, d_gstart2, d_uniq;
     DevBuf<uint32_t> d_used;           // per-record occurrence counters
     DevBuf<uint32_t> d_big_groups;     // large-row mutation paths
+    DevBuf<uint8_t> d_flags;           // group-head flags of a sorted key array
     DevBuf<unsigned long long> d_counters;  // phase counters (see store.hip)
     unsigned long long *h_counters = nullptr;
 

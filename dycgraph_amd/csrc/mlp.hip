@@ -227,12 +227,22 @@ __global__ __launch_bounds__(64) void k_mlp_tile(const float *X, const float *Y,
     for (int k = 0; k < kIn; ++k) g[P_W1 + k * kHid + l] = gw1[k];
 }
 
-// grad = (sum of the first nb workgroup rows of part) / rows (mean loss).
-__global__ void k_grad_reduce(const float *part, uint32_t nb, float inv, float *grad) {
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < P_NUM; p += gridDim.x * blockDim.x) {
-        float a = 0.f;
-        for (uint32_t b = 0; b < nb; ++b) a += part[(uint64_t)b * P_NUM + p];
-        grad[p] = a * inv;
+// grad = (sum of the first nb workgroup rows of part) / rows (mean loss).  A
+// workgroup takes 64 parameters; its 16 waves sum interleaved rows, then one
+// wave adds the 16 partial sums in order (the same result on every run).
+__global__ __launch_bounds__(1024) void k_grad_reduce(const float *part, uint32_t nb, float inv, float *grad) {
+    __shared__ float acc[16][64];
+    const uint32_t c = threadIdx.x & 63, r0 = threadIdx.x >> 6;
+    const uint32_t p = blockIdx.x * 64 + c;
+    float a = 0.f;
+    if (p < P_NUM)
+        for (uint32_t b = r0; b < nb; b += 16) a += part[(uint64_t)b * P_NUM + p];
+    acc[r0][c] = a;
+    __syncthreads();
+    if (threadIdx.x < 64 && p < P_NUM) {
+        float sum = 0.f;
+        for (int r = 0; r < 16; ++r) sum += acc[r][c];
+        grad[p] = sum * inv;
     }
 }
 
@@ -249,7 +259,7 @@ static void mlp_gradient(hipStream_t st, const float *X, const float *Y, uint64_
                          const float *master, float *part, float *grad) {
     const uint32_t nb = mlp_grid(rows);
     hipLaunchKernelGGL(k_mlp_tile, dim3(nb), dim3(64), 0, st, X, Y, rows, w16, master, (float *)nullptr, part);
-    hipLaunchKernelGGL(k_grad_reduce, dim3(grid_for(P_NUM)), dim3(256), 0, st, part, nb, 1.f / (float)rows, grad);
+    hipLaunchKernelGGL(k_grad_reduce, dim3((P_NUM + 63) / 64), dim3(1024), 0, st, part, nb, 1.f / (float)rows, grad);
 }
 
 __global__ void k_adam(float *master, const float *grad, float *m1, float *m2, _Float16 *w16, int n, float lr,

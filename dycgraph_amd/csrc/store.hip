@@ -389,24 +389,26 @@ template <>
 __device__ __forceinline__ uint32_t row_of<uint32_t>(uint32_t k) { return k; }
 
 template <typename K>
-__global__ void k_group_heads(const K *keys, uint64_t count, uint32_t *gstart, unsigned long long *ngroups) {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t base = blockIdx.x * (uint64_t)blockDim.x; base < count; base += stride) {
-        const uint64_t i = base + threadIdx.x;
-        bool head = false;
-        if (i < count) {
-            const uint32_t r = row_of<K>(keys[i]);
-            head = r != ~0u && (i == 0 || row_of<K>(keys[i - 1]) != r);
-        }
-        const uint64_t mask = __ballot(head);
-        if (mask) {
-            const int leader = __ffsll((unsigned long long)mask) - 1;
-            unsigned long long b = 0;
-            if ((int)lane_id() == leader) b = atomicAdd(ngroups, (unsigned long long)__popcll(mask));
-            b = __shfl(b, leader, kWave);
-            if (head) gstart[b + lanes_below(mask)] = (uint32_t)i;
-        }
+__global__ void k_head_flags(const K *keys, uint64_t count, uint8_t *flags) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t r = row_of<K>(keys[i]);
+        flags[i] = r != ~0u && (i == 0 || row_of<K>(keys[i - 1]) != r) ? 1 : 0;
     }
+}
+
+// Heads of the runs of equal rows in a sorted key array (records of invalid
+// edges, row ~0, sort last and form no group), compacted in key order: group g
+// spans [gstart[g], gstart[g + 1]).
+template <typename K>
+static void group_heads(Engine &e, const K *keys, uint64_t count, uint32_t *gstart, unsigned long long *ngroups) {
+    e.d_flags.ensure(count);
+    hipLaunchKernelGGL(k_head_flags<K>, dim3(grid_for(count)), dim3(256), 0, e.stream, keys, count, e.d_flags.p);
+    hipcub::CountingInputIterator<uint32_t> idx(0u);
+    size_t tb = 0;
+    DCG_HIP(hipcub::DeviceSelect::Flagged(nullptr, tb, idx, e.d_flags.p, gstart, ngroups, (int)count, e.stream));
+    void *t = e.temp(tb);
+    DCG_HIP(hipcub::DeviceSelect::Flagged(t, tb, idx, e.d_flags.p, gstart, ngroups, (int)count, e.stream));
 }
 
 template <typename K>
@@ -418,6 +420,14 @@ __device__ __forceinline__ uint64_t group_end(const K *keys, uint64_t s, uint64_
         else hi = mid;
     }
     return lo;
+}
+
+// End of group g: the next group's head, or for the last group the end of its
+// run of equal rows.
+template <typename K>
+__device__ __forceinline__ uint64_t group_end_of(const K *keys, const uint32_t *gstart, uint64_t g, uint64_t ngroups,
+                                                 uint64_t gs, uint64_t count, uint32_t row) {
+    return g + 1 < ngroups ? (uint64_t)gstart[g + 1] : group_end<K>(keys, gs, count, row);
 }
 
 __device__ __forceinline__ uint32_t load_slot(const StoreView &s, uint64_t slot) { return *row_base(s, slot); }
@@ -686,7 +696,7 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
         const uint64_t gs = gstart[g];
         const uint32_t row = (uint32_t)(keys[gs] >> 32);
         const RowDesc d = desc[row];
-        const uint64_t ge = group_end<uint64_t>(keys, gs, count, row);
+        const uint64_t ge = group_end_of<uint64_t>(keys, gstart, g, ngroups, gs, count, row);
         const uint32_t S = sorted[row] < d.ext ? sorted[row] : d.ext;
         if (d.ext > kBigDelete && !wave_searchable(d, S, ge - gs)) {
             if (lane == 0) big_groups[atomicAdd(nbig, 1ull)] = (uint32_t)gs;
@@ -1006,7 +1016,7 @@ __global__ __launch_bounds__(256) void k_add_plan(const RowDesc *desc, const uin
          g += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t gs = gstart[g];
         const uint32_t row = keys[gs];
-        const uint64_t ge = group_end<uint32_t>(keys, gs, count, row);
+        const uint64_t ge = group_end_of<uint32_t>(keys, gstart, g, ngroups, gs, count, row);
         const RowDesc d = desc[row];
         const uint64_t nd = (uint64_t)d.ext + (ge - gs);
         if (nd > row_cap(d)) {
@@ -1072,7 +1082,7 @@ __global__ __launch_bounds__(256) void k_add_assign(const RowDesc *desc, const u
         if (g < ngroups) {
             const uint64_t gs = gstart[g];
             const uint32_t row = keys[gs];
-            const uint64_t ge = group_end<uint32_t>(keys, gs, count, row);
+            const uint64_t ge = group_end_of<uint32_t>(keys, gstart, g, ngroups, gs, count, row);
             const RowDesc d = desc[row];
             const uint64_t nd = (uint64_t)d.ext + (ge - gs);
             if (nd > row_cap(d)) {
@@ -1130,7 +1140,7 @@ __global__ __launch_bounds__(256) void k_add_rows(StoreView s, RowDesc *desc, ui
     for (uint64_t g = wave; g < ngroups; g += nwaves) {
         const uint64_t gs = gstart[g];
         const uint32_t row = keys[gs];
-        const uint64_t ge = group_end<uint32_t>(keys, gs, count, row);
+        const uint64_t ge = group_end_of<uint32_t>(keys, gstart, g, ngroups, gs, count, row);
         const uint32_t k = (uint32_t)(ge - gs);
         RowDesc d = desc[row];
         const uint64_t nd = (uint64_t)d.ext + k;
@@ -1236,10 +1246,8 @@ void Engine::mutate_delete(uint64_t begin, uint64_t count, dcg_phase_stats *st) 
     void *t = temp(tb);
     DCG_HIP(hipcub::DeviceRadixSort::SortKeys(t, tb, d_k64c.p, kout, (int)count, 0, 64, stream));
     DCG_HIP(hipcub::DeviceRadixSort::SortKeys(t, tb, d_k64d.p, kin, (int)count, 0, 64, stream));
-    hipLaunchKernelGGL(k_group_heads<uint64_t>, dim3(grid_for(count)), dim3(256), 0, stream, kout, count, d_gstart.p,
-                       d_counters.p + C_GROUPS_OUT);
-    hipLaunchKernelGGL(k_group_heads<uint64_t>, dim3(grid_for(count)), dim3(256), 0, stream, kin, count, d_gstart2.p,
-                       d_counters.p + C_GROUPS_IN);
+    group_heads<uint64_t>(*this, kout, count, d_gstart.p, d_counters.p + C_GROUPS_OUT);
+    group_heads<uint64_t>(*this, kin, count, d_gstart2.p, d_counters.p + C_GROUPS_IN);
     StoreView v = view();
     const uint32_t wg = wave_grid(count);
     d_big_groups.ensure(count);
@@ -1307,10 +1315,8 @@ void Engine::mutate_add(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
     void *t = temp(tb);
     DCG_HIP(hipcub::DeviceRadixSort::SortPairs(t, tb, tk, d_k32a.p, tv, d_v32a.p, (int)count, 0, 32, stream));
     DCG_HIP(hipcub::DeviceRadixSort::SortPairs(t, tb, tk2, d_k32b.p, tv2, d_v32b.p, (int)count, 0, 32, stream));
-    hipLaunchKernelGGL(k_group_heads<uint32_t>, dim3(grid_for(count)), dim3(256), 0, stream, d_k32a.p, count,
-                       d_gstart.p, d_counters.p + C_GROUPS_OUT);
-    hipLaunchKernelGGL(k_group_heads<uint32_t>, dim3(grid_for(count)), dim3(256), 0, stream, d_k32b.p, count,
-                       d_gstart2.p, d_counters.p + C_GROUPS_IN);
+    group_heads<uint32_t>(*this, d_k32a.p, count, d_gstart.p, d_counters.p + C_GROUPS_OUT);
+    group_heads<uint32_t>(*this, d_k32b.p, count, d_gstart2.p, d_counters.p + C_GROUPS_IN);
     hipLaunchKernelGGL(k_add_plan, dim3(grid_for(count, 256, 1024)), dim3(256), 0, stream, d_out.p, d_k32a.p, count,
                        d_gstart.p, d_counters.p + C_GROUPS_OUT, d_tail.p);
     hipLaunchKernelGGL(k_add_plan, dim3(grid_for(count, 256, 1024)), dim3(256), 0, stream, d_in.p, d_k32b.p, count,
