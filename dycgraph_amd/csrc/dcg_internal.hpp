@@ -56,7 +56,21 @@ struct StoreView {
     const int32_t *chunk_frame;// chunk -> frame, -1 when not resident
     uint32_t *hot_out;         // per-vertex out-row expansions this snapshot
     uint32_t *hot_in;          // per-vertex in-row expansions this snapshot
+    uint32_t *hot_list;        // rows whose counter left zero (v | direction << 31); null when device-placed
+    unsigned long long *hot_count;
 };
+
+// Count one read pass over row v (hot is hot_out or hot_in).  With a host
+// store the first count of a row since the last fold lists the row, so the
+// fold visits only rows that were read.
+__device__ __forceinline__ void mark_read(const StoreView &s, uint32_t *hot, uint32_t v) {
+    if (!s.hot_list) {
+        hot[v] += 1;
+        return;
+    }
+    if (atomicAdd(&hot[v], 1u) == 0u)
+        s.hot_list[atomicAdd(s.hot_count, 1ull)] = v | (hot == s.hot_in ? 0x80000000u : 0u);
+}
 
 __device__ __forceinline__ const uint32_t *row_base(const StoreView &s, uint64_t off) {
     const uint32_t c = (uint32_t)(off >> kChunkLog2);

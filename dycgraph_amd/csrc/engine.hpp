@@ -105,6 +105,8 @@ struct Engine {
     uint32_t *h_slab = nullptr;        // pinned host slots (DCG_PLACE_HOST)
     DevBuf<uint64_t> d_tail;           // [0] bump tail, [1] overflow flag, [2..] class cursors
     DevBuf<uint32_t> d_hot_out, d_hot_in;
+    DevBuf<uint32_t> d_hot_list;               // rows read since the last fold (host store only)
+    DevBuf<unsigned long long> d_hot_count;
     DevBuf<uint32_t> d_sorted_out, d_sorted_in;   // per row: length of the sorted prefix
     // Fences of the sorted prefixes, in HBM: row r owns fence[fbase[r] .. fbase[r+1]),
     // fence j holding the id at slot 16j of the row (valid for 16j below its

@@ -55,6 +55,8 @@ StoreView Engine::view() {
     v.chunk_frame = cache.d_chunk_frame.p;
     v.hot_out = d_hot_out.p;
     v.hot_in = d_hot_in.p;
+    v.hot_list = h_slab ? d_hot_list.p : nullptr;
+    v.hot_count = d_hot_count.p;
     return v;
 }
 
@@ -327,6 +329,11 @@ void Engine::load_edges_device(uint32_t nn, const uint32_t *d_src, const uint32_
     d_hot_in.alloc(n ? n : 1);
     fill<uint32_t>(stream, d_hot_out.p, n, 0u);
     fill<uint32_t>(stream, d_hot_in.p, n, 0u);
+    d_hot_count.alloc(1);
+    fill<unsigned long long>(stream, d_hot_count.p, 1, 0ull);
+    // each row enters the list at most once per fold: capacity 2n
+    if (cfg.placement != DCG_PLACE_DEVICE) d_hot_list.alloc(2ull * (n ? n : 1));
+    else d_hot_list.release();
 
     cache.chunks = total_slots / kChunkEdges;
     if (cfg.placement == DCG_PLACE_DEVICE) {
@@ -778,7 +785,7 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
                 desc[row].meta = make_meta(d.meta & 63u, tombs);
             }
             if (S2 != S || S != sorted[row]) sorted[row] = S2;
-            if (full) hot[row] += 1;   // a pass read the whole row (cache-policy signal)
+            if (full) mark_read(s, hot, row);   // a pass read the whole row (cache-policy signal)
             missing += (ge - gs) - removed;
             changed += removed ? 1 : 0;
             rd += read;
@@ -996,7 +1003,7 @@ __global__ __launch_bounds__(1024) void k_del_big(StoreView s, RowDesc *desc, ui
             atomicAdd(&counters[C_SLOTS_READ], scanned);
             atomicAdd(&counters[C_B_ROWS], 1ull);
             if (new_tombs) atomicAdd(&counters[C_SLOTS_WRITTEN], (unsigned long long)new_tombs);
-            if (full_read) hot[row] += 1;   // a pass read the whole row (cache-policy signal)
+            if (full_read) mark_read(s, hot, row);   // a pass read the whole row (cache-policy signal)
         }
         if (moved) atomicAdd(&counters[C_SLOTS_WRITTEN], (unsigned long long)moved);
         __syncthreads();
@@ -1155,7 +1162,7 @@ __global__ __launch_bounds__(256) void k_add_rows(StoreView s, RowDesc *desc, ui
             // entries (tombstones included) keep their positions, so the
             // sorted prefix length of the row stays valid
             for (uint32_t i = lane; i < d.ext; i += kWave) store_slot(s, noff + i, load_slot(s, d.off + i));
-            if (lane == 0 && d.ext) hot[row] += 1;
+            if (lane == 0 && d.ext) mark_read(s, hot, row);
             if (lane == 0 && d.meta) retire_block(tail, pool, d);
             d.off = noff;
             d.meta = make_meta(cls, row_tombs(d));
@@ -1210,7 +1217,7 @@ __global__ __launch_bounds__(1024) void k_add_big(StoreView s, RowDesc *desc, ui
         if (threadIdx.x == 0) {
             desc[row] = RowDesc{noff, (uint32_t)nd, make_meta(cls, row_tombs(d))};
             retire_block(tail, pool, d);
-            hot[row] += 1;
+            mark_read(s, hot, row);
             if (count_stats) atomicAdd(&counters[C_RELOCATED], 1ull);
             atomicAdd(&counters[C_SLOTS_READ], (unsigned long long)d.ext);
             atomicAdd(&counters[C_SLOTS_WRITTEN], (unsigned long long)nd);
@@ -1447,32 +1454,34 @@ void Engine::export_csr(uint64_t *row_ptr, uint32_t *col, uint64_t *m) {
 // --------------------------------------------------- per-phase read accounting
 // Folds per-vertex expansion counters into per-chunk edge reads (the cache
 // policy's signal) and splits reads into HBM-cache and host-link reads.
-__global__ void k_fold_reads(StoreView s, uint32_t n, float *reads, float *rows, unsigned long long *counters) {
+// Fold the listed rows' read passes into per-chunk edge reads (and rows) for
+// the cache policy and into the cached / host read counters; counters and the
+// list are cleared for the next fold.
+__global__ void k_fold_reads(StoreView s, const unsigned long long *nlist_p, float *reads, float *rows,
+                             unsigned long long *counters) {
     unsigned long long cached = 0, host = 0;
-    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t ho = s.hot_out[v], hi = s.hot_in[v];
-        if (!(ho | hi)) continue;
-        for (int dir = 0; dir < 2; ++dir) {
-            const uint32_t h = dir ? hi : ho;
-            if (!h) continue;
-            const RowDesc d = dir ? s.in[v] : s.out[v];
-            uint64_t a = d.off, e = d.off + d.ext;
-            while (a < e) {
-                const uint64_t c = a >> kChunkLog2;
-                const uint64_t ce = (c + 1) << kChunkLog2;
-                const uint64_t b = e < ce ? e : ce;
-                const float edges = (float)h * (float)(b - a);
-                if (reads) {
-                    atomicAdd(&reads[c], edges);
-                    atomicAdd(&rows[c], (float)h);
-                }
-                if (s.chunk_frame[c] >= 0) cached += (unsigned long long)h * (b - a);
-                else host += (unsigned long long)h * (b - a);
-                a = b;
+    const uint64_t nlist = *nlist_p;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nlist;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t e = s.hot_list[i], v = e & 0x7FFFFFFFu;
+        const bool in = (e >> 31) != 0;
+        uint32_t *hot = in ? s.hot_in : s.hot_out;
+        const uint32_t h = hot[v];
+        hot[v] = 0;
+        const RowDesc d = in ? s.in[v] : s.out[v];
+        uint64_t a = d.off, end = d.off + d.ext;
+        while (a < end) {
+            const uint64_t c = a >> kChunkLog2;
+            const uint64_t ce = (c + 1) << kChunkLog2;
+            const uint64_t b = end < ce ? end : ce;
+            if (reads) {
+                atomicAdd(&reads[c], (float)h * (float)(b - a));
+                atomicAdd(&rows[c], (float)h);
             }
+            if (s.chunk_frame[c] >= 0) cached += (unsigned long long)h * (b - a);
+            else host += (unsigned long long)h * (b - a);
+            a = b;
         }
-        s.hot_out[v] = 0;
-        s.hot_in[v] = 0;
     }
     // wave-reduce the two counters before the global atomics
     for (int d = 32; d >= 1; d >>= 1) {
@@ -1495,8 +1504,10 @@ void Engine::fold_reads(dcg_phase_stats *st) {
     fill<unsigned long long>(stream, d_counters.p + C_CACHED_READS, 2, 0ull);
     float *reads = cache.d_reads.n ? cache.d_reads.p : nullptr;
     float *rows = cache.d_reads.n ? cache.d_reads.p + cache.chunks : nullptr;
-    hipLaunchKernelGGL(k_fold_reads, dim3(grid_for(n, 256, 16384)), dim3(256), 0, stream, view(), n, reads, rows,
-                       d_counters.p);
+    // at most 2n rows are listed; the grid strides over what the list holds
+    hipLaunchKernelGGL(k_fold_reads, dim3(grid_for(2ull * n, 256, 4096)), dim3(256), 0, stream, view(), d_hot_count.p,
+                       reads, rows, d_counters.p);
+    fill<unsigned long long>(stream, d_hot_count.p, 1, 0ull);
     DCG_HIP(hipGetLastError());
     unsigned long long c[2];
     DCG_HIP(hipMemcpyAsync(c, d_counters.p + C_CACHED_READS, 16, hipMemcpyDeviceToHost, stream));

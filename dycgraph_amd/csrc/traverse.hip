@@ -433,7 +433,7 @@ __device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L
                     deg = d.ext;
                     off = d.off;
                     val = __float_as_uint(0.85f * x / (float)live);
-                    a.s.hot_out[v] += 1;
+                    mark_read(a.s, a.s.hot_out, v);
                 }
             } else if (MODE == M_PR_DENSE) {
                 v = g;   // every vertex pulls over its in-row
@@ -441,7 +441,7 @@ __device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L
                 if (d.ext) {
                     deg = d.ext;
                     off = d.off;
-                    a.s.hot_in[v] += 1;
+                    mark_read(a.s, a.s.hot_in, v);
                 }
             } else {
                 decode_g(a, g, q, v);
@@ -450,8 +450,7 @@ __device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L
                 if (d.ext && (MODE == M_PULL || (uint32_t)(val >> 32) != kInf)) {
                     deg = d.ext;
                     off = d.off;
-                    if (MODE == M_PULL) a.s.hot_in[v] += 1;
-                    else a.s.hot_out[v] += 1;
+                    mark_read(a.s, MODE == M_PULL ? a.s.hot_in : a.s.hot_out, v);
                 }
             }
         }
