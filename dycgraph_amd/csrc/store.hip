@@ -674,19 +674,23 @@ __device__ __forceinline__ bool wave_searchable(const RowDesc &d, uint32_t S, ui
            tombs <= kMaxTombs / 2;
 }
 
-// One wave per touched row of at most kBigDelete slots, or a larger row that
+// Touched rows of at most kBigDelete slots, and larger rows that
 // wave_searchable admits; the rest are listed for k_del_big.  `sorted` holds
-// each row's sorted prefix length S.
-//  * S >= kWaveSearch: the prefix is searched through the row's fences
-//    (fenced_take: the whole wave for a single run of equal targets, a
-//    quarter-wave per run for up to kQuarterRuns targets; search_prefix, a
-//    lane per run, beyond), then (only when a run still wants occurrences)
-//    wave_delete_range over the tail [S, ext), which holds appended entries
-//    and never tombstones.  The prefix precedes the tail in the row's
-//    sequence, so exactly the first occurrences go.
-//  * otherwise one wave_delete_range pass over the row; S shrinks by the
-//    entries removed below it (removal keeps a prefix sorted).
-//  * tombstones above a quarter of the row: wave_compact_tombs.
+// each row's sorted prefix length S.  A wave takes four groups at a time, one
+// per quarter-wave:
+//  * phase A (quarter): a group of at most kQuarterRuns targets on a prefix of
+//    S >= kWaveSearch is searched through the row's fences (fenced_take, run
+//    by run).  When every run was satisfied and the tombstones stay below a
+//    quarter of the row, the quarter finishes the group (row descriptor, stats).
+//  * phase B (whole wave, one remaining group at a time): for a searched
+//    group that still wants occurrences, wave_delete_range over the tail
+//    [S, ext), which holds appended entries and never tombstones (the prefix
+//    precedes the tail in the row's sequence, so exactly the first
+//    occurrences go); a group of more targets is searched here (the whole wave
+//    for one run of equal targets, search_prefix with a lane per run
+//    otherwise); a prefix below kWaveSearch takes one wave_delete_range pass
+//    over the row (S shrinks by the entries removed below it); and
+//    tombstones above a quarter of the row take wave_compact_tombs.
 // Compaction refreshes the fences of the prefix entries it moves.
 __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, uint32_t *sorted, uint32_t *hot,
                                                   const uint64_t *keys, uint64_t count,
@@ -694,104 +698,144 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
                                                   uint32_t *used, unsigned long long *counters, int count_stats,
                                                   uint32_t *big_groups, unsigned long long *nbig,
                                                   const uint32_t *fbase, uint32_t *fence) {
-    const uint32_t lane = lane_id();
+    const uint32_t lane = lane_id(), qd = lane >> 4, ql = lane & 15;
     const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kWave;
     const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / kWave;
     const uint64_t ngroups = *ngroups_p;
     unsigned long long missing = 0, changed = 0, rd = 0, wr = 0, rows = 0;
-    for (uint64_t g = wave; g < ngroups; g += nwaves) {
-        const uint64_t gs = gstart[g];
-        const uint32_t row = (uint32_t)(keys[gs] >> 32);
-        const RowDesc d = desc[row];
-        const uint64_t ge = group_end_of<uint64_t>(keys, gstart, g, ngroups, gs, count, row);
-        const uint32_t S = sorted[row] < d.ext ? sorted[row] : d.ext;
-        if (d.ext > kBigDelete && !wave_searchable(d, S, ge - gs)) {
-            if (lane == 0) big_groups[atomicAdd(nbig, 1ull)] = (uint32_t)gs;
-            continue;
+    for (uint64_t base = wave * 4; base < ngroups; base += nwaves * 4) {
+        // ---- phase A: one group per quarter-wave (values uniform in the quarter)
+        const uint64_t g = base + qd;
+        const bool valid = g < ngroups;
+        uint64_t gs = 0, ge = 0;
+        uint32_t row = 0, S = 0;
+        RowDesc d{0, 0, 0};
+        if (valid) {
+            gs = gstart[g];
+            row = (uint32_t)(keys[gs] >> 32);
+            d = desc[row];
+            ge = group_end_of<uint64_t>(keys, gstart, g, ngroups, gs, count, row);
+            S = sorted[row] < d.ext ? sorted[row] : d.ext;
         }
-        uint32_t ext = d.ext, S2 = S, removed = 0, moved = 0, new_tombs = 0, read = 0;
-        bool full = false;
-        const Fences F = fences_of(fbase, fence, row);
-        if (S >= kWaveSearch) {
-            uint32_t probes = 0, need = 0;
-            const uint32_t v0 = (uint32_t)keys[gs];
-            if ((uint32_t)keys[ge - 1] == v0) {
-                // one run (the group's targets are sorted): the whole wave searches it
-                const uint32_t c = (uint32_t)(ge - gs);
-                const uint32_t got = fenced_take<64>(s, d.off, S, F, v0, c, lane, 0, probes);
-                if (lane == 0) {
-                    used[gs] = got;
-                    new_tombs = got;
-                    need = got < c ? 1u : 0u;
-                }
-            } else if (ge - gs <= kQuarterRuns) {
-                // a quarter-wave per run, four runs at a time
-                const uint32_t quarter = lane >> 4, q = lane & 15;
-                for (uint64_t q0 = gs; q0 < ge; q0 += 4) {
-                    const uint64_t t = q0 + quarter;
-                    if (t < ge && (t == gs || (uint32_t)keys[t - 1] != (uint32_t)keys[t])) {
-                        const uint32_t v = (uint32_t)keys[t];
-                        const uint32_t c = (uint32_t)(target_ub(keys, t + 1, ge, v) - t);
-                        const uint32_t got = fenced_take<16>(s, d.off, S, F, v, c, q, quarter * 16, probes);
-                        if (q == 0) {
-                            used[t] = got;
-                            new_tombs += got;
-                            need |= got < c ? 1u : 0u;
-                        }
-                    }
-                }
-            } else {
-                // many runs: one lane per run
-                for (uint64_t q0 = gs; q0 < ge; q0 += kWave)
-                    if (q0 + lane < ge)
-                        new_tombs += search_prefix(s, d.off, S, F, keys, q0 + lane, gs, ge, used, probes, need);
+        const bool big = valid && d.ext > kBigDelete && !wave_searchable(d, S, ge - gs);
+        if (big && ql == 0) big_groups[atomicAdd(nbig, 1ull)] = (uint32_t)gs;
+        const bool qsearch = valid && !big && S >= kWaveSearch && ge - gs <= kQuarterRuns;
+        uint32_t new_tombs = 0, probes = 0, need = 0;
+        if (qsearch) {
+            const Fences F = fences_of(fbase, fence, row);
+            for (uint64_t t = gs; t < ge; ++t) {
+                const uint32_t v = (uint32_t)keys[t];
+                if (t > gs && (uint32_t)keys[t - 1] == v) continue;
+                const uint32_t c = (uint32_t)(target_ub(keys, t + 1, ge, v) - t);
+                const uint32_t got = fenced_take<16>(s, d.off, S, F, v, c, ql, qd * 16, probes);
+                if (ql == 0) used[t] = got;
+                new_tombs += got;
+                need |= got < c ? 1u : 0u;
             }
-            for (int o = 32; o >= 1; o >>= 1) {
-                new_tombs += __shfl_xor(new_tombs, o, kWave);
-                probes += __shfl_xor(probes, o, kWave);
-            }
-            read += probes;
-            removed += new_tombs;
-            if (__ballot(need != 0) && S < ext) {
-                const DelResult r = wave_delete_range(s, d.off, S, ext, 0, keys, gs, ge, used, F);
-                read += ext - S;
-                ext = r.end;
-                removed += r.removed;
-                moved += r.moved;
-            }
-        } else {
-            const DelResult r = wave_delete_range(s, d.off, 0, ext, S, keys, gs, ge, used, F);
-            read += ext;
-            ext = r.end;
-            removed = r.removed;
-            moved = r.moved;
-            S2 = S - r.removed_below;
-            full = true;
         }
-        uint32_t tombs = row_tombs(d) + new_tombs;
-        if (tombs && (uint64_t)tombs * 4 > ext && ext <= kBigDelete) {   // larger rows: k_del_big compacts
-            __threadfence_block();   // the lanes' tombstone stores complete before the pass reads them
-            const DelResult c = wave_compact_tombs(s, d.off, ext, S2, F);
-            read += ext;
-            ext = c.end;
-            S2 = c.removed_below;
-            moved += c.moved;
-            tombs = 0;
-            full = true;
-        }
-        if (lane == 0) {
-            if (ext != d.ext || tombs != row_tombs(d)) {
-                desc[row].ext = ext;
-                desc[row].meta = make_meta(d.meta & 63u, tombs);
-            }
-            if (S2 != S || S != sorted[row]) sorted[row] = S2;
-            if (full) mark_read(s, hot, row);   // a pass read the whole row (cache-policy signal)
-            missing += (ge - gs) - removed;
-            changed += removed ? 1 : 0;
-            rd += read;
-            wr += moved + new_tombs;
+        const uint32_t tombs_a = row_tombs(d) + new_tombs;
+        const bool compact_a = tombs_a && (uint64_t)tombs_a * 4 > d.ext && d.ext <= kBigDelete;
+        const bool done = qsearch && !(need && S < d.ext) && !compact_a;
+        if (done && ql == 0) {
+            if (new_tombs) desc[row].meta = make_meta(d.meta & 63u, tombs_a);
+            missing += (ge - gs) - new_tombs;
+            changed += new_tombs ? 1 : 0;
+            rd += probes;
+            wr += new_tombs;
             rows++;
         }
+        // ---- phase B: the whole wave finishes the remaining groups one by one
+        uint64_t rest = __ballot(valid && !big && !done && ql == 0);
+        while (rest) {
+            const int src = __ffsll((unsigned long long)rest) - 1;
+            rest &= rest - 1;
+            const uint64_t bgs = __shfl(gs, src, kWave), bge = __shfl(ge, src, kWave);
+            const uint32_t brow = __shfl(row, src, kWave), bS = __shfl(S, src, kWave);
+            const RowDesc bd{__shfl(d.off, src, kWave), __shfl(d.ext, src, kWave), __shfl(d.meta, src, kWave)};
+            const bool searched = __shfl(qsearch ? 1 : 0, src, kWave) != 0;
+            uint32_t ext = bd.ext, S2 = bS, removed = 0, moved = 0, read = 0;
+            uint32_t bt = searched ? __shfl(new_tombs, src, kWave) : 0u;
+            uint32_t bneed = searched ? __shfl(need, src, kWave) : 0u;
+            uint32_t bprobes = searched ? __shfl(probes, src, kWave) : 0u;
+            bool full = false;
+            const Fences F = fences_of(fbase, fence, brow);
+            if (bS >= kWaveSearch) {
+                if (!searched) {
+                    uint32_t p = 0, nd = 0, nt = 0;
+                    const uint32_t v0 = (uint32_t)keys[bgs];
+                    if ((uint32_t)keys[bge - 1] == v0) {
+                        // one run (the group's targets are sorted): the whole wave searches it
+                        const uint32_t c = (uint32_t)(bge - bgs);
+                        const uint32_t got = fenced_take<64>(s, bd.off, bS, F, v0, c, lane, 0, p);
+                        if (lane == 0) {
+                            used[bgs] = got;
+                            nt = got;
+                            nd = got < c ? 1u : 0u;
+                        }
+                    } else {
+                        for (uint64_t q0 = bgs; q0 < bge; q0 += kWave)
+                            if (q0 + lane < bge)
+                                nt += search_prefix(s, bd.off, bS, F, keys, q0 + lane, bgs, bge, used, p, nd);
+                    }
+                    for (int o = 32; o >= 1; o >>= 1) {
+                        nt += __shfl_xor(nt, o, kWave);
+                        p += __shfl_xor(p, o, kWave);
+                    }
+                    bt = nt;
+                    bprobes = p;
+                    bneed = __ballot(nd != 0) ? 1u : 0u;
+                }
+                read += bprobes;
+                removed += bt;
+                if (bneed && bS < ext) {
+                    __threadfence_block();   // the quarter's tombstone stores complete before the pass reads
+                    const DelResult r = wave_delete_range(s, bd.off, bS, ext, 0, keys, bgs, bge, used, F);
+                    read += ext - bS;
+                    ext = r.end;
+                    removed += r.removed;
+                    moved += r.moved;
+                }
+            } else {
+                const DelResult r = wave_delete_range(s, bd.off, 0, ext, bS, keys, bgs, bge, used, F);
+                read += ext;
+                ext = r.end;
+                removed = r.removed;
+                moved = r.moved;
+                S2 = bS - r.removed_below;
+                full = true;
+            }
+            uint32_t tombs = row_tombs(bd) + bt;
+            if (tombs && (uint64_t)tombs * 4 > ext && ext <= kBigDelete) {   // larger rows: k_del_big compacts
+                __threadfence_block();   // the tombstone stores complete before the pass reads them
+                const DelResult c = wave_compact_tombs(s, bd.off, ext, S2, F);
+                read += ext;
+                ext = c.end;
+                S2 = c.removed_below;
+                moved += c.moved;
+                tombs = 0;
+                full = true;
+            }
+            if (lane == 0) {
+                if (ext != bd.ext || tombs != row_tombs(bd)) {
+                    desc[brow].ext = ext;
+                    desc[brow].meta = make_meta(bd.meta & 63u, tombs);
+                }
+                if (S2 != bS || bS != sorted[brow]) sorted[brow] = S2;
+                if (full) mark_read(s, hot, brow);   // a pass read the whole row (cache-policy signal)
+                missing += (bge - bgs) - removed;
+                changed += removed ? 1 : 0;
+                rd += read;
+                wr += moved + bt;
+                rows++;
+            }
+        }
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        missing += __shfl_xor(missing, o, kWave);
+        changed += __shfl_xor(changed, o, kWave);
+        rd += __shfl_xor(rd, o, kWave);
+        wr += __shfl_xor(wr, o, kWave);
+        rows += __shfl_xor(rows, o, kWave);
     }
     if (lane == 0) {
         if (count_stats && missing) atomicAdd(&counters[C_MISSING], missing);
