@@ -1104,13 +1104,18 @@ __global__ void k_alloc_classes(uint64_t *tail, uint64_t total) {
 
 // Free-pool segments of the class-sorted pool keys (taken entries, ~0, sort last).
 __global__ void k_pool_segments(const uint64_t *pool, uint64_t len, uint64_t *tail) {
+    // pool[0, len) sorted by class: each class segment's first entry records
+    // its start, its last entry its end (k_pool_counts turns ends into counts)
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < len; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t k = pool[i];
-        if (k == ~0ull) continue;
-        const uint32_t c = (uint32_t)(k >> 40);
+        const uint32_t c = (uint32_t)(pool[i] >> 40);
         if (i == 0 || (uint32_t)(pool[i - 1] >> 40) != c) tail[T_PSTART + c] = i;
-        atomicAdd((unsigned long long *)&tail[T_FREE + c], 1ull);
+        if (i + 1 == len || (uint32_t)(pool[i + 1] >> 40) != c) tail[T_FREE + c] = i + 1;
     }
+}
+
+__global__ void k_pool_counts(uint64_t *tail) {
+    const uint32_t c = threadIdx.x;
+    if (c < kClasses && tail[T_FREE + c]) tail[T_FREE + c] -= tail[T_PSTART + c];
 }
 
 // New blocks of the rows that relocate: one pass per 256 groups reserves ranks
@@ -1384,6 +1389,7 @@ void Engine::mutate_add(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
         DCG_HIP(hipcub::DeviceRadixSort::SortKeys(pt, pb, d_pool.p, d_k64c.p, (int)pool_len, 0, 64, stream));
         hipLaunchKernelGGL(k_pool_segments, dim3(grid_for(pool_live, 256, 4096)), dim3(256), 0, stream, d_k64c.p,
                            pool_live, d_tail.p);
+        hipLaunchKernelGGL(k_pool_counts, dim3(1), dim3(64), 0, stream, d_tail.p);
     }
     if (d_pool.n < pool_live + 2 * count + 1) {
         DevBuf<uint64_t> grown;

@@ -977,16 +977,16 @@ __global__ void k_reset_affected(uint64_t *st, const uint32_t *aff, const Ctl *c
 // After a closure every CC vertex it touched (tick >= stamp_lo, or on the
 // affected list) has its parent checked: kept if tight (equal label, depth - 1),
 // else replaced by the first tight in-neighbour; depth 0 is a root (none).
-__global__ void k_cc_fix_parents(TravArgs a, const uint32_t *list, const Ctl *list_ctl) {
+__global__ void k_cc_fix_parents(TravArgs a, const uint32_t *list, const Ctl *list_ctl, uint32_t base) {
     const uint32_t stamp_lo = stamp_base(a);
-    const uint64_t total = list ? (uint64_t)list_ctl->aff_count : (uint64_t)a.n * a.Q;
+    const uint64_t total = list ? (uint64_t)list_ctl->aff_count : (uint64_t)a.n;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
          i += (uint64_t)gridDim.x * blockDim.x) {
         uint32_t g;
         if (list) {
             g = list[i];
         } else {
-            g = (uint32_t)i;
+            g = base + (uint32_t)i;
             if (a.tick[g] < stamp_lo) continue;
         }
         uint32_t q, v;
@@ -1054,10 +1054,20 @@ static void check_launch(const Ctl &c, const char *stage) {
 }
 
 // `a`'s stamp base (stamp0, or its step slot) is the first stamp of the phase.
+// With a list: the listed vertex-query slots.  Without: the slots of each CC
+// query stamped in this stage (one pass over that query's n vertices).
 static void cc_fix_parents(Engine &e, const TravArgs &a, const uint32_t *list, const Ctl *lc) {
     if (!e.d_par.p) return;
-    const uint64_t work = (uint64_t)e.n * e.Q;
-    hipLaunchKernelGGL(k_cc_fix_parents, dim3(grid_for(work, 256, 16384)), dim3(256), 0, e.stream, a, list, lc);
+    if (list) {
+        const uint64_t work = (uint64_t)e.n * e.Q;
+        hipLaunchKernelGGL(k_cc_fix_parents, dim3(grid_for(work, 256, 16384)), dim3(256), 0, e.stream, a, list, lc,
+                           0u);
+        return;
+    }
+    for (uint32_t q = 0; q < e.Q; ++q)
+        if (e.qapps[q] == DCG_APP_CC)
+            hipLaunchKernelGGL(k_cc_fix_parents, dim3(grid_for(e.n, 256, 16384)), dim3(256), 0, e.stream, a, list, lc,
+                               q * e.n);
 }
 
 uint32_t Engine::coop_blocks(const void *kernel) {
