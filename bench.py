@@ -42,6 +42,17 @@ KERNEL_SYMBOL = {"push": "k_persistent<0u>", "pull": "k_persistent<1u>", "invali
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic.json")
 
 
+def host_link(k, host_lps):
+    """The dominant kernel against the host link's random-read ceiling: 64-byte lines
+    it read from pinned host memory per launch, their rate, and the measured ceiling."""
+    if not k.get("host_lines") or not k["ms"]:
+        return None
+    rate = k["host_lines"] / (k["ms"] / 1e3)
+    return {"bound": "random 64-byte reads of pinned host memory", "lines_per_launch":
+            round(k["host_lines"] / max(k["launches"], 1)), "achieved_lines_per_s": round(rate),
+            "peak_lines_per_s": round(host_lps), "frac": round(rate / host_lps, 4) if host_lps else None}
+
+
 def measured_traffic(config, kernel):
     try:
         with open(TRAFFIC_FILE) as f:
@@ -282,7 +293,9 @@ def main():
     n, src, dst, adds, dels, ap, dp = make_inputs(cfg, snapshots)
     import dycgraph_amd as d
     hbm_gbs, host_gbs = d.calibrate()
-    log(f"calibration: HBM copy {hbm_gbs:.0f} GB/s, pinned-host zero-copy read {host_gbs:.1f} GB/s")
+    host_lps, hbm_lps = d.calibrate_lines()
+    log(f"calibration: HBM copy {hbm_gbs:.0f} GB/s, pinned-host zero-copy read {host_gbs:.1f} GB/s, "
+        f"random 64-byte lines: host {host_lps / 1e6:.0f} M/s, HBM {hbm_lps / 1e6:.0f} M/s")
     res = run_gpu(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, cfg["policy"])
     upd = int(ap[warmup:].sum() + dp[warmup:].sum())
     value = upd / res["total_s"]
@@ -305,13 +318,15 @@ def main():
         "p50_ms": round(float(np.percentile(lat, 50)), 3), "p90_ms": round(float(np.percentile(lat, 90)), 3),
         "cache_hit_rate": round(res["hits"] / reads, 4) if reads else None,
         "host_link_edge_bytes": int(res["host"] * 4),
-        "calibration": {"hbm_copy_GBps": round(hbm_gbs, 1), "host_zero_copy_read_GBps": round(host_gbs, 2)},
+        "calibration": {"hbm_copy_GBps": round(hbm_gbs, 1), "host_zero_copy_read_GBps": round(host_gbs, 2),
+                        "host_random_lines_per_s": round(host_lps), "hbm_random_lines_per_s": round(hbm_lps)},
         "roofline": {"kernel": kname, "hip_kernel": KERNEL_SYMBOL.get(kname, kname), "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(args.config, kname),
                      "launches": k["launches"], "avg_launch_ms": round(k["ms"] / max(k["launches"], 1), 4),
-                     "algo_bytes_per_launch": round(k["algo_bytes"] / max(k["launches"], 1), 1)},
+                     "algo_bytes_per_launch": round(k["algo_bytes"] / max(k["launches"], 1), 1),
+                     "host_link": host_link(k, host_lps)},
         "per_snapshot": res["closure"],
         "kernels": {name: {"ms": round(v["ms"], 3), "launches": v["launches"],
                            "GBps": round(v["algo_bytes"] / (v["ms"] / 1e3) / 1e9, 1) if v["ms"] else 0.0}

@@ -1,5 +1,6 @@
-// calibrate.hip -- link microbenchmarks: HBM copy rate and zero-copy read rate
-// of pinned host memory (the host link the uncached chunks are read over).
+// calibrate.hip -- link microbenchmarks: HBM copy rate, zero-copy read rate
+// of pinned host memory (the host link the uncached chunks are read over) and
+// random 64-byte line read rates of both.
 // Also the known-size copy kernel used to scale rocprof FETCH_SIZE/WRITE_SIZE.
 #include "engine.hpp"
 
@@ -20,7 +21,67 @@ __global__ __launch_bounds__(256) void k_stream_read(const uint4 *src, uint64_t 
     if (acc == 0x12345678u) atomicAdd(sink, 1ull);
 }
 
+// Each quarter-wave reads `reads` random 64-byte lines of buf[0, lines * 16).
+__global__ __launch_bounds__(256) void k_random_lines(const uint32_t *buf, uint64_t lines, int reads,
+                                                      unsigned long long *sink) {
+    const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    uint32_t acc = 0;
+    for (int r = 0; r < reads; ++r) {
+        const uint64_t line = mix64((t >> 4) * 1000003ull + (uint64_t)r) % lines;
+        acc ^= buf[line * 16 + (t & 15)];
+    }
+    if (acc == 0x12345678u) atomicAdd(sink, 1ull);
+}
+
 }  // namespace dcg
+
+extern "C" int dcg_calibrate_lines(int32_t device, uint64_t bytes, double *host_lines_per_s, double *hbm_lines_per_s) {
+    using namespace dcg;
+    if (!host_lines_per_s || !hbm_lines_per_s || bytes < (1u << 20)) return DCG_ERR_ARG;
+    try {
+        DCG_HIP(hipSetDevice(device));
+        hipStream_t s;
+        DCG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        hipDeviceProp_t prop;
+        DCG_HIP(hipGetDeviceProperties(&prop, device));
+        const uint64_t words = bytes / 64 * 16, lines = words / 16;
+        const uint32_t grid = (uint32_t)prop.multiProcessorCount * 16;
+        const int reads = 16;
+        DevBuf<uint32_t> dev;
+        DevBuf<unsigned long long> sink;
+        dev.alloc(words);
+        sink.alloc(1);
+        fill<uint32_t>(s, dev.p, words, 1u);
+        fill<unsigned long long>(s, sink.p, 1, 0ull);
+        uint32_t *h = nullptr;
+        DCG_HIP(hipHostMalloc((void **)&h, words * 4, hipHostMallocDefault));
+        std::memset(h, 1, words * 4);
+        hipEvent_t e0, e1;
+        DCG_HIP(hipEventCreate(&e0));
+        DCG_HIP(hipEventCreate(&e1));
+        const double total = (double)grid * 256 / 16 * reads;
+        for (int mem = 0; mem < 2; ++mem) {
+            float best = 1e30f;
+            for (int it = 0; it < 4; ++it) {
+                DCG_HIP(hipEventRecord(e0, s));
+                hipLaunchKernelGGL(k_random_lines, dim3(grid), dim3(256), 0, s, mem ? dev.p : h, lines, reads, sink.p);
+                DCG_HIP(hipEventRecord(e1, s));
+                DCG_HIP(hipEventSynchronize(e1));
+                float ms = 0;
+                DCG_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (it > 0 && ms < best) best = ms;
+            }
+            (mem ? *hbm_lines_per_s : *host_lines_per_s) = total / (best * 1e-3);
+        }
+        DCG_HIP(hipHostFree(h));
+        DCG_HIP(hipEventDestroy(e0));
+        DCG_HIP(hipEventDestroy(e1));
+        DCG_HIP(hipStreamDestroy(s));
+    } catch (const Error &e) {
+        return e.status;
+    }
+    return DCG_OK;
+}
 
 extern "C" int dcg_calibrate(int32_t device, uint64_t bytes, double *hbm_copy_gbs, double *host_read_gbs) {
     using namespace dcg;

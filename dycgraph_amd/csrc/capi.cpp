@@ -25,7 +25,7 @@ Engine::Engine(const dcg_config &c) : cfg(c) {
     DCG_HIP(hipHostMalloc((void **)&h_ctl, 8 * sizeof(Ctl), hipHostMallocDefault));
     DCG_HIP(hipHostMalloc((void **)&h_step, 2 * sizeof(StepArgs), hipHostMallocDefault));
     d_step.alloc(2);
-    DCG_HIP(hipHostMalloc((void **)&h_counters, 16 * sizeof(unsigned long long), hipHostMallocDefault));
+    DCG_HIP(hipHostMalloc((void **)&h_counters, Engine::kCounterSlots * sizeof(unsigned long long), hipHostMallocDefault));
     app = c.app;
     const bool mixed = c.query_apps && app != DCG_APP_PR;
     Q = app == DCG_APP_PR || (app == DCG_APP_CC && !mixed) ? 1u : std::max(1u, c.num_queries);
@@ -81,6 +81,7 @@ void Engine::collect_timing() {
             k.algo_bytes += 24.0 * (double)c.expansions + 12.0 * (double)c.edges;
         } else {
             k.algo_bytes += t.bytes;
+            k.host_lines += t.host_lines;
         }
     }
     timed.clear();

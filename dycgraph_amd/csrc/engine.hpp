@@ -163,6 +163,7 @@ struct Engine {
     DevBuf<uint32_t> d_big_groups;     // large-row mutation paths
     DevBuf<uint8_t> d_flags;           // group-head flags of a sorted key array
     DevBuf<unsigned long long> d_counters;  // phase counters (see store.hip)
+    static constexpr int kCounterSlots = 32;   // pinned mirror of d_counters
     unsigned long long *h_counters = nullptr;
 
     // cumulative counters
@@ -174,6 +175,7 @@ struct Engine {
         int ev;
         int ctl;       // Ctl block index whose counters price the launch (-1: mutation)
         double bytes;  // preset bytes (mutation kernels)
+        uint64_t host_lines = 0;
     };
     dcg_kernel_stats kstat[DCG_K_NUM] = {};
     std::vector<hipEvent_t> events;   // pairs (start, end)

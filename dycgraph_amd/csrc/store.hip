@@ -34,8 +34,10 @@ enum Counter : int {
     C_W_WRITTEN = 13,
     C_W_ROWS = 14,
     C_B_ROWS = 15,       // workgroup-path delete rows
-    C_NUM = 16
+    C_W_HOST_LINES = 16, // wave-path delete: 64-byte lines read from pinned host memory
+    C_NUM = 17
 };
+static_assert(C_NUM <= Engine::kCounterSlots, "pinned counter mirror too small");
 
 // d_tail layout
 // Store tail block (u64 words): bump pointer, overflow flag, and per capacity
@@ -438,6 +440,14 @@ __device__ __forceinline__ uint64_t group_end_of(const K *keys, const uint32_t *
 }
 
 __device__ __forceinline__ uint32_t load_slot(const StoreView &s, uint64_t slot) { return *row_base(s, slot); }
+// Whether a slot is read from pinned host memory (its chunk is not resident).
+__device__ __forceinline__ bool host_slot(const StoreView &s, uint64_t slot) {
+    return s.host && s.chunk_frame[(uint32_t)(slot >> kChunkLog2)] < 0;
+}
+// 64-byte lines of slots [from, to) of a row read from host memory.
+__device__ __forceinline__ uint32_t host_lines(const StoreView &s, uint64_t off, uint32_t from, uint32_t to) {
+    return to > from && host_slot(s, off + from) ? (to - from + 15) / 16 : 0u;
+}
 
 constexpr uint32_t kBigDelete = 1024;   // rows above this use one 1024-thread workgroup
 
@@ -561,11 +571,11 @@ __device__ DelResult wave_compact_tombs(const StoreView &s, uint64_t off, uint32
 //     row) and tombstone the first live occurrences of v in it (bit 31 set, id
 //     bits kept, so the prefix stays ordered and its fences valid); while more
 //     are wanted and the window's last slot does not pass v, the next window.
-// Returns the occurrences taken; `probes` (kept by subgroup lane 0) counts the
-// row slots read.
+// Returns the occurrences taken; `probes` and `lines` (kept by subgroup lane 0)
+// count the row slots read and the windows read from host memory.
 template <uint32_t W>
 __device__ uint32_t fenced_take(const StoreView &s, uint64_t off, uint32_t S, const Fences &F, uint32_t v,
-                                uint32_t c, uint32_t q, uint32_t shift, uint32_t &probes) {
+                                uint32_t c, uint32_t q, uint32_t shift, uint32_t &probes, uint32_t &lines) {
     static_assert(W == 16 || W == 64, "subgroup of 16 or 64 lanes");
     const uint64_t below = (1ull << q) - 1;
     const uint32_t nfc = (S + kFenceStride - 1) / kFenceStride;
@@ -602,7 +612,10 @@ __device__ uint32_t fenced_take(const StoreView &s, uint64_t off, uint32_t S, co
         if (live && (uint32_t)__popcll(lm & below) < c - taken) store_slot(s, off + i, x | kTomb);
         const uint32_t nlive = (uint32_t)__popcll(lm);
         taken += nlive < c - taken ? nlive : c - taken;
-        if (q == 0) probes += S - pos < kFenceStride ? S - pos : kFenceStride;
+        if (q == 0) {
+            probes += S - pos < kFenceStride ? S - pos : kFenceStride;
+            lines += host_slot(s, off + pos) ? 1u : 0u;
+        }
         // the run may go on past the window when its last slot is <= v
         uint64_t em = __ballot(valid && q == kFenceStride - 1 && id <= v);
         if (W != 64) em = (em >> shift) & 0xFFFFull;
@@ -640,10 +653,12 @@ __device__ __forceinline__ uint32_t fenced_lb(const StoreView &s, uint64_t off, 
 // run took; need_tail tells whether a run wants more.
 __device__ __forceinline__ uint32_t search_prefix(const StoreView &s, uint64_t off, uint32_t S, const Fences &F,
                                                   const uint64_t *keys, uint64_t q, uint64_t gs, uint64_t ge,
-                                                  uint32_t *used, uint32_t &probes, uint32_t &need_tail) {
+                                                  uint32_t *used, uint32_t &probes, uint32_t &need_tail,
+                                                  uint32_t &lines) {
     const uint32_t v = (uint32_t)keys[q];
     if (q > gs && (uint32_t)keys[q - 1] == v) return 0;
     const uint32_t c = (uint32_t)(target_ub(keys, q + 1, ge, v) - q);
+    lines += host_slot(s, off) ? 1u : 0u;   // the window the fences leave open
     uint32_t taken = 0;
     for (uint32_t j = fenced_lb(s, off, S, F, v, probes); j < S && taken < c; ++j) {
         const uint32_t x = load_slot(s, off + j);
@@ -702,7 +717,7 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
     const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kWave;
     const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / kWave;
     const uint64_t ngroups = *ngroups_p;
-    unsigned long long missing = 0, changed = 0, rd = 0, wr = 0, rows = 0;
+    unsigned long long missing = 0, changed = 0, rd = 0, wr = 0, rows = 0, hl = 0;
     for (uint64_t base = wave * 4; base < ngroups; base += nwaves * 4) {
         // ---- phase A: one group per quarter-wave (values uniform in the quarter)
         const uint64_t g = base + qd;
@@ -720,14 +735,14 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
         const bool big = valid && d.ext > kBigDelete && !wave_searchable(d, S, ge - gs);
         if (big && ql == 0) big_groups[atomicAdd(nbig, 1ull)] = (uint32_t)gs;
         const bool qsearch = valid && !big && S >= kWaveSearch && ge - gs <= kQuarterRuns;
-        uint32_t new_tombs = 0, probes = 0, need = 0;
+        uint32_t new_tombs = 0, probes = 0, need = 0, qlines = 0;
         if (qsearch) {
             const Fences F = fences_of(fbase, fence, row);
             for (uint64_t t = gs; t < ge; ++t) {
                 const uint32_t v = (uint32_t)keys[t];
                 if (t > gs && (uint32_t)keys[t - 1] == v) continue;
                 const uint32_t c = (uint32_t)(target_ub(keys, t + 1, ge, v) - t);
-                const uint32_t got = fenced_take<16>(s, d.off, S, F, v, c, ql, qd * 16, probes);
+                const uint32_t got = fenced_take<16>(s, d.off, S, F, v, c, ql, qd * 16, probes, qlines);
                 if (ql == 0) used[t] = got;
                 new_tombs += got;
                 need |= got < c ? 1u : 0u;
@@ -736,6 +751,7 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
         const uint32_t tombs_a = row_tombs(d) + new_tombs;
         const bool compact_a = tombs_a && (uint64_t)tombs_a * 4 > d.ext && d.ext <= kBigDelete;
         const bool done = qsearch && !(need && S < d.ext) && !compact_a;
+        if (ql == 0) hl += qlines;   // windows read in phase A, whoever finishes the group
         if (done && ql == 0) {
             if (new_tombs) desc[row].meta = make_meta(d.meta & 63u, tombs_a);
             missing += (ge - gs) - new_tombs;
@@ -761,12 +777,12 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
             const Fences F = fences_of(fbase, fence, brow);
             if (bS >= kWaveSearch) {
                 if (!searched) {
-                    uint32_t p = 0, nd = 0, nt = 0;
+                    uint32_t p = 0, nd = 0, nt = 0, ln = 0;
                     const uint32_t v0 = (uint32_t)keys[bgs];
                     if ((uint32_t)keys[bge - 1] == v0) {
                         // one run (the group's targets are sorted): the whole wave searches it
                         const uint32_t c = (uint32_t)(bge - bgs);
-                        const uint32_t got = fenced_take<64>(s, bd.off, bS, F, v0, c, lane, 0, p);
+                        const uint32_t got = fenced_take<64>(s, bd.off, bS, F, v0, c, lane, 0, p, ln);
                         if (lane == 0) {
                             used[bgs] = got;
                             nt = got;
@@ -775,12 +791,13 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
                     } else {
                         for (uint64_t q0 = bgs; q0 < bge; q0 += kWave)
                             if (q0 + lane < bge)
-                                nt += search_prefix(s, bd.off, bS, F, keys, q0 + lane, bgs, bge, used, p, nd);
+                                nt += search_prefix(s, bd.off, bS, F, keys, q0 + lane, bgs, bge, used, p, nd, ln);
                     }
                     for (int o = 32; o >= 1; o >>= 1) {
                         nt += __shfl_xor(nt, o, kWave);
                         p += __shfl_xor(p, o, kWave);
                     }
+                    hl += ln;
                     bt = nt;
                     bprobes = p;
                     bneed = __ballot(nd != 0) ? 1u : 0u;
@@ -790,6 +807,7 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
                 if (bneed && bS < ext) {
                     __threadfence_block();   // the quarter's tombstone stores complete before the pass reads
                     const DelResult r = wave_delete_range(s, bd.off, bS, ext, 0, keys, bgs, bge, used, F);
+                    if (lane == 0) hl += host_lines(s, bd.off, bS, ext);
                     read += ext - bS;
                     ext = r.end;
                     removed += r.removed;
@@ -797,6 +815,7 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
                 }
             } else {
                 const DelResult r = wave_delete_range(s, bd.off, 0, ext, bS, keys, bgs, bge, used, F);
+                if (lane == 0) hl += host_lines(s, bd.off, 0, ext);
                 read += ext;
                 ext = r.end;
                 removed = r.removed;
@@ -808,6 +827,7 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
             if (tombs && (uint64_t)tombs * 4 > ext && ext <= kBigDelete) {   // larger rows: k_del_big compacts
                 __threadfence_block();   // the tombstone stores complete before the pass reads them
                 const DelResult c = wave_compact_tombs(s, bd.off, ext, S2, F);
+                if (lane == 0) hl += host_lines(s, bd.off, 0, ext);
                 read += ext;
                 ext = c.end;
                 S2 = c.removed_below;
@@ -836,8 +856,10 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
         rd += __shfl_xor(rd, o, kWave);
         wr += __shfl_xor(wr, o, kWave);
         rows += __shfl_xor(rows, o, kWave);
+        hl += __shfl_xor(hl, o, kWave);
     }
     if (lane == 0) {
+        if (hl) atomicAdd(&counters[C_W_HOST_LINES], hl);
         if (count_stats && missing) atomicAdd(&counters[C_MISSING], missing);
         if (count_stats && changed) atomicAdd(&counters[C_CHANGED], changed);
         if (rd) atomicAdd(&counters[C_W_READ], rd);
@@ -1337,6 +1359,7 @@ void Engine::mutate_delete(uint64_t begin, uint64_t count, dcg_phase_stats *st) 
     sync();
     timed[w0].bytes = 4.0 * (double)(h_counters[C_W_READ] + h_counters[C_W_WRITTEN]) + 16.0 * (double)h_counters[C_W_ROWS];
     timed[w1].bytes = 0.0;
+    timed[w0].host_lines = h_counters[C_W_HOST_LINES];
     timed[b0].bytes = 4.0 * (double)(h_counters[C_SLOTS_READ] + h_counters[C_SLOTS_WRITTEN]) +
                       16.0 * (double)h_counters[C_B_ROWS];
     timed[b1].bytes = 0.0;

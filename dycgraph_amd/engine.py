@@ -70,7 +70,7 @@ class CacheStats(C.Structure):
 
 class KernelStats(C.Structure):
     _fields_ = [("launches", C.c_uint64), ("ms", C.c_double), ("algo_bytes", C.c_double),
-                ("edges", C.c_uint64), ("expansions", C.c_uint64)]
+                ("edges", C.c_uint64), ("expansions", C.c_uint64), ("host_lines", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -144,6 +144,7 @@ def lib() -> C.CDLL:
         "dcg_gen_rmat_gpu": (I, [C.c_int32, U32, U64, C.c_double, C.c_double, C.c_double, U64, P, P]),
         "dcg_gen_stream": (I, [U32, P, P, U64, U32, P, P, U64, P, P]),
         "dcg_calibrate": (I, [C.c_int32, U64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "dcg_calibrate_lines": (I, [C.c_int32, U64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "dcg_gen_grid": (I, [U32, C.c_double, U64, P, P, C.POINTER(U64)]),
         "dcg_gen_lognormal_sizes": (I, [U32, C.c_double, C.c_double, U64, U64, U64, P]),
     }
@@ -188,6 +189,15 @@ def calibrate(nbytes=1 << 30, device=0):
     if st:
         raise DcgError(st, "dcg_calibrate failed")
     return h.value, z.value
+
+
+def calibrate_lines(nbytes=1 << 30, device=0):
+    """(pinned-host, HBM) random 64-byte line reads per second measured on the GPU."""
+    h, d = C.c_double(), C.c_double()
+    st = lib().dcg_calibrate_lines(device, nbytes, C.byref(h), C.byref(d))
+    if st:
+        raise DcgError(st, "dcg_calibrate_lines failed")
+    return h.value, d.value
 
 
 def gen_grid(side, diag_frac=0.05, seed=1):

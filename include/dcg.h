@@ -210,6 +210,7 @@ typedef struct {
     double   ms;
     double   algo_bytes;
     uint64_t edges, expansions;
+    uint64_t host_lines;   /* delete wave path: 64-byte lines read from pinned host memory */
 } dcg_kernel_stats;
 int dcg_get_kernel_stats(dcg_engine *eng, uint32_t kernel, dcg_kernel_stats *out);
 int dcg_reset_kernel_stats(dcg_engine *eng);
@@ -256,6 +257,11 @@ int dcg_set_graphs(dcg_engine *eng, int on);
 /* Link microbenchmarks on `device`: HBM copy rate (read+write bytes / time of a
  * `bytes`-sized copy kernel) and zero-copy read rate of pinned host memory. */
 int dcg_calibrate(int32_t device, uint64_t bytes, double *hbm_copy_gbs, double *host_read_gbs);
+/* Random 64-byte line reads per second (16 lanes read one line, as the delete
+ * search reads a prefix window) over `bytes` of pinned host memory and of HBM:
+ * the host link's random-read ceiling that bounds the delete mutation of a
+ * host-placed store.  No reference counterpart (measurement utility). */
+int dcg_calibrate_lines(int32_t device, uint64_t bytes, double *host_lines_per_s, double *hbm_lines_per_s);
 
 /* Synchronise the engine's stream (timing boundaries). */
 int dcg_sync(dcg_engine *eng);
