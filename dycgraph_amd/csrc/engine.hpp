@@ -110,7 +110,7 @@ struct Engine {
     DevBuf<uint32_t> d_sorted_out, d_sorted_in;   // per row: length of the sorted prefix
     // Fences of the sorted prefixes, in HBM: row r owns fence[fbase[r] .. fbase[r+1]),
     // fence j holding the id at slot 16j of the row (valid for 16j below its
-    // current prefix length); rows whose prefix started below 64 have none.
+    // current prefix length); rows whose prefix started below 16 have none.
     DevBuf<uint32_t> d_fbase_out, d_fbase_in, d_fence_out, d_fence_in;
     DevBuf<uint64_t> d_pool;           // free blocks (class << 40 | offset) of relocated rows
     uint64_t pool_len = 0, pool_taken = 0, reused_blocks = 0;

@@ -140,7 +140,7 @@ __global__ void k_scatter_rows(const uint32_t *keys, const uint32_t *vals, uint6
 }
 
 constexpr uint32_t kFenceStride = 16;   // prefix slots per fence (one 64-byte line)
-constexpr uint32_t kFenceMin = 64;      // prefixes shorter than this get no fences
+constexpr uint32_t kFenceMin = 16;      // prefixes shorter than this (one line or less) get no fences
 
 // Length of each row's longest non-decreasing prefix (initialised to the degree):
 // the first descent position of a row-sorted edge list.
