@@ -697,6 +697,8 @@ __device__ __forceinline__ bool wave_searchable(const RowDesc &d, uint32_t S, ui
 //    S >= kWaveSearch is searched through the row's fences (fenced_take, run
 //    by run).  When every run was satisfied and the tombstones stay below a
 //    quarter of the row, the quarter finishes the group (row descriptor, stats).
+//    A single target in a row of at most 16 slots without such a prefix is
+//    removed by the quarter from one read of the row.
 //  * phase B (whole wave, one remaining group at a time): for a searched
 //    group that still wants occurrences, wave_delete_range over the tail
 //    [S, ext), which holds appended entries and never tombstones (the prefix
@@ -723,14 +725,15 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
         const uint64_t g = base + qd;
         const bool valid = g < ngroups;
         uint64_t gs = 0, ge = 0;
-        uint32_t row = 0, S = 0;
+        uint32_t row = 0, S = 0, Sraw = 0;
         RowDesc d{0, 0, 0};
         if (valid) {
             gs = gstart[g];
             row = (uint32_t)(keys[gs] >> 32);
             d = desc[row];
             ge = group_end_of<uint64_t>(keys, gstart, g, ngroups, gs, count, row);
-            S = sorted[row] < d.ext ? sorted[row] : d.ext;
+            Sraw = sorted[row];
+            S = Sraw < d.ext ? Sraw : d.ext;
         }
         const bool big = valid && d.ext > kBigDelete && !wave_searchable(d, S, ge - gs);
         if (big && ql == 0) big_groups[atomicAdd(nbig, 1ull)] = (uint32_t)gs;
@@ -748,16 +751,50 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
                 need |= got < c ? 1u : 0u;
             }
         }
+        // one target in a row of at most 16 slots without a searchable prefix:
+        // the quarter reads the row (one line) and removes the first live
+        // occurrence, the entries after it moving down one slot (order kept)
+        const bool qtiny = valid && !big && !qsearch && ge - gs == 1 && d.ext <= kFenceStride && d.ext &&
+                           (uint64_t)row_tombs(d) * 4 <= (uint64_t)d.ext - 1;
+        uint32_t t_removed = 0, t_moved = 0, t_S2 = S;
+        if (qtiny) {
+            const uint32_t v = (uint32_t)keys[gs];
+            const Fences F = fences_of(fbase, fence, row);
+            const bool in = ql < d.ext;
+            const uint32_t x = in ? load_slot(s, d.off + ql) : 0u;
+            const uint64_t mm = (__ballot(in && x == v) >> (qd * 16)) & 0xFFFFull;
+            if (mm) {
+                const uint32_t first = (uint32_t)__ffsll((unsigned long long)mm) - 1;
+                if (in && ql > first) {
+                    store_slot(s, d.off + ql - 1, x);
+                    if (ql < S) fence_put(F, ql - 1, x);
+                }
+                t_removed = 1;
+                t_moved = d.ext - 1 - first;
+                t_S2 = first < S ? S - 1 : S;
+            }
+            qlines += host_lines(s, d.off, 0, d.ext);
+        }
         const uint32_t tombs_a = row_tombs(d) + new_tombs;
         const bool compact_a = tombs_a && (uint64_t)tombs_a * 4 > d.ext && d.ext <= kBigDelete;
-        const bool done = qsearch && !(need && S < d.ext) && !compact_a;
-        if (ql == 0) hl += qlines;   // windows read in phase A, whoever finishes the group
+        const bool done = (qsearch && !(need && S < d.ext) && !compact_a) || qtiny;
+        if (ql == 0) hl += qlines;   // lines read in phase A, whoever finishes the group
         if (done && ql == 0) {
-            if (new_tombs) desc[row].meta = make_meta(d.meta & 63u, tombs_a);
-            missing += (ge - gs) - new_tombs;
-            changed += new_tombs ? 1 : 0;
-            rd += probes;
-            wr += new_tombs;
+            if (qtiny) {
+                if (t_removed) desc[row].ext = d.ext - 1;
+                if (t_S2 != S || S != Sraw) sorted[row] = t_S2;
+                mark_read(s, hot, row);   // the row was read whole (cache-policy signal)
+                missing += 1 - t_removed;
+                changed += t_removed;
+                rd += d.ext;
+                wr += t_moved;
+            } else {
+                if (new_tombs) desc[row].meta = make_meta(d.meta & 63u, tombs_a);
+                missing += (ge - gs) - new_tombs;
+                changed += new_tombs ? 1 : 0;
+                rd += probes;
+                wr += new_tombs;
+            }
             rows++;
         }
         // ---- phase B: the whole wave finishes the remaining groups one by one
@@ -1202,49 +1239,83 @@ __device__ __forceinline__ uint32_t class_of(uint64_t nd) {
     return 64u - (uint32_t)__clzll((long long)(c - 1));
 }
 
-// One wave per touched row: relocate if needed, then append in request order.
-// Rows above kBigAppend entries that must relocate are listed for k_add_big.
+// Append (and relocation) per touched row.  A wave takes four groups at a
+// time, one per quarter-wave: a quarter appends up to kQuarterAppend entries
+// and copies a relocated row of up to kQuarterCopy slots itself; the whole
+// wave then takes the remaining groups one at a time (larger appends and
+// copies).  Rows above kBigAppend slots that must move are listed for
+// k_add_big.  Relocated entries (tombstones included) keep their positions,
+// so the row's sorted prefix length and fences stay valid.
+constexpr uint32_t kQuarterAppend = 64, kQuarterCopy = 256;
+
 __global__ __launch_bounds__(256) void k_add_rows(StoreView s, RowDesc *desc, uint32_t *hot, const uint32_t *keys,
                                                   const uint32_t *vals, uint64_t count, const uint32_t *gstart,
                                                   const unsigned long long *ngroups_p, uint64_t *tail,
                                                   uint64_t *pool, const uint64_t *reloc, unsigned long long *counters,
                                                   int count_stats, uint32_t *big_groups, unsigned long long *nbig) {
-    const uint32_t lane = lane_id();
+    const uint32_t lane = lane_id(), qd = lane >> 4, ql = lane & 15;
     const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kWave;
     const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / kWave;
     const uint64_t ngroups = *ngroups_p;
     const bool overflow = tail[T_OVERFLOW] != 0;
     unsigned long long relocated = 0, rd = 0, wr = 0;
-    for (uint64_t g = wave; g < ngroups; g += nwaves) {
-        const uint64_t gs = gstart[g];
-        const uint32_t row = keys[gs];
-        const uint64_t ge = group_end_of<uint32_t>(keys, gstart, g, ngroups, gs, count, row);
-        const uint32_t k = (uint32_t)(ge - gs);
-        RowDesc d = desc[row];
+    // one group's append by `width` lanes starting at lane index `li`
+    auto append = [&](uint64_t g, uint64_t gs, uint32_t row, uint32_t k, RowDesc d, uint32_t li, uint32_t width,
+                      bool leader) {
         const uint64_t nd = (uint64_t)d.ext + k;
         if (nd > row_cap(d)) {
-            if (overflow) continue;
-            if (d.ext > kBigAppend) {
-                if (lane == 0) big_groups[atomicAdd(nbig, 1ull)] = (uint32_t)g;
-                continue;
-            }
             const uint32_t cls = class_of(nd);
             const uint64_t noff = reloc[g];
-            // entries (tombstones included) keep their positions, so the
-            // sorted prefix length of the row stays valid
-            for (uint32_t i = lane; i < d.ext; i += kWave) store_slot(s, noff + i, load_slot(s, d.off + i));
-            if (lane == 0 && d.ext) mark_read(s, hot, row);
-            if (lane == 0 && d.meta) retire_block(tail, pool, d);
+            for (uint32_t i = li; i < d.ext; i += width) store_slot(s, noff + i, load_slot(s, d.off + i));
+            if (leader) {
+                if (d.ext) mark_read(s, hot, row);
+                if (d.meta) retire_block(tail, pool, d);
+                relocated++;
+                rd += d.ext;
+                wr += d.ext;
+            }
             d.off = noff;
             d.meta = make_meta(cls, row_tombs(d));
-            relocated++;
-            rd += d.ext;
-            wr += d.ext;
         }
-        for (uint32_t i = lane; i < k; i += kWave) store_slot(s, d.off + d.ext + i, vals[gs + i]);
-        wr += k;
+        for (uint32_t i = li; i < k; i += width) store_slot(s, d.off + d.ext + i, vals[gs + i]);
         d.ext = (uint32_t)nd;
-        if (lane == 0) desc[row] = d;
+        if (leader) {
+            wr += k;
+            desc[row] = d;
+        }
+    };
+    for (uint64_t base = wave * 4; base < ngroups; base += nwaves * 4) {
+        const uint64_t g = base + qd;
+        const bool valid = g < ngroups;
+        uint64_t gs = 0;
+        uint32_t row = 0, k = 0;
+        RowDesc d{0, 0, 0};
+        bool move = false;
+        if (valid) {
+            gs = gstart[g];
+            row = keys[gs];
+            k = (uint32_t)(group_end_of<uint32_t>(keys, gstart, g, ngroups, gs, count, row) - gs);
+            d = desc[row];
+            move = (uint64_t)d.ext + k > row_cap(d);
+        }
+        const bool skip = valid && move && overflow;   // no room: the host reports overflow
+        const bool big = valid && move && !overflow && d.ext > kBigAppend;
+        if (big && ql == 0) big_groups[atomicAdd(nbig, 1ull)] = (uint32_t)g;
+        const bool quarter = valid && !skip && !big && k <= kQuarterAppend && (!move || d.ext <= kQuarterCopy);
+        if (quarter) append(g, gs, row, k, d, ql, 16, ql == 0);
+        uint64_t rest = __ballot(valid && !skip && !big && !quarter && ql == 0);
+        while (rest) {
+            const int src = __ffsll((unsigned long long)rest) - 1;
+            rest &= rest - 1;
+            const RowDesc bd{__shfl(d.off, src, kWave), __shfl(d.ext, src, kWave), __shfl(d.meta, src, kWave)};
+            append(base + (uint32_t)(src >> 4), __shfl(gs, src, kWave), __shfl(row, src, kWave),
+                   __shfl(k, src, kWave), bd, lane, kWave, lane == 0);
+        }
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        relocated += __shfl_xor(relocated, o, kWave);
+        rd += __shfl_xor(rd, o, kWave);
+        wr += __shfl_xor(wr, o, kWave);
     }
     if (lane == 0) {
         if (count_stats && relocated) atomicAdd(&counters[C_RELOCATED], relocated);
