@@ -7,8 +7,9 @@
 // only unmaps the frame.
 //
 // HOTNESS policy (the reference's plain policy, driver.cuh:486-579 and
-// framework.cuh:1264-1567, at chunk granularity): score = edges read over the
-// last four snapshots, ties by most recent use (LRU), then by chunk id; the
+// framework.cuh:1264-1567, at chunk granularity): score = 64-byte lines read
+// over the last four snapshots (row passes and delete windows; the host link's
+// cost is per line), ties by most recent use (LRU), then by chunk id; the
 // top `frames` chunks are admitted.  A refresh happens only when the admitted
 // set differs from the resident set (cache_refresh_gate.h).
 #include <algorithm>
@@ -165,7 +166,7 @@ uint64_t Engine::move_budget() const { return std::max<uint64_t>(4, cache.frames
 
 // Admission plan shared by both policies.  Candidates are ranked by score
 // (ties: most recent use, then chunk id).  A chunk is admitted only if its
-// score reaches `admit_min` edge reads, it displaces a resident chunk only if
+// score reaches `admit_min` line reads, it displaces a resident chunk only if
 // its score exceeds the resident's by `margin`, and at most move_budget()
 // chunks are admitted, so only the budget strongest candidates and the budget
 // weakest residents are ranked (partial selection, then a sort of those).
@@ -239,8 +240,8 @@ void Engine::cache_maintain(dcg_cache_stats *st) {
     DCG_HIP(hipMemcpyAsync(cache.d_rows_last.p, cache.d_reads.p + C, C * 4, hipMemcpyDeviceToDevice, stream));
     // the learned policy admits at dcg_cache_prefetch, before the next batch
     if (cfg.cache_policy == DCG_CACHE_HOTNESS) {
-        // window of 4 snapshots: admit chunks read more than once per snapshot on average
-        cache_apply_plan(plan_admission(score, 4.0 * (double)kChunkEdges, 1.5), st);
+        // window of 4 snapshots: admit chunks whose reads would repay a load every snapshot
+        cache_apply_plan(plan_admission(score, 4.0 * kLoadLines, 1.5), st);
     }
     fill<float>(stream, cache.d_reads.p, 2 * C, 0.f);
     fill_common(*this, st);

@@ -220,6 +220,9 @@ struct Engine {
     void cache_apply_plan(const std::vector<uint64_t> &desired, dcg_cache_stats *st);
     std::vector<uint64_t> plan_admission(const std::vector<float> &score, double admit_min, double margin);
     uint64_t move_budget() const;   // admissions per decision
+    // per-chunk read counters of the cache policy (64-byte lines), null
+    // without a cache
+    float *policy_reads() { return cfg.placement == DCG_PLACE_DEVICE || !cache.d_reads.n ? nullptr : cache.d_reads.p; }
 
     void sync() { DCG_HIP(hipStreamSynchronize(stream)); }
     void *temp(size_t bytes) {
@@ -227,6 +230,10 @@ struct Engine {
         return d_temp.p;
     }
 };
+
+// Random 64-byte line reads of pinned host memory that one chunk load (256 KiB
+// streamed) costs in host-link time: ~4.6 us at 56 GB/s against ~320 M lines/s.
+constexpr double kLoadLines = 1024.0;
 
 inline double now_ms() {
     using namespace std::chrono;

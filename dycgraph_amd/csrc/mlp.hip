@@ -515,9 +515,9 @@ std::vector<uint64_t> mlp_plan(Engine &e, uint64_t ab, uint64_t ac, uint64_t db,
     m.decisions++;
     m.fallbacks += m.fallback ? 1 : 0;
     m.have_decision = true;
-    if (m.fallback) return e.plan_admission(score, 4.0 * (double)kChunkEdges, 1.5);
-    // the network predicts one snapshot's reads: admit above one chunk's worth
-    return e.plan_admission(score, (double)kChunkEdges, 1.5);
+    if (m.fallback) return e.plan_admission(score, 4.0 * kLoadLines, 1.5);
+    // the network predicts one snapshot's line reads: admit above a load's cost
+    return e.plan_admission(score, kLoadLines, 1.5);
 }
 
 }  // namespace dcg

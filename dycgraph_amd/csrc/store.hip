@@ -448,6 +448,10 @@ __device__ __forceinline__ bool host_slot(const StoreView &s, uint64_t slot) {
 __device__ __forceinline__ uint32_t host_lines(const StoreView &s, uint64_t off, uint32_t from, uint32_t to) {
     return to > from && host_slot(s, off + from) ? (to - from + 15) / 16 : 0u;
 }
+// Per-chunk line reads for the cache policy (null when there is no cache).
+__device__ __forceinline__ void note_lines(float *reads, uint64_t off, uint32_t from, uint32_t to) {
+    if (reads && to > from) atomicAdd(&reads[(off + from) >> kChunkLog2], (float)((to - from + 15) / 16));
+}
 
 constexpr uint32_t kBigDelete = 1024;   // rows above this use one 1024-thread workgroup
 
@@ -575,7 +579,8 @@ __device__ DelResult wave_compact_tombs(const StoreView &s, uint64_t off, uint32
 // count the row slots read and the windows read from host memory.
 template <uint32_t W>
 __device__ uint32_t fenced_take(const StoreView &s, uint64_t off, uint32_t S, const Fences &F, uint32_t v,
-                                uint32_t c, uint32_t q, uint32_t shift, uint32_t &probes, uint32_t &lines) {
+                                uint32_t c, uint32_t q, uint32_t shift, uint32_t &probes, uint32_t &lines,
+                                float *reads) {
     static_assert(W == 16 || W == 64, "subgroup of 16 or 64 lanes");
     const uint64_t below = (1ull << q) - 1;
     const uint32_t nfc = (S + kFenceStride - 1) / kFenceStride;
@@ -615,6 +620,7 @@ __device__ uint32_t fenced_take(const StoreView &s, uint64_t off, uint32_t S, co
         if (q == 0) {
             probes += S - pos < kFenceStride ? S - pos : kFenceStride;
             lines += host_slot(s, off + pos) ? 1u : 0u;
+            note_lines(reads, off, pos, pos + 1);
         }
         // the run may go on past the window when its last slot is <= v
         uint64_t em = __ballot(valid && q == kFenceStride - 1 && id <= v);
@@ -654,11 +660,12 @@ __device__ __forceinline__ uint32_t fenced_lb(const StoreView &s, uint64_t off, 
 __device__ __forceinline__ uint32_t search_prefix(const StoreView &s, uint64_t off, uint32_t S, const Fences &F,
                                                   const uint64_t *keys, uint64_t q, uint64_t gs, uint64_t ge,
                                                   uint32_t *used, uint32_t &probes, uint32_t &need_tail,
-                                                  uint32_t &lines) {
+                                                  uint32_t &lines, float *reads) {
     const uint32_t v = (uint32_t)keys[q];
     if (q > gs && (uint32_t)keys[q - 1] == v) return 0;
     const uint32_t c = (uint32_t)(target_ub(keys, q + 1, ge, v) - q);
     lines += host_slot(s, off) ? 1u : 0u;   // the window the fences leave open
+    note_lines(reads, off, 0, 1);
     uint32_t taken = 0;
     for (uint32_t j = fenced_lb(s, off, S, F, v, probes); j < S && taken < c; ++j) {
         const uint32_t x = load_slot(s, off + j);
@@ -714,7 +721,7 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
                                                   const uint32_t *gstart, const unsigned long long *ngroups_p,
                                                   uint32_t *used, unsigned long long *counters, int count_stats,
                                                   uint32_t *big_groups, unsigned long long *nbig,
-                                                  const uint32_t *fbase, uint32_t *fence) {
+                                                  const uint32_t *fbase, uint32_t *fence, float *reads) {
     const uint32_t lane = lane_id(), qd = lane >> 4, ql = lane & 15;
     const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kWave;
     const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / kWave;
@@ -745,7 +752,7 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
                 const uint32_t v = (uint32_t)keys[t];
                 if (t > gs && (uint32_t)keys[t - 1] == v) continue;
                 const uint32_t c = (uint32_t)(target_ub(keys, t + 1, ge, v) - t);
-                const uint32_t got = fenced_take<16>(s, d.off, S, F, v, c, ql, qd * 16, probes, qlines);
+                const uint32_t got = fenced_take<16>(s, d.off, S, F, v, c, ql, qd * 16, probes, qlines, reads);
                 if (ql == 0) used[t] = got;
                 new_tombs += got;
                 need |= got < c ? 1u : 0u;
@@ -819,7 +826,7 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
                     if ((uint32_t)keys[bge - 1] == v0) {
                         // one run (the group's targets are sorted): the whole wave searches it
                         const uint32_t c = (uint32_t)(bge - bgs);
-                        const uint32_t got = fenced_take<64>(s, bd.off, bS, F, v0, c, lane, 0, p, ln);
+                        const uint32_t got = fenced_take<64>(s, bd.off, bS, F, v0, c, lane, 0, p, ln, reads);
                         if (lane == 0) {
                             used[bgs] = got;
                             nt = got;
@@ -828,7 +835,8 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
                     } else {
                         for (uint64_t q0 = bgs; q0 < bge; q0 += kWave)
                             if (q0 + lane < bge)
-                                nt += search_prefix(s, bd.off, bS, F, keys, q0 + lane, bgs, bge, used, p, nd, ln);
+                                nt += search_prefix(s, bd.off, bS, F, keys, q0 + lane, bgs, bge, used, p, nd, ln,
+                                                    reads);
                     }
                     for (int o = 32; o >= 1; o >>= 1) {
                         nt += __shfl_xor(nt, o, kWave);
@@ -844,7 +852,10 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
                 if (bneed && bS < ext) {
                     __threadfence_block();   // the quarter's tombstone stores complete before the pass reads
                     const DelResult r = wave_delete_range(s, bd.off, bS, ext, 0, keys, bgs, bge, used, F);
-                    if (lane == 0) hl += host_lines(s, bd.off, bS, ext);
+                    if (lane == 0) {
+                        hl += host_lines(s, bd.off, bS, ext);
+                        note_lines(reads, bd.off, bS, ext);
+                    }
                     read += ext - bS;
                     ext = r.end;
                     removed += r.removed;
@@ -1406,7 +1417,7 @@ void Engine::mutate_delete(uint64_t begin, uint64_t count, dcg_phase_stats *st) 
     const int w0 = time_begin(DCG_K_DELETE_ROWS, -1);
     hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream, v, d_out.p, d_sorted_out.p, d_hot_out.p, kout,
                        count, d_gstart.p, d_counters.p + C_GROUPS_OUT, d_used.p, d_counters.p, 1, d_big_groups.p,
-                       d_counters.p + C_NBIG, d_fbase_out.p, d_fence_out.p);
+                       d_counters.p + C_NBIG, d_fbase_out.p, d_fence_out.p, policy_reads());
     time_end(w0);
     const int b0 = time_begin(DCG_K_DELETE_BIG, -1);
     hipLaunchKernelGGL(k_del_big, dim3(kDelBigGrid), dim3(1024), 0, stream, v, d_out.p, d_sorted_out.p, d_hot_out.p,
@@ -1418,7 +1429,7 @@ void Engine::mutate_delete(uint64_t begin, uint64_t count, dcg_phase_stats *st) 
     const int w1 = time_begin(DCG_K_DELETE_ROWS, -1);
     hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream, v, d_in.p, d_sorted_in.p, d_hot_in.p, kin,
                        count, d_gstart2.p, d_counters.p + C_GROUPS_IN, d_used.p, d_counters.p, 0, d_big_groups.p,
-                       d_counters.p + C_NBIG, d_fbase_in.p, d_fence_in.p);
+                       d_counters.p + C_NBIG, d_fbase_in.p, d_fence_in.p, policy_reads());
     time_end(w1);
     const int b1 = time_begin(DCG_K_DELETE_BIG, -1);
     hipLaunchKernelGGL(k_del_big, dim3(kDelBigGrid), dim3(1024), 0, stream, v, d_in.p, d_sorted_in.p, d_hot_in.p,
@@ -1598,9 +1609,10 @@ void Engine::export_csr(uint64_t *row_ptr, uint32_t *col, uint64_t *m) {
 // --------------------------------------------------- per-phase read accounting
 // Folds per-vertex expansion counters into per-chunk edge reads (the cache
 // policy's signal) and splits reads into HBM-cache and host-link reads.
-// Fold the listed rows' read passes into per-chunk edge reads (and rows) for
-// the cache policy and into the cached / host read counters; counters and the
-// list are cleared for the next fold.
+// Fold the listed rows' read passes into per-chunk reads for the cache policy
+// (64-byte lines: a pass over a row segment reads ceil(len / 16) lines; rows)
+// and into the cached / host edge-read counters; counters and the list are
+// cleared for the next fold.
 __global__ void k_fold_reads(StoreView s, const unsigned long long *nlist_p, float *reads, float *rows,
                              unsigned long long *counters) {
     unsigned long long cached = 0, host = 0;
@@ -1619,7 +1631,7 @@ __global__ void k_fold_reads(StoreView s, const unsigned long long *nlist_p, flo
             const uint64_t ce = (c + 1) << kChunkLog2;
             const uint64_t b = end < ce ? end : ce;
             if (reads) {
-                atomicAdd(&reads[c], (float)h * (float)(b - a));
+                atomicAdd(&reads[c], (float)h * (float)((b - a + 15) / 16));   // 64-byte lines read
                 atomicAdd(&rows[c], (float)h);
             }
             if (s.chunk_frame[c] >= 0) cached += (unsigned long long)h * (b - a);
