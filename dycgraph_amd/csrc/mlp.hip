@@ -81,6 +81,7 @@ template <typename FA, typename FB>
 __device__ __forceinline__ float16_t mfma_tile(FA a, FB b, int K, float16_t acc) {
     const int l = (int)lane_id();
     const int i = l & 31, kq = 4 * (l >> 5);
+#pragma unroll 2
     for (int k0 = 0; k0 < K; k0 += 8) {
         half4_t av, bv;
 #pragma unroll
@@ -138,6 +139,7 @@ __global__ __launch_bounds__(64) void k_mlp_tile(const float *X, const float *Y,
         }
         __syncthreads();
         // layer 1 (FP32 FMA): lane owns hidden unit l for all 32 rows
+        #pragma unroll 4
         for (int i = 0; i < kTile; ++i) {
             float a = L.b1[l];
 #pragma unroll
@@ -158,6 +160,7 @@ __global__ __launch_bounds__(64) void k_mlp_tile(const float *X, const float *Y,
         {
             const int i = l >> 1, o = l & 1;
             float a = L.b3[o];
+            #pragma unroll 4
             for (int k = 0; k < kHid; ++k) {
                 const float h = L.z2[i][k] > 0.f ? L.z2[i][k] : 0.f;
                 a += h * L.w3[k * kOut + o];
@@ -178,6 +181,7 @@ __global__ __launch_bounds__(64) void k_mlp_tile(const float *X, const float *Y,
         if (!Y) continue;
         __syncthreads();
         // dW3, db3, dZ2 = (dout * W3^T) .* relu'(Z2)
+        #pragma unroll 4
         for (int i = 0; i < kTile; ++i) {
             const float h = L.z2[i][l] > 0.f ? L.z2[i][l] : 0.f;
             gw3a += h * L.dout[i][0];
@@ -205,6 +209,7 @@ __global__ __launch_bounds__(64) void k_mlp_tile(const float *X, const float *Y,
         }
         __syncthreads();
         // dZ1 = dH1 .* relu'(Z1); lane owns hidden unit l of layer 1
+        #pragma unroll 4
         for (int i = 0; i < kTile; ++i) {
             const float dz = (float)L.h1[i][l] > 0.f ? L.z2[i][l] : 0.f;
             gb1 += dz;
@@ -234,10 +239,14 @@ __global__ __launch_bounds__(1024) void k_grad_reduce(const float *part, uint32_
     __shared__ float acc[16][64];
     const uint32_t c = threadIdx.x & 63, r0 = threadIdx.x >> 6;
     const uint32_t p = blockIdx.x * 64 + c;
-    float a = 0.f;
-    if (p < P_NUM)
-        for (uint32_t b = r0; b < nb; b += 16) a += part[(uint64_t)b * P_NUM + p];
-    acc[r0][c] = a;
+    float a[4] = {0.f, 0.f, 0.f, 0.f};   // four chains keep four loads in flight
+    if (p < P_NUM) {
+        uint32_t b = r0;
+        for (; b + 48 < nb; b += 64)
+            for (int u = 0; u < 4; ++u) a[u] += part[(uint64_t)(b + 16 * u) * P_NUM + p];
+        for (; b < nb; b += 16) a[0] += part[(uint64_t)b * P_NUM + p];
+    }
+    acc[r0][c] = (a[0] + a[1]) + (a[2] + a[3]);
     __syncthreads();
     if (threadIdx.x < 64 && p < P_NUM) {
         float sum = 0.f;
