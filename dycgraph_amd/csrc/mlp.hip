@@ -40,18 +40,21 @@ constexpr int P_W1 = 0, P_B1 = P_W1 + kW1, P_W2 = P_B1 + kHid, P_B2 = P_W2 + kW2
 constexpr int kReplay = 4;   // decisions kept for training (ring)
 
 struct MlpState {
-    // Training runs on its own stream, overlapping the snapshot's mutation and
-    // repair work; `trained` marks the end of the last training launch.  The
-    // engine stream waits on it before a pass reads the weights or a decision
-    // is written into the ring.
+    // Training runs on its own stream, overlapping the snapshot's repair work
+    // (it starts once the delete mutation is enqueued, `released`, so the
+    // latency-bound mutation kernels run alone); `trained` marks the end of
+    // the last training launch.  The engine stream waits on it before a pass
+    // reads the weights or a decision is written into the ring.
     hipStream_t train = nullptr;
-    hipEvent_t trained = nullptr;
+    hipEvent_t trained = nullptr, released = nullptr;
+    bool train_due = false;   // a decision was made; its training launch waits for the delete mutation
     ~MlpState() {
         if (train) {
             (void)hipStreamSynchronize(train);
             (void)hipStreamDestroy(train);
         }
         if (trained) (void)hipEventDestroy(trained);
+        if (released) (void)hipEventDestroy(released);
     }
     DevBuf<float> master, grad, m1, m2;   // FP32 masters, gradients, Adam moments
     DevBuf<float> part;                   // per-workgroup gradient sums of a pass
@@ -416,6 +419,7 @@ void mlp_create(Engine &e) {
     m->w16.alloc(P_NUM);
     DCG_HIP(hipStreamCreateWithFlags(&m->train, hipStreamNonBlocking));
     DCG_HIP(hipEventCreateWithFlags(&m->trained, hipEventDisableTiming));
+    DCG_HIP(hipEventCreateWithFlags(&m->released, hipEventDisableTiming));
     m->X.alloc(kReplay * C * kIn);
     m->Xcur.alloc(C * kIn);
     m->Y.alloc(kReplay * C * kOut);
@@ -487,6 +491,16 @@ void mlp_observe(Engine &e) {
     m.have_decision = false;
 }
 
+void mlp_release_training(Engine &e) {
+    if (!e.cache.mlp) return;
+    MlpState &m = *e.cache.mlp;
+    if (!m.train_due) return;
+    DCG_HIP(hipEventRecord(m.released, e.stream));
+    DCG_HIP(hipStreamWaitEvent(m.train, m.released, 0));
+    mlp_train_steps(m, 4);
+    m.train_due = false;
+}
+
 void mlp_stats(const Engine &e, dcg_cache_stats *st) {
     const MlpState &m = *e.cache.mlp;
     st->learned_decisions = (uint64_t)(m.decisions - m.fallbacks);
@@ -498,6 +512,10 @@ void mlp_stats(const Engine &e, dcg_cache_stats *st) {
 std::vector<uint64_t> mlp_plan(Engine &e, uint64_t ab, uint64_t ac, uint64_t db, uint64_t dc) {
     MlpState &m = *e.cache.mlp;
     const uint64_t C = e.cache.chunks;
+    if (m.train_due) {   // no delete stage ran since the last decision
+        mlp_train_steps(m, 4);
+        m.train_due = false;
+    }
     DCG_HIP(hipStreamWaitEvent(e.stream, m.trained, 0));   // the forward pass reads the trained weights
     fill<float>(e.stream, e.cache.d_touch.p, C, 0.f);
     hipLaunchKernelGGL(k_touch, dim3(grid_for(ac + dc, 256, 8192)), dim3(256), 0, e.stream, e.d_out.p, e.d_in.p,
@@ -519,8 +537,9 @@ std::vector<uint64_t> mlp_plan(Engine &e, uint64_t ab, uint64_t ac, uint64_t db,
     DCG_HIP(hipMemcpyAsync(score.data(), m.fallback ? m.hot_dec.p : m.plan.p, C * 4, hipMemcpyDeviceToHost,
                            e.stream));
     e.sync();
-    // online training on the labelled decisions so far overlaps the batch
-    mlp_train_steps(m, 4);
+    // online training on the labelled decisions so far overlaps the batch's
+    // repair work (mlp_release_training)
+    m.train_due = true;
     m.decisions++;
     m.fallbacks += m.fallback ? 1 : 0;
     m.have_decision = true;

@@ -21,6 +21,7 @@
 //   phase B: all waves share the pieces of the large rows.
 // New frontier entries are appended with one atomic per wave (ballot + mbcnt).
 #include "engine.hpp"
+#include "mlp.hpp"
 
 namespace dcg {
 
@@ -1349,6 +1350,7 @@ void Engine::start_min_lattice() {
 // which only drops deleted edges that the seed step already examined).
 void Engine::delete_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
     mutate_delete(begin, count, st);
+    mlp_release_training(*this);   // the learned policy trains beside the repair, not the mutation
     if (!count) return;
     reserve_stamps(*this, 2);
     h_step[0] = StepArgs{begin, count, {take_stamps(*this), take_stamps(*this), 0u, 0u}};
