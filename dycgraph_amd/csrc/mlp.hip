@@ -46,8 +46,10 @@ struct MlpState {
     // the last training launch.  The engine stream waits on it before a pass
     // reads the weights or a decision is written into the ring.
     hipStream_t train = nullptr;
-    hipEvent_t trained = nullptr, released = nullptr;
+    hipEvent_t trained = nullptr, released = nullptr, labelled = nullptr;
     bool train_due = false;   // a decision was made; its training launch waits for the delete mutation
+    bool judge_pending = false;   // the last judge's sums are on their way to h_judge
+    double *h_judge = nullptr;    // pinned: network top-k, hotness top-k, all reads
     ~MlpState() {
         if (train) {
             (void)hipStreamSynchronize(train);
@@ -55,6 +57,8 @@ struct MlpState {
         }
         if (trained) (void)hipEventDestroy(trained);
         if (released) (void)hipEventDestroy(released);
+        if (labelled) (void)hipEventDestroy(labelled);
+        if (h_judge) (void)hipHostFree(h_judge);
     }
     DevBuf<float> master, grad, m1, m2;   // FP32 masters, gradients, Adam moments
     DevBuf<float> part;                   // per-workgroup gradient sums of a pass
@@ -70,6 +74,8 @@ struct MlpState {
     DevBuf<float> plan;                   // the decision's network score as predicted reads
     DevBuf<float> sort_k, sort_v;         // judge: scores sorted descending, reads carried along
     DevBuf<double> judge;                 // judge sums: network top-k, hotness top-k, all reads
+    DevBuf<float> judge_reads;            // the snapshot's reads, kept for the judge
+    DevBuf<uint8_t> judge_tmp;            // radix-sort scratch of the training stream
     int decisions = 0, fallbacks = 0;
 };
 
@@ -420,6 +426,9 @@ void mlp_create(Engine &e) {
     DCG_HIP(hipStreamCreateWithFlags(&m->train, hipStreamNonBlocking));
     DCG_HIP(hipEventCreateWithFlags(&m->trained, hipEventDisableTiming));
     DCG_HIP(hipEventCreateWithFlags(&m->released, hipEventDisableTiming));
+    DCG_HIP(hipEventCreateWithFlags(&m->labelled, hipEventDisableTiming));
+    DCG_HIP(hipHostMalloc((void **)&m->h_judge, 3 * sizeof(double), hipHostMallocDefault));
+    m->judge_reads.alloc(C);
     m->X.alloc(kReplay * C * kIn);
     m->Xcur.alloc(C * kIn);
     m->Y.alloc(kReplay * C * kOut);
@@ -447,26 +456,38 @@ void mlp_create(Engine &e) {
 
 // Judge: reads of the snapshot held by each ranking's top `frames` chunks.
 // Rankings sort descending and stably (ties keep chunk order, lowest id
-// first); the reads travel with the scores.  Returns {network, hotness}
-// fractions of all reads (1 when nothing was read).
-static std::pair<double, double> judge(Engine &e, MlpState &m, const float *reads) {
+// first); the reads travel with the scores.  Runs on the training stream on a
+// copy of the reads; judge_resolve reads the sums at the next decision.
+static void judge_launch(Engine &e, MlpState &m) {
     const uint64_t C = e.cache.chunks, k = e.cache.frames;
+    DCG_HIP(hipMemcpyAsync(m.judge_reads.p, e.cache.d_reads.p, C * 4, hipMemcpyDeviceToDevice, e.stream));
+    DCG_HIP(hipEventRecord(m.labelled, e.stream));
+    DCG_HIP(hipStreamWaitEvent(m.train, m.labelled, 0));
     size_t tb = 0;
-    DCG_HIP(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tb, m.mscore.p, m.sort_k.p, reads, m.sort_v.p,
-                                                         (int)C, 0, 32, e.stream));
-    void *t = e.temp(tb);
+    DCG_HIP(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tb, m.mscore.p, m.sort_k.p, m.judge_reads.p,
+                                                         m.sort_v.p, (int)C, 0, 32, m.train));
+    m.judge_tmp.ensure(tb);
     const float *ranks[2] = {m.mscore.p, m.hot_dec.p};
     for (int r = 0; r < 2; ++r) {
-        DCG_HIP(hipcub::DeviceRadixSort::SortPairsDescending(t, tb, ranks[r], m.sort_k.p, reads, m.sort_v.p, (int)C,
-                                                             0, 32, e.stream));
-        hipLaunchKernelGGL(k_judge_sum, dim3(1), dim3(1024), 0, e.stream, m.sort_v.p, C, k, m.judge.p, r);
+        DCG_HIP(hipcub::DeviceRadixSort::SortPairsDescending(m.judge_tmp.p, tb, ranks[r], m.sort_k.p, m.judge_reads.p,
+                                                             m.sort_v.p, (int)C, 0, 32, m.train));
+        hipLaunchKernelGGL(k_judge_sum, dim3(1), dim3(1024), 0, m.train, m.sort_v.p, C, k, m.judge.p, r);
     }
     DCG_HIP(hipGetLastError());
-    double sums[3];
-    DCG_HIP(hipMemcpyAsync(sums, m.judge.p, sizeof(sums), hipMemcpyDeviceToHost, e.stream));
-    e.sync();
-    if (!(sums[2] > 0)) return {1.0, 1.0};
-    return {sums[0] / sums[2], sums[1] / sums[2]};
+    DCG_HIP(hipMemcpyAsync(m.h_judge, m.judge.p, 3 * sizeof(double), hipMemcpyDeviceToHost, m.train));
+    DCG_HIP(hipEventRecord(m.trained, m.train));
+    m.judge_pending = true;
+}
+
+// The network's ranking is used only once trained and clearly ahead (call
+// after the engine stream waited for `trained` and synchronised).
+static void judge_resolve(MlpState &m) {
+    if (!m.judge_pending) return;
+    const double *sums = m.h_judge;
+    m.mlp_capture = sums[2] > 0 ? sums[0] / sums[2] : 1.0;
+    m.hot_capture = sums[2] > 0 ? sums[1] / sums[2] : 1.0;
+    m.fallback = m.steps < 8 || !(m.mlp_capture > 1.02 * m.hot_capture);
+    m.judge_pending = false;
 }
 
 void mlp_observe(Engine &e) {
@@ -481,13 +502,9 @@ void mlp_observe(Engine &e) {
     hipLaunchKernelGGL(k_labels, dim3(grid_for(C)), dim3(256), 0, e.stream, e.cache.d_reads.p, C,
                        m.Y.p + (uint64_t)m.slot * C * kOut);
     // judge: which ranking would have captured more of the observed reads
-    const std::pair<double, double> cap = judge(e, m, e.cache.d_reads.p);
-    m.mlp_capture = cap.first;
-    m.hot_capture = cap.second;
+    judge_launch(e, m);
     m.filled = std::min(kReplay, m.filled + 1);
     m.slot = (m.slot + 1) % kReplay;
-    // the network's ranking is used only once trained and clearly ahead
-    m.fallback = m.steps < 8 || !(m.mlp_capture > 1.02 * m.hot_capture);
     m.have_decision = false;
 }
 
@@ -533,19 +550,20 @@ std::vector<uint64_t> mlp_plan(Engine &e, uint64_t ab, uint64_t ac, uint64_t db,
     hipLaunchKernelGGL(k_plan_scores, dim3(grid_for(C)), dim3(256), 0, e.stream, m.out.p, e.cache.d_hist.p, C,
                        m.plan.p, m.mscore.p, m.hot_dec.p);
     DCG_HIP(hipGetLastError());
-    std::vector<float> score(C);
-    DCG_HIP(hipMemcpyAsync(score.data(), m.fallback ? m.hot_dec.p : m.plan.p, C * 4, hipMemcpyDeviceToHost,
-                           e.stream));
+    std::vector<float> hot(C), net(C);
+    DCG_HIP(hipMemcpyAsync(hot.data(), m.hot_dec.p, C * 4, hipMemcpyDeviceToHost, e.stream));
+    DCG_HIP(hipMemcpyAsync(net.data(), m.plan.p, C * 4, hipMemcpyDeviceToHost, e.stream));
     e.sync();
+    judge_resolve(m);   // the previous decision's judge (the stream waited for it above)
     // online training on the labelled decisions so far overlaps the batch's
     // repair work (mlp_release_training)
     m.train_due = true;
     m.decisions++;
     m.fallbacks += m.fallback ? 1 : 0;
     m.have_decision = true;
-    if (m.fallback) return e.plan_admission(score, 4.0 * kLoadLines, 1.5);
+    if (m.fallback) return e.plan_admission(hot, 4.0 * kLoadLines, 1.5);
     // the network predicts one snapshot's line reads: admit above a load's cost
-    return e.plan_admission(score, kLoadLines, 1.5);
+    return e.plan_admission(net, kLoadLines, 1.5);
 }
 
 }  // namespace dcg
