@@ -6,11 +6,15 @@ region) takes update batches that move from region to region.  The hotness/LRU
 ranking only knows the reads of past snapshots and keeps caching the regions
 just left; the network also sees which chunks the staged batch touches.  Same
 graph, stream and queries under both policies: the learned policy must serve
-clearly more edge reads from the device cache, with bit-identical values."""
+clearly more edge reads from the device cache, with bit-identical values, and
+both engines' rows (chunks written while cached, then evicted) must equal the
+multiset model's."""
 import numpy as np
 import pytest
 
+import oracle
 from tests.conftest import announce
+from tests.helpers import same_rows
 
 pytestmark = pytest.mark.gpu
 
@@ -92,8 +96,9 @@ def _run(policy, n, src, dst, adds, dels):
         ab += na
         db += nd
     vals = eng.gather_value()
+    rows = eng.csr()
     eng.close()
-    return hits / max(reads, 1), vals, st
+    return hits / max(reads, 1), vals, st, rows
 
 
 def test_learned_policy_gains_on_moving_batches():
@@ -101,11 +106,18 @@ def test_learned_policy_gains_on_moving_batches():
     announce("learned vs hotness on region-hopping batches")
     n, src, dst = _region_graph()
     adds, dels = _stream(n, src, dst, 40, 5000, hub_churn=48)
-    hot_rate, hot_vals, _ = _run(d.CACHE_HOTNESS, n, src, dst, adds, dels)
-    mlp_rate, mlp_vals, st = _run(d.CACHE_LEARNED, n, src, dst, adds, dels)
+    hot_rate, hot_vals, _, hot_rows = _run(d.CACHE_HOTNESS, n, src, dst, adds, dels)
+    mlp_rate, mlp_vals, st, mlp_rows = _run(d.CACHE_LEARNED, n, src, dst, adds, dels)
     print(f"hit rate second half: hotness {hot_rate:.4f} learned {mlp_rate:.4f} "
           f"(learned decisions {st['learned_decisions']}, fallbacks {st['fallback_decisions']}, "
           f"capture mlp {st['mlp_capture']:.3f} hot {st['hot_capture']:.3f})", flush=True)
     assert np.array_equal(hot_vals, mlp_vals)
+    # chunks written while resident and evicted later: the rows read back from
+    # host memory equal the multiset model's (write-through frames)
+    store = oracle.Store(n, src, dst)
+    for al, dl in zip(adds, dels):
+        store.apply(dl, [])
+        store.apply([], al)
+    assert same_rows(*hot_rows, store) and same_rows(*mlp_rows, store)
     assert st["learned_decisions"] > 0
     assert mlp_rate > 1.2 * hot_rate + 0.02
