@@ -406,11 +406,48 @@ __global__ void k_head_flags(const K *keys, uint64_t count, uint8_t *flags) {
     }
 }
 
+// Small batches: one 1,024-thread workgroup walks the keys in tiles and
+// compacts the heads in order (one launch instead of a flag pass and a
+// device-wide selection).
+constexpr uint64_t kSmallHeads = 8192;
+
+template <typename K>
+__global__ __launch_bounds__(1024) void k_group_heads_small(const K *keys, uint64_t count, uint32_t *gstart,
+                                                            unsigned long long *ngroups) {
+    __shared__ uint32_t wsum[16];
+    const uint32_t w = threadIdx.x / kWave;
+    uint32_t base = 0;
+    for (uint64_t t0 = 0; t0 < count; t0 += 1024) {
+        const uint64_t i = t0 + threadIdx.x;
+        bool head = false;
+        if (i < count) {
+            const uint32_t r = row_of<K>(keys[i]);
+            head = r != ~0u && (i == 0 || row_of<K>(keys[i - 1]) != r);
+        }
+        const uint64_t m = __ballot(head);
+        if (lane_id() == 0) wsum[w] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t before = 0, tot = 0;
+        for (uint32_t q = 0; q < 16; ++q) {
+            before += q < w ? wsum[q] : 0u;
+            tot += wsum[q];
+        }
+        if (head) gstart[base + before + lanes_below(m)] = (uint32_t)i;
+        base += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *ngroups = base;
+}
+
 // Heads of the runs of equal rows in a sorted key array (records of invalid
 // edges, row ~0, sort last and form no group), compacted in key order: group g
 // spans [gstart[g], gstart[g + 1]).
 template <typename K>
 static void group_heads(Engine &e, const K *keys, uint64_t count, uint32_t *gstart, unsigned long long *ngroups) {
+    if (count <= kSmallHeads) {
+        hipLaunchKernelGGL(k_group_heads_small<K>, dim3(1), dim3(1024), 0, e.stream, keys, count, gstart, ngroups);
+        return;
+    }
     e.d_flags.ensure(count);
     hipLaunchKernelGGL(k_head_flags<K>, dim3(grid_for(count)), dim3(256), 0, e.stream, keys, count, e.d_flags.p);
     hipcub::CountingInputIterator<uint32_t> idx(0u);
