@@ -26,6 +26,8 @@ Engine::Engine(const dcg_config &c) : cfg(c) {
     DCG_HIP(hipHostMalloc((void **)&h_step, 2 * sizeof(StepArgs), hipHostMallocDefault));
     d_step.alloc(2);
     DCG_HIP(hipHostMalloc((void **)&h_counters, Engine::kCounterSlots * sizeof(unsigned long long), hipHostMallocDefault));
+    DCG_HIP(hipHostMalloc((void **)&h_tail, Engine::kTailWords * sizeof(uint64_t), hipHostMallocDefault));
+    DCG_HIP(hipHostMalloc((void **)&h_fold, 2 * sizeof(unsigned long long), hipHostMallocDefault));
     app = c.app;
     const bool mixed = c.query_apps && app != DCG_APP_PR;
     Q = app == DCG_APP_PR || (app == DCG_APP_CC && !mixed) ? 1u : std::max(1u, c.num_queries);
@@ -64,7 +66,9 @@ void Engine::time_end(int slot) { DCG_HIP(hipEventRecord(events[timed[slot].ev +
 void Engine::collect_timing() {
     if (timed.empty()) return;
     Ctl ctl[8];
-    if (d_ctl.p) {
+    bool priced = false;   // closure launches (graphs off) are priced from their Ctl blocks
+    for (const Timed &t : timed) priced |= t.ctl >= 0;
+    if (priced && d_ctl.p) {
         DCG_HIP(hipMemcpyAsync(ctl, d_ctl.p, sizeof(ctl), hipMemcpyDeviceToHost, stream));
         sync();
     }
@@ -95,6 +99,8 @@ Engine::~Engine() {
     if (h_step) (void)hipHostFree(h_step);
     drop_graphs();
     if (h_counters) (void)hipHostFree(h_counters);
+    if (h_tail) (void)hipHostFree(h_tail);
+    if (h_fold) (void)hipHostFree(h_fold);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -269,7 +275,6 @@ int dcg_del_edge(dcg_engine *h, uint64_t begin, uint64_t count, dcg_phase_stats 
         const double t0 = now_ms();
         e.delete_phase(begin, count, st);
         e.collect_timing();
-        e.fold_reads(st);
         if (st) {
             st->requests = count;
             st->ms = now_ms() - t0;
@@ -286,7 +291,6 @@ int dcg_add_edge(dcg_engine *h, uint64_t begin, uint64_t count, dcg_phase_stats 
         const double t0 = now_ms();
         e.add_phase(begin, count, st);
         e.collect_timing();
-        e.fold_reads(st);
         if (st) {
             st->requests = count;
             st->ms = now_ms() - t0;

@@ -165,6 +165,10 @@ struct Engine {
     DevBuf<unsigned long long> d_counters;  // phase counters (see store.hip)
     static constexpr int kCounterSlots = 32;   // pinned mirror of d_counters
     unsigned long long *h_counters = nullptr;
+    static constexpr int kTailWords = 3 + 6 * kClasses;   // store tail block (u64 words)
+    uint64_t *h_tail = nullptr;                            // pinned copy of it after an add mutation
+    unsigned long long *h_fold = nullptr;                  // pinned: cached / host edge reads of a fold
+    int mut_slot[4] = {-1, -1, -1, -1};                    // timing slots of the last delete mutation
 
     // cumulative counters
     uint64_t cum_cached_reads = 0, cum_host_reads = 0;
@@ -195,7 +199,13 @@ struct Engine {
     void mutate_add(uint64_t begin, uint64_t count, dcg_phase_stats *st);
     void export_csr(uint64_t *row_ptr, uint32_t *col, uint64_t *m);
     StoreView view();
-    void fold_reads(dcg_phase_stats *st);
+    void fold_reads(dcg_phase_stats *st);   // enqueue + synchronise + finish
+    void fold_enqueue();
+    void fold_finish(dcg_phase_stats *st);
+    // the mutations enqueue their kernels and counter readbacks; finish_* read
+    // them after the caller's synchronisation
+    void finish_delete(uint64_t count, dcg_phase_stats *st);
+    void finish_add(uint64_t count, dcg_phase_stats *st);
 
     // traverse.hip
     void alloc_state();

@@ -46,6 +46,7 @@ static_assert(C_NUM <= Engine::kCounterSlots, "pinned counter mirror too small")
 constexpr int T_TAIL = 0, T_OVERFLOW = 1, T_COUNT = 2, T_CURSOR = 2 + kClasses, T_RANK = 2 + 2 * kClasses,
               T_TAKE = 2 + 3 * kClasses, T_PSTART = 2 + 4 * kClasses, T_FREE = 2 + 5 * kClasses,
               T_RETIRED = 2 + 6 * kClasses, T_NUM = 3 + 6 * kClasses;
+static_assert(T_NUM == Engine::kTailWords, "pinned tail mirror size");
 constexpr uint64_t kPoolOff = (1ull << 40) - 1;   // pool key = class << 40 | slot offset
 
 StoreView Engine::view() {
@@ -1475,13 +1476,21 @@ void Engine::mutate_delete(uint64_t begin, uint64_t count, dcg_phase_stats *st) 
     time_end(b1);
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipMemcpyAsync(h_counters, d_counters.p, C_NUM * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
-    sync();
-    timed[w0].bytes = 4.0 * (double)(h_counters[C_W_READ] + h_counters[C_W_WRITTEN]) + 16.0 * (double)h_counters[C_W_ROWS];
-    timed[w1].bytes = 0.0;
-    timed[w0].host_lines = h_counters[C_W_HOST_LINES];
-    timed[b0].bytes = 4.0 * (double)(h_counters[C_SLOTS_READ] + h_counters[C_SLOTS_WRITTEN]) +
-                      16.0 * (double)h_counters[C_B_ROWS];
-    timed[b1].bytes = 0.0;
+    mut_slot[0] = w0;
+    mut_slot[1] = w1;
+    mut_slot[2] = b0;
+    mut_slot[3] = b1;
+}
+
+void Engine::finish_delete(uint64_t count, dcg_phase_stats *st) {
+    if (!count) return;
+    timed[mut_slot[0]].bytes =
+        4.0 * (double)(h_counters[C_W_READ] + h_counters[C_W_WRITTEN]) + 16.0 * (double)h_counters[C_W_ROWS];
+    timed[mut_slot[1]].bytes = 0.0;
+    timed[mut_slot[0]].host_lines = h_counters[C_W_HOST_LINES];
+    timed[mut_slot[2]].bytes = 4.0 * (double)(h_counters[C_SLOTS_READ] + h_counters[C_SLOTS_WRITTEN]) +
+                               16.0 * (double)h_counters[C_B_ROWS];
+    timed[mut_slot[3]].bytes = 0.0;
     if (st) {
         st->touched_sources = h_counters[C_GROUPS_OUT];
         st->changed_sources = h_counters[C_CHANGED];
@@ -1570,9 +1579,17 @@ void Engine::mutate_add(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
     time_end(slot);
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipMemcpyAsync(h_counters, d_counters.p, C_NUM * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
-    uint64_t tl[T_NUM];
-    DCG_HIP(hipMemcpyAsync(tl, d_tail.p, sizeof(tl), hipMemcpyDeviceToHost, stream));
-    sync();
+    DCG_HIP(hipMemcpyAsync(h_tail, d_tail.p, T_NUM * sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+    mut_slot[0] = slot;
+}
+
+// After the add mutation's readbacks landed.  A store that ran out of slots
+// skipped the rows it could not relocate (rows stay consistent) and is
+// reported here.
+void Engine::finish_add(uint64_t count, dcg_phase_stats *st) {
+    if (!count) return;
+    const uint64_t *tl = h_tail;
+    const int slot = mut_slot[0];
     pool_len = tl[T_RETIRED];
     pool_taken = 0;
     for (int c = 0; c < kClasses; ++c) pool_taken += tl[T_TAKE + c];
@@ -1687,13 +1704,8 @@ __global__ void k_fold_reads(StoreView s, const unsigned long long *nlist_p, flo
     }
 }
 
-void Engine::fold_reads(dcg_phase_stats *st) {
-    if (cfg.placement == DCG_PLACE_DEVICE) {
-        // every chunk is resident: all adjacency reads are HBM reads
-        if (st) st->cached_edge_reads += st->edges_scanned;
-        cum_cached_reads += st ? st->edges_scanned : 0;
-        return;
-    }
+void Engine::fold_enqueue() {
+    if (cfg.placement == DCG_PLACE_DEVICE) return;
     fill<unsigned long long>(stream, d_counters.p + C_CACHED_READS, 2, 0ull);
     float *reads = cache.d_reads.n ? cache.d_reads.p : nullptr;
     float *rows = cache.d_reads.n ? cache.d_reads.p + cache.chunks : nullptr;
@@ -1702,15 +1714,30 @@ void Engine::fold_reads(dcg_phase_stats *st) {
                        reads, rows, d_counters.p);
     fill<unsigned long long>(stream, d_hot_count.p, 1, 0ull);
     DCG_HIP(hipGetLastError());
-    unsigned long long c[2];
-    DCG_HIP(hipMemcpyAsync(c, d_counters.p + C_CACHED_READS, 16, hipMemcpyDeviceToHost, stream));
-    sync();
-    cum_cached_reads += c[0];
-    cum_host_reads += c[1];
-    if (st) {
-        st->cached_edge_reads += c[0];
-        st->host_edge_reads += c[1];
+    DCG_HIP(hipMemcpyAsync(h_fold, d_counters.p + C_CACHED_READS, 2 * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, stream));
+}
+
+// After the fold's readback landed (and, for a device store, after the
+// phase's edges_scanned is known: every read is an HBM read).
+void Engine::fold_finish(dcg_phase_stats *st) {
+    if (cfg.placement == DCG_PLACE_DEVICE) {
+        if (st) st->cached_edge_reads += st->edges_scanned;
+        cum_cached_reads += st ? st->edges_scanned : 0;
+        return;
     }
+    cum_cached_reads += h_fold[0];
+    cum_host_reads += h_fold[1];
+    if (st) {
+        st->cached_edge_reads += h_fold[0];
+        st->host_edge_reads += h_fold[1];
+    }
+}
+
+void Engine::fold_reads(dcg_phase_stats *st) {
+    fold_enqueue();
+    sync();
+    fold_finish(st);
 }
 
 }  // namespace dcg

@@ -1356,10 +1356,12 @@ void Engine::delete_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
     h_step[0] = StepArgs{begin, count, {take_stamps(*this), take_stamps(*this), 0u, 0u}};
     DCG_HIP(hipMemcpyAsync(d_step.p, h_step, sizeof(StepArgs), hipMemcpyHostToDevice, stream));
     run_stage(0);
+    fold_enqueue();
     Ctl *c = d_ctl.p;
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipMemcpyAsync(h_ctl, c, 4 * sizeof(Ctl), hipMemcpyDeviceToHost, stream));
-    sync();
+    sync();   // one wait per stage: mutation counters, repair control blocks, read fold
+    finish_delete(count, st);
     for (int i = 0; i < 4; ++i)
         check_launch(h_ctl[i], "the delete stage");
     if (st) {
@@ -1369,6 +1371,7 @@ void Engine::delete_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
         st->far_entries = h_ctl[3].far_scanned;
         st->edges_scanned = h_ctl[1].edges + h_ctl[2].edges + h_ctl[3].edges;
     }
+    fold_finish(st);
 }
 
 // Add stage (framework.cuh:3525-3756): append, seed from the added edges, close.
@@ -1379,10 +1382,12 @@ void Engine::add_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
     h_step[1] = StepArgs{begin, count, {take_stamps(*this), take_stamps(*this), 0u, 0u}};
     DCG_HIP(hipMemcpyAsync(d_step.p + 1, h_step + 1, sizeof(StepArgs), hipMemcpyHostToDevice, stream));
     run_stage(1);
+    fold_enqueue();
     Ctl *c = d_ctl.p;
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipMemcpyAsync(h_ctl + 4, c + 4, 2 * sizeof(Ctl), hipMemcpyDeviceToHost, stream));
-    sync();
+    sync();   // one wait per stage: mutation counters, repair control blocks, read fold
+    finish_add(count, st);
     check_launch(h_ctl[4], "the add stage");
     check_launch(h_ctl[5], "the add stage");
     if (st) {
@@ -1391,6 +1396,7 @@ void Engine::add_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
         st->far_entries = h_ctl[5].far_scanned;
         st->edges_scanned = h_ctl[5].edges;
     }
+    fold_finish(st);
 }
 
 // The repair launches of a stage, reading the batch and the round stamps from
@@ -1647,7 +1653,11 @@ void Engine::pagerank_batch(uint64_t ab, uint64_t ac, uint64_t db, uint64_t dc, 
     // topology mutation of the whole batch
     dcg_phase_stats dst{}, ast{};
     mutate_delete(db, dc, &dst);
+    sync();
+    finish_delete(dc, &dst);
     mutate_add(ab, ac, &ast);
+    sync();
+    finish_add(ac, &ast);
     // Amend(+1) over the new rows; count[1] continues from ctl[1].count[1]
     hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 2, &c[0].aff_count, 0u, 0u,
                        (const uint32_t *)nullptr);
