@@ -162,7 +162,7 @@ struct Ctl {
     uint32_t rounds;          // rounds executed
     uint32_t overflow;        // a queue overflowed its capacity
     uint32_t aff_count;       // affected-list length (invalidation)
-    uint32_t pad;
+    uint32_t done;            // dense PageRank sweeps: converged or at the round cap
     unsigned long long edges; // adjacency entries scanned
     unsigned long long expansions; // rows expanded
     uint32_t solo_round;      // round reached by a workgroup-0 stretch

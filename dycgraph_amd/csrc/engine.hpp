@@ -222,6 +222,8 @@ struct Engine {
     // dense sweeps pay once the frontier holds more than 1/32 of the vertices
     bool dense_pr(uint64_t frontier) const { return frontier * 32 > (uint64_t)n; }
     DevBuf<uint32_t> d_iota;
+    DevBuf<uint32_t> d_prd_small;        // dense sweeps: in-rows of at most 64 slots
+    DevBuf<uint2> d_prd_big;             // dense sweeps: (row, first slot) pieces of larger in-rows
 
     // cache_policy.hip
     void cache_init();

@@ -1592,25 +1592,165 @@ __global__ void k_iota(uint32_t *q, uint32_t n) {
 // residual over its in-row (segmented wave sums, one atomic per row segment
 // instead of one per edge), until no residual exceeds eps or the round cap.
 // Same fixed point and the same invariant x + r as the push solve.
-void Engine::pagerank_dense(Ctl *c, dcg_phase_stats *st) {
-    if (!d_iota.p) {
-        d_iota.alloc(n ? n : 1);
-        hipLaunchKernelGGL(k_iota, dim3(grid_for(n)), dim3(256), 0, stream, d_iota.p, n);
+// ---- dense PageRank sweeps (a frontier above 1/32 of the vertices) ----------
+// One round: consume every residual into the rank and the share each
+// out-neighbour pulls (pr_residual.cuh:53-69 without the eps filter), then
+// every vertex sums the shares over its in-row.  Rounds are plain launches; a
+// round starts only while the previous round's largest |residual| is above
+// eps and the round cap is not reached (Ctl::done), so launches past the end
+// return at once.
+constexpr uint32_t kDenseSmall = 64;     // in-rows up to this many slots: a quarter-wave each
+constexpr uint32_t kDensePiece = 2048;   // larger in-rows: one wave per piece of this many slots
+
+// In-rows by size: small rows listed, larger rows cut into pieces; the total
+// slot count prices a round (edges).
+__global__ void k_prd_split(StoreView s, uint32_t n, uint32_t *small, uint2 *big, Ctl *c) {
+    unsigned long long slots = 0;
+    for (uint32_t b = blockIdx.x * blockDim.x; b < n; b += gridDim.x * blockDim.x) {
+        const uint32_t v = b + threadIdx.x;
+        const uint32_t ext = v < n ? s.in[v].ext : 0u;
+        slots += ext;
+        const bool sm = ext && ext <= kDenseSmall;
+        const uint32_t np = ext > kDenseSmall ? (ext + kDensePiece - 1) / kDensePiece : 0u;
+        const uint64_t ms = __ballot(sm);
+        if (ms) {
+            const int leader = __ffsll((unsigned long long)ms) - 1;
+            uint32_t base = 0;
+            if ((int)lane_id() == leader) base = atomicAdd(&c->count[0], (uint32_t)__popcll(ms));
+            base = __shfl(base, leader, kWave);
+            if (sm) small[base + lanes_below(ms)] = v;
+        }
+        if (np) {
+            const uint32_t base = atomicAdd(&c->count[1], np);
+            for (uint32_t p = 0; p < np; ++p) big[base + p] = make_uint2(v, p * kDensePiece);
+        }
     }
-    hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c, (const uint32_t *)nullptr, n, 0u,
+    for (int d = 32; d >= 1; d >>= 1) slots += __shfl_xor(slots, d, kWave);
+    if (lane_id() == 0 && slots) atomicAdd(&c->far_scanned, slots);   // slots per round
+}
+
+__global__ void k_prd_consume(StoreView s, uint32_t n, uint32_t r, uint32_t max_rounds, float eps, float *rank,
+                              float *res, float *delta, uint32_t *queued, Ctl *c) {
+    if (*(volatile uint32_t *)&c->done) return;
+    if (r >= max_rounds || (r > 0 && !(__uint_as_float(*(volatile uint32_t *)&c->rmax[(r - 1) & 1]) > eps))) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) c->done = 1;
+        return;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        c->rmax[r & 1] = 0;
+        c->rounds = r + 1;
+        c->edges += c->far_scanned;
+    }
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        const float x = res[v];
+        if (!isfinite(x) || !isfinite(rank[v])) atomicExch((unsigned long long *)&c->bad, 1ull);
+        queued[v] = 0;
+        rank[v] += x;
+        res[v] = 0.f;
+        const uint32_t live = row_live(s.out[v]);
+        delta[v] = live ? 0.85f * x / (float)live : 0.f;
+    }
+}
+
+// Small in-rows: 16 lanes read a row 16 slots at a time and add the shares.
+__global__ __launch_bounds__(256) void k_prd_pull_small(StoreView s, const uint32_t *rows, const Ctl *c,
+                                                        const float *delta, float *res) {
+    if (*(volatile const uint32_t *)&c->done) return;
+    const uint32_t nrows = c->count[0];
+    const uint32_t lane = lane_id(), ql = lane & 15;
+    const uint64_t quarters = (uint64_t)gridDim.x * blockDim.x / 16;
+    for (uint64_t i = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / 16; i < nrows; i += quarters) {
+        const uint32_t v = rows[i];
+        const RowDesc d = s.in[v];
+        const uint32_t *base = row_base(s, d.off);   // a block of at most 64 slots lies in one chunk
+        float sum = 0.f;
+        for (uint32_t j = ql; j < d.ext; j += 16) {
+            const uint32_t x = base[j];
+            if (!(x & kTomb)) sum += delta[x];
+        }
+        for (int o = 8; o >= 1; o >>= 1) sum += __shfl_xor(sum, o, 16);
+        if (ql == 0) {
+            res[v] = sum;
+            mark_read(s, s.hot_in, v);
+        }
+    }
+}
+
+// Pieces of larger in-rows: one wave per piece, one atomic per piece.
+__global__ __launch_bounds__(256) void k_prd_pull_big(StoreView s, const uint2 *pieces, const Ctl *c,
+                                                      const float *delta, float *res) {
+    if (*(volatile const uint32_t *)&c->done) return;
+    const uint32_t np = c->count[1];
+    const uint32_t lane = lane_id();
+    const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / kWave;
+    for (uint64_t i = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kWave; i < np; i += nwaves) {
+        const uint2 pc = pieces[i];
+        const RowDesc d = s.in[pc.x];
+        const uint32_t end = d.ext - pc.y < kDensePiece ? d.ext : pc.y + kDensePiece;
+        float sum = 0.f;
+        for (uint32_t j = pc.y + lane; j < end; j += kWave) {
+            const uint32_t x = *row_base(s, d.off + j);
+            if (!(x & kTomb)) sum += delta[x];
+        }
+        for (int o = 32; o >= 1; o >>= 1) sum += __shfl_xor(sum, o, kWave);
+        if (lane == 0) {
+            atomicAdd(&res[pc.x], sum);
+            if (pc.y == 0) mark_read(s, s.hot_in, pc.x);
+        }
+    }
+}
+
+// Largest |residual| after round r's pull.
+__global__ void k_prd_rmax(const float *res, uint32_t n, uint32_t r, Ctl *c) {
+    if (*(volatile uint32_t *)&c->done) return;
+    float m = 0.f;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+        m = fmaxf(m, fabsf(res[v]));
+    for (int d = 32; d >= 1; d >>= 1) m = fmaxf(m, __shfl_xor(m, d, kWave));
+    if (lane_id() == 0 && m > 0.f) atomicMax(&c->rmax[r & 1], __float_as_uint(m));
+}
+
+void Engine::pagerank_dense(Ctl *c, dcg_phase_stats *st) {
+    const uint64_t slots_bound = total_slots / kDensePiece + n + 1;
+    d_prd_small.ensure(n ? n : 1);
+    d_prd_big.ensure(slots_bound);
+    hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c, (const uint32_t *)nullptr, 0u, 0u,
                        (const uint32_t *)nullptr);
-    TravArgs a = base_args(*this);
-    a.ctl = c;
-    a.in0 = d_iota.p;
-    a.max_rounds = cfg.pr_max_rounds;
-    launch_persistent(*this, M_PR_DENSE, a);
+    fill<unsigned long long>(stream, &c->far_scanned, 1, 0ull);
+    fill<uint32_t>(stream, &c->done, 1, 0u);
+    fill<uint32_t>(stream, c->rmax, 2, 0u);
+    const StoreView v = view();
+    hipLaunchKernelGGL(k_prd_split, dim3(grid_for(n)), dim3(256), 0, stream, v, n, d_prd_small.p, d_prd_big.p, c);
+    const uint32_t max_rounds = cfg.pr_max_rounds;   // 1..100, checked at creation
+    const uint32_t cg = grid_for(n, 256, 4096), sg = grid_for((uint64_t)n * 16, 256, 65536);
+    const float eps = (float)cfg.pr_epsilon;
+    // rounds in chunks of 16 between host checks; launches past convergence return at once
+    const int slot = time_begin(DCG_K_PR_SOLVE, -1);
+    for (uint32_t r0 = 0;; r0 += 16) {
+        for (uint32_t r = r0; r < r0 + 16; ++r) {
+            hipLaunchKernelGGL(k_prd_consume, dim3(cg), dim3(256), 0, stream, v, n, r, max_rounds, eps, d_rank.p,
+                               d_res.p, d_delta.p, d_queued.p, c);
+            hipLaunchKernelGGL(k_prd_pull_small, dim3(sg), dim3(256), 0, stream, v, d_prd_small.p, c, d_delta.p,
+                               d_res.p);
+            hipLaunchKernelGGL(k_prd_pull_big, dim3(8192), dim3(256), 0, stream, v, d_prd_big.p, c, d_delta.p,
+                               d_res.p);
+            hipLaunchKernelGGL(k_prd_rmax, dim3(cg), dim3(256), 0, stream, d_res.p, n, r, c);
+        }
+        DCG_HIP(hipGetLastError());
+        uint32_t done = 0;
+        DCG_HIP(hipMemcpyAsync(&done, &c->done, 4, hipMemcpyDeviceToHost, stream));
+        sync();
+        if (done || (uint64_t)r0 + 16 > (uint64_t)max_rounds) break;
+    }
+    time_end(slot);
     hipLaunchKernelGGL(k_pr_pending, dim3(grid_for(n)), dim3(256), 0, stream, d_res.p, d_queued.p, n,
                        (float)cfg.pr_epsilon, d_qc.p, c);
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipMemcpyAsync(h_ctl + (c - d_ctl.p), c, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
     sync();
     const Ctl &h = h_ctl[c - d_ctl.p];
-    check_launch(h, "PageRank");
+    // 24 B per row pulled (descriptor + share and residual) + 12 B per slot
+    timed[slot].bytes = 24.0 * (double)n * (double)h.rounds + 12.0 * (double)h.edges;
     if (h.bad) throw Error(DCG_ERR_NUMERIC, "PR non-finite rank/residual");
     pr_pending = d_qc.p;
     pr_pending_count = h.aff_count;
