@@ -529,3 +529,41 @@ def test_fenced_prefix_search_rows(app, placement):
         ab += len(al)
         db += len(dl)
     eng.close()
+
+
+@pytest.mark.parametrize("placement", [0, 1])
+def test_pagerank_dense_sweeps(placement):
+    """PageRank batches large enough for dense sweeps (more than 1/32 of the
+    vertices pending): small in-rows pulled by quarter-waves, a hub in-row of
+    5,000 slots cut into three pieces, vertices without in-edges.  After the
+    static solve and every batch the ranks pass the reference's checker
+    (hybrid_pr_host.cu:9-56) against the multiset model's rows."""
+    announce(f"pagerank dense sweeps, placement={placement}")
+    import dycgraph_amd as d
+    n, src, dst = rmat_graph(13, 60000, seed=8)
+    rng = np.random.default_rng(3)
+    hub = 7
+    extra = rng.choice(np.arange(100, n), 5000, replace=False).astype(np.uint32)
+    src = np.concatenate([src, extra])
+    dst = np.concatenate([dst, np.full(extra.size, hub, np.uint32)])
+    per = [1500, 1500, 1500]
+    adds, dels = d.gen_stream(n, src, dst, per, per, seed=5)
+    eng = _engine(app="pr", placement=placement, cache_bytes=1 << 18, pr_base=0.15, pr_epsilon=1e-6)
+    eng.load_graph(n, src, dst)
+    eng.start()
+    store = oracle.Store(n, src, dst)
+    rp, col = store.csr()
+    chk = oracle.check_pagerank(n, rp, col, eng.gather_value(), eng.gather_buffer())
+    assert chk["ok"], chk
+    eng.load_updates(adds, dels)
+    ab = db = 0
+    for s, k in enumerate(per):
+        eng.update_pagerank(ab, k, db, k)
+        store.apply(dels[db:db + k], [])
+        store.apply([], adds[ab:ab + k])
+        rp, col = store.csr()
+        chk = oracle.check_pagerank(n, rp, col, eng.gather_value(), eng.gather_buffer())
+        assert chk["ok"], (s, chk)
+        ab += k
+        db += k
+    eng.close()
