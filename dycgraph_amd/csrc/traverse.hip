@@ -25,7 +25,7 @@
 
 namespace dcg {
 
-enum Mode : uint32_t { M_PUSH = 0, M_PULL = 1, M_INVALIDATE = 2, M_PR = 3, M_PR_AMEND = 4, M_PR_DENSE = 5 };
+enum Mode : uint32_t { M_PUSH = 0, M_PULL = 1, M_INVALIDATE = 2, M_PR = 3, M_PR_AMEND = 4 };
 
 struct TravArgs {
     StoreView s;
@@ -409,8 +409,7 @@ __device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L
     const uint32_t cur = r % 3, nxt = (r + 1) % 3;
     const uint32_t lane = lane_id();
     const uint32_t cnt = *(volatile uint32_t *)&a.ctl->count[cur];
-    // dense PageRank sweeps take every vertex each round (the fixed list in0)
-    const uint32_t *in = (r == 0 || MODE == M_PR_DENSE) ? a.in0 : ((r & 1) ? a.qa : a.qb);
+    const uint32_t *in = r == 0 ? a.in0 : ((r & 1) ? a.qa : a.qb);
     uint32_t *out = (r & 1) ? a.qb : a.qa;
     const uint32_t stamp = stamp_base(a) + r;
     uint32_t batch = t.wave, spare = 0;   // the first batch is static; then kClaim per atomic
@@ -435,14 +434,6 @@ __device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L
                     off = d.off;
                     val = __float_as_uint(0.85f * x / (float)live);
                     mark_read(a.s, a.s.hot_out, v);
-                }
-            } else if (MODE == M_PR_DENSE) {
-                v = g;   // every vertex pulls over its in-row
-                const RowDesc d = a.s.in[v];
-                if (d.ext) {
-                    deg = d.ext;
-                    off = d.off;
-                    mark_read(a.s, a.s.hot_in, v);
                 }
             } else {
                 decode_g(a, g, q, v);
@@ -483,31 +474,7 @@ __device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         edges += total;
-        for (uint32_t k0 = 0; MODE == M_PR_DENSE && k0 < total; k0 += kWave) {
-            // owners are contiguous along the lanes: segmented sum, one atomic per owner
-            const uint32_t k = k0 + lane;
-            uint32_t o = 64;
-            float c = 0.f;
-            if (k < total) {
-                uint32_t lo = 0, hi = 64;
-                while (hi - lo > 1) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (L.pref[mid] <= k) lo = mid;
-                    else hi = mid;
-                }
-                o = lo;
-                const uint32_t x = L.base[lo][k - L.pref[lo]];
-                c = (x & kTomb) ? 0.f : a.delta[x];
-            }
-            for (uint32_t dd = 1; dd < kWave; dd <<= 1) {
-                const float y = __shfl_up(c, dd, kWave);
-                const uint32_t oy = __shfl_up(o, dd, kWave);
-                if (lane >= dd && oy == o) c += y;
-            }
-            const uint32_t onext = __shfl_down(o, 1, kWave);
-            if (o < 64 && (lane == kWave - 1 || onext != o) && c != 0.f) atomicAdd(&a.res[L.owner[o]], c);
-        }
-        for (uint32_t k0 = 0; MODE != M_PR_DENSE && k0 < total; k0 += kWave) {
+        for (uint32_t k0 = 0; k0 < total; k0 += kWave) {
             const uint32_t k = k0 + lane;
             uint32_t kind = 0;
             uint32_t og = 0;
@@ -564,18 +531,10 @@ __device__ void phase_b(const TravArgs &a, uint32_t r, const Team &t, OutLds *O,
             const uint32_t len = be.deg - start < a.piece ? be.deg - start : a.piece;
             const uint32_t *base = row_base(a.s, be.off + start);
             uint32_t q = 0, v = 0;
-            if (MODE == M_PR || MODE == M_PR_AMEND || MODE == M_PR_DENSE) v = be.g;
+            if (MODE == M_PR || MODE == M_PR_AMEND) v = be.g;
             else decode_g(a, be.g, q, v);
             edges += lane == 0 ? len : 0;
-            if (MODE == M_PR_DENSE) {
-                float c = 0.f;
-                for (uint32_t k = lane; k < len; k += kWave) {
-                    const uint32_t x = base[k];
-                    if (!(x & kTomb)) c += a.delta[x];
-                }
-                for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, kWave);
-                if (lane == 0 && c != 0.f) atomicAdd(&a.res[v], c);
-            } else if (MODE == M_PULL && a.qapp[q] == DCG_APP_CC) {
+            if (MODE == M_PULL && a.qapp[q] == DCG_APP_CC) {
                 for (uint32_t k = lane; k < len; k += kWave) {
                     const uint32_t x = base[k];
                     if (!(x & kTomb)) pull_cc(a, q, v, x);
@@ -619,21 +578,6 @@ __device__ void round_open(const TravArgs &a, uint32_t r, const Team &t) {
         a.ctl->big[nxt] = 0;
         a.ctl->big_cursor[nxt] = 0;
         a.ctl->rounds = r + 1;
-    }
-    if (MODE == M_PR_DENSE) {
-        // every round is a full sweep: the next one again takes all n vertices
-        if (t.tid == 0) a.ctl->count[nxt] = a.n;
-        // Consume every residual (pr_residual.cuh:53-69 without the eps filter):
-        // rank += r, and the share 0.85 r / outdeg each out-neighbour will pull.
-        for (uint32_t v = t.tid; v < a.n; v += t.nthreads) {
-            const float res = a.res[v];
-            if (!isfinite(res) || !isfinite(a.rank[v])) atomicExch((unsigned int *)&a.ctl->bad, 1u);
-            a.queued[v] = 0;
-            a.rank[v] += res;
-            a.res[v] = 0.f;
-            const uint32_t live = row_live(a.s.out[v]);
-            a.delta[v] = live ? 0.85f * res / (float)live : 0.f;
-        }
     }
     if (MODE == M_PR) {
         // Consume (pr_residual.cuh:53-69): fold residuals above eps into ranks.
@@ -766,7 +710,7 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(TravArgs a) {
     // publish round rr's buffered entries (modes that emit; whole-grid rounds
     // only: workgroup 0 alone appends directly); a workgroup barrier either way
     auto flush = [&](uint32_t rr) {
-        if (MODE != M_PULL && MODE != M_PR_DENSE)
+        if (MODE != M_PULL)
             flush_out<MODE>(O, a, (rr & 1) ? a.qb : a.qa, &a.ctl->count[(rr + 1) % 3]);
         else
             __syncthreads();
@@ -792,7 +736,7 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(TravArgs a) {
             break;
         }
         idle_advances = 0;
-        if (MODE != M_PR_DENSE && cnt <= kSoloMax && gridDim.x > 1) {
+        if (cnt <= kSoloMax && gridDim.x > 1) {
             if (blockIdx.x == 0) {
                 // solo stretch: rounds on workgroup 0 until the frontier grows,
                 // empties, or a round queues many large-row pieces
@@ -831,8 +775,7 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(TravArgs a) {
             continue;
         }
         round_open<MODE>(a, r, gt);
-        if (MODE == M_PR || MODE == M_PR_DENSE) gsync(a);
-        if (MODE == M_PR_DENSE && gt.tid == 0) a.ctl->rmax[(r + 1) & 1] = 0;   // all read it last round
+        if (MODE == M_PR) gsync(a);
         phase_a<MODE>(a, r, gt, L, &O, edges, expansions);
         flush(r);
         gsync(a);
@@ -840,18 +783,6 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(TravArgs a) {
             phase_b<MODE>(a, r, gt, &O, edges);
             flush(r);
             gsync(a);
-        }
-        if (MODE == M_PR_DENSE) {
-            // converged when no residual is above eps
-            float m = 0.f;
-            for (uint32_t v = gt.tid; v < a.n; v += gt.nthreads) m = fmaxf(m, fabsf(a.res[v]));
-            for (int d = 32; d >= 1; d >>= 1) m = fmaxf(m, __shfl_xor(m, d, kWave));
-            if (lane == 0 && m > 0.f) atomicMax(&a.ctl->rmax[r & 1], __float_as_uint(m));
-            gsync(a);
-            const float rm = __uint_as_float(*(volatile uint32_t *)&a.ctl->rmax[r & 1]);
-            ++r;
-            if (!(rm > a.eps)) break;
-            continue;
         }
         ++r;
     }
@@ -1086,7 +1017,6 @@ static const void *persistent_fn(uint32_t mode) {
     case M_PULL: return (const void *)&k_persistent<M_PULL>;
     case M_INVALIDATE: return (const void *)&k_persistent<M_INVALIDATE>;
     case M_PR: return (const void *)&k_persistent<M_PR>;
-    case M_PR_DENSE: return (const void *)&k_persistent<M_PR_DENSE>;
     default: return (const void *)&k_persistent<M_PR_AMEND>;
     }
 }
@@ -1095,8 +1025,7 @@ void launch_persistent(Engine &e, uint32_t mode, TravArgs &a) {
     const void *fn = persistent_fn(mode);
     if (!e.coop_grid[mode]) e.coop_grid[mode] = e.coop_blocks(fn);
     void *args[] = {&a};
-    static const int kid[6] = {DCG_K_PUSH, DCG_K_PULL, DCG_K_INVALIDATE, DCG_K_PR_SOLVE, DCG_K_PR_AMEND,
-                               DCG_K_PR_SOLVE};
+    static const int kid[5] = {DCG_K_PUSH, DCG_K_PULL, DCG_K_INVALIDATE, DCG_K_PR_SOLVE, DCG_K_PR_AMEND};
     if (mode == M_PUSH && a.far0)
         hipLaunchKernelGGL(k_far_setup, dim3(1), dim3(64), 0, e.stream, a.thr, a.minfar, a.fdelta, a.Q);
     if (e.capturing) {
