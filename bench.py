@@ -35,7 +35,7 @@ METRIC = "incremental updates/s + p50 repair latency/batch; learned vs LRU cache
 HBM_PEAK_GBS = 8000.0
 # timing category -> device kernel (rocprofv3 names)
 KERNEL_SYMBOL = {"push": "k_persistent<0u>", "pull": "k_persistent<1u>", "invalidate": "k_persistent<2u>",
-                 "pr_solve": "k_persistent<3u>", "pr_amend": "k_persistent<4u>", "delete_rows": "k_del_rows",
+                 "pr_solve": "k_persistent<3u> (push) / k_prd_consume + k_prd_pull_small + k_prd_pull_big + k_prd_rmax (dense sweeps)", "pr_amend": "k_persistent<4u>", "delete_rows": "k_del_rows",
                  "delete_big": "k_del_big", "append_rows": "k_add_rows + k_add_big"}
 # HBM traffic per launch measured with rocprofv3 PMC counters (FETCH_SIZE and
 # WRITE_SIZE runs, scaled by the counters of a copy kernel of known size)
