@@ -1072,9 +1072,10 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t x, uint32_t &slot) {
 // Large rows (more than kBigDelete slots, or holding tombstones): one
 // 1024-thread workgroup per row.
 //  * Sorted prefix [0, S) of at least kSearchPrefix entries: the head of every
-//    run of equal targets binary-searches the run's value and tombstones its
-//    first live occurrences (bit 31 set, id bits kept, so the prefix stays
-//    ordered): O(log S) slot reads per target instead of a scan of the row.
+//    run of equal targets finds the run's value through the row's fences
+//    (fenced_lb: a binary search of the fences in HBM, then of one 16-slot
+//    window) and tombstones its first live occurrences (bit 31 set, id bits
+//    kept, so the prefix stays ordered and its fences valid).
 //    Then, only when some run still wants occurrences, compact_range over the
 //    tail [S, ext) (appended entries, never tombstoned) continues the per-run
 //    counters.  The prefix precedes the tail in the row's sequence, so exactly
