@@ -89,6 +89,15 @@ def test_rmat12_host_cache(app):
     _run_stream(app, n, src, dst, [300] * 4, [300] * 4, placement=1, cache_bytes=1 << 18)
 
 
+@pytest.mark.parametrize("app", ["sssp", "cc"])
+def test_rmat14_large_batches(app):
+    """Batches of 20,000 deletes and inserts: group heads come from the
+    device-wide ordered compaction (small batches use one workgroup)."""
+    announce(f"rmat14 large batches {app}")
+    n, src, dst = rmat_graph(14, 200000, seed=6)
+    _run_stream(app, n, src, dst, [20000] * 2, [20000] * 2, placement=1, cache_bytes=1 << 20)
+
+
 def test_rmat16_sssp_seeded_weights():
     announce("rmat16 sssp seeded weights, 3 snapshots x 1000/1000")
     n, src, dst = rmat_graph(16, 1 << 20)
