@@ -77,8 +77,6 @@ struct Engine {
     std::string err;
     int device = 0;
     hipStream_t stream = nullptr;
-    hipStream_t stream2 = nullptr;          // the in-row delete pass runs here, beside the out-row pass
-    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     int cu_count = 0;
     uint32_t coop_grid[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     struct Tune {
@@ -163,7 +161,6 @@ struct Engine {
     DevBuf<uint32_t> d_gstart, d_gstart2, d_uniq;
     DevBuf<uint32_t> d_used;           // per-record occurrence counters
     DevBuf<uint32_t> d_big_groups;     // large-row mutation paths
-    DevBuf<uint32_t> d_used2, d_big_groups2;   // the in-row delete pass's own counters and list
     DevBuf<uint8_t> d_flags;           // group-head flags of a sorted key array
     DevBuf<unsigned long long> d_counters;  // phase counters (see store.hip)
     static constexpr int kCounterSlots = 32;   // pinned mirror of d_counters
@@ -183,12 +180,11 @@ struct Engine {
         int ctl;       // Ctl block index whose counters price the launch (-1: mutation)
         double bytes;  // preset bytes (mutation kernels)
         uint64_t host_lines = 0;
-        hipStream_t on = nullptr;   // stream the launch and its events are on
     };
     dcg_kernel_stats kstat[DCG_K_NUM] = {};
     std::vector<hipEvent_t> events;   // pairs (start, end)
     std::vector<Timed> timed;
-    int time_begin(int kernel, int ctl, double bytes = 0, hipStream_t on = nullptr);   // on: default the engine stream
+    int time_begin(int kernel, int ctl, double bytes = 0);
     void time_end(int slot);
     void collect_timing();
 

@@ -35,8 +35,7 @@ enum Counter : int {
     C_W_ROWS = 14,
     C_B_ROWS = 15,       // workgroup-path delete rows
     C_W_HOST_LINES = 16, // wave-path delete: 64-byte lines read from pinned host memory
-    C_NBIG2 = 17,        // large-row list of the in-row delete pass
-    C_NUM = 18
+    C_NUM = 17
 };
 static_assert(C_NUM <= Engine::kCounterSlots, "pinned counter mirror too small");
 
@@ -1451,17 +1450,9 @@ void Engine::mutate_delete(uint64_t begin, uint64_t count, dcg_phase_stats *st) 
     StoreView v = view();
     const uint32_t wg = wave_grid(count);
     d_big_groups.ensure(count);
-    d_big_groups2.ensure(count);
-    d_used2.ensure(count);
     fill<uint32_t>(stream, d_used.p, count, 0u);
-    fill<uint32_t>(stream, d_used2.p, count, 0u);
-    // The out-row pass runs on the engine stream and the in-row pass beside it
-    // on stream2 (other rows, other descriptors, prefixes and fences, their own
-    // occurrence counters and large-row list; shared counters are atomics).
-    // Each launch is timed on its own stream; the bytes of both directions go
-    // to the first launch of each kernel, so sums and averages hold.
-    DCG_HIP(hipEventRecord(fork_ev, stream));
-    DCG_HIP(hipStreamWaitEvent(stream2, fork_ev, 0));
+    // each launch timed on its own (HIP events); the bytes of both directions
+    // go to the first launch of each kernel, so sums and averages hold
     const int w0 = time_begin(DCG_K_DELETE_ROWS, -1);
     hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream, v, d_out.p, d_sorted_out.p, d_hot_out.p, kout,
                        count, d_gstart.p, d_counters.p + C_GROUPS_OUT, d_used.p, d_counters.p, 1, d_big_groups.p,
@@ -1472,18 +1463,18 @@ void Engine::mutate_delete(uint64_t begin, uint64_t count, dcg_phase_stats *st) 
                        kout, count, d_big_groups.p, d_counters.p + C_NBIG, d_used.p, d_counters.p, 1, d_fbase_out.p,
                        d_fence_out.p);
     time_end(b0);
-    const int w1 = time_begin(DCG_K_DELETE_ROWS, -1, 0.0, stream2);
-    hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream2, v, d_in.p, d_sorted_in.p, d_hot_in.p, kin,
-                       count, d_gstart2.p, d_counters.p + C_GROUPS_IN, d_used2.p, d_counters.p, 0, d_big_groups2.p,
-                       d_counters.p + C_NBIG2, d_fbase_in.p, d_fence_in.p, policy_reads());
+    fill<uint32_t>(stream, d_used.p, count, 0u);
+    fill<unsigned long long>(stream, d_counters.p + C_NBIG, 1, 0ull);
+    const int w1 = time_begin(DCG_K_DELETE_ROWS, -1);
+    hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream, v, d_in.p, d_sorted_in.p, d_hot_in.p, kin,
+                       count, d_gstart2.p, d_counters.p + C_GROUPS_IN, d_used.p, d_counters.p, 0, d_big_groups.p,
+                       d_counters.p + C_NBIG, d_fbase_in.p, d_fence_in.p, policy_reads());
     time_end(w1);
-    const int b1 = time_begin(DCG_K_DELETE_BIG, -1, 0.0, stream2);
-    hipLaunchKernelGGL(k_del_big, dim3(kDelBigGrid), dim3(1024), 0, stream2, v, d_in.p, d_sorted_in.p, d_hot_in.p,
-                       kin, count, d_big_groups2.p, d_counters.p + C_NBIG2, d_used2.p, d_counters.p, 0, d_fbase_in.p,
+    const int b1 = time_begin(DCG_K_DELETE_BIG, -1);
+    hipLaunchKernelGGL(k_del_big, dim3(kDelBigGrid), dim3(1024), 0, stream, v, d_in.p, d_sorted_in.p, d_hot_in.p,
+                       kin, count, d_big_groups.p, d_counters.p + C_NBIG, d_used.p, d_counters.p, 0, d_fbase_in.p,
                        d_fence_in.p);
     time_end(b1);
-    DCG_HIP(hipEventRecord(join_ev, stream2));
-    DCG_HIP(hipStreamWaitEvent(stream, join_ev, 0));
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipMemcpyAsync(h_counters, d_counters.p, C_NUM * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
     mut_slot[0] = w0;
