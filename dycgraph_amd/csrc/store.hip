@@ -35,7 +35,8 @@ enum Counter : int {
     C_W_ROWS = 14,
     C_B_ROWS = 15,       // workgroup-path delete rows
     C_W_HOST_LINES = 16, // wave-path delete: 64-byte lines read from pinned host memory
-    C_NUM = 17
+    C_NBIG_IN = 17,      // large-row list of the in direction (C_NBIG: out direction)
+    C_NUM = 18
 };
 static_assert(C_NUM <= Engine::kCounterSlots, "pinned counter mirror too small");
 
@@ -1192,8 +1193,7 @@ __global__ __launch_bounds__(256) void k_add_plan(const RowDesc *desc, const uin
 
 // Per class: blocks retired by earlier batches are taken first, the rest come
 // from the bump tail (largest classes first, each aligned to its size).
-__global__ void k_alloc_classes(uint64_t *tail, uint64_t total) {
-    if (threadIdx.x || blockIdx.x) return;
+__device__ void alloc_classes(uint64_t *tail, uint64_t total) {
     uint64_t t = tail[T_TAIL];
     for (int c = kClasses - 1; c >= 0; --c) {
         const uint64_t cnt = tail[T_COUNT + c];
@@ -1209,6 +1209,11 @@ __global__ void k_alloc_classes(uint64_t *tail, uint64_t total) {
     }
     if (t > total) tail[T_OVERFLOW] = 1;
     else tail[T_TAIL] = t;
+}
+
+__global__ void k_alloc_classes(uint64_t *tail, uint64_t total) {
+    if (threadIdx.x || blockIdx.x) return;
+    alloc_classes(tail, total);
 }
 
 // Free-pool segments of the class-sorted pool keys (taken entries, ~0, sort last).
@@ -1231,14 +1236,26 @@ __global__ void k_pool_counts(uint64_t *tail) {
 // in each class with one global atomic per class (LDS ranks inside the
 // workgroup), so relocations do not serialise on a cursor.  Rank i of class c
 // takes the i-th free-pool block of c, or a bump block past them.  Same
-// relocation test as k_add_plan.
-__global__ __launch_bounds__(256) void k_add_assign(const RowDesc *desc, const uint32_t *keys, uint64_t count,
-                                                    const uint32_t *gstart, const unsigned long long *ngroups_p,
-                                                    uint64_t *tail, uint64_t *pool, uint64_t *reloc) {
+// relocation test as k_add_plan.  Both directions in one launch (blockIdx.y);
+// their ranks share the class counters.
+struct AppendSide {
+    const RowDesc *desc;
+    const uint32_t *keys;
+    const uint32_t *gstart;
+    const unsigned long long *ngroups;
+    uint64_t *reloc;
+};
+
+__global__ __launch_bounds__(256) void k_add_assign(AppendSide out, AppendSide in, uint64_t count, uint64_t *tail,
+                                                    uint64_t *pool) {
     __shared__ uint32_t hist[kClasses];
     __shared__ unsigned long long base[kClasses];
     if (tail[T_OVERFLOW]) return;
-    const uint64_t ngroups = *ngroups_p;
+    const AppendSide &sd = blockIdx.y ? in : out;
+    const RowDesc *desc = sd.desc;
+    const uint32_t *keys = sd.keys, *gstart = sd.gstart;
+    uint64_t *reloc = sd.reloc;
+    const uint64_t ngroups = *sd.ngroups;
     for (uint64_t g0 = (uint64_t)blockIdx.x * blockDim.x; g0 < ngroups; g0 += (uint64_t)gridDim.x * blockDim.x) {
         for (uint32_t i = threadIdx.x; i < kClasses; i += blockDim.x) hist[i] = 0;
         __syncthreads();
@@ -1287,6 +1304,200 @@ constexpr uint32_t kBigAppend = 4096;   // relocations of larger rows use one 10
 __device__ __forceinline__ uint32_t class_of(uint64_t nd) {
     const uint64_t c = nd < kMinRowCap ? kMinRowCap : nd;
     return 64u - (uint32_t)__clzll((long long)(c - 1));
+}
+
+// ------------------------------------------------- small batches, one launch
+// Batches of up to kSmallSort records (and free pools of up to kSmallSort
+// blocks): one 1,024-thread workgroup per direction sorts the keys in LDS with
+// a bitonic network and writes them with their group heads (and, for appends,
+// the relocation demand per class), in place of the key pass, the device radix
+// sorts, the head compaction and the plan pass.  Sorted keys, payload order
+// (stable, through the record index in the low word) and heads are the same
+// as those of the device-wide path.
+constexpr uint32_t kSmallSort = 4096;
+
+__device__ void lds_bitonic(uint64_t *s, uint32_t N) {   // N a power of two, s[0, N) ascending
+    for (uint32_t k = 2; k <= N; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < N / 2; i += blockDim.x) {
+                const uint32_t lo = 2 * i - (i & (j - 1)), hi = lo + j;
+                const uint64_t a = s[lo], b = s[hi];
+                if ((a > b) == ((lo & k) == 0)) {
+                    s[lo] = b;
+                    s[hi] = a;
+                }
+            }
+            __syncthreads();
+        }
+}
+
+// Heads of the runs of equal rows (high word) of s[0, count), row ~0 forming
+// no group, compacted in order into gstart (and gs when given); every thread
+// returns the group count.
+__device__ uint32_t lds_heads(const uint64_t *s, uint32_t count, uint32_t *gstart, uint32_t *gs, uint32_t *wsum) {
+    const uint32_t w = threadIdx.x / kWave;
+    uint32_t base = 0;
+    for (uint32_t t0 = 0; t0 < count; t0 += 1024) {
+        const uint32_t i = t0 + threadIdx.x;
+        bool head = false;
+        if (i < count) {
+            const uint32_t r = (uint32_t)(s[i] >> 32);
+            head = r != ~0u && (i == 0 || (uint32_t)(s[i - 1] >> 32) != r);
+        }
+        const uint64_t m = __ballot(head);
+        if (lane_id() == 0) wsum[w] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t before = 0, tot = 0;
+        for (uint32_t q = 0; q < 16; ++q) {
+            before += q < w ? wsum[q] : 0u;
+            tot += wsum[q];
+        }
+        if (head) {
+            const uint32_t g = base + before + lanes_below(m);
+            gstart[g] = i;
+            if (gs) gs[g] = i;
+        }
+        base += tot;
+        __syncthreads();
+    }
+    return base;
+}
+
+__device__ __forceinline__ uint32_t pow2_at_least(uint32_t x) {
+    uint32_t N = 1;
+    while (N < x) N <<= 1;
+    return N;
+}
+
+// Delete keys (u << 32 | v) for the out direction (block 0) and (v << 32 | u)
+// for the in direction (block 1); records with an id out of range key ~0 and
+// count as missing.  Also clears the mutation counters and both directions'
+// occurrence counters (used[0, 2 count)).
+__global__ __launch_bounds__(1024) void k_del_prep_small(const dcg_edge *recs, uint32_t count, uint32_t n,
+                                                         uint64_t *kout, uint64_t *kin, uint32_t *gout,
+                                                         uint32_t *gin, uint32_t *used,
+                                                         unsigned long long *counters) {
+    __shared__ uint64_t s[kSmallSort];
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t bad;
+    const bool out = blockIdx.x == 0;
+    if (threadIdx.x == 0) bad = 0;
+    if (out)
+        for (uint32_t j = threadIdx.x; j < C_NUM; j += blockDim.x)
+            if (j != C_MISSING && j != C_GROUPS_OUT && j != C_GROUPS_IN) counters[j] = 0;
+    for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) used[blockIdx.x * count + i] = 0;
+    const uint32_t N = pow2_at_least(count);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < N; i += blockDim.x) {
+        uint64_t k = ~0ull;
+        if (i < count) {
+            const dcg_edge e = recs[i];
+            if (e.u >= n || e.v >= n) {
+                if (out) atomicAdd(&bad, 1u);
+            } else {
+                k = out ? ((uint64_t)e.u << 32) | e.v : ((uint64_t)e.v << 32) | e.u;
+            }
+        }
+        s[i] = k;
+    }
+    __syncthreads();
+    lds_bitonic(s, N);
+    uint64_t *dst = out ? kout : kin;
+    for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) dst[i] = s[i];
+    const uint32_t groups = lds_heads(s, count, out ? gout : gin, nullptr, wsum);
+    if (threadIdx.x == 0) {
+        counters[out ? C_GROUPS_OUT : C_GROUPS_IN] = groups;
+        if (out) counters[C_MISSING] = bad;
+    }
+}
+
+// Append keys: row u with payload v (block 0, out direction), row v with
+// payload u (block 1); invalid records key ~0 and sort last in record order.
+// Sorting (row << 32 | record index) gives the stable order of the pairs.
+// Then the relocation demand per capacity class (the k_add_plan test) from
+// the heads kept in LDS; block 0 also clears the mutation counters.
+__global__ __launch_bounds__(1024) void k_add_prep_small(const dcg_edge *recs, uint32_t count, uint32_t n,
+                                                         uint32_t *kout, uint32_t *vout, uint32_t *kin,
+                                                         uint32_t *vin, uint32_t *gout, uint32_t *gin,
+                                                         const RowDesc *dout, const RowDesc *din, uint64_t *tail,
+                                                         unsigned long long *counters) {
+    __shared__ uint64_t s[kSmallSort];
+    __shared__ uint32_t gs[kSmallSort];
+    __shared__ uint32_t wsum[16], hist[kClasses];
+    __shared__ uint32_t bad;
+    const bool out = blockIdx.x == 0;
+    if (threadIdx.x == 0) bad = 0;
+    for (uint32_t c = threadIdx.x; c < kClasses; c += blockDim.x) hist[c] = 0;
+    if (out)
+        for (uint32_t j = threadIdx.x; j < C_NUM; j += blockDim.x)
+            if (j != C_INVALID && j != C_GROUPS_OUT && j != C_GROUPS_IN) counters[j] = 0;
+    const uint32_t N = pow2_at_least(count);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < N; i += blockDim.x) {
+        uint64_t k = ~0ull;
+        if (i < count) {
+            const dcg_edge e = recs[i];
+            const bool ok = e.u < n && e.v < n;
+            if (!ok) atomicAdd(&bad, 1u);
+            k = ((uint64_t)(ok ? (out ? e.u : e.v) : ~0u) << 32) | i;
+        }
+        s[i] = k;
+    }
+    __syncthreads();
+    lds_bitonic(s, N);
+    uint32_t *kd = out ? kout : kin, *vd = out ? vout : vin;
+    for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) {
+        const uint64_t x = s[i];
+        const dcg_edge e = recs[(uint32_t)x];
+        kd[i] = (uint32_t)(x >> 32);
+        vd[i] = out ? e.v : e.u;
+    }
+    const uint32_t groups = lds_heads(s, count, out ? gout : gin, gs, wsum);
+    const uint32_t valid = count - bad;   // invalid records sort last
+    const RowDesc *desc = out ? dout : din;
+    for (uint32_t g = threadIdx.x; g < groups; g += blockDim.x) {
+        const uint32_t a = gs[g], b = g + 1 < groups ? gs[g + 1] : valid;
+        const RowDesc d = desc[(uint32_t)(s[a] >> 32)];
+        const uint64_t nd = (uint64_t)d.ext + (b - a);
+        if (nd > row_cap(d)) atomicAdd(&hist[class_of(nd)], 1u);
+    }
+    __syncthreads();
+    for (uint32_t c = threadIdx.x; c < kClasses; c += blockDim.x)
+        if (hist[c]) atomicAdd((unsigned long long *)&tail[T_COUNT + c], (unsigned long long)hist[c]);
+    if (threadIdx.x == 0) {
+        counters[out ? C_GROUPS_OUT : C_GROUPS_IN] = groups;
+        if (out) counters[C_INVALID] = bad;
+    }
+}
+
+// Free pool of up to kSmallSort entries: sort by class in LDS, keep the live
+// entries in place, record each class's segment, set the retired cursor past
+// them, then assign blocks per class (k_alloc_classes).  One workgroup.
+__global__ __launch_bounds__(1024) void k_pool_small(uint64_t *pool, uint32_t len, uint32_t live, uint64_t *tail,
+                                                     uint64_t total) {
+    __shared__ uint64_t s[kSmallSort];
+    __shared__ uint32_t pst[kClasses], pen[kClasses];
+    const uint32_t N = pow2_at_least(len);
+    for (uint32_t c = threadIdx.x; c < kClasses; c += blockDim.x) pst[c] = pen[c] = 0;
+    for (uint32_t i = threadIdx.x; i < N; i += blockDim.x) s[i] = i < len ? pool[i] : ~0ull;
+    __syncthreads();
+    lds_bitonic(s, N);
+    for (uint32_t i = threadIdx.x; i < live; i += blockDim.x) {
+        const uint64_t x = s[i];
+        const uint32_t c = (uint32_t)(x >> 40);
+        if (i == 0 || (uint32_t)(s[i - 1] >> 40) != c) pst[c] = i;
+        if (i + 1 == live || (uint32_t)(s[i + 1] >> 40) != c) pen[c] = i + 1;
+        pool[i] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t c = 0; c < (uint32_t)kClasses; ++c) {
+            tail[T_PSTART + c] = pst[c];
+            tail[T_FREE + c] = pen[c] ? pen[c] - pst[c] : 0;
+        }
+        tail[T_RETIRED] = live;
+        alloc_classes(tail, total);
+    }
 }
 
 // Append (and relocation) per touched row.  A wave takes four groups at a
@@ -1426,53 +1637,61 @@ static uint32_t wave_grid(uint64_t groups) {
 }
 
 void Engine::mutate_delete(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
-    fill<unsigned long long>(stream, d_counters.p, C_NUM, 0ull);
-    if (!count) return;
+    if (!count) {
+        fill<unsigned long long>(stream, d_counters.p, C_NUM, 0ull);
+        return;
+    }
     d_k64a.ensure(count);
     d_k64b.ensure(count);
     d_gstart.ensure(count);
     d_gstart2.ensure(count);
-    d_used.ensure(count);
-    d_k64c.ensure(count);
-    d_k64d.ensure(count);
+    d_used.ensure(2 * count);   // occurrence counters, out then in direction
     const dcg_edge *recs = d_dels.p + begin;
-    // unsorted (u,v) and (v,u) keys, then sorted copies in kout / kin
-    hipLaunchKernelGGL(k_del_keys, dim3(grid_for(count)), dim3(256), 0, stream, recs, count, n, d_k64c.p, d_k64d.p,
-                       d_counters.p);
     uint64_t *kout = d_k64a.p, *kin = d_k64b.p;
-    size_t tb = 0;
-    DCG_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, d_k64c.p, kout, (int)count, 0, 64, stream));
-    void *t = temp(tb);
-    DCG_HIP(hipcub::DeviceRadixSort::SortKeys(t, tb, d_k64c.p, kout, (int)count, 0, 64, stream));
-    DCG_HIP(hipcub::DeviceRadixSort::SortKeys(t, tb, d_k64d.p, kin, (int)count, 0, 64, stream));
-    group_heads<uint64_t>(*this, kout, count, d_gstart.p, d_counters.p + C_GROUPS_OUT);
-    group_heads<uint64_t>(*this, kin, count, d_gstart2.p, d_counters.p + C_GROUPS_IN);
+    uint32_t *used_out = d_used.p, *used_in = d_used.p + count;
+    if (count <= kSmallSort) {
+        hipLaunchKernelGGL(k_del_prep_small, dim3(2), dim3(1024), 0, stream, recs, (uint32_t)count, n, kout, kin,
+                           d_gstart.p, d_gstart2.p, d_used.p, d_counters.p);
+    } else {
+        fill<unsigned long long>(stream, d_counters.p, C_NUM, 0ull);
+        d_k64c.ensure(count);
+        d_k64d.ensure(count);
+        // unsorted (u,v) and (v,u) keys, then sorted copies in kout / kin
+        hipLaunchKernelGGL(k_del_keys, dim3(grid_for(count)), dim3(256), 0, stream, recs, count, n, d_k64c.p,
+                           d_k64d.p, d_counters.p);
+        size_t tb = 0;
+        DCG_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, d_k64c.p, kout, (int)count, 0, 64, stream));
+        void *t = temp(tb);
+        DCG_HIP(hipcub::DeviceRadixSort::SortKeys(t, tb, d_k64c.p, kout, (int)count, 0, 64, stream));
+        DCG_HIP(hipcub::DeviceRadixSort::SortKeys(t, tb, d_k64d.p, kin, (int)count, 0, 64, stream));
+        group_heads<uint64_t>(*this, kout, count, d_gstart.p, d_counters.p + C_GROUPS_OUT);
+        group_heads<uint64_t>(*this, kin, count, d_gstart2.p, d_counters.p + C_GROUPS_IN);
+        fill<uint32_t>(stream, d_used.p, 2 * count, 0u);
+    }
     StoreView v = view();
     const uint32_t wg = wave_grid(count);
-    d_big_groups.ensure(count);
-    fill<uint32_t>(stream, d_used.p, count, 0u);
+    d_big_groups.ensure(2 * count);   // large-row lists, out then in direction
+    uint32_t *big_out = d_big_groups.p, *big_in = d_big_groups.p + count;
     // each launch timed on its own (HIP events); the bytes of both directions
     // go to the first launch of each kernel, so sums and averages hold
     const int w0 = time_begin(DCG_K_DELETE_ROWS, -1);
     hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream, v, d_out.p, d_sorted_out.p, d_hot_out.p, kout,
-                       count, d_gstart.p, d_counters.p + C_GROUPS_OUT, d_used.p, d_counters.p, 1, d_big_groups.p,
+                       count, d_gstart.p, d_counters.p + C_GROUPS_OUT, used_out, d_counters.p, 1, big_out,
                        d_counters.p + C_NBIG, d_fbase_out.p, d_fence_out.p, policy_reads());
     time_end(w0);
     const int b0 = time_begin(DCG_K_DELETE_BIG, -1);
     hipLaunchKernelGGL(k_del_big, dim3(kDelBigGrid), dim3(1024), 0, stream, v, d_out.p, d_sorted_out.p, d_hot_out.p,
-                       kout, count, d_big_groups.p, d_counters.p + C_NBIG, d_used.p, d_counters.p, 1, d_fbase_out.p,
+                       kout, count, big_out, d_counters.p + C_NBIG, used_out, d_counters.p, 1, d_fbase_out.p,
                        d_fence_out.p);
     time_end(b0);
-    fill<uint32_t>(stream, d_used.p, count, 0u);
-    fill<unsigned long long>(stream, d_counters.p + C_NBIG, 1, 0ull);
     const int w1 = time_begin(DCG_K_DELETE_ROWS, -1);
     hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream, v, d_in.p, d_sorted_in.p, d_hot_in.p, kin,
-                       count, d_gstart2.p, d_counters.p + C_GROUPS_IN, d_used.p, d_counters.p, 0, d_big_groups.p,
-                       d_counters.p + C_NBIG, d_fbase_in.p, d_fence_in.p, policy_reads());
+                       count, d_gstart2.p, d_counters.p + C_GROUPS_IN, used_in, d_counters.p, 0, big_in,
+                       d_counters.p + C_NBIG_IN, d_fbase_in.p, d_fence_in.p, policy_reads());
     time_end(w1);
     const int b1 = time_begin(DCG_K_DELETE_BIG, -1);
     hipLaunchKernelGGL(k_del_big, dim3(kDelBigGrid), dim3(1024), 0, stream, v, d_in.p, d_sorted_in.p, d_hot_in.p,
-                       kin, count, d_big_groups.p, d_counters.p + C_NBIG, d_used.p, d_counters.p, 0, d_fbase_in.p,
+                       kin, count, big_in, d_counters.p + C_NBIG_IN, used_in, d_counters.p, 0, d_fbase_in.p,
                        d_fence_in.p);
     time_end(b1);
     DCG_HIP(hipGetLastError());
@@ -1501,39 +1720,60 @@ void Engine::finish_delete(uint64_t count, dcg_phase_stats *st) {
 }
 
 void Engine::mutate_add(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
-    fill<unsigned long long>(stream, d_counters.p, C_NUM, 0ull);
-    if (!count) return;
+    if (!count) {
+        fill<unsigned long long>(stream, d_counters.p, C_NUM, 0ull);
+        return;
+    }
     d_k32a.ensure(count);
     d_k32b.ensure(count);
     d_v32a.ensure(count);
     d_v32b.ensure(count);
     d_gstart.ensure(count);
     d_gstart2.ensure(count);
-    d_k32c.ensure(count);
-    d_k32d.ensure(count);
-    d_v32c.ensure(count);
-    d_v32d.ensure(count);
     const dcg_edge *recs = d_adds.p + begin;
-    // unsorted (key, payload) pairs in the c/d buffers, stable-sorted into a/b
-    uint32_t *tk = d_k32c.p, *tv = d_v32c.p, *tk2 = d_k32d.p, *tv2 = d_v32d.p;
-    hipLaunchKernelGGL(k_add_keys, dim3(grid_for(count)), dim3(256), 0, stream, recs, count, n, tk, tv, tk2,
-                       tv2, d_counters.p);
-    size_t tb = 0;
-    DCG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, tk, d_k32a.p, tv, d_v32a.p, (int)count, 0, 32, stream));
-    void *t = temp(tb);
-    DCG_HIP(hipcub::DeviceRadixSort::SortPairs(t, tb, tk, d_k32a.p, tv, d_v32a.p, (int)count, 0, 32, stream));
-    DCG_HIP(hipcub::DeviceRadixSort::SortPairs(t, tb, tk2, d_k32b.p, tv2, d_v32b.p, (int)count, 0, 32, stream));
-    group_heads<uint32_t>(*this, d_k32a.p, count, d_gstart.p, d_counters.p + C_GROUPS_OUT);
-    group_heads<uint32_t>(*this, d_k32b.p, count, d_gstart2.p, d_counters.p + C_GROUPS_IN);
-    hipLaunchKernelGGL(k_add_plan, dim3(grid_for(count, 256, 1024)), dim3(256), 0, stream, d_out.p, d_k32a.p, count,
-                       d_gstart.p, d_counters.p + C_GROUPS_OUT, d_tail.p);
-    hipLaunchKernelGGL(k_add_plan, dim3(grid_for(count, 256, 1024)), dim3(256), 0, stream, d_in.p, d_k32b.p, count,
-                       d_gstart2.p, d_counters.p + C_GROUPS_IN, d_tail.p);
+    if (count <= kSmallSort) {
+        hipLaunchKernelGGL(k_add_prep_small, dim3(2), dim3(1024), 0, stream, recs, (uint32_t)count, n, d_k32a.p,
+                           d_v32a.p, d_k32b.p, d_v32b.p, d_gstart.p, d_gstart2.p, d_out.p, d_in.p, d_tail.p,
+                           d_counters.p);
+    } else {
+        fill<unsigned long long>(stream, d_counters.p, C_NUM, 0ull);
+        d_k32c.ensure(count);
+        d_k32d.ensure(count);
+        d_v32c.ensure(count);
+        d_v32d.ensure(count);
+        // unsorted (key, payload) pairs in the c/d buffers, stable-sorted into a/b
+        uint32_t *tk = d_k32c.p, *tv = d_v32c.p, *tk2 = d_k32d.p, *tv2 = d_v32d.p;
+        hipLaunchKernelGGL(k_add_keys, dim3(grid_for(count)), dim3(256), 0, stream, recs, count, n, tk, tv, tk2,
+                           tv2, d_counters.p);
+        size_t tb = 0;
+        DCG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, tk, d_k32a.p, tv, d_v32a.p, (int)count, 0, 32,
+                                                   stream));
+        void *t = temp(tb);
+        DCG_HIP(hipcub::DeviceRadixSort::SortPairs(t, tb, tk, d_k32a.p, tv, d_v32a.p, (int)count, 0, 32, stream));
+        DCG_HIP(hipcub::DeviceRadixSort::SortPairs(t, tb, tk2, d_k32b.p, tv2, d_v32b.p, (int)count, 0, 32, stream));
+        group_heads<uint32_t>(*this, d_k32a.p, count, d_gstart.p, d_counters.p + C_GROUPS_OUT);
+        group_heads<uint32_t>(*this, d_k32b.p, count, d_gstart2.p, d_counters.p + C_GROUPS_IN);
+        hipLaunchKernelGGL(k_add_plan, dim3(grid_for(count, 256, 1024)), dim3(256), 0, stream, d_out.p, d_k32a.p,
+                           count, d_gstart.p, d_counters.p + C_GROUPS_OUT, d_tail.p);
+        hipLaunchKernelGGL(k_add_plan, dim3(grid_for(count, 256, 1024)), dim3(256), 0, stream, d_in.p, d_k32b.p,
+                           count, d_gstart2.p, d_counters.p + C_GROUPS_IN, d_tail.p);
+    }
     // free pool: live entries and the blocks the previous batch retired, sorted
     // by class (taken entries, ~0, sort last); this batch retires past them
-    fill<uint64_t>(stream, d_tail.p + T_FREE, kClasses, 0ull);
     const uint64_t pool_live = pool_len - pool_taken;
-    if (pool_len) {
+    if (d_pool.n < pool_live + 2 * count + 1) {
+        DevBuf<uint64_t> grown;
+        grown.alloc(pool_live + 2 * count + 1 + pool_live / 2);
+        if (pool_len)
+            DCG_HIP(hipMemcpyAsync(grown.p, d_pool.p, pool_len * 8, hipMemcpyDeviceToDevice, stream));
+        std::swap(grown.p, d_pool.p);
+        std::swap(grown.n, d_pool.n);
+    }
+    if (pool_len <= kSmallSort) {
+        hipLaunchKernelGGL(k_pool_small, dim3(1), dim3(1024), 0, stream, d_pool.p, (uint32_t)pool_len,
+                           (uint32_t)pool_live, d_tail.p, total_slots);
+    } else {
+        fill<uint64_t>(stream, d_tail.p + T_FREE, kClasses, 0ull);
         d_k64c.ensure(pool_len);
         size_t pb = 0;
         DCG_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, pb, d_pool.p, d_k64c.p, (int)pool_len, 0, 64, stream));
@@ -1542,41 +1782,33 @@ void Engine::mutate_add(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
         hipLaunchKernelGGL(k_pool_segments, dim3(grid_for(pool_live, 256, 4096)), dim3(256), 0, stream, d_k64c.p,
                            pool_live, d_tail.p);
         hipLaunchKernelGGL(k_pool_counts, dim3(1), dim3(64), 0, stream, d_tail.p);
+        if (pool_live)
+            DCG_HIP(hipMemcpyAsync(d_pool.p, d_k64c.p, pool_live * 8, hipMemcpyDeviceToDevice, stream));
+        fill<uint64_t>(stream, d_tail.p + T_RETIRED, 1, pool_live);
+        hipLaunchKernelGGL(k_alloc_classes, dim3(1), dim3(64), 0, stream, d_tail.p, total_slots);
     }
-    if (d_pool.n < pool_live + 2 * count + 1) {
-        DevBuf<uint64_t> grown;
-        grown.alloc(pool_live + 2 * count + 1 + pool_live / 2);
-        std::swap(grown.p, d_pool.p);
-        std::swap(grown.n, d_pool.n);
-    }
-    if (pool_live)
-        DCG_HIP(hipMemcpyAsync(d_pool.p, d_k64c.p, pool_live * 8, hipMemcpyDeviceToDevice, stream));
-    fill<uint64_t>(stream, d_tail.p + T_RETIRED, 1, pool_live);
-    hipLaunchKernelGGL(k_alloc_classes, dim3(1), dim3(64), 0, stream, d_tail.p, total_slots);
     d_k64a.ensure(count);
     d_k64b.ensure(count);
     uint64_t *reloc_out = d_k64a.p, *reloc_in = d_k64b.p;
-    hipLaunchKernelGGL(k_add_assign, dim3(grid_for(count, 256, 1024)), dim3(256), 0, stream, d_out.p, d_k32a.p, count,
-                       d_gstart.p, d_counters.p + C_GROUPS_OUT, d_tail.p, d_pool.p, reloc_out);
-    hipLaunchKernelGGL(k_add_assign, dim3(grid_for(count, 256, 1024)), dim3(256), 0, stream, d_in.p, d_k32b.p, count,
-                       d_gstart2.p, d_counters.p + C_GROUPS_IN, d_tail.p, d_pool.p, reloc_in);
+    const AppendSide side_out{d_out.p, d_k32a.p, d_gstart.p, d_counters.p + C_GROUPS_OUT, reloc_out};
+    const AppendSide side_in{d_in.p, d_k32b.p, d_gstart2.p, d_counters.p + C_GROUPS_IN, reloc_in};
+    hipLaunchKernelGGL(k_add_assign, dim3(grid_for(count, 256, 1024), 2), dim3(256), 0, stream, side_out, side_in,
+                       count, d_tail.p, d_pool.p);
     StoreView v = view();
     const uint32_t wg = wave_grid(count);
-    d_big_groups.ensure(count);
+    d_big_groups.ensure(2 * count);   // large-row lists, out then in direction
+    uint32_t *big_out = d_big_groups.p, *big_in = d_big_groups.p + count;
     const int slot = time_begin(DCG_K_APPEND_ROWS, -1);
     hipLaunchKernelGGL(k_add_rows, dim3(wg), dim3(256), 0, stream, v, d_out.p, d_hot_out.p, d_k32a.p, d_v32a.p, count,
                        d_gstart.p, d_counters.p + C_GROUPS_OUT, d_tail.p, d_pool.p, reloc_out, d_counters.p, 1,
-                       d_big_groups.p, d_counters.p + C_NBIG);
+                       big_out, d_counters.p + C_NBIG);
     hipLaunchKernelGGL(k_add_big, dim3(64), dim3(1024), 0, stream, v, d_out.p, d_hot_out.p, d_k32a.p, d_v32a.p, count,
-                       d_gstart.p, d_big_groups.p, d_counters.p + C_NBIG, reloc_out, d_tail.p, d_pool.p, d_counters.p,
-                       1);
-    fill<unsigned long long>(stream, d_counters.p + C_NBIG, 1, 0ull);
+                       d_gstart.p, big_out, d_counters.p + C_NBIG, reloc_out, d_tail.p, d_pool.p, d_counters.p, 1);
     hipLaunchKernelGGL(k_add_rows, dim3(wg), dim3(256), 0, stream, v, d_in.p, d_hot_in.p, d_k32b.p, d_v32b.p, count,
                        d_gstart2.p, d_counters.p + C_GROUPS_IN, d_tail.p, d_pool.p, reloc_in, d_counters.p, 1,
-                       d_big_groups.p, d_counters.p + C_NBIG);
+                       big_in, d_counters.p + C_NBIG_IN);
     hipLaunchKernelGGL(k_add_big, dim3(64), dim3(1024), 0, stream, v, d_in.p, d_hot_in.p, d_k32b.p, d_v32b.p, count,
-                       d_gstart2.p, d_big_groups.p, d_counters.p + C_NBIG, reloc_in, d_tail.p, d_pool.p, d_counters.p,
-                       1);
+                       d_gstart2.p, big_in, d_counters.p + C_NBIG_IN, reloc_in, d_tail.p, d_pool.p, d_counters.p, 1);
     time_end(slot);
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipMemcpyAsync(h_counters, d_counters.p, C_NUM * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
