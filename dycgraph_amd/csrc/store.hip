@@ -1316,16 +1316,44 @@ __device__ __forceinline__ uint32_t class_of(uint64_t nd) {
 // as those of the device-wide path.
 constexpr uint32_t kSmallSort = 4096;
 
+__device__ __forceinline__ void cmp_swap(uint64_t &a, uint64_t &b, bool up) {
+    if ((a > b) == up) {
+        const uint64_t t = a;
+        a = b;
+        b = t;
+    }
+}
+
+// Merge steps j and j/2 of one bitonic stage go together: the four entries
+// o, o + j/2, o + j, o + 3j/2 of a 2j-block are closed under both, so a thread
+// exchanges them in registers and the stage needs half the barriers.
 __device__ void lds_bitonic(uint64_t *s, uint32_t N) {   // N a power of two, s[0, N) ascending
     for (uint32_t k = 2; k <= N; k <<= 1)
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = threadIdx.x; i < N / 2; i += blockDim.x) {
-                const uint32_t lo = 2 * i - (i & (j - 1)), hi = lo + j;
-                const uint64_t a = s[lo], b = s[hi];
-                if ((a > b) == ((lo & k) == 0)) {
-                    s[lo] = b;
-                    s[hi] = a;
+        for (uint32_t j = k >> 1; j > 0;) {
+            if (j >= 2) {
+                const uint32_t h = j >> 1;
+                for (uint32_t i = threadIdx.x; i < N / 4; i += blockDim.x) {
+                    const uint32_t b = 4 * i - 3 * (i & (h - 1));
+                    const bool up = (b & k) == 0;
+                    uint64_t x0 = s[b], x1 = s[b + h], x2 = s[b + j], x3 = s[b + j + h];
+                    cmp_swap(x0, x2, up);
+                    cmp_swap(x1, x3, up);
+                    cmp_swap(x0, x1, up);
+                    cmp_swap(x2, x3, up);
+                    s[b] = x0;
+                    s[b + h] = x1;
+                    s[b + j] = x2;
+                    s[b + j + h] = x3;
                 }
+                j >>= 2;
+            } else {
+                for (uint32_t i = threadIdx.x; i < N / 2; i += blockDim.x) {
+                    uint64_t x0 = s[2 * i], x1 = s[2 * i + 1];
+                    cmp_swap(x0, x1, ((2 * i) & k) == 0);
+                    s[2 * i] = x0;
+                    s[2 * i + 1] = x1;
+                }
+                j = 0;
             }
             __syncthreads();
         }

@@ -21,9 +21,9 @@ def _engine(**kw):
 
 
 def _run_stream(app, n, src, dst, adds_per, dels_per, placement, cache_bytes=0, source=0, wmode=0, seed=1,
-                wmax=100, check_rows=True, stream_seed=7):
+                wmax=100, check_rows=True, stream_seed=7, stream=None):
     import dycgraph_amd as d
-    adds, dels = d.gen_stream(n, src, dst, adds_per, dels_per, seed=stream_seed)
+    adds, dels = stream if stream is not None else d.gen_stream(n, src, dst, adds_per, dels_per, seed=stream_seed)
     eng = _engine(app=app, sources=(source,), weight_mode=wmode, weight_seed=seed, weight_max=wmax,
                   placement=placement, cache_bytes=cache_bytes)
     eng.load_graph(n, src, dst)
@@ -35,6 +35,7 @@ def _run_stream(app, n, src, dst, adds_per, dels_per, placement, cache_bytes=0, 
     assert np.array_equal(got, want), f"{app} static mismatch: {np.sum(got != want)} vertices"
     eng.load_updates(adds, dels)
     ab = db = 0
+    add_stats = []
     for s, (na, nd) in enumerate(zip(adds_per, dels_per)):
         st = eng.del_edge(db, nd)
         m = store.apply(dels[db:db + nd], [])
@@ -69,9 +70,11 @@ def _run_stream(app, n, src, dst, adds_per, dels_per, placement, cache_bytes=0, 
         else:
             assert cc_parents_ok(n, rp, col, got, eng.gather_parent()), f"CC parents, add stage {s}"
         eng.cache_maintain()
+        add_stats.append(st)
         ab += na
         db += nd
     eng.close()
+    return add_stats
 
 
 @pytest.mark.parametrize("app", ["sssp", "bfs", "cc"])
@@ -96,6 +99,30 @@ def test_rmat14_large_batches(app):
     announce(f"rmat14 large batches {app}")
     n, src, dst = rmat_graph(14, 200000, seed=6)
     _run_stream(app, n, src, dst, [20000] * 2, [20000] * 2, placement=1, cache_bytes=1 << 20)
+
+
+@pytest.mark.parametrize("app", ["sssp", "cc"])
+def test_mixed_batch_sizes_out_of_range(app):
+    """A large batch (device-wide sorts, thousands of relocated rows) followed by
+    small ones (one-workgroup LDS path) whose free pool of retired blocks is
+    larger than one workgroup sorts, then a mid-size one; about 1% of the
+    records have an id out of range (additions: source; deletions: either end)
+    and count as invalid additions / missing deletes without touching a row."""
+    announce(f"mixed batch sizes with out-of-range records {app}")
+    import dycgraph_amd as d
+    n, src, dst = rmat_graph(14, 100000, seed=9)
+    sizes = [30000, 700, 5000, 900]
+    adds, dels = d.gen_stream(n, src, dst, sizes, sizes, seed=13)
+    rng = np.random.default_rng(21)
+    ia = rng.choice(len(adds), len(adds) // 100, replace=False)
+    adds[ia, 0] = n + rng.integers(0, 1000, len(ia)).astype(np.uint32)
+    idl = rng.choice(len(dels), len(dels) // 100, replace=False)
+    half = len(idl) // 2
+    dels[idl[:half], 0] = n + rng.integers(0, 1000, half).astype(np.uint32)
+    dels[idl[half:], 1] = n + rng.integers(0, 1000, len(idl) - half).astype(np.uint32)
+    st = _run_stream(app, n, src, dst, sizes, sizes, placement=1, cache_bytes=1 << 20, stream=(adds, dels))
+    # the retired blocks of the first batch form the second batch's free pool
+    assert st[0]["relocated_rows"] > 4096, st[0]["relocated_rows"]
 
 
 def test_rmat16_sssp_seeded_weights():
