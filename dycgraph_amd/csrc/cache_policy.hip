@@ -126,12 +126,13 @@ void Engine::cache_apply_plan(const std::vector<uint64_t> &desired, dcg_cache_st
         load_list.push_back(f);
         load_list.push_back((int64_t)c);
     }
-    DevBuf<int64_t> dl;
-    dl.alloc(std::max<size_t>(load_list.size(), 2));
     if (!load_list.empty()) {
-        DCG_HIP(hipMemcpyAsync(dl.p, load_list.data(), load_list.size() * 8, hipMemcpyHostToDevice, stream));
+        // the list buffer is kept between decisions (the stream synchronises below before it is reused)
+        cache.d_load.ensure(load_list.size());
+        DCG_HIP(hipMemcpyAsync(cache.d_load.p, load_list.data(), load_list.size() * 8, hipMemcpyHostToDevice,
+                               stream));
         hipLaunchKernelGGL(k_move_chunks, dim3(grid_for(load_list.size() / 2, 1, 4096)), dim3(256), 0, stream,
-                           cache.d_frames.p, h_slab, dl.p, (uint32_t)(load_list.size() / 2));
+                           cache.d_frames.p, h_slab, cache.d_load.p, (uint32_t)(load_list.size() / 2));
         DCG_HIP(hipGetLastError());
     }
     DCG_HIP(hipMemcpyAsync(cache.d_chunk_frame.p, cache.chunk_frame.data(), C * 4, hipMemcpyHostToDevice, stream));

@@ -67,6 +67,7 @@ struct CacheState {
     DevBuf<float> d_touch;              // update records touching each chunk, this batch
     DevBuf<float> d_rows_last;          // rows expanded per chunk in the last snapshot
     DevBuf<float> d_score;
+    DevBuf<int64_t> d_load;             // (frame, chunk) pairs of one admission decision
     std::vector<uint32_t> last_use;     // snapshot of last access (LRU tie-break)
     uint64_t loaded = 0, evicted = 0, h2d = 0;
     std::unique_ptr<MlpState, MlpDeleter> mlp;
