@@ -156,9 +156,10 @@ struct Engine {
 
     // ---- phase scratch ----
     DevBuf<uint8_t> d_temp;            // hipcub temporary storage
-    DevBuf<uint64_t> d_k64a, d_k64b, d_k64c, d_k64d;   // delete keys (sorted a/b, unsorted c/d)
-    DevBuf<uint32_t> d_k32a, d_k32b, d_v32a, d_v32b;   // add keys/payloads (sorted)
-    DevBuf<uint32_t> d_k32c, d_k32d, d_v32c, d_v32d;   // add keys/payloads (unsorted)
+    DevBuf<uint64_t> d_k64a, d_k64b;   // sorted delete keys (out then in) / relocation targets
+    DevBuf<uint64_t> d_k64c;           // unsorted delete keys; sorted free pool
+    DevBuf<uint32_t> d_k32a, d_v32a;   // add keys / payloads, sorted (out then in)
+    DevBuf<uint32_t> d_k32c, d_v32c;   // add keys / payloads, unsorted
     DevBuf<uint32_t> d_gstart, d_gstart2, d_uniq;
     DevBuf<uint32_t> d_used;           // per-record occurrence counters
     DevBuf<uint32_t> d_big_groups;     // large-row mutation paths

@@ -361,33 +361,66 @@ void Engine::load_edges_device(uint32_t nn, const uint32_t *d_src, const uint32_
 }
 
 // ------------------------------------------------------------- mutation kernels
-__global__ void k_del_keys(const dcg_edge *recs, uint64_t count, uint32_t n, uint64_t *kout, uint64_t *kin,
+// Keys of both directions in one array, out keys first and in keys with the
+// top bit set, so one sort orders each direction and keeps them apart.  A
+// record with an id out of range keys the top of its direction's range (out:
+// the largest key below the in keys, in: ~0) and counts as missing / invalid;
+// the split pass after the sort restores ~0 for those and clears the top bit.
+constexpr uint64_t kDirBit64 = 1ull << 63, kOutTop64 = kDirBit64 - 1;
+constexpr uint32_t kDirBit32 = 1u << 31, kOutTop32 = kDirBit32 - 1;
+
+__global__ void k_del_keys(const dcg_edge *recs, uint64_t count, uint32_t n, uint64_t *keys,
                            unsigned long long *counters) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count;
          i += (uint64_t)gridDim.x * blockDim.x) {
         const dcg_edge e = recs[i];
         if (e.u >= n || e.v >= n) {
-            kout[i] = ~0ull;
-            kin[i] = ~0ull;
+            keys[i] = kOutTop64;
+            keys[count + i] = ~0ull;
             atomicAdd(&counters[C_MISSING], 1ull);
         } else {
-            kout[i] = ((uint64_t)e.u << 32) | e.v;
-            kin[i] = ((uint64_t)e.v << 32) | e.u;
+            keys[i] = ((uint64_t)e.u << 32) | e.v;
+            keys[count + i] = kDirBit64 | ((uint64_t)e.v << 32) | e.u;
         }
     }
 }
 
-__global__ void k_add_keys(const dcg_edge *recs, uint64_t count, uint32_t n, uint32_t *kout, uint32_t *vout,
-                           uint32_t *kin, uint32_t *vin, unsigned long long *counters) {
+__global__ void k_split_del_keys(uint64_t *keys, uint64_t count) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < 2 * count;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t x = keys[i];
+        if (i < count) {
+            if (x == kOutTop64) keys[i] = ~0ull;
+        } else if (x != ~0ull) {
+            keys[i] = x & ~kDirBit64;
+        }
+    }
+}
+
+// Append pairs: row u with payload v (out), then row v with payload u (in).
+__global__ void k_add_keys(const dcg_edge *recs, uint64_t count, uint32_t n, uint32_t *keys, uint32_t *vals,
+                           unsigned long long *counters) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count;
          i += (uint64_t)gridDim.x * blockDim.x) {
         const dcg_edge e = recs[i];
         const bool ok = e.u < n && e.v < n;
-        kout[i] = ok ? e.u : ~0u;
-        vout[i] = e.v;
-        kin[i] = ok ? e.v : ~0u;
-        vin[i] = e.u;
+        keys[i] = ok ? e.u : kOutTop32;
+        vals[i] = e.v;
+        keys[count + i] = ok ? (kDirBit32 | e.v) : ~0u;
+        vals[count + i] = e.u;
         if (!ok) atomicAdd(&counters[C_INVALID], 1ull);
+    }
+}
+
+__global__ void k_split_add_keys(uint32_t *keys, uint64_t count) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < 2 * count;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t x = keys[i];
+        if (i < count) {
+            if (x == kOutTop32) keys[i] = ~0u;
+        } else if (x != ~0u) {
+            keys[i] = x & ~kDirBit32;
+        }
     }
 }
 
@@ -1315,6 +1348,7 @@ __device__ __forceinline__ uint32_t class_of(uint64_t nd) {
 // (stable, through the record index in the low word) and heads are the same
 // as those of the device-wide path.
 constexpr uint32_t kSmallSort = 4096;
+constexpr uint64_t kMaxBatch = 1ull << 30;   // both directions' keys sort as one array of int-counted items
 
 __device__ __forceinline__ void cmp_swap(uint64_t &a, uint64_t &b, bool up) {
     if ((a > b) == up) {
@@ -1669,29 +1703,28 @@ void Engine::mutate_delete(uint64_t begin, uint64_t count, dcg_phase_stats *st) 
         fill<unsigned long long>(stream, d_counters.p, C_NUM, 0ull);
         return;
     }
-    d_k64a.ensure(count);
-    d_k64b.ensure(count);
+    if (count >= kMaxBatch) throw Error(DCG_ERR_ARG, "update batches hold fewer than 2^30 records");
+    d_k64a.ensure(2 * count);   // sorted keys, out then in direction
     d_gstart.ensure(count);
     d_gstart2.ensure(count);
     d_used.ensure(2 * count);   // occurrence counters, out then in direction
     const dcg_edge *recs = d_dels.p + begin;
-    uint64_t *kout = d_k64a.p, *kin = d_k64b.p;
+    uint64_t *kout = d_k64a.p, *kin = d_k64a.p + count;
     uint32_t *used_out = d_used.p, *used_in = d_used.p + count;
     if (count <= kSmallSort) {
         hipLaunchKernelGGL(k_del_prep_small, dim3(2), dim3(1024), 0, stream, recs, (uint32_t)count, n, kout, kin,
                            d_gstart.p, d_gstart2.p, d_used.p, d_counters.p);
     } else {
         fill<unsigned long long>(stream, d_counters.p, C_NUM, 0ull);
-        d_k64c.ensure(count);
-        d_k64d.ensure(count);
-        // unsorted (u,v) and (v,u) keys, then sorted copies in kout / kin
+        d_k64c.ensure(2 * count);
+        // both directions' keys unsorted in d_k64c, one sort into kout | kin
         hipLaunchKernelGGL(k_del_keys, dim3(grid_for(count)), dim3(256), 0, stream, recs, count, n, d_k64c.p,
-                           d_k64d.p, d_counters.p);
+                           d_counters.p);
         size_t tb = 0;
-        DCG_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, d_k64c.p, kout, (int)count, 0, 64, stream));
+        DCG_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, d_k64c.p, kout, (int)(2 * count), 0, 64, stream));
         void *t = temp(tb);
-        DCG_HIP(hipcub::DeviceRadixSort::SortKeys(t, tb, d_k64c.p, kout, (int)count, 0, 64, stream));
-        DCG_HIP(hipcub::DeviceRadixSort::SortKeys(t, tb, d_k64d.p, kin, (int)count, 0, 64, stream));
+        DCG_HIP(hipcub::DeviceRadixSort::SortKeys(t, tb, d_k64c.p, kout, (int)(2 * count), 0, 64, stream));
+        hipLaunchKernelGGL(k_split_del_keys, dim3(grid_for(2 * count)), dim3(256), 0, stream, kout, count);
         group_heads<uint64_t>(*this, kout, count, d_gstart.p, d_counters.p + C_GROUPS_OUT);
         group_heads<uint64_t>(*this, kin, count, d_gstart2.p, d_counters.p + C_GROUPS_IN);
         fill<uint32_t>(stream, d_used.p, 2 * count, 0u);
@@ -1752,38 +1785,36 @@ void Engine::mutate_add(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
         fill<unsigned long long>(stream, d_counters.p, C_NUM, 0ull);
         return;
     }
-    d_k32a.ensure(count);
-    d_k32b.ensure(count);
-    d_v32a.ensure(count);
-    d_v32b.ensure(count);
+    if (count >= kMaxBatch) throw Error(DCG_ERR_ARG, "update batches hold fewer than 2^30 records");
+    d_k32a.ensure(2 * count);   // sorted keys and payloads, out then in direction
+    d_v32a.ensure(2 * count);
     d_gstart.ensure(count);
     d_gstart2.ensure(count);
     const dcg_edge *recs = d_adds.p + begin;
+    uint32_t *kout = d_k32a.p, *kin = d_k32a.p + count, *vout = d_v32a.p, *vin = d_v32a.p + count;
     if (count <= kSmallSort) {
-        hipLaunchKernelGGL(k_add_prep_small, dim3(2), dim3(1024), 0, stream, recs, (uint32_t)count, n, d_k32a.p,
-                           d_v32a.p, d_k32b.p, d_v32b.p, d_gstart.p, d_gstart2.p, d_out.p, d_in.p, d_tail.p,
-                           d_counters.p);
+        hipLaunchKernelGGL(k_add_prep_small, dim3(2), dim3(1024), 0, stream, recs, (uint32_t)count, n, kout, vout,
+                           kin, vin, d_gstart.p, d_gstart2.p, d_out.p, d_in.p, d_tail.p, d_counters.p);
     } else {
         fill<unsigned long long>(stream, d_counters.p, C_NUM, 0ull);
-        d_k32c.ensure(count);
-        d_k32d.ensure(count);
-        d_v32c.ensure(count);
-        d_v32d.ensure(count);
-        // unsorted (key, payload) pairs in the c/d buffers, stable-sorted into a/b
-        uint32_t *tk = d_k32c.p, *tv = d_v32c.p, *tk2 = d_k32d.p, *tv2 = d_v32d.p;
-        hipLaunchKernelGGL(k_add_keys, dim3(grid_for(count)), dim3(256), 0, stream, recs, count, n, tk, tv, tk2,
-                           tv2, d_counters.p);
+        d_k32c.ensure(2 * count);
+        d_v32c.ensure(2 * count);
+        // both directions' (key, payload) pairs unsorted in the c buffers, one
+        // stable sort into kout | kin (payloads into vout | vin)
+        hipLaunchKernelGGL(k_add_keys, dim3(grid_for(count)), dim3(256), 0, stream, recs, count, n, d_k32c.p,
+                           d_v32c.p, d_counters.p);
         size_t tb = 0;
-        DCG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, tk, d_k32a.p, tv, d_v32a.p, (int)count, 0, 32,
-                                                   stream));
+        DCG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, d_k32c.p, kout, d_v32c.p, vout, (int)(2 * count), 0,
+                                                   32, stream));
         void *t = temp(tb);
-        DCG_HIP(hipcub::DeviceRadixSort::SortPairs(t, tb, tk, d_k32a.p, tv, d_v32a.p, (int)count, 0, 32, stream));
-        DCG_HIP(hipcub::DeviceRadixSort::SortPairs(t, tb, tk2, d_k32b.p, tv2, d_v32b.p, (int)count, 0, 32, stream));
-        group_heads<uint32_t>(*this, d_k32a.p, count, d_gstart.p, d_counters.p + C_GROUPS_OUT);
-        group_heads<uint32_t>(*this, d_k32b.p, count, d_gstart2.p, d_counters.p + C_GROUPS_IN);
-        hipLaunchKernelGGL(k_add_plan, dim3(grid_for(count, 256, 1024)), dim3(256), 0, stream, d_out.p, d_k32a.p,
+        DCG_HIP(hipcub::DeviceRadixSort::SortPairs(t, tb, d_k32c.p, kout, d_v32c.p, vout, (int)(2 * count), 0, 32,
+                                                   stream));
+        hipLaunchKernelGGL(k_split_add_keys, dim3(grid_for(2 * count)), dim3(256), 0, stream, kout, count);
+        group_heads<uint32_t>(*this, kout, count, d_gstart.p, d_counters.p + C_GROUPS_OUT);
+        group_heads<uint32_t>(*this, kin, count, d_gstart2.p, d_counters.p + C_GROUPS_IN);
+        hipLaunchKernelGGL(k_add_plan, dim3(grid_for(count, 256, 1024)), dim3(256), 0, stream, d_out.p, kout,
                            count, d_gstart.p, d_counters.p + C_GROUPS_OUT, d_tail.p);
-        hipLaunchKernelGGL(k_add_plan, dim3(grid_for(count, 256, 1024)), dim3(256), 0, stream, d_in.p, d_k32b.p,
+        hipLaunchKernelGGL(k_add_plan, dim3(grid_for(count, 256, 1024)), dim3(256), 0, stream, d_in.p, kin,
                            count, d_gstart2.p, d_counters.p + C_GROUPS_IN, d_tail.p);
     }
     // free pool: live entries and the blocks the previous batch retired, sorted
@@ -1818,8 +1849,8 @@ void Engine::mutate_add(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
     d_k64a.ensure(count);
     d_k64b.ensure(count);
     uint64_t *reloc_out = d_k64a.p, *reloc_in = d_k64b.p;
-    const AppendSide side_out{d_out.p, d_k32a.p, d_gstart.p, d_counters.p + C_GROUPS_OUT, reloc_out};
-    const AppendSide side_in{d_in.p, d_k32b.p, d_gstart2.p, d_counters.p + C_GROUPS_IN, reloc_in};
+    const AppendSide side_out{d_out.p, kout, d_gstart.p, d_counters.p + C_GROUPS_OUT, reloc_out};
+    const AppendSide side_in{d_in.p, kin, d_gstart2.p, d_counters.p + C_GROUPS_IN, reloc_in};
     hipLaunchKernelGGL(k_add_assign, dim3(grid_for(count, 256, 1024), 2), dim3(256), 0, stream, side_out, side_in,
                        count, d_tail.p, d_pool.p);
     StoreView v = view();
@@ -1827,15 +1858,15 @@ void Engine::mutate_add(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
     d_big_groups.ensure(2 * count);   // large-row lists, out then in direction
     uint32_t *big_out = d_big_groups.p, *big_in = d_big_groups.p + count;
     const int slot = time_begin(DCG_K_APPEND_ROWS, -1);
-    hipLaunchKernelGGL(k_add_rows, dim3(wg), dim3(256), 0, stream, v, d_out.p, d_hot_out.p, d_k32a.p, d_v32a.p, count,
+    hipLaunchKernelGGL(k_add_rows, dim3(wg), dim3(256), 0, stream, v, d_out.p, d_hot_out.p, kout, vout, count,
                        d_gstart.p, d_counters.p + C_GROUPS_OUT, d_tail.p, d_pool.p, reloc_out, d_counters.p, 1,
                        big_out, d_counters.p + C_NBIG);
-    hipLaunchKernelGGL(k_add_big, dim3(64), dim3(1024), 0, stream, v, d_out.p, d_hot_out.p, d_k32a.p, d_v32a.p, count,
+    hipLaunchKernelGGL(k_add_big, dim3(64), dim3(1024), 0, stream, v, d_out.p, d_hot_out.p, kout, vout, count,
                        d_gstart.p, big_out, d_counters.p + C_NBIG, reloc_out, d_tail.p, d_pool.p, d_counters.p, 1);
-    hipLaunchKernelGGL(k_add_rows, dim3(wg), dim3(256), 0, stream, v, d_in.p, d_hot_in.p, d_k32b.p, d_v32b.p, count,
+    hipLaunchKernelGGL(k_add_rows, dim3(wg), dim3(256), 0, stream, v, d_in.p, d_hot_in.p, kin, vin, count,
                        d_gstart2.p, d_counters.p + C_GROUPS_IN, d_tail.p, d_pool.p, reloc_in, d_counters.p, 1,
                        big_in, d_counters.p + C_NBIG_IN);
-    hipLaunchKernelGGL(k_add_big, dim3(64), dim3(1024), 0, stream, v, d_in.p, d_hot_in.p, d_k32b.p, d_v32b.p, count,
+    hipLaunchKernelGGL(k_add_big, dim3(64), dim3(1024), 0, stream, v, d_in.p, d_hot_in.p, kin, vin, count,
                        d_gstart2.p, big_in, d_counters.p + C_NBIG_IN, reloc_in, d_tail.p, d_pool.p, d_counters.p, 1);
     time_end(slot);
     DCG_HIP(hipGetLastError());
