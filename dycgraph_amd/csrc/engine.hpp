@@ -120,6 +120,7 @@ struct Engine {
     // ---- vertex state ----
     DevBuf<uint64_t> d_st;             // Q*n packed (value<<32 | parent); CC: (label<<32 | depth)
     DevBuf<uint32_t> d_par;            // Q*n CC parents (allocated when a query runs CC)
+    DevBuf<uint8_t> d_cc_dirty;        // per 1,024 state slots: stamped by a CC improvement since the last parent check
     DevBuf<uint32_t> d_far0, d_far1, d_fstamp;   // bucketed push closure: far piles, pile epochs
     DevBuf<uint32_t> d_thr, d_minfar, d_fdelta, d_epoch;
     void update_buckets();

@@ -26,6 +26,7 @@
 namespace dcg {
 
 enum Mode : uint32_t { M_PUSH = 0, M_PULL = 1, M_INVALIDATE = 2, M_PR = 3, M_PR_AMEND = 4 };
+constexpr uint32_t kDirtyLog2 = 10;   // CC parent-check block: 1,024 state slots
 
 struct TravArgs {
     StoreView s;
@@ -33,6 +34,7 @@ struct TravArgs {
     uint64_t *st;
     uint32_t *par;             // CC parents (q*n + v), null without CC queries
     uint32_t *tick;
+    uint8_t *cc_dirty;         // per 1,024 slots: a CC improvement stamped one of them (null without CC)
     const uint32_t *sources;
     const uint32_t *qapp;      // per-query program
     WeightParams w;            // SSSP weights (BFS: 1, CC: 0)
@@ -146,6 +148,7 @@ __device__ __forceinline__ uint32_t push_op(const TravArgs &a, uint32_t q, uint3
         if (cand >= old) return 0;
         a.par[gd] = s;
         if (atomicExch(&a.tick[gd], stamp) == stamp) return 0;
+        a.cc_dirty[gd >> kDirtyLog2] = 1;
         out_g = (uint32_t)gd;
         return 1;
     }
@@ -909,39 +912,51 @@ __global__ void k_reset_affected(uint64_t *st, const uint32_t *aff, const Ctl *c
 // After a closure every CC vertex it touched (tick >= stamp_lo, or on the
 // affected list) has its parent checked: kept if tight (equal label, depth - 1),
 // else replaced by the first tight in-neighbour; depth 0 is a root (none).
+__device__ __forceinline__ void cc_fix_one(const TravArgs &a, uint32_t g) {
+    uint32_t q, v;
+    decode_g(a, g, q, v);
+    if (a.qapp[q] != DCG_APP_CC) return;
+    const uint64_t qoff = (uint64_t)q * a.n;
+    const uint64_t sv = a.st[g];
+    const uint32_t p = a.par[g];
+    if ((uint32_t)sv == 0) {
+        if (p != kInf) a.par[g] = kInf;
+        return;
+    }
+    if (p != kInf && a.st[qoff + p] + 1 == sv) return;
+    const RowDesc d = a.s.in[v];
+    uint32_t best = kInf;
+    for (uint32_t j = 0; j < d.ext; ++j) {
+        const uint32_t x = *row_base(a.s, d.off + j);
+        if (!(x & kTomb) && a.st[qoff + x] + 1 == sv) {
+            best = x;
+            break;
+        }
+    }
+    a.par[g] = best;
+}
+
+// With a list: the listed slots (one thread each).  Without: query slots
+// [base, base + n) stamped since stamp_lo, visiting only the 1,024-slot blocks
+// that a CC improvement marked (one workgroup per block); the marks are
+// cleared after every CC query of the stage was checked.
 __global__ void k_cc_fix_parents(TravArgs a, const uint32_t *list, const Ctl *list_ctl, uint32_t base) {
     const uint32_t stamp_lo = stamp_base(a);
-    const uint64_t total = list ? (uint64_t)list_ctl->aff_count : (uint64_t)a.n;
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        uint32_t g;
-        if (list) {
-            g = list[i];
-        } else {
-            g = base + (uint32_t)i;
-            if (a.tick[g] < stamp_lo) continue;
+    if (list) {
+        const uint64_t total = (uint64_t)list_ctl->aff_count;
+        for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
+             i += (uint64_t)gridDim.x * blockDim.x)
+            cc_fix_one(a, list[i]);
+        return;
+    }
+    const uint64_t lo = base, hi = (uint64_t)base + a.n;
+    for (uint64_t b = (lo >> kDirtyLog2) + blockIdx.x; (b << kDirtyLog2) < hi; b += gridDim.x) {
+        if (!a.cc_dirty[b]) continue;   // the same byte for the whole workgroup
+        for (uint32_t j = threadIdx.x; j < (1u << kDirtyLog2); j += blockDim.x) {
+            const uint64_t g = (b << kDirtyLog2) + j;
+            if (g < lo || g >= hi || a.tick[g] < stamp_lo) continue;
+            cc_fix_one(a, (uint32_t)g);
         }
-        uint32_t q, v;
-        decode_g(a, g, q, v);
-        if (a.qapp[q] != DCG_APP_CC) continue;
-        const uint64_t qoff = (uint64_t)q * a.n;
-        const uint64_t sv = a.st[g];
-        const uint32_t p = a.par[g];
-        if ((uint32_t)sv == 0) {
-            if (p != kInf) a.par[g] = kInf;
-            continue;
-        }
-        if (p != kInf && a.st[qoff + p] + 1 == sv) continue;
-        const RowDesc d = a.s.in[v];
-        uint32_t best = kInf;
-        for (uint32_t j = 0; j < d.ext; ++j) {
-            const uint32_t x = *row_base(a.s, d.off + j);
-            if (!(x & kTomb) && a.st[qoff + x] + 1 == sv) {
-                best = x;
-                break;
-            }
-        }
-        a.par[g] = best;
     }
 }
 
@@ -987,7 +1002,8 @@ static void check_launch(const Ctl &c, const char *stage) {
 
 // `a`'s stamp base (stamp0, or its step slot) is the first stamp of the phase.
 // With a list: the listed vertex-query slots.  Without: the slots of each CC
-// query stamped in this stage (one pass over that query's n vertices).
+// query stamped in this stage, in the 1,024-slot blocks a CC improvement
+// marked; the marks are cleared once every CC query was checked.
 static void cc_fix_parents(Engine &e, const TravArgs &a, const uint32_t *list, const Ctl *lc) {
     if (!e.d_par.p) return;
     if (list) {
@@ -996,10 +1012,12 @@ static void cc_fix_parents(Engine &e, const TravArgs &a, const uint32_t *list, c
                            0u);
         return;
     }
+    const uint64_t blocks = (e.n >> kDirtyLog2) + 2;   // a query's range spans at most this many blocks
     for (uint32_t q = 0; q < e.Q; ++q)
         if (e.qapps[q] == DCG_APP_CC)
-            hipLaunchKernelGGL(k_cc_fix_parents, dim3(grid_for(e.n, 256, 16384)), dim3(256), 0, e.stream, a, list, lc,
-                               q * e.n);
+            hipLaunchKernelGGL(k_cc_fix_parents, dim3(grid_for(blocks, 1, 16384)), dim3(256), 0, e.stream, a, list,
+                               lc, q * e.n);
+    fill<uint8_t>(e.stream, e.d_cc_dirty.p, e.d_cc_dirty.n, (uint8_t)0);
 }
 
 uint32_t Engine::coop_blocks(const void *kernel) {
@@ -1071,6 +1089,7 @@ static TravArgs base_args(Engine &e) {
     a.st = e.d_st.p;
     a.par = e.d_par.p;
     a.tick = e.d_tick.p;
+    a.cc_dirty = e.d_cc_dirty.p;
     a.sources = e.d_sources.p;
     a.qapp = e.d_qapps.p;
     a.w = e.wp;
@@ -1142,7 +1161,12 @@ void Engine::alloc_state() {
         d_aff.alloc(cap);
         bool cc = false, ordered = false;
         for (uint32_t q = 0; q < Q; ++q) (qapps[q] == DCG_APP_CC ? cc : ordered) = true;
-        if (cc) d_par.alloc(cap);
+        if (cc) {
+            d_par.alloc(cap);
+            const uint64_t blocks = (cap >> kDirtyLog2) + 1;
+            d_cc_dirty.alloc(blocks);
+            fill<uint8_t>(stream, d_cc_dirty.p, blocks, (uint8_t)0);
+        }
         if (ordered) {
             d_far0.alloc(cap);
             d_far1.alloc(cap);
