@@ -211,6 +211,46 @@ def test_mixed_programs_one_engine(graphs):
     eng.close()
 
 
+def test_cc_queries_at_unaligned_ranges():
+    """Three CC queries between SSSP / BFS ones on 3,001 vertices: each CC
+    query's slot range starts off a 1,024-slot boundary and blocks straddle two
+    queries, so the parent check's block marks are shared between queries.
+    Labels bit-exact and parents rooted witnesses after every stage."""
+    announce("cc queries at unaligned slot ranges, n=3001")
+    import dycgraph_amd as d
+    _, src, dst = rmat_graph(12, 40000, seed=4)
+    n = 3001
+    keep = (src < n) & (dst < n)
+    src, dst = src[keep], dst[keep]
+    adds, dels = d.gen_stream(n, src, dst, [300] * 3, [300] * 3, seed=8)
+    apps = ("cc", "sssp", "cc", "bfs", "cc")
+    sources = (0, 5, 0, 7, 0)
+    eng = _engine(app="sssp", sources=sources, query_apps=apps, weight_mode=1, weight_seed=2, weight_max=40)
+    eng.load_graph(n, src, dst)
+    eng.start()
+    eng.load_updates(adds, dels)
+    store = oracle.Store(n, src, dst)
+
+    def check(stage):
+        rp, col = store.csr()
+        for q, (app, s) in enumerate(zip(apps, sources)):
+            want = oracle_values(app, n, rp, col, s, oracle.W_SEEDED, 2, 40)
+            got = eng.gather_value(q)
+            assert np.array_equal(got, want), (stage, q, app, int(np.sum(got != want)))
+            if app == "cc":
+                assert cc_parents_ok(n, rp, col, got, eng.gather_parent(q)), (stage, q)
+
+    check("static")
+    for s in range(3):
+        eng.del_edge(300 * s, 300)
+        store.apply(dels[300 * s:300 * (s + 1)], [])
+        check(f"delete {s}")
+        eng.add_edge(300 * s, 300)
+        store.apply([], adds[300 * s:300 * (s + 1)])
+        check(f"add {s}")
+    eng.close()
+
+
 def test_autotune_keeps_values_and_picks_candidates():
     announce("autotune rmat14 sssp")
     n, src, dst = rmat_graph(14, 200000, seed=2)
