@@ -148,7 +148,8 @@ int dcg_load_updates(dcg_engine *eng, const dcg_edge *adds, uint64_t num_adds,
 
 /* Delete stage of one snapshot: mutation, invalidation, repair.  Replaces
  * Engine::del_edge / update_tree_del (framework.cuh:2595-2618, 4811-4957).
- * [begin, begin+count) indexes the staged deletions. */
+ * [begin, begin+count) indexes the staged deletions; a batch holds fewer than
+ * 2^30 records (DCG_ERR_ARG otherwise; the same bound applies to additions). */
 int dcg_del_edge(dcg_engine *eng, uint64_t begin, uint64_t count, dcg_phase_stats *stats);
 /* Add stage: mutation, seeding, closure.  Replaces Engine::add_edge /
  * update_tree_add (framework.cuh:3525-3756). */
