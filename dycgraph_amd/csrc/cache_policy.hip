@@ -215,6 +215,29 @@ std::vector<uint64_t> Engine::plan_admission(const std::vector<float> &score, do
     return keep;
 }
 
+// After the static convergence: admit the chunks with the most line reads in
+// it, without the per-decision budget (the reference's LoadCache after Start
+// admits by hotness; framework.cuh:1321-1350).  Ties keep resident chunks, then
+// the lower chunk id.  The reads stay in the window of the first decision.
+void Engine::cache_admit_measured() {
+    if (cfg.placement == DCG_PLACE_DEVICE || cache.frames == 0 || cache.frames >= cache.chunks) return;
+    const uint64_t C = cache.chunks, F = cache.frames;
+    std::vector<float> reads(C);
+    DCG_HIP(hipMemcpyAsync(reads.data(), cache.d_reads.p, C * 4, hipMemcpyDeviceToHost, stream));
+    sync();
+    std::vector<uint64_t> order(C);
+    for (uint64_t c = 0; c < C; ++c) order[c] = c;
+    auto better = [&](uint64_t a, uint64_t b) {
+        if (reads[a] != reads[b]) return reads[a] > reads[b];
+        const bool ra = cache.chunk_frame[a] >= 0, rb = cache.chunk_frame[b] >= 0;
+        if (ra != rb) return ra;
+        return a < b;
+    };
+    std::nth_element(order.begin(), order.begin() + (ptrdiff_t)F, order.end(), better);
+    order.resize(F);
+    cache_apply_plan(order, nullptr);
+}
+
 void Engine::cache_maintain(dcg_cache_stats *st) {
     const double t0 = now_ms();
     if (st) *st = dcg_cache_stats{};

@@ -242,6 +242,7 @@ int dcg_start(dcg_engine *h) {
         DCG_HIP(hipMemcpyAsync(e.d_qapps.p, e.qapps.data(), e.qapps.size() * 4, hipMemcpyHostToDevice, e.stream));
         if (e.app == DCG_APP_PR) e.start_pagerank();
         else e.start_min_lattice();
+        e.cache_admit_measured();
         e.collect_timing();
         e.started = true;
     });

@@ -230,6 +230,7 @@ struct Engine {
 
     // cache_policy.hip
     void cache_init();
+    void cache_admit_measured();
     void cache_maintain(dcg_cache_stats *st);
     void cache_prefetch(uint64_t ab, uint64_t ac, uint64_t db, uint64_t dc, dcg_cache_stats *st);
     void cache_apply_plan(const std::vector<uint64_t> &desired, dcg_cache_stats *st);
