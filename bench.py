@@ -42,14 +42,20 @@ KERNEL_SYMBOL = {"push": "k_persistent<0u>", "pull": "k_persistent<1u>", "invali
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic.json")
 
 
-def host_link(k, host_lps):
+def host_link(k, host_lps, delete_pass=None):
     """The dominant kernel against the host link's random-read ceiling: 64-byte lines
-    it read from pinned host memory per launch, their rate, and the measured ceiling."""
-    if not k.get("host_lines") or not k["ms"]:
+    it read from pinned host memory, their rate, and the measured ceiling.  The delete
+    mutation's out-row and in-row passes run concurrently, so for them the rate is
+    taken over the wall time of both passes together (one record per snapshot)."""
+    src, scope = k, "per launch"
+    if delete_pass and delete_pass.get("launches") and delete_pass.get("host_lines"):
+        src, scope = delete_pass, "both delete passes of a snapshot, run concurrently (wall time)"
+    if not src.get("host_lines") or not src["ms"]:
         return None
-    rate = k["host_lines"] / (k["ms"] / 1e3)
-    return {"bound": "random 64-byte reads of pinned host memory", "lines_per_launch":
-            round(k["host_lines"] / max(k["launches"], 1)), "achieved_lines_per_s": round(rate),
+    rate = src["host_lines"] / (src["ms"] / 1e3)
+    return {"bound": "random 64-byte reads of pinned host memory", "scope": scope,
+            "lines_per_unit": round(src["host_lines"] / max(src["launches"], 1)),
+            "ms_per_unit": round(src["ms"] / max(src["launches"], 1), 4), "achieved_lines_per_s": round(rate),
             "peak_lines_per_s": round(host_lps), "frac": round(rate / host_lps, 4) if host_lps else None}
 
 
@@ -301,7 +307,8 @@ def main():
     value = upd / res["total_s"]
     lat = np.array(res["lat"])
     ks = res["ks"]
-    dom = max(ks.items(), key=lambda kv: kv[1]["ms"])
+    # the delete-pass record spans launches of two kernels; it is not a kernel itself
+    dom = max(((n, v) for n, v in ks.items() if n != "delete_pass"), key=lambda kv: kv[1]["ms"])
     kname, k = dom
     achieved = (k["algo_bytes"] / (k["ms"] / 1e3) / 1e9) if k["ms"] > 0 else 0.0
     reads = res["hits"] + res["host"]
@@ -326,7 +333,7 @@ def main():
                      "traffic": measured_traffic(args.config, kname),
                      "launches": k["launches"], "avg_launch_ms": round(k["ms"] / max(k["launches"], 1), 4),
                      "algo_bytes_per_launch": round(k["algo_bytes"] / max(k["launches"], 1), 1),
-                     "host_link": host_link(k, host_lps)},
+                     "host_link": host_link(k, host_lps, ks.get("delete_pass") if kname == "delete_rows" else None)},
         "per_snapshot": res["closure"],
         "kernels": {name: {"ms": round(v["ms"], 3), "launches": v["launches"],
                            "GBps": round(v["algo_bytes"] / (v["ms"] / 1e3) / 1e9, 1) if v["ms"] else 0.0}

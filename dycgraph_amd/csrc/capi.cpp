@@ -19,6 +19,9 @@ Engine::Engine(const dcg_config &c) : cfg(c) {
     device = c.device;
     DCG_HIP(hipSetDevice(device));
     DCG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    DCG_HIP(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
+    DCG_HIP(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+    DCG_HIP(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming));
     hipDeviceProp_t prop;
     DCG_HIP(hipGetDeviceProperties(&prop, device));
     cu_count = prop.multiProcessorCount;
@@ -46,7 +49,7 @@ Engine::Engine(const dcg_config &c) : cfg(c) {
     wp.kind = c.weight_mode == DCG_W_SEEDED ? W_SEEDED : W_REFERENCE;
 }
 
-int Engine::time_begin(int kernel, int ctl, double bytes) {
+int Engine::time_begin(int kernel, int ctl, double bytes, hipStream_t on) {
     const size_t slot = timed.size();
     if (2 * slot + 2 > events.size()) {
         for (int i = 0; i < 32; ++i) {
@@ -55,12 +58,12 @@ int Engine::time_begin(int kernel, int ctl, double bytes) {
             events.push_back(ev);
         }
     }
-    timed.push_back(Timed{kernel, (int)(2 * slot), ctl, bytes});
-    DCG_HIP(hipEventRecord(events[2 * slot], stream));
+    timed.push_back(Timed{kernel, (int)(2 * slot), ctl, bytes, 0, on ? on : stream});
+    DCG_HIP(hipEventRecord(events[2 * slot], on ? on : stream));
     return (int)slot;
 }
 
-void Engine::time_end(int slot) { DCG_HIP(hipEventRecord(events[timed[slot].ev + 1], stream)); }
+void Engine::time_end(int slot) { DCG_HIP(hipEventRecord(events[timed[slot].ev + 1], timed[slot].on)); }
 
 // Fold recorded launches into kstat (call after the stream has synchronised).
 void Engine::collect_timing() {
@@ -93,6 +96,7 @@ void Engine::collect_timing() {
 
 Engine::~Engine() {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (stream2) (void)hipStreamSynchronize(stream2);
     for (hipEvent_t ev : events) (void)hipEventDestroy(ev);
     if (h_slab) (void)hipHostFree(h_slab);
     if (h_ctl) (void)hipHostFree(h_ctl);
@@ -101,6 +105,9 @@ Engine::~Engine() {
     if (h_counters) (void)hipHostFree(h_counters);
     if (h_tail) (void)hipHostFree(h_tail);
     if (h_fold) (void)hipHostFree(h_fold);
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+    if (join_ev) (void)hipEventDestroy(join_ev);
+    if (stream2) (void)hipStreamDestroy(stream2);
     if (stream) (void)hipStreamDestroy(stream);
 }
 

@@ -78,6 +78,8 @@ struct Engine {
     std::string err;
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;          // the in-row delete pass runs here, beside the out-row pass
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     int cu_count = 0;
     uint32_t coop_grid[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     struct Tune {
@@ -171,7 +173,7 @@ struct Engine {
     static constexpr int kTailWords = 3 + 6 * kClasses;   // store tail block (u64 words)
     uint64_t *h_tail = nullptr;                            // pinned copy of it after an add mutation
     unsigned long long *h_fold = nullptr;                  // pinned: cached / host edge reads of a fold
-    int mut_slot[4] = {-1, -1, -1, -1};                    // timing slots of the last delete mutation
+    int mut_slot[5] = {-1, -1, -1, -1, -1};                // timing slots of the last delete mutation
 
     // cumulative counters
     uint64_t cum_cached_reads = 0, cum_host_reads = 0;
@@ -183,11 +185,12 @@ struct Engine {
         int ctl;       // Ctl block index whose counters price the launch (-1: mutation)
         double bytes;  // preset bytes (mutation kernels)
         uint64_t host_lines = 0;
+        hipStream_t on = nullptr;   // stream the launch and its events are on
     };
     dcg_kernel_stats kstat[DCG_K_NUM] = {};
     std::vector<hipEvent_t> events;   // pairs (start, end)
     std::vector<Timed> timed;
-    int time_begin(int kernel, int ctl, double bytes = 0);
+    int time_begin(int kernel, int ctl, double bytes = 0, hipStream_t on = nullptr);   // on: default the engine stream
     void time_end(int slot);
     void collect_timing();
 

@@ -1734,7 +1734,13 @@ void Engine::mutate_delete(uint64_t begin, uint64_t count, dcg_phase_stats *st) 
     d_big_groups.ensure(2 * count);   // large-row lists, out then in direction
     uint32_t *big_out = d_big_groups.p, *big_in = d_big_groups.p + count;
     // each launch timed on its own (HIP events); the bytes of both directions
-    // go to the first launch of each kernel, so sums and averages hold
+    // go to the first launch of each kernel, so sums and averages hold.  The
+    // in-row pass runs on stream2 beside the out-row pass: the two touch
+    // disjoint rows and have their own occurrence counters and large-row lists
+    // (shared counters and read tallies are atomics).
+    const int pass = time_begin(DCG_K_DELETE_PASS, -1);
+    DCG_HIP(hipEventRecord(fork_ev, stream));
+    DCG_HIP(hipStreamWaitEvent(stream2, fork_ev, 0));
     const int w0 = time_begin(DCG_K_DELETE_ROWS, -1);
     hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream, v, d_out.p, d_sorted_out.p, d_hot_out.p, kout,
                        count, d_gstart.p, d_counters.p + C_GROUPS_OUT, used_out, d_counters.p, 1, big_out,
@@ -1745,22 +1751,26 @@ void Engine::mutate_delete(uint64_t begin, uint64_t count, dcg_phase_stats *st) 
                        kout, count, big_out, d_counters.p + C_NBIG, used_out, d_counters.p, 1, d_fbase_out.p,
                        d_fence_out.p);
     time_end(b0);
-    const int w1 = time_begin(DCG_K_DELETE_ROWS, -1);
-    hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream, v, d_in.p, d_sorted_in.p, d_hot_in.p, kin,
+    const int w1 = time_begin(DCG_K_DELETE_ROWS, -1, 0, stream2);
+    hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream2, v, d_in.p, d_sorted_in.p, d_hot_in.p, kin,
                        count, d_gstart2.p, d_counters.p + C_GROUPS_IN, used_in, d_counters.p, 0, big_in,
                        d_counters.p + C_NBIG_IN, d_fbase_in.p, d_fence_in.p, policy_reads());
     time_end(w1);
-    const int b1 = time_begin(DCG_K_DELETE_BIG, -1);
-    hipLaunchKernelGGL(k_del_big, dim3(kDelBigGrid), dim3(1024), 0, stream, v, d_in.p, d_sorted_in.p, d_hot_in.p,
+    const int b1 = time_begin(DCG_K_DELETE_BIG, -1, 0, stream2);
+    hipLaunchKernelGGL(k_del_big, dim3(kDelBigGrid), dim3(1024), 0, stream2, v, d_in.p, d_sorted_in.p, d_hot_in.p,
                        kin, count, big_in, d_counters.p + C_NBIG_IN, used_in, d_counters.p, 0, d_fbase_in.p,
                        d_fence_in.p);
     time_end(b1);
+    DCG_HIP(hipEventRecord(join_ev, stream2));
+    DCG_HIP(hipStreamWaitEvent(stream, join_ev, 0));
+    time_end(pass);
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipMemcpyAsync(h_counters, d_counters.p, C_NUM * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
     mut_slot[0] = w0;
     mut_slot[1] = w1;
     mut_slot[2] = b0;
     mut_slot[3] = b1;
+    mut_slot[4] = pass;
 }
 
 void Engine::finish_delete(uint64_t count, dcg_phase_stats *st) {
@@ -1772,6 +1782,8 @@ void Engine::finish_delete(uint64_t count, dcg_phase_stats *st) {
     timed[mut_slot[2]].bytes = 4.0 * (double)(h_counters[C_SLOTS_READ] + h_counters[C_SLOTS_WRITTEN]) +
                                16.0 * (double)h_counters[C_B_ROWS];
     timed[mut_slot[3]].bytes = 0.0;
+    timed[mut_slot[4]].bytes = timed[mut_slot[0]].bytes + timed[mut_slot[2]].bytes;
+    timed[mut_slot[4]].host_lines = h_counters[C_W_HOST_LINES];
     if (st) {
         st->touched_sources = h_counters[C_GROUPS_OUT];
         st->changed_sources = h_counters[C_CHANGED];

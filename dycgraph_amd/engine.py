@@ -77,7 +77,7 @@ class KernelStats(C.Structure):
 
 
 KERNELS = {"push": 0, "pull": 1, "invalidate": 2, "pr_solve": 3, "pr_amend": 4, "delete_rows": 5,
-           "append_rows": 6, "delete_big": 7}
+           "append_rows": 6, "delete_big": 7, "delete_pass": 8}
 
 
 class TuneResult(C.Structure):

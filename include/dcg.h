@@ -204,7 +204,10 @@ typedef enum {
     DCG_K_DELETE_ROWS = 5,   /* delete mutation, wave path (rows of at most 1,024 slots) */
     DCG_K_APPEND_ROWS = 6,   /* append / relocation, both paths */
     DCG_K_DELETE_BIG = 7,    /* delete mutation, workgroup path (larger rows) */
-    DCG_K_NUM = 8
+    DCG_K_DELETE_PASS = 8,   /* both directions' delete launches together (the in-row pass runs
+                                beside the out-row pass): wall time from the first launch to the
+                                end of the last, with the bytes and host lines of all four */
+    DCG_K_NUM = 9
 } dcg_kernel_id;
 typedef struct {
     uint64_t launches;
