@@ -35,6 +35,19 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name)
 
 
+def test_kernel_ids_match_the_header():
+    """The Python mirror's kernel-id table (kernel_stats keys) lists exactly the
+    header's dcg_kernel_id values below DCG_K_NUM."""
+    from dycgraph_amd.engine import KERNELS
+    text = open(os.path.join(ROOT, "include", "dcg.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    body = re.search(r"typedef enum \{(.*?)\} dcg_kernel_id;", text, flags=re.S).group(1)
+    ids = {name: int(v) for name, v in re.findall(r"DCG_K_([A-Z_]+)\s*=\s*(\d+)", body)}
+    num = ids.pop("NUM")
+    assert sorted(ids.values()) == list(range(num))
+    assert {k.upper(): v for k, v in KERNELS.items()} == ids
+
+
 def test_config_defaults_and_status_strings():
     import dycgraph_amd as d
     from dycgraph_amd.engine import Config
