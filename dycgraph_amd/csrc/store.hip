@@ -794,6 +794,9 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
                                                   uint32_t *used, unsigned long long *counters, int count_stats,
                                                   uint32_t *big_groups, unsigned long long *nbig,
                                                   const uint32_t *fbase, uint32_t *fence, float *reads) {
+    __shared__ WgSum<6> tot;
+    wg_sum_init(tot);
+    __syncthreads();
     const uint32_t lane = lane_id(), qd = lane >> 4, ql = lane & 15;
     const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kWave;
     const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / kWave;
@@ -978,14 +981,11 @@ __global__ __launch_bounds__(256) void k_del_rows(StoreView s, RowDesc *desc, ui
         rows += __shfl_xor(rows, o, kWave);
         hl += __shfl_xor(hl, o, kWave);
     }
-    if (lane == 0) {
-        if (hl) atomicAdd(&counters[C_W_HOST_LINES], hl);
-        if (count_stats && missing) atomicAdd(&counters[C_MISSING], missing);
-        if (count_stats && changed) atomicAdd(&counters[C_CHANGED], changed);
-        if (rd) atomicAdd(&counters[C_W_READ], rd);
-        if (wr) atomicAdd(&counters[C_W_WRITTEN], wr);
-        if (rows) atomicAdd(&counters[C_W_ROWS], rows);
-    }
+    unsigned long long *const dst[6] = {&counters[C_W_HOST_LINES], count_stats ? &counters[C_MISSING] : nullptr,
+                                        count_stats ? &counters[C_CHANGED] : nullptr, &counters[C_W_READ],
+                                        &counters[C_W_WRITTEN], &counters[C_W_ROWS]};
+    const unsigned long long x[6] = {hl, missing, changed, rd, wr, rows};
+    wg_sum_flush(tot, x, dst);
 }
 
 constexpr uint32_t kTileE = 4096;   // entries per workgroup tile of the large-row passes
@@ -1576,6 +1576,9 @@ __global__ __launch_bounds__(256) void k_add_rows(StoreView s, RowDesc *desc, ui
                                                   const unsigned long long *ngroups_p, uint64_t *tail,
                                                   uint64_t *pool, const uint64_t *reloc, unsigned long long *counters,
                                                   int count_stats, uint32_t *big_groups, unsigned long long *nbig) {
+    __shared__ WgSum<3> tot;
+    wg_sum_init(tot);
+    __syncthreads();
     const uint32_t lane = lane_id(), qd = lane >> 4, ql = lane & 15;
     const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kWave;
     const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / kWave;
@@ -1640,11 +1643,10 @@ __global__ __launch_bounds__(256) void k_add_rows(StoreView s, RowDesc *desc, ui
         rd += __shfl_xor(rd, o, kWave);
         wr += __shfl_xor(wr, o, kWave);
     }
-    if (lane == 0) {
-        if (count_stats && relocated) atomicAdd(&counters[C_RELOCATED], relocated);
-        if (rd) atomicAdd(&counters[C_SLOTS_READ], rd);
-        if (wr) atomicAdd(&counters[C_SLOTS_WRITTEN], wr);
-    }
+    unsigned long long *const dst[3] = {count_stats ? &counters[C_RELOCATED] : nullptr, &counters[C_SLOTS_READ],
+                                        &counters[C_SLOTS_WRITTEN]};
+    const unsigned long long x[3] = {relocated, rd, wr};
+    wg_sum_flush(tot, x, dst);
 }
 
 // Relocation of a large row: one 1024-thread workgroup copies the row into its
@@ -1973,6 +1975,9 @@ void Engine::export_csr(uint64_t *row_ptr, uint32_t *col, uint64_t *m) {
 // cleared for the next fold.
 __global__ void k_fold_reads(StoreView s, const unsigned long long *nlist_p, float *reads, float *rows,
                              unsigned long long *counters) {
+    __shared__ WgSum<2> tot;
+    wg_sum_init(tot);
+    __syncthreads();
     unsigned long long cached = 0, host = 0;
     const uint64_t nlist = *nlist_p;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nlist;
@@ -2002,10 +2007,9 @@ __global__ void k_fold_reads(StoreView s, const unsigned long long *nlist_p, flo
         cached += __shfl_xor(cached, d, kWave);
         host += __shfl_xor(host, d, kWave);
     }
-    if (lane_id() == 0) {
-        if (cached) atomicAdd(&counters[C_CACHED_READS], cached);
-        if (host) atomicAdd(&counters[C_HOST_READS], host);
-    }
+    unsigned long long *const dst[2] = {&counters[C_CACHED_READS], &counters[C_HOST_READS]};
+    const unsigned long long x[2] = {cached, host};
+    wg_sum_flush(tot, x, dst);
 }
 
 void Engine::fold_enqueue() {

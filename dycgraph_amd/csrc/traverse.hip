@@ -708,7 +708,9 @@ template <uint32_t MODE>
 __global__ __launch_bounds__(kPBlock) void k_persistent(TravArgs a) {
     __shared__ WaveLds lds_all[kPBlock / kWave];
     __shared__ OutLds O;
+    __shared__ WgSum<2> tot;
     if (threadIdx.x == 0) O.n = O.nf = 0;
+    wg_sum_init(tot);
     __syncthreads();
     // publish round rr's buffered entries (modes that emit; whole-grid rounds
     // only: workgroup 0 alone appends directly); a workgroup barrier either way
@@ -789,9 +791,11 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(TravArgs a) {
         }
         ++r;
     }
-    // every lane of a wave holds the same totals; lane 0 reports them
-    if (lane == 0 && edges) atomicAdd(&a.ctl->edges, edges);
-    if (lane == 0 && expansions) atomicAdd(&a.ctl->expansions, expansions);
+    // every lane of a wave holds the same totals; lane 0 reports them, one
+    // global atomic per workgroup
+    unsigned long long *const dst[2] = {&a.ctl->edges, &a.ctl->expansions};
+    const unsigned long long x[2] = {edges, expansions};
+    wg_sum_flush(tot, x, dst);
 }
 
 // ------------------------------------------------------------- plain kernels
@@ -865,7 +869,8 @@ __global__ void k_iota_sources(uint32_t *q0, uint32_t n, uint32_t Q, const uint3
 }
 
 // Delete seeds (driver.cuh:394-441): test every deleted edge against every query.
-__global__ void k_seed_dels(TravArgs a, const dcg_edge *base) {
+__global__ __launch_bounds__(1024) void k_seed_dels(TravArgs a, const dcg_edge *base) {
+    __shared__ uint32_t wcnt[16], wbase;   // list positions: one atomic per workgroup and step
     const uint64_t count = a.step->count;
     const dcg_edge *recs = base + a.step->begin;
     const uint64_t total = count * a.Q;
@@ -879,17 +884,10 @@ __global__ void k_seed_dels(TravArgs a, const dcg_edge *base) {
             const dcg_edge e = recs[i - (uint64_t)q * count];
             if (e.u < a.n && e.v < a.n) want = invalidate_op(a, q, e.u, a.st[(uint64_t)q * a.n + e.u], e.v, og);
         }
-        const uint64_t mask = __ballot(want);
-        if (mask) {
-            const int leader = __ffsll((unsigned long long)mask) - 1;
-            uint32_t base = 0;
-            if ((int)lane_id() == leader) base = atomicAdd(&a.ctl->aff_count, (uint32_t)__popcll(mask));
-            base = __shfl(base, leader, kWave);
-            if (want) {
-                const uint32_t pos = base + lanes_below(mask);
-                if (pos < a.affcap) a.aff[pos] = og;
-                else a.ctl->overflow = 1;
-            }
+        const uint32_t pos = wg_append(want, &a.ctl->aff_count, wcnt, wbase);
+        if (want) {
+            if (pos < a.affcap) a.aff[pos] = og;
+            else a.ctl->overflow = 1;
         }
     }
 }
@@ -961,7 +959,8 @@ __global__ void k_cc_fix_parents(TravArgs a, const uint32_t *list, const Ctl *li
 }
 
 // Insertion seeds (framework.cuh:297-370): relax every added edge for every query.
-__global__ void k_seed_adds(TravArgs a, const dcg_edge *base, uint32_t *outq) {
+__global__ __launch_bounds__(1024) void k_seed_adds(TravArgs a, const dcg_edge *base, uint32_t *outq) {
+    __shared__ uint32_t wcnt[16], wbase;   // list positions: one atomic per workgroup and step
     const uint64_t count = a.step->count;
     const dcg_edge *recs = base + a.step->begin;
     const uint32_t stamp0 = stamp_base(a);
@@ -979,17 +978,10 @@ __global__ void k_seed_adds(TravArgs a, const dcg_edge *base, uint32_t *outq) {
                 if ((uint32_t)(sv >> 32) != kInf) want = push_op(a, q, e.u, sv, e.v, stamp0, og, false) == 1;
             }
         }
-        const uint64_t mask = __ballot(want);
-        if (mask) {
-            const int leader = __ffsll((unsigned long long)mask) - 1;
-            uint32_t base = 0;
-            if ((int)lane_id() == leader) base = atomicAdd(&a.ctl->aff_count, (uint32_t)__popcll(mask));
-            base = __shfl(base, leader, kWave);
-            if (want) {
-                const uint32_t pos = base + lanes_below(mask);
-                if (pos < a.qcap) outq[pos] = og;
-                else a.ctl->overflow = 1;
-            }
+        const uint32_t pos = wg_append(want, &a.ctl->aff_count, wcnt, wbase);
+        if (want) {
+            if (pos < a.qcap) outq[pos] = og;
+            else a.ctl->overflow = 1;
         }
     }
 }
@@ -1361,12 +1353,12 @@ void Engine::enqueue_repair(int stage) {
     Ctl *c = d_ctl.p;
     TravArgs a = base_args(*this);
     a.step = d_step.p + stage;
-    constexpr uint32_t kSeedBlocks = 2048;   // grid-stride over count * Q records
+    constexpr uint32_t kSeedBlocks = 512;   // 1,024-thread workgroups, grid-stride over count * Q records
     if (stage == 0) {
         hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 0, (const uint32_t *)nullptr, 0u, 0u,
                            (const uint32_t *)nullptr);
         a.ctl = c + 0;
-        hipLaunchKernelGGL(k_seed_dels, dim3(kSeedBlocks), dim3(256), 0, stream, a, (const dcg_edge *)d_dels.p);
+        hipLaunchKernelGGL(k_seed_dels, dim3(kSeedBlocks), dim3(1024), 0, stream, a, (const dcg_edge *)d_dels.p);
         hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 1, &c[0].aff_count, 0u, 0u,
                            &c[0].aff_count);
         a.ctl = c + 1;
@@ -1393,7 +1385,7 @@ void Engine::enqueue_repair(int stage) {
                            (const uint32_t *)nullptr);
         a.ctl = c + 4;
         a.stamp_slot = 0;
-        hipLaunchKernelGGL(k_seed_adds, dim3(kSeedBlocks), dim3(256), 0, stream, a, (const dcg_edge *)d_adds.p,
+        hipLaunchKernelGGL(k_seed_adds, dim3(kSeedBlocks), dim3(1024), 0, stream, a, (const dcg_edge *)d_adds.p,
                            d_qc.p);
         hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 5, &c[4].aff_count, 0u, 0u,
                            (const uint32_t *)nullptr);
