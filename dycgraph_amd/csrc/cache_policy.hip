@@ -256,6 +256,7 @@ void Engine::cache_maintain(dcg_cache_stats *st) {
     std::vector<float> score(C);
     DCG_HIP(hipMemcpyAsync(score.data(), cache.d_score.p, C * 4, hipMemcpyDeviceToHost, stream));
     sync();
+    if (cfg.cache_policy == DCG_CACHE_LEARNED) cache.hot_score = score;   // the next decision may take the hotness plan
     for (uint64_t c = 0; c < C; ++c)
         if (reads[c] > 0) cache.last_use[c] = (uint32_t)snapshots;
     if (cfg.cache_policy == DCG_CACHE_LEARNED && cache.mlp) {

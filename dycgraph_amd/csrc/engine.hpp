@@ -69,6 +69,7 @@ struct CacheState {
     DevBuf<float> d_score;
     DevBuf<int64_t> d_load;             // (frame, chunk) pairs of one admission decision
     std::vector<uint32_t> last_use;     // snapshot of last access (LRU tie-break)
+    std::vector<float> hot_score;       // the last maintenance's 4-snapshot read sums (hotness ranking)
     uint64_t loaded = 0, evicted = 0, h2d = 0;
     std::unique_ptr<MlpState, MlpDeleter> mlp;
 };

@@ -77,7 +77,14 @@ struct MlpState {
     DevBuf<float> judge_reads;            // the snapshot's reads, kept for the judge
     DevBuf<uint8_t> judge_tmp;            // radix-sort scratch of the training stream
     int decisions = 0, fallbacks = 0;
+    int rejected = 0;   // judged decisions in a row, after warm-up, that kept the hotness ranking
 };
+
+// While the judge keeps rejecting the trained network (kQuiet judged decisions
+// in a row), only every kProbe-th decision runs it (features, forward pass,
+// judge and a training step); the others take the hotness plan at no extra
+// cost.  One network decision accepted by the judge restores every decision.
+constexpr int kQuiet = 2, kProbe = 4;
 
 void MlpDeleter::operator()(MlpState *m) const { delete m; }
 
@@ -487,6 +494,7 @@ static void judge_resolve(MlpState &m) {
     m.mlp_capture = sums[2] > 0 ? sums[0] / sums[2] : 1.0;
     m.hot_capture = sums[2] > 0 ? sums[1] / sums[2] : 1.0;
     m.fallback = m.steps < 8 || !(m.mlp_capture > 1.02 * m.hot_capture);
+    if (m.steps >= 8) m.rejected = m.fallback ? m.rejected + 1 : 0;
     m.judge_pending = false;
 }
 
@@ -529,6 +537,11 @@ void mlp_stats(const Engine &e, dcg_cache_stats *st) {
 std::vector<uint64_t> mlp_plan(Engine &e, uint64_t ab, uint64_t ac, uint64_t db, uint64_t dc) {
     MlpState &m = *e.cache.mlp;
     const uint64_t C = e.cache.chunks;
+    if (m.rejected >= kQuiet && (m.decisions % kProbe) != 0 && e.cache.hot_score.size() == C && !m.train_due) {
+        m.decisions++;
+        m.fallbacks++;
+        return e.plan_admission(e.cache.hot_score, 4.0 * kLoadLines, 1.5);
+    }
     if (m.train_due) {   // no delete stage ran since the last decision
         mlp_train_steps(m, 4);
         m.train_due = false;
