@@ -27,6 +27,7 @@ namespace dcg {
 
 enum Mode : uint32_t { M_PUSH = 0, M_PULL = 1, M_INVALIDATE = 2, M_PR = 3, M_PR_AMEND = 4 };
 constexpr uint32_t kDirtyLog2 = 10;   // CC parent-check block: 1,024 state slots
+constexpr uint32_t kLdsQueries = 1024;   // queries whose far-pile minima a workgroup gathers in LDS
 
 struct TravArgs {
     StoreView s;
@@ -655,11 +656,27 @@ __device__ bool far_advance(const TravArgs &a, uint32_t r, const Team &t) {
     const uint32_t *src = fp ? a.far1 : a.far0;
     uint32_t *dst = fp ? a.far0 : a.far1;
     uint32_t *mf = a.minfar + (uint64_t)fp * a.Q;
+    // per-query minima in LDS first, then one global atomic per workgroup and
+    // query (one global atomic per entry on Q words serialised at the L2:
+    // ~30 M per snapshot on the 65-query grid)
+    __shared__ uint32_t qmin[kLdsQueries];
+    const bool lds = a.Q <= kLdsQueries;
+    if (lds) {
+        for (uint32_t q = threadIdx.x; q < a.Q; q += blockDim.x) qmin[q] = kInf;
+        __syncthreads();
+    }
     for (uint32_t i = t.tid; i < fc; i += t.nthreads) {
         uint32_t q, v;
         const uint32_t g = src[i];
         decode_g(a, g, q, v);
-        atomicMin(&mf[q], (uint32_t)(a.st[g] >> 32));
+        const uint32_t val = (uint32_t)(a.st[g] >> 32);
+        if (lds) atomicMin(&qmin[q], val);
+        else atomicMin(&mf[q], val);
+    }
+    if (lds) {
+        __syncthreads();
+        for (uint32_t q = threadIdx.x; q < a.Q; q += blockDim.x)
+            if (qmin[q] != kInf) atomicMin(&mf[q], qmin[q]);
     }
     gsync(a);
     const uint32_t ep = *(volatile uint32_t *)a.epoch + 1;
