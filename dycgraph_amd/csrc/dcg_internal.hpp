@@ -200,6 +200,27 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
     return x;
 }
 
+// Like wg_append for k entries per lane: the lane's first position in a global
+// list, with one atomic on the list counter per workgroup.  Every thread of the
+// workgroup calls it (block-uniform control flow); wcnt and wbase live in LDS.
+__device__ __forceinline__ uint32_t wg_reserve(uint32_t k, uint32_t *counter, uint32_t *wcnt, uint32_t &wbase) {
+    const uint32_t incl = wave_incl_scan(k);
+    const uint32_t w = threadIdx.x / kWave, nw = (blockDim.x + kWave - 1) / kWave;
+    if (lane_id() == kWave - 1) wcnt[w] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (uint32_t q = 0; q < nw; ++q) t += wcnt[q];
+        wbase = t ? atomicAdd(counter, t) : 0u;
+    }
+    __syncthreads();
+    uint32_t pos = wbase;
+    for (uint32_t q = 0; q < w; ++q) pos += wcnt[q];
+    pos += incl - k;
+    __syncthreads();   // wcnt and wbase are reused by the next call
+    return pos;
+}
+
 // Wave-aggregated append: lanes with `want` get consecutive slots of `counter`.
 // Returns the slot for this lane (undefined where !want).
 __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool want) {

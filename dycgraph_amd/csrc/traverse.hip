@@ -1566,7 +1566,13 @@ constexpr uint32_t kDensePiece = 2048;   // larger in-rows: one wave per piece o
 
 // In-rows by size: small rows listed, larger rows cut into pieces; the total
 // slot count prices a round (edges).
-__global__ void k_prd_split(StoreView s, uint32_t n, uint32_t *small, uint2 *big, Ctl *c) {
+__global__ __launch_bounds__(1024) void k_prd_split(StoreView s, uint32_t n, uint32_t *small, uint2 *big, Ctl *c) {
+    // list positions with one atomic per workgroup and list (one per wave on the
+    // two list counters serialised into ~0.9 ms at n = 2^22)
+    __shared__ uint32_t wcnt[16], wbase;
+    __shared__ WgSum<1> tot;
+    wg_sum_init(tot);
+    __syncthreads();
     unsigned long long slots = 0;
     for (uint32_t b = blockIdx.x * blockDim.x; b < n; b += gridDim.x * blockDim.x) {
         const uint32_t v = b + threadIdx.x;
@@ -1574,21 +1580,15 @@ __global__ void k_prd_split(StoreView s, uint32_t n, uint32_t *small, uint2 *big
         slots += ext;
         const bool sm = ext && ext <= kDenseSmall;
         const uint32_t np = ext > kDenseSmall ? (ext + kDensePiece - 1) / kDensePiece : 0u;
-        const uint64_t ms = __ballot(sm);
-        if (ms) {
-            const int leader = __ffsll((unsigned long long)ms) - 1;
-            uint32_t base = 0;
-            if ((int)lane_id() == leader) base = atomicAdd(&c->count[0], (uint32_t)__popcll(ms));
-            base = __shfl(base, leader, kWave);
-            if (sm) small[base + lanes_below(ms)] = v;
-        }
-        if (np) {
-            const uint32_t base = atomicAdd(&c->count[1], np);
-            for (uint32_t p = 0; p < np; ++p) big[base + p] = make_uint2(v, p * kDensePiece);
-        }
+        const uint32_t ps = wg_reserve(sm ? 1u : 0u, &c->count[0], wcnt, wbase);
+        if (sm) small[ps] = v;
+        const uint32_t pb = wg_reserve(np, &c->count[1], wcnt, wbase);
+        for (uint32_t p = 0; p < np; ++p) big[pb + p] = make_uint2(v, p * kDensePiece);
     }
     for (int d = 32; d >= 1; d >>= 1) slots += __shfl_xor(slots, d, kWave);
-    if (lane_id() == 0 && slots) atomicAdd(&c->far_scanned, slots);   // slots per round
+    unsigned long long *const dst[1] = {&c->far_scanned};   // slots per round
+    const unsigned long long x[1] = {slots};
+    wg_sum_flush(tot, x, dst);
 }
 
 __global__ void k_prd_consume(StoreView s, uint32_t n, uint32_t r, uint32_t max_rounds, float eps, float *rank,
@@ -1694,7 +1694,8 @@ void Engine::pagerank_dense(Ctl *c, dcg_phase_stats *st) {
     fill<uint32_t>(stream, &c->done, 1, 0u);
     fill<uint32_t>(stream, c->rmax, 2, 0u);
     const StoreView v = view();
-    hipLaunchKernelGGL(k_prd_split, dim3(grid_for(n)), dim3(256), 0, stream, v, n, d_prd_small.p, d_prd_big.p, c);
+    hipLaunchKernelGGL(k_prd_split, dim3(grid_for(n, 1024, 4096)), dim3(1024), 0, stream, v, n, d_prd_small.p,
+                       d_prd_big.p, c);
     const uint32_t max_rounds = cfg.pr_max_rounds;   // 1..100, checked at creation
     const uint32_t cg = grid_for(n, 256, 4096), sg = grid_for((uint64_t)n * 16, 256, 65536);
     const uint32_t rg = grid_for(n, 256, 1024);   // residual maximum: ~16 entries per thread at n = 2^22
