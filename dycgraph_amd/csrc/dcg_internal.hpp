@@ -127,13 +127,6 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-// Workgroup totals for a few global counters: every wave adds its totals
-// (held by lane 0) into LDS and the last wave of the workgroup to arrive adds
-// the sums to the global counters, so a workgroup costs one atomic per counter
-// instead of one per wave (atomics on one word serialise at the L2: ~16 k of
-// them take ~0.2 ms).  No barrier at the end, so waves may finish at different
-// times.  wg_sum_init runs before a barrier at kernel start; every wave of the
-// workgroup calls wg_sum_flush exactly once.
 // Positions in a global list for the flagged lanes of a workgroup, with one
 // atomic on the list counter per workgroup.  Every thread of the workgroup
 // calls it (block-uniform control flow); wcnt (one word per wave) and wbase
@@ -156,6 +149,13 @@ __device__ __forceinline__ uint32_t wg_append(bool want, uint32_t *counter, uint
     return pos;
 }
 
+// Workgroup totals for a few global counters: every wave adds its totals
+// (held by lane 0) into LDS and the last wave of the workgroup to arrive adds
+// the sums to the global counters, so a workgroup costs one atomic per counter
+// instead of one per wave (atomics on one word serialise at the L2: ~16 k of
+// them take ~0.2 ms).  No barrier at the end, so waves may finish at different
+// times.  wg_sum_init runs before a barrier at kernel start; every wave of the
+// workgroup calls wg_sum_flush exactly once.
 template <int K>
 struct WgSum {
     unsigned long long v[K];
