@@ -5,6 +5,7 @@
 #include <cstring>
 
 #include "engine.hpp"
+#include "mlp.hpp"
 
 using namespace dcg;
 
@@ -22,9 +23,18 @@ Engine::Engine(const dcg_config &c) : cfg(c) {
     DCG_HIP(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
     DCG_HIP(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
     DCG_HIP(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming));
+    DCG_HIP(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
+    for (auto &kind : slots)
+        for (BatchSlot &b : kind) {
+            DCG_HIP(hipEventCreateWithFlags(&b.ready, hipEventDisableTiming));
+            DCG_HIP(hipEventCreateWithFlags(&b.used, hipEventDisableTiming));
+        }
     hipDeviceProp_t prop;
     DCG_HIP(hipGetDeviceProperties(&prop, device));
     cu_count = prop.multiProcessorCount;
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+        wall_khz = (double)khz;
     DCG_HIP(hipHostMalloc((void **)&h_ctl, 8 * sizeof(Ctl), hipHostMallocDefault));
     DCG_HIP(hipHostMalloc((void **)&h_step, 2 * sizeof(StepArgs), hipHostMallocDefault));
     d_step.alloc(2);
@@ -65,9 +75,111 @@ int Engine::time_begin(int kernel, int ctl, double bytes, hipStream_t on) {
 
 void Engine::time_end(int slot) { DCG_HIP(hipEventRecord(events[timed[slot].ev + 1], timed[slot].on)); }
 
+// ---- update stream: device batch slots --------------------------------------
+void Engine::forget_batches() {
+    for (auto &kind : slots)
+        for (BatchSlot &b : kind) {
+            if (b.pending) DCG_HIP(hipEventSynchronize(b.ready));
+            b.pending = false;
+            b.begin = ~0ull;
+            b.count = 0;
+        }
+}
+
+// Copy [begin, begin+count) of one direction into slot `b` on stream `on`
+// (after every earlier reader of the slot).  Host records are pinned, so the
+// copy is an asynchronous DMA in stream order.
+void Engine::fill_slot(int kind, BatchSlot &b, uint64_t begin, uint64_t count, hipStream_t on) {
+    if (b.pending) DCG_HIP(hipEventSynchronize(b.ready));
+    b.pending = false;
+    if (b.buf.n < count) {
+        DCG_HIP(hipEventSynchronize(b.used));   // the old buffer's readers have finished
+        b.buf.alloc(count);
+    }
+    DCG_HIP(hipStreamWaitEvent(on, b.used, 0));
+    const dcg_edge *h = (kind == kDels ? h_dels : h_adds) + begin;
+    const int slot = time_begin(DCG_K_INGEST, -1, 8.0 * (double)count, on);
+    DCG_HIP(hipMemcpyAsync(b.buf.p, h, count * sizeof(dcg_edge), hipMemcpyHostToDevice, on));
+    time_end(slot);
+    b.begin = begin;
+    b.count = count;
+    b.stamp = ++slot_clock;
+}
+
+const dcg_edge *Engine::batch_records(int kind, uint64_t begin, uint64_t count) {
+    if (!count) return nullptr;
+    BatchSlot *pick = nullptr;
+    for (BatchSlot &b : slots[kind])
+        if (b.begin == begin && b.count == count) pick = &b;
+    if (pick) {
+        if (pick->pending) {
+            DCG_HIP(hipStreamWaitEvent(stream, pick->ready, 0));
+            pick->pending = false;
+        }
+    } else {
+        // refill the older slot that no prefetch is still filling
+        BatchSlot &a = slots[kind][0], &b = slots[kind][1];
+        if (a.pending != b.pending) pick = a.pending ? &b : &a;
+        else pick = a.stamp <= b.stamp ? &a : &b;
+        fill_slot(kind, *pick, begin, count, stream);
+    }
+    return pick->buf.p;
+}
+
+// After a call enqueued the readers of its batch slots: their end, which a later
+// copy into the slot waits for.
+void Engine::mark_batches_used() {
+    for (auto &kind : slots)
+        for (BatchSlot &b : kind)
+            if (b.count && !b.pending) DCG_HIP(hipEventRecord(b.used, stream));
+}
+
+void Engine::prefetch_records(int kind, uint64_t begin, uint64_t count) {
+    if (!count) return;
+    for (BatchSlot &b : slots[kind])
+        if (b.begin == begin && b.count == count) return;   // already there or on its way
+    BatchSlot &a = slots[kind][0], &b = slots[kind][1];
+    BatchSlot &t = a.stamp <= b.stamp ? a : b;
+    fill_slot(kind, t, begin, count, copy_stream);
+    DCG_HIP(hipEventRecord(t.ready, copy_stream));
+    t.pending = true;
+}
+
+void Engine::sync_all() {
+    DCG_HIP(hipStreamSynchronize(stream));
+    DCG_HIP(hipStreamSynchronize(stream2));
+    DCG_HIP(hipStreamSynchronize(copy_stream));
+    mlp_sync(*this);
+}
+
+// Closures of a replayed stage graph: each one's span between its wall-clock
+// stamps, priced from its Ctl block (copied to h_ctl before the stage's wait).
+void Engine::collect_graph_timers(int stage) {
+    for (const GraphTimer &g : graph_timers[stage]) {
+        const Ctl &c = h_ctl[g.ctl];
+        const double ms = c.t_last > c.t_first ? (double)(c.t_last - c.t_first) / wall_khz : 0.0;
+        dcg_kernel_stats &k = kstat[g.kernel];
+        k.launches++;
+        k.ms += ms;
+        k.edges += c.edges;
+        k.expansions += c.expansions;
+        k.algo_bytes += 24.0 * (double)c.expansions + 12.0 * (double)c.edges;
+    }
+}
+
 // Fold recorded launches into kstat (call after the stream has synchronised).
 void Engine::collect_timing() {
     if (timed.empty()) return;
+    // a copy still running on copy_stream (a prefetch) is folded by a later call
+    std::vector<Timed> later;
+    for (size_t i = 0; i < timed.size();) {
+        if (timed[i].on == copy_stream && hipEventQuery(events[timed[i].ev + 1]) == hipErrorNotReady) {
+            later.push_back(timed[i]);
+            timed.erase(timed.begin() + (long)i);
+        } else {
+            ++i;
+        }
+    }
     Ctl ctl[8];
     bool priced = false;   // closure launches (graphs off) are priced from their Ctl blocks
     for (const Timed &t : timed) priced |= t.ctl >= 0;
@@ -92,12 +204,30 @@ void Engine::collect_timing() {
         }
     }
     timed.clear();
+    // keep the pending copies' event pairs: move them to the front of the pool
+    for (Timed &t : later) {
+        const size_t slot = timed.size();
+        std::swap(events[2 * slot], events[t.ev]);
+        std::swap(events[2 * slot + 1], events[t.ev + 1]);
+        t.ev = (int)(2 * slot);
+        timed.push_back(t);
+    }
 }
 
 Engine::~Engine() {
     if (stream) (void)hipStreamSynchronize(stream);
     if (stream2) (void)hipStreamSynchronize(stream2);
+    if (copy_stream) (void)hipStreamSynchronize(copy_stream);
+    cache.mlp.reset();   // its training stream and events
     for (hipEvent_t ev : events) (void)hipEventDestroy(ev);
+    for (auto &kind : slots)
+        for (BatchSlot &b : kind) {
+            b.buf.release();
+            if (b.ready) (void)hipEventDestroy(b.ready);
+            if (b.used) (void)hipEventDestroy(b.used);
+        }
+    if (h_adds) (void)hipHostFree(h_adds);
+    if (h_dels) (void)hipHostFree(h_dels);
     if (h_slab) (void)hipHostFree(h_slab);
     if (h_ctl) (void)hipHostFree(h_ctl);
     if (h_step) (void)hipHostFree(h_step);
@@ -107,11 +237,16 @@ Engine::~Engine() {
     if (h_fold) (void)hipHostFree(h_fold);
     if (fork_ev) (void)hipEventDestroy(fork_ev);
     if (join_ev) (void)hipEventDestroy(join_ev);
+    if (copy_stream) (void)hipStreamDestroy(copy_stream);
     if (stream2) (void)hipStreamDestroy(stream2);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
 }  // namespace dcg
+
+static void check_slice(uint64_t begin, uint64_t count, uint64_t total) {
+    if (begin > total || count > total - begin) throw Error(DCG_ERR_ARG, "update slice out of range");
+}
 
 template <typename F>
 static int guarded(dcg_engine *h, F &&f) {
@@ -258,20 +393,31 @@ int dcg_start(dcg_engine *h) {
 int dcg_load_updates(dcg_engine *h, const dcg_edge *adds, uint64_t num_adds, const dcg_edge *dels, uint64_t num_dels) {
     return guarded(h, [&](Engine &e) {
         if ((num_adds && !adds) || (num_dels && !dels)) throw Error(DCG_ERR_ARG, "null update array");
-        e.d_adds.alloc(num_adds ? num_adds : 1);
-        e.d_dels.alloc(num_dels ? num_dels : 1);
-        if (num_adds)
-            DCG_HIP(hipMemcpyAsync(e.d_adds.p, adds, num_adds * sizeof(dcg_edge), hipMemcpyHostToDevice, e.stream));
-        if (num_dels)
-            DCG_HIP(hipMemcpyAsync(e.d_dels.p, dels, num_dels * sizeof(dcg_edge), hipMemcpyHostToDevice, e.stream));
+        // no copy of the old stream is still read: every stage waits for its work,
+        // and a pending prefetch is waited for here
+        e.sync_all();
+        e.forget_batches();
+        if (e.h_adds) (void)hipHostFree(e.h_adds);
+        if (e.h_dels) (void)hipHostFree(e.h_dels);
+        e.h_adds = e.h_dels = nullptr;
+        e.n_adds = e.n_dels = 0;
+        DCG_HIP(hipHostMalloc((void **)&e.h_adds, (num_adds ? num_adds : 1) * sizeof(dcg_edge), hipHostMallocDefault));
+        DCG_HIP(hipHostMalloc((void **)&e.h_dels, (num_dels ? num_dels : 1) * sizeof(dcg_edge), hipHostMallocDefault));
+        if (num_adds) std::memcpy(e.h_adds, adds, num_adds * sizeof(dcg_edge));
+        if (num_dels) std::memcpy(e.h_dels, dels, num_dels * sizeof(dcg_edge));
         e.n_adds = num_adds;
         e.n_dels = num_dels;
-        e.sync();
     });
 }
 
-static void check_slice(uint64_t begin, uint64_t count, uint64_t total) {
-    if (begin > total || count > total - begin) throw Error(DCG_ERR_ARG, "update slice out of range");
+int dcg_prefetch_updates(dcg_engine *h, uint64_t add_begin, uint64_t add_count, uint64_t del_begin,
+                         uint64_t del_count) {
+    return guarded(h, [&](Engine &e) {
+        check_slice(add_begin, add_count, e.n_adds);
+        check_slice(del_begin, del_count, e.n_dels);
+        e.prefetch_records(Engine::kDels, del_begin, del_count);
+        e.prefetch_records(Engine::kAdds, add_begin, add_count);
+    });
 }
 
 int dcg_del_edge(dcg_engine *h, uint64_t begin, uint64_t count, dcg_phase_stats *st) {
@@ -476,7 +622,7 @@ int dcg_set_buckets(dcg_engine *h, uint32_t quarters) {
 }
 
 int dcg_sync(dcg_engine *h) {
-    return guarded(h, [&](Engine &e) { e.sync(); });
+    return guarded(h, [&](Engine &e) { e.sync_all(); });
 }
 
 }  // extern "C"

@@ -256,13 +256,19 @@ struct Ctl {
     uint32_t bar_arrive;      // grid barrier: workgroups arrived in the current generation
     uint32_t bar_gen;         // grid barrier generation
     uint32_t bar_timeout;     // a workgroup waited ~10 s at the barrier (the launch gives up)
+    // device wall clock (s_memrealtime ticks) at the first workgroup's start and
+    // the last workgroup's end of a persistent launch: its duration also when
+    // the launch is a node of a replayed graph
+    unsigned long long t_first, t_last;
 };
 
 // Per-stage arguments read by the kernels from device memory, so one captured
-// graph of a stage's launches serves every snapshot: the stage's records
-// (begin, count) and the round-stamp bases of its closures.
+// graph of a stage's launches serves every snapshot: the stage's records (the
+// device batch slot they were copied to, and their count) and the round-stamp
+// bases of its closures.  A graph never holds a record address itself.
 struct StepArgs {
-    uint64_t begin, count;
+    const dcg_edge *recs;
+    uint64_t count;
     uint32_t stamp[4];
 };
 

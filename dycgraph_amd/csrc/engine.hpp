@@ -82,7 +82,7 @@ struct Engine {
     hipStream_t stream2 = nullptr;          // the in-row delete pass runs here, beside the out-row pass
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     int cu_count = 0;
-    uint32_t coop_grid[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t coop_grid[10] = {};   // persistent grid per mode (5 + mode: static convergence)
     struct Tune {
         uint32_t blocks_per_cu = 1;   // persistent workgroups (1,024 threads) per CU
         uint32_t big_row = kBigRow;   // rows above this are split into pieces
@@ -154,9 +154,34 @@ struct Engine {
     uint32_t *pr_pending = nullptr;    // points at one of d_qa/d_qb/d_qc
     uint64_t pr_pending_count = 0;
 
-    // ---- staged update stream ----
-    DevBuf<dcg_edge> d_adds, d_dels;
+    // ---- update stream ----
+    // The whole stream stays in engine-owned pinned host memory (the role of
+    // the reference's Loader arrays and mapped WEdge buffers); each stage copies
+    // its slice into a device batch slot inside the stage (or finds it there,
+    // copied ahead on copy_stream by prefetch_records).
+    dcg_edge *h_adds = nullptr, *h_dels = nullptr;
     uint64_t n_adds = 0, n_dels = 0;
+    struct BatchSlot {
+        DevBuf<dcg_edge> buf;
+        uint64_t begin = ~0ull, count = 0;   // the slice it holds (~0: none)
+        bool pending = false;                // copied on copy_stream; `ready` not yet waited for
+        hipEvent_t ready = nullptr;          // end of its copy on copy_stream
+        hipEvent_t used = nullptr;           // end of the last work that read it
+        uint64_t stamp = 0;                  // fill order (the older slot is refilled first)
+    };
+    enum { kDels = 0, kAdds = 1 };
+    BatchSlot slots[2][2];                   // [deletions | additions][two slots]
+    uint64_t slot_clock = 0;
+    hipStream_t copy_stream = nullptr;
+    // The device records of [begin, begin+count) of one direction, copied in
+    // engine-stream order unless a prefetch already holds them; null for 0.
+    const dcg_edge *batch_records(int kind, uint64_t begin, uint64_t count);
+    // Copy a slice into the other slot on copy_stream (double buffering).
+    void prefetch_records(int kind, uint64_t begin, uint64_t count);
+    void fill_slot(int kind, BatchSlot &s, uint64_t begin, uint64_t count, hipStream_t on);
+    void forget_batches();
+    void mark_batches_used();
+    void sync_all();   // engine, second, copy and training streams
 
     // ---- phase scratch ----
     DevBuf<uint8_t> d_temp;            // hipcub temporary storage
@@ -194,6 +219,18 @@ struct Engine {
     int time_begin(int kernel, int ctl, double bytes = 0, hipStream_t on = nullptr);   // on: default the engine stream
     void time_end(int slot);
     void collect_timing();
+    // Closures inside a captured stage graph: each replay stamps the device wall
+    // clock at a closure's first workgroup start and last workgroup end in its
+    // Ctl block; after the stage's wait the host copy (h_ctl) gives its time and
+    // its rows and adjacency entries.
+    struct GraphTimer {
+        int kernel;
+        int ctl;
+    };
+    double wall_khz = 100000.0;   // device wall-clock rate (hipDeviceAttributeWallClockRate)
+    std::vector<GraphTimer> graph_timers[2];
+    int capture_stage = -1;
+    void collect_graph_timers(int stage);
 
     explicit Engine(const dcg_config &c);
     ~Engine();
@@ -202,8 +239,8 @@ struct Engine {
     void build_fences(const uint32_t *keys, const uint32_t *vals, uint64_t m, const uint64_t *rp,
                       const uint32_t *sorted, DevBuf<uint32_t> &fbase, DevBuf<uint32_t> &fence);
     void load_edges_device(uint32_t nn, const uint32_t *d_src, const uint32_t *d_dst, uint64_t m);
-    void mutate_delete(uint64_t begin, uint64_t count, dcg_phase_stats *st);
-    void mutate_add(uint64_t begin, uint64_t count, dcg_phase_stats *st);
+    void mutate_delete(const dcg_edge *recs, uint64_t count, dcg_phase_stats *st);
+    void mutate_add(const dcg_edge *recs, uint64_t count, dcg_phase_stats *st);
     void export_csr(uint64_t *row_ptr, uint32_t *col, uint64_t *m);
     StoreView view();
     void fold_reads(dcg_phase_stats *st);   // enqueue + synchronise + finish

@@ -526,6 +526,10 @@ void mlp_release_training(Engine &e) {
     m.train_due = false;
 }
 
+void mlp_sync(Engine &e) {
+    if (e.cache.mlp && e.cache.mlp->train) DCG_HIP(hipStreamSynchronize(e.cache.mlp->train));
+}
+
 void mlp_stats(const Engine &e, dcg_cache_stats *st) {
     const MlpState &m = *e.cache.mlp;
     st->learned_decisions = (uint64_t)(m.decisions - m.fallbacks);
@@ -548,8 +552,10 @@ std::vector<uint64_t> mlp_plan(Engine &e, uint64_t ab, uint64_t ac, uint64_t db,
     }
     DCG_HIP(hipStreamWaitEvent(e.stream, m.trained, 0));   // the forward pass reads the trained weights
     fill<float>(e.stream, e.cache.d_touch.p, C, 0.f);
+    const dcg_edge *adds = e.batch_records(Engine::kAdds, ab, ac), *dels = e.batch_records(Engine::kDels, db, dc);
     hipLaunchKernelGGL(k_touch, dim3(grid_for(ac + dc, 256, 8192)), dim3(256), 0, e.stream, e.d_out.p, e.d_in.p,
-                       e.d_adds.p + ab, ac, e.d_dels.p + db, dc, e.n, e.cache.d_touch.p);
+                       adds, ac, dels, dc, e.n, e.cache.d_touch.p);
+    e.mark_batches_used();
     std::vector<float> lu(C);
     for (uint64_t c = 0; c < C; ++c) lu[c] = (float)e.cache.last_use[c];
     DCG_HIP(hipMemcpyAsync(m.lastuse.p, lu.data(), C * 4, hipMemcpyHostToDevice, e.stream));

@@ -1700,7 +1700,7 @@ static uint32_t wave_grid(uint64_t groups) {
     return (uint32_t)blocks;
 }
 
-void Engine::mutate_delete(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
+void Engine::mutate_delete(const dcg_edge *recs, uint64_t count, dcg_phase_stats *st) {
     if (!count) {
         fill<unsigned long long>(stream, d_counters.p, C_NUM, 0ull);
         return;
@@ -1710,7 +1710,6 @@ void Engine::mutate_delete(uint64_t begin, uint64_t count, dcg_phase_stats *st) 
     d_gstart.ensure(count);
     d_gstart2.ensure(count);
     d_used.ensure(2 * count);   // occurrence counters, out then in direction
-    const dcg_edge *recs = d_dels.p + begin;
     uint64_t *kout = d_k64a.p, *kin = d_k64a.p + count;
     uint32_t *used_out = d_used.p, *used_in = d_used.p + count;
     if (count <= kSmallSort) {
@@ -1794,7 +1793,7 @@ void Engine::finish_delete(uint64_t count, dcg_phase_stats *st) {
     num_edges -= count - h_counters[C_MISSING];
 }
 
-void Engine::mutate_add(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
+void Engine::mutate_add(const dcg_edge *recs, uint64_t count, dcg_phase_stats *st) {
     if (!count) {
         fill<unsigned long long>(stream, d_counters.p, C_NUM, 0ull);
         return;
@@ -1804,7 +1803,6 @@ void Engine::mutate_add(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
     d_v32a.ensure(2 * count);
     d_gstart.ensure(count);
     d_gstart2.ensure(count);
-    const dcg_edge *recs = d_adds.p + begin;
     uint32_t *kout = d_k32a.p, *kin = d_k32a.p + count, *vout = d_v32a.p, *vin = d_v32a.p + count;
     if (count <= kSmallSort) {
         hipLaunchKernelGGL(k_add_prep_small, dim3(2), dim3(1024), 0, stream, recs, (uint32_t)count, n, kout, vout,

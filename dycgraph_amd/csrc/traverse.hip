@@ -721,12 +721,17 @@ constexpr uint32_t kSoloPieces = 64;     // more pending pieces than this end a 
 // (the rest of the grid waits at one grid barrier); otherwise all workgroups
 // share each round, separated by grid barriers (a second one only when the
 // round queued pieces of large rows).
-template <uint32_t MODE>
+// TAG only separates launch names: 1 for static convergence (dcg_start, the
+// autotuner), 0 for incremental closures, so profiles list them apart.
+template <uint32_t MODE, uint32_t TAG = 0>
 __global__ __launch_bounds__(kPBlock) void k_persistent(TravArgs a) {
     __shared__ WaveLds lds_all[kPBlock / kWave];
     __shared__ OutLds O;
     __shared__ WgSum<2> tot;
-    if (threadIdx.x == 0) O.n = O.nf = 0;
+    if (threadIdx.x == 0) {
+        O.n = O.nf = 0;
+        atomicMin(&a.ctl->t_first, (unsigned long long)wall_clock64());
+    }
     wg_sum_init(tot);
     __syncthreads();
     // publish round rr's buffered entries (modes that emit; whole-grid rounds
@@ -737,7 +742,6 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(TravArgs a) {
         else
             __syncthreads();
     };
-    const uint32_t lane = lane_id();
     WaveLds &L = lds_all[threadIdx.x / kWave];
     const Team gt{(blockIdx.x * blockDim.x + threadIdx.x) / kWave, gridDim.x * blockDim.x / kWave,
                   blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x};
@@ -813,6 +817,8 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(TravArgs a) {
     unsigned long long *const dst[2] = {&a.ctl->edges, &a.ctl->expansions};
     const unsigned long long x[2] = {edges, expansions};
     wg_sum_flush(tot, x, dst);
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(&a.ctl->t_last, (unsigned long long)wall_clock64());
 }
 
 // ------------------------------------------------------------- plain kernels
@@ -840,6 +846,8 @@ __global__ void k_ctl_setup(Ctl *c, const uint32_t *src0, uint32_t add0, uint32_
     c->far_scanned = 0;
     c->rmax[0] = c->rmax[1] = 0;
     c->bar_arrive = c->bar_gen = c->bar_timeout = 0;
+    c->t_first = ~0ull;
+    c->t_last = 0;
 }
 
 // Bucket thresholds of a push closure: 0 (everything relaxed waits in the far
@@ -886,10 +894,10 @@ __global__ void k_iota_sources(uint32_t *q0, uint32_t n, uint32_t Q, const uint3
 }
 
 // Delete seeds (driver.cuh:394-441): test every deleted edge against every query.
-__global__ __launch_bounds__(1024) void k_seed_dels(TravArgs a, const dcg_edge *base) {
+__global__ __launch_bounds__(1024) void k_seed_dels(TravArgs a) {
     __shared__ uint32_t wcnt[16], wbase;   // list positions: one atomic per workgroup and step
     const uint64_t count = a.step->count;
-    const dcg_edge *recs = base + a.step->begin;
+    const dcg_edge *recs = a.step->recs;
     const uint64_t total = count * a.Q;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t b = blockIdx.x * (uint64_t)blockDim.x; b < total; b += stride) {
@@ -976,10 +984,10 @@ __global__ void k_cc_fix_parents(TravArgs a, const uint32_t *list, const Ctl *li
 }
 
 // Insertion seeds (framework.cuh:297-370): relax every added edge for every query.
-__global__ __launch_bounds__(1024) void k_seed_adds(TravArgs a, const dcg_edge *base, uint32_t *outq) {
+__global__ __launch_bounds__(1024) void k_seed_adds(TravArgs a, uint32_t *outq) {
     __shared__ uint32_t wcnt[16], wbase;   // list positions: one atomic per workgroup and step
     const uint64_t count = a.step->count;
-    const dcg_edge *recs = base + a.step->begin;
+    const dcg_edge *recs = a.step->recs;
     const uint32_t stamp0 = stamp_base(a);
     const uint64_t total = count * a.Q;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -1038,7 +1046,8 @@ uint32_t Engine::coop_blocks(const void *kernel) {
     return (uint32_t)(per_cu * cu_count);
 }
 
-static const void *persistent_fn(uint32_t mode) {
+static const void *persistent_fn(uint32_t mode, bool is_static) {
+    if (is_static) return mode == M_PR ? (const void *)&k_persistent<M_PR, 1> : (const void *)&k_persistent<M_PUSH, 1>;
     switch (mode) {
     case M_PUSH: return (const void *)&k_persistent<M_PUSH>;
     case M_PULL: return (const void *)&k_persistent<M_PULL>;
@@ -1048,21 +1057,25 @@ static const void *persistent_fn(uint32_t mode) {
     }
 }
 
-void launch_persistent(Engine &e, uint32_t mode, TravArgs &a) {
-    const void *fn = persistent_fn(mode);
-    if (!e.coop_grid[mode]) e.coop_grid[mode] = e.coop_blocks(fn);
+void launch_persistent(Engine &e, uint32_t mode, TravArgs &a, bool is_static = false) {
+    const void *fn = persistent_fn(mode, is_static);
+    const uint32_t gi = mode + (is_static ? 5u : 0u);   // grid sizes per instantiation
+    if (!e.coop_grid[gi]) e.coop_grid[gi] = e.coop_blocks(fn);
     void *args[] = {&a};
     static const int kid[5] = {DCG_K_PUSH, DCG_K_PULL, DCG_K_INVALIDATE, DCG_K_PR_SOLVE, DCG_K_PR_AMEND};
+    const int k = is_static ? DCG_K_STATIC : kid[mode];
     if (mode == M_PUSH && a.far0)
         hipLaunchKernelGGL(k_far_setup, dim3(1), dim3(64), 0, e.stream, a.thr, a.minfar, a.fdelta, a.Q);
     if (e.capturing) {
         // graph node: a plain launch of the same co-resident grid (the grid
-        // barrier needs only co-residency; it gives up after ~10 s otherwise)
-        DCG_HIP(hipLaunchKernel(fn, dim3(e.coop_grid[mode]), dim3(kPBlock), args, 0, e.stream));
+        // barrier needs only co-residency; it gives up after ~10 s otherwise);
+        // every replay times it by the wall-clock stamps in its control block
+        e.graph_timers[e.capture_stage].push_back(Engine::GraphTimer{k, (int)(a.ctl - e.d_ctl.p)});
+        DCG_HIP(hipLaunchKernel(fn, dim3(e.coop_grid[gi]), dim3(kPBlock), args, 0, e.stream));
         return;
     }
-    const int slot = e.time_begin(kid[mode], (int)(a.ctl - e.d_ctl.p));
-    DCG_HIP(hipLaunchCooperativeKernel(fn, dim3(e.coop_grid[mode]), dim3(kPBlock), args, 0, e.stream));
+    const int slot = e.time_begin(k, (int)(a.ctl - e.d_ctl.p));
+    DCG_HIP(hipLaunchCooperativeKernel(fn, dim3(e.coop_grid[gi]), dim3(kPBlock), args, 0, e.stream));
     e.time_end(slot);
 }
 
@@ -1202,12 +1215,12 @@ void Engine::autotune(dcg_tune_result *res) {
         start_min_lattice();
         sync();
         collect_timing();
-        const double ms = kstat[DCG_K_PUSH].ms;
-        kstat[DCG_K_PUSH] = dcg_kernel_stats{};
+        const double ms = kstat[DCG_K_STATIC].ms;
+        kstat[DCG_K_STATIC] = dcg_kernel_stats{};
         return ms;
     };
-    const dcg_kernel_stats keep = kstat[DCG_K_PUSH];
-    kstat[DCG_K_PUSH] = dcg_kernel_stats{};
+    const dcg_kernel_stats keep = kstat[DCG_K_STATIC];
+    kstat[DCG_K_STATIC] = dcg_kernel_stats{};
     double best = measure();
     uint32_t evals = 1;
     const uint32_t bpc[] = {1, 2};
@@ -1263,7 +1276,7 @@ void Engine::autotune(dcg_tune_result *res) {
     start_min_lattice();   // leave the converged state behind
     sync();
     collect_timing();
-    kstat[DCG_K_PUSH] = keep;
+    kstat[DCG_K_STATIC] = keep;
     if (res) {
         res->blocks_per_cu = tune.blocks_per_cu;
         res->big_row = tune.big_row;
@@ -1298,7 +1311,7 @@ void Engine::start_min_lattice() {
     a.in0 = d_qc.p;
     a.ctl = c;
     a.stamp0 = take_stamps(*this);
-    launch_persistent(*this, M_PUSH, a);
+    launch_persistent(*this, M_PUSH, a, true);
     cc_fix_parents(*this, a, nullptr, nullptr);
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipMemcpyAsync(h_ctl, c, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
@@ -1307,17 +1320,29 @@ void Engine::start_min_lattice() {
     fold_reads(nullptr);
 }
 
+// Algorithmic bytes of a stage's closures (control blocks [lo, hi)): 24 B per
+// expanded row, 12 B per adjacency entry.
+static double stage_bytes(const Ctl *c, int lo, int hi) {
+    double b = 0;
+    for (int i = lo; i < hi; ++i) b += 24.0 * (double)c[i].expansions + 12.0 * (double)c[i].edges;
+    return b;
+}
+
 // Delete stage of one snapshot (framework.cuh:4811-4957, re-ordered: the
 // physical deletion runs first; the invalidation then scans post-delete rows,
 // which only drops deleted edges that the seed step already examined).
 void Engine::delete_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
-    mutate_delete(begin, count, st);
+    const dcg_edge *recs = batch_records(kDels, begin, count);
+    mutate_delete(recs, count, st);
     mlp_release_training(*this);   // the learned policy trains beside the repair, not the mutation
     if (!count) return;
     reserve_stamps(*this, 2);
-    h_step[0] = StepArgs{begin, count, {take_stamps(*this), take_stamps(*this), 0u, 0u}};
+    h_step[0] = StepArgs{recs, count, {take_stamps(*this), take_stamps(*this), 0u, 0u}};
     DCG_HIP(hipMemcpyAsync(d_step.p, h_step, sizeof(StepArgs), hipMemcpyHostToDevice, stream));
+    const int rs = time_begin(DCG_K_REPAIR_DELETE, -1);
     run_stage(0);
+    time_end(rs);
+    mark_batches_used();
     fold_enqueue();
     Ctl *c = d_ctl.p;
     DCG_HIP(hipGetLastError());
@@ -1326,6 +1351,8 @@ void Engine::delete_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
     finish_delete(count, st);
     for (int i = 0; i < 4; ++i)
         check_launch(h_ctl[i], "the delete stage");
+    timed[rs].bytes = stage_bytes(h_ctl, 1, 4);
+    if (graph_exec[0]) collect_graph_timers(0);
     if (st) {
         st->affected = h_ctl[1].aff_count;
         st->rounds = h_ctl[1].rounds + h_ctl[3].rounds;
@@ -1338,12 +1365,16 @@ void Engine::delete_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
 
 // Add stage (framework.cuh:3525-3756): append, seed from the added edges, close.
 void Engine::add_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
-    mutate_add(begin, count, st);
+    const dcg_edge *recs = batch_records(kAdds, begin, count);
+    mutate_add(recs, count, st);
     if (!count) return;
     reserve_stamps(*this, 2);
-    h_step[1] = StepArgs{begin, count, {take_stamps(*this), take_stamps(*this), 0u, 0u}};
+    h_step[1] = StepArgs{recs, count, {take_stamps(*this), take_stamps(*this), 0u, 0u}};
     DCG_HIP(hipMemcpyAsync(d_step.p + 1, h_step + 1, sizeof(StepArgs), hipMemcpyHostToDevice, stream));
+    const int rs = time_begin(DCG_K_REPAIR_ADD, -1);
     run_stage(1);
+    time_end(rs);
+    mark_batches_used();
     fold_enqueue();
     Ctl *c = d_ctl.p;
     DCG_HIP(hipGetLastError());
@@ -1352,6 +1383,8 @@ void Engine::add_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
     finish_add(count, st);
     check_launch(h_ctl[4], "the add stage");
     check_launch(h_ctl[5], "the add stage");
+    timed[rs].bytes = stage_bytes(h_ctl, 5, 6);
+    if (graph_exec[1]) collect_graph_timers(1);
     if (st) {
         st->rounds = h_ctl[5].rounds;
         st->bucket_advances = h_ctl[5].advances;
@@ -1375,7 +1408,7 @@ void Engine::enqueue_repair(int stage) {
         hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 0, (const uint32_t *)nullptr, 0u, 0u,
                            (const uint32_t *)nullptr);
         a.ctl = c + 0;
-        hipLaunchKernelGGL(k_seed_dels, dim3(kSeedBlocks), dim3(1024), 0, stream, a, (const dcg_edge *)d_dels.p);
+        hipLaunchKernelGGL(k_seed_dels, dim3(kSeedBlocks), dim3(1024), 0, stream, a);
         hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 1, &c[0].aff_count, 0u, 0u,
                            &c[0].aff_count);
         a.ctl = c + 1;
@@ -1402,8 +1435,7 @@ void Engine::enqueue_repair(int stage) {
                            (const uint32_t *)nullptr);
         a.ctl = c + 4;
         a.stamp_slot = 0;
-        hipLaunchKernelGGL(k_seed_adds, dim3(kSeedBlocks), dim3(1024), 0, stream, a, (const dcg_edge *)d_adds.p,
-                           d_qc.p);
+        hipLaunchKernelGGL(k_seed_adds, dim3(kSeedBlocks), dim3(1024), 0, stream, a, d_qc.p);
         hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 5, &c[4].aff_count, 0u, 0u,
                            (const uint32_t *)nullptr);
         a.ctl = c + 5;
@@ -1428,6 +1460,7 @@ void Engine::run_stage(int stage) {
         bool ok = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
         if (ok) {
             capturing = true;
+            capture_stage = stage;
             try {
                 enqueue_repair(stage);
             } catch (...) {
@@ -1454,6 +1487,7 @@ void Engine::drop_graphs() {
         if (g) (void)hipGraphExecDestroy(g);
         g = nullptr;
     }
+    for (auto &list : graph_timers) list.clear();
 }
 
 // ------------------------------------------------------------------ PageRank
@@ -1510,7 +1544,7 @@ void Engine::start_pagerank() {
     a.ctl = c + 6;
     a.in0 = pr_pending;
     a.max_rounds = cfg.pr_max_rounds;
-    launch_persistent(*this, M_PR, a);
+    launch_persistent(*this, M_PR, a, true);
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipMemcpyAsync(h_ctl + 6, c + 6, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
     sync();
@@ -1753,7 +1787,8 @@ void Engine::pagerank_batch(uint64_t ab, uint64_t ac, uint64_t db, uint64_t dc, 
     hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 0, (const uint32_t *)nullptr, 0u, 0u,
                        (const uint32_t *)nullptr);
     const uint32_t stmp = take_stamps(*this);
-    hipLaunchKernelGGL(k_pr_sources, dim3(grid_for(ac + dc)), dim3(256), 0, stream, d_adds.p + ab, ac, d_dels.p + db,
+    const dcg_edge *adds = batch_records(kAdds, ab, ac), *dels = batch_records(kDels, db, dc);
+    hipLaunchKernelGGL(k_pr_sources, dim3(grid_for(ac + dc)), dim3(256), 0, stream, adds, ac, dels,
                        dc, n, d_tick.p, stmp, srcs, c + 0);
     // Amend(-1): appends after the carried entries in qc (count[1] starts there)
     hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 1, &c[0].aff_count, 0u,
@@ -1768,10 +1803,11 @@ void Engine::pagerank_batch(uint64_t ab, uint64_t ac, uint64_t db, uint64_t dc, 
     DCG_HIP(hipGetLastError());
     // topology mutation of the whole batch
     dcg_phase_stats dst{}, ast{};
-    mutate_delete(db, dc, &dst);
+    mutate_delete(dels, dc, &dst);
     sync();
     finish_delete(dc, &dst);
-    mutate_add(ab, ac, &ast);
+    mutate_add(adds, ac, &ast);
+    mark_batches_used();
     sync();
     finish_add(ac, &ast);
     // Amend(+1) over the new rows; count[1] continues from ctl[1].count[1]

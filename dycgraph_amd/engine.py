@@ -77,7 +77,10 @@ class KernelStats(C.Structure):
 
 
 KERNELS = {"push": 0, "pull": 1, "invalidate": 2, "pr_solve": 3, "pr_amend": 4, "delete_rows": 5,
-           "append_rows": 6, "delete_big": 7, "delete_pass": 8}
+           "append_rows": 6, "delete_big": 7, "delete_pass": 8, "repair_delete": 9, "repair_add": 10,
+           "ingest": 11, "static": 12}
+# records spanning several kernels (not a single launch)
+MULTI_KERNEL = ("delete_pass", "repair_delete", "repair_add")
 
 
 class TuneResult(C.Structure):
@@ -120,6 +123,7 @@ def lib() -> C.CDLL:
         "dcg_load_csr": (I, [P, U32, P, P]),
         "dcg_start": (I, [P]),
         "dcg_load_updates": (I, [P, P, U64, P, U64]),
+        "dcg_prefetch_updates": (I, [P, U64, U64, U64, U64]),
         "dcg_del_edge": (I, [P, U64, U64, C.POINTER(PhaseStats)]),
         "dcg_add_edge": (I, [P, U64, U64, C.POINTER(PhaseStats)]),
         "dcg_update_pagerank": (I, [P, U64, U64, U64, U64, C.POINTER(PhaseStats)]),
@@ -349,6 +353,11 @@ class Engine:
         self._adds, self._dels = adds, dels
         self._check(lib().dcg_load_updates(self._h, _ptr(adds), len(adds), _ptr(dels), len(dels)),
                     "dcg_load_updates")
+
+    def prefetch_updates(self, add_begin, add_count, del_begin, del_count):
+        """Copy a coming snapshot's records to the device on the copy stream."""
+        self._check(lib().dcg_prefetch_updates(self._h, add_begin, add_count, del_begin, del_count),
+                    "dcg_prefetch_updates")
 
     def del_edge(self, begin, count):
         st = PhaseStats()

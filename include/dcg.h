@@ -142,9 +142,18 @@ int dcg_start(dcg_engine *eng);
 
 /* Update stream.  Replaces Loader + Engine::get_update_file
  * (include/framework/Loader.h:30-101, framework.cuh:2387-2418): the whole
- * stream is staged in device memory once; phases address slices of it. */
+ * stream is copied into engine-owned pinned host memory; each stage then
+ * copies its slice [begin, begin+count) to the device inside the stage, as the
+ * reference's del_edge/add_edge fill their mapped WEdge buffers per batch
+ * (framework.cuh:2595-2677).  A second load replaces the stream. */
 int dcg_load_updates(dcg_engine *eng, const dcg_edge *adds, uint64_t num_adds,
                      const dcg_edge *dels, uint64_t num_dels);
+/* Copy the slices of a coming snapshot to the device on the engine's copy
+ * stream, overlapping the snapshot that runs now (double buffering: two device
+ * slots per direction).  The stage that asks for exactly that slice waits for
+ * the copy instead of copying.  No reference counterpart. */
+int dcg_prefetch_updates(dcg_engine *eng, uint64_t add_begin, uint64_t add_count,
+                         uint64_t del_begin, uint64_t del_count);
 
 /* Delete stage of one snapshot: mutation, invalidation, repair.  Replaces
  * Engine::del_edge / update_tree_del (framework.cuh:2595-2618, 4811-4957).
@@ -207,7 +216,13 @@ typedef enum {
     DCG_K_DELETE_PASS = 8,   /* both directions' delete launches together (the in-row pass runs
                                 beside the out-row pass): wall time from the first launch to the
                                 end of the last, with the bytes and host lines of all four */
-    DCG_K_NUM = 9
+    DCG_K_REPAIR_DELETE = 9, /* the delete stage's repair launches together (seeds, invalidation,
+                                reset, pull, push closure, CC parent check): one record per stage,
+                                priced by its closures' rows and adjacency entries */
+    DCG_K_REPAIR_ADD = 10,   /* the add stage's repair launches together (seeds, push closure) */
+    DCG_K_INGEST = 11,       /* host-to-device copies of update records (8 B per record) */
+    DCG_K_STATIC = 12,       /* static convergence closures (dcg_start, the autotuner) */
+    DCG_K_NUM = 13
 } dcg_kernel_id;
 typedef struct {
     uint64_t launches;
@@ -252,10 +267,10 @@ int dcg_set_tuning(dcg_engine *eng, uint32_t blocks_per_cu, uint32_t big_row, ui
 int dcg_set_buckets(dcg_engine *eng, uint32_t quarters);
 /* Replay each stage's repair launches (seeds, invalidation, reset, pull, push
  * closures, CC parent check) as one captured hipGraph (default on; no reference
- * counterpart).  Per-snapshot arguments live in device memory, so the graph
- * captured at the first snapshot serves all later ones.  With graphs on, the
- * closures are not timed per launch (dcg_get_kernel_stats keeps the mutation
- * kernels).  Values never depend on it. */
+ * counterpart).  Per-snapshot arguments (the batch's device records, its
+ * count, round stamps) live in device memory, so the graph captured at the
+ * first snapshot serves all later ones.  Each closure inside a graph is timed by
+ * an event pair captured with it.  Values never depend on it. */
 int dcg_set_graphs(dcg_engine *eng, int on);
 
 /* Link microbenchmarks on `device`: HBM copy rate (read+write bytes / time of a
@@ -267,7 +282,9 @@ int dcg_calibrate(int32_t device, uint64_t bytes, double *hbm_copy_gbs, double *
  * host-placed store.  No reference counterpart (measurement utility). */
 int dcg_calibrate_lines(int32_t device, uint64_t bytes, double *host_lines_per_s, double *hbm_lines_per_s);
 
-/* Synchronise the engine's stream (timing boundaries). */
+/* Wait for every stream of the engine: the engine stream, the second mutation
+ * stream, the batch copy stream and the learned policy's training stream
+ * (timing boundaries). */
 int dcg_sync(dcg_engine *eng);
 
 /* Seeded input generators (no reference counterpart: the reference reads

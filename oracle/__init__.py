@@ -74,7 +74,7 @@ def lib():
             "orc_pagerank": (I, [U32, P, P, D, D, I, P]),
             "orc_value_checksum": (U64, [P, U32]),
             "orc_check_sssp": (None, [U32, P, P, U32, I, U64, U32, P, P, C.POINTER(SsspCheck)]),
-            "orc_check_pagerank": (None, [U32, P, P, D, D, P, P, C.POINTER(PrCheck)]),
+            "orc_check_pagerank": (None, [U32, P, P, D, D, D, P, P, C.POINTER(PrCheck)]),
             "orc_incr_create": (P, [U32, P, P, I, U32, I, U64, U32, D, D, I]),
             "orc_incr_destroy": (None, [P]),
             "orc_incr_set_state": (None, [P, P, P]),
@@ -206,10 +206,13 @@ def check_sssp(n, rp, col, source, dist, parent=None, wmode=W_REFERENCE, seed=1,
     return r.as_dict()
 
 
-def check_pagerank(n, rp, col, rank, residual, base=0.15, eps=1e-6):
+def check_pagerank(n, rp, col, rank, residual, base=0.15, eps=1e-6, max_abs=None):
+    """The reference's acceptance (hybrid_pr_host.cu:9-56) against an FP64 solve,
+    plus max |rank - FP64| <= max_abs when given."""
     r = PrCheck()
     lib().orc_check_pagerank(n, _p(np.ascontiguousarray(rp, np.uint64)), _p(np.ascontiguousarray(col, np.uint32)),
-                             base, eps, _p(np.ascontiguousarray(rank, np.float32)),
+                             base, eps, 0.0 if max_abs is None else float(max_abs),
+                             _p(np.ascontiguousarray(rank, np.float32)),
                              _p(np.ascontiguousarray(residual, np.float32)), C.byref(r))
     return r.as_dict()
 

@@ -367,7 +367,8 @@ void orc_check_sssp(uint32_t n, const uint64_t *rp, const uint32_t *col, uint32_
 /* PageRankCheck (samples/hybrid_pr/hybrid_pr_host.cu:9-56), with the base term
  * and epsilon given in the caller's scale. */
 void orc_check_pagerank(uint32_t n, const uint64_t *rp, const uint32_t *col, double base,
-                        double eps, const float *rank, const float *residual, orc_pr_check *out) {
+                        double eps, double max_abs, const float *rank, const float *residual,
+                        orc_pr_check *out) {
     orc_pr_check r;
     memset(&r, 0, sizeof(r));
     double *ref = (double *)malloc((size_t)(n ? n : 1) * sizeof(double));
@@ -393,8 +394,10 @@ void orc_check_pagerank(uint32_t n, const uint64_t *rp, const uint32_t *col, dou
     double unit = base / 0.15; /* one "vertex" of mass in the caller's scale */
     r.roundoff = 64.0 * FLT_EPSILON * (mass > unit ? mass : unit);
     r.bound = ((double)n * eps + r.roundoff) / 0.15;
+    /* the reference's acceptance (L1 bound, invariant, residual threshold), plus
+     * a max-abs bound against the FP64 solve when the caller gives one */
     r.ok = r.finite && r.converged && r.l1 <= r.bound && r.defect <= r.roundoff &&
-           r.max_residual <= (double)(float)eps;
+           r.max_residual <= (double)(float)eps && (max_abs <= 0 || r.max_error <= max_abs);
     free(ref); free(act); free(next);
     *out = r;
 }

@@ -83,7 +83,8 @@ typedef struct {
     int converged, finite, ok;
 } orc_pr_check;
 void orc_check_pagerank(uint32_t n, const uint64_t *rp, const uint32_t *col, double base,
-                        double eps, const float *rank, const float *residual, orc_pr_check *out);
+                        double eps, double max_abs, const float *rank, const float *residual,
+                        orc_pr_check *out);
 
 /* ---- CPU incremental path (port of the reference's CPU-capable path) ---- */
 enum { ORC_APP_SSSP = 0, ORC_APP_BFS = 1, ORC_APP_CC = 2, ORC_APP_PR = 3 };

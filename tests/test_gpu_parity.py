@@ -21,11 +21,15 @@ def _engine(**kw):
 
 
 def _run_stream(app, n, src, dst, adds_per, dels_per, placement, cache_bytes=0, source=0, wmode=0, seed=1,
-                wmax=100, check_rows=True, stream_seed=7, stream=None):
+                wmax=100, check_rows=True, stream_seed=7, stream=None, policy=1, prefetch=False):
+    """Apply the stream snapshot by snapshot and compare values, rows and parents
+    with the oracle after every stage.  policy 2 (learned) runs the policy's
+    prefetch before each snapshot, as the benchmark does; prefetch=True copies
+    the next snapshot's records ahead on the copy stream."""
     import dycgraph_amd as d
     adds, dels = stream if stream is not None else d.gen_stream(n, src, dst, adds_per, dels_per, seed=stream_seed)
     eng = _engine(app=app, sources=(source,), weight_mode=wmode, weight_seed=seed, weight_max=wmax,
-                  placement=placement, cache_bytes=cache_bytes)
+                  placement=placement, cache_bytes=cache_bytes, cache_policy=policy)
     eng.load_graph(n, src, dst)
     eng.start()
     store = oracle.Store(n, src, dst)
@@ -37,6 +41,10 @@ def _run_stream(app, n, src, dst, adds_per, dels_per, placement, cache_bytes=0, 
     ab = db = 0
     add_stats = []
     for s, (na, nd) in enumerate(zip(adds_per, dels_per)):
+        if prefetch and s + 1 < len(adds_per):
+            eng.prefetch_updates(ab + na, adds_per[s + 1], db + nd, dels_per[s + 1])
+        if policy == d.CACHE_LEARNED:
+            eng.cache_prefetch(ab, na, db, nd)
         st = eng.del_edge(db, nd)
         m = store.apply(dels[db:db + nd], [])
         assert st["missing_deletes"] == m["missing_deletes"]
@@ -84,12 +92,26 @@ def test_rmat12_device(app):
     _run_stream(app, n, src, dst, [300] * 4, [300] * 4, placement=0)
 
 
+@pytest.mark.parametrize("policy", [1, 2])
 @pytest.mark.parametrize("app", ["sssp", "bfs", "cc"])
-def test_rmat12_host_cache(app):
-    announce(f"rmat12 host-cache {app}")
+def test_rmat12_host_cache(app, policy):
+    """Host placement with a one-frame cache under the hotness policy (1) and the
+    learned policy (2, with its prefetch before every snapshot)."""
+    announce(f"rmat12 host-cache {app} policy={policy}")
     n, src, dst = rmat_graph(12, 40000)
     # ~40k out + 40k in slots: 1 frame of 65,536 slots leaves most chunks zero-copy
-    _run_stream(app, n, src, dst, [300] * 4, [300] * 4, placement=1, cache_bytes=1 << 18)
+    _run_stream(app, n, src, dst, [300] * 6, [300] * 6, placement=1, cache_bytes=1 << 18, policy=policy)
+
+
+@pytest.mark.parametrize("app", ["sssp", "cc"])
+def test_prefetched_batches(app):
+    """Each snapshot's records copied ahead on the copy stream while the previous
+    snapshot runs (two device slots per direction, reused in turn); batch sizes
+    change between snapshots, so slots are refilled and regrown."""
+    announce(f"prefetched batches {app}")
+    n, src, dst = rmat_graph(12, 40000, seed=3)
+    sizes = [300, 900, 50, 2000, 300]
+    _run_stream(app, n, src, dst, sizes, sizes[::-1], placement=1, cache_bytes=1 << 18, prefetch=True)
 
 
 @pytest.mark.parametrize("app", ["sssp", "cc"])
@@ -173,19 +195,21 @@ def test_large_row_deletes_exact_rows(dups, placement):
     eng.close()
 
 
+@pytest.mark.parametrize("policy", [1, 2])
 @pytest.mark.parametrize("graphs", [True, False])
-def test_mixed_programs_one_engine(graphs):
+def test_mixed_programs_one_engine(graphs, policy):
     """SSSP, BFS and CC queries (several sources) repaired together over one chunk
     store: every query bit-exact after every stage, with each stage's repair
-    launches replayed from a captured graph or issued directly."""
-    announce(f"mixed programs: sssp x2, bfs x2, cc on rmat12 host-cache, graphs={graphs}")
+    launches replayed from a captured graph or issued directly, under the
+    hotness (1) and the learned (2) cache policy."""
+    announce(f"mixed programs: sssp x2, bfs x2, cc on rmat12 host-cache, graphs={graphs}, policy={policy}")
     import dycgraph_amd as d
     n, src, dst = rmat_graph(12, 40000, seed=9)
     adds, dels = d.gen_stream(n, src, dst, [400] * 3, [400] * 3, seed=3)
     apps = ("sssp", "bfs", "cc", "sssp", "bfs")
     sources = (0, 0, 0, 17, 1234)
     eng = _engine(app="sssp", sources=sources, query_apps=apps, weight_mode=1, weight_seed=5, weight_max=50,
-                  placement=1, cache_bytes=1 << 18)
+                  placement=1, cache_bytes=1 << 18, cache_policy=policy)
     eng.set_tuning(graphs=graphs)
     eng.load_graph(n, src, dst)
     eng.start()
@@ -201,6 +225,8 @@ def test_mixed_programs_one_engine(graphs):
 
     check("static")
     for s in range(3):
+        if policy == d.CACHE_LEARNED:
+            eng.cache_prefetch(400 * s, 400, 400 * s, 400)
         eng.del_edge(400 * s, 400)
         store.apply(dels[400 * s:400 * (s + 1)], [])
         check(f"delete {s}")
@@ -643,4 +669,39 @@ def test_pagerank_dense_sweeps(placement):
         assert chk["ok"], repr((s, chk))
         ab += k
         db += k
+    eng.close()
+
+
+@pytest.mark.parametrize("app", ["sssp", "cc"])
+def test_reload_updates_between_snapshots(app):
+    """A second dcg_load_updates after stages already ran (and their repair
+    launches were captured as graphs): the captured graphs read the batch's
+    device records through the stage arguments, so the replays after the reload
+    see the new stream.  Values and rows match the oracle after every stage."""
+    announce(f"reload updates between snapshots under captured graphs, {app}")
+    import dycgraph_amd as d
+    n, src, dst = rmat_graph(12, 40000, seed=11)
+    eng = _engine(app=app, sources=(0,), placement=1, cache_bytes=1 << 18)
+    eng.set_tuning(graphs=True)
+    eng.load_graph(n, src, dst)
+    eng.start()
+    store = oracle.Store(n, src, dst)
+    for part in range(3):
+        rp, col = store.csr()
+        srcs = np.repeat(np.arange(n, dtype=np.uint32), np.diff(rp).astype(np.int64))
+        order = np.lexsort((col, srcs))   # the generator takes the edge set sorted by (src, dst)
+        per = [250 + 100 * part] * 2
+        adds, dels = d.gen_stream(n, srcs[order], col[order], per, per, seed=40 + part)
+        eng.load_updates(adds, dels)   # replaces the stream: slices index the new one from 0
+        for s, k in enumerate(per):
+            eng.del_edge(s * k, k)
+            store.apply(dels[s * k:(s + 1) * k], [])
+            rp, col = store.csr()
+            assert np.array_equal(eng.gather_value(), oracle_values(app, n, rp, col)), (part, s, "delete")
+            eng.add_edge(s * k, k)
+            store.apply([], adds[s * k:(s + 1) * k])
+            rp, col = store.csr()
+            assert np.array_equal(eng.gather_value(), oracle_values(app, n, rp, col)), (part, s, "add")
+        erp, ecol = eng.csr()
+        assert same_rows(erp, ecol, store), f"rows after part {part}"
     eng.close()

@@ -273,6 +273,81 @@ def test_pagerank_check_accepts_exact_and_rejects_perturbed():
     assert not oracle.check_pagerank(n, rp, col, bad, res)["ok"]
 
 
+# ------------------------------------------------- the reference's own oracles
+def unpack_u32(text):
+    """Values of tests/golden/reference_vectors.json (base64 of zlib of uint32 LE)."""
+    import base64
+    import zlib
+    return np.frombuffer(zlib.decompress(base64.b64decode(text)), dtype="<u4").astype(np.uint32)
+
+
+def reference_vectors():
+    import json
+    with open(os.path.join(GOLDEN, "reference_vectors.json")) as f:
+        return json.load(f)
+
+
+def bfs_kat_stages(doc):
+    """(stage record, current edge list) of the BFS/SSSP KAT graph after each stage."""
+    b = doc["bfs_sssp"]
+    edges = set(map(tuple, b["initial_edges"]))
+    for st in b["stages"]:
+        edges -= set(map(tuple, st["dels"]))
+        edges |= set(map(tuple, st["adds"]))
+        yield st, sorted(edges)
+
+
+def test_reference_vectors_bfs_sssp_values_and_checksums():
+    """tests/bfs_dynamic_smoke.py's oracle() (FIFO BFS / Dijkstra with (u+v)%128+1,
+    then the checksum of hybrid_sssp.cu:335-347) on its KAT graph after every stage:
+    the oracle's BFS, SSSP and checksum give the same values and the same 64-bit
+    checksums, which exercise the <<6 and >>2 terms on non-zero distances."""
+    doc = reference_vectors()
+    n = doc["bfs_sssp"]["n"]
+    for st, edges in bfs_kat_stages(doc):
+        e = np.array(edges, np.int64)
+        rp, col = oracle.csr_from_edges(n, e[:, 0], e[:, 1])
+        bfs = oracle.bfs(n, rp, col, 0)
+        sssp = oracle.sssp(n, rp, col, 0, oracle.W_REFERENCE)
+        assert np.array_equal(bfs, unpack_u32(st["bfs"]["values"])), st["stage"]
+        assert np.array_equal(sssp, unpack_u32(st["sssp"]["values"])), st["stage"]
+        assert oracle.checksum(bfs) == st["bfs"]["checksum"], st["stage"]
+        assert oracle.checksum(sssp) == st["sssp"]["checksum"], st["stage"]
+        assert sssp.max() > 1000   # large distances: every term of the checksum matters
+
+
+def test_reference_vectors_cc_labels_and_checksums():
+    """tests/cc_dynamic_smoke.py's oracle() on its KAT graph and all 20 batches (the
+    last six drawn with random.Random(20260923)), applied through the oracle's
+    multiset store: labels and checksums after every delete and add stage."""
+    doc = reference_vectors()["cc"]
+    n = doc["n"]
+    base = np.array(doc["base"], np.int64)
+    store = oracle.Store(n, base[:, 0], base[:, 1])
+    rp, col = store.csr()
+    lab = oracle.cc(n, rp, col)
+    assert np.array_equal(lab, unpack_u32(doc["initial"]["labels"]))
+    assert oracle.checksum(lab) == doc["initial"]["checksum"]
+    for i, b in enumerate(doc["batches"]):
+        for phase, dels, adds in (("after_delete", b["dels"], []), ("after_add", [], b["adds"])):
+            store.apply(dels=dels, adds=adds)
+            rp, col = store.csr()
+            lab = oracle.cc(n, rp, col)
+            assert np.array_equal(lab, unpack_u32(b[phase]["labels"])), (i, phase)
+            assert oracle.checksum(lab) == b[phase]["checksum"], (i, phase)
+
+
+def test_mix64_is_splitmix64():
+    """orc_mix64(i * 0x9e3779b97f4a7c15) is the i+1-th output of SplitMix64 seeded
+    with 0 (Steele, Lea, Flood 2014); the first four published outputs."""
+    g = 0x9e3779b97f4a7c15
+    want = [0xe220a8397b1dcdaf, 0x6e789e6aa1b965f4, 0x06c45d188009454f, 0xf88bb8a8724c81ec]
+    got = [oracle.lib().orc_mix64((i * g) % 2**64) for i in range(4)]
+    assert got == want
+    # seeded weights follow: 1 + mix64(seed ^ (u << 32 | v)) % W
+    assert oracle.weight(oracle.W_SEEDED, 0, 100, 0, 0) == 1 + want[0] % 100
+
+
 # ------------------------------------------------------------------- checkers
 def test_checksum_closed_forms():
     assert oracle.checksum(np.full(5, INF, np.uint32)) == 1469598103934665603
