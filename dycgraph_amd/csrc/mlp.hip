@@ -20,6 +20,7 @@
 // only; values never depend on it.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <numeric>
 
 #include <hipcub/hipcub.hpp>
@@ -586,6 +587,69 @@ std::vector<uint64_t> mlp_plan(Engine &e, uint64_t ab, uint64_t ac, uint64_t db,
 }
 
 }  // namespace dcg
+
+namespace dcg {
+
+// Workspace of dcg_mlp_eval_device, which allocates nothing: the per-workgroup
+// gradient rows of one pass (kMlpGrid x P_NUM floats, 19.5 MiB) followed by the
+// FP16 copy of the parameters.  Every GPU the library runs on holds its own
+// copy until the process ends.  Host code never passes the array's name to a
+// kernel: it uses the device address hipGetSymbolAddress returns for the
+// current device (scratch_ptr).
+constexpr size_t kScratchPart = (size_t)kMlpGrid * P_NUM;
+constexpr size_t kScratchFloats = kScratchPart + (P_NUM + 1) / 2;
+static __device__ __align__(256) float scratch[kScratchFloats];
+
+constexpr uint32_t kNoAllocMaxRows = 1u << 24;   // rows of one dcg_mlp_eval_device call
+constexpr int kMaxDevices = 64;
+
+// Device address of `scratch` on `device` (the current device), looked up once
+// per device.
+static float *scratch_ptr(int device) {
+    static std::mutex mu;
+    static float *ptr[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices) throw Error(DCG_ERR_ARG, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[device]) {
+        void *p = nullptr;
+        DCG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        ptr[device] = static_cast<float *>(p);
+    }
+    return ptr[device];
+}
+
+}  // namespace dcg
+
+// The same passes as dcg_mlp_eval on caller device buffers, with every
+// intermediate in the library's static workspace: no allocation of any kind.
+// Runs on the null stream and returns after the passes finished.  Calls that
+// overlap on one GPU share its workspace and race.
+extern "C" int dcg_mlp_eval_device(int32_t device, uint32_t rows, const float *d_x, const float *d_y,
+                                   const float *d_params, float *d_out, float *d_grad) {
+    using namespace dcg;
+    if (!d_x || !d_params || !d_out || rows == 0) return DCG_ERR_ARG;
+    if (rows > kNoAllocMaxRows) return DCG_ERR_CAPACITY;
+    if (d_grad && !d_y) return DCG_ERR_ARG;
+    try {
+        DCG_HIP(hipSetDevice(device));
+        float *ws = scratch_ptr(device);
+        float *part = ws;
+        _Float16 *w16 = reinterpret_cast<_Float16 *>(ws + kScratchPart);
+        const hipStream_t s = nullptr;
+        hipLaunchKernelGGL(k_to_half, dim3(grid_for(P_NUM)), dim3(256), 0, s, d_params, w16, P_NUM);
+        hipLaunchKernelGGL(k_mlp_tile, dim3(mlp_grid(rows)), dim3(64), 0, s, d_x, (const float *)nullptr,
+                           (uint64_t)rows, w16, d_params, d_out, (float *)nullptr);
+        if (d_y && d_grad) {
+            // mlp_grid(rows) <= kMlpGrid workgroup rows: within the workspace
+            mlp_gradient(s, d_x, d_y, rows, w16, d_params, part, d_grad);
+        }
+        DCG_HIP(hipGetLastError());
+        DCG_HIP(hipStreamSynchronize(s));
+    } catch (const Error &e) {
+        return e.status;
+    }
+    return DCG_OK;
+}
 
 // Evaluate the predictor on caller-provided rows and FP32 parameters (rounded to
 // FP16 exactly as the policy does): outputs, and with y != NULL the gradient of

@@ -144,6 +144,7 @@ def lib() -> C.CDLL:
         "dcg_set_graphs": (I, [P, I]),
         "dcg_reset_kernel_stats": (I, [P]),
         "dcg_mlp_eval": (I, [C.c_int32, U32, P, P, P, P, P]),
+        "dcg_mlp_eval_device": (I, [C.c_int32, U32, P, P, P, P, P]),
         "dcg_gen_rmat": (I, [U32, U64, C.c_double, C.c_double, C.c_double, U64, P, P]),
         "dcg_gen_rmat_gpu": (I, [C.c_int32, U32, U64, C.c_double, C.c_double, C.c_double, U64, P, P]),
         "dcg_gen_stream": (I, [U32, P, P, U64, U32, P, P, U64, P, P]),

@@ -243,6 +243,17 @@ int dcg_reset_kernel_stats(dcg_engine *eng);
  * reference. */
 int dcg_mlp_eval(int32_t device, uint32_t rows, const float *x, const float *y, const float *params,
                  float *out, float *grad);
+/* The same evaluation on caller DEVICE buffers (x rows*8, y rows*2 or NULL,
+ * params 4,868, out rows*2, grad 4,868 or NULL; grad needs y), allocating
+ * nothing: every intermediate (the per-workgroup gradient rows, the FP16
+ * parameters) lives in a static workspace of the library, 19.5 MiB on each GPU
+ * it runs on, held until the process ends.  rows <= 2^24 (DCG_ERR_CAPACITY
+ * beyond).  Runs on the null stream of `device` and returns when done; calls
+ * that overlap on one GPU share the workspace and race.  Results equal
+ * dcg_mlp_eval's bit for bit.  No reference counterpart: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops). */
+int dcg_mlp_eval_device(int32_t device, uint32_t rows, const float *x, const float *y,
+                        const float *params, float *out, float *grad);
 
 /* Autotuner (no reference counterpart; the reference fixes block 512 and its
  * CTA/warp thresholds, include/utils/graphs/traversal.h:80-90, and its ordered
