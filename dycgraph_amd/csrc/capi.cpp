@@ -46,10 +46,14 @@ Engine::Engine(const dcg_config &c) : cfg(c) {
     Q = app == DCG_APP_PR || (app == DCG_APP_CC && !mixed) ? 1u : std::max(1u, c.num_queries);
     if (c.sources && app != DCG_APP_PR) sources.assign(c.sources, c.sources + Q);
     else sources.assign(Q, 0u);
-    qapps.assign(Q, app);
+    prog = builtin_program();
+    qapps.assign(Q, app == DCG_APP_PR ? 0u : app);
     if (mixed) {
+        apps_given = true;
         for (uint32_t q = 0; q < Q; ++q) {
-            if (c.query_apps[q] > DCG_APP_CC) throw Error(DCG_ERR_ARG, "query_apps entries must be SSSP, BFS or CC");
+            // program indices: SSSP, BFS, CC in the built-in set (checked against
+            // the installed set at graph load)
+            if (c.query_apps[q] >= DCG_PROGRAM_MAX) throw Error(DCG_ERR_ARG, "query_apps entries out of range");
             qapps[q] = c.query_apps[q];
         }
     }
@@ -338,11 +342,37 @@ void dcg_destroy(dcg_engine *h) {
 
 const char *dcg_last_error(const dcg_engine *h) { return h ? h->err.c_str() : "null engine"; }
 
+int dcg_set_program(dcg_engine *h, const dcg_program *p) {
+    return guarded(h, [&](Engine &e) {
+        if (e.loaded) throw Error(DCG_ERR_STATE, "dcg_set_program after a graph load");
+        if (e.app == DCG_APP_PR) throw Error(DCG_ERR_UNSUPPORTED, "PageRank runs the built-in residual push");
+        if (p) {
+            if (p->abi != DCG_PROGRAM_ABI || p->tag != program_layout_tag())
+                throw Error(DCG_ERR_ARG, "program table of another ABI or kernel-argument layout");
+            if (p->num_programs < 1 || p->num_programs > DCG_PROGRAM_MAX || !p->bucket_width)
+                throw Error(DCG_ERR_ARG, "bad program table");
+            for (const void *k : p->kernels)
+                if (!k) throw Error(DCG_ERR_ARG, "program table without a kernel");
+        }
+        e.prog = p ? p : builtin_program();
+        if (!e.apps_given) e.qapps.assign(e.Q, p ? 0u : e.app);
+        for (auto &g : e.coop_grid) g = 0;
+        e.drop_graphs();
+    });
+}
+
+// Query program indices must exist in the installed program set.
+static void check_programs(const Engine &e) {
+    for (uint32_t q : e.qapps)
+        if (q >= e.prog->num_programs) throw Error(DCG_ERR_ARG, "query_apps entry beyond the program set");
+}
+
 int dcg_load_edges_device(dcg_engine *h, uint32_t num_nodes, const uint32_t *d_src, const uint32_t *d_dst,
                           uint64_t num_edges) {
     return guarded(h, [&](Engine &e) {
         if (e.loaded) throw Error(DCG_ERR_STATE, "graph already loaded");
         if (num_nodes == 0 || (num_edges && (!d_src || !d_dst))) throw Error(DCG_ERR_ARG, "bad graph arguments");
+        if (e.app != DCG_APP_PR) check_programs(e);
         e.load_edges_device(num_nodes, d_src, d_dst, num_edges);
         e.alloc_state();
     });
@@ -352,6 +382,7 @@ int dcg_load_edges(dcg_engine *h, uint32_t num_nodes, const uint32_t *src, const
     return guarded(h, [&](Engine &e) {
         if (e.loaded) throw Error(DCG_ERR_STATE, "graph already loaded");
         if (num_nodes == 0 || (num_edges && (!src || !dst))) throw Error(DCG_ERR_ARG, "bad graph arguments");
+        if (e.app != DCG_APP_PR) check_programs(e);
         DevBuf<uint32_t> s, d;
         s.alloc(num_edges ? num_edges : 1);
         d.alloc(num_edges ? num_edges : 1);
@@ -513,8 +544,8 @@ int dcg_gather_parents(dcg_engine *h, uint32_t query, uint32_t *out, size_t n) {
         if (!e.started) throw Error(DCG_ERR_STATE, "gather before dcg_start");
         if (!out || n != e.n || query >= e.Q) throw Error(DCG_ERR_ARG, "gather size/query mismatch");
         if (e.app == DCG_APP_PR) throw Error(DCG_ERR_UNSUPPORTED, "PageRank has no parents");
-        if (e.qapps[query] == DCG_APP_CC) {
-            // CC keeps (label, depth) in the state word and parents apart
+        if (e.rooted(query)) {
+            // rooted programs (CC) keep (value, depth) in the state word and parents apart
             DCG_HIP(hipMemcpyAsync(out, e.d_par.p + (uint64_t)query * e.n, n * 4, hipMemcpyDeviceToHost, e.stream));
             e.sync();
             return;

@@ -95,7 +95,13 @@ struct Engine {
     uint32_t Q = 1;
     uint32_t app = DCG_APP_SSSP;
     std::vector<uint32_t> sources;
-    std::vector<uint32_t> qapps;        // per-query program (SSSP/BFS/CC)
+    std::vector<uint32_t> qapps;        // per-query program index in the program table
+    bool apps_given = false;            // dcg_config.query_apps was set
+    // The program table whose kernels the engine launches (the built-in
+    // SSSP/BFS/CC set unless dcg_set_program installed another).
+    const dcg_program *prog = nullptr;
+    bool rooted(uint32_t q) const { return prog->flags[qapps[q]] & DCG_PROGRAM_ROOTED; }
+    bool vertex_seeds(uint32_t q) const { return prog->flags[qapps[q]] & DCG_PROGRAM_VERTEX_SEEDS; }
     DevBuf<uint32_t> d_sources, d_qapps;
     WeightParams wp{};
     bool loaded = false, started = false;
@@ -287,6 +293,11 @@ struct Engine {
         return d_temp.p;
     }
 };
+
+// The built-in program table (traverse.hip) and the TravArgs/Ctl layout tag a
+// plugin's table must carry.
+const dcg_program *builtin_program();
+uint32_t program_layout_tag();
 
 // Random 64-byte line reads of pinned host memory that one chunk load (256 KiB
 // streamed) costs in host-link time: ~4.6 us at 56 GB/s against ~320 M lines/s.

@@ -122,6 +122,7 @@ def lib() -> C.CDLL:
         "dcg_load_edges_device": (I, [P, U32, P, P, U64]),
         "dcg_load_csr": (I, [P, U32, P, P]),
         "dcg_start": (I, [P]),
+        "dcg_set_program": (I, [P, P]),
         "dcg_load_updates": (I, [P, P, U64, P, U64]),
         "dcg_prefetch_updates": (I, [P, U64, U64, U64, U64]),
         "dcg_del_edge": (I, [P, U64, U64, C.POINTER(PhaseStats)]),
@@ -243,6 +244,25 @@ def gen_stream(n, src, dst, adds_per, dels_per, seed=7):
 MLP_SHAPES = [(8, 64), (64,), (64, 64), (64,), (64, 2), (2,)]
 
 
+_plugins = {}
+
+
+def load_program_plugin(path, symbol):
+    """Load a vertex-program plugin (a shared library built from
+    include/dcg_program_plugin.hpp) and return its dcg_program table pointer;
+    the library stays loaded for the life of the process."""
+    lib()   # the engine's HIP runtime first
+    if path not in _plugins:
+        _plugins[path] = C.CDLL(os.path.abspath(path))
+    fn = getattr(_plugins[path], symbol)
+    fn.restype = C.c_void_p
+    fn.argtypes = []
+    ptr = fn()
+    if not ptr:
+        raise DcgError(-1, f"{symbol} returned no table")
+    return C.c_void_p(ptr)
+
+
 def mlp_eval(x, params, y=None, device=0):
     """Run the policy's predictor on the GPU: returns (out, grads or None)."""
     x = np.ascontiguousarray(x, np.float32)
@@ -336,6 +356,12 @@ class Engine:
 
     def __exit__(self, *exc):
         self.close()
+
+    def set_program(self, table):
+        """Install a program table (a dcg_program pointer from load_program_plugin,
+        or None for the built-in SSSP/BFS/CC set) before load_graph."""
+        self._program = table
+        self._check(lib().dcg_set_program(self._h, table), "dcg_set_program")
 
     # --- LoadGraph / InitGraph / Start
     def load_graph(self, n, src, dst):

@@ -116,6 +116,42 @@ typedef struct {
     double   mlp_capture, hot_capture;
 } dcg_cache_stats;
 
+/* A set of vertex programs: the engine's repair kernels instantiated for up to
+ * 8 min-lattice programs (include/dcg_program.hpp) as device-function handles
+ * that the engine launches.  The built-in set (SSSP, BFS, CC: indices 0, 1, 2 =
+ * dcg_app) is compiled into the library; a plugin built from
+ * include/dcg_program_plugin.hpp exports a table for its own programs.  Replaces
+ * the compile-time TApp parameter of sepgraph::engine::Engine
+ * (framework.cuh:780-783): the reference instantiates the engine for one app
+ * struct; here the table selects the instantiation at run time. */
+#define DCG_PROGRAM_ABI 1
+#define DCG_PROGRAM_MAX 8
+enum { DCG_PROGRAM_ROOTED = 1, DCG_PROGRAM_VERTEX_SEEDS = 2 };
+typedef enum {
+    DCG_PK_PUSH = 0,        /* persistent closures: (TravArgs) */
+    DCG_PK_PULL = 1,
+    DCG_PK_INVALIDATE = 2,
+    DCG_PK_STATIC = 3,      /* static convergence (push) */
+    DCG_PK_SEED_DELS = 4,   /* (TravArgs) */
+    DCG_PK_SEED_ADDS = 5,   /* (TravArgs, uint32_t *frontier) */
+    DCG_PK_RESET = 6,       /* (TravArgs) */
+    DCG_PK_FIX_PARENTS = 7, /* (TravArgs, const uint32_t *list, const void *list_ctl, uint32_t base) */
+    DCG_PK_INIT = 8,        /* (TravArgs) */
+    DCG_PK_FRONTIER = 9,    /* (TravArgs, uint32_t *frontier, const uint64_t *slot0) */
+    DCG_PK_NUM = 10
+} dcg_program_kernel;
+typedef struct {
+    uint32_t abi;                          /* DCG_PROGRAM_ABI */
+    uint32_t num_programs;                 /* 1..DCG_PROGRAM_MAX */
+    uint32_t flags[DCG_PROGRAM_MAX];       /* DCG_PROGRAM_ROOTED | DCG_PROGRAM_VERTEX_SEEDS per program */
+    const char *name;
+    const void *kernels[DCG_PK_NUM];       /* host handles of the device kernels */
+    /* bucket width of program p's ordered closure for the width setting
+     * `quarters` and the weight cap; 0xFFFFFFFF: never ordered */
+    uint32_t (*bucket_width)(uint32_t program, uint32_t quarters, uint32_t weight_max);
+    uint32_t tag;                          /* checked against the library's TravArgs layout */
+} dcg_program;
+
 /* Engine construction.  Replaces Engine::Engine + Engine::SetOptions
  * (framework.cuh:1158-1176, 1245-1262). */
 int dcg_config_init(dcg_config *cfg);
@@ -123,6 +159,13 @@ int dcg_create(const dcg_config *cfg, dcg_engine **out);
 void dcg_destroy(dcg_engine *eng);
 const char *dcg_last_error(const dcg_engine *eng);
 const char *dcg_status_string(int status);
+
+/* Run the engine's queries with a program set (call after dcg_create, before a
+ * graph load; NULL restores the built-in set).  dcg_config.query_apps then hold
+ * program indices below prog->num_programs (NULL: every query runs program 0);
+ * the engine's app must not be PR.  The table must stay valid while the engine
+ * lives.  DCG_ERR_ARG for a table of another ABI or TravArgs layout. */
+int dcg_set_program(dcg_engine *eng, const dcg_program *prog);
 
 /* Graph load.  Replaces Engine::LoadGraph (framework.cuh:1569-1798): rows keep
  * file order (src/utils/parser.cpp:379-400), n = num_nodes.  src/dst may be
