@@ -149,7 +149,7 @@ def engine_config(cfg, n, policy, sources, programs, placement=None):
                           cache_bytes=cfg["cache_bytes"], pr_base=0.15 / n, pr_epsilon=1e-7 / n, pr_max_rounds=100)
 
 
-def run_gpu(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, policy, queries=None):
+def run_gpu(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, policy, queries=None, keep_initial=False):
     import dycgraph_amd as d
     pr = cfg["app"] == "pr"
     sources, programs = queries if queries is not None else queries_of(cfg, n, src)
@@ -160,6 +160,8 @@ def run_gpu(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, policy, queries
     t0 = time.time()
     eng.load_graph(n, src, dst)
     eng.start()
+    # the converged state before any snapshot: the CPU baseline starts from it
+    initial = [(eng.gather_value(q), eng.gather_parent(q)) for q in range(eng.queries)] if keep_initial else None
     eng.load_updates(adds, dels)
     eng.sync()
     log(f"load+start {time.time() - t0:.1f}s")
@@ -204,7 +206,8 @@ def run_gpu(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, policy, queries
     info = eng.info()
     values = [eng.gather_value(q) for q in range(eng.queries)]
     return dict(total_s=total, lat=lat, ks=ks, info=info, hits=hits, host=host, eng=eng, values=values,
-                queries=(sources, programs), closure={k: round(v / steps, 1) for k, v in closure.items()})
+                queries=(sources, programs), closure={k: round(v / steps, 1) for k, v in closure.items()},
+                initial=initial)
 
 
 def run_plain(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, queries):
@@ -263,50 +266,58 @@ def pagerank_accuracy(n, src, dst, adds, dels, ap, dp, eng):
             "reference": "FP64 Jacobi power iteration to sum|dx| <= 1e-12 (probability scale)"}
 
 
-def cpu_baseline(cfg, n, src, dst, adds, dels, warmup, budget_s=20.0, note=""):
-    """The CPU port of the reference's incremental path (oracle/), 1 thread, on the
-    same stream: one incremental state per query (for several programs, each applies
-    the snapshot in turn and the snapshot costs the sum), snapshots after the warmup
-    ones until ~budget_s of CPU work."""
+def host_threads():
+    """Host cores this process may use: the CPU affinity set, capped by
+    OMP_NUM_THREADS when that is set."""
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        cores = min(cores, int(env))
+    return max(1, cores)
+
+
+def cpu_baseline(cfg, n, src, dst, adds, dels, ap, dp, queries, initial, budget_s=20.0):
+    """The reference's CPU path on all host cores (oracle/cpu_path.c: grouped row
+    mutation by a worker pool, parallel invalidation, pull repair and push
+    closures; the same queries over one store) on the same graph and stream.  It
+    starts from the state the GPU engine converged to before the first snapshot
+    (the same values a from-scratch convergence gives) and applies snapshots from
+    the first one, timed one by one, until ~budget_s of CPU work."""
     import oracle
-    rp, col = oracle.csr_from_edges(n, src, dst)
-    codes = {"sssp": oracle.APP_SSSP, "bfs": oracle.APP_BFS, "cc": oracle.APP_CC, "pr": oracle.APP_PR}
-    sources, programs = queries_of(cfg, n, src)
+    codes = {"sssp": oracle.APP_SSSP, "bfs": oracle.APP_BFS, "cc": oracle.APP_CC}
+    sources, programs = queries
     programs = programs or (cfg["app"],) * len(sources)
-    states = [oracle.Incr(n, rp, col, codes[p], source=s, wmode=oracle.W_SEEDED, seed=11, wmax=100,
-                          pr_base=0.15 / n, pr_eps=1e-7 / n, pr_max_rounds=100) for s, p in zip(sources, programs)]
-    for inc in states:
-        inc.converge()
-    na, nd = cfg["adds"], cfg["dels"]
-
-    def snapshot(inc, s):
-        if inc.app == oracle.APP_PR:
-            inc.pr_batch(dels[s * nd:(s + 1) * nd], adds[s * na:(s + 1) * na])
+    threads = host_threads()
+    wmax = 1000 if "grid" in cfg else 100
+    t0 = time.time()
+    cpu = oracle.CpuPath(n, src, dst, threads=threads)
+    for q, (s, p) in enumerate(zip(sources, programs)):
+        k = cpu.add_query(codes[p], s, oracle.W_SEEDED, 11, wmax)
+        if initial is not None:
+            cpu.set_state(k, initial[q][0], initial[q][1])
         else:
-            inc.delete_phase(dels[s * nd:(s + 1) * nd])
-            inc.add_phase(adds[s * na:(s + 1) * na])
-
-    for s in range(warmup):
-        for inc in states:
-            snapshot(inc, s)
-    done, spent = 0, 0.0
-    s = warmup
-    while spent < budget_s and (s + 1) * max(na, nd) <= len(adds):
+            cpu.converge(k)
+    build_s = time.time() - t0
+    aoff = np.concatenate([[0], np.cumsum(ap)]).astype(np.int64)
+    doff = np.concatenate([[0], np.cumsum(dp)]).astype(np.int64)
+    done, spent, upd, times = 0, 0.0, 0, []
+    while done < len(ap) and (done == 0 or spent < budget_s):
         t = time.perf_counter()
-        for inc in states:
-            snapshot(inc, s)
-        spent += time.perf_counter() - t
+        cpu.delete_phase(dels[doff[done]:doff[done + 1]])
+        cpu.add_phase(adds[aoff[done]:aoff[done + 1]])
+        dt = time.perf_counter() - t
+        times.append(round(dt * 1e3, 1))
+        spent += dt
+        upd += int(ap[done] + dp[done])
         done += 1
-        s += 1
-    return dict(value=done * (na + nd) / spent if spent else 0.0, unit="updates/s", cores=1, kind="port",
-                sample=f"{note}{done} snapshots after {warmup} warm-up snapshots, single-threaded oracle port "
-                       f"(delete stage + add stage) of {'+'.join(programs)}, {spent:.2f}s")
-
-
-# The CPU port needs minutes just to build its rows and converge on 2^30 edges,
-# so the RMAT-26 workloads are sampled on RMAT-22 with the same update/edge ratio.
-CPU_SAMPLE = {c: dict(scale=22, edges=1 << 26, app="sssp", programs=("sssp", "bfs", "cc"), adds=31250, dels=31250)
-              for c in ("c3", "c4")}
+        log(f"cpu snapshot {done}: {dt * 1e3:.1f} ms ({threads} threads)")
+    cpu.close()
+    return dict(value=upd / spent if spent else 0.0, unit="updates/s", cores=threads, kind="port",
+                nproc=os.cpu_count(),
+                sample=f"the same graph and stream: snapshots 1..{done} of this run's stream "
+                       f"({upd} updates; per-snapshot ms {times}), from the GPU's converged state before "
+                       f"snapshot 1; oracle/cpu_path.c on {threads} threads ({'+'.join(programs)} over one "
+                       f"store; store built in {build_s:.1f}s, not timed)")
 
 
 def main():
@@ -336,7 +347,9 @@ def main():
     host_lps, hbm_lps = d.calibrate_lines()
     log(f"calibration: HBM copy {hbm_gbs:.0f} GB/s, pinned-host zero-copy read {host_gbs:.1f} GB/s, "
         f"random 64-byte lines: host {host_lps / 1e6:.0f} M/s, HBM {hbm_lps / 1e6:.0f} M/s")
-    res = run_gpu(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, cfg["policy"])
+    # the grid's 65 queries would keep the CPU path busy for minutes per snapshot
+    cpu_wanted = not args.no_cpu and cfg["app"] != "pr" and "grid" not in cfg
+    res = run_gpu(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, cfg["policy"], keep_initial=cpu_wanted)
     upd = int(ap[warmup:].sum() + dp[warmup:].sum())
     value = upd / res["total_s"]
     lat = np.array(res["lat"])
@@ -430,16 +443,10 @@ def main():
             out["exact_check"] = "final SSSP distances == oracle Dijkstra on the oracle's final multiset"
     res["eng"].close()
     cpu = None
-    if not args.no_cpu and cfg["app"] != "pr" and "grid" not in cfg:
-        if args.config in CPU_SAMPLE:
-            sc = CPU_SAMPLE[args.config]
-            sn, ss, sd, sa, sdl, _, _ = make_inputs(sc, warmup + 4)
-            cpu = cpu_baseline(sc, sn, ss, sd, sa, sdl, warmup,
-                               note=f"RMAT-{sc['scale']} sample ({sc['edges']} edges, {sc['adds'] + sc['dels']} "
-                                    f"updates/snapshot, same generator and update/edge ratio): ")
-        else:
-            cpu = cpu_baseline(cfg, n, src, dst, adds, dels, warmup)
+    if cpu_wanted:
+        cpu = cpu_baseline(cfg, n, src, dst, adds, dels, ap, dp, res["queries"], res["initial"])
         out["cpu_baseline"] = {k: (round(v, 1) if isinstance(v, float) else v) for k, v in cpu.items()}
+        out["gpu_over_cpu"] = round(value / cpu["value"], 1) if cpu["value"] else None
     print(json.dumps(out), flush=True)
 
 

@@ -88,6 +88,17 @@ def lib():
             "orc_incr_edges": (U64, [P]),
             "orc_incr_csr": (None, [P, P, P]),
             "orc_mlp_forward_backward": (None, [I, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P]),
+            "orc_cpu_create": (P, [U32, P, P, U64, I]),
+            "orc_cpu_destroy": (None, [P]),
+            "orc_cpu_threads": (I, [P]),
+            "orc_cpu_edges": (U64, [P]),
+            "orc_cpu_add_query": (I, [P, I, U32, I, U64, U32]),
+            "orc_cpu_converge": (None, [P, I]),
+            "orc_cpu_set_state": (None, [P, I, P, P]),
+            "orc_cpu_delete_phase": (U64, [P, P, U64]),
+            "orc_cpu_add_phase": (U64, [P, P, U64]),
+            "orc_cpu_values": (None, [P, I, P, P]),
+            "orc_cpu_csr": (None, [P, P, P]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -215,6 +226,61 @@ def check_pagerank(n, rp, col, rank, residual, base=0.15, eps=1e-6, max_abs=None
                              _p(np.ascontiguousarray(rank, np.float32)),
                              _p(np.ascontiguousarray(residual, np.float32)), C.byref(r))
     return r.as_dict()
+
+
+class CpuPath:
+    """The reference's CPU path on all host cores (cpu_path.c): SSSP/BFS/CC
+    queries over one store; the timed CPU baseline."""
+
+    def __init__(self, n, src, dst, threads=0):
+        self.n = n
+        self._src = np.ascontiguousarray(src, np.uint32)
+        self._dst = np.ascontiguousarray(dst, np.uint32)
+        self._h = lib().orc_cpu_create(n, _p(self._src), _p(self._dst), len(self._src), threads)
+        self._src = self._dst = None
+        self.threads = lib().orc_cpu_threads(self._h)
+        self.queries = 0
+
+    def add_query(self, app, source=0, wmode=W_REFERENCE, seed=1, wmax=100):
+        self.queries += 1
+        return lib().orc_cpu_add_query(self._h, app, source, wmode, seed, wmax)
+
+    def converge(self, q):
+        lib().orc_cpu_converge(self._h, q)
+
+    def set_state(self, q, value, parent=None):
+        v = np.ascontiguousarray(value, np.uint32)
+        p = None if parent is None else np.ascontiguousarray(parent, np.uint32)
+        lib().orc_cpu_set_state(self._h, q, _p(v), None if p is None else _p(p))
+
+    def delete_phase(self, dels):
+        e = _edges(dels)
+        return lib().orc_cpu_delete_phase(self._h, _p(e), len(e))
+
+    def add_phase(self, adds):
+        e = _edges(adds)
+        return lib().orc_cpu_add_phase(self._h, _p(e), len(e))
+
+    def values(self, q):
+        v = np.empty(self.n, np.uint32)
+        p = np.empty(self.n, np.uint32)
+        lib().orc_cpu_values(self._h, q, _p(v), _p(p))
+        return v, p
+
+    def csr(self):
+        m = lib().orc_cpu_edges(self._h)
+        rp = np.empty(self.n + 1, np.uint64)
+        col = np.empty(max(m, 1), np.uint32)
+        lib().orc_cpu_csr(self._h, _p(rp), _p(col))
+        return rp, col[:m]
+
+    def close(self):
+        if self._h:
+            lib().orc_cpu_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 class Incr:

@@ -106,6 +106,24 @@ void orc_incr_pr_values(const orc_incr *e, float *rank, float *residual);
 uint64_t orc_incr_edges(const orc_incr *e);
 void orc_incr_csr(const orc_incr *e, uint64_t *rp, uint32_t *col);
 
+/* ---- the same path on all host cores (cpu_path.c): several SSSP/BFS/CC
+ * queries over one store, the timed CPU baseline.  Rows are built from an edge
+ * list in any order; `threads` <= 0 uses every core OpenMP sees. ---- */
+typedef struct orc_cpu orc_cpu;
+orc_cpu *orc_cpu_create(uint32_t n, const uint32_t *src, const uint32_t *dst, uint64_t m, int threads);
+void orc_cpu_destroy(orc_cpu *c);
+int orc_cpu_threads(const orc_cpu *c);
+uint64_t orc_cpu_edges(const orc_cpu *c);
+/* Returns the query index. */
+int orc_cpu_add_query(orc_cpu *c, int app, uint32_t source, int wmode, uint64_t seed, uint32_t wmax);
+void orc_cpu_converge(orc_cpu *c, int query);
+void orc_cpu_set_state(orc_cpu *c, int query, const uint32_t *value, const uint32_t *parent);
+/* Every query; return the missing deletes / the valid additions. */
+uint64_t orc_cpu_delete_phase(orc_cpu *c, const orc_edge *dels, uint64_t nd);
+uint64_t orc_cpu_add_phase(orc_cpu *c, const orc_edge *adds, uint64_t na);
+void orc_cpu_values(const orc_cpu *c, int query, uint32_t *value, uint32_t *parent);
+void orc_cpu_csr(const orc_cpu *c, uint64_t *rp, uint32_t *col);
+
 /* ---- MLP 8->64->64->2 reference (FP64) for gradient checks ---- */
 /* Weights are given as float (the device keeps FP16 copies); the forward pass,
  * loss and gradients are evaluated in double.  Loss = mean over rows of
