@@ -17,6 +17,7 @@
 
 #include "engine.hpp"
 #include "mlp.hpp"
+#include "radix_sort.hpp"
 
 namespace dcg {
 
@@ -285,6 +286,132 @@ void Engine::cache_prefetch(uint64_t ab, uint64_t ac, uint64_t db, uint64_t dc, 
     cache_apply_plan(desired, st);
     fill_common(*this, st);
     if (st) st->ms = now_ms() - t0;
+}
+
+}  // namespace dcg
+
+// ---- the reference's vertex hotness ranking ------------------------------------
+namespace dcg {
+
+// Device placement (no read list): fold the snapshot's out-row expansions
+// into the window's current byte.
+__global__ void k_window_fold(uint32_t *hot_out, uint32_t *win, uint32_t n) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        const uint32_t h = hot_out[v];
+        if (h) {
+            win[v] = window_add(win[v], h);
+            hot_out[v] = 0;
+        }
+    }
+}
+
+// comp_hotness_sssp (driver.cuh:495-517): score = the window's four counters
+// summed, 0 where the query has not reached the vertex; the sort key
+// (1023 - score) << idbits | v orders by descending score, then ascending id.
+__global__ void k_hot_keys(const uint32_t *win, const uint64_t *st, uint32_t n, uint32_t idbits, uint64_t *keys) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        const uint32_t w = win[v];
+        uint32_t score = (w & 0xFFu) + ((w >> 8) & 0xFFu) + ((w >> 16) & 0xFFu) + (w >> 24);
+        if (st && (uint32_t)(st[v] >> 32) == kInf) score = 0;
+        keys[v] = ((uint64_t)(1023u - score) << idbits) | v;
+    }
+}
+
+constexpr uint32_t kRankChunk = 1024;   // ranked entries per degree-sum block
+
+// extract_vtx_degree (driver.cuh:548-560): live out-degree of each ranked
+// vertex, and the degree sum of every kRankChunk ranked entries.
+__global__ __launch_bounds__(256) void k_hot_degrees(const uint64_t *sorted, uint64_t idmask, const RowDesc *out,
+                                                     uint32_t n, uint32_t *deg, unsigned long long *bsum) {
+    __shared__ unsigned long long part[256 / kWave];
+    const uint64_t base = (uint64_t)blockIdx.x * kRankChunk;
+    unsigned long long s = 0;
+    for (uint32_t j = threadIdx.x; j < kRankChunk; j += blockDim.x) {
+        const uint64_t i = base + j;
+        if (i >= n) break;
+        const uint32_t d = row_live(out[(uint32_t)(sorted[i] & idmask)]);
+        deg[i] = d;
+        s += d;
+    }
+    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, kWave);
+    if (lane_id() == 0) part[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (uint32_t w = 0; w < blockDim.x / kWave; ++w) t += part[w];
+        bsum[blockIdx.x] = t;
+    }
+}
+
+// reset_hotness (driver.cuh:562-579): every counter moves one snapshot back.
+__global__ void k_window_shift(uint32_t *win, uint32_t n) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) win[v] <<= 8;
+}
+
+// A host store folds its listed rows' counters into the window after every
+// stage; a device store lists no rows, so its counters are folded here.
+void Engine::fold_window() {
+    if (cfg.placement == DCG_PLACE_DEVICE)
+        hipLaunchKernelGGL(k_window_fold, dim3(grid_for(n)), dim3(256), 0, stream, d_hot_out.p, d_hot_window.p, n);
+}
+
+uint64_t Engine::hot_candidates(uint32_t query, uint64_t cap, uint32_t *ids, uint64_t ids_len) {
+    fold_window();
+    uint32_t idbits = 1;
+    while (idbits < 32 && (1ull << idbits) < n) ++idbits;
+    DevBuf<uint64_t> keys, sorted;
+    DevBuf<uint32_t> deg;
+    DevBuf<unsigned long long> bsum;
+    DevBuf<uint8_t> tmp;
+    keys.alloc(n);
+    sorted.alloc(n);
+    deg.alloc(n);
+    const uint64_t nb = (n + kRankChunk - 1) / kRankChunk;
+    bsum.alloc(nb);
+    tmp.alloc(radix_sort_temp_bytes(n, false));
+    const uint64_t *st = app == DCG_APP_PR ? nullptr : d_st.p + (uint64_t)query * n;
+    hipLaunchKernelGGL(k_hot_keys, dim3(grid_for(n)), dim3(256), 0, stream, d_hot_window.p, st, n, idbits, keys.p);
+    radix_sort(stream, keys.p, sorted.p, nullptr, nullptr, n, 0, (int)idbits + 10, tmp.p, tmp.n);
+    const uint64_t idmask = (1ull << idbits) - 1;
+    hipLaunchKernelGGL(k_hot_degrees, dim3((uint32_t)nb), dim3(256), 0, stream, sorted.p, idmask, d_out.p, n, deg.p,
+                       bsum.p);
+    DCG_HIP(hipGetLastError());
+    std::vector<unsigned long long> hb(nb);
+    DCG_HIP(hipMemcpyAsync(hb.data(), bsum.p, nb * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+    sync();
+    // CacheCandidateCount (framework.cuh:1321-1350): the ranked vertices whose
+    // inclusive degree prefix stays below the cap; all of them when the total
+    // fits, none without a cache
+    uint64_t count = 0;
+    if (cap > 0) {
+        unsigned long long run = 0;
+        uint64_t b = 0;
+        while (b < nb && run + hb[b] < cap) run += hb[b++];
+        if (b == nb) {
+            count = n;
+        } else {
+            const uint64_t lo = b * kRankChunk, len = std::min<uint64_t>(kRankChunk, n - lo);
+            std::vector<uint32_t> hd(len);
+            DCG_HIP(hipMemcpyAsync(hd.data(), deg.p + lo, len * 4, hipMemcpyDeviceToHost, stream));
+            sync();
+            count = lo;
+            for (uint64_t j = 0; j < len && run + hd[j] < cap; ++j) {
+                run += hd[j];
+                ++count;
+            }
+        }
+    }
+    const uint64_t k = std::min<uint64_t>(count, ids_len);
+    if (k) {
+        std::vector<uint64_t> hk(k);
+        DCG_HIP(hipMemcpyAsync(hk.data(), sorted.p, k * 8, hipMemcpyDeviceToHost, stream));
+        sync();
+        for (uint64_t i = 0; i < k; ++i) ids[i] = (uint32_t)(hk[i] & idmask);
+    }
+    hipLaunchKernelGGL(k_window_shift, dim3(grid_for(n)), dim3(256), 0, stream, d_hot_window.p, n);
+    DCG_HIP(hipGetLastError());
+    sync();
+    return count;
 }
 
 }  // namespace dcg

@@ -522,6 +522,25 @@ int dcg_cache_policy_set(dcg_engine *h, uint32_t policy) {
     });
 }
 
+int dcg_hot_candidates(dcg_engine *h, uint32_t query, uint64_t cap_slots, uint32_t *ids, uint64_t ids_len,
+                       uint64_t *count) {
+    return guarded(h, [&](Engine &e) {
+        if (!e.started) throw Error(DCG_ERR_STATE, "hot candidates before dcg_start");
+        if (query >= e.Q || !count || (ids_len && !ids)) throw Error(DCG_ERR_ARG, "bad hot-candidate arguments");
+        *count = e.hot_candidates(query, cap_slots, ids, ids_len);
+    });
+}
+
+int dcg_hot_window(dcg_engine *h, uint32_t *out, size_t n) {
+    return guarded(h, [&](Engine &e) {
+        if (!e.loaded) throw Error(DCG_ERR_STATE, "no graph loaded");
+        if (!out || n != e.n) throw Error(DCG_ERR_ARG, "window size mismatch");
+        e.fold_window();   // the counters the ranking would fold first
+        DCG_HIP(hipMemcpyAsync(out, e.d_hot_window.p, n * 4, hipMemcpyDeviceToHost, e.stream));
+        e.sync();
+    });
+}
+
 int dcg_gather_values(dcg_engine *h, uint32_t query, void *out, size_t n) {
     return guarded(h, [&](Engine &e) {
         if (!e.started) throw Error(DCG_ERR_STATE, "gather before dcg_start");

@@ -58,7 +58,14 @@ struct StoreView {
     uint32_t *hot_in;          // per-vertex in-row expansions this snapshot
     uint32_t *hot_list;        // rows whose counter left zero (v | direction << 31); null when device-placed
     unsigned long long *hot_count;
+    uint32_t *hot_window;      // per vertex: out-row expansions of the last 4 snapshots (saturating bytes)
 };
+
+// Add h to the current snapshot's byte of a hotness window word (saturating at 255).
+__host__ __device__ __forceinline__ uint32_t window_add(uint32_t w, uint32_t h) {
+    const uint32_t b = (w & 0xFFu) + h;
+    return (w & ~0xFFu) | (b > 255u ? 255u : b);
+}
 
 // Count one read pass over row v (hot is hot_out or hot_in).  With a host
 // store the first count of a row since the last fold lists the row, so the

@@ -115,6 +115,7 @@ struct Engine {
     uint32_t *h_slab = nullptr;        // pinned host slots (DCG_PLACE_HOST)
     DevBuf<uint64_t> d_tail;           // [0] bump tail, [1] overflow flag, [2..] class cursors
     DevBuf<uint32_t> d_hot_out, d_hot_in;
+    DevBuf<uint32_t> d_hot_window;             // reference hotness window (4 saturating bytes per vertex)
     DevBuf<uint32_t> d_hot_list;               // rows read since the last fold (host store only)
     DevBuf<unsigned long long> d_hot_count;
     DevBuf<uint32_t> d_sorted_out, d_sorted_in;   // per row: length of the sorted prefix
@@ -277,6 +278,8 @@ struct Engine {
 
     // cache_policy.hip
     void cache_init();
+    uint64_t hot_candidates(uint32_t query, uint64_t cap, uint32_t *ids, uint64_t ids_len);
+    void fold_window();
     void cache_admit_measured();
     void cache_maintain(dcg_cache_stats *st);
     void cache_prefetch(uint64_t ab, uint64_t ac, uint64_t db, uint64_t dc, dcg_cache_stats *st);

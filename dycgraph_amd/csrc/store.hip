@@ -59,6 +59,7 @@ StoreView Engine::view() {
     v.chunk_frame = cache.d_chunk_frame.p;
     v.hot_out = d_hot_out.p;
     v.hot_in = d_hot_in.p;
+    v.hot_window = d_hot_window.p;
     v.hot_list = h_slab ? d_hot_list.p : nullptr;
     v.hot_count = d_hot_count.p;
     return v;
@@ -331,8 +332,10 @@ void Engine::load_edges_device(uint32_t nn, const uint32_t *d_src, const uint32_
     DCG_HIP(hipMemcpyAsync(d_tail.p + T_TAIL, &used_slots, 8, hipMemcpyHostToDevice, stream));
     d_hot_out.alloc(n ? n : 1);
     d_hot_in.alloc(n ? n : 1);
+    d_hot_window.alloc(n ? n : 1);
     fill<uint32_t>(stream, d_hot_out.p, n, 0u);
     fill<uint32_t>(stream, d_hot_in.p, n, 0u);
+    fill<uint32_t>(stream, d_hot_window.p, n, 0u);
     d_hot_count.alloc(1);
     fill<unsigned long long>(stream, d_hot_count.p, 1, 0ull);
     // each row enters the list at most once per fold: capacity 2n
@@ -1985,6 +1988,7 @@ __global__ void k_fold_reads(StoreView s, const unsigned long long *nlist_p, flo
         uint32_t *hot = in ? s.hot_in : s.hot_out;
         const uint32_t h = hot[v];
         hot[v] = 0;
+        if (!in) s.hot_window[v] = window_add(s.hot_window[v], h);   // a listed row is folded once
         const RowDesc d = in ? s.in[v] : s.out[v];
         uint64_t a = d.off, end = d.off + d.ext;
         while (a < end) {

@@ -131,6 +131,9 @@ def lib() -> C.CDLL:
         "dcg_cache_maintain": (I, [P, C.POINTER(CacheStats)]),
         "dcg_cache_prefetch": (I, [P, U64, U64, U64, U64, C.POINTER(CacheStats)]),
         "dcg_cache_policy_set": (I, [P, U32]),
+        "dcg_hot_candidates": (I, [P, U32, U64, P, U64, C.POINTER(U64)]),
+        "dcg_hot_window": (I, [P, P, C.c_size_t]),
+        "dcg_radix_sort": (I, [C.c_int32, P, P, U64, I, I]),
         "dcg_gather_values": (I, [P, U32, P, C.c_size_t]),
         "dcg_gather_parents": (I, [P, U32, P, C.c_size_t]),
         "dcg_gather_buffers": (I, [P, U32, P, C.c_size_t]),
@@ -261,6 +264,19 @@ def load_program_plugin(path, symbol):
     if not ptr:
         raise DcgError(-1, f"{symbol} returned no table")
     return C.c_void_p(ptr)
+
+
+def radix_sort(keys, vals=None, begin_bit=0, end_bit=64, device=0):
+    """Stable GPU radix sort of uint64 keys (and uint32 values) by bits
+    [begin_bit, end_bit); returns sorted copies."""
+    k = np.array(keys, dtype=np.uint64, copy=True)
+    v = None if vals is None else np.array(vals, dtype=np.uint32, copy=True)
+    if v is not None and len(v) != len(k):
+        raise ValueError("keys and values differ in length")
+    st = lib().dcg_radix_sort(device, _ptr(k), None if v is None else _ptr(v), len(k), begin_bit, end_bit)
+    if st != 0:
+        raise DcgError(st, "dcg_radix_sort failed")
+    return k if v is None else (k, v)
 
 
 def mlp_eval(x, params, y=None, device=0):
@@ -417,6 +433,21 @@ class Engine:
         self._check(lib().dcg_cache_policy_set(self._h, policy), "dcg_cache_policy_set")
 
     # --- Gather*
+    def hot_candidates(self, cap_slots, query=0, limit=None):
+        """The reference's hotness ranking and cache admission: (count, ids),
+        ids = the first min(count, limit) ranked vertices; shifts the window."""
+        limit = self.n if limit is None else limit
+        ids = np.empty(max(limit, 1), np.uint32)
+        cnt = C.c_uint64()
+        self._check(lib().dcg_hot_candidates(self._h, query, cap_slots, _ptr(ids), limit, C.byref(cnt)),
+                    "dcg_hot_candidates")
+        return cnt.value, ids[:min(cnt.value, limit)]
+
+    def hot_window(self):
+        out = np.empty(self.n, np.uint32)
+        self._check(lib().dcg_hot_window(self._h, _ptr(out), self.n), "dcg_hot_window")
+        return out
+
     def gather_value(self, query=0):
         out = np.empty(self.n, np.float32 if self.app == APP_PR else np.uint32)
         self._check(lib().dcg_gather_values(self._h, query, _ptr(out), self.n), "dcg_gather_values")

@@ -222,6 +222,28 @@ int dcg_cache_prefetch(dcg_engine *eng, uint64_t add_begin, uint64_t add_count,
                        uint64_t del_begin, uint64_t del_count, dcg_cache_stats *stats);
 int dcg_cache_policy_set(dcg_engine *eng, uint32_t policy);
 
+/* The reference's vertex hotness ranking and cache admission
+ * (compute_hot_vertices_sssp + CacheCandidateCount, driver.cuh:486-579,
+ * graph_datum.cuh:291-318, framework.cuh:1321-1350), computed on the GPU from
+ * the engine's per-vertex window of out-row expansions in the last four
+ * snapshots (four saturating 8-bit counters per vertex; the current snapshot's
+ * in the low byte): score = the window's sum, 0 for vertices `query` has not
+ * reached (PageRank: no mask); vertices ranked by descending score, ties by
+ * ascending id; the candidates are the longest ranked prefix whose out-degree
+ * sum stays below cap_slots (all when the total fits, none when cap_slots is
+ * 0).  Writes the first min(count, ids_len) ranked ids, sets *count, and then
+ * shifts the window one snapshot (reset_hotness).  The engine's own chunk
+ * cache ranks chunks, not vertices; this is the reference's vertex-granular
+ * ranking for callers that cache rows. */
+int dcg_hot_candidates(dcg_engine *eng, uint32_t query, uint64_t cap_slots, uint32_t *ids, uint64_t ids_len,
+                       uint64_t *count);
+/* The window words (n entries; byte k = expansions k snapshots ago, saturated at 255). */
+int dcg_hot_window(dcg_engine *eng, uint32_t *out, size_t n);
+/* Stable GPU radix sort of caller keys by bits [begin_bit, end_bit), values
+ * (optional) carried along; host arrays sorted in place.  The sort the engine
+ * uses for its rankings, exposed for checks.  No reference counterpart. */
+int dcg_radix_sort(int32_t device, uint64_t *keys, uint32_t *vals, uint64_t n, int begin_bit, int end_bit);
+
 /* Result gather.  Replaces Engine::GatherValue/GatherParent/GatherBuffer
  * (framework.cuh:4959-4984, graph_datum.cuh:346-398).  values: u32 (SSSP/BFS
  * distances, CC labels) or f32 PR ranks; buffers: PR residuals; parents: u32.

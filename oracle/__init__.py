@@ -88,6 +88,7 @@ def lib():
             "orc_incr_edges": (U64, [P]),
             "orc_incr_csr": (None, [P, P, P]),
             "orc_mlp_forward_backward": (None, [I, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P]),
+            "orc_hot_candidates": (U64, [U32, P, P, P, U64, P]),
             "orc_cpu_create": (P, [U32, P, P, U64, I]),
             "orc_cpu_destroy": (None, [P]),
             "orc_cpu_threads": (I, [P]),
@@ -226,6 +227,16 @@ def check_pagerank(n, rp, col, rank, residual, base=0.15, eps=1e-6, max_abs=None
                              _p(np.ascontiguousarray(rank, np.float32)),
                              _p(np.ascontiguousarray(residual, np.float32)), C.byref(r))
     return r.as_dict()
+
+
+def hot_candidates(window, degree, cap, value=None):
+    """The reference's hotness ranking and admission: (count, full ranking)."""
+    w = np.ascontiguousarray(window, np.uint32)
+    d = np.ascontiguousarray(degree, np.uint32)
+    v = None if value is None else np.ascontiguousarray(value, np.uint32)
+    ids = np.empty(len(w), np.uint32)
+    k = lib().orc_hot_candidates(len(w), _p(w), None if v is None else _p(v), _p(d), int(cap), _p(ids))
+    return k, ids
 
 
 class CpuPath:

@@ -402,6 +402,55 @@ void orc_check_pagerank(uint32_t n, const uint64_t *rp, const uint32_t *col, dou
     *out = r;
 }
 
+/* ------------------------------------------------ hotness ranking and admission */
+/* comp_hotness_sssp (driver.cuh:495-517: score = hotness[0..3] summed, 0 where
+ * the buffer is INF unless the program keeps signed residuals), the stable
+ * descending sort of (score, id) (graph_datum.cuh:291-299: ties keep ascending
+ * ids), extract_vtx_degree + ExclusiveSum (driver.cuh:548-560,
+ * graph_datum.cuh:300-318) and CacheCandidateCount (framework.cuh:1321-1350),
+ * written as the reference does it: all vertices when the last exclusive prefix
+ * plus the last degree is below the cap, otherwise the binary search for the
+ * first k whose prefix of k+1 vertices reaches the cap.  window holds the four
+ * 8-bit counters of a vertex in one word; value (may be NULL) masks unreached
+ * vertices.  Fills ids (n entries, the whole ranking) and returns the count. */
+uint64_t orc_hot_candidates(uint32_t n, const uint32_t *window, const uint32_t *value, const uint32_t *degree,
+                            uint64_t cap, uint32_t *ids) {
+    if (n == 0) return 0;
+    uint32_t *score = (uint32_t *)malloc((size_t)n * sizeof(uint32_t));
+    uint64_t bucket[1021];
+    memset(bucket, 0, sizeof(bucket));
+    for (uint32_t v = 0; v < n; ++v) {
+        const uint32_t w = window[v];
+        uint32_t sc = (w & 0xFFu) + ((w >> 8) & 0xFFu) + ((w >> 16) & 0xFFu) + (w >> 24);
+        if (value && value[v] == ORC_INF) sc = 0;
+        score[v] = sc;
+        bucket[sc]++;
+    }
+    /* counting sort, descending score; vertices of one score in ascending id */
+    uint64_t start[1021], pos = 0;
+    for (int sc = 1020; sc >= 0; --sc) { start[sc] = pos; pos += bucket[sc]; }
+    for (uint32_t v = 0; v < n; ++v) ids[start[score[v]]++] = v;
+    free(score);
+    if (cap == 0) return 0;   /* num_of_cache == 0 */
+    uint64_t *sum = (uint64_t *)malloc((size_t)n * sizeof(uint64_t));
+    uint64_t run = 0;
+    for (uint32_t i = 0; i < n; ++i) { sum[i] = run; run += degree[ids[i]]; }
+    uint64_t k;
+    if (sum[n - 1] + degree[ids[n - 1]] < cap) {
+        k = n;
+    } else {
+        uint64_t low = 0, high = n - 1;
+        while (low < high) {
+            const uint64_t middle = low + (high - low) / 2;
+            if (sum[middle + 1] < cap) low = middle + 1;
+            else high = middle;
+        }
+        k = low;
+    }
+    free(sum);
+    return k;
+}
+
 /* ------------------------------------------------------- CPU incremental path */
 struct orc_incr {
     uint32_t n;

@@ -86,6 +86,11 @@ void orc_check_pagerank(uint32_t n, const uint64_t *rp, const uint32_t *col, dou
                         double eps, double max_abs, const float *rank, const float *residual,
                         orc_pr_check *out);
 
+/* ---- hotness ranking and cache admission (driver.cuh:486-579,
+ * framework.cuh:1321-1350); window: four 8-bit counters per vertex ---- */
+uint64_t orc_hot_candidates(uint32_t n, const uint32_t *window, const uint32_t *value, const uint32_t *degree,
+                            uint64_t cap, uint32_t *ids);
+
 /* ---- CPU incremental path (port of the reference's CPU-capable path) ---- */
 enum { ORC_APP_SSSP = 0, ORC_APP_BFS = 1, ORC_APP_CC = 2, ORC_APP_PR = 3 };
 typedef struct orc_incr orc_incr;

@@ -461,3 +461,36 @@ def test_mlp_gradients_central_differences():
             assert abs(num - g.reshape(-1)[idx]) < 1e-3 * max(1.0, abs(num)), (pi, idx, num, g.reshape(-1)[idx])
             checked += 1
     assert checked >= 24
+
+
+# ------------------------------------------------- hotness ranking / admission
+def _window(rng, n):
+    b = rng.integers(0, 256, (n, 4)).astype(np.uint32)
+    b[rng.random(n) < 0.3] = 0                     # many zero scores: ties by id
+    b[rng.random(n) < 0.1] = [255, 255, 255, 255]  # the largest score, tied
+    return b[:, 0] | (b[:, 1] << 8) | (b[:, 2] << 16) | (b[:, 3] << 24)
+
+
+def test_hot_candidates_brute_force():
+    """The ranking is the permutation sorted by (score descending, id ascending)
+    with unreached vertices scored 0; the count is the largest k whose first k
+    ranked degrees sum below the cap (k = n when everything fits, 0 without a
+    cache), found here by trying every k."""
+    rng = np.random.default_rng(12)
+    for trial in range(200):
+        n = int(rng.integers(1, 40))
+        win = _window(rng, n)
+        deg = rng.integers(0, 6, n).astype(np.uint32)
+        value = np.where(rng.random(n) < 0.2, INF, rng.integers(0, 50, n)).astype(np.uint32)
+        mask = None if trial % 3 == 0 else value
+        score = sum(((win >> (8 * k)) & 0xFF).astype(np.int64) for k in range(4))
+        if mask is not None:
+            score = np.where(mask == INF, 0, score)
+        order = sorted(range(n), key=lambda v: (-int(score[v]), v))
+        total = int(deg.sum())
+        for cap in (0, 1, int(rng.integers(1, total + 2)), total, total + 1):
+            k, ids = oracle.hot_candidates(win, deg, cap, mask)
+            assert ids.tolist() == order
+            pref = np.concatenate([[0], np.cumsum(deg[order])])
+            want = 0 if cap == 0 else max(j for j in range(n + 1) if pref[j] < cap)
+            assert k == want, (trial, cap, k, want)
