@@ -17,7 +17,7 @@
 
 #include "engine.hpp"
 #include "mlp.hpp"
-#include "radix_sort.hpp"
+#include "primitives.hpp"
 
 namespace dcg {
 
@@ -368,10 +368,10 @@ uint64_t Engine::hot_candidates(uint32_t query, uint64_t cap, uint32_t *ids, uin
     deg.alloc(n);
     const uint64_t nb = (n + kRankChunk - 1) / kRankChunk;
     bsum.alloc(nb);
-    tmp.alloc(radix_sort_temp_bytes(n, false));
+    tmp.alloc(radix_sort_temp_bytes<uint64_t>(n, false));
     const uint64_t *st = app == DCG_APP_PR ? nullptr : d_st.p + (uint64_t)query * n;
     hipLaunchKernelGGL(k_hot_keys, dim3(grid_for(n)), dim3(256), 0, stream, d_hot_window.p, st, n, idbits, keys.p);
-    radix_sort(stream, keys.p, sorted.p, nullptr, nullptr, n, 0, (int)idbits + 10, tmp.p, tmp.n);
+    radix_sort<uint64_t>(stream, keys.p, sorted.p, nullptr, nullptr, n, 0, (int)idbits + 10, tmp.p, tmp.n);
     const uint64_t idmask = (1ull << idbits) - 1;
     hipLaunchKernelGGL(k_hot_degrees, dim3((uint32_t)nb), dim3(256), 0, stream, sorted.p, idmask, d_out.p, n, deg.p,
                        bsum.p);

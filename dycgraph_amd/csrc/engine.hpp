@@ -191,7 +191,7 @@ struct Engine {
     void sync_all();   // engine, second, copy and training streams
 
     // ---- phase scratch ----
-    DevBuf<uint8_t> d_temp;            // hipcub temporary storage
+    DevBuf<uint8_t> d_temp;            // scratch of the sort, scan and selection primitives
     DevBuf<uint64_t> d_k64a, d_k64b;   // sorted delete keys (out then in) / relocation targets
     DevBuf<uint64_t> d_k64c;           // unsorted delete keys; sorted free pool
     DevBuf<uint32_t> d_k32a, d_v32a;   // add keys / payloads, sorted (out then in)

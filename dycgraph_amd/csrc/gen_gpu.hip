@@ -9,10 +9,10 @@
 //      draw at which the num_edges-th distinct pair appeared (M grows by 25%
 //      and the pass repeats if M draws hold too few distinct pairs);
 //   4. keep the run heads whose first draw is <= T: already sorted by (u, v).
-#include <hipcub/hipcub.hpp>
 
 #include "engine.hpp"
 #include "gen_common.hpp"
+#include "primitives.hpp"
 
 namespace dcg {
 
@@ -88,20 +88,18 @@ extern "C" int dcg_gen_rmat_gpu(int32_t device, uint32_t scale, uint64_t num_edg
             i0.alloc(M);
             i1.alloc(M);
             hipLaunchKernelGGL(k_rmat_keys, dim3(grid_for(M)), dim3(256), 0, s, p, M, k0.p, i0.p);
-            size_t tb = 0;
-            DCG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0.p, k1.p, i0.p, i1.p, (int)M, 0, end_bit, s));
+            size_t tb = radix_sort_temp_bytes<uint64_t>(M, true);
             tmp.alloc(tb);
-            DCG_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, k0.p, k1.p, i0.p, i1.p, (int)M, 0, end_bit, s));
+            radix_sort<uint64_t>(s, k0.p, k1.p, i0.p, i1.p, M, 0, end_bit, tmp.p, tb);
             k0.release();
             i0.release();
             mark.alloc(M);
             fill<uint32_t>(s, mark.p, M, 0u);
             hipLaunchKernelGGL(k_mark_first, dim3(grid_for(M)), dim3(256), 0, s, k1.p, i1.p, M, invalid, mark.p);
             scan.alloc(M);
-            tb = 0;
-            DCG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, mark.p, scan.p, (int)M, s));
+            tb = scan_temp_bytes<uint32_t>(M);
             tmp.alloc(tb);
-            DCG_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, mark.p, scan.p, (int)M, s));
+            exclusive_sum<uint32_t>(s, mark.p, scan.p, M, tmp.p, tb);
             uint32_t last_scan = 0, last_mark = 0;
             DCG_HIP(hipMemcpyAsync(&last_scan, scan.p + M - 1, 4, hipMemcpyDeviceToHost, s));
             DCG_HIP(hipMemcpyAsync(&last_mark, mark.p + M - 1, 4, hipMemcpyDeviceToHost, s));
@@ -122,10 +120,9 @@ extern "C" int dcg_gen_rmat_gpu(int32_t device, uint32_t scale, uint64_t num_edg
             // output positions of the kept heads (sorted order)
             hipLaunchKernelGGL(k_keep_flags, dim3(grid_for(M)), dim3(256), 0, s, k1.p, i1.p, M, invalid, (uint64_t)T,
                                mark.p);
-            tb = 0;
-            DCG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, mark.p, scan.p, (int)M, s));
+            tb = scan_temp_bytes<uint32_t>(M);
             tmp.alloc(tb);
-            DCG_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, mark.p, scan.p, (int)M, s));
+            exclusive_sum<uint32_t>(s, mark.p, scan.p, M, tmp.p, tb);
             DevBuf<uint32_t> ds, dd;
             ds.alloc(num_edges);
             dd.alloc(num_edges);

@@ -23,9 +23,9 @@
 #include <mutex>
 #include <numeric>
 
-#include <hipcub/hipcub.hpp>
 
 #include "mlp.hpp"
+#include "primitives.hpp"
 
 namespace dcg {
 
@@ -73,7 +73,8 @@ struct MlpState {
     double mlp_capture = 0, hot_capture = 0;
     DevBuf<float> mscore, hot_dec;        // the decision's network and hotness scores (judge)
     DevBuf<float> plan;                   // the decision's network score as predicted reads
-    DevBuf<float> sort_k, sort_v;         // judge: scores sorted descending, reads carried along
+    DevBuf<uint32_t> sort_k;              // judge: descending-order keys of a ranking, then sorted keys
+    DevBuf<float> sort_v;                 // judge: reads in ranking order
     DevBuf<double> judge;                 // judge sums: network top-k, hotness top-k, all reads
     DevBuf<float> judge_reads;            // the snapshot's reads, kept for the judge
     DevBuf<uint8_t> judge_tmp;            // radix-sort scratch of the training stream
@@ -425,7 +426,7 @@ void mlp_create(Engine &e) {
     m->mscore.alloc(C);
     m->plan.alloc(C);
     m->hot_dec.alloc(C);
-    m->sort_k.alloc(C);
+    m->sort_k.alloc(2 * C);
     m->sort_v.alloc(C);
     m->judge.alloc(3);
     m->m1.alloc(P_NUM);
@@ -471,14 +472,14 @@ static void judge_launch(Engine &e, MlpState &m) {
     DCG_HIP(hipMemcpyAsync(m.judge_reads.p, e.cache.d_reads.p, C * 4, hipMemcpyDeviceToDevice, e.stream));
     DCG_HIP(hipEventRecord(m.labelled, e.stream));
     DCG_HIP(hipStreamWaitEvent(m.train, m.labelled, 0));
-    size_t tb = 0;
-    DCG_HIP(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tb, m.mscore.p, m.sort_k.p, m.judge_reads.p,
-                                                         m.sort_v.p, (int)C, 0, 32, m.train));
+    const size_t tb = radix_sort_temp_bytes<uint32_t>(C, true);
     m.judge_tmp.ensure(tb);
     const float *ranks[2] = {m.mscore.p, m.hot_dec.p};
     for (int r = 0; r < 2; ++r) {
-        DCG_HIP(hipcub::DeviceRadixSort::SortPairsDescending(m.judge_tmp.p, tb, ranks[r], m.sort_k.p, m.judge_reads.p,
-                                                             m.sort_v.p, (int)C, 0, 32, m.train));
+        // stable descending sort of the scores, the reads (as bits) carried along
+        float_desc_keys(m.train, ranks[r], m.sort_k.p, C);
+        radix_sort<uint32_t>(m.train, m.sort_k.p, m.sort_k.p + C, reinterpret_cast<const uint32_t *>(m.judge_reads.p),
+                             reinterpret_cast<uint32_t *>(m.sort_v.p), C, 0, 32, m.judge_tmp.p, tb);
         hipLaunchKernelGGL(k_judge_sum, dim3(1), dim3(1024), 0, m.train, m.sort_v.p, C, k, m.judge.p, r);
     }
     DCG_HIP(hipGetLastError());

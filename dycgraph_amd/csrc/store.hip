@@ -11,9 +11,9 @@
 //    per-class bump allocator so blocks stay aligned to their capacity.
 // Both the out-rows and the in-rows (reverse index, the role of
 // include/framework/dynamic_reverse_index.h) live in the same slot space.
-#include <hipcub/hipcub.hpp>
 
 #include "engine.hpp"
+#include "primitives.hpp"
 
 namespace dcg {
 
@@ -49,6 +49,8 @@ constexpr int T_TAIL = 0, T_OVERFLOW = 1, T_COUNT = 2, T_CURSOR = 2 + kClasses, 
               T_RETIRED = 2 + 6 * kClasses, T_NUM = 3 + 6 * kClasses;
 static_assert(T_NUM == Engine::kTailWords, "pinned tail mirror size");
 constexpr uint64_t kPoolOff = (1ull << 40) - 1;   // pool key = class << 40 | slot offset
+constexpr int kPoolKeyBits = 46;                   // class < 64; taken entries (~0) sort last
+static_assert(kClasses < 64, "pool classes fit 6 bits below the taken marker");
 
 StoreView Engine::view() {
     StoreView v;
@@ -204,10 +206,8 @@ void Engine::build_fences(const uint32_t *keys, const uint32_t *vals, uint64_t m
     caps.alloc((uint64_t)n + 1);
     fbase.alloc((uint64_t)n + 1);
     hipLaunchKernelGGL(k_fence_caps, dim3(grid_for((uint64_t)n + 1)), dim3(256), 0, stream, sorted, n, caps.p);
-    size_t tb = 0;
-    DCG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, caps.p, fbase.p, (int)n + 1, stream));
-    void *t = temp(tb);
-    DCG_HIP(hipcub::DeviceScan::ExclusiveSum(t, tb, caps.p, fbase.p, (int)n + 1, stream));
+    const size_t tb = scan_temp_bytes<uint32_t>((uint64_t)n + 1);
+    exclusive_sum<uint32_t>(stream, caps.p, fbase.p, (uint64_t)n + 1, temp(tb), tb);
     uint32_t total = 0;
     DCG_HIP(hipMemcpyAsync(&total, fbase.p + n, 4, hipMemcpyDeviceToHost, stream));
     sync();
@@ -246,12 +246,10 @@ void Engine::load_edges_device(uint32_t nn, const uint32_t *d_src, const uint32_
     DevBuf<uint64_t> deg64;
     deg64.alloc(rows + 1);
     hipLaunchKernelGGL(k_widen, dim3(grid_for(rows)), dim3(256), 0, stream, deg.p, deg64.p, rows);
-    size_t tb = 0;
     // out rows: rp[0..n), in rows: rp[n+1 .. 2n+1) (each half scanned separately)
-    DCG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, deg64.p, rp.p, (int)(n ? n : 1), stream));
-    void *t = temp(tb);
-    DCG_HIP(hipcub::DeviceScan::ExclusiveSum(t, tb, deg64.p, rp.p, (int)n, stream));
-    DCG_HIP(hipcub::DeviceScan::ExclusiveSum(t, tb, deg64.p + n, rp.p + n + 1, (int)n, stream));
+    size_t tb = scan_temp_bytes<uint64_t>(n);
+    exclusive_sum<uint64_t>(stream, deg64.p, rp.p, n, temp(tb), tb);
+    exclusive_sum<uint64_t>(stream, deg64.p + n, rp.p + n + 1, n, temp(tb), tb);
 
     // Stable sorts: out-CSR by src (payload dst), in-CSR by dst (payload src).
     DevBuf<uint32_t> ks, vs;
@@ -266,15 +264,12 @@ void Engine::load_edges_device(uint32_t nn, const uint32_t *d_src, const uint32_
     caps.alloc(rows ? rows : 1);
     offs.alloc(rows ? rows : 1);
     hipLaunchKernelGGL(k_place_keys, dim3(grid_for(rows)), dim3(256), 0, stream, deg.p, rows, pkeys.p);
-    tb = 0;
-    DCG_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, pkeys.p, pkeys2.p, (int)rows, 0, 64, stream));
-    t = temp(tb);
-    DCG_HIP(hipcub::DeviceRadixSort::SortKeys(t, tb, pkeys.p, pkeys2.p, (int)rows, 0, 64, stream));
+    // key = (63 - class) << 36 | row with rows < 2^32, ~0 for empty rows: bits [0, 42)
+    tb = radix_sort_temp_bytes<uint64_t>(rows, false);
+    radix_sort<uint64_t>(stream, pkeys.p, pkeys2.p, nullptr, nullptr, rows, 0, 42, temp(tb), tb);
     hipLaunchKernelGGL(k_place_caps, dim3(grid_for(rows)), dim3(256), 0, stream, pkeys2.p, rows, caps.p);
-    tb = 0;
-    DCG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, caps.p, offs.p, (int)rows, stream));
-    t = temp(tb);
-    DCG_HIP(hipcub::DeviceScan::ExclusiveSum(t, tb, caps.p, offs.p, (int)rows, stream));
+    tb = scan_temp_bytes<uint64_t>(rows);
+    exclusive_sum<uint64_t>(stream, caps.p, offs.p, rows, temp(tb), tb);
     uint64_t last_off = 0, last_cap = 0;
     if (rows) {
         DCG_HIP(hipMemcpyAsync(&last_off, offs.p + rows - 1, 8, hipMemcpyDeviceToHost, stream));
@@ -301,10 +296,9 @@ void Engine::load_edges_device(uint32_t nn, const uint32_t *d_src, const uint32_
     DevBuf<uint32_t> slots;
     slots.alloc(total_slots);
     // out rows
-    tb = 0;
-    DCG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, d_src, ks.p, d_dst, vs.p, (int)m, 0, kb, stream));
-    t = temp(tb);
-    DCG_HIP(hipcub::DeviceRadixSort::SortPairs(t, tb, d_src, ks.p, d_dst, vs.p, (int)m, 0, kb, stream));
+    tb = radix_sort_temp_bytes<uint32_t>(m, true);
+    void *t = temp(tb);
+    radix_sort<uint32_t>(stream, d_src, ks.p, d_dst, vs.p, m, 0, kb, t, tb);
     hipLaunchKernelGGL(k_scatter_rows, dim3(grid_for(m)), dim3(256), 0, stream, ks.p, vs.p, m, rp.p, d_out.p, slots.p);
     d_sorted_out.alloc(n ? n : 1);
     d_sorted_in.alloc(n ? n : 1);
@@ -314,7 +308,7 @@ void Engine::load_edges_device(uint32_t nn, const uint32_t *d_src, const uint32_
                        d_sorted_out.p);
     build_fences(ks.p, vs.p, m, rp.p, d_sorted_out.p, d_fbase_out, d_fence_out);
     // in rows
-    DCG_HIP(hipcub::DeviceRadixSort::SortPairs(t, tb, d_dst, ks.p, d_src, vs.p, (int)m, 0, kb, stream));
+    radix_sort<uint32_t>(stream, d_dst, ks.p, d_src, vs.p, m, 0, kb, t, tb);
     hipLaunchKernelGGL(k_scatter_rows, dim3(grid_for(m)), dim3(256), 0, stream, ks.p, vs.p, m, rp.p + n + 1, d_in.p,
                        slots.p);
     hipLaunchKernelGGL(k_sorted_prefix, dim3(grid_for(m)), dim3(256), 0, stream, ks.p, vs.p, m, rp.p + n + 1,
@@ -364,66 +358,75 @@ void Engine::load_edges_device(uint32_t nn, const uint32_t *d_src, const uint32_
 }
 
 // ------------------------------------------------------------- mutation kernels
-// Keys of both directions in one array, out keys first and in keys with the
-// top bit set, so one sort orders each direction and keeps them apart.  A
-// record with an id out of range keys the top of its direction's range (out:
-// the largest key below the in keys, in: ~0) and counts as missing / invalid;
-// the split pass after the sort restores ~0 for those and clears the top bit.
-constexpr uint64_t kDirBit64 = 1ull << 63, kOutTop64 = kDirBit64 - 1;
-constexpr uint32_t kDirBit32 = 1u << 31, kOutTop32 = kDirBit32 - 1;
+// Keys of both directions in one array, sorted together.  A packed key holds
+// the row (and, for deletes, the column) in just the bits the vertex count
+// needs, with a direction bit above the row field, so out keys sort below in
+// keys and one sort over [0, direction bit] orders each direction and keeps
+// them apart in as few 8-bit passes as the graph allows.  The row field has a
+// spare top bit: a record with an id out of range keys all ones there (the top
+// of its direction) and counts as missing / invalid.  The split pass after the
+// sort restores the layout the mutation kernels read (row << 32 | column for
+// deletes, the row for appends) and ~0 for the out-of-range records.
+//
+// Delete keys (nb = bits of a vertex id): column in [0, nb), row in
+// [nb, 2 nb + 1), direction bit 2 nb + 1.
+__host__ __device__ __forceinline__ uint32_t del_dir_bit(uint32_t nb) { return 2 * nb + 1; }
+// Append keys: row in [0, dir), direction bit dir = min(nb + 1, 31) (vertex
+// ids stay below 2^31 - 1, so a 31-bit row field keeps its all-ones top).
+__host__ __device__ __forceinline__ uint32_t add_dir_bit(uint32_t nb) { return nb + 1 < 31 ? nb + 1 : 31; }
 
-__global__ void k_del_keys(const dcg_edge *recs, uint64_t count, uint32_t n, uint64_t *keys,
+__global__ void k_del_keys(const dcg_edge *recs, uint64_t count, uint32_t n, uint32_t nb, uint64_t *keys,
                            unsigned long long *counters) {
+    const uint64_t dirbit = 1ull << del_dir_bit(nb);
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count;
          i += (uint64_t)gridDim.x * blockDim.x) {
         const dcg_edge e = recs[i];
         if (e.u >= n || e.v >= n) {
-            keys[i] = kOutTop64;
-            keys[count + i] = ~0ull;
+            keys[i] = dirbit - 1;
+            keys[count + i] = (dirbit << 1) - 1;
             atomicAdd(&counters[C_MISSING], 1ull);
         } else {
-            keys[i] = ((uint64_t)e.u << 32) | e.v;
-            keys[count + i] = kDirBit64 | ((uint64_t)e.v << 32) | e.u;
+            keys[i] = ((uint64_t)e.u << nb) | e.v;
+            keys[count + i] = dirbit | ((uint64_t)e.v << nb) | e.u;
         }
     }
 }
 
-__global__ void k_split_del_keys(uint64_t *keys, uint64_t count) {
+__global__ void k_split_del_keys(uint64_t *keys, uint64_t count, uint32_t nb) {
+    const uint64_t dirbit = 1ull << del_dir_bit(nb), colmask = (1ull << nb) - 1;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < 2 * count;
          i += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t x = keys[i];
-        if (i < count) {
-            if (x == kOutTop64) keys[i] = ~0ull;
-        } else if (x != ~0ull) {
-            keys[i] = x & ~kDirBit64;
-        }
+        const uint64_t top = i < count ? dirbit - 1 : (dirbit << 1) - 1;
+        keys[i] = x == top ? ~0ull : (((x & (dirbit - 1)) >> nb) << 32) | (x & colmask);
     }
 }
 
 // Append pairs: row u with payload v (out), then row v with payload u (in).
-__global__ void k_add_keys(const dcg_edge *recs, uint64_t count, uint32_t n, uint32_t *keys, uint32_t *vals,
-                           unsigned long long *counters) {
+__global__ void k_add_keys(const dcg_edge *recs, uint64_t count, uint32_t n, uint32_t nb, uint32_t *keys,
+                           uint32_t *vals, unsigned long long *counters) {
+    const uint32_t dirbit = 1u << add_dir_bit(nb);
+    const uint32_t in_top = (uint32_t)(((uint64_t)dirbit << 1) - 1);
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count;
          i += (uint64_t)gridDim.x * blockDim.x) {
         const dcg_edge e = recs[i];
         const bool ok = e.u < n && e.v < n;
-        keys[i] = ok ? e.u : kOutTop32;
+        keys[i] = ok ? e.u : dirbit - 1;
         vals[i] = e.v;
-        keys[count + i] = ok ? (kDirBit32 | e.v) : ~0u;
+        keys[count + i] = ok ? (dirbit | e.v) : in_top;
         vals[count + i] = e.u;
         if (!ok) atomicAdd(&counters[C_INVALID], 1ull);
     }
 }
 
-__global__ void k_split_add_keys(uint32_t *keys, uint64_t count) {
+__global__ void k_split_add_keys(uint32_t *keys, uint64_t count, uint32_t nb) {
+    const uint32_t dirbit = 1u << add_dir_bit(nb);
+    const uint32_t in_top = (uint32_t)(((uint64_t)dirbit << 1) - 1);
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < 2 * count;
          i += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t x = keys[i];
-        if (i < count) {
-            if (x == kOutTop32) keys[i] = ~0u;
-        } else if (x != ~0u) {
-            keys[i] = x & ~kDirBit32;
-        }
+        const uint32_t top = i < count ? dirbit - 1 : in_top;
+        keys[i] = x == top ? ~0u : (x & (dirbit - 1));
     }
 }
 
@@ -488,11 +491,8 @@ static void group_heads(Engine &e, const K *keys, uint64_t count, uint32_t *gsta
     }
     e.d_flags.ensure(count);
     hipLaunchKernelGGL(k_head_flags<K>, dim3(grid_for(count)), dim3(256), 0, e.stream, keys, count, e.d_flags.p);
-    hipcub::CountingInputIterator<uint32_t> idx(0u);
-    size_t tb = 0;
-    DCG_HIP(hipcub::DeviceSelect::Flagged(nullptr, tb, idx, e.d_flags.p, gstart, ngroups, (int)count, e.stream));
-    void *t = e.temp(tb);
-    DCG_HIP(hipcub::DeviceSelect::Flagged(t, tb, idx, e.d_flags.p, gstart, ngroups, (int)count, e.stream));
+    const size_t tb = select_temp_bytes(count);
+    select_flagged(e.stream, e.d_flags.p, count, gstart, ngroups, e.temp(tb), tb);
 }
 
 template <typename K>
@@ -1722,13 +1722,13 @@ void Engine::mutate_delete(const dcg_edge *recs, uint64_t count, dcg_phase_stats
         fill<unsigned long long>(stream, d_counters.p, C_NUM, 0ull);
         d_k64c.ensure(2 * count);
         // both directions' keys unsorted in d_k64c, one sort into kout | kin
-        hipLaunchKernelGGL(k_del_keys, dim3(grid_for(count)), dim3(256), 0, stream, recs, count, n, d_k64c.p,
+        const uint32_t nb = (uint32_t)bits_for(n ? n - 1 : 0);
+        hipLaunchKernelGGL(k_del_keys, dim3(grid_for(count)), dim3(256), 0, stream, recs, count, n, nb, d_k64c.p,
                            d_counters.p);
-        size_t tb = 0;
-        DCG_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, d_k64c.p, kout, (int)(2 * count), 0, 64, stream));
-        void *t = temp(tb);
-        DCG_HIP(hipcub::DeviceRadixSort::SortKeys(t, tb, d_k64c.p, kout, (int)(2 * count), 0, 64, stream));
-        hipLaunchKernelGGL(k_split_del_keys, dim3(grid_for(2 * count)), dim3(256), 0, stream, kout, count);
+        const size_t tb = radix_sort_temp_bytes<uint64_t>(2 * count, false);
+        radix_sort<uint64_t>(stream, d_k64c.p, kout, nullptr, nullptr, 2 * count, 0, (int)del_dir_bit(nb) + 1,
+                             temp(tb), tb);
+        hipLaunchKernelGGL(k_split_del_keys, dim3(grid_for(2 * count)), dim3(256), 0, stream, kout, count, nb);
         group_heads<uint64_t>(*this, kout, count, d_gstart.p, d_counters.p + C_GROUPS_OUT);
         group_heads<uint64_t>(*this, kin, count, d_gstart2.p, d_counters.p + C_GROUPS_IN);
         fill<uint32_t>(stream, d_used.p, 2 * count, 0u);
@@ -1816,15 +1816,13 @@ void Engine::mutate_add(const dcg_edge *recs, uint64_t count, dcg_phase_stats *s
         d_v32c.ensure(2 * count);
         // both directions' (key, payload) pairs unsorted in the c buffers, one
         // stable sort into kout | kin (payloads into vout | vin)
-        hipLaunchKernelGGL(k_add_keys, dim3(grid_for(count)), dim3(256), 0, stream, recs, count, n, d_k32c.p,
+        const uint32_t nb = (uint32_t)bits_for(n ? n - 1 : 0);
+        hipLaunchKernelGGL(k_add_keys, dim3(grid_for(count)), dim3(256), 0, stream, recs, count, n, nb, d_k32c.p,
                            d_v32c.p, d_counters.p);
-        size_t tb = 0;
-        DCG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, d_k32c.p, kout, d_v32c.p, vout, (int)(2 * count), 0,
-                                                   32, stream));
-        void *t = temp(tb);
-        DCG_HIP(hipcub::DeviceRadixSort::SortPairs(t, tb, d_k32c.p, kout, d_v32c.p, vout, (int)(2 * count), 0, 32,
-                                                   stream));
-        hipLaunchKernelGGL(k_split_add_keys, dim3(grid_for(2 * count)), dim3(256), 0, stream, kout, count);
+        const size_t tb = radix_sort_temp_bytes<uint32_t>(2 * count, true);
+        radix_sort<uint32_t>(stream, d_k32c.p, kout, d_v32c.p, vout, 2 * count, 0, (int)add_dir_bit(nb) + 1,
+                             temp(tb), tb);
+        hipLaunchKernelGGL(k_split_add_keys, dim3(grid_for(2 * count)), dim3(256), 0, stream, kout, count, nb);
         group_heads<uint32_t>(*this, kout, count, d_gstart.p, d_counters.p + C_GROUPS_OUT);
         group_heads<uint32_t>(*this, kin, count, d_gstart2.p, d_counters.p + C_GROUPS_IN);
         hipLaunchKernelGGL(k_add_plan, dim3(grid_for(count, 256, 1024)), dim3(256), 0, stream, d_out.p, kout,
@@ -1849,10 +1847,8 @@ void Engine::mutate_add(const dcg_edge *recs, uint64_t count, dcg_phase_stats *s
     } else {
         fill<uint64_t>(stream, d_tail.p + T_FREE, kClasses, 0ull);
         d_k64c.ensure(pool_len);
-        size_t pb = 0;
-        DCG_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, pb, d_pool.p, d_k64c.p, (int)pool_len, 0, 64, stream));
-        void *pt = temp(pb);
-        DCG_HIP(hipcub::DeviceRadixSort::SortKeys(pt, pb, d_pool.p, d_k64c.p, (int)pool_len, 0, 64, stream));
+        const size_t pb = radix_sort_temp_bytes<uint64_t>(pool_len, false);
+        radix_sort<uint64_t>(stream, d_pool.p, d_k64c.p, nullptr, nullptr, pool_len, 0, kPoolKeyBits, temp(pb), pb);
         hipLaunchKernelGGL(k_pool_segments, dim3(grid_for(pool_live, 256, 4096)), dim3(256), 0, stream, d_k64c.p,
                            pool_live, d_tail.p);
         hipLaunchKernelGGL(k_pool_counts, dim3(1), dim3(64), 0, stream, d_tail.p);
@@ -1947,10 +1943,8 @@ void Engine::export_csr(uint64_t *row_ptr, uint32_t *col, uint64_t *m) {
     rp.alloc(n + 1);
     fill<uint64_t>(stream, deg.p, n + 1, 0ull);
     hipLaunchKernelGGL(k_export_deg, dim3(grid_for(n)), dim3(256), 0, stream, d_out.p, n, deg.p);
-    size_t tb = 0;
-    DCG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, deg.p, rp.p, (int)(n + 1), stream));
-    void *t = temp(tb);
-    DCG_HIP(hipcub::DeviceScan::ExclusiveSum(t, tb, deg.p, rp.p, (int)(n + 1), stream));
+    const size_t tb = scan_temp_bytes<uint64_t>((uint64_t)n + 1);
+    exclusive_sum<uint64_t>(stream, deg.p, rp.p, (uint64_t)n + 1, temp(tb), tb);
     uint64_t total = 0;
     DCG_HIP(hipMemcpyAsync(&total, rp.p + n, 8, hipMemcpyDeviceToHost, stream));
     sync();
