@@ -36,12 +36,15 @@ sys.path.insert(0, ROOT)
 METRIC = "incremental updates/s + p50 repair latency/batch; learned vs LRU cache, 1 GPU"
 HBM_PEAK_GBS = 8000.0
 # timing category -> device kernel (rocprofv3 names)
-KERNEL_SYMBOL = {"push": "k_persistent<0u, 0u>", "pull": "k_persistent<1u, 0u>",
-                 "invalidate": "k_persistent<2u, 0u>",
-                 "pr_solve": "k_persistent<3u, 0u> (push) / k_prd_consume + k_prd_pull_small + k_prd_pull_big + "
-                             "k_prd_rmax (dense sweeps)",
-                 "pr_amend": "k_persistent<4u, 0u>", "delete_rows": "k_del_rows", "delete_big": "k_del_big",
-                 "append_rows": "k_add_rows + k_add_big", "static": "k_persistent<0u, 1u> / k_persistent<3u, 1u>"}
+# the closures are instantiated per program set; the built-in set is SSSP, BFS, CC
+_PS = "dcg::ProgramSet<dcg::apps::SSSP, dcg::apps::BFS, dcg::apps::CC>"
+KERNEL_SYMBOL = {"push": f"k_persistent<{_PS}, 0u, 0u>", "pull": f"k_persistent<{_PS}, 1u, 0u>",
+                 "invalidate": f"k_persistent<{_PS}, 2u, 0u>",
+                 "pr_solve": f"k_persistent<{_PS}, 3u, 0u> (push) / k_prd_consume + k_prd_pull_small + "
+                             "k_prd_pull_big + k_prd_rmax (dense sweeps)",
+                 "pr_amend": f"k_persistent<{_PS}, 4u, 0u>", "delete_rows": "k_del_rows", "delete_big": "k_del_big",
+                 "append_rows": "k_add_rows + k_add_big",
+                 "static": f"k_persistent<{_PS}, 0u, 1u> / k_persistent<{_PS}, 3u, 1u>"}
 # records that are not one kernel's launches: several kernels together, or copies
 NOT_A_KERNEL = ("delete_pass", "repair_delete", "repair_add", "ingest", "static")
 # HBM traffic per launch measured with rocprofv3 PMC counters (FETCH_SIZE and
@@ -84,10 +87,10 @@ CONFIGS = {
                app="pr", adds=50000, dels=50000, placement=0, policy=1, cache_bytes=0, snapshots=20),
     "c3": dict(workload="RMAT-26 incremental SSSP+BFS+CC batched, 4e9-byte chunk cache, hotness/LRU", scale=26,
                edges=1 << 30, app="sssp", programs=("sssp", "bfs", "cc"), adds=500000, dels=500000, placement=1,
-               policy=1, cache_bytes=4_000_000_000, snapshots=10),
+               policy=1, cache_bytes=4_000_000_000, snapshots=25, warmup=5),
     "c4": dict(workload="RMAT-26 incremental SSSP+BFS+CC batched, 4e9-byte chunk cache, learned policy",
                scale=26, edges=1 << 30, app="sssp", programs=("sssp", "bfs", "cc"), adds=500000, dels=500000,
-               placement=1, policy=2, cache_bytes=4_000_000_000, snapshots=10),
+               placement=1, policy=2, cache_bytes=4_000_000_000, snapshots=25, warmup=5),
     "c5": dict(workload="4096x4096 grid + 5% diagonals, 32 SSSP + 32 BFS sources + CC in one engine, "
                         "lognormal(8,1.5) batches", grid=4096, app="sssp", placement=0, policy=1, cache_bytes=0,
                snapshots=30, queries=64),
@@ -186,9 +189,14 @@ def run_gpu(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, policy, queries
         eng.cache_maintain()
         return phases
 
+    if cfg.get("tune") and not pr and warmup:
+        # the stage autotuner: the first warm-up snapshot's delete and add stages
+        # each run their repair once per candidate, from the same state
+        eng.autotune_stages(2)
     for s in range(warmup):
         snapshot(s)
     eng.sync()
+    tune = eng.last_tune() if cfg.get("tune") and not pr and warmup else None
     eng.reset_kernel_stats()
     t_all = time.perf_counter()
     for s in range(warmup, warmup + steps):
@@ -207,7 +215,7 @@ def run_gpu(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, policy, queries
     values = [eng.gather_value(q) for q in range(eng.queries)]
     return dict(total_s=total, lat=lat, ks=ks, info=info, hits=hits, host=host, eng=eng, values=values,
                 queries=(sources, programs), closure={k: round(v / steps, 1) for k, v in closure.items()},
-                initial=initial)
+                initial=initial, tune=tune)
 
 
 def run_plain(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, queries):
@@ -324,7 +332,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed snapshots first (default: 5 for the RMAT-26 streams, whose cache policies "
+                         "settle over the first snapshots; 1 otherwise)")
     ap.add_argument("--config", default=os.environ.get("DCG_BENCH_CONFIG", "c4"), choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", action="store_true", help="verify the final state against the oracle")
@@ -332,13 +342,15 @@ def main():
                     help="with the learned policy, skip the hotness/LRU run on the same stream")
     ap.add_argument("--buckets", type=int, default=None,
                     help="push-closure bucket width in quarters (0 off; default: the engine's automatic choice)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the stage autotuner on the first warm-up snapshot")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
                           "n_gpus": args.gpus}))
         sys.exit(2)
-    cfg = dict(CONFIGS[args.config], buckets=args.buckets)
-    warmup = args.warmup
+    cfg = dict(CONFIGS[args.config], buckets=args.buckets, tune=not args.no_tune and args.buckets is None)
+    warmup = args.warmup if args.warmup is not None else cfg.get("warmup", 1)
     steps = args.steps if args.steps is not None else cfg["snapshots"] - warmup
     snapshots = warmup + steps
     n, src, dst, adds, dels, ap, dp = make_inputs(cfg, snapshots)
@@ -382,6 +394,10 @@ def main():
                      "algo_bytes_per_launch": round(k["algo_bytes"] / max(k["launches"], 1), 1),
                      "host_link": host_link(k, host_lps, ks.get("delete_pass") if kname == "delete_rows" else None)},
         "per_snapshot": res["closure"],
+        # parameters the stage autotuner chose on the first warm-up snapshot
+        # (best_ms / first_ms: the tuned and the default parameters' repair time
+        # of that snapshot's add stage)
+        "autotune": res.get("tune"),
         # per timed snapshot: each record's time, launches and algorithmic rate
         # (stage records span all of a stage's repair launches; ingest is the
         # records' host-to-device copies)

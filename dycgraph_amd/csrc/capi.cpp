@@ -642,6 +642,21 @@ int dcg_autotune(dcg_engine *h, dcg_tune_result *out) {
     });
 }
 
+int dcg_autotune_stages(dcg_engine *h, uint32_t stages) {
+    return guarded(h, [&](Engine &e) {
+        if (!e.started) throw Error(DCG_ERR_STATE, "dcg_autotune_stages before dcg_start");
+        if (e.app == DCG_APP_PR) throw Error(DCG_ERR_UNSUPPORTED, "the stage tuner tunes the min-lattice repair");
+        e.tune_arm = stages;
+    });
+}
+
+int dcg_autotune_last(dcg_engine *h, dcg_tune_result *out) {
+    return guarded(h, [&](Engine &e) {
+        if (!out) throw Error(DCG_ERR_ARG, "null result");
+        *out = e.last_tune;
+    });
+}
+
 int dcg_set_tuning(dcg_engine *h, uint32_t blocks_per_cu, uint32_t big_row, uint32_t piece) {
     return guarded(h, [&](Engine &e) {
         auto pow2 = [](uint32_t x) { return x && !(x & (x - 1)); };

@@ -89,7 +89,10 @@ struct Engine {
         uint32_t piece = kPiece;
         uint32_t graphs = 1;            // replay each stage's repair launches as a captured hipGraph
         uint32_t bucket_quarters = ~0u;   // push-closure bucket width: SSSP w_max*q/4, BFS q; 0 off, ~0 auto
+        uint32_t solo_max = 2048;         // frontiers this small run on one workgroup (no grid barriers)
     } tune;
+    uint32_t tune_arm = 0;                // repair stages still to be tuned (tune_stage)
+    dcg_tune_result last_tune{};          // outcome of the last tuned stage
 
     uint32_t n = 0;
     uint32_t Q = 1;
@@ -265,6 +268,8 @@ struct Engine {
     void add_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st);
     uint32_t coop_blocks(const void *kernel);
     void autotune(dcg_tune_result *res);
+    void repair_stage(int stage);   // run_stage, or tune_stage while tuning is armed
+    void tune_stage(int stage);
 
     // pagerank.hip
     void start_pagerank();

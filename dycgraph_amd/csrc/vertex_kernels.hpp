@@ -62,6 +62,7 @@ struct TravArgs {
     uint32_t stamp_slot;
     uint32_t big_row;          // rows above this many entries are split into pieces
     uint32_t piece;            // piece size (power of two dividing 65,536)
+    uint32_t solo_max;         // frontiers of at most this many entries run on workgroup 0 alone
     // bucketed push closure: values at or above thr[q] wait in a far pile until
     // the near frontier drains; null far0 turns it off
     uint32_t *far0, *far1;
@@ -799,7 +800,6 @@ __device__ inline bool far_advance(const TravArgs &a, uint32_t r, const Team &t)
     return true;
 }
 
-constexpr uint32_t kSoloMax = 2048;      // frontier entries workgroup 0 handles alone
 constexpr uint32_t kSoloPieces = 64;     // more pending pieces than this end a solo stretch
 
 // One cooperative launch (1,024-thread workgroups) runs rounds until the
@@ -849,7 +849,7 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(TravArgs a) {
             break;
         }
         idle_advances = 0;
-        if (cnt <= kSoloMax && gridDim.x > 1) {
+        if (cnt <= a.solo_max && gridDim.x > 1) {
             if (blockIdx.x == 0) {
                 // solo stretch: rounds on workgroup 0 until the frontier grows,
                 // empties, or a round queues many large-row pieces
@@ -870,7 +870,7 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(TravArgs a) {
                     }
                     ++r;
                     const uint32_t c2 = *(volatile uint32_t *)&a.ctl->count[r % 3];
-                    if (c2 == 0 || c2 > kSoloMax || r >= a.max_rounds) break;
+                    if (c2 == 0 || c2 > a.solo_max || r >= a.max_rounds) break;
                 }
                 if (threadIdx.x == 0) {
                     a.ctl->solo_round = r;

@@ -85,7 +85,8 @@ MULTI_KERNEL = ("delete_pass", "repair_delete", "repair_add")
 
 class TuneResult(C.Structure):
     _fields_ = [("blocks_per_cu", C.c_uint32), ("big_row", C.c_uint32), ("piece", C.c_uint32),
-                ("evaluations", C.c_uint32), ("best_ms", C.c_double), ("bucket_quarters", C.c_uint32)]
+                ("evaluations", C.c_uint32), ("best_ms", C.c_double), ("bucket_quarters", C.c_uint32),
+                ("solo_max", C.c_uint32), ("first_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -144,6 +145,8 @@ def lib() -> C.CDLL:
         "dcg_get_kernel_stats": (I, [P, U32, C.POINTER(KernelStats)]),
         "dcg_autotune": (I, [P, C.POINTER(TuneResult)]),
         "dcg_set_tuning": (I, [P, U32, U32, U32]),
+        "dcg_autotune_stages": (I, [P, U32]),
+        "dcg_autotune_last": (I, [P, C.POINTER(TuneResult)]),
         "dcg_set_buckets": (I, [P, U32]),
         "dcg_set_graphs": (I, [P, I]),
         "dcg_reset_kernel_stats": (I, [P]),
@@ -487,6 +490,16 @@ class Engine:
     def autotune(self):
         st = TuneResult()
         self._check(lib().dcg_autotune(self._h, C.byref(st)), "dcg_autotune")
+        return st.as_dict()
+
+    def autotune_stages(self, stages=1):
+        """Tune the next `stages` repair stages on their own workload (each stage's
+        repair runs once per candidate from the same post-mutation state)."""
+        self._check(lib().dcg_autotune_stages(self._h, stages), "dcg_autotune_stages")
+
+    def last_tune(self):
+        st = TuneResult()
+        self._check(lib().dcg_autotune_last(self._h, C.byref(st)), "dcg_autotune_last")
         return st.as_dict()
 
     def set_tuning(self, blocks_per_cu=0, big_row=0, piece=0, bucket_quarters=None, graphs=None):

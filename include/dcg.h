@@ -331,8 +331,20 @@ typedef struct {
     uint32_t blocks_per_cu, big_row, piece, evaluations;
     double best_ms;
     uint32_t bucket_quarters;
+    uint32_t solo_max;        /* frontiers this small run on one workgroup */
+    double first_ms;          /* the parameters in use before tuning */
 } dcg_tune_result;
 int dcg_autotune(dcg_engine *eng, dcg_tune_result *out);
+/* Autotune on the incremental workload itself: each of the next `stages`
+ * repair stages (dcg_del_edge / dcg_add_edge; not PageRank) runs its repair
+ * once per candidate, every run from the same post-mutation state, and keeps
+ * the fastest parameters (coordinate descent over bucket width, workgroups per
+ * CU, solo-frontier threshold, large-row threshold and piece size; each run
+ * timed with HIP events).  Values are those of any single run.  The stage's
+ * timing records include the extra runs, so tune outside a measured window.
+ * dcg_autotune_last reports the last tuned stage (evaluations 0 if none). */
+int dcg_autotune_stages(dcg_engine *eng, uint32_t stages);
+int dcg_autotune_last(dcg_engine *eng, dcg_tune_result *out);
 /* Set the tuned parameters directly (0 keeps a value). */
 int dcg_set_tuning(dcg_engine *eng, uint32_t blocks_per_cu, uint32_t big_row, uint32_t piece);
 /* Bucket width of the push closure (ordered repair, ordered_gpu_repair.cuh:116-160,

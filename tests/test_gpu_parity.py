@@ -448,6 +448,64 @@ def test_grid_mixed_queries_bucket_widths(quarters):
     eng.close()
 
 
+@pytest.mark.parametrize("graph,placement", [("grid", 0), ("rmat", 1)])
+def test_stage_autotune_exact(graph, placement):
+    """The stage autotuner re-runs each armed stage's repair once per candidate
+    from the same post-mutation state: values stay bit-exact after every stage
+    (CC parents rooted witnesses), the tuner tried several candidates and kept
+    one no slower than the defaults, and later stages run with its choice."""
+    announce(f"stage autotune {graph} placement {placement}")
+    import dycgraph_amd as d
+    from tests.helpers import cc_parents_ok
+    if graph == "grid":
+        side = 64
+        src, dst = d.gen_grid(side, 0.05, seed=5)
+        n = side * side
+        apps, sources, wmax = ("sssp", "bfs", "sssp", "cc"), (0, 100, 4000, 0), 1000
+    else:
+        n, src, dst = rmat_graph(13, 1 << 17, seed=9)
+        apps, sources, wmax = ("sssp", "bfs", "cc"), (0, 1, 0), 100
+    sizes = [1500, 300, 800]
+    adds, dels = d.gen_stream(n, src, dst, sizes, sizes, seed=6)
+    eng = _engine(app="sssp", sources=sources, query_apps=apps, weight_mode=1, weight_seed=7, weight_max=wmax,
+                  placement=placement, cache_bytes=2 << 18)
+    eng.load_graph(n, src, dst)
+    eng.start()
+    eng.load_updates(adds, dels)
+    store = oracle.Store(n, src, dst)
+
+    def check(stage):
+        rp, col = store.csr()
+        for q, (app, s) in enumerate(zip(apps, sources)):
+            want = oracle_values(app, n, rp, col, s, oracle.W_SEEDED, 7, wmax)
+            got = eng.gather_value(q)
+            assert np.array_equal(got, want), (stage, q, app, int(np.sum(got != want)))
+            if app == "cc":
+                assert cc_parents_ok(n, rp, col, got, eng.gather_parent(q)), (stage, "cc parents")
+            else:
+                chk = oracle.check_sssp(n, rp, col, s, got, eng.gather_parent(q),
+                                        oracle.W_SEEDED if app == "sssp" else oracle.W_UNIT, 7, wmax)
+                assert chk["invalid_parents"] == 0 and chk["relaxable_edges"] == 0, (stage, q, chk)
+
+    assert eng.last_tune()["evaluations"] == 0
+    eng.autotune_stages(3)   # both stages of snapshot 0, the delete stage of snapshot 1
+    off = 0
+    for s, k in enumerate(sizes):
+        eng.del_edge(off, k)
+        store.apply(dels[off:off + k], [])
+        check(f"delete {s}")
+        eng.add_edge(off, k)
+        store.apply([], adds[off:off + k])
+        check(f"add {s}")
+        off += k
+        if s == 0:
+            t = eng.last_tune()
+            assert t["evaluations"] >= 8, t
+            assert t["best_ms"] <= t["first_ms"], t
+            assert t["blocks_per_cu"] in (1, 2) and t["solo_max"] in (0, 512, 2048, 8192), t
+    eng.close()
+
+
 @pytest.mark.parametrize("placement", [0, 1])
 def test_sorted_medium_rows_wave_search(placement):
     """Rows of 64..1,024 entries with sorted prefixes take the wave path's
