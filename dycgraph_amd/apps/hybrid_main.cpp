@@ -336,6 +336,30 @@ double ms_since(std::chrono::steady_clock::time_point t) {
 
 }  // namespace
 
+// Cache capacity in edges of the reference's cache-candidate ranking for its
+// integer -cache flag (graph_datum.cuh:164-176).
+uint64_t reference_cache_edges(int cache) {
+    switch (cache) {
+    case 3: return 805306365ull;
+    case 2: return 536870921ull;
+    case 1: return 268435455ull;
+    case 0: return 0;
+    default: return 1073741824ull;
+    }
+}
+
+// State count of a written repair snapshot (the `before` vector's length).
+uint64_t snapshot_states(const std::string &path) {
+    std::ifstream in(path, std::ios::binary);
+    uint64_t count = 0;
+    in.seekg(32);   // magic, version, infinity, weight rule, batch, source, nodes
+    for (int v = 0; v < 4 && in; ++v) {
+        in.read(reinterpret_cast<char *>(&count), 8);
+        if (v < 3) in.seekg((std::streamoff)(count * (v == 1 ? 8 : 4)), std::ios::cur);
+    }
+    return in ? count : 0;
+}
+
 int main(int argc, char **argv) {
     const int app = DCG_CLI_APP;
     const std::string nm = kName[app];
@@ -451,6 +475,28 @@ int main(int argc, char **argv) {
     if (app == DCG_APP_CC) LOG("[CC-SEMANTICS] directed_min_label input_edges=as_is updates=as_is\n");
     dcg_cache_stats cst;
     dcg_cache_maintain(eng, &cst);   // initial LoadCache after Start
+    // diagnostic traces (hybrid_sssp.cu:407-414, 499-506)
+    const std::string i16 = f.s("i16_repair_snapshot"), f1 = f.s("f1_cache_trace_file");
+    if (!i16.empty() && app != DCG_APP_PR && (st = dcg_set_repair_snapshot(eng, i16.c_str(), 0)))
+        return fail("i16_repair_snapshot", st);
+    const uint64_t f1_cap = reference_cache_edges((int)f.d("cache"));
+    auto trace_candidates = [&](uint32_t batch) {
+        uint64_t admitted = 0;
+        dcg_trace_stats ts;
+        int s = dcg_hot_candidates(eng, 0, f1_cap, nullptr, 0, &admitted);
+        if (!s) s = dcg_trace_cache_candidates(eng, batch, &ts);
+        if (s) return s;
+        LOG("[F1-CACHE-TRACE] batch=%u admitted_vertices=%llu admitted_edges=%llu changed_records=%llu "
+            "capacity_edges=%llu bytes=%llu\n",
+            batch, (unsigned long long)ts.admitted_vertices, (unsigned long long)ts.admitted_edges,
+            (unsigned long long)ts.changed_records, (unsigned long long)ts.capacity_edges,
+            (unsigned long long)ts.bytes);
+        return 0;
+    };
+    if (!f1.empty()) {
+        if ((st = dcg_set_cache_trace(eng, f1.c_str()))) return fail("f1_cache_trace_file", st);
+        if ((st = trace_candidates(std::numeric_limits<uint32_t>::max()))) return fail("f1_cache_trace", st);
+    }
 
     std::vector<std::pair<uint64_t, uint64_t>> batches;
     std::vector<dcg_edge> adds, dels;
@@ -518,6 +564,7 @@ int main(int argc, char **argv) {
             dcg_phase_stats ps;
             if ((st = dcg_update_pagerank(eng, ab, na, db, nd, &ps))) return fail("update_pagerank", st);
             dcg_cache_maintain(eng, &cst);
+            if (!f1.empty() && (st = trace_candidates((uint32_t)b))) return fail("f1_cache_trace", st);
             LOG("[PR-BATCH] batch=%u touched_sources=%llu rounds=%llu stop=%s active=%llu limit=%u\n", (unsigned)b,
                 (unsigned long long)ps.touched_sources, (unsigned long long)ps.rounds,
                 ps.affected ? "iteration_limit" : "converged", (unsigned long long)ps.affected, cfg.pr_max_rounds);
@@ -529,6 +576,9 @@ int main(int argc, char **argv) {
             dcg_phase_stats ds, as;
             if ((st = dcg_del_edge(eng, db, nd, &ds))) return fail("del_edge", st);
             double total_ms = ms_since(t);
+            if (b == 0 && !i16.empty() && nd)
+                LOG("[I16-SNAPSHOT][batch 0] states=%llu diagnostic_only=1\n",
+                    (unsigned long long)snapshot_states(i16));
             if (check && !min_check("DELETE-STAGE", b, true)) success = false;
             t = std::chrono::steady_clock::now();
             if ((st = dcg_add_edge(eng, ab, na, &as))) return fail("add_edge", st);
@@ -536,6 +586,7 @@ int main(int argc, char **argv) {
                 dcg_cache_prefetch(eng, ab + na, batches[b + 1].first, db + nd, batches[b + 1].second, &cst);
             dcg_cache_maintain(eng, &cst);
             total_ms += ms_since(t);
+            if (!f1.empty() && (st = trace_candidates((uint32_t)b))) return fail("f1_cache_trace", st);
             LOG("[P0-ATTR][%s][batch %d] deletion=%.3f add=%.3f cache=%.3f affected=%llu rounds=%llu total=%.3f\n",
                 kTag[app], (int)b, ds.ms, as.ms, cst.ms, (unsigned long long)ds.affected,
                 (unsigned long long)(ds.rounds + as.rounds), total_ms);

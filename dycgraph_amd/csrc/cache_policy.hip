@@ -401,13 +401,27 @@ uint64_t Engine::hot_candidates(uint32_t query, uint64_t cap, uint32_t *ids, uin
             }
         }
     }
-    const uint64_t k = std::min<uint64_t>(count, ids_len);
+    // the caller's ids; with a cache-candidate trace open, the whole admitted
+    // prefix and its degrees for the next trace record
+    const uint64_t k = std::min<uint64_t>(count, f1_on ? n : ids_len);
     if (k) {
         std::vector<uint64_t> hk(k);
         DCG_HIP(hipMemcpyAsync(hk.data(), sorted.p, k * 8, hipMemcpyDeviceToHost, stream));
+        if (f1_on) {
+            rank_deg.resize(count);
+            DCG_HIP(hipMemcpyAsync(rank_deg.data(), deg.p, count * 4, hipMemcpyDeviceToHost, stream));
+        }
         sync();
-        for (uint64_t i = 0; i < k; ++i) ids[i] = (uint32_t)(hk[i] & idmask);
+        for (uint64_t i = 0; i < std::min<uint64_t>(k, ids_len); ++i) ids[i] = (uint32_t)(hk[i] & idmask);
+        if (f1_on) {
+            rank_ids.resize(count);
+            for (uint64_t i = 0; i < count; ++i) rank_ids[i] = (uint32_t)(hk[i] & idmask);
+        }
+    } else if (f1_on) {
+        rank_ids.clear();
+        rank_deg.clear();
     }
+    rank_cap = cap;
     hipLaunchKernelGGL(k_window_shift, dim3(grid_for(n)), dim3(256), 0, stream, d_hot_window.p, n);
     DCG_HIP(hipGetLastError());
     sync();

@@ -222,7 +222,7 @@ Engine::~Engine() {
     if (stream) (void)hipStreamSynchronize(stream);
     if (stream2) (void)hipStreamSynchronize(stream2);
     if (copy_stream) (void)hipStreamSynchronize(copy_stream);
-    cache.mlp.reset();   // its training stream and events
+    cache.mlp.reset();   // its events and pinned judge sums
     for (hipEvent_t ev : events) (void)hipEventDestroy(ev);
     for (auto &kind : slots)
         for (BatchSlot &b : kind) {
@@ -302,6 +302,7 @@ const char *dcg_status_string(int status) {
     case DCG_ERR_CAPACITY: return "capacity exhausted";
     case DCG_ERR_NUMERIC: return "non-finite PageRank state";
     case DCG_ERR_UNSUPPORTED: return "unsupported";
+    case DCG_ERR_IO: return "file I/O failed";
     default: return "unknown status";
     }
 }
@@ -529,6 +530,28 @@ int dcg_hot_candidates(dcg_engine *h, uint32_t query, uint64_t cap_slots, uint32
         if (query >= e.Q || !count || (ids_len && !ids)) throw Error(DCG_ERR_ARG, "bad hot-candidate arguments");
         *count = e.hot_candidates(query, cap_slots, ids, ids_len);
     });
+}
+
+int dcg_set_repair_snapshot(dcg_engine *h, const char *path, uint32_t query) {
+    return guarded(h, [&](Engine &e) {
+        if (!path || !*path) {
+            e.i16_armed = false;
+            return;
+        }
+        if (e.app == DCG_APP_PR) throw Error(DCG_ERR_UNSUPPORTED, "repair snapshots are for min-lattice programs");
+        if (query >= e.Q) throw Error(DCG_ERR_ARG, "query out of range");
+        e.i16_path = path;
+        e.i16_query = query;
+        e.i16_armed = true;
+    });
+}
+
+int dcg_set_cache_trace(dcg_engine *h, const char *path) {
+    return guarded(h, [&](Engine &e) { e.f1_open(path); });
+}
+
+int dcg_trace_cache_candidates(dcg_engine *h, uint32_t batch, dcg_trace_stats *stats) {
+    return guarded(h, [&](Engine &e) { e.f1_record(batch, stats); });
 }
 
 int dcg_hot_window(dcg_engine *h, uint32_t *out, size_t n) {

@@ -3,6 +3,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -156,7 +157,24 @@ struct Engine {
     hipGraphExec_t graph_exec[2] = {nullptr, nullptr};   // captured repair launches of each stage
     bool capturing = false;
     void run_stage(int stage);
-    void enqueue_repair(int stage);
+    // part 0: the whole stage; for the delete stage, part 1 ends after the
+    // reset of the affected set and part 2 runs the rest (pull, push, CC check)
+    void enqueue_repair(int stage, int part = 0);
+
+    // Diagnostic traces (traces.hip).  Repair snapshot: the first delete stage
+    // after arming writes the affected set, its in-rows and the states of the
+    // involved vertices before and after the repair.
+    std::string i16_path;
+    uint32_t i16_query = 0;
+    bool i16_armed = false;
+    void i16_capture();
+    // Cache-candidate trace: the admitted prefix of the last hotness ranking.
+    std::ofstream f1;
+    bool f1_on = false, f1_init = false;
+    std::vector<uint32_t> rank_ids, rank_deg, f1_prev_ids, f1_prev_deg;
+    uint64_t rank_cap = 0, f1_prev_edges = 0;
+    void f1_open(const char *path);
+    void f1_record(uint32_t batch, dcg_trace_stats *out);
     void drop_graphs();
     // PageRank
     DevBuf<float> d_rank, d_res, d_delta;
@@ -191,7 +209,7 @@ struct Engine {
     void fill_slot(int kind, BatchSlot &s, uint64_t begin, uint64_t count, hipStream_t on);
     void forget_batches();
     void mark_batches_used();
-    void sync_all();   // engine, second, copy and training streams
+    void sync_all();   // engine, second and copy streams
 
     // ---- phase scratch ----
     DevBuf<uint8_t> d_temp;            // scratch of the sort, scan and selection primitives
@@ -305,6 +323,8 @@ struct Engine {
 // The built-in program table (traverse.hip) and the TravArgs/Ctl layout tag a
 // plugin's table must carry.
 const dcg_program *builtin_program();
+// Throws for a closure whose control block reports a barrier timeout or an overflow.
+void check_launch(const Ctl &c, const char *stage);
 uint32_t program_layout_tag();
 
 // Random 64-byte line reads of pinned host memory that one chunk load (256 KiB

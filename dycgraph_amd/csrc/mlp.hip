@@ -41,21 +41,20 @@ constexpr int P_W1 = 0, P_B1 = P_W1 + kW1, P_W2 = P_B1 + kHid, P_B2 = P_W2 + kW2
 constexpr int kReplay = 4;   // decisions kept for training (ring)
 
 struct MlpState {
-    // Training runs on its own stream, overlapping the snapshot's repair work
-    // (it starts once the delete mutation is enqueued, `released`, so the
-    // latency-bound mutation kernels run alone); `trained` marks the end of
-    // the last training launch.  The engine stream waits on it before a pass
-    // reads the weights or a decision is written into the ring.
+    // Training and the judge run in order on the engine stream, after the
+    // delete mutation (`released`): a kernel on another stream beside the
+    // persistent repair closures could hold CU slots that a closure workgroup
+    // needs, and the closures' grid barrier requires every workgroup resident.
+    // `trained` marks the end of the last training launch; the engine stream
+    // waits on it before a pass reads the weights or a decision is written
+    // into the ring (a no-op on the same stream, kept so the order is explicit).
     hipStream_t train = nullptr;
     hipEvent_t trained = nullptr, released = nullptr, labelled = nullptr;
     bool train_due = false;   // a decision was made; its training launch waits for the delete mutation
     bool judge_pending = false;   // the last judge's sums are on their way to h_judge
     double *h_judge = nullptr;    // pinned: network top-k, hotness top-k, all reads
     ~MlpState() {
-        if (train) {
-            (void)hipStreamSynchronize(train);
-            (void)hipStreamDestroy(train);
-        }
+        if (train) (void)hipStreamSynchronize(train);
         if (trained) (void)hipEventDestroy(trained);
         if (released) (void)hipEventDestroy(released);
         if (labelled) (void)hipEventDestroy(labelled);
@@ -77,16 +76,19 @@ struct MlpState {
     DevBuf<float> sort_v;                 // judge: reads in ranking order
     DevBuf<double> judge;                 // judge sums: network top-k, hotness top-k, all reads
     DevBuf<float> judge_reads;            // the snapshot's reads, kept for the judge
-    DevBuf<uint8_t> judge_tmp;            // radix-sort scratch of the training stream
+    DevBuf<uint8_t> judge_tmp;            // radix-sort scratch of the judge
     int decisions = 0, fallbacks = 0;
     int rejected = 0;   // judged decisions in a row, after warm-up, that kept the hotness ranking
+    int next_probe = 0;   // while quiet: the next decision that runs the network
 };
 
 // While the judge keeps rejecting the trained network (kQuiet judged decisions
-// in a row), only every kProbe-th decision runs it (features, forward pass,
-// judge and a training step); the others take the hotness plan at no extra
-// cost.  One network decision accepted by the judge restores every decision.
-constexpr int kQuiet = 2, kProbe = 4;
+// in a row), the network runs (features, forward pass, judge and a training
+// step) only at probe decisions, kProbe decisions apart after the first quiet
+// rejection and twice as far after each further one, up to kProbeMax; the
+// other decisions take the hotness plan at no extra cost.  One network decision
+// accepted by the judge restores every decision.
+constexpr int kQuiet = 2, kProbe = 4, kProbeMax = 64;
 
 void MlpDeleter::operator()(MlpState *m) const { delete m; }
 
@@ -432,7 +434,7 @@ void mlp_create(Engine &e) {
     m->m1.alloc(P_NUM);
     m->m2.alloc(P_NUM);
     m->w16.alloc(P_NUM);
-    DCG_HIP(hipStreamCreateWithFlags(&m->train, hipStreamNonBlocking));
+    m->train = e.stream;
     DCG_HIP(hipEventCreateWithFlags(&m->trained, hipEventDisableTiming));
     DCG_HIP(hipEventCreateWithFlags(&m->released, hipEventDisableTiming));
     DCG_HIP(hipEventCreateWithFlags(&m->labelled, hipEventDisableTiming));
@@ -465,8 +467,8 @@ void mlp_create(Engine &e) {
 
 // Judge: reads of the snapshot held by each ranking's top `frames` chunks.
 // Rankings sort descending and stably (ties keep chunk order, lowest id
-// first); the reads travel with the scores.  Runs on the training stream on a
-// copy of the reads; judge_resolve reads the sums at the next decision.
+// first); the reads travel with the scores.  Runs after the training launches
+// on a copy of the reads; judge_resolve reads the sums at the next decision.
 static void judge_launch(Engine &e, MlpState &m) {
     const uint64_t C = e.cache.chunks, k = e.cache.frames;
     DCG_HIP(hipMemcpyAsync(m.judge_reads.p, e.cache.d_reads.p, C * 4, hipMemcpyDeviceToDevice, e.stream));
@@ -543,7 +545,7 @@ void mlp_stats(const Engine &e, dcg_cache_stats *st) {
 std::vector<uint64_t> mlp_plan(Engine &e, uint64_t ab, uint64_t ac, uint64_t db, uint64_t dc) {
     MlpState &m = *e.cache.mlp;
     const uint64_t C = e.cache.chunks;
-    if (m.rejected >= kQuiet && (m.decisions % kProbe) != 0 && e.cache.hot_score.size() == C && !m.train_due) {
+    if (m.rejected >= kQuiet && m.decisions < m.next_probe && e.cache.hot_score.size() == C && !m.train_due) {
         m.decisions++;
         m.fallbacks++;
         return e.plan_admission(e.cache.hot_score, 4.0 * kLoadLines, 1.5);
@@ -576,6 +578,8 @@ std::vector<uint64_t> mlp_plan(Engine &e, uint64_t ab, uint64_t ac, uint64_t db,
     DCG_HIP(hipMemcpyAsync(net.data(), m.plan.p, C * 4, hipMemcpyDeviceToHost, e.stream));
     e.sync();
     judge_resolve(m);   // the previous decision's judge (the stream waited for it above)
+    if (m.rejected >= kQuiet)
+        m.next_probe = m.decisions + 1 + std::min(kProbeMax, kProbe << std::min(m.rejected - kQuiet, 4));
     // online training on the labelled decisions so far overlaps the batch's
     // repair work (mlp_release_training)
     m.train_due = true;

@@ -15,7 +15,7 @@ void mlp_observe(Engine &e);
 void mlp_release_training(Engine &e);
 // Chunks to keep resident for the staged batch about to run.
 std::vector<uint64_t> mlp_plan(Engine &e, uint64_t ab, uint64_t ac, uint64_t db, uint64_t dc);
-// Wait for the training stream.
+// Wait for the training launches (they run on the engine stream).
 void mlp_sync(Engine &e);
 // Decision counters and the judge's last comparison (dcg_cache_stats fields).
 void mlp_stats(const Engine &e, dcg_cache_stats *st);

@@ -67,7 +67,7 @@ __global__ void k_far_setup(uint32_t *thr, uint32_t *minfar, const uint32_t *fde
 }
 
 // ------------------------------------------------------------------- host side
-static void check_launch(const Ctl &c, const char *stage) {
+void check_launch(const Ctl &c, const char *stage) {
     if (c.bar_timeout) throw Error(DCG_ERR_HIP, std::string("a grid barrier timed out in ") + stage);
     if (c.overflow) throw Error(DCG_ERR_CAPACITY, std::string("frontier overflow in ") + stage);
 }
@@ -408,7 +408,8 @@ void Engine::delete_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
     h_step[0] = StepArgs{recs, count, {take_stamps(*this), take_stamps(*this), 0u, 0u}};
     DCG_HIP(hipMemcpyAsync(d_step.p, h_step, sizeof(StepArgs), hipMemcpyHostToDevice, stream));
     const int rs = time_begin(DCG_K_REPAIR_DELETE, -1);
-    repair_stage(0);
+    if (i16_armed) i16_capture();   // the repair in two parts around the snapshot's captures
+    else repair_stage(0);
     time_end(rs);
     mark_batches_used();
     fold_enqueue();
@@ -467,29 +468,34 @@ void Engine::add_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
 // deletion has run; the invalidation scans post-delete rows, which only drops
 // deleted edges that the seed step already examined): seeds -> invalidation
 // closure -> reset -> one pull pass -> push closure.  Add: seeds -> push closure.
-void Engine::enqueue_repair(int stage) {
+void Engine::enqueue_repair(int stage, int part) {
     Ctl *c = d_ctl.p;
     TravArgs a = base_args(*this);
     a.step = d_step.p + stage;
     constexpr uint32_t kSeedBlocks = 512;   // 1,024-thread workgroups, grid-stride over count * Q records
     if (stage == 0) {
-        hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 0, (const uint32_t *)nullptr, 0u, 0u,
-                           (const uint32_t *)nullptr);
-        a.ctl = c + 0;
-        {
-            void *args[] = {&a};
-            launch_pk(*this, DCG_PK_SEED_DELS, kSeedBlocks, 1024, args);
+        if (part != 2) {
+            hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 0, (const uint32_t *)nullptr, 0u, 0u,
+                               (const uint32_t *)nullptr);
+            a.ctl = c + 0;
+            {
+                void *args[] = {&a};
+                launch_pk(*this, DCG_PK_SEED_DELS, kSeedBlocks, 1024, args);
+            }
+            hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 1, &c[0].aff_count, 0u, 0u,
+                               &c[0].aff_count);
+            a.ctl = c + 1;
+            a.in0 = d_aff.p;
+            a.stamp_slot = 0;
+            launch_persistent(*this, M_INVALIDATE, a);
+            {
+                void *args[] = {&a};   // a.ctl = c + 1: the affected list's length
+                launch_pk(*this, DCG_PK_RESET, grid_for((uint64_t)n * Q, 256, 8192), 256, args);
+            }
+            if (part == 1) return;
         }
-        hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 1, &c[0].aff_count, 0u, 0u,
-                           &c[0].aff_count);
-        a.ctl = c + 1;
         a.in0 = d_aff.p;
         a.stamp_slot = 0;
-        launch_persistent(*this, M_INVALIDATE, a);
-        {
-            void *args[] = {&a};   // a.ctl = c + 1: the affected list's length
-            launch_pk(*this, DCG_PK_RESET, grid_for((uint64_t)n * Q, 256, 8192), 256, args);
-        }
         hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 2, &c[1].aff_count, 0u, 0u,
                            (const uint32_t *)nullptr);
         a.ctl = c + 2;

@@ -92,6 +92,14 @@ class TuneResult(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class TraceStats(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("admitted_vertices", "admitted_edges", "changed_records",
+                                          "capacity_edges", "bytes")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class EngineInfo(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in (
         "snapshots", "store_slots", "store_used_slots", "chunks", "frames", "host_bytes",
@@ -134,6 +142,9 @@ def lib() -> C.CDLL:
         "dcg_cache_policy_set": (I, [P, U32]),
         "dcg_hot_candidates": (I, [P, U32, U64, P, U64, C.POINTER(U64)]),
         "dcg_hot_window": (I, [P, P, C.c_size_t]),
+        "dcg_set_repair_snapshot": (I, [P, C.c_char_p, U32]),
+        "dcg_set_cache_trace": (I, [P, C.c_char_p]),
+        "dcg_trace_cache_candidates": (I, [P, U32, C.POINTER(TraceStats)]),
         "dcg_radix_sort": (I, [C.c_int32, P, P, U64, I, I]),
         "dcg_gather_values": (I, [P, U32, P, C.c_size_t]),
         "dcg_gather_parents": (I, [P, U32, P, C.c_size_t]),
@@ -445,6 +456,20 @@ class Engine:
         self._check(lib().dcg_hot_candidates(self._h, query, cap_slots, _ptr(ids), limit, C.byref(cnt)),
                     "dcg_hot_candidates")
         return cnt.value, ids[:min(cnt.value, limit)]
+
+    def set_repair_snapshot(self, path, query=0):
+        """Write the next delete stage's repair snapshot to `path` ('' disarms)."""
+        self._check(lib().dcg_set_repair_snapshot(self._h, os.fsencode(path), query), "dcg_set_repair_snapshot")
+
+    def set_cache_trace(self, path):
+        """Open (truncate) a cache-candidate trace at `path` ('' closes it)."""
+        self._check(lib().dcg_set_cache_trace(self._h, os.fsencode(path)), "dcg_set_cache_trace")
+
+    def trace_cache_candidates(self, batch):
+        """Append the last hot_candidates ranking's admitted prefix as one record."""
+        st = TraceStats()
+        self._check(lib().dcg_trace_cache_candidates(self._h, batch, C.byref(st)), "dcg_trace_cache_candidates")
+        return st.as_dict()
 
     def hot_window(self):
         out = np.empty(self.n, np.uint32)

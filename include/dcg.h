@@ -37,7 +37,8 @@ typedef enum {
     DCG_ERR_STATE = -3,     /* call out of order (e.g. add before load)               */
     DCG_ERR_CAPACITY = -4,  /* chunk store or frontier capacity exhausted             */
     DCG_ERR_NUMERIC = -5,   /* PageRank produced a non-finite rank/residual           */
-    DCG_ERR_UNSUPPORTED = -6
+    DCG_ERR_UNSUPPORTED = -6,
+    DCG_ERR_IO = -7         /* a trace file could not be written                       */
 } dcg_status;
 
 typedef enum { DCG_APP_SSSP = 0, DCG_APP_BFS = 1, DCG_APP_CC = 2, DCG_APP_PR = 3 } dcg_app;
@@ -239,6 +240,36 @@ int dcg_hot_candidates(dcg_engine *eng, uint32_t query, uint64_t cap_slots, uint
                        uint64_t *count);
 /* The window words (n entries; byte k = expansions k snapshots ago, saturated at 255). */
 int dcg_hot_window(dcg_engine *eng, uint32_t *out, size_t n);
+
+/* Diagnostic traces, in the reference's file formats.
+ *
+ * Repair snapshot (I16, include/framework/i16_repair_snapshot.h:9-52, written
+ * by framework.cuh:4189-4212 and 4668-4677): the first delete stage after
+ * this call writes `path` (through path.tmp, renamed when complete): magic
+ * 0x4931365354415445, version 1, infinity 0xFFFFFFFF, weight rule 1, batch 0,
+ * the query's source, n; then the affected vertices of `query` after the
+ * invalidation (u64 count + u32 ids, ascending), their live in-rows as u64
+ * offsets (count + 1) and u32 sources, and the states {vertex, value,
+ * buffer, parent} (u64 count + 16 B each) of the sorted unique union of the
+ * sources, the affected vertices and the query's source, once before the
+ * repair (affected values reset) and once after it.  buffer = value (this
+ * engine keeps one state word per vertex).  Min-lattice programs only; the
+ * stage runs its repair without the captured graph.  An empty path disarms. */
+int dcg_set_repair_snapshot(dcg_engine *eng, const char *path, uint32_t query);
+
+/* Cache-candidate trace (F1, TraceCacheCandidates, framework.cuh:1414-1507;
+ * reader cache_patch_trace.h:15-60): dcg_set_cache_trace truncates `path` and
+ * writes the magic 0x4631434143484531 (an empty path closes the trace);
+ * dcg_trace_cache_candidates appends one record for the last
+ * dcg_hot_candidates ranking: u32 batch, u64 capacity (its cap_slots), u64
+ * count, then count x {u32 vertex, u32 live out-degree} for the admitted
+ * prefix in rank order; count is 0 when the admitted list equals the previous
+ * record's. */
+typedef struct {
+    uint64_t admitted_vertices, admitted_edges, changed_records, capacity_edges, bytes;
+} dcg_trace_stats;
+int dcg_set_cache_trace(dcg_engine *eng, const char *path);
+int dcg_trace_cache_candidates(dcg_engine *eng, uint32_t batch, dcg_trace_stats *stats);
 /* Stable GPU radix sort of caller keys by bits [begin_bit, end_bit), values
  * (optional) carried along; host arrays sorted in place.  The sort the engine
  * uses for its rankings, exposed for checks.  No reference counterpart. */
@@ -370,9 +401,9 @@ int dcg_calibrate(int32_t device, uint64_t bytes, double *hbm_copy_gbs, double *
  * host-placed store.  No reference counterpart (measurement utility). */
 int dcg_calibrate_lines(int32_t device, uint64_t bytes, double *host_lines_per_s, double *hbm_lines_per_s);
 
-/* Wait for every stream of the engine: the engine stream, the second mutation
- * stream, the batch copy stream and the learned policy's training stream
- * (timing boundaries). */
+/* Wait for every stream of the engine: the engine stream (which also runs the
+ * learned policy's training), the second mutation stream and the batch copy
+ * stream (timing boundaries). */
 int dcg_sync(dcg_engine *eng);
 
 /* Seeded input generators (no reference counterpart: the reference reads

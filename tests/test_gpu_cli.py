@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 import oracle
+from dycgraph_amd import traces
 from tests.conftest import announce
 
 pytestmark = pytest.mark.gpu
@@ -150,3 +151,36 @@ def test_pr_smoke_scenario(tmp_path):
     out = _run("pr", tmp_path, ["--error=1e-6", f"--pr_max_batches={len(batches)}", "--placement=device"])
     checks = re.findall(r"\[PR-CHECK\] stage=(\w+) batch=(-?\d+) (passed|failed)", out)
     assert len(checks) == len(batches) + 2 and all(c[2] == "passed" for c in checks), out[-2000:]
+
+
+def test_cli_trace_flags(tmp_path):
+    """hybrid_sssp with --i16_repair_snapshot and --f1_cache_trace_file writes
+    both files and logs their lines; the snapshot replays without mismatches.
+    (The binaries run as child processes: their cooperative launches need the
+    device's global wave sync, which a parent that already launched
+    cooperatively keeps, so the command-line tests run before any test that
+    creates an engine in the test process.)"""
+    announce("cli trace flags")
+    n = 1024
+    rng = np.random.default_rng(5)
+    edges = sorted({(int(u), int(v)) for u, v in rng.integers(0, n, (6000, 2)) if u != v})
+    (tmp_path / "graph").write_text("".join(f"{u} {v}\n" for u, v in edges))
+    dl = [edges[i] for i in rng.choice(len(edges), 200, replace=False)]
+    ad = [(int(u), int(v)) for u, v in rng.integers(0, n, (200, 2))]
+    (tmp_path / "updates").write_text("".join(f"d {u} {v} 1\n" for u, v in dl) +
+                                      "".join(f"a {u} {v} 1\n" for u, v in ad))
+    (tmp_path / "sizes").write_text("200 200\n")
+    snap, f1 = tmp_path / "i16.bin", tmp_path / "f1.bin"
+    cmd = [os.path.join(BIN, "hybrid_sssp"), f"--graphfile={tmp_path}/graph",
+           f"--updatefile={tmp_path}/updates", f"--update_size={tmp_path}/sizes", "--format=market_big",
+           "--source_node=0", "--check=true", f"--i16_repair_snapshot={snap}", f"--f1_cache_trace_file={f1}",
+           "--cache=1"]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "[I16-SNAPSHOT][batch 0] states=" in r.stdout and r.stdout.count("[F1-CACHE-TRACE]") == 2
+    sn = traces.read_repair_snapshot(snap)
+    res = traces.replay_repair_snapshot(sn)
+    assert res["mismatches"] == 0 and res["external_changes"] == 0
+    recs = traces.read_cache_trace(f1)
+    assert [b for b, _, _ in recs] == [0xFFFFFFFF, 0] and all(c == 268435455 for _, c, _ in recs)
+    assert len(recs[0][2]) == n   # everything fits a 1-unit cache on this graph
