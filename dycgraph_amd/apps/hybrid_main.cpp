@@ -336,6 +336,38 @@ double ms_since(std::chrono::steady_clock::time_point t) {
 
 }  // namespace
 
+// Algorithm hints from the environment, validated like the reference
+// (CG_ORDERED_REPAIR / CG_INSERTION_SCHEDULE: framework.cuh:3775-3783;
+// CG_BATCH_MAINTENANCE: effective_update_batch.h:40-45).  Results never depend
+// on them; they pick the executor: ordered = the bucketed push closure,
+// block / thread = plain frontier order (one executor here: the edge-balanced
+// wave expansion), batch maintenance large = the device-wide grouping sort for
+// every batch (auto: for batches of at least 1,000,000 updates).
+struct Hints {
+    int ordered = -1;   // -1: the engine's automatic choice, 0 off, 1 on
+    std::string batch = "regular";
+};
+
+bool read_hints(Hints &h, const char *tag) {
+    const char *o = std::getenv("CG_ORDERED_REPAIR");
+    const char *s = std::getenv("CG_INSERTION_SCHEDULE");
+    const char *b = std::getenv("CG_BATCH_MAINTENANCE");
+    const bool large_diameter = o && std::string(o) == "1";
+    const std::string schedule = large_diameter ? "ordered" : (s ? s : "block");
+    if (schedule != "block" && schedule != "thread" && schedule != "ordered") {
+        std::fprintf(stderr, "[%s-ERROR] CG_INSERTION_SCHEDULE must be block, thread or ordered\n", tag);
+        return false;
+    }
+    h.batch = b ? b : "regular";
+    if (h.batch != "regular" && h.batch != "large" && h.batch != "auto") {
+        std::fprintf(stderr, "[%s-ERROR] CG_BATCH_MAINTENANCE must be regular, large or auto\n", tag);
+        return false;
+    }
+    if (schedule == "ordered") h.ordered = 1;
+    else if (o || s) h.ordered = 0;   // an explicit unordered choice
+    return true;
+}
+
 // Cache capacity in edges of the reference's cache-candidate ranking for its
 // integer -cache flag (graph_datum.cuh:164-176).
 uint64_t reference_cache_edges(int cache) {
@@ -414,6 +446,8 @@ int main(int argc, char **argv) {
     f.def("weight_max", "100");
     f.def("weight_seed", "1");
     if (parse_flags(f, argc, argv)) return 1;
+    Hints hints;
+    if (!read_hints(hints, kTag[app])) return 1;
     if (f.s("graphfile").empty()) {
         std::fprintf(stderr, "[%s-ERROR] graphfile is required\n", kTag[app]);
         return 1;
@@ -467,6 +501,9 @@ int main(int argc, char **argv) {
         dcg_destroy(eng);
         return 1;
     };
+    if (app != DCG_APP_PR && hints.ordered >= 0 && (st = dcg_set_buckets(eng, hints.ordered ? 4u : 0u)))
+        return fail("CG_ORDERED_REPAIR", st);   // 4 quarters: SSSP window w_max (128, ordered_gpu_repair.cuh:116-160)
+    if (hints.batch == "large" && (st = dcg_set_batch_maintenance(eng, 1))) return fail("CG_BATCH_MAINTENANCE", st);
     if ((st = dcg_load_edges(eng, n, src.data(), dst.data(), src.size()))) return fail("load", st);
     auto t_init = std::chrono::steady_clock::now();
     if ((st = dcg_start(eng))) return fail("start", st);
@@ -559,6 +596,7 @@ int main(int argc, char **argv) {
     uint64_t ab = 0, db = 0;
     for (uint64_t b = 0; b < nb; ++b) {
         const uint64_t na = batches[b].first, nd = batches[b].second;
+        if (hints.batch == "auto") dcg_set_batch_maintenance(eng, na + nd >= 1000000 ? 1 : 0);
         if (app == DCG_APP_PR) {
             auto t = std::chrono::steady_clock::now();
             dcg_phase_stats ps;

@@ -693,6 +693,10 @@ int dcg_set_tuning(dcg_engine *h, uint32_t blocks_per_cu, uint32_t big_row, uint
     });
 }
 
+int dcg_set_batch_maintenance(dcg_engine *h, int large) {
+    return guarded(h, [&](Engine &e) { e.large_batches = large != 0; });
+}
+
 int dcg_set_graphs(dcg_engine *h, int on) {
     return guarded(h, [&](Engine &e) {
         e.tune.graphs = on ? 1u : 0u;

@@ -92,6 +92,7 @@ struct Engine {
         uint32_t bucket_quarters = ~0u;   // push-closure bucket width: SSSP w_max*q/4, BFS q; 0 off, ~0 auto
         uint32_t solo_max = 2048;         // frontiers this small run on one workgroup (no grid barriers)
     } tune;
+    bool large_batches = false;           // every batch through the device-wide sort path (no one-workgroup sort)
     uint32_t tune_arm = 0;                // repair stages still to be tuned (tune_stage)
     dcg_tune_result last_tune{};          // outcome of the last tuned stage
 

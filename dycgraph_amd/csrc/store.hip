@@ -1715,7 +1715,7 @@ void Engine::mutate_delete(const dcg_edge *recs, uint64_t count, dcg_phase_stats
     d_used.ensure(2 * count);   // occurrence counters, out then in direction
     uint64_t *kout = d_k64a.p, *kin = d_k64a.p + count;
     uint32_t *used_out = d_used.p, *used_in = d_used.p + count;
-    if (count <= kSmallSort) {
+    if (count <= kSmallSort && !large_batches) {
         hipLaunchKernelGGL(k_del_prep_small, dim3(2), dim3(1024), 0, stream, recs, (uint32_t)count, n, kout, kin,
                            d_gstart.p, d_gstart2.p, d_used.p, d_counters.p);
     } else {
@@ -1807,7 +1807,7 @@ void Engine::mutate_add(const dcg_edge *recs, uint64_t count, dcg_phase_stats *s
     d_gstart.ensure(count);
     d_gstart2.ensure(count);
     uint32_t *kout = d_k32a.p, *kin = d_k32a.p + count, *vout = d_v32a.p, *vin = d_v32a.p + count;
-    if (count <= kSmallSort) {
+    if (count <= kSmallSort && !large_batches) {
         hipLaunchKernelGGL(k_add_prep_small, dim3(2), dim3(1024), 0, stream, recs, (uint32_t)count, n, kout, vout,
                            kin, vin, d_gstart.p, d_gstart2.p, d_out.p, d_in.p, d_tail.p, d_counters.p);
     } else {
@@ -1841,7 +1841,7 @@ void Engine::mutate_add(const dcg_edge *recs, uint64_t count, dcg_phase_stats *s
         std::swap(grown.p, d_pool.p);
         std::swap(grown.n, d_pool.n);
     }
-    if (pool_len <= kSmallSort) {
+    if (pool_len <= kSmallSort && !large_batches) {
         hipLaunchKernelGGL(k_pool_small, dim3(1), dim3(1024), 0, stream, d_pool.p, (uint32_t)pool_len,
                            (uint32_t)pool_live, d_tail.p, total_slots);
     } else {

@@ -157,6 +157,7 @@ def lib() -> C.CDLL:
         "dcg_autotune": (I, [P, C.POINTER(TuneResult)]),
         "dcg_set_tuning": (I, [P, U32, U32, U32]),
         "dcg_autotune_stages": (I, [P, U32]),
+        "dcg_set_batch_maintenance": (I, [P, I]),
         "dcg_autotune_last": (I, [P, C.POINTER(TuneResult)]),
         "dcg_set_buckets": (I, [P, U32]),
         "dcg_set_graphs": (I, [P, I]),
@@ -516,6 +517,10 @@ class Engine:
         st = TuneResult()
         self._check(lib().dcg_autotune(self._h, C.byref(st)), "dcg_autotune")
         return st.as_dict()
+
+    def set_batch_maintenance(self, large):
+        """True: every batch through the device-wide sort path (CG_BATCH_MAINTENANCE=large)."""
+        self._check(lib().dcg_set_batch_maintenance(self._h, 1 if large else 0), "dcg_set_batch_maintenance")
 
     def autotune_stages(self, stages=1):
         """Tune the next `stages` repair stages on their own workload (each stage's

@@ -376,6 +376,11 @@ int dcg_autotune(dcg_engine *eng, dcg_tune_result *out);
  * dcg_autotune_last reports the last tuned stage (evaluations 0 if none). */
 int dcg_autotune_stages(dcg_engine *eng, uint32_t stages);
 int dcg_autotune_last(dcg_engine *eng, dcg_tune_result *out);
+/* Batch maintenance (the reference's CG_BATCH_MAINTENANCE, effective_update_batch.h:
+ * 34-45): 0 (default) groups batches of up to 4,096 records in one workgroup's
+ * LDS sort and larger ones with the device-wide radix sort; 1 ("large") sends
+ * every batch through the device-wide sort.  Values and rows never depend on it. */
+int dcg_set_batch_maintenance(dcg_engine *eng, int large);
 /* Set the tuned parameters directly (0 keeps a value). */
 int dcg_set_tuning(dcg_engine *eng, uint32_t blocks_per_cu, uint32_t big_row, uint32_t piece);
 /* Bucket width of the push closure (ordered repair, ordered_gpu_repair.cuh:116-160,

@@ -37,3 +37,21 @@ def test_reference_command_line_parses(tmp_path):
              "-weight", "-cache", "4", "-source_node", "0", "--check=true", "--sssp_max_batches=10", "-nosparse")
     # the graph is empty, so the run stops at the graph read, after the flags parsed
     assert r.returncode == 1 and "cannot read graph" in r.stderr
+
+
+@pytest.mark.parametrize("var,value,msg", [
+    ("CG_INSERTION_SCHEDULE", "warp", "CG_INSERTION_SCHEDULE must be block, thread or ordered"),
+    ("CG_BATCH_MAINTENANCE", "huge", "CG_BATCH_MAINTENANCE must be regular, large or auto"),
+])
+def test_algorithm_hints_are_validated(var, value, msg):
+    # framework.cuh:3775-3783, effective_update_batch.h:40-45: a bad value is an error
+    env = dict(os.environ, **{var: value})
+    r = subprocess.run([os.path.join(BIN, "hybrid_sssp"), "--graphfile=/nonexistent"], capture_output=True,
+                       text=True, timeout=30, env=env)
+    assert r.returncode == 1 and msg in r.stderr
+    # CG_ORDERED_REPAIR=1 forces the ordered schedule, so a stale schedule value is not read
+    if var == "CG_INSERTION_SCHEDULE":
+        env["CG_ORDERED_REPAIR"] = "1"
+        r = subprocess.run([os.path.join(BIN, "hybrid_sssp"), "--graphfile=/nonexistent"], capture_output=True,
+                           text=True, timeout=30, env=env)
+        assert r.returncode == 1 and "cannot read graph" in r.stderr

@@ -18,11 +18,12 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "dycgraph_amd", "bin")
 
 
-def _run(app, tmp, extra):
+def _run(app, tmp, extra, env=None):
     cmd = [os.path.join(BIN, f"hybrid_{app}"), f"--graphfile={tmp}/graph", f"--updatefile={tmp}/updates",
            f"--update_size={tmp}/sizes", "--format=market_big", "--source_node=0", "--SEGMENT=32", "--n_stream=3",
            "--hybrid=0", "--check=true", "--verbose=false"] + extra
-    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120,
+                       env=None if env is None else dict(os.environ, **env))
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
     return r.stdout
 
@@ -35,10 +36,17 @@ def _csr_of(n, counter):
     return oracle.csr_from_edges(n, np.array(src, np.uint32), np.array(dst, np.uint32))
 
 
+HINTS = {"default": None, "ordered-large": {"CG_ORDERED_REPAIR": "1", "CG_BATCH_MAINTENANCE": "large"},
+         "thread-auto": {"CG_INSERTION_SCHEDULE": "thread", "CG_BATCH_MAINTENANCE": "auto"}}
+
+
 @pytest.mark.parametrize("app", ["bfs", "sssp"])
 @pytest.mark.parametrize("placement", ["device", "host"])
-def test_bfs_sssp_smoke_scenario(tmp_path, app, placement):
-    announce(f"cli {app} smoke scenario placement={placement}")
+@pytest.mark.parametrize("hints", sorted(HINTS))
+def test_bfs_sssp_smoke_scenario(tmp_path, app, placement, hints):
+    """Also under the reference's algorithm hints (CG_ORDERED_REPAIR,
+    CG_INSERTION_SCHEDULE, CG_BATCH_MAINTENANCE): same checksums."""
+    announce(f"cli {app} smoke scenario placement={placement} hints={hints}")
     n = 2048
     edges = {(u, (u + 1) % n) for u in range(n)}
     edges |= {(u, 1024) for u in range(1024)} | {(0, 0), (0, 2), (1, 3)}
@@ -64,7 +72,7 @@ def test_bfs_sssp_smoke_scenario(tmp_path, app, placement):
     (tmp_path / "updates").write_text("".join(updates))
     (tmp_path / "sizes").write_text("4 4\n" * 3)
     out = _run(app, tmp_path, [f"--{app}_max_batches=3", f"--{app}_print_checksum=true", f"--placement={placement}",
-                               "--cache=0.0003", f"--output={tmp_path}/out"])
+                               "--cache=0.0003", f"--output={tmp_path}/out"], env=HINTS[hints])
     got = [(t, int(v)) for t, v in re.findall(rf"\[({tag}-DELETE-STAGE-CHECK|{tag}-BATCH-CHECK)\][^\n]*"
                                               rf"distance_checksum=(\d+)", out)]
     assert got == expected
