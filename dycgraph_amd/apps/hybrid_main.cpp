@@ -104,70 +104,32 @@ void LOG(const char *fmt, ...) {
 }
 
 // ------------------------------------------------------------------ inputs
-// market_big reader (src/utils/parser.cpp:296-406): "src dst [w]" per line,
-// n = max id + 1, rows in file order.
+// market_big graph (src/utils/parser.cpp:296-406) or the binary edge list,
+// read by the library on all host cores.
 bool read_graph(const std::string &path, std::vector<uint32_t> &src, std::vector<uint32_t> &dst, uint32_t &n) {
-    FILE *fp = std::fopen(path.c_str(), "rb");
-    if (!fp) return false;
-    std::vector<char> buf(1 << 24);
-    uint64_t maxid = 0;
-    bool any = false;
-    std::string carry;
-    size_t got;
-    auto parse_line = [&](const char *p, const char *e) {
-        auto next = [&](uint64_t &x) {
-            while (p < e && (*p == ' ' || *p == '\t' || *p == '\r')) ++p;
-            if (p >= e || *p < '0' || *p > '9') return false;
-            x = 0;
-            while (p < e && *p >= '0' && *p <= '9') x = x * 10 + (uint64_t)(*p++ - '0');
-            return true;
-        };
-        uint64_t u, v;
-        if (!next(u) || !next(v)) return;
-        src.push_back((uint32_t)u);
-        dst.push_back((uint32_t)v);
-        maxid = std::max(maxid, std::max(u, v));
-        any = true;
-    };
-    while ((got = std::fread(buf.data(), 1, buf.size(), fp)) > 0) {
-        size_t start = 0;
-        for (size_t i = 0; i < got; ++i) {
-            if (buf[i] != '\n') continue;
-            if (!carry.empty()) {
-                carry.append(buf.data() + start, i - start);
-                parse_line(carry.data(), carry.data() + carry.size());
-                carry.clear();
-            } else {
-                parse_line(buf.data() + start, buf.data() + i);
-            }
-            start = i + 1;
-        }
-        carry.append(buf.data() + start, got - start);
-    }
-    if (!carry.empty()) parse_line(carry.data(), carry.data() + carry.size());
-    std::fclose(fp);
-    n = any ? (uint32_t)(maxid + 1) : 0;
+    uint32_t *s = nullptr, *d = nullptr;
+    uint64_t m = 0;
+    if (dcg_read_graph(path.c_str(), &n, &s, &d, &m, 0) != DCG_OK) return false;
+    src.assign(s, s + m);
+    dst.assign(d, d + m);
+    dcg_free(s);
+    dcg_free(d);
     return true;
 }
 
 // Loader (include/framework/Loader.h:30-101).
 bool read_updates(const std::string &sizes_path, const std::string &upd_path, std::vector<std::pair<uint64_t, uint64_t>> &batches,
                   std::vector<dcg_edge> &adds, std::vector<dcg_edge> &dels) {
-    std::ifstream sz(sizes_path);
-    if (!sz) return false;
-    uint64_t a, d;
-    while (sz >> a >> d) batches.push_back({a, d});
-    std::ifstream up(upd_path);
-    if (!up) return false;
-    std::string type;
-    uint64_t u, v, w;
-    std::string line;
-    while (std::getline(up, line)) {
-        char t = 0;
-        if (std::sscanf(line.c_str(), " %c %lu %lu %lu", &t, &u, &v, &w) < 3) continue;
-        if (t == 'a') adds.push_back(dcg_edge{(uint32_t)u, (uint32_t)v});
-        else if (t == 'd') dels.push_back(dcg_edge{(uint32_t)u, (uint32_t)v});
-    }
+    uint64_t *sz = nullptr, nb = 0, na = 0, nd = 0;
+    dcg_edge *a = nullptr, *d = nullptr;
+    if (dcg_read_updates(sizes_path.c_str(), upd_path.c_str(), &sz, &nb, &a, &na, &d, &nd, 0) != DCG_OK)
+        return false;
+    for (uint64_t b = 0; b < nb; ++b) batches.push_back({sz[2 * b], sz[2 * b + 1]});
+    adds.assign(a, a + na);
+    dels.assign(d, d + nd);
+    dcg_free(sz);
+    dcg_free(a);
+    dcg_free(d);
     return true;
 }
 

@@ -171,6 +171,12 @@ def lib() -> C.CDLL:
         "dcg_calibrate_lines": (I, [C.c_int32, U64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "dcg_gen_grid": (I, [U32, C.c_double, U64, P, P, C.POINTER(U64)]),
         "dcg_gen_lognormal_sizes": (I, [U32, C.c_double, C.c_double, U64, U64, U64, P]),
+        "dcg_read_graph": (I, [C.c_char_p, C.POINTER(U32), C.POINTER(P), C.POINTER(P), C.POINTER(U64), I]),
+        "dcg_write_graph": (I, [C.c_char_p, U32, P, P, U64, I]),
+        "dcg_read_updates": (I, [C.c_char_p, C.c_char_p, C.POINTER(P), C.POINTER(U64), C.POINTER(P),
+                                 C.POINTER(U64), C.POINTER(P), C.POINTER(U64), I]),
+        "dcg_write_updates": (I, [C.c_char_p, C.c_char_p, P, U64, P, P]),
+        "dcg_free": (None, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -279,6 +285,55 @@ def load_program_plugin(path, symbol):
     if not ptr:
         raise DcgError(-1, f"{symbol} returned no table")
     return C.c_void_p(ptr)
+
+
+def _take(ptr, count, dtype):
+    """Copy a library-allocated array into numpy and release it."""
+    out = np.empty(count, dtype)
+    if count:
+        C.memmove(out.ctypes.data, ptr, out.nbytes)
+    lib().dcg_free(ptr)
+    return out
+
+
+def read_graph(path, threads=0):
+    """(n, src, dst) from a market_big text file or a binary edge list."""
+    n, m = C.c_uint32(), C.c_uint64()
+    s, d = C.c_void_p(), C.c_void_p()
+    st = lib().dcg_read_graph(os.fsencode(path), C.byref(n), C.byref(s), C.byref(d), C.byref(m), threads)
+    if st != 0:
+        raise DcgError(st, f"dcg_read_graph failed: {path}")
+    return n.value, _take(s, m.value, np.uint32), _take(d, m.value, np.uint32)
+
+
+def write_graph(path, n, src, dst, binary=False):
+    s = np.ascontiguousarray(src, np.uint32)
+    d = np.ascontiguousarray(dst, np.uint32)
+    st = lib().dcg_write_graph(os.fsencode(path), n, _ptr(s), _ptr(d), len(s), 1 if binary else 0)
+    if st != 0:
+        raise DcgError(st, f"dcg_write_graph failed: {path}")
+
+
+def read_updates(sizes_path, updates_path, threads=0):
+    """(sizes (batches x [adds, dels]), adds (k x 2), dels (k x 2)) from the reference's update files."""
+    sz, a, d = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    nb, na, nd = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    st = lib().dcg_read_updates(os.fsencode(sizes_path), os.fsencode(updates_path), C.byref(sz), C.byref(nb),
+                                C.byref(a), C.byref(na), C.byref(d), C.byref(nd), threads)
+    if st != 0:
+        raise DcgError(st, "dcg_read_updates failed")
+    return (_take(sz, 2 * nb.value, np.uint64).reshape(-1, 2), _take(a, 2 * na.value, np.uint32).reshape(-1, 2),
+            _take(d, 2 * nd.value, np.uint32).reshape(-1, 2))
+
+
+def write_updates(sizes_path, updates_path, sizes, adds, dels):
+    sz = np.ascontiguousarray(np.asarray(sizes, np.uint64).reshape(-1, 2))
+    a = np.ascontiguousarray(np.asarray(adds, np.uint32).reshape(-1, 2))
+    d = np.ascontiguousarray(np.asarray(dels, np.uint32).reshape(-1, 2))
+    st = lib().dcg_write_updates(os.fsencode(sizes_path), os.fsencode(updates_path), _ptr(sz), len(sz), _ptr(a),
+                                 _ptr(d))
+    if st != 0:
+        raise DcgError(st, "dcg_write_updates failed")
 
 
 def radix_sort(keys, vals=None, begin_bit=0, end_bit=64, device=0):

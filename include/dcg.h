@@ -411,6 +411,29 @@ int dcg_calibrate_lines(int32_t device, uint64_t bytes, double *host_lines_per_s
  * stream (timing boundaries). */
 int dcg_sync(dcg_engine *eng);
 
+/* Graph and update files.  market_big text (src/utils/parser.cpp:296-406):
+ * "src dst [w]" per line, n = largest id + 1, edges in file order; lines that
+ * do not start with two integers are skipped.  A file that starts with the
+ * 8 bytes "DCGEDGE1" is this library's binary edge list instead (u32 version
+ * 1, u32 num_nodes, u64 num_edges, u32 sources, u32 destinations), read
+ * without parsing.  threads <= 0: all host cores.  The arrays are allocated
+ * here and released with dcg_free.  No device is used. */
+int dcg_read_graph(const char *path, uint32_t *num_nodes, uint32_t **src, uint32_t **dst, uint64_t *num_edges,
+                   int threads);
+/* binary != 0 writes the binary edge list, else market_big lines "src dst". */
+int dcg_write_graph(const char *path, uint32_t num_nodes, const uint32_t *src, const uint32_t *dst,
+                    uint64_t num_edges, int binary);
+/* Update stream files (include/framework/Loader.h:30-101): the size file holds
+ * "<adds> <dels>" per batch (sizes[2b], sizes[2b+1]); the update file "a u v w"
+ * and "d u v w" lines, additions and deletions each in file order (w is not
+ * kept: the engine applies its program's weights). */
+int dcg_read_updates(const char *sizes_path, const char *updates_path, uint64_t **sizes, uint64_t *num_batches,
+                     dcg_edge **adds, uint64_t *num_adds, dcg_edge **dels, uint64_t *num_dels, int threads);
+/* Writes each batch's deletions then its additions, w = (u + v) % 128 + 1. */
+int dcg_write_updates(const char *sizes_path, const char *updates_path, const uint64_t *sizes, uint64_t num_batches,
+                      const dcg_edge *adds, const dcg_edge *dels);
+void dcg_free(void *p);
+
 /* Seeded input generators (no reference counterpart: the reference reads
  * files produced by its scripts/ tools).  RMAT: the first num_edges distinct
  * non-self-loop pairs in draw order of a counter-based RMAT(a,b,c) stream,
