@@ -383,6 +383,15 @@ def main():
                    "parallelism": "single GPU"},
         "p50_ms": round(float(np.percentile(lat, 50)), 3), "p90_ms": round(float(np.percentile(lat, 90)), 3),
         "cache_hit_rate": round(res["hits"] / reads, 4) if reads else None,
+        # device memory held by the engine at the end of the run (bytes): the
+        # chunk cache's frames (the capped part), the rest of the store in HBM
+        # (row descriptors, sorted-prefix lengths, fences, read counters), the
+        # per-query state, and everything together with batch scratch
+        "device_footprint": {"frames": res["info"]["frame_bytes"],
+                             "store": res["info"]["store_device_bytes"],
+                             "state": res["info"]["state_device_bytes"],
+                             "total": res["info"]["device_bytes"],
+                             "host_store": res["info"]["host_bytes"]},
         "host_link_edge_bytes": int(res["host"] * 4),
         "calibration": {"hbm_copy_GBps": round(hbm_gbs, 1), "host_zero_copy_read_GBps": round(host_gbs, 2),
                         "host_random_lines_per_s": round(host_lps), "hbm_random_lines_per_s": round(hbm_lps)},

@@ -632,8 +632,12 @@ int dcg_get_info(dcg_engine *h, dcg_engine_info *info) {
         info->chunks = e.cache.chunks;
         info->frames = e.cache.frames;
         info->host_bytes = e.h_slab ? e.total_slots * 4 : 0;
-        info->device_bytes = e.cache.d_frames.bytes() + e.d_out.bytes() + e.d_in.bytes() + e.d_st.bytes() +
-                             e.d_tick.bytes() + e.d_qa.bytes() * 3 + e.d_aff.bytes() + e.d_big.bytes();
+        uint64_t store = 0, state = 0, work = 0;
+        e.footprint(store, state, work);
+        info->device_bytes = store + state + work;
+        info->store_device_bytes = store;
+        info->state_device_bytes = state;
+        info->frame_bytes = e.cache.d_frames.bytes();
         info->cached_edge_reads = e.cum_cached_reads;
         info->host_edge_reads = e.cum_host_reads;
         info->free_blocks = e.pool_len - e.pool_taken;

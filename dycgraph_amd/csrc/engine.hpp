@@ -315,6 +315,28 @@ struct Engine {
     float *policy_reads() { return cfg.placement == DCG_PLACE_DEVICE || !cache.d_reads.n ? nullptr : cache.d_reads.p; }
 
     void sync() { DCG_HIP(hipStreamSynchronize(stream)); }
+    // Device bytes held: the store, the per-query state, batch scratch.
+    void footprint(uint64_t &store, uint64_t &state, uint64_t &work) const {
+        store = cache.d_chunk_frame.bytes() + cache.d_frames.bytes() + cache.d_reads.bytes() + cache.d_hist.bytes() +
+                cache.d_touch.bytes() + cache.d_rows_last.bytes() + cache.d_score.bytes() + cache.d_load.bytes() +
+                d_out.bytes() + d_in.bytes() + d_tail.bytes() + d_hot_out.bytes() + d_hot_in.bytes() +
+                d_hot_window.bytes() + d_hot_list.bytes() + d_hot_count.bytes() + d_sorted_out.bytes() +
+                d_sorted_in.bytes() + d_fbase_out.bytes() + d_fbase_in.bytes() + d_fence_out.bytes() +
+                d_fence_in.bytes() + d_pool.bytes();
+        state = d_sources.bytes() + d_qapps.bytes() + d_st.bytes() + d_par.bytes() + d_cc_dirty.bytes() +
+                d_far0.bytes() + d_far1.bytes() + d_fstamp.bytes() + d_thr.bytes() + d_minfar.bytes() +
+                d_fdelta.bytes() + d_epoch.bytes() + d_tick.bytes() + d_qa.bytes() + d_qb.bytes() + d_qc.bytes() +
+                d_aff.bytes() + d_big.bytes() + d_ctl.bytes() + d_step.bytes() + d_rank.bytes() + d_res.bytes() +
+                d_delta.bytes() + d_queued.bytes();
+        work = d_temp.bytes() + d_k64a.bytes() + d_k64b.bytes() + d_k64c.bytes() + d_k32a.bytes() +
+               d_v32a.bytes() + d_k32c.bytes() + d_v32c.bytes() + d_gstart.bytes() + d_gstart2.bytes() +
+               d_uniq.bytes() + d_used.bytes() + d_big_groups.bytes() + d_flags.bytes() + d_counters.bytes() +
+               d_iota.bytes() + d_prd_small.bytes() + d_prd_big.bytes() + mlp_bytes();
+        for (const auto &kind : slots)
+            for (const BatchSlot &b : kind) work += b.buf.bytes();
+    }
+    uint64_t mlp_bytes() const;   // the learned policy's device buffers (mlp.hip)
+
     void *temp(size_t bytes) {
         d_temp.ensure(bytes);
         return d_temp.p;

@@ -542,6 +542,15 @@ void mlp_stats(const Engine &e, dcg_cache_stats *st) {
     st->hot_capture = m.hot_capture;
 }
 
+uint64_t Engine::mlp_bytes() const {
+    if (!cache.mlp) return 0;
+    const MlpState &m = *cache.mlp;
+    return m.master.bytes() + m.grad.bytes() + m.m1.bytes() + m.m2.bytes() + m.part.bytes() + m.w16.bytes() +
+           m.X.bytes() + m.Y.bytes() + m.out.bytes() + m.degsum.bytes() + m.lastuse.bytes() + m.Xcur.bytes() +
+           m.mscore.bytes() + m.hot_dec.bytes() + m.plan.bytes() + m.sort_k.bytes() + m.sort_v.bytes() +
+           m.judge.bytes() + m.judge_reads.bytes() + m.judge_tmp.bytes();
+}
+
 std::vector<uint64_t> mlp_plan(Engine &e, uint64_t ab, uint64_t ac, uint64_t db, uint64_t dc) {
     MlpState &m = *e.cache.mlp;
     const uint64_t C = e.cache.chunks;

@@ -354,6 +354,8 @@ void Engine::load_edges_device(uint32_t nn, const uint32_t *d_src, const uint32_
     }
     sync();
     cache_init();
+    sync();
+    d_temp.release();   // the load's sort scratch (~8 bytes per edge); batches regrow it to their size
     loaded = true;
 }
 

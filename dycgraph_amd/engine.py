@@ -103,7 +103,8 @@ class TraceStats(C.Structure):
 class EngineInfo(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in (
         "snapshots", "store_slots", "store_used_slots", "chunks", "frames", "host_bytes",
-        "device_bytes", "cached_edge_reads", "host_edge_reads", "free_blocks", "reused_blocks")]
+        "device_bytes", "cached_edge_reads", "host_edge_reads", "free_blocks", "reused_blocks",
+        "store_device_bytes", "state_device_bytes", "frame_bytes")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -289,12 +289,19 @@ int dcg_gather_buffers(dcg_engine *eng, uint32_t query, void *out, size_t n);
 int dcg_get_csr(dcg_engine *eng, uint64_t *row_ptr, uint32_t *col, uint64_t *num_edges);
 int dcg_num_nodes(const dcg_engine *eng, uint32_t *n);
 
-/* Counters of the whole run so far. */
+/* Counters of the whole run so far.  device_bytes = everything the engine
+ * holds in device memory: store_device_bytes (cache frames, or the whole slot
+ * space with device placement; row descriptors, sorted-prefix lengths, fences,
+ * free pool, read counters, hotness window, cache-policy arrays) +
+ * state_device_bytes (per-query state, parents, stamps, frontier queues, far
+ * piles, PageRank arrays) + batch scratch (grouping keys, sort scratch, staged
+ * records, the learned policy's buffers). */
 typedef struct {
     uint64_t snapshots, store_slots, store_used_slots, chunks, frames;
     uint64_t host_bytes, device_bytes;
     uint64_t cached_edge_reads, host_edge_reads;
     uint64_t free_blocks, reused_blocks;   /* relocation blocks in the free pool / reused from it */
+    uint64_t store_device_bytes, state_device_bytes, frame_bytes;
 } dcg_engine_info;
 int dcg_get_info(dcg_engine *eng, dcg_engine_info *info);
 
