@@ -427,12 +427,52 @@ int main(int argc, char **argv) {
             return 1;
         }
     }
+    // CPU/GPU co-execution (framework.cuh:923-1000, 3344-3395): a domain map (one
+    // u16 per vertex, 0 = CPU owner) or the CPU's share of SEGMENT partitions
+    const std::string dmap = f.s("sssp_cpu_domain_map");
+    const long long cpu_cap = f.i("sssp_cpu_partition_capacity");
+    const long long segs = std::max(1LL, f.i("SEGMENT"));
+    std::vector<uint64_t> seg_start;   // capacity mode: partition boundaries with equal out-edge counts
+    if (!dmap.empty() && cpu_cap != 0) {
+        LOG("[E2-DOMAIN] protocol_error=domain_map_conflicts_with_capacity capacity=%lld\n", cpu_cap);
+        return 1;
+    }
+    if (dmap.empty() && (cpu_cap < 0 || cpu_cap > segs)) {
+        LOG("[DUAL-RUNTIME] protocol_error=invalid_cpu_capacity capacity=%lld partitions=%lld\n", cpu_cap, segs);
+        return 1;
+    }
+    if ((!dmap.empty() || cpu_cap) && (app == DCG_APP_PR || app == DCG_APP_CC)) {
+        std::fprintf(stderr, "[%s-ERROR] CPU/GPU co-execution covers SSSP and BFS\n", kTag[app]);
+        return 1;
+    }
     std::vector<uint32_t> src, dst;
     uint32_t n = 0;
     if (!read_graph(f.s("graphfile"), src, dst, n) || n == 0) {
         std::fprintf(stderr, "[%s-ERROR] cannot read graph %s\n", kTag[app], f.s("graphfile").c_str());
         return 1;
     }
+    std::vector<uint8_t> cpu_mask;   // domain map: 1 = CPU-owned
+    if (!dmap.empty()) {
+        std::ifstream in(dmap, std::ios::binary | std::ios::ate);
+        const long long expected = (long long)n * 2, actual = in.is_open() ? (long long)in.tellg() : -1;
+        if (actual != expected) {
+            LOG("[E2-DOMAIN] invalid_map path=%s expected_bytes=%lld actual_bytes=%lld\n", dmap.c_str(), expected,
+                actual);
+            return 1;
+        }
+        in.seekg(0);
+        std::vector<uint16_t> dom(n);
+        in.read(reinterpret_cast<char *>(dom.data()), expected);
+        cpu_mask.resize(n);
+        for (uint32_t v = 0; v < n; ++v) {
+            if (dom[v] > 1) {
+                LOG("[E2-DOMAIN] invalid_owner vertex=%u owner=%u\n", v, (unsigned)dom[v]);
+                return 1;
+            }
+            cpu_mask[v] = dom[v] == 0;
+        }
+    }
+
     const bool check = f.b("check");
     const uint32_t source = (uint32_t)std::min<long long>(std::max(0LL, f.i("source_node")), (long long)n - 1);
     dcg_config cfg;
@@ -467,6 +507,21 @@ int main(int argc, char **argv) {
         return fail("CG_ORDERED_REPAIR", st);   // 4 quarters: SSSP window w_max (128, ordered_gpu_repair.cuh:116-160)
     if (hints.batch == "large" && (st = dcg_set_batch_maintenance(eng, 1))) return fail("CG_BATCH_MAINTENANCE", st);
     if ((st = dcg_load_edges(eng, n, src.data(), dst.data(), src.size()))) return fail("load", st);
+    if (!cpu_mask.empty()) {
+        if ((st = dcg_set_cpu_domain(eng, cpu_mask.data(), n))) return fail("sssp_cpu_domain_map", st);
+        uint64_t cpu = 0;
+        for (uint8_t m : cpu_mask) cpu += m;
+        LOG("[E2-DOMAIN] loaded path=%s cpu_vertices=%llu cpu_vertex_share=%.6f\n", dmap.c_str(),
+            (unsigned long long)cpu, n ? (double)cpu / n : 0.0);
+    } else if (cpu_cap) {
+        std::vector<uint64_t> deg(n + 1, 0);
+        for (uint32_t u : src) ++deg[u + 1];
+        for (uint32_t v = 0; v < n; ++v) deg[v + 1] += deg[v];
+        seg_start.assign(segs + 1, n);
+        seg_start[0] = 0;
+        for (long long s = 1; s < segs; ++s)
+            seg_start[s] = (uint64_t)(std::lower_bound(deg.begin(), deg.end(), deg[n] * s / segs) - deg.begin());
+    }
     auto t_init = std::chrono::steady_clock::now();
     if ((st = dcg_start(eng))) return fail("start", st);
     if (app == DCG_APP_PR)
@@ -580,8 +635,30 @@ int main(int argc, char **argv) {
                 LOG("[I16-SNAPSHOT][batch 0] states=%llu diagnostic_only=1\n",
                     (unsigned long long)snapshot_states(i16));
             if (check && !min_check("DELETE-STAGE", b, true)) success = false;
+            if (cpu_cap) {
+                // the CPU takes the cpu_cap partitions that receive the most of this
+                // batch's insertion seeds (ties: the lower partition)
+                std::vector<std::pair<long long, long long>> rank(segs);
+                for (long long s = 0; s < segs; ++s) rank[s] = {0, s};
+                for (uint64_t k = 0; k < na; ++k) {
+                    const uint32_t v = adds[ab + k].v;
+                    const long long s = (long long)(std::upper_bound(seg_start.begin(), seg_start.end(), v) -
+                                                    seg_start.begin()) - 1;
+                    if (v < n && s >= 0 && s < segs) --rank[s].first;
+                }
+                std::sort(rank.begin(), rank.end());
+                std::vector<uint8_t> mask(n, 0);
+                for (long long k = 0; k < cpu_cap; ++k)
+                    std::fill(mask.begin() + (long)seg_start[rank[k].second],
+                              mask.begin() + (long)seg_start[rank[k].second + 1], (uint8_t)1);
+                if ((st = dcg_set_cpu_domain(eng, mask.data(), n))) return fail("sssp_cpu_partition_capacity", st);
+            }
             t = std::chrono::steady_clock::now();
             if ((st = dcg_add_edge(eng, ab, na, &as))) return fail("add_edge", st);
+            if (!dmap.empty() || cpu_cap)
+                LOG("[E2-COEXEC][batch %d] gpu_closures=%llu cpu_relaxations=%llu gpu_to_cpu=%llu cpu_to_gpu=%llu\n",
+                    (int)b, (unsigned long long)as.coexec_rounds, (unsigned long long)as.cpu_relaxations,
+                    (unsigned long long)as.proposals_to_cpu, (unsigned long long)as.proposals_to_gpu);
             if (cfg.cache_policy == DCG_CACHE_LEARNED && b + 1 < nb)
                 dcg_cache_prefetch(eng, ab + na, batches[b + 1].first, db + nd, batches[b + 1].second, &cst);
             dcg_cache_maintain(eng, &cst);

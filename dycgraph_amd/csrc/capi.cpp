@@ -697,6 +697,18 @@ int dcg_set_tuning(dcg_engine *h, uint32_t blocks_per_cu, uint32_t big_row, uint
     });
 }
 
+int dcg_set_cpu_domain(dcg_engine *h, const uint8_t *cpu_owned, size_t n) {
+    return guarded(h, [&](Engine &e) {
+        if (!e.loaded) throw Error(DCG_ERR_STATE, "set the CPU domain after the graph load");
+        if (e.app == DCG_APP_PR) throw Error(DCG_ERR_UNSUPPORTED, "co-execution covers the min-lattice closure");
+        if (cpu_owned && n != e.n) throw Error(DCG_ERR_ARG, "the CPU domain mask needs one byte per vertex");
+        if (cpu_owned && e.prog != builtin_program())
+            throw Error(DCG_ERR_UNSUPPORTED, "co-execution runs the built-in programs on the host");
+        e.sync_all();
+        e.coexec_set(cpu_owned);
+    });
+}
+
 int dcg_set_batch_maintenance(dcg_engine *h, int large) {
     return guarded(h, [&](Engine &e) { e.large_batches = large != 0; });
 }

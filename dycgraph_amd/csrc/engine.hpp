@@ -84,6 +84,7 @@ struct Engine {
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     int cu_count = 0;
     uint32_t coop_grid[10] = {};   // persistent grid per mode (5 + mode: static convergence)
+    bool coop_plain = false;       // cooperative launches refused: plain launches of the same grid
     struct Tune {
         uint32_t blocks_per_cu = 1;   // persistent workgroups (1,024 threads) per CU
         uint32_t big_row = kBigRow;   // rows above this are split into pieces
@@ -162,6 +163,49 @@ struct Engine {
     // reset of the affected set and part 2 runs the rest (pull, push, CC check)
     void enqueue_repair(int stage, int part = 0);
 
+    // CPU/GPU co-execution of the insertion closure (coexec.hip).  The CPU
+    // owns a vertex set; during an add stage it keeps those vertices' state
+    // and relaxes their out-rows on host threads, the GPU closure sends it
+    // proposals for them and receives its proposals for the others, until
+    // neither side has any.
+    struct CoExec {
+        bool on = false, active = false;
+        DevBuf<uint8_t> d_owner;                 // n: 1 = CPU-owned
+        DevBuf<unsigned long long> d_bprop;      // Q*n proposal minima (~0: none)
+        DevBuf<uint32_t> d_blist, d_bcount;      // proposed slots, their count
+        DevBuf<uint32_t> d_idx;                  // the CPU-owned vertices (ascending)
+        DevBuf<uint64_t> d_vals;                 // gather / scatter staging (Q x |C|)
+        DevBuf<uint64_t> d_props;                // CPU -> GPU proposals (g << 32 | unused, packed)
+        std::vector<uint32_t> vertices;          // CPU-owned vertex ids, ascending
+        std::vector<int32_t> index;              // n: position in `vertices`, or -1
+        std::vector<uint64_t> rp;                // host CSR of the CPU-owned out-rows
+        std::vector<uint32_t> col;
+        std::vector<uint64_t> st;                // Q x |C| packed (value << 32 | parent)
+        std::vector<uint32_t> stamp;             // Q x |C| frontier dedup
+        uint32_t round_stamp = 0;
+        int threads = 1;
+        uint64_t relax = 0, to_cpu = 0, to_gpu = 0, rounds = 0;   // last stage's counters
+        void reset() {
+            on = active = false;
+            for (DevBuf<uint32_t> *b : {&d_blist, &d_bcount, &d_idx}) b->release();
+            d_owner.release();
+            d_bprop.release();
+            d_vals.release();
+            d_props.release();
+            vertices.clear();
+            index.clear();
+            rp.clear();
+            col.clear();
+            st.clear();
+            stamp.clear();
+        }
+    } cx;
+    void coexec_set(const uint8_t *cpu_owned);
+    void coexec_add_stage();
+    void enqueue_coexec_seeds();
+    void enqueue_coexec_round(const std::vector<uint32_t> &gs, const std::vector<uint64_t> &packed);
+    void coexec_apply(const std::vector<uint32_t> &gs, const std::vector<uint64_t> &packed, uint32_t stamp, Ctl *c4);
+
     // Diagnostic traces (traces.hip).  Repair snapshot: the first delete stage
     // after arming writes the affected set, its in-rows and the states of the
     // involved vertices before and after the repair.
@@ -169,6 +213,8 @@ struct Engine {
     uint32_t i16_query = 0;
     bool i16_armed = false;
     void i16_capture();
+    void gather_rows(const RowDesc *desc, const std::vector<uint32_t> &ids, std::vector<uint64_t> &offsets,
+                     std::vector<uint32_t> &out);
     // Cache-candidate trace: the admitted prefix of the last hotness ranking.
     std::ofstream f1;
     bool f1_on = false, f1_init = false;

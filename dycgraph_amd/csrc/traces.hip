@@ -68,6 +68,35 @@ static void put_vector(std::ostream &o, const std::vector<T> &v) {
     o.write(reinterpret_cast<const char *>(v.data()), (std::streamsize)(v.size() * sizeof(T)));
 }
 
+// The live entries of the rows `ids` of `desc` (out- or in-rows), in row order,
+// as a host CSR: offsets (ids.size() + 1) and entries.
+void Engine::gather_rows(const RowDesc *desc, const std::vector<uint32_t> &ids, std::vector<uint64_t> &offsets,
+                         std::vector<uint32_t> &out) {
+    const uint32_t a = (uint32_t)ids.size();
+    offsets.assign(a + 1, 0);
+    out.clear();
+    if (!a) return;
+    DevBuf<uint32_t> d_ids;
+    DevBuf<uint64_t> d_offs;
+    d_ids.alloc(a);
+    d_offs.alloc(a);
+    DCG_HIP(hipMemcpyAsync(d_ids.p, ids.data(), (size_t)a * 4, hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(k_rows_live, dim3(grid_for(a)), dim3(256), 0, stream, desc, d_ids.p, a, d_offs.p);
+    std::vector<uint64_t> live(a);
+    DCG_HIP(hipMemcpyAsync(live.data(), d_offs.p, (size_t)a * 8, hipMemcpyDeviceToHost, stream));
+    sync();
+    for (uint32_t i = 0; i < a; ++i) offsets[i + 1] = offsets[i] + live[i];
+    out.resize(offsets[a]);
+    if (!offsets[a]) return;
+    DCG_HIP(hipMemcpyAsync(d_offs.p, offsets.data(), (size_t)a * 8, hipMemcpyHostToDevice, stream));
+    DevBuf<uint32_t> d_out_rows;
+    d_out_rows.alloc(offsets[a]);
+    hipLaunchKernelGGL(k_gather_rows, dim3(grid_for((uint64_t)a * kWave, 256, 4096)), dim3(256), 0, stream, view(),
+                       desc, d_ids.p, a, d_offs.p, d_out_rows.p);
+    DCG_HIP(hipMemcpyAsync(out.data(), d_out_rows.p, offsets[a] * 4, hipMemcpyDeviceToHost, stream));
+    sync();
+}
+
 // Runs the delete stage's repair in two parts around the two captures.
 void Engine::i16_capture() {
     const uint32_t q = i16_query;
@@ -85,32 +114,10 @@ void Engine::i16_capture() {
         if (g >= base && g - base < n) affected.push_back((uint32_t)(g - base));
     std::sort(affected.begin(), affected.end());
     affected.erase(std::unique(affected.begin(), affected.end()), affected.end());
-    const uint32_t a = (uint32_t)affected.size();
     // in-rows of the affected vertices (post-delete rows, tombstones skipped)
-    std::vector<uint64_t> offsets(a + 1, 0);
+    std::vector<uint64_t> offsets;
     std::vector<uint32_t> incoming;
-    DevBuf<uint32_t> d_ids;
-    DevBuf<uint64_t> d_offs;
-    if (a) {
-        d_ids.alloc(a);
-        d_offs.alloc(a + 1);
-        DCG_HIP(hipMemcpyAsync(d_ids.p, affected.data(), (size_t)a * 4, hipMemcpyHostToDevice, stream));
-        hipLaunchKernelGGL(k_rows_live, dim3(grid_for(a)), dim3(256), 0, stream, d_in.p, d_ids.p, a, d_offs.p);
-        std::vector<uint64_t> live(a);
-        DCG_HIP(hipMemcpyAsync(live.data(), d_offs.p, (size_t)a * 8, hipMemcpyDeviceToHost, stream));
-        sync();
-        for (uint32_t i = 0; i < a; ++i) offsets[i + 1] = offsets[i] + live[i];
-        incoming.resize(offsets[a]);
-        if (offsets[a]) {
-            DCG_HIP(hipMemcpyAsync(d_offs.p, offsets.data(), (size_t)a * 8, hipMemcpyHostToDevice, stream));
-            DevBuf<uint32_t> d_inc;
-            d_inc.alloc(offsets[a]);
-            hipLaunchKernelGGL(k_gather_rows, dim3(grid_for((uint64_t)a * kWave, 256, 4096)), dim3(256), 0, stream,
-                               view(), d_in.p, d_ids.p, a, d_offs.p, d_inc.p);
-            DCG_HIP(hipMemcpyAsync(incoming.data(), d_inc.p, offsets[a] * 4, hipMemcpyDeviceToHost, stream));
-            sync();
-        }
-    }
+    gather_rows(d_in.p, affected, offsets, incoming);
     std::vector<uint32_t> ids = incoming;
     ids.insert(ids.end(), affected.begin(), affected.end());
     ids.push_back(sources[q]);

@@ -136,7 +136,19 @@ void launch_persistent(Engine &e, uint32_t mode, TravArgs &a, bool is_static = f
         return;
     }
     const int slot = e.time_begin(k, (int)(a.ctl - e.d_ctl.p));
-    DCG_HIP(hipLaunchCooperativeKernel(fn, dim3(e.coop_grid[gi]), dim3(kPBlock), args, 0, e.stream));
+    const hipError_t err = e.coop_plain ? hipErrorIllegalState
+                                        : hipLaunchCooperativeKernel(fn, dim3(e.coop_grid[gi]), dim3(kPBlock), args,
+                                                                     0, e.stream);
+    if (err == hipErrorIllegalState) {
+        // the device's cooperative-launch resource is held by another process
+        // (one that launched cooperatively before): launch the same
+        // occupancy-sized grid plainly, as inside graphs, from now on
+        (void)hipGetLastError();
+        e.coop_plain = true;
+        DCG_HIP(hipLaunchKernel(fn, dim3(e.coop_grid[gi]), dim3(kPBlock), args, 0, e.stream));
+    } else {
+        DCG_HIP(err);
+    }
     e.time_end(slot);
 }
 
@@ -194,6 +206,13 @@ static TravArgs base_args(Engine &e) {
     a.big_row = e.tune.big_row;
     a.piece = e.tune.piece;
     a.solo_max = e.tune.solo_max;
+    if (e.cx.active) {
+        a.owner = e.cx.d_owner.p;
+        a.bprop = e.cx.d_bprop.p;
+        a.blist = e.cx.d_blist.p;
+        a.bcount = e.cx.d_bcount.p;
+        a.bcap = (uint32_t)e.cx.d_blist.n;
+    }
     if (e.bucket_width() && e.d_far0.p) {
         a.far0 = e.d_far0.p;
         a.far1 = e.d_far1.p;
@@ -441,7 +460,8 @@ void Engine::add_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
     h_step[1] = StepArgs{recs, count, {take_stamps(*this), take_stamps(*this), 0u, 0u}};
     DCG_HIP(hipMemcpyAsync(d_step.p + 1, h_step + 1, sizeof(StepArgs), hipMemcpyHostToDevice, stream));
     const int rs = time_begin(DCG_K_REPAIR_ADD, -1);
-    repair_stage(1);
+    if (cx.on) coexec_add_stage();   // the closure shared with host threads
+    else repair_stage(1);
     time_end(rs);
     mark_batches_used();
     fold_enqueue();
@@ -454,6 +474,12 @@ void Engine::add_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
     check_launch(h_ctl[5], "the add stage");
     timed[rs].bytes = stage_bytes(h_ctl, 5, 6);
     if (graph_exec[1]) collect_graph_timers(1);
+    if (st && cx.on) {
+        st->coexec_rounds = cx.rounds;
+        st->cpu_relaxations = cx.relax;
+        st->proposals_to_cpu = cx.to_cpu;
+        st->proposals_to_gpu = cx.to_gpu;
+    }
     if (st) {
         st->rounds = h_ctl[5].rounds;
         st->bucket_advances = h_ctl[5].advances;
@@ -563,6 +589,32 @@ void Engine::run_stage(int stage) {
         }
     }
     DCG_HIP(hipGraphLaunch(graph_exec[stage], stream));
+}
+
+// Co-executed add stage (coexec.hip drives it): the stage's seeds and first
+// closure, issued directly (the owner arguments vary per stage) ...
+void Engine::enqueue_coexec_seeds() { enqueue_repair(1); }
+
+// ... and one round of the host's proposals: applied with fresh round stamps
+// into the seed queue, then a push closure from the improved slots.
+void Engine::enqueue_coexec_round(const std::vector<uint32_t> &gs, const std::vector<uint64_t> &packed) {
+    reserve_stamps(*this, 2);
+    h_step[1].stamp[0] = take_stamps(*this);
+    h_step[1].stamp[1] = take_stamps(*this);
+    DCG_HIP(hipMemcpyAsync(d_step.p + 1, h_step + 1, sizeof(StepArgs), hipMemcpyHostToDevice, stream));
+    Ctl *c = d_ctl.p;
+    hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 4, (const uint32_t *)nullptr, 0u, 0u,
+                       (const uint32_t *)nullptr);
+    coexec_apply(gs, packed, h_step[1].stamp[0], c + 4);
+    hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 5, &c[4].aff_count, 0u, 0u,
+                       (const uint32_t *)nullptr);
+    TravArgs a = base_args(*this);
+    a.step = d_step.p + 1;
+    a.ctl = c + 5;
+    a.in0 = d_qc.p;
+    a.stamp_slot = 1;
+    launch_persistent(*this, M_PUSH, a);
+    DCG_HIP(hipGetLastError());
 }
 
 void Engine::repair_stage(int stage) {

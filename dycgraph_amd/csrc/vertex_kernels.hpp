@@ -63,6 +63,13 @@ struct TravArgs {
     uint32_t big_row;          // rows above this many entries are split into pieces
     uint32_t piece;            // piece size (power of two dividing 65,536)
     uint32_t solo_max;         // frontiers of at most this many entries run on workgroup 0 alone
+    // CPU/GPU co-execution of an insertion closure (null owner: off): a
+    // relaxation into a CPU-owned vertex becomes a proposal, the minimum per
+    // (query, vertex) in bprop, its slot listed once in blist
+    const uint8_t *owner;
+    unsigned long long *bprop;
+    uint32_t *blist, *bcount;
+    uint32_t bcap;
     // bucketed push closure: values at or above thr[q] wait in a far pile until
     // the near frontier drains; null far0 turns it off
     uint32_t *far0, *far1;
@@ -232,7 +239,16 @@ __device__ __forceinline__ uint32_t push_op(const TravArgs &a, uint32_t q, uint3
             const uint32_t c = app.AccumulateBuffer(s, d, app.TraversalEdgeWeight(s, d), vs);
             if (c == kInf) return 0;
             const uint64_t cur = a.st[gd];
-            if (c >= (uint32_t)(cur >> 32)) return 0;
+            if (c >= (uint32_t)(cur >> 32)) return 0;   // (a CPU-owned vertex's copy here only gets lower on the CPU)
+            if (a.owner && a.owner[d]) {
+                const unsigned long long p = ((unsigned long long)c << 32) | s;
+                if (atomicMin(&a.bprop[gd], p) == ~0ull) {
+                    const uint32_t i = atomicAdd(a.bcount, 1u);
+                    if (i < a.bcap) a.blist[i] = (uint32_t)gd;
+                    else a.ctl->overflow = 1;
+                }
+                return 0;
+            }
             const uint64_t old =
                 atomicMin((unsigned long long *)&a.st[gd], (unsigned long long)(((uint64_t)c << 32) | s));
             if (c >= (uint32_t)(old >> 32)) return 0;

@@ -51,7 +51,9 @@ class PhaseStats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in (
         "requests", "touched_sources", "changed_sources", "missing_deletes", "invalid_additions",
         "relocated_rows", "affected", "rounds", "edges_scanned", "cached_edge_reads",
-        "host_edge_reads")] + [("ms", C.c_double), ("bucket_advances", C.c_uint64), ("far_entries", C.c_uint64)]
+        "host_edge_reads")] + [("ms", C.c_double)] + [(k, C.c_uint64) for k in (
+        "bucket_advances", "far_entries", "coexec_rounds", "cpu_relaxations", "proposals_to_cpu",
+        "proposals_to_gpu")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -159,6 +161,7 @@ def lib() -> C.CDLL:
         "dcg_set_tuning": (I, [P, U32, U32, U32]),
         "dcg_autotune_stages": (I, [P, U32]),
         "dcg_set_batch_maintenance": (I, [P, I]),
+        "dcg_set_cpu_domain": (I, [P, P, C.c_size_t]),
         "dcg_autotune_last": (I, [P, C.POINTER(TuneResult)]),
         "dcg_set_buckets": (I, [P, U32]),
         "dcg_set_graphs": (I, [P, I]),
@@ -573,6 +576,15 @@ class Engine:
         st = TuneResult()
         self._check(lib().dcg_autotune(self._h, C.byref(st)), "dcg_autotune")
         return st.as_dict()
+
+    def set_cpu_domain(self, cpu_owned):
+        """Give the vertices with cpu_owned[v] != 0 to host threads in every add
+        stage (co-execution); None turns it off."""
+        if cpu_owned is None:
+            self._check(lib().dcg_set_cpu_domain(self._h, None, 0), "dcg_set_cpu_domain")
+            return
+        m = np.ascontiguousarray(cpu_owned, np.uint8)
+        self._check(lib().dcg_set_cpu_domain(self._h, _ptr(m), len(m)), "dcg_set_cpu_domain")
 
     def set_batch_maintenance(self, large):
         """True: every batch through the device-wide sort path (CG_BATCH_MAINTENANCE=large)."""

@@ -99,6 +99,10 @@ typedef struct {
     double   ms;                  /* wall time of the phase                            */
     uint64_t bucket_advances;     /* bucket advances of the ordered push closure       */
     uint64_t far_entries;         /* far-pile entries those advances examined          */
+    uint64_t coexec_rounds;       /* GPU closures of a co-executed add stage           */
+    uint64_t cpu_relaxations;     /* edges the host threads relaxed                    */
+    uint64_t proposals_to_cpu;    /* GPU -> CPU proposals (minimum per vertex)         */
+    uint64_t proposals_to_gpu;    /* CPU -> GPU proposals (minimum per vertex)         */
 } dcg_phase_stats;
 
 /* Cache maintenance counters.  With the learned policy, learned_decisions and
@@ -383,6 +387,16 @@ int dcg_autotune(dcg_engine *eng, dcg_tune_result *out);
  * dcg_autotune_last reports the last tuned stage (evaluations 0 if none). */
 int dcg_autotune_stages(dcg_engine *eng, uint32_t stages);
 int dcg_autotune_last(dcg_engine *eng, dcg_tune_result *out);
+/* CPU/GPU co-execution of the insertion closure (the reference's owner runtime:
+ * --sssp_cpu_partition_capacity / --sssp_cpu_domain_map, framework.cuh:923-1155,
+ * 5198-5517, dual_domain_event_runtime.h:81-268).  cpu_owned[v] != 0 gives vertex
+ * v to host threads: in every add stage their state of each SSSP/BFS query lives
+ * on the host, GPU relaxations into them become proposals, the host relaxes their
+ * out-rows and proposes back for GPU-owned vertices, bulk synchronously, until
+ * neither side has proposals; values are those of the all-GPU closure.  Built-in
+ * programs only; CC queries and delete stages run on the GPU alone.  n must be
+ * the vertex count; a null mask (or one with no vertex) turns it off. */
+int dcg_set_cpu_domain(dcg_engine *eng, const uint8_t *cpu_owned, size_t n);
 /* Batch maintenance (the reference's CG_BATCH_MAINTENANCE, effective_update_batch.h:
  * 34-45): 0 (default) groups batches of up to 4,096 records in one workgroup's
  * LDS sort and larger ones with the device-wide radix sort; 1 ("large") sends

@@ -55,3 +55,23 @@ def test_algorithm_hints_are_validated(var, value, msg):
         r = subprocess.run([os.path.join(BIN, "hybrid_sssp"), "--graphfile=/nonexistent"], capture_output=True,
                            text=True, timeout=30, env=env)
         assert r.returncode == 1 and "cannot read graph" in r.stderr
+
+
+def test_coexecution_flag_contract(tmp_path):
+    # framework.cuh:925-936: a map and a capacity together, a capacity above SEGMENT
+    r = _run("sssp", "--graphfile=/x", "--sssp_cpu_domain_map=/m", "--sssp_cpu_partition_capacity=2")
+    assert r.returncode == 1 and "domain_map_conflicts_with_capacity" in r.stdout + r.stderr
+    r = _run("sssp", "--graphfile=/x", "--sssp_cpu_partition_capacity=9", "--SEGMENT=8")
+    assert r.returncode == 1 and "invalid_cpu_capacity" in r.stdout + r.stderr
+    r = _run("pr", "--graphfile=/x", "--sssp_cpu_partition_capacity=1", "--SEGMENT=8")
+    assert r.returncode == 1
+    # a map of the wrong size, or with an owner other than 0/1, stops before the device is used
+    g = tmp_path / "g"
+    g.write_text("0 1\n1 2\n")
+    m = tmp_path / "m"
+    m.write_bytes(b"\0\0\1\0")
+    r = _run("sssp", f"--graphfile={g}", f"--sssp_cpu_domain_map={m}")
+    assert r.returncode == 1 and "invalid_map" in r.stdout + r.stderr and "expected_bytes=6" in r.stdout + r.stderr
+    m.write_bytes(b"\0\0\1\0\2\0")
+    r = _run("sssp", f"--graphfile={g}", f"--sssp_cpu_domain_map={m}")
+    assert r.returncode == 1 and "invalid_owner vertex=2 owner=2" in r.stdout + r.stderr

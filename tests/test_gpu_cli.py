@@ -163,11 +163,7 @@ def test_pr_smoke_scenario(tmp_path):
 
 def test_cli_trace_flags(tmp_path):
     """hybrid_sssp with --i16_repair_snapshot and --f1_cache_trace_file writes
-    both files and logs their lines; the snapshot replays without mismatches.
-    (The binaries run as child processes: their cooperative launches need the
-    device's global wave sync, which a parent that already launched
-    cooperatively keeps, so the command-line tests run before any test that
-    creates an engine in the test process.)"""
+    both files and logs their lines; the snapshot replays without mismatches."""
     announce("cli trace flags")
     n = 1024
     rng = np.random.default_rng(5)
@@ -192,3 +188,42 @@ def test_cli_trace_flags(tmp_path):
     recs = traces.read_cache_trace(f1)
     assert [b for b, _, _ in recs] == [0xFFFFFFFF, 0] and all(c == 268435455 for _, c, _ in recs)
     assert len(recs[0][2]) == n   # everything fits a 1-unit cache on this graph
+
+
+@pytest.mark.parametrize("mode", ["capacity", "map"])
+def test_cli_cpu_gpu_coexecution(tmp_path, mode):
+    """hybrid_sssp with --sssp_cpu_partition_capacity (the CPU takes the
+    partitions with the most insertion seeds of each batch) or
+    --sssp_cpu_domain_map (u16 per vertex, 0 = CPU): checks pass, the
+    distances equal the oracle's and the host did work."""
+    announce(f"cli co-execution {mode}")
+    n = 2048
+    rng = np.random.default_rng(9)
+    edges = sorted({(int(u), int(v)) for u, v in rng.integers(0, n, (12000, 2)) if u != v})
+    (tmp_path / "graph").write_text("".join(f"{u} {v}\n" for u, v in edges))
+    dl = [edges[i] for i in rng.choice(len(edges), 300, replace=False)]
+    ad = [(int(u), int(v)) for u, v in rng.integers(0, n, (300, 2)) if u != v][:300]
+    (tmp_path / "updates").write_text("".join(f"d {u} {v} 1\n" for u, v in dl) +
+                                      "".join(f"a {u} {v} 1\n" for u, v in ad))
+    (tmp_path / "sizes").write_text(f"{len(ad)} {len(dl)}\n")
+    if mode == "map":
+        dom = np.ones(n, np.uint16)
+        dom[n // 4: n // 2] = 0
+        dom.tofile(tmp_path / "domain")
+        extra = [f"--sssp_cpu_domain_map={tmp_path}/domain"]
+    else:
+        extra = ["--sssp_cpu_partition_capacity=3", "--SEGMENT=8"]
+    out = _run("sssp", tmp_path, ["--sssp_max_batches=1", f"--output={tmp_path}/out"] + extra)
+    assert "[E2-COEXEC][batch 0]" in out
+    relax = int(re.search(r"cpu_relaxations=(\d+)", out).group(1))
+    assert relax > 0
+    cur = Counter(edges)
+    for u, v in dl:
+        cur[(u, v)] -= 1
+    for u, v in ad:
+        cur[(u, v)] += 1
+    rp, col = _csr_of(n, +cur)
+    want = oracle.sssp(n, rp, col, 0)
+    rows = [list(map(int, ln.split())) for ln in (tmp_path / "out").read_text().splitlines()]
+    assert [r[1] for r in rows] == want.tolist()
+
