@@ -709,6 +709,14 @@ int dcg_set_cpu_domain(dcg_engine *h, const uint8_t *cpu_owned, size_t n) {
     });
 }
 
+int dcg_set_barrier_timeout(dcg_engine *h, uint64_t microseconds) {
+    return guarded(h, [&](Engine &e) {
+        const double ticks = (double)microseconds * (e.wall_khz ? e.wall_khz : 100000.0) / 1000.0;
+        e.bar_spin = ticks > 1.8e19 ? ~0ull : (uint64_t)ticks;
+        e.drop_graphs();   // captured launches hold the old value
+    });
+}
+
 int dcg_set_batch_maintenance(dcg_engine *h, int large) {
     return guarded(h, [&](Engine &e) { e.large_batches = large != 0; });
 }

@@ -85,6 +85,7 @@ struct Engine {
     int cu_count = 0;
     uint32_t coop_grid[10] = {};   // persistent grid per mode (5 + mode: static convergence)
     bool coop_plain = false;       // cooperative launches refused: plain launches of the same grid
+    uint64_t bar_spin = 1000000000ull;   // grid-barrier patience in wall-clock ticks (10 s at 100 MHz)
     struct Tune {
         uint32_t blocks_per_cu = 1;   // persistent workgroups (1,024 threads) per CU
         uint32_t big_row = kBigRow;   // rows above this are split into pieces

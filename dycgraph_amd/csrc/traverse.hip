@@ -206,6 +206,7 @@ static TravArgs base_args(Engine &e) {
     a.big_row = e.tune.big_row;
     a.piece = e.tune.piece;
     a.solo_max = e.tune.solo_max;
+    a.bar_spin = e.bar_spin;
     if (e.cx.active) {
         a.owner = e.cx.d_owner.p;
         a.bprop = e.cx.d_bprop.p;

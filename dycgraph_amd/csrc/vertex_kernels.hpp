@@ -63,6 +63,7 @@ struct TravArgs {
     uint32_t big_row;          // rows above this many entries are split into pieces
     uint32_t piece;            // piece size (power of two dividing 65,536)
     uint32_t solo_max;         // frontiers of at most this many entries run on workgroup 0 alone
+    uint64_t bar_spin;         // wall-clock ticks a workgroup waits at the grid barrier before giving up
     // CPU/GPU co-execution of an insertion closure (null owner: off): a
     // relaxation into a CPU-owned vertex becomes a proposal, the minimum per
     // (query, vertex) in bprop, its slot listed once in blist
@@ -717,28 +718,40 @@ __device__ __forceinline__ bool pieces_pending(const TravArgs &a, uint32_t r) {
 // generation moves publish every workgroup's writes to all XCDs.  A waiter that
 // sees no progress for ~10 s (100 MHz wall clock) marks the launch timed out
 // and every later barrier returns at once, so the launch always ends.
-__device__ __forceinline__ void gsync(const TravArgs &a) {
+// Grid barrier of the persistent closures.  Returns false when the barrier
+// gave up: some workgroup waited longer than a.bar_spin wall-clock ticks
+// (~10 s by default) and set bar_timeout.  Every caller then ends its round
+// and the launch, so no workgroup reads entries that workgroups which never
+// arrived would have written; the host reports the timeout as an error.
+__device__ __forceinline__ bool gsync(const TravArgs &a) {
+    __shared__ uint32_t passed;
     __syncthreads();
-    if (threadIdx.x == 0 && !__hip_atomic_load(&a.ctl->bar_timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-        uint32_t *arrive = &a.ctl->bar_arrive, *gen = &a.ctl->bar_gen;
-        const uint32_t g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        if (__hip_atomic_fetch_add(arrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1) {
-            __hip_atomic_store(arrive, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(gen, g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            const uint64_t t0 = wall_clock64();
-            while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
-                __builtin_amdgcn_s_sleep(1);
-                if (wall_clock64() - t0 > 1000000000ull) {
-                    __hip_atomic_store(&a.ctl->bar_timeout, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    break;
+    if (threadIdx.x == 0) {
+        uint32_t ok = 0;
+        if (!__hip_atomic_load(&a.ctl->bar_timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            uint32_t *arrive = &a.ctl->bar_arrive, *gen = &a.ctl->bar_gen;
+            const uint32_t g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            if (__hip_atomic_fetch_add(arrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1) {
+                __hip_atomic_store(arrive, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(gen, g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                const uint64_t t0 = wall_clock64();
+                while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if (wall_clock64() - t0 > a.bar_spin) {
+                        __hip_atomic_store(&a.ctl->bar_timeout, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
                 }
             }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            ok = __hip_atomic_load(&a.ctl->bar_timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ? 0u : 1u;
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        passed = ok;
     }
     __syncthreads();
+    return passed != 0;
 }
 
 __device__ __forceinline__ uint32_t thr_of(uint32_t minv, uint32_t delta) {
@@ -782,7 +795,7 @@ __device__ inline bool far_advance(const TravArgs &a, uint32_t r, const Team &t)
         for (uint32_t q = threadIdx.x; q < a.Q; q += blockDim.x)
             if (qmin[q] != kInf) atomicMin(&mf[q], qmin[q]);
     }
-    gsync(a);
+    if (!gsync(a)) return true;   // the caller's next loop test ends the launch
     const uint32_t ep = *(volatile uint32_t *)a.epoch + 1;
     if (t.tid == 0) {
         a.ctl->far_count[fp] = 0;
@@ -893,25 +906,25 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(TravArgs a) {
                     a.ctl->solo_resume_b = resume_b;
                 }
             }
-            gsync(a);
+            if (!gsync(a)) break;
             r = *(volatile uint32_t *)&a.ctl->solo_round;
             if (*(volatile uint32_t *)&a.ctl->solo_resume_b) {
                 phase_b<PS, MODE>(a, r, gt, &O, edges);
                 flush(r);
-                gsync(a);
+                if (!gsync(a)) break;
                 ++r;
             }
             continue;
         }
         round_open<PS, MODE>(a, r, gt);
-        if (MODE == M_PR) gsync(a);
+        if (MODE == M_PR && !gsync(a)) break;
         phase_a<PS, MODE>(a, r, gt, L, &O, edges, expansions);
         flush(r);
-        gsync(a);
+        if (!gsync(a)) break;
         if (pieces_pending(a, r)) {
             phase_b<PS, MODE>(a, r, gt, &O, edges);
             flush(r);
-            gsync(a);
+            if (!gsync(a)) break;
         }
         ++r;
     }

@@ -162,6 +162,7 @@ def lib() -> C.CDLL:
         "dcg_autotune_stages": (I, [P, U32]),
         "dcg_set_batch_maintenance": (I, [P, I]),
         "dcg_set_cpu_domain": (I, [P, P, C.c_size_t]),
+        "dcg_set_barrier_timeout": (I, [P, U64]),
         "dcg_autotune_last": (I, [P, C.POINTER(TuneResult)]),
         "dcg_set_buckets": (I, [P, U32]),
         "dcg_set_graphs": (I, [P, I]),
@@ -585,6 +586,10 @@ class Engine:
             return
         m = np.ascontiguousarray(cpu_owned, np.uint8)
         self._check(lib().dcg_set_cpu_domain(self._h, _ptr(m), len(m)), "dcg_set_cpu_domain")
+
+    def set_barrier_timeout(self, microseconds):
+        """Grid-barrier patience of the persistent closures (0: give up at the first wait)."""
+        self._check(lib().dcg_set_barrier_timeout(self._h, microseconds), "dcg_set_barrier_timeout")
 
     def set_batch_maintenance(self, large):
         """True: every batch through the device-wide sort path (CG_BATCH_MAINTENANCE=large)."""

@@ -397,6 +397,12 @@ int dcg_autotune_last(dcg_engine *eng, dcg_tune_result *out);
  * programs only; CC queries and delete stages run on the GPU alone.  n must be
  * the vertex count; a null mask (or one with no vertex) turns it off. */
 int dcg_set_cpu_domain(dcg_engine *eng, const uint8_t *cpu_owned, size_t n);
+/* How long a workgroup of a persistent closure waits at the grid barrier before the
+ * launch gives up (default 10 s; no reference counterpart: grid.sync() waits
+ * forever).  A launch that gives up ends every round at once and the stage fails
+ * with DCG_ERR_HIP ("a grid barrier timed out"); the engine's state is then
+ * undefined and it should be destroyed.  0 gives up at the first wait. */
+int dcg_set_barrier_timeout(dcg_engine *eng, uint64_t microseconds);
 /* Batch maintenance (the reference's CG_BATCH_MAINTENANCE, effective_update_batch.h:
  * 34-45): 0 (default) groups batches of up to 4,096 records in one workgroup's
  * LDS sort and larger ones with the device-wide radix sort; 1 ("large") sends

@@ -763,3 +763,35 @@ def test_reload_updates_between_snapshots(app):
         erp, ecol = eng.csr()
         assert same_rows(erp, ecol, store), f"rows after part {part}"
     eng.close()
+
+
+def test_barrier_timeout_ends_the_launch_and_reports():
+    """A grid barrier that gives up (forced from the host with a zero patience,
+    so every workgroup that waits gives up at once) ends every round of the
+    launch: the stage fails with DCG_ERR_HIP and every kernel exits; a fresh
+    engine on the same device is exact afterwards."""
+    import dycgraph_amd as d
+    announce("barrier timeout error path")
+    n, src, dst = rmat_graph(13, 1 << 17, seed=4)
+    adds, dels = d.gen_stream(n, src, dst, [3000], [3000], seed=2)
+    eng = _engine(app="sssp", weight_mode=1, weight_seed=3, weight_max=100)
+    eng.load_graph(n, src, dst)
+    eng.start()
+    eng.load_updates(adds, dels)
+    eng.set_barrier_timeout(0)
+    with pytest.raises(d.DcgError) as err:
+        eng.del_edge(0, 3000)
+        eng.add_edge(0, 3000)
+    assert err.value.status == -2 and "grid barrier timed out" in str(err.value)
+    eng.close()
+    eng = _engine(app="sssp", weight_mode=1, weight_seed=3, weight_max=100)
+    eng.load_graph(n, src, dst)
+    eng.start()
+    eng.load_updates(adds, dels)
+    eng.del_edge(0, 3000)
+    eng.add_edge(0, 3000)
+    store = oracle.Store(n, src, dst)
+    store.apply(dels, adds)
+    rp, col = store.csr()
+    assert np.array_equal(eng.gather_value(), oracle.sssp(n, rp, col, 0, oracle.W_SEEDED, 3, 100))
+    eng.close()
