@@ -308,11 +308,11 @@ __global__ void k_window_fold(uint32_t *hot_out, uint32_t *win, uint32_t n) {
 // comp_hotness_sssp (driver.cuh:495-517): score = the window's four counters
 // summed, 0 where the query has not reached the vertex; the sort key
 // (1023 - score) << idbits | v orders by descending score, then ascending id.
-__global__ void k_hot_keys(const uint32_t *win, const uint64_t *st, uint32_t n, uint32_t idbits, uint64_t *keys) {
+__global__ void k_hot_keys(const uint32_t *win, const VSlot *st, uint32_t n, uint32_t idbits, uint64_t *keys) {
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
         const uint32_t w = win[v];
         uint32_t score = (w & 0xFFu) + ((w >> 8) & 0xFFu) + ((w >> 16) & 0xFFu) + (w >> 24);
-        if (st && (uint32_t)(st[v] >> 32) == kInf) score = 0;
+        if (st && (uint32_t)(st[v].st >> 32) == kInf) score = 0;
         keys[v] = ((uint64_t)(1023u - score) << idbits) | v;
     }
 }
@@ -369,7 +369,7 @@ uint64_t Engine::hot_candidates(uint32_t query, uint64_t cap, uint32_t *ids, uin
     const uint64_t nb = (n + kRankChunk - 1) / kRankChunk;
     bsum.alloc(nb);
     tmp.alloc(radix_sort_temp_bytes<uint64_t>(n, false));
-    const uint64_t *st = app == DCG_APP_PR ? nullptr : d_st.p + (uint64_t)query * n;
+    const VSlot *st = app == DCG_APP_PR ? nullptr : d_vs.p + (uint64_t)query * n;
     hipLaunchKernelGGL(k_hot_keys, dim3(grid_for(n)), dim3(256), 0, stream, d_hot_window.p, st, n, idbits, keys.p);
     radix_sort<uint64_t>(stream, keys.p, sorted.p, nullptr, nullptr, n, 0, (int)idbits + 10, tmp.p, tmp.n);
     const uint64_t idmask = (1ull << idbits) - 1;

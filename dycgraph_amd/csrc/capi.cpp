@@ -573,11 +573,12 @@ int dcg_gather_values(dcg_engine *h, uint32_t query, void *out, size_t n) {
             e.sync();
             return;
         }
-        std::vector<uint64_t> tmp(n);
-        DCG_HIP(hipMemcpyAsync(tmp.data(), e.d_st.p + (uint64_t)query * e.n, n * 8, hipMemcpyDeviceToHost, e.stream));
+        std::vector<VSlot> tmp(n);
+        DCG_HIP(hipMemcpyAsync(tmp.data(), e.d_vs.p + (uint64_t)query * e.n, n * sizeof(VSlot),
+                               hipMemcpyDeviceToHost, e.stream));
         e.sync();
         uint32_t *o = static_cast<uint32_t *>(out);
-        for (size_t i = 0; i < n; ++i) o[i] = (uint32_t)(tmp[i] >> 32);
+        for (size_t i = 0; i < n; ++i) o[i] = (uint32_t)(tmp[i].st >> 32);
     });
 }
 
@@ -586,16 +587,13 @@ int dcg_gather_parents(dcg_engine *h, uint32_t query, uint32_t *out, size_t n) {
         if (!e.started) throw Error(DCG_ERR_STATE, "gather before dcg_start");
         if (!out || n != e.n || query >= e.Q) throw Error(DCG_ERR_ARG, "gather size/query mismatch");
         if (e.app == DCG_APP_PR) throw Error(DCG_ERR_UNSUPPORTED, "PageRank has no parents");
-        if (e.rooted(query)) {
-            // rooted programs (CC) keep (value, depth) in the state word and parents apart
-            DCG_HIP(hipMemcpyAsync(out, e.d_par.p + (uint64_t)query * e.n, n * 4, hipMemcpyDeviceToHost, e.stream));
-            e.sync();
-            return;
-        }
-        std::vector<uint64_t> tmp(n);
-        DCG_HIP(hipMemcpyAsync(tmp.data(), e.d_st.p + (uint64_t)query * e.n, n * 8, hipMemcpyDeviceToHost, e.stream));
+        // rooted programs (CC) keep (value, depth) in the state word and the parent apart
+        std::vector<VSlot> tmp(n);
+        DCG_HIP(hipMemcpyAsync(tmp.data(), e.d_vs.p + (uint64_t)query * e.n, n * sizeof(VSlot),
+                               hipMemcpyDeviceToHost, e.stream));
         e.sync();
-        for (size_t i = 0; i < n; ++i) out[i] = (uint32_t)tmp[i];
+        const bool rooted = e.rooted(query);
+        for (size_t i = 0; i < n; ++i) out[i] = rooted ? tmp[i].par : (uint32_t)tmp[i].st;
     });
 }
 

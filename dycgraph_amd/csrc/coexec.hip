@@ -23,14 +23,14 @@
 namespace dcg {
 
 // out[q * C + i] = st[q * n + idx[i]] for one query's slice (i < C).
-__global__ void k_cx_gather(const uint64_t *st, uint64_t base, const uint32_t *idx, uint32_t C, uint64_t *out) {
+__global__ void k_cx_gather(const VSlot *vs, uint64_t base, const uint32_t *idx, uint32_t C, uint64_t *out) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < C; i += gridDim.x * blockDim.x)
-        out[i] = st[base + idx[i]];
+        out[i] = vs[base + idx[i]].st;
 }
 
-__global__ void k_cx_scatter(uint64_t *st, uint64_t base, const uint32_t *idx, uint32_t C, const uint64_t *in) {
+__global__ void k_cx_scatter(VSlot *vs, uint64_t base, const uint32_t *idx, uint32_t C, const uint64_t *in) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < C; i += gridDim.x * blockDim.x)
-        st[base + idx[i]] = in[i];
+        vs[base + idx[i]].st = in[i];
 }
 
 // The GPU's proposals to the CPU: (slot, packed) of every listed slot, and the
@@ -46,8 +46,8 @@ __global__ void k_cx_take(unsigned long long *bprop, const uint32_t *blist, uint
 // The CPU's proposals to the GPU: a strict improvement of the value lowers the
 // state word and enqueues the slot once per stamp (the seeds of the next
 // closure), appended with one atomic per wave.
-__global__ void k_cx_apply(uint64_t *st, uint32_t *tick, uint32_t stamp, const uint64_t *packed, const uint32_t *gs,
-                           uint32_t count, uint32_t *outq, uint64_t qcap, Ctl *c) {
+__global__ void k_cx_apply(VSlot *vs, uint32_t stamp, const uint64_t *packed, const uint32_t *gs, uint32_t count,
+                           uint32_t *outq, uint64_t qcap, Ctl *c) {
     for (uint32_t b = blockIdx.x * blockDim.x; b < count; b += gridDim.x * blockDim.x) {
         const uint32_t i = b + threadIdx.x;
         bool want = false;
@@ -55,9 +55,9 @@ __global__ void k_cx_apply(uint64_t *st, uint32_t *tick, uint32_t stamp, const u
         if (i < count) {
             g = gs[i];
             const uint64_t p = packed[i];
-            if ((uint32_t)(p >> 32) < (uint32_t)(st[g] >> 32)) {
-                const uint64_t old = atomicMin((unsigned long long *)&st[g], (unsigned long long)p);
-                want = (uint32_t)(p >> 32) < (uint32_t)(old >> 32) && atomicExch(&tick[g], stamp) != stamp;
+            if ((uint32_t)(p >> 32) < (uint32_t)(vs[g].st >> 32)) {
+                const uint64_t old = atomicMin((unsigned long long *)&vs[g].st, (unsigned long long)p);
+                want = (uint32_t)(p >> 32) < (uint32_t)(old >> 32) && atomicExch(&vs[g].tick, stamp) != stamp;
             }
         }
         const uint64_t m = __ballot(want);
@@ -159,7 +159,7 @@ void Engine::coexec_add_stage() {
     // epoch start: the CPU-owned vertices' state and out-rows to the host
     cx.st.assign((size_t)Q * C, ~0ull);
     for (uint32_t q : coq)
-        hipLaunchKernelGGL(k_cx_gather, dim3(grid_for(C)), dim3(256), 0, stream, d_st.p, (uint64_t)q * n, cx.d_idx.p,
+        hipLaunchKernelGGL(k_cx_gather, dim3(grid_for(C)), dim3(256), 0, stream, d_vs.p, (uint64_t)q * n, cx.d_idx.p,
                            C, cx.d_vals.p + (uint64_t)q * C);
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipMemcpyAsync(cx.st.data(), cx.d_vals.p, (size_t)Q * C * 8, hipMemcpyDeviceToHost, stream));
@@ -268,7 +268,7 @@ void Engine::coexec_add_stage() {
     // epoch end: the CPU-owned state back into the state array
     DCG_HIP(hipMemcpyAsync(cx.d_vals.p, cx.st.data(), (size_t)Q * C * 8, hipMemcpyHostToDevice, stream));
     for (uint32_t q : coq)
-        hipLaunchKernelGGL(k_cx_scatter, dim3(grid_for(C)), dim3(256), 0, stream, d_st.p, (uint64_t)q * n, cx.d_idx.p,
+        hipLaunchKernelGGL(k_cx_scatter, dim3(grid_for(C)), dim3(256), 0, stream, d_vs.p, (uint64_t)q * n, cx.d_idx.p,
                            C, cx.d_vals.p + (uint64_t)q * C);
     DCG_HIP(hipGetLastError());
 }
@@ -283,8 +283,8 @@ void Engine::coexec_apply(const std::vector<uint32_t> &gs, const std::vector<uin
     uint32_t *d_g = reinterpret_cast<uint32_t *>(cx.d_props.p + k);
     DCG_HIP(hipMemcpyAsync(d_packed, packed.data(), (size_t)k * 8, hipMemcpyHostToDevice, stream));
     DCG_HIP(hipMemcpyAsync(d_g, gs.data(), (size_t)k * 4, hipMemcpyHostToDevice, stream));
-    hipLaunchKernelGGL(k_cx_apply, dim3(grid_for(k)), dim3(256), 0, stream, d_st.p, d_tick.p, stamp, d_packed, d_g, k,
-                       d_qc.p, (uint64_t)d_qc.n, c4);
+    hipLaunchKernelGGL(k_cx_apply, dim3(grid_for(k)), dim3(256), 0, stream, d_vs.p, stamp, d_packed, d_g, k, d_qc.p,
+                       (uint64_t)d_qc.n, c4);
     DCG_HIP(hipGetLastError());
 }
 

@@ -241,6 +241,16 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool want) {
 }
 
 // Control block of one persistent launch (frontier counters ring of 3).
+// One (query, vertex) slot: the packed state word (value << 32 | parent; rooted
+// programs: value << 32 | depth), the round stamp that dedups frontier entries,
+// and the rooted programs' parent.  Kept together (16 B, one 64-byte line holds
+// four) so a relaxation touches one line for its word and stamp, not two.
+struct alignas(16) VSlot {
+    uint64_t st;
+    uint32_t tick;
+    uint32_t par;
+};
+
 struct Ctl {
     uint32_t count[3];        // entries in the queue consumed in round r (index r%3)
     uint32_t cursor[3];       // small-row work cursor of round r

@@ -135,8 +135,9 @@ struct Engine {
     CacheState cache;
 
     // ---- vertex state ----
-    DevBuf<uint64_t> d_st;             // Q*n packed (value<<32 | parent); CC: (label<<32 | depth)
-    DevBuf<uint32_t> d_par;            // Q*n CC parents (allocated when a query runs CC)
+    DevBuf<VSlot> d_vs;                // Q*n slots: packed state (value<<32 | parent; CC: label<<32 | depth),
+                                       // dedup stamp, CC parent
+    bool has_par = false;              // a query runs a rooted program (CC): parents in the slots
     DevBuf<uint8_t> d_cc_dirty;        // per 1,024 state slots: stamped by a CC improvement since the last parent check
     DevBuf<uint32_t> d_far0, d_far1, d_fstamp;   // bucketed push closure: far piles, pile epochs
     DevBuf<uint32_t> d_thr, d_minfar, d_fdelta, d_epoch;
@@ -149,7 +150,6 @@ struct Engine {
         if (tune.bucket_quarters != ~0u) return tune.bucket_quarters;
         return max_out_deg <= 64 ? 64u : 0u;
     }
-    DevBuf<uint32_t> d_tick;           // Q*n dedup stamps
     uint32_t stamp = 1;
     DevBuf<uint32_t> d_qa, d_qb, d_qc, d_aff;
     DevBuf<BigEntry> d_big;
@@ -370,9 +370,9 @@ struct Engine {
                 d_hot_window.bytes() + d_hot_list.bytes() + d_hot_count.bytes() + d_sorted_out.bytes() +
                 d_sorted_in.bytes() + d_fbase_out.bytes() + d_fbase_in.bytes() + d_fence_out.bytes() +
                 d_fence_in.bytes() + d_pool.bytes();
-        state = d_sources.bytes() + d_qapps.bytes() + d_st.bytes() + d_par.bytes() + d_cc_dirty.bytes() +
+        state = d_sources.bytes() + d_qapps.bytes() + d_vs.bytes() + d_cc_dirty.bytes() +
                 d_far0.bytes() + d_far1.bytes() + d_fstamp.bytes() + d_thr.bytes() + d_minfar.bytes() +
-                d_fdelta.bytes() + d_epoch.bytes() + d_tick.bytes() + d_qa.bytes() + d_qb.bytes() + d_qc.bytes() +
+                d_fdelta.bytes() + d_epoch.bytes() + d_qa.bytes() + d_qb.bytes() + d_qc.bytes() +
                 d_aff.bytes() + d_big.bytes() + d_ctl.bytes() + d_step.bytes() + d_rank.bytes() + d_res.bytes() +
                 d_delta.bytes() + d_queued.bytes();
         work = d_temp.bytes() + d_k64a.bytes() + d_k64b.bytes() + d_k64c.bytes() + d_k32a.bytes() +

@@ -47,14 +47,14 @@ __global__ void k_gather_rows(StoreView s, const RowDesc *desc, const uint32_t *
 }
 
 // States of query q at ids[0..count): value and parent from the state word
-// (or, for rooted programs, the parent array); buffer = value.
-__global__ void k_gather_states(const uint64_t *st, const uint32_t *par, uint64_t base, const uint32_t *ids,
-                                uint32_t count, I16State *out) {
+// (or, for rooted programs, the slot's parent); buffer = value.
+__global__ void k_gather_states(const VSlot *vs, bool rooted, uint64_t base, const uint32_t *ids, uint32_t count,
+                                I16State *out) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
         const uint32_t v = ids[i];
-        const uint64_t x = st[base + v];
-        const uint32_t value = (uint32_t)(x >> 32);
-        out[i] = I16State{v, value, value, par ? par[base + v] : (uint32_t)x};
+        const VSlot x = vs[base + v];
+        const uint32_t value = (uint32_t)(x.st >> 32);
+        out[i] = I16State{v, value, value, rooted ? x.par : (uint32_t)x.st};
     }
 }
 
@@ -129,10 +129,10 @@ void Engine::i16_capture() {
     d_sid.alloc(m);
     d_state.alloc(m);
     DCG_HIP(hipMemcpyAsync(d_sid.p, ids.data(), (size_t)m * 4, hipMemcpyHostToDevice, stream));
-    const uint32_t *par = rooted(q) ? d_par.p : nullptr;
+    const bool root = rooted(q);
     auto states = [&]() {
         std::vector<I16State> out(m);
-        hipLaunchKernelGGL(k_gather_states, dim3(grid_for(m)), dim3(256), 0, stream, d_st.p, par, base, d_sid.p, m,
+        hipLaunchKernelGGL(k_gather_states, dim3(grid_for(m)), dim3(256), 0, stream, d_vs.p, root, base, d_sid.p, m,
                            d_state.p);
         DCG_HIP(hipMemcpyAsync(out.data(), d_state.p, (size_t)m * sizeof(I16State), hipMemcpyDeviceToHost, stream));
         sync();

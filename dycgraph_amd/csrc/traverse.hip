@@ -77,7 +77,7 @@ void check_launch(const Ctl &c, const char *stage) {
 // query stamped in this stage, in the 1,024-slot blocks a CC improvement
 // marked; the marks are cleared once every CC query was checked.
 static void cc_fix_parents(Engine &e, const TravArgs &a, const uint32_t *list, const Ctl *lc) {
-    if (!e.d_par.p) return;
+    if (!e.has_par) return;
     TravArgs ta = a;
     if (list) {
         const uint64_t work = (uint64_t)e.n * e.Q;
@@ -152,15 +152,25 @@ void launch_persistent(Engine &e, uint32_t mode, TravArgs &a, bool is_static = f
     e.time_end(slot);
 }
 
-// Stamps: each launch owns a window of 2^20 round stamps; the tick array is
+// Stamps: each launch owns a window of 2^20 round stamps; the slots' stamps are
 // cleared when the 32-bit stamp space would wrap.
+__global__ void k_clear_ticks(VSlot *vs, uint64_t count) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x)
+        vs[i].tick = 0;
+}
+
+static void clear_ticks(Engine &e) {
+    const uint64_t count = (uint64_t)e.n * e.Q;
+    if (count) hipLaunchKernelGGL(k_clear_ticks, dim3(grid_for(count)), dim3(256), 0, e.stream, e.d_vs.p, count);
+}
+
 // Before a phase: keep `windows` stamp windows ahead without a wrap, so every
 // stamp the phase takes is above the first one (the CC parent check below
 // selects the phase's vertices by stamp).
 static void reserve_stamps(Engine &e, uint32_t windows) {
     constexpr uint64_t kWindow = 1ull << 20;
     if ((uint64_t)e.stamp + (uint64_t)(windows + 2) * kWindow > 0xFFFFFFFFull) {
-        fill<uint32_t>(e.stream, e.d_tick.p, (uint64_t)e.n * e.Q, 0u);
+        clear_ticks(e);
         e.stamp = 1;
     }
 }
@@ -168,7 +178,7 @@ static void reserve_stamps(Engine &e, uint32_t windows) {
 static uint32_t take_stamps(Engine &e) {
     constexpr uint32_t kWindow = 1u << 20;
     if (e.stamp > 0xFFFFFFFFu - 2 * kWindow) {
-        fill<uint32_t>(e.stream, e.d_tick.p, (uint64_t)e.n * e.Q, 0u);
+        clear_ticks(e);
         e.stamp = 1;
     }
     const uint32_t s = e.stamp;
@@ -181,9 +191,7 @@ static TravArgs base_args(Engine &e) {
     a.s = e.view();
     a.n = e.n;
     a.Q = e.Q;
-    a.st = e.d_st.p;
-    a.par = e.d_par.p;
-    a.tick = e.d_tick.p;
+    a.vs = e.d_vs.p;
     a.cc_dirty = e.d_cc_dirty.p;
     a.sources = e.d_sources.p;
     a.qapp = e.d_qapps.p;
@@ -250,20 +258,20 @@ void Engine::alloc_state() {
     d_qc.alloc(cap);
     d_big.alloc(cap < (1ull << 26) ? cap : (1ull << 26));
     d_ctl.alloc(8);
-    d_tick.alloc(cap);
-    fill<uint32_t>(stream, d_tick.p, cap, 0u);
+    d_vs.alloc(cap);
+    fill<VSlot>(stream, d_vs.p, cap, VSlot{~0ull, 0u, kInf});
+    has_par = false;
     if (app == DCG_APP_PR) {
         d_rank.alloc(cap);
         d_res.alloc(cap);
         d_delta.alloc(cap);
         d_queued.alloc(cap);
     } else {
-        d_st.alloc(cap);
         d_aff.alloc(cap);
         bool cc = false, ordered = false;
         for (uint32_t q = 0; q < Q; ++q) (rooted(q) ? cc : ordered) = true;
         if (cc) {
-            d_par.alloc(cap);
+            has_par = true;
             const uint64_t blocks = (cap >> kDirtyLog2) + 1;
             d_cc_dirty.alloc(blocks);
             fill<uint8_t>(stream, d_cc_dirty.p, blocks, (uint8_t)0);
@@ -637,22 +645,16 @@ void Engine::repair_stage(int stage) {
 // leaves is the stage's result.
 void Engine::tune_stage(int stage) {
     const uint64_t qn = (uint64_t)n * Q;
-    DevBuf<uint64_t> st0;
-    DevBuf<uint32_t> par0;
-    st0.alloc(qn);
-    DCG_HIP(hipMemcpyAsync(st0.p, d_st.p, qn * 8, hipMemcpyDeviceToDevice, stream));
-    if (d_par.p) {
-        par0.alloc(qn);
-        DCG_HIP(hipMemcpyAsync(par0.p, d_par.p, qn * 4, hipMemcpyDeviceToDevice, stream));
-    }
+    DevBuf<VSlot> vs0;   // state words, stamps and parents after the mutation
+    vs0.alloc(qn);
+    DCG_HIP(hipMemcpyAsync(vs0.p, d_vs.p, qn * sizeof(VSlot), hipMemcpyDeviceToDevice, stream));
     hipEvent_t e0, e1;
     DCG_HIP(hipEventCreate(&e0));
     DCG_HIP(hipEventCreate(&e1));
     bool restore = false;
     auto run = [&]() {
         if (restore) {
-            DCG_HIP(hipMemcpyAsync(d_st.p, st0.p, qn * 8, hipMemcpyDeviceToDevice, stream));
-            if (d_par.p) DCG_HIP(hipMemcpyAsync(d_par.p, par0.p, qn * 4, hipMemcpyDeviceToDevice, stream));
+            DCG_HIP(hipMemcpyAsync(d_vs.p, vs0.p, qn * sizeof(VSlot), hipMemcpyDeviceToDevice, stream));
         }
         restore = true;
         reserve_stamps(*this, 2);
@@ -742,7 +744,7 @@ __global__ void k_pr_init(float *rank, float *res, uint32_t *queued, uint32_t *q
 
 // Unique sources of one PR batch (adds and deletes), deduplicated with stamps.
 __global__ void k_pr_sources(const dcg_edge *adds, uint64_t na, const dcg_edge *dels, uint64_t nd, uint32_t n,
-                             uint32_t *tick, uint32_t stamp, uint32_t *out, Ctl *c) {
+                             VSlot *vs, uint32_t stamp, uint32_t *out, Ctl *c) {
     const uint64_t total = na + nd;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t b = blockIdx.x * (uint64_t)blockDim.x; b < total; b += stride) {
@@ -752,7 +754,7 @@ __global__ void k_pr_sources(const dcg_edge *adds, uint64_t na, const dcg_edge *
         if (i < total) {
             const dcg_edge e = i < na ? adds[i] : dels[i - na];
             u = e.u;
-            if (e.u < n && e.v < n) want = atomicExch(&tick[u], stamp) != stamp;
+            if (e.u < n && e.v < n) want = atomicExch(&vs[u].tick, stamp) != stamp;
         }
         const uint64_t mask = __ballot(want);
         if (mask) {
@@ -1028,7 +1030,7 @@ void Engine::pagerank_batch(uint64_t ab, uint64_t ac, uint64_t db, uint64_t dc, 
     const uint32_t stmp = take_stamps(*this);
     const dcg_edge *adds = batch_records(kAdds, ab, ac), *dels = batch_records(kDels, db, dc);
     hipLaunchKernelGGL(k_pr_sources, dim3(grid_for(ac + dc)), dim3(256), 0, stream, adds, ac, dels,
-                       dc, n, d_tick.p, stmp, srcs, c + 0);
+                       dc, n, d_vs.p, stmp, srcs, c + 0);
     // Amend(-1): appends after the carried entries in qc (count[1] starts there)
     hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c + 1, &c[0].aff_count, 0u,
                        (uint32_t)pr_pending_count, (const uint32_t *)nullptr);

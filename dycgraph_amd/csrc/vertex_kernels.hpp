@@ -40,9 +40,7 @@ constexpr uint32_t kLdsQueries = 1024;   // queries whose far-pile minima a work
 struct TravArgs {
     StoreView s;
     uint32_t n, Q;
-    uint64_t *st;
-    uint32_t *par;             // CC parents (q*n + v), null without CC queries
-    uint32_t *tick;
+    VSlot *vs;                 // per (query, vertex): state word, dedup stamp, CC parent
     uint8_t *cc_dirty;         // per 1,024 slots: a CC improvement stamped one of them (null without CC)
     const uint32_t *sources;
     const uint32_t *qapp;      // per-query program
@@ -184,30 +182,30 @@ __device__ __forceinline__ bool invalidate_op(const TravArgs &a, uint32_t q, uin
     return Visit<PS>::on(a, q, [&](const auto &app) -> bool {
         using App = app_t<decltype(app)>;
         if constexpr (App::kRootedDeletionWitness) {
-            const uint32_t pc = a.par[gd];
+            const uint32_t pc = a.vs[gd].par;
             if (pc == kInf) return false;   // a root or already claimed
-            const uint64_t sd = a.st[gd];
+            const uint64_t sd = a.vs[gd].st;
             bool dep;
             if (pc == s) dep = true;
-            else if (rooted_offer(app, s, sv, d) == sd) dep = rooted_offer(app, pc, a.st[qoff + pc], d) != sd;
+            else if (rooted_offer(app, s, sv, d) == sd) dep = rooted_offer(app, pc, a.vs[qoff + pc].st, d) != sd;
             else dep = false;
             if (!dep) return false;
-            if (atomicExch(&a.par[gd], kInf) == kInf) return false;
+            if (atomicExch(&a.vs[gd].par, kInf) == kInf) return false;
             out_g = (uint32_t)gd;
             return true;
         } else {
-            const uint64_t sd = a.st[gd];
+            const uint64_t sd = a.vs[gd].st;
             const uint32_t vd = (uint32_t)(sd >> 32), pd = (uint32_t)sd;
             if (d == a.sources[q] || pd == kInf) return false;
             const uint32_t vs = (uint32_t)(sv >> 32);
             const uint32_t cand = vs == kInf ? kInf : app.AccumulateBuffer(s, d, app.DeletionEdgeWeight(s, d), vs);
             bool ptight = false;
             if (pd != s && vs != kInf && cand == vd) {
-                const uint32_t vp = (uint32_t)(a.st[qoff + pd] >> 32);
+                const uint32_t vp = (uint32_t)(a.vs[qoff + pd].st >> 32);
                 ptight = vp != kInf && app.AccumulateBuffer(pd, d, app.DeletionEdgeWeight(pd, d), vp) == vd;
             }
             if (!App::IsDeletionDependency(s, d, vs, vd, (uint64_t)cand, pd, ptight)) return false;
-            if (atomicExch(reinterpret_cast<uint32_t *>(&a.st[gd]), kInf) == kInf) return false;
+            if (atomicExch(reinterpret_cast<uint32_t *>(&a.vs[gd].st), kInf) == kInf) return false;
             out_g = (uint32_t)gd;
             return true;
         }
@@ -227,11 +225,11 @@ __device__ __forceinline__ uint32_t push_op(const TravArgs &a, uint32_t q, uint3
         using App = app_t<decltype(app)>;
         if constexpr (App::kRootedDeletionWitness) {
             const uint64_t cand = rooted_offer(app, s, sv, d);
-            if (cand >= a.st[gd]) return 0;
-            const uint64_t old = atomicMin((unsigned long long *)&a.st[gd], (unsigned long long)cand);
+            if (cand >= a.vs[gd].st) return 0;
+            const uint64_t old = atomicMin((unsigned long long *)&a.vs[gd].st, (unsigned long long)cand);
             if (cand >= old) return 0;
-            a.par[gd] = s;
-            if (atomicExch(&a.tick[gd], stamp) == stamp) return 0;
+            a.vs[gd].par = s;
+            if (atomicExch(&a.vs[gd].tick, stamp) == stamp) return 0;
             a.cc_dirty[gd >> kDirtyLog2] = 1;
             out_g = (uint32_t)gd;
             return 1;
@@ -239,7 +237,7 @@ __device__ __forceinline__ uint32_t push_op(const TravArgs &a, uint32_t q, uint3
             const uint32_t vs = (uint32_t)(sv >> 32);
             const uint32_t c = app.AccumulateBuffer(s, d, app.TraversalEdgeWeight(s, d), vs);
             if (c == kInf) return 0;
-            const uint64_t cur = a.st[gd];
+            const uint64_t cur = a.vs[gd].st;
             if (c >= (uint32_t)(cur >> 32)) return 0;   // (a CPU-owned vertex's copy here only gets lower on the CPU)
             if (a.owner && a.owner[d]) {
                 const unsigned long long p = ((unsigned long long)c << 32) | s;
@@ -251,7 +249,7 @@ __device__ __forceinline__ uint32_t push_op(const TravArgs &a, uint32_t q, uint3
                 return 0;
             }
             const uint64_t old =
-                atomicMin((unsigned long long *)&a.st[gd], (unsigned long long)(((uint64_t)c << 32) | s));
+                atomicMin((unsigned long long *)&a.vs[gd].st, (unsigned long long)(((uint64_t)c << 32) | s));
             if (c >= (uint32_t)(old >> 32)) return 0;
             if (bucket && a.far0 && c >= a.thr[q]) {
                 const uint32_t ep = *(volatile uint32_t *)a.epoch;
@@ -259,7 +257,7 @@ __device__ __forceinline__ uint32_t push_op(const TravArgs &a, uint32_t q, uint3
                 out_g = (uint32_t)gd;
                 return 2;
             }
-            if (atomicExch(&a.tick[gd], stamp) == stamp) return 0;
+            if (atomicExch(&a.vs[gd].tick, stamp) == stamp) return 0;
             out_g = (uint32_t)gd;
             return 1;
         }
@@ -272,7 +270,7 @@ template <class PS>
 __device__ __forceinline__ uint64_t pull_cand(const TravArgs &a, uint32_t q, uint32_t d, uint32_t x) {
     const uint64_t qoff = (uint64_t)q * a.n;
     return Visit<PS>::on(a, q, [&](const auto &app) -> uint64_t {
-        const uint32_t vx = (uint32_t)(a.st[qoff + x] >> 32);
+        const uint32_t vx = (uint32_t)(a.vs[qoff + x].st >> 32);
         if (vx == kInf) return ~0ull;
         const uint32_t c = app.AccumulateBuffer(x, d, app.TraversalEdgeWeight(x, d), vx);
         if (c == kInf) return ~0ull;
@@ -283,8 +281,8 @@ __device__ __forceinline__ uint64_t pull_cand(const TravArgs &a, uint32_t q, uin
 __device__ __forceinline__ void pull_apply(const TravArgs &a, uint32_t q, uint32_t d, uint64_t packed) {
     if (packed == ~0ull) return;
     const uint64_t gd = (uint64_t)q * a.n + d;
-    if (packed >= a.st[gd]) return;
-    atomicMin((unsigned long long *)&a.st[gd], (unsigned long long)packed);
+    if (packed >= a.vs[gd].st) return;
+    atomicMin((unsigned long long *)&a.vs[gd].st, (unsigned long long)packed);
 }
 
 // Pull of one in-edge x -> d: rooted programs take x's offer and its parent
@@ -295,11 +293,11 @@ __device__ __forceinline__ void pull_edge(const TravArgs &a, uint32_t q, uint32_
     Visit<PS>::on(a, q, [&](const auto &app) -> int {
         using App = app_t<decltype(app)>;
         if constexpr (App::kRootedDeletionWitness) {
-            const uint64_t cand = rooted_offer(app, x, a.st[qoff + x], d);
+            const uint64_t cand = rooted_offer(app, x, a.vs[qoff + x].st, d);
             const uint64_t gd = qoff + d;
-            if (cand >= a.st[gd]) return 0;
-            const uint64_t old = atomicMin((unsigned long long *)&a.st[gd], (unsigned long long)cand);
-            if (cand < old) a.par[gd] = x;
+            if (cand >= a.vs[gd].st) return 0;
+            const uint64_t old = atomicMin((unsigned long long *)&a.vs[gd].st, (unsigned long long)cand);
+            if (cand < old) a.vs[gd].par = x;
         } else {
             pull_apply(a, q, d, pull_cand<PS>(a, q, d, x));
         }
@@ -514,12 +512,20 @@ struct Team {
 // `big_row` expanded edge-balanced in the wave, larger rows listed as pieces.
 constexpr uint32_t kClaim = 4;   // batches a wave claims per cursor atomic
 
+// Entries queued for the round in slot `slot`: the counter may pass the queue
+// capacity when an append overflowed (the write was dropped and `overflow`
+// set), so readers never index past qcap.
+__device__ __forceinline__ uint32_t front_count(const TravArgs &a, uint32_t slot) {
+    const uint32_t c = *(volatile uint32_t *)&a.ctl->count[slot];
+    return c < a.qcap ? c : (uint32_t)a.qcap;
+}
+
 template <class PS, uint32_t MODE>
 __device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L, OutLds *O, uint64_t &edges,
                         uint64_t &expansions) {
     const uint32_t cur = r % 3, nxt = (r + 1) % 3;
     const uint32_t lane = lane_id();
-    const uint32_t cnt = *(volatile uint32_t *)&a.ctl->count[cur];
+    const uint32_t cnt = front_count(a, cur);
     const uint32_t *in = r == 0 ? a.in0 : ((r & 1) ? a.qa : a.qb);
     uint32_t *out = (r & 1) ? a.qb : a.qa;
     const uint32_t stamp = stamp_base(a) + r;
@@ -549,7 +555,7 @@ __device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L
             } else {
                 decode_g(a, g, q, v);
                 const RowDesc d = MODE == M_PULL ? a.s.in[v] : a.s.out[v];
-                val = MODE == M_PULL ? 0ull : a.st[g];
+                val = MODE == M_PULL ? 0ull : a.vs[g].st;
                 if (d.ext && (MODE == M_PULL || (uint32_t)(val >> 32) != kInf)) {
                     deg = d.ext;
                     off = d.off;
@@ -694,7 +700,7 @@ __device__ void round_open(const TravArgs &a, uint32_t r, const Team &t) {
         // Consume (pr_residual.cuh:53-69): fold residuals above eps into ranks
         // through the program's IsActiveNode / CombineValueBuffer hooks.
         const apps::PageRank pr(a.eps, 0.f);
-        const uint32_t cnt = *(volatile uint32_t *)&a.ctl->count[cur];
+        const uint32_t cnt = front_count(a, cur);
         const uint32_t *in = r == 0 ? a.in0 : ((r & 1) ? a.qa : a.qb);
         for (uint32_t i = t.tid; i < cnt; i += t.nthreads) {
             const uint32_t v = in[i];
@@ -786,7 +792,7 @@ __device__ inline bool far_advance(const TravArgs &a, uint32_t r, const Team &t)
         uint32_t q, v;
         const uint32_t g = src[i];
         decode_g(a, g, q, v);
-        const uint32_t val = (uint32_t)(a.st[g] >> 32);
+        const uint32_t val = (uint32_t)(a.vs[g].st >> 32);
         if (lds) atomicMin(&qmin[q], val);
         else atomicMin(&mf[q], val);
     }
@@ -817,7 +823,7 @@ __device__ inline bool far_advance(const TravArgs &a, uint32_t r, const Team &t)
             uint32_t q, v;
             g = src[i];
             decode_g(a, g, q, v);
-            near = (uint32_t)(a.st[g] >> 32) < thr_of(mf[q], a.fdelta[q]);
+            near = (uint32_t)(a.vs[g].st >> 32) < thr_of(mf[q], a.fdelta[q]);
             far = !near;
             if (far) a.fstamp[g] = ep;
         }
@@ -947,17 +953,17 @@ __global__ void k_init_state(TravArgs a) {
     for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < total;
          g += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t q = (uint32_t)(g / a.n), v = (uint32_t)(g - (uint64_t)q * a.n);
-        a.st[g] = Visit<PS>::on(a, q, [&](const auto &app) -> uint64_t {
+        a.vs[g].st = Visit<PS>::on(a, q, [&](const auto &app) -> uint64_t {
             using App = app_t<decltype(app)>;
             const uint32_t seed = (App::kVertexSeeds || v == a.sources[q]) ? app.GetInitBuffer(v) : kInf;
             if constexpr (App::kRootedDeletionWitness) {
-                a.par[g] = kInf;
+                a.vs[g].par = kInf;
                 return seed == kInf ? kInfState : ((uint64_t)seed << 32);
             } else {
                 return seed == kInf ? kInfState : (((uint64_t)seed << 32) | kInf);
             }
         });
-        a.tick[g] = 0;
+        a.vs[g].tick = 0;
     }
 }
 
@@ -997,7 +1003,7 @@ __global__ __launch_bounds__(1024) void k_seed_dels(TravArgs a) {
         if (i < total) {
             const uint32_t q = (uint32_t)(i / count);
             const dcg_edge e = recs[i - (uint64_t)q * count];
-            if (e.u < a.n && e.v < a.n) want = invalidate_op<PS>(a, q, e.u, a.st[(uint64_t)q * a.n + e.u], e.v, og);
+            if (e.u < a.n && e.v < a.n) want = invalidate_op<PS>(a, q, e.u, a.vs[(uint64_t)q * a.n + e.u].st, e.v, og);
         }
         const uint32_t pos = wg_append(want, &a.ctl->aff_count, wcnt, wbase);
         if (want) {
@@ -1017,7 +1023,7 @@ __global__ void k_reset_affected(TravArgs a) {
         const uint32_t g = a.aff[i];
         uint32_t q, v;
         decode_g(a, g, q, v);
-        a.st[g] = Visit<PS>::on(a, q, [&](const auto &app) -> uint64_t {
+        a.vs[g].st = Visit<PS>::on(a, q, [&](const auto &app) -> uint64_t {
             using App = app_t<decltype(app)>;
             if constexpr (App::kRootedDeletionWitness && App::kVertexSeeds) {
                 const uint32_t seed = app.GetInitBuffer(v);   // parent already INF
@@ -1044,23 +1050,23 @@ __device__ __forceinline__ void cc_fix_one(const TravArgs &a, uint32_t g) {
         using App = app_t<decltype(app)>;
         if constexpr (App::kRootedDeletionWitness) {
             const uint64_t qoff = (uint64_t)q * a.n;
-            const uint64_t sv = a.st[g];
-            const uint32_t p = a.par[g];
+            const uint64_t sv = a.vs[g].st;
+            const uint32_t p = a.vs[g].par;
             if ((uint32_t)sv == 0 || (uint32_t)(sv >> 32) == kInf) {
-                if (p != kInf) a.par[g] = kInf;
+                if (p != kInf) a.vs[g].par = kInf;
                 return 0;
             }
-            if (p != kInf && rooted_offer(app, p, a.st[qoff + p], v) == sv) return 0;
+            if (p != kInf && rooted_offer(app, p, a.vs[qoff + p].st, v) == sv) return 0;
             const RowDesc d = a.s.in[v];
             uint32_t best = kInf;
             for (uint32_t j = 0; j < d.ext; ++j) {
                 const uint32_t x = *row_base(a.s, d.off + j);
-                if (!(x & kTomb) && rooted_offer(app, x, a.st[qoff + x], v) == sv) {
+                if (!(x & kTomb) && rooted_offer(app, x, a.vs[qoff + x].st, v) == sv) {
                     best = x;
                     break;
                 }
             }
-            a.par[g] = best;
+            a.vs[g].par = best;
         }
         return 0;
     });
@@ -1085,7 +1091,7 @@ __global__ void k_cc_fix_parents(TravArgs a, const uint32_t *list, const Ctl *li
         if (!a.cc_dirty[b]) continue;   // the same byte for the whole workgroup
         for (uint32_t j = threadIdx.x; j < (1u << kDirtyLog2); j += blockDim.x) {
             const uint64_t g = (b << kDirtyLog2) + j;
-            if (g < lo || g >= hi || a.tick[g] < stamp_lo) continue;
+            if (g < lo || g >= hi || a.vs[g].tick < stamp_lo) continue;
             cc_fix_one<PS>(a, (uint32_t)g);
         }
     }
@@ -1108,7 +1114,7 @@ __global__ __launch_bounds__(1024) void k_seed_adds(TravArgs a, uint32_t *outq) 
             const uint32_t q = (uint32_t)(i / count);
             const dcg_edge e = recs[i - (uint64_t)q * count];
             if (e.u < a.n && e.v < a.n) {
-                const uint64_t sv = a.st[(uint64_t)q * a.n + e.u];
+                const uint64_t sv = a.vs[(uint64_t)q * a.n + e.u].st;
                 if ((uint32_t)(sv >> 32) != kInf) want = push_op<PS>(a, q, e.u, sv, e.v, stamp0, og, false) == 1;
             }
         }
