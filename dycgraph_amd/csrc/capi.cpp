@@ -735,6 +735,39 @@ int dcg_set_buckets(dcg_engine *h, uint32_t quarters) {
     });
 }
 
+int dcg_set_round_trace(dcg_engine *h, uint32_t capacity) {
+    return guarded(h, [&](Engine &e) {
+        if (capacity > (1u << 26)) throw Error(DCG_ERR_ARG, "round-trace capacity above 2^26 records");
+        e.sync_all();
+        e.drop_graphs();   // captured launches hold the trace's pointers
+        e.d_rtrace.alloc(capacity);
+        if (capacity) {
+            e.d_rtcur.alloc(1);
+            fill<uint32_t>(e.stream, e.d_rtcur.p, 1, 0u);
+        } else {
+            e.d_rtcur.release();
+        }
+        e.sync();
+    });
+}
+
+int dcg_round_trace(dcg_engine *h, dcg_round_rec *out, size_t cap, size_t *written) {
+    return guarded(h, [&](Engine &e) {
+        if (!written || (cap && !out)) throw Error(DCG_ERR_ARG, "null output");
+        *written = 0;
+        if (!e.d_rtrace.p) throw Error(DCG_ERR_STATE, "the round trace is off");
+        e.sync_all();
+        uint32_t cur = 0;
+        DCG_HIP(hipMemcpy(&cur, e.d_rtcur.p, 4, hipMemcpyDeviceToHost));
+        const size_t held = std::min<size_t>(cur, e.d_rtrace.n);
+        const size_t take = std::min(held, cap);
+        if (take) DCG_HIP(hipMemcpy(out, e.d_rtrace.p, take * sizeof(dcg_round_rec), hipMemcpyDeviceToHost));
+        *written = cur;
+        fill<uint32_t>(e.stream, e.d_rtcur.p, 1, 0u);
+        e.sync();
+    });
+}
+
 int dcg_sync(dcg_engine *h) {
     return guarded(h, [&](Engine &e) { e.sync_all(); });
 }

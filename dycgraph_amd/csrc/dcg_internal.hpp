@@ -251,7 +251,16 @@ struct alignas(16) VSlot {
     uint32_t par;
 };
 
-struct Ctl {
+// Grid barriers of kBarTreeMin or more workgroups arrive in two levels: the
+// workgroups of each of kBarGroups groups (blockIdx % kBarGroups) at their
+// group's counter, the last of each group at the top counter.
+constexpr uint32_t kBarGroups = 8;
+constexpr uint32_t kBarTreeMin = 256;
+struct alignas(256) BarWord {
+    uint32_t w;
+};
+
+struct alignas(256) Ctl {
     uint32_t count[3];        // entries in the queue consumed in round r (index r%3)
     uint32_t cursor[3];       // small-row work cursor of round r
     unsigned long long big[3];// (entries << 40) | pieces for round r
@@ -270,13 +279,17 @@ struct Ctl {
     uint32_t advances;        // bucket advances taken
     unsigned long long far_scanned;   // far-pile entries examined by the advances
     uint32_t rmax[2];         // dense PageRank sweeps: largest |residual| (float bits) by round parity
-    uint32_t bar_arrive;      // grid barrier: workgroups arrived in the current generation
-    uint32_t bar_gen;         // grid barrier generation
     uint32_t bar_timeout;     // a workgroup waited ~10 s at the barrier (the launch gives up)
     // device wall clock (s_memrealtime ticks) at the first workgroup's start and
     // the last workgroup's end of a persistent launch: its duration also when
     // the launch is a node of a replayed graph
     unsigned long long t_first, t_last;
+    // grid-barrier words, each on a 256-byte line of its own: every workgroup
+    // takes one atomic on an arrival counter while the waiters poll the
+    // generation word, and a shared line queues the polls behind the atomics
+    BarWord bar_arrive;               // arrivals at the top level in the current generation
+    BarWord bar_gen;                  // generation, moved by the last arrival
+    BarWord bar_group[kBarGroups];    // arrivals per group of workgroups (two-level barrier)
 };
 
 // Per-stage arguments read by the kernels from device memory, so one captured

@@ -93,6 +93,7 @@ struct Engine {
         uint32_t graphs = 1;            // replay each stage's repair launches as a captured hipGraph
         uint32_t bucket_quarters = ~0u;   // push-closure bucket width: SSSP w_max*q/4, BFS q; 0 off, ~0 auto
         uint32_t solo_max = 2048;         // frontiers this small run on one workgroup (no grid barriers)
+        uint32_t grid_shift = 0;          // persistent grids span (workgroups per CU * CUs) >> grid_shift
     } tune;
     bool large_batches = false;           // every batch through the device-wide sort path (no one-workgroup sort)
     uint32_t tune_arm = 0;                // repair stages still to be tuned (tune_stage)
@@ -154,6 +155,8 @@ struct Engine {
     DevBuf<uint32_t> d_qa, d_qb, d_qc, d_aff;
     DevBuf<BigEntry> d_big;
     DevBuf<Ctl> d_ctl;
+    DevBuf<dcg_round_rec> d_rtrace;    // closure round trace (empty: off)
+    DevBuf<uint32_t> d_rtcur;          // its append cursor
     Ctl *h_ctl = nullptr;              // pinned readback
     StepArgs *h_step = nullptr;        // pinned: [0] delete stage, [1] add stage
     DevBuf<StepArgs> d_step;
@@ -373,7 +376,7 @@ struct Engine {
         state = d_sources.bytes() + d_qapps.bytes() + d_vs.bytes() + d_cc_dirty.bytes() +
                 d_far0.bytes() + d_far1.bytes() + d_fstamp.bytes() + d_thr.bytes() + d_minfar.bytes() +
                 d_fdelta.bytes() + d_epoch.bytes() + d_qa.bytes() + d_qb.bytes() + d_qc.bytes() +
-                d_aff.bytes() + d_big.bytes() + d_ctl.bytes() + d_step.bytes() + d_rank.bytes() + d_res.bytes() +
+                d_aff.bytes() + d_big.bytes() + d_ctl.bytes() + d_rtrace.bytes() + d_rtcur.bytes() + d_step.bytes() + d_rank.bytes() + d_res.bytes() +
                 d_delta.bytes() + d_queued.bytes();
         work = d_temp.bytes() + d_k64a.bytes() + d_k64b.bytes() + d_k64c.bytes() + d_k32a.bytes() +
                d_v32a.bytes() + d_k32c.bytes() + d_v32c.bytes() + d_gstart.bytes() + d_gstart2.bytes() +

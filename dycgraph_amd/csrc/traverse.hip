@@ -51,7 +51,9 @@ __global__ void k_ctl_setup(Ctl *c, const uint32_t *src0, uint32_t add0, uint32_
     c->advances = 0;
     c->far_scanned = 0;
     c->rmax[0] = c->rmax[1] = 0;
-    c->bar_arrive = c->bar_gen = c->bar_timeout = 0;
+    c->bar_timeout = 0;
+    c->bar_arrive.w = c->bar_gen.w = 0;
+    for (uint32_t i = 0; i < kBarGroups; ++i) c->bar_group[i].w = 0;
     c->t_first = ~0ull;
     c->t_last = 0;
 }
@@ -102,7 +104,10 @@ uint32_t Engine::coop_blocks(const void *kernel) {
     if (per_cu < 1) per_cu = 1;
     const int want = tune.blocks_per_cu ? (int)tune.blocks_per_cu : 1;
     if (per_cu > want) per_cu = want;
-    return (uint32_t)(per_cu * cu_count);
+    // a smaller grid pays less per grid barrier (fewer arrivals) on graphs
+    // whose frontiers never fill the chip
+    const uint32_t grid = (uint32_t)(per_cu * cu_count) >> tune.grid_shift;
+    return grid ? grid : 1u;
 }
 
 // The min-lattice closures come from the engine's program table; PageRank's
@@ -122,6 +127,7 @@ void launch_persistent(Engine &e, uint32_t mode, TravArgs &a, bool is_static = f
     const void *fn = persistent_fn(e, mode, is_static);
     const uint32_t gi = mode + (is_static ? 5u : 0u);   // grid sizes per instantiation
     if (!e.coop_grid[gi]) e.coop_grid[gi] = e.coop_blocks(fn);
+    a.rtag = (uint32_t)(a.ctl - e.d_ctl.p) | mode << 8 | (is_static ? DCG_ROUND_STATIC : 0u);
     void *args[] = {&a};
     static const int kid[5] = {DCG_K_PUSH, DCG_K_PULL, DCG_K_INVALIDATE, DCG_K_PR_SOLVE, DCG_K_PR_AMEND};
     const int k = is_static ? DCG_K_STATIC : kid[mode];
@@ -215,6 +221,11 @@ static TravArgs base_args(Engine &e) {
     a.piece = e.tune.piece;
     a.solo_max = e.tune.solo_max;
     a.bar_spin = e.bar_spin;
+    if (e.d_rtrace.p) {
+        a.rtrace = e.d_rtrace.p;
+        a.rtcur = e.d_rtcur.p;
+        a.rtcap = (uint32_t)e.d_rtrace.n;
+    }
     if (e.cx.active) {
         a.owner = e.cx.d_owner.p;
         a.bprop = e.cx.d_bprop.p;
@@ -374,6 +385,7 @@ void Engine::autotune(dcg_tune_result *res) {
         res->bucket_quarters = tune.bucket_quarters;
         res->solo_max = tune.solo_max;
         res->first_ms = first;
+        res->grid_shift = tune.grid_shift;
     }
 }
 
@@ -706,9 +718,10 @@ void Engine::tune_stage(int stage) {
     };
     if (ordered) axis(tune.bucket_quarters, {4, 16, 64, 256}, true);
     axis(tune.blocks_per_cu, {1, 2}, false);
+    axis(tune.grid_shift, {1, 2, 3}, false);
     axis(tune.solo_max, {0, 512, 2048, 8192}, false);
     axis(tune.big_row, {256, 1024, 4096}, false);
-    axis(tune.piece, {512, 2048}, false);
+    axis(tune.piece, {128, 512, 2048}, false);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     for (auto &g : coop_grid) g = 0;
@@ -722,6 +735,7 @@ void Engine::tune_stage(int stage) {
     last_tune.bucket_quarters = ordered ? tune.bucket_quarters : 0u;
     last_tune.solo_max = tune.solo_max;
     last_tune.first_ms = first;
+    last_tune.grid_shift = tune.grid_shift;
 }
 
 void Engine::drop_graphs() {

@@ -82,7 +82,18 @@ struct TravArgs {
     uint32_t *queued;
     float eps;
     float sign;
+    // round trace (null: off): workgroup 0 appends one record per round
+    dcg_round_rec *rtrace;
+    uint32_t *rtcur;
+    uint32_t rtcap;
+    uint32_t rtag;             // control-block index | mode << 8 | DCG_ROUND_STATIC
 };
+
+__device__ __forceinline__ void trace_round(const TravArgs &a, uint32_t r, uint32_t frontier, uint32_t flags) {
+    if (!a.rtrace) return;
+    const uint32_t i = atomicAdd(a.rtcur, 1u);
+    if (i < a.rtcap) a.rtrace[i] = dcg_round_rec{a.rtag | flags, r, frontier, 0u, (uint64_t)wall_clock64()};
+}
 
 constexpr uint64_t kBigShift = 36;
 constexpr uint64_t kBigMask = (1ull << kBigShift) - 1;
@@ -166,6 +177,20 @@ __device__ __forceinline__ uint64_t rooted_offer(const App &app, uint32_t s, uin
 }
 
 // Dependency test of the delete stage, claiming a dependent vertex once.
+// A destination slot's state word and parent, loaded ahead of the edge
+// operation so a wave issues the loads of several 64-edge steps together.
+// States only decrease (and claims only set parents to INF), so a copy that is
+// stale by the time the edge is examined only weakens the early-out tests; the
+// atomics decide.
+struct DstPre {
+    uint64_t st;
+    uint32_t par;
+};
+__device__ __forceinline__ DstPre load_pre(const TravArgs &a, uint32_t q, uint32_t x) {
+    const VSlot s = a.vs[(uint64_t)q * a.n + x];
+    return DstPre{s.st, s.par};
+}
+
 // Rooted-witness programs: the parent edge, or an offer equal to the vertex's
 // (value, depth) word while the parent's offer is not (the reference's CC marks
 // every equal-label successor instead, hybrid_cc.cu:60-64); the claim resets
@@ -176,15 +201,15 @@ __device__ __forceinline__ uint64_t rooted_offer(const App &app, uint32_t s, uin
 // parent half of the state word.  `sv` is the state word of the edge's source.
 template <class PS>
 __device__ __forceinline__ bool invalidate_op(const TravArgs &a, uint32_t q, uint32_t s, uint64_t sv, uint32_t d,
-                                              uint32_t &out_g) {
+                                              uint32_t &out_g, bool has_pre = false, DstPre pre = {}) {
     const uint64_t qoff = (uint64_t)q * a.n;
     const uint64_t gd = qoff + d;
     return Visit<PS>::on(a, q, [&](const auto &app) -> bool {
         using App = app_t<decltype(app)>;
         if constexpr (App::kRootedDeletionWitness) {
-            const uint32_t pc = a.vs[gd].par;
+            const uint32_t pc = has_pre ? pre.par : a.vs[gd].par;
             if (pc == kInf) return false;   // a root or already claimed
-            const uint64_t sd = a.vs[gd].st;
+            const uint64_t sd = has_pre ? pre.st : a.vs[gd].st;
             bool dep;
             if (pc == s) dep = true;
             else if (rooted_offer(app, s, sv, d) == sd) dep = rooted_offer(app, pc, a.vs[qoff + pc].st, d) != sd;
@@ -194,7 +219,7 @@ __device__ __forceinline__ bool invalidate_op(const TravArgs &a, uint32_t q, uin
             out_g = (uint32_t)gd;
             return true;
         } else {
-            const uint64_t sd = a.vs[gd].st;
+            const uint64_t sd = has_pre ? pre.st : a.vs[gd].st;
             const uint32_t vd = (uint32_t)(sd >> 32), pd = (uint32_t)sd;
             if (d == a.sources[q] || pd == kInf) return false;
             const uint32_t vs = (uint32_t)(sv >> 32);
@@ -219,13 +244,14 @@ __device__ __forceinline__ bool invalidate_op(const TravArgs &a, uint32_t q, uin
 // so the smallest source wins a tie (framework.cuh:229-244).
 template <class PS>
 __device__ __forceinline__ uint32_t push_op(const TravArgs &a, uint32_t q, uint32_t s, uint64_t sv, uint32_t d,
-                                            uint32_t stamp, uint32_t &out_g, bool bucket = true) {
+                                            uint32_t stamp, uint32_t &out_g, bool bucket = true,
+                                            bool has_pre = false, DstPre pre = {}) {
     const uint64_t gd = (uint64_t)q * a.n + d;
     return Visit<PS>::on(a, q, [&](const auto &app) -> uint32_t {
         using App = app_t<decltype(app)>;
         if constexpr (App::kRootedDeletionWitness) {
             const uint64_t cand = rooted_offer(app, s, sv, d);
-            if (cand >= a.vs[gd].st) return 0;
+            if (cand >= (has_pre ? pre.st : a.vs[gd].st)) return 0;
             const uint64_t old = atomicMin((unsigned long long *)&a.vs[gd].st, (unsigned long long)cand);
             if (cand >= old) return 0;
             a.vs[gd].par = s;
@@ -237,7 +263,7 @@ __device__ __forceinline__ uint32_t push_op(const TravArgs &a, uint32_t q, uint3
             const uint32_t vs = (uint32_t)(sv >> 32);
             const uint32_t c = app.AccumulateBuffer(s, d, app.TraversalEdgeWeight(s, d), vs);
             if (c == kInf) return 0;
-            const uint64_t cur = a.vs[gd].st;
+            const uint64_t cur = has_pre ? pre.st : a.vs[gd].st;
             if (c >= (uint32_t)(cur >> 32)) return 0;   // (a CPU-owned vertex's copy here only gets lower on the CPU)
             if (a.owner && a.owner[d]) {
                 const unsigned long long p = ((unsigned long long)c << 32) | s;
@@ -267,10 +293,11 @@ __device__ __forceinline__ uint32_t push_op(const TravArgs &a, uint32_t q, uint3
 // Pull (non-rooted programs): the row owner `d` is the destination, `x` an
 // in-neighbour; the packed (value, parent) candidate or ~0.
 template <class PS>
-__device__ __forceinline__ uint64_t pull_cand(const TravArgs &a, uint32_t q, uint32_t d, uint32_t x) {
+__device__ __forceinline__ uint64_t pull_cand(const TravArgs &a, uint32_t q, uint32_t d, uint32_t x,
+                                             bool has_pre = false, DstPre pre = {}) {
     const uint64_t qoff = (uint64_t)q * a.n;
     return Visit<PS>::on(a, q, [&](const auto &app) -> uint64_t {
-        const uint32_t vx = (uint32_t)(a.vs[qoff + x].st >> 32);
+        const uint32_t vx = (uint32_t)((has_pre ? pre.st : a.vs[qoff + x].st) >> 32);
         if (vx == kInf) return ~0ull;
         const uint32_t c = app.AccumulateBuffer(x, d, app.TraversalEdgeWeight(x, d), vx);
         if (c == kInf) return ~0ull;
@@ -288,18 +315,19 @@ __device__ __forceinline__ void pull_apply(const TravArgs &a, uint32_t q, uint32
 // Pull of one in-edge x -> d: rooted programs take x's offer and its parent
 // entry; others min-combine the packed candidate.
 template <class PS>
-__device__ __forceinline__ void pull_edge(const TravArgs &a, uint32_t q, uint32_t d, uint32_t x) {
+__device__ __forceinline__ void pull_edge(const TravArgs &a, uint32_t q, uint32_t d, uint32_t x,
+                                          bool has_pre = false, DstPre pre = {}) {
     const uint64_t qoff = (uint64_t)q * a.n;
     Visit<PS>::on(a, q, [&](const auto &app) -> int {
         using App = app_t<decltype(app)>;
         if constexpr (App::kRootedDeletionWitness) {
-            const uint64_t cand = rooted_offer(app, x, a.vs[qoff + x].st, d);
+            const uint64_t cand = rooted_offer(app, x, has_pre ? pre.st : a.vs[qoff + x].st, d);
             const uint64_t gd = qoff + d;
             if (cand >= a.vs[gd].st) return 0;
             const uint64_t old = atomicMin((unsigned long long *)&a.vs[gd].st, (unsigned long long)cand);
             if (cand < old) a.vs[gd].par = x;
         } else {
-            pull_apply(a, q, d, pull_cand<PS>(a, q, d, x));
+            pull_apply(a, q, d, pull_cand<PS>(a, q, d, x, has_pre, pre));
         }
         return 0;
     });
@@ -322,14 +350,25 @@ __device__ __forceinline__ bool pr_add(const TravArgs &a, uint32_t d, float c, u
 template <class PS, uint32_t MODE>
 // 0: nothing to publish, 1: next frontier, 2: far pile (push only).
 __device__ __forceinline__ uint32_t edge_op(const TravArgs &a, uint32_t q, uint32_t owner, uint64_t val, uint32_t x,
-                                            uint32_t stamp, uint32_t &out_g) {
+                                            uint32_t stamp, uint32_t &out_g, bool has_pre = false, DstPre pre = {}) {
     if (x & kTomb) return 0;   // deleted entry of a large row
-    if (MODE == M_PUSH) return push_op<PS>(a, q, owner, val, x, stamp, out_g);
-    if (MODE == M_INVALIDATE) return invalidate_op<PS>(a, q, owner, val, x, out_g) ? 1u : 0u;
+    if (MODE == M_PUSH) return push_op<PS>(a, q, owner, val, x, stamp, out_g, true, has_pre, pre);
+    if (MODE == M_INVALIDATE) return invalidate_op<PS>(a, q, owner, val, x, out_g, has_pre, pre) ? 1u : 0u;
     if (MODE == M_PR || MODE == M_PR_AMEND) return pr_add(a, x, __uint_as_float((uint32_t)val), out_g) ? 1u : 0u;
-    pull_edge<PS>(a, q, owner, x);
+    pull_edge<PS>(a, q, owner, x, has_pre, pre);
     return 0;
 }
+
+// The edge loops of the min-lattice modes are software-pipelined: a wave loads
+// the row entries of the step after next and the destination slots of the next
+// step before it runs the current step's operations, so the two dependent loads
+// of a step overlap the work of the steps before it.  PageRank's residual adds
+// read no slot.
+template <uint32_t MODE>
+constexpr bool preloads() {
+    return MODE == M_PUSH || MODE == M_INVALIDATE || MODE == M_PULL;
+}
+
 // Append the lanes' results to the next frontier (and, while invalidating, to
 // the affected list) with one atomic per wave.
 template <uint32_t MODE>
@@ -525,10 +564,10 @@ __device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L
                         uint64_t &expansions) {
     const uint32_t cur = r % 3, nxt = (r + 1) % 3;
     const uint32_t lane = lane_id();
-    const uint32_t cnt = front_count(a, cur);
     const uint32_t *in = r == 0 ? a.in0 : ((r & 1) ? a.qa : a.qb);
     uint32_t *out = (r & 1) ? a.qb : a.qa;
     const uint32_t stamp = stamp_base(a) + r;
+    const uint32_t cnt = front_count(a, cur);
     uint32_t batch = t.wave, spare = 0;   // the first batch is static; then kClaim per atomic
     for (;;) {
         const uint32_t b = batch * kWave;
@@ -591,22 +630,42 @@ __device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         edges += total;
-        for (uint32_t k0 = 0; k0 < total; k0 += kWave) {
-            const uint32_t k = k0 + lane;
-            uint32_t kind = 0;
-            uint32_t og = 0;
+        // entry k of the wave's edges: its row entry and owner slot in L
+        auto fetch_x = [&](uint32_t k, uint32_t &x, uint32_t &lo) {
+            x = kTomb;   // no edge (a tombstone is skipped the same way)
+            lo = 0;
             if (k < total) {
-                uint32_t lo = 0, hi = 64;   // last owner with pref <= k
+                uint32_t hi = 64;   // last owner with pref <= k
                 while (hi - lo > 1) {
                     const uint32_t mid = (lo + hi) >> 1;
                     if (L.pref[mid] <= k) lo = mid;
                     else hi = mid;
                 }
-                const uint32_t x = L.base[lo][k - L.pref[lo]];
-                kind = edge_op<PS, MODE>(a, L.q[lo], L.owner[lo], L.val[lo], x, stamp, og);
+                x = L.base[lo][k - L.pref[lo]];
             }
+        };
+        auto fetch_pre = [&](uint32_t x, uint32_t lo) {
+            return (preloads<MODE>() && !(x & kTomb)) ? load_pre(a, L.q[lo], x) : DstPre{};
+        };
+        uint32_t x1, lo1, x2, lo2;
+        fetch_x(lane, x1, lo1);
+        fetch_x(kWave + lane, x2, lo2);
+        DstPre p1 = fetch_pre(x1, lo1);
+        for (uint32_t k0 = 0; k0 < total; k0 += kWave) {
+            uint32_t x3, lo3;
+            fetch_x(k0 + 2 * kWave + lane, x3, lo3);
+            const DstPre p2 = fetch_pre(x2, lo2);
+            uint32_t kind = 0;
+            uint32_t og = 0;
+            if (!(x1 & kTomb))
+                kind = edge_op<PS, MODE>(a, L.q[lo1], L.owner[lo1], L.val[lo1], x1, stamp, og, preloads<MODE>(), p1);
             if (MODE != M_PULL) emit_b<MODE>(O, a, out, &a.ctl->count[nxt], kind == 1, og);
             if (MODE == M_PUSH && a.far0) emit_far_b(O, a, kind == 2, og);
+            x1 = x2;
+            lo1 = lo2;
+            p1 = p2;
+            x2 = x3;
+            lo2 = lo3;
         }
         __builtin_amdgcn_wave_barrier();
         if (spare) {
@@ -651,17 +710,32 @@ __device__ void phase_b(const TravArgs &a, uint32_t r, const Team &t, OutLds *O,
             if (MODE == M_PR || MODE == M_PR_AMEND) v = be.g;
             else decode_g(a, be.g, q, v);
             edges += lane == 0 ? len : 0;
+            // pipelined as in phase A: entries two steps ahead, slots one step ahead
+            auto fetch_x = [&](uint32_t k) { return k < len ? base[k] : kTomb; };
+            auto fetch_pre = [&](uint32_t x) {
+                return (preloads<MODE>() && !(x & kTomb)) ? load_pre(a, q, x) : DstPre{};
+            };
+            uint32_t x1 = fetch_x(lane), x2 = fetch_x(kWave + lane);
+            DstPre p1 = fetch_pre(x1);
             if (MODE == M_PULL && rooted_query<PS>(a, q)) {
-                for (uint32_t k = lane; k < len; k += kWave) {
-                    const uint32_t x = base[k];
-                    if (!(x & kTomb)) pull_edge<PS>(a, q, v, x);
+                for (uint32_t k0 = 0; k0 < len; k0 += kWave) {
+                    const uint32_t x3 = fetch_x(k0 + 2 * kWave + lane);
+                    const DstPre p2 = fetch_pre(x2);
+                    if (!(x1 & kTomb)) pull_edge<PS>(a, q, v, x1, true, p1);
+                    x1 = x2;
+                    p1 = p2;
+                    x2 = x3;
                 }
             } else if (MODE == M_PULL) {
                 uint64_t best = ~0ull;
-                for (uint32_t k = lane; k < len; k += kWave) {
-                    const uint32_t x = base[k];
-                    const uint64_t c = (x & kTomb) ? ~0ull : pull_cand<PS>(a, q, v, x);
+                for (uint32_t k0 = 0; k0 < len; k0 += kWave) {
+                    const uint32_t x3 = fetch_x(k0 + 2 * kWave + lane);
+                    const DstPre p2 = fetch_pre(x2);
+                    const uint64_t c = (x1 & kTomb) ? ~0ull : pull_cand<PS>(a, q, v, x1, true, p1);
                     best = c < best ? c : best;
+                    x1 = x2;
+                    p1 = p2;
+                    x2 = x3;
                 }
                 for (int d = 32; d >= 1; d >>= 1) {
                     const uint64_t o = __shfl_xor(best, d, kWave);
@@ -670,12 +744,17 @@ __device__ void phase_b(const TravArgs &a, uint32_t r, const Team &t, OutLds *O,
                 if (lane == 0) pull_apply(a, q, v, best);
             } else {
                 for (uint32_t k0 = 0; k0 < len; k0 += kWave) {
-                    const uint32_t k = k0 + lane;
+                    const uint32_t x3 = fetch_x(k0 + 2 * kWave + lane);
+                    const DstPre p2 = fetch_pre(x2);
                     uint32_t kind = 0;
                     uint32_t og = 0;
-                    if (k < len) kind = edge_op<PS, MODE>(a, q, v, be.val, base[k], stamp, og);
+                    if (!(x1 & kTomb))
+                        kind = edge_op<PS, MODE>(a, q, v, be.val, x1, stamp, og, preloads<MODE>(), p1);
                     emit_b<MODE>(O, a, out, &a.ctl->count[nxt], kind == 1, og);
                     if (MODE == M_PUSH && a.far0) emit_far_b(O, a, kind == 2, og);
+                    x1 = x2;
+                    p1 = p2;
+                    x2 = x3;
                 }
             }
         }
@@ -719,40 +798,62 @@ __device__ __forceinline__ bool pieces_pending(const TravArgs &a, uint32_t r) {
 }
 
 // Grid barrier of a persistent launch (co-resident workgroups: the launch is
-// cooperative).  One arrival counter and a generation word per launch in its
-// control block; agent-scope release before arriving and acquire after the
-// generation moves publish every workgroup's writes to all XCDs.  A waiter that
-// sees no progress for ~10 s (100 MHz wall clock) marks the launch timed out
-// and every later barrier returns at once, so the launch always ends.
-// Grid barrier of the persistent closures.  Returns false when the barrier
-// gave up: some workgroup waited longer than a.bar_spin wall-clock ticks
-// (~10 s by default) and set bar_timeout.  Every caller then ends its round
-// and the launch, so no workgroup reads entries that workgroups which never
-// arrived would have written; the host reports the timeout as an error.
+// cooperative).  Agent-scope release before arriving and acquire after the
+// generation moves publish every workgroup's writes to all XCDs.  Grids of
+// kBarTreeMin or more workgroups arrive in two levels (the group counters
+// spread the atomics that one counter would serialise).  Returns false when the
+// barrier gave up: some workgroup waited longer than a.bar_spin wall-clock
+// ticks (~10 s by default) and set bar_timeout.  Every caller then ends its
+// round and the launch, so no workgroup reads entries that workgroups which
+// never arrived would have written; the host reports the timeout as an error.
+__device__ __forceinline__ uint32_t ld_agent(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(uint32_t *p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t add_agent(uint32_t *p, uint32_t v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ bool gsync(const TravArgs &a) {
     __shared__ uint32_t passed;
     __syncthreads();
     if (threadIdx.x == 0) {
+        Ctl *c = a.ctl;
         uint32_t ok = 0;
-        if (!__hip_atomic_load(&a.ctl->bar_timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-            uint32_t *arrive = &a.ctl->bar_arrive, *gen = &a.ctl->bar_gen;
-            const uint32_t g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (!ld_agent(&c->bar_timeout)) {
+            const uint32_t g = ld_agent(&c->bar_gen.w);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            if (__hip_atomic_fetch_add(arrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1) {
-                __hip_atomic_store(arrive, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(gen, g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t top = gridDim.x;
+            bool arrive_top = true;
+            if (gridDim.x >= kBarTreeMin) {
+                const uint32_t grp = blockIdx.x % kBarGroups;
+                const uint32_t members = gridDim.x / kBarGroups + (grp < gridDim.x % kBarGroups ? 1u : 0u);
+                top = kBarGroups;
+                arrive_top = add_agent(&c->bar_group[grp].w, 1u) == members - 1;
+                if (arrive_top) {
+                    st_agent(&c->bar_group[grp].w, 0u);
+                    // the group's writes travel on with this workgroup's arrival
+                    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+                }
+            }
+            if (arrive_top && add_agent(&c->bar_arrive.w, 1u) == top - 1) {
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                st_agent(&c->bar_arrive.w, 0u);
+                __hip_atomic_store(&c->bar_gen.w, g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             } else {
                 const uint64_t t0 = wall_clock64();
-                while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                while (ld_agent(&c->bar_gen.w) == g) {
                     __builtin_amdgcn_s_sleep(1);
                     if (wall_clock64() - t0 > a.bar_spin) {
-                        __hip_atomic_store(&a.ctl->bar_timeout, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        st_agent(&c->bar_timeout, 1u);
                         break;
                     }
                 }
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            ok = __hip_atomic_load(&a.ctl->bar_timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ? 0u : 1u;
+            ok = ld_agent(&c->bar_timeout) ? 0u : 1u;
         }
         passed = ok;
     }
@@ -835,7 +936,7 @@ __device__ inline bool far_advance(const TravArgs &a, uint32_t r, const Team &t)
     return true;
 }
 
-constexpr uint32_t kSoloPieces = 64;     // more pending pieces than this end a solo stretch
+constexpr uint32_t kSoloPieces = kPBlock / kWave;   // more pending pieces than one per wave end a solo stretch
 
 // One cooperative launch (1,024-thread workgroups) runs rounds until the
 // frontier is empty, with the gsync grid barrier between rounds.  While the
@@ -878,6 +979,8 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(TravArgs a) {
             // threshold; two in a row without a round would mean none moved
             if (MODE == M_PUSH && a.far0 && idle_advances < 2 && far_advance(a, r, gt)) {
                 ++idle_advances;
+                if (blockIdx.x == 0 && threadIdx.x == 0)
+                    trace_round(a, r, *(volatile uint32_t *)&a.ctl->count[r % 3], DCG_ROUND_ADVANCE);
                 continue;
             }
             if (idle_advances >= 2 && threadIdx.x == 0 && blockIdx.x == 0) a.ctl->overflow = 1;
@@ -890,6 +993,7 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(TravArgs a) {
                 // empties, or a round queues many large-row pieces
                 uint32_t resume_b = 0;
                 for (;;) {
+                    if (threadIdx.x == 0) trace_round(a, r, front_count(a, r % 3), DCG_ROUND_SOLO);
                     round_open<PS, MODE>(a, r, bt);
                     __syncthreads();
                     phase_a<PS, MODE>(a, r, bt, L, nullptr, edges, expansions);
@@ -922,6 +1026,7 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(TravArgs a) {
             }
             continue;
         }
+        if (blockIdx.x == 0 && threadIdx.x == 0) trace_round(a, r, cnt, 0u);
         round_open<PS, MODE>(a, r, gt);
         if (MODE == M_PR && !gsync(a)) break;
         phase_a<PS, MODE>(a, r, gt, L, &O, edges, expansions);
@@ -941,6 +1046,7 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(TravArgs a) {
     wg_sum_flush(tot, x, dst);
     __syncthreads();
     if (threadIdx.x == 0) atomicMax(&a.ctl->t_last, (unsigned long long)wall_clock64());
+    if (blockIdx.x == 0 && threadIdx.x == 0) trace_round(a, r, 0u, DCG_ROUND_END);
 }
 
 // ------------------------------------------------------------- plain kernels

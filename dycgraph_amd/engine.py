@@ -88,10 +88,14 @@ MULTI_KERNEL = ("delete_pass", "repair_delete", "repair_add")
 class TuneResult(C.Structure):
     _fields_ = [("blocks_per_cu", C.c_uint32), ("big_row", C.c_uint32), ("piece", C.c_uint32),
                 ("evaluations", C.c_uint32), ("best_ms", C.c_double), ("bucket_quarters", C.c_uint32),
-                ("solo_max", C.c_uint32), ("first_ms", C.c_double)]
+                ("solo_max", C.c_uint32), ("first_ms", C.c_double), ("grid_shift", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+ROUND_REC = np.dtype([("tag", "<u4"), ("round", "<u4"), ("frontier", "<u4"), ("reserved", "<u4"), ("t", "<u8")])
+ROUND_STATIC, ROUND_ADVANCE, ROUND_END, ROUND_SOLO = 1 << 12, 1 << 13, 1 << 14, 1 << 15
 
 
 class TraceStats(C.Structure):
@@ -163,6 +167,8 @@ def lib() -> C.CDLL:
         "dcg_set_batch_maintenance": (I, [P, I]),
         "dcg_set_cpu_domain": (I, [P, P, C.c_size_t]),
         "dcg_set_barrier_timeout": (I, [P, U64]),
+        "dcg_set_round_trace": (I, [P, U32]),
+        "dcg_round_trace": (I, [P, P, C.c_size_t, C.POINTER(C.c_size_t)]),
         "dcg_autotune_last": (I, [P, C.POINTER(TuneResult)]),
         "dcg_set_buckets": (I, [P, U32]),
         "dcg_set_graphs": (I, [P, I]),
@@ -590,6 +596,20 @@ class Engine:
     def set_barrier_timeout(self, microseconds):
         """Grid-barrier patience of the persistent closures (0: give up at the first wait)."""
         self._check(lib().dcg_set_barrier_timeout(self._h, microseconds), "dcg_set_barrier_timeout")
+
+    def set_round_trace(self, capacity):
+        """Record one entry per round of every closure launch (capacity records; 0 off)."""
+        self._check(lib().dcg_set_round_trace(self._h, capacity), "dcg_set_round_trace")
+        self._rt_cap = capacity
+
+    def round_trace(self, cap=1 << 20):
+        """The records since the last call (numpy structured array: tag, round,
+        frontier, t) and the count appended (above len() when some were dropped);
+        clears the trace."""
+        out = np.zeros(cap, ROUND_REC)
+        written = C.c_size_t(0)
+        self._check(lib().dcg_round_trace(self._h, out.ctypes.data, cap, C.byref(written)), "dcg_round_trace")
+        return out[:min(cap, written.value, getattr(self, "_rt_cap", 0))], written.value
 
     def set_batch_maintenance(self, large):
         """True: every batch through the device-wide sort path (CG_BATCH_MAINTENANCE=large)."""

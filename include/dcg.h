@@ -375,13 +375,14 @@ typedef struct {
     uint32_t bucket_quarters;
     uint32_t solo_max;        /* frontiers this small run on one workgroup */
     double first_ms;          /* the parameters in use before tuning */
+    uint32_t grid_shift;      /* closure grids span (workgroups per CU * CUs) >> grid_shift */
 } dcg_tune_result;
 int dcg_autotune(dcg_engine *eng, dcg_tune_result *out);
 /* Autotune on the incremental workload itself: each of the next `stages`
  * repair stages (dcg_del_edge / dcg_add_edge; not PageRank) runs its repair
  * once per candidate, every run from the same post-mutation state, and keeps
  * the fastest parameters (coordinate descent over bucket width, workgroups per
- * CU, solo-frontier threshold, large-row threshold and piece size; each run
+ * CU, grid size, solo-frontier threshold, large-row threshold and piece size; each run
  * timed with HIP events).  Values are those of any single run.  The stage's
  * timing records include the extra runs, so tune outside a measured window.
  * dcg_autotune_last reports the last tuned stage (evaluations 0 if none). */
@@ -408,6 +409,26 @@ int dcg_set_barrier_timeout(dcg_engine *eng, uint64_t microseconds);
  * LDS sort and larger ones with the device-wide radix sort; 1 ("large") sends
  * every batch through the device-wide sort.  Values and rows never depend on it. */
 int dcg_set_batch_maintenance(dcg_engine *eng, int large);
+/* Closure round trace (a diagnostic with no reference counterpart): with
+ * capacity > 0, workgroup 0 of every persistent closure launch appends one
+ * record at the start of each round (and one per bucket advance and one when it
+ * leaves the launch) until `capacity` records are held; 0 turns it off.  tag =
+ * control-block index | mode << 8 (0 push, 1 pull, 2 invalidate, 3 PageRank
+ * push, 4 PageRank amend) | DCG_ROUND_* flags; frontier = entries queued for the
+ * round; t = device wall clock (100 MHz). */
+typedef struct {
+    uint32_t tag, round, frontier, reserved;
+    uint64_t t;
+} dcg_round_rec;
+#define DCG_ROUND_STATIC  (1u << 12) /* static convergence (dcg_start, dcg_autotune) */
+#define DCG_ROUND_ADVANCE (1u << 13) /* a bucket advance refilled the frontier */
+#define DCG_ROUND_END     (1u << 14) /* workgroup 0 left the launch */
+#define DCG_ROUND_SOLO    (1u << 15) /* the round ran on workgroup 0 alone */
+int dcg_set_round_trace(dcg_engine *eng, uint32_t capacity);
+/* Copies up to `cap` records (oldest first) into out, sets *written to the count
+ * appended since the last call (more than the capacity when some were
+ * dropped) and clears the trace. */
+int dcg_round_trace(dcg_engine *eng, dcg_round_rec *out, size_t cap, size_t *written);
 /* Set the tuned parameters directly (0 keeps a value). */
 int dcg_set_tuning(dcg_engine *eng, uint32_t blocks_per_cu, uint32_t big_row, uint32_t piece);
 /* Bucket width of the push closure (ordered repair, ordered_gpu_repair.cuh:116-160,
