@@ -418,13 +418,80 @@ __device__ __forceinline__ void emit_far(const TravArgs &a, bool want, uint32_t 
     emit_to(a, fp ? a.far1 : a.far0, &a.ctl->far_count[fp], want, g);
 }
 
+// A wave's private staging run in LDS: entries wait there and move to a global
+// list in one atomic on the list's counter per flush (a flush when the run
+// could not take another 64 entries, and at the end of the phase), so the
+// counter that every wave of the grid shares sees one atomic per ~200 entries.
+constexpr uint32_t kStage = 256;
+struct Stage {
+    uint32_t *buf;   // kStage entries of this wave's LDS
+    uint32_t n;      // entries waiting (wave-uniform)
+};
+
+// Move the staged entries to list[count...] (and, while invalidating, to the
+// affected list as well).
+template <uint32_t MODE>
+__device__ __forceinline__ void stage_flush(const TravArgs &a, Stage &s, uint32_t *list, uint32_t *count) {
+    if (!s.n) return;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const uint32_t lane = lane_id();
+    uint32_t base = 0, abase = 0;
+    if (lane == 0) {
+        base = atomicAdd(count, s.n);
+        if (MODE == M_INVALIDATE) abase = atomicAdd(&a.ctl->aff_count, s.n);
+    }
+    base = __shfl(base, 0, kWave);
+    if (MODE == M_INVALIDATE) abase = __shfl(abase, 0, kWave);
+    for (uint32_t i = lane; i < s.n; i += kWave) {
+        const uint32_t g = s.buf[i];
+        if (base + i < a.qcap) list[base + i] = g;
+        else a.ctl->overflow = 1;
+        if (MODE == M_INVALIDATE) {
+            if (abase + i < a.affcap) a.aff[abase + i] = g;
+            else a.ctl->overflow = 1;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    s.n = 0;
+}
+
+// Stage the ballot's entries (mask = ballot(want)); flush first when the run
+// could not take them.
+template <uint32_t MODE>
+__device__ __forceinline__ void stage_put(const TravArgs &a, Stage &s, uint64_t mask, bool want, uint32_t g,
+                                          uint32_t *list, uint32_t *count) {
+    if (s.n + kWave > kStage) stage_flush<MODE>(a, s, list, count);
+    if (want) s.buf[s.n + lanes_below(mask)] = g;
+    s.n += (uint32_t)__popcll(mask);
+}
+
+// The far pile that appends go to in this round.
+__device__ __forceinline__ uint32_t *far_pile(const TravArgs &a, uint32_t *&count) {
+    const uint32_t fp = *(volatile uint32_t *)&a.ctl->far_par;
+    count = &a.ctl->far_count[fp];
+    return fp ? a.far1 : a.far0;
+}
+
+// End of a phase: the wave's staged frontier and far-pile entries go out.
+template <uint32_t MODE>
+__device__ __forceinline__ void stage_close(const TravArgs &a, Stage &sn, Stage &sf, uint32_t *out, uint32_t *count) {
+    stage_flush<MODE>(a, sn, out, count);
+    if (sf.n) {
+        uint32_t *fc;
+        uint32_t *pile = far_pile(a, fc);
+        stage_flush<M_PUSH>(a, sf, pile, fc);
+    }
+}
+
 // Workgroup output buffer of a persistent launch: waves append new frontier
 // (and far-pile) entries with LDS atomics and the workgroup publishes them
 // with one global atomic per phase (flush_out), instead of one global atomic
-// per wave and 64-edge chunk on a single counter.  A group that finds the
-// buffer full goes to global memory directly and leaves sentinels behind,
-// which the flush compacts away.
-constexpr uint32_t kOutCap = 4096, kFarCap = 2048;
+// per wave and 64-edge chunk on a single counter.  A ballot that finds the
+// buffer full leaves sentinels behind, which the flush compacts away, and goes
+// to the wave's staging run (flushed by the wave itself).
+constexpr uint32_t kOutCap = 8192, kFarCap = 4096;
 struct OutLds {
     uint32_t n, nf, cn, cf, base, fbase, abase;
     uint32_t near[kOutCap];
@@ -432,8 +499,8 @@ struct OutLds {
 };
 
 template <uint32_t MODE>
-__device__ __forceinline__ void emit_b(OutLds *Op, const TravArgs &a, uint32_t *outq, uint32_t *count, bool want,
-                                       uint32_t g) {
+__device__ __forceinline__ void emit_b(OutLds *Op, Stage &st, const TravArgs &a, uint32_t *outq, uint32_t *count,
+                                       bool want, uint32_t g) {
     if (!Op) {   // workgroup 0 alone: direct appends
         emit<MODE>(a, outq, count, want, g);
         return;
@@ -451,10 +518,10 @@ __device__ __forceinline__ void emit_b(OutLds *Op, const TravArgs &a, uint32_t *
         return;
     }
     if (want && base + r < kOutCap) O.near[base + r] = kInf;
-    emit<MODE>(a, outq, count, want, g);
+    stage_put<MODE>(a, st, mask, want, g, outq, count);
 }
 
-__device__ __forceinline__ void emit_far_b(OutLds *Op, const TravArgs &a, bool want, uint32_t g) {
+__device__ __forceinline__ void emit_far_b(OutLds *Op, Stage &st, const TravArgs &a, bool want, uint32_t g) {
     if (!Op) {
         emit_far(a, want, g);
         return;
@@ -472,7 +539,9 @@ __device__ __forceinline__ void emit_far_b(OutLds *Op, const TravArgs &a, bool w
         return;
     }
     if (want && base + r < kFarCap) O.far[base + r] = kInf;
-    emit_far(a, want, g);
+    uint32_t *fc;
+    uint32_t *pile = far_pile(a, fc);
+    stage_put<M_PUSH>(a, st, mask, want, g, pile, fc);
 }
 
 // Wave 0 drops the sentinels of an overflowed buffer in place; returns the count.
@@ -538,6 +607,8 @@ struct WaveLds {
     uint32_t owner[64];
     uint32_t q[64];
     const uint32_t *base[64];
+    uint32_t stage_near[kStage];   // overflow staging runs (Stage)
+    uint32_t stage_far[kStage];
 };
 
 // Work split of one group of waves (the whole grid, or workgroup 0 alone).
@@ -568,6 +639,7 @@ __device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L
     uint32_t *out = (r & 1) ? a.qb : a.qa;
     const uint32_t stamp = stamp_base(a) + r;
     const uint32_t cnt = front_count(a, cur);
+    Stage sn{L.stage_near, 0}, sf{L.stage_far, 0};
     uint32_t batch = t.wave, spare = 0;   // the first batch is static; then kClaim per atomic
     for (;;) {
         const uint32_t b = batch * kWave;
@@ -659,8 +731,8 @@ __device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L
             uint32_t og = 0;
             if (!(x1 & kTomb))
                 kind = edge_op<PS, MODE>(a, L.q[lo1], L.owner[lo1], L.val[lo1], x1, stamp, og, preloads<MODE>(), p1);
-            if (MODE != M_PULL) emit_b<MODE>(O, a, out, &a.ctl->count[nxt], kind == 1, og);
-            if (MODE == M_PUSH && a.far0) emit_far_b(O, a, kind == 2, og);
+            if (MODE != M_PULL) emit_b<MODE>(O, sn, a, out, &a.ctl->count[nxt], kind == 1, og);
+            if (MODE == M_PUSH && a.far0) emit_far_b(O, sf, a, kind == 2, og);
             x1 = x2;
             lo1 = lo2;
             p1 = p2;
@@ -678,16 +750,18 @@ __device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L
             spare = kClaim - 1;
         }
     }
+    stage_close<MODE>(a, sn, sf, out, &a.ctl->count[nxt]);
 }
 
 // Phase B of round r: the pieces of the large rows (first piece of every wave
 // static, then an atomic cursor).
 template <class PS, uint32_t MODE>
-__device__ void phase_b(const TravArgs &a, uint32_t r, const Team &t, OutLds *O, uint64_t &edges) {
+__device__ void phase_b(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L, OutLds *O, uint64_t &edges) {
     const uint32_t cur = r % 3, nxt = (r + 1) % 3;
     const uint32_t lane = lane_id();
     uint32_t *out = (r & 1) ? a.qb : a.qa;
     const uint32_t stamp = stamp_base(a) + r;
+    Stage sn{L.stage_near, 0}, sf{L.stage_far, 0};
     const unsigned long long bp = *(volatile unsigned long long *)&a.ctl->big[cur];
     uint64_t nb = bp >> kBigShift;
     if (nb > a.bigcap) nb = a.bigcap;
@@ -750,8 +824,8 @@ __device__ void phase_b(const TravArgs &a, uint32_t r, const Team &t, OutLds *O,
                     uint32_t og = 0;
                     if (!(x1 & kTomb))
                         kind = edge_op<PS, MODE>(a, q, v, be.val, x1, stamp, og, preloads<MODE>(), p1);
-                    emit_b<MODE>(O, a, out, &a.ctl->count[nxt], kind == 1, og);
-                    if (MODE == M_PUSH && a.far0) emit_far_b(O, a, kind == 2, og);
+                    emit_b<MODE>(O, sn, a, out, &a.ctl->count[nxt], kind == 1, og);
+                    if (MODE == M_PUSH && a.far0) emit_far_b(O, sf, a, kind == 2, og);
                     x1 = x2;
                     p1 = p2;
                     x2 = x3;
@@ -762,6 +836,7 @@ __device__ void phase_b(const TravArgs &a, uint32_t r, const Team &t, OutLds *O,
         if (lane == 0) np2 = t.nwaves + atomicAdd(&a.ctl->big_cursor[cur], 1u);
         p = __shfl(np2, 0, kWave);
     }
+    stage_close<MODE>(a, sn, sf, out, &a.ctl->count[nxt]);
 }
 
 // Start of round r: ring counters for rounds r+1 / r+2, and PR's consume step.
@@ -872,7 +947,7 @@ __device__ __forceinline__ uint32_t thr_of(uint32_t minv, uint32_t delta) {
 // its smallest far value + its bucket width, entries now below it become round
 // r's frontier and the rest move to the other pile for the next epoch.  Called
 // by the whole grid; returns false when the far pile is empty (closure done).
-__device__ inline bool far_advance(const TravArgs &a, uint32_t r, const Team &t) {
+__device__ inline bool far_advance(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L) {
     const uint32_t fp = *(volatile uint32_t *)&a.ctl->far_par;
     uint32_t fc = *(volatile uint32_t *)&a.ctl->far_count[fp];
     if (fc == 0 || r == 0) return false;
@@ -916,6 +991,10 @@ __device__ inline bool far_advance(const TravArgs &a, uint32_t r, const Team &t)
     }
     uint32_t *in = (r & 1) ? a.qa : a.qb;   // round r's input queue
     const uint32_t lane = lane_id();
+    // both lists through the wave's staging runs: one atomic per ~200 entries
+    // on each list's counter
+    Stage sn{L.stage_near, 0}, sf{L.stage_far, 0};
+    uint32_t *ncount = &a.ctl->count[r % 3], *fcount = &a.ctl->far_count[fp ^ 1];
     for (uint32_t b = t.wave * kWave; b < fc; b += t.nwaves * kWave) {
         const uint32_t i = b + lane;
         bool near = false, far = false;
@@ -928,9 +1007,12 @@ __device__ inline bool far_advance(const TravArgs &a, uint32_t r, const Team &t)
             far = !near;
             if (far) a.fstamp[g] = ep;
         }
-        emit<M_PUSH>(a, in, &a.ctl->count[r % 3], near, g);
-        emit_to(a, dst, &a.ctl->far_count[fp ^ 1], far, g);
+        const uint64_t nm = __ballot(near), fm = __ballot(far);
+        if (nm) stage_put<M_PUSH>(a, sn, nm, near, g, in, ncount);
+        if (fm) stage_put<M_PUSH>(a, sf, fm, far, g, dst, fcount);
     }
+    stage_flush<M_PUSH>(a, sn, in, ncount);
+    stage_flush<M_PUSH>(a, sf, dst, fcount);
     if (t.tid == 0) *(volatile uint32_t *)a.epoch = ep;
     gsync(a);
     return true;
@@ -977,7 +1059,7 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(TravArgs a) {
         if (cnt == 0) {
             // an advance always moves the smallest far entry below the new
             // threshold; two in a row without a round would mean none moved
-            if (MODE == M_PUSH && a.far0 && idle_advances < 2 && far_advance(a, r, gt)) {
+            if (MODE == M_PUSH && a.far0 && idle_advances < 2 && far_advance(a, r, gt, L)) {
                 ++idle_advances;
                 if (blockIdx.x == 0 && threadIdx.x == 0)
                     trace_round(a, r, *(volatile uint32_t *)&a.ctl->count[r % 3], DCG_ROUND_ADVANCE);
@@ -1004,7 +1086,7 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(TravArgs a) {
                         break;
                     }
                     if (np) {
-                        phase_b<PS, MODE>(a, r, bt, nullptr, edges);
+                        phase_b<PS, MODE>(a, r, bt, L, nullptr, edges);
                         __syncthreads();
                     }
                     ++r;
@@ -1019,7 +1101,7 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(TravArgs a) {
             if (!gsync(a)) break;
             r = *(volatile uint32_t *)&a.ctl->solo_round;
             if (*(volatile uint32_t *)&a.ctl->solo_resume_b) {
-                phase_b<PS, MODE>(a, r, gt, &O, edges);
+                phase_b<PS, MODE>(a, r, gt, L, &O, edges);
                 flush(r);
                 if (!gsync(a)) break;
                 ++r;
@@ -1033,7 +1115,7 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(TravArgs a) {
         flush(r);
         if (!gsync(a)) break;
         if (pieces_pending(a, r)) {
-            phase_b<PS, MODE>(a, r, gt, &O, edges);
+            phase_b<PS, MODE>(a, r, gt, L, &O, edges);
             flush(r);
             if (!gsync(a)) break;
         }
