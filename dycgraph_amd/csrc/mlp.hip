@@ -41,20 +41,26 @@ constexpr int P_W1 = 0, P_B1 = P_W1 + kW1, P_W2 = P_B1 + kHid, P_B2 = P_W2 + kW2
 constexpr int kReplay = 4;   // decisions kept for training (ring)
 
 struct MlpState {
-    // Training and the judge run in order on the engine stream, after the
-    // delete mutation (`released`): a kernel on another stream beside the
-    // persistent repair closures could hold CU slots that a closure workgroup
-    // needs, and the closures' grid barrier requires every workgroup resident.
-    // `trained` marks the end of the last training launch; the engine stream
-    // waits on it before a pass reads the weights or a decision is written
-    // into the ring (a no-op on the same stream, kept so the order is explicit).
+    // Training and the judge run on their own stream (default priority)
+    // beside the delete mutation, whose row passes wait on the host link:
+    // training forks from the engine stream when a delete stage starts
+    // (`released`).  No kernel of it may run beside a persistent repair
+    // closure (it could hold CU slots that a closure workgroup needs, and the
+    // closures' grid barrier requires every workgroup resident), so the engine
+    // stream waits on `trained`, the end of the last launch on this stream,
+    // before every closure launch and stage graph (mlp_join), before a pass
+    // reads the weights, and before a decision is written into the ring.
     hipStream_t train = nullptr;
     hipEvent_t trained = nullptr, released = nullptr, labelled = nullptr;
-    bool train_due = false;   // a decision was made; its training launch waits for the delete mutation
+    bool train_due = false;   // a decision was made; its training launch waits for the next delete stage
+    bool joined = true;       // the engine stream has waited for everything issued on `train`
     bool judge_pending = false;   // the last judge's sums are on their way to h_judge
     double *h_judge = nullptr;    // pinned: network top-k, hotness top-k, all reads
     ~MlpState() {
-        if (train) (void)hipStreamSynchronize(train);
+        if (train) {
+            (void)hipStreamSynchronize(train);
+            (void)hipStreamDestroy(train);
+        }
         if (trained) (void)hipEventDestroy(trained);
         if (released) (void)hipEventDestroy(released);
         if (labelled) (void)hipEventDestroy(labelled);
@@ -416,6 +422,7 @@ static void mlp_train_steps(MlpState &m, int steps) {
     }
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipEventRecord(m.trained, m.train));
+    m.joined = false;
 }
 
 void mlp_create(Engine &e) {
@@ -434,7 +441,7 @@ void mlp_create(Engine &e) {
     m->m1.alloc(P_NUM);
     m->m2.alloc(P_NUM);
     m->w16.alloc(P_NUM);
-    m->train = e.stream;
+    DCG_HIP(hipStreamCreateWithFlags(&m->train, hipStreamNonBlocking));
     DCG_HIP(hipEventCreateWithFlags(&m->trained, hipEventDisableTiming));
     DCG_HIP(hipEventCreateWithFlags(&m->released, hipEventDisableTiming));
     DCG_HIP(hipEventCreateWithFlags(&m->labelled, hipEventDisableTiming));
@@ -487,6 +494,7 @@ static void judge_launch(Engine &e, MlpState &m) {
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipMemcpyAsync(m.h_judge, m.judge.p, 3 * sizeof(double), hipMemcpyDeviceToHost, m.train));
     DCG_HIP(hipEventRecord(m.trained, m.train));
+    m.joined = false;
     m.judge_pending = true;
 }
 
@@ -507,7 +515,7 @@ void mlp_observe(Engine &e) {
     if (!m.have_decision) return;
     const uint64_t C = e.cache.chunks;
     // the ring is read by a training launch that may still run
-    DCG_HIP(hipStreamWaitEvent(e.stream, m.trained, 0));
+    mlp_join(e);
     // label the decision's features with what the snapshot actually read
     DCG_HIP(hipMemcpyAsync(m.X.p + (uint64_t)m.slot * C * kIn, m.Xcur.p, C * kIn * 4, hipMemcpyDeviceToDevice,
                            e.stream));
@@ -528,6 +536,12 @@ void mlp_release_training(Engine &e) {
     DCG_HIP(hipStreamWaitEvent(m.train, m.released, 0));
     mlp_train_steps(m, 4);
     m.train_due = false;
+}
+
+void mlp_join(Engine &e) {
+    if (!e.cache.mlp || e.cache.mlp->joined) return;
+    DCG_HIP(hipStreamWaitEvent(e.stream, e.cache.mlp->trained, 0));
+    e.cache.mlp->joined = true;
 }
 
 void mlp_sync(Engine &e) {
@@ -563,7 +577,7 @@ std::vector<uint64_t> mlp_plan(Engine &e, uint64_t ab, uint64_t ac, uint64_t db,
         mlp_train_steps(m, 4);
         m.train_due = false;
     }
-    DCG_HIP(hipStreamWaitEvent(e.stream, m.trained, 0));   // the forward pass reads the trained weights
+    mlp_join(e);   // the forward pass reads the trained weights
     fill<float>(e.stream, e.cache.d_touch.p, C, 0.f);
     const dcg_edge *adds = e.batch_records(Engine::kAdds, ab, ac), *dels = e.batch_records(Engine::kDels, db, dc);
     hipLaunchKernelGGL(k_touch, dim3(grid_for(ac + dc, 256, 8192)), dim3(256), 0, e.stream, e.d_out.p, e.d_in.p,
@@ -589,8 +603,8 @@ std::vector<uint64_t> mlp_plan(Engine &e, uint64_t ab, uint64_t ac, uint64_t db,
     judge_resolve(m);   // the previous decision's judge (the stream waited for it above)
     if (m.rejected >= kQuiet)
         m.next_probe = m.decisions + 1 + std::min(kProbeMax, kProbe << std::min(m.rejected - kQuiet, 4));
-    // online training on the labelled decisions so far overlaps the batch's
-    // repair work (mlp_release_training)
+    // online training on the labelled decisions so far overlaps the next
+    // delete mutation (mlp_release_training)
     m.train_due = true;
     m.decisions++;
     m.fallbacks += m.fallback ? 1 : 0;

@@ -141,6 +141,7 @@ void launch_persistent(Engine &e, uint32_t mode, TravArgs &a, bool is_static = f
         DCG_HIP(hipLaunchKernel(fn, dim3(e.coop_grid[gi]), dim3(kPBlock), args, 0, e.stream));
         return;
     }
+    mlp_join(e);   // no training kernel beside a closure
     const int slot = e.time_begin(k, (int)(a.ctl - e.d_ctl.p));
     const hipError_t err = e.coop_plain ? hipErrorIllegalState
                                         : hipLaunchCooperativeKernel(fn, dim3(e.coop_grid[gi]), dim3(kPBlock), args,
@@ -441,8 +442,8 @@ static double stage_bytes(const Ctl *c, int lo, int hi) {
 // which only drops deleted edges that the seed step already examined).
 void Engine::delete_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
     const dcg_edge *recs = batch_records(kDels, begin, count);
+    mlp_release_training(*this);   // the learned policy trains beside the delete mutation
     mutate_delete(recs, count, st);
-    mlp_release_training(*this);   // the learned policy trains beside the repair, not the mutation
     if (!count) return;
     reserve_stamps(*this, 2);
     h_step[0] = StepArgs{recs, count, {take_stamps(*this), take_stamps(*this), 0u, 0u}};
@@ -581,6 +582,7 @@ void Engine::enqueue_repair(int stage, int part) {
 // first call; a capture the runtime rejects turns graphs off for the engine and
 // the launches are issued directly), or issue them directly.
 void Engine::run_stage(int stage) {
+    mlp_join(*this);   // before the stage's closures (and outside any capture)
     if (!tune.graphs) {
         enqueue_repair(stage);
         return;
