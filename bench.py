@@ -185,7 +185,7 @@ def run_gpu(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, policy, queries
         else:
             if policy == d.CACHE_LEARNED:
                 eng.cache_prefetch(ab, na, db, nd)
-            phases = [eng.del_edge(db, nd), eng.add_edge(ab, na)]
+            phases = list(eng.snapshot(db, nd, ab, na))
         eng.cache_maintain()
         return phases
 

@@ -38,9 +38,10 @@ Engine::Engine(const dcg_config &c) : cfg(c) {
     DCG_HIP(hipHostMalloc((void **)&h_ctl, 8 * sizeof(Ctl), hipHostMallocDefault));
     DCG_HIP(hipHostMalloc((void **)&h_step, 2 * sizeof(StepArgs), hipHostMallocDefault));
     d_step.alloc(2);
-    DCG_HIP(hipHostMalloc((void **)&h_counters, Engine::kCounterSlots * sizeof(unsigned long long), hipHostMallocDefault));
+    DCG_HIP(hipHostMalloc((void **)&h_counters, 2 * Engine::kCounterSlots * sizeof(unsigned long long),
+                          hipHostMallocDefault));
     DCG_HIP(hipHostMalloc((void **)&h_tail, Engine::kTailWords * sizeof(uint64_t), hipHostMallocDefault));
-    DCG_HIP(hipHostMalloc((void **)&h_fold, 2 * sizeof(unsigned long long), hipHostMallocDefault));
+    DCG_HIP(hipHostMalloc((void **)&h_fold, 4 * sizeof(unsigned long long), hipHostMallocDefault));
     app = c.app;
     const bool mixed = c.query_apps && app != DCG_APP_PR;
     Q = app == DCG_APP_PR || (app == DCG_APP_CC && !mixed) ? 1u : std::max(1u, c.num_queries);
@@ -480,6 +481,30 @@ int dcg_add_edge(dcg_engine *h, uint64_t begin, uint64_t count, dcg_phase_stats 
         if (st) {
             st->requests = count;
             st->ms = now_ms() - t0;
+        }
+    });
+}
+
+int dcg_snapshot(dcg_engine *h, uint64_t del_begin, uint64_t del_count, uint64_t add_begin, uint64_t add_count,
+                 dcg_phase_stats *dst, dcg_phase_stats *ast) {
+    return guarded(h, [&](Engine &e) {
+        if (!e.started) throw Error(DCG_ERR_STATE, "dcg_snapshot before dcg_start");
+        if (e.app == DCG_APP_PR) throw Error(DCG_ERR_STATE, "PageRank batches use dcg_update_pagerank");
+        check_slice(del_begin, del_count, e.n_dels);
+        check_slice(add_begin, add_count, e.n_adds);
+        if (dst) *dst = dcg_phase_stats{};
+        if (ast) *ast = dcg_phase_stats{};
+        const double t0 = now_ms();
+        e.snapshot_step(del_begin, del_count, add_begin, add_count, dst, ast);
+        e.collect_timing();
+        const double ms = now_ms() - t0;
+        if (dst) {
+            dst->requests = del_count;
+            dst->ms = ms;
+        }
+        if (ast) {
+            ast->requests = add_count;
+            ast->ms = 0.0;   // the step's time is on the delete record
         }
     });
 }

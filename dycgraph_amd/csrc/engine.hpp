@@ -277,8 +277,9 @@ struct Engine {
     unsigned long long *h_counters = nullptr;
     static constexpr int kTailWords = 3 + 6 * kClasses;   // store tail block (u64 words)
     uint64_t *h_tail = nullptr;                            // pinned copy of it after an add mutation
-    unsigned long long *h_fold = nullptr;                  // pinned: cached / host edge reads of a fold
+    unsigned long long *h_fold = nullptr;                  // pinned: cached / host edge reads of a fold, per stage
     int mut_slot[5] = {-1, -1, -1, -1, -1};                // timing slots of the last delete mutation
+    int add_slot = -1;                                     // timing slot of the last add mutation
 
     // cumulative counters
     uint64_t cum_cached_reads = 0, cum_host_reads = 0;
@@ -323,8 +324,8 @@ struct Engine {
     void export_csr(uint64_t *row_ptr, uint32_t *col, uint64_t *m);
     StoreView view();
     void fold_reads(dcg_phase_stats *st);   // enqueue + synchronise + finish
-    void fold_enqueue();
-    void fold_finish(dcg_phase_stats *st);
+    void fold_enqueue(int stage = 0);   // the readback goes to the stage's pair in h_fold
+    void fold_finish(dcg_phase_stats *st, int stage = 0);
     // the mutations enqueue their kernels and counter readbacks; finish_* read
     // them after the caller's synchronisation
     void finish_delete(uint64_t count, dcg_phase_stats *st);
@@ -335,6 +336,15 @@ struct Engine {
     void start_min_lattice();
     void delete_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st);
     void add_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st);
+    // a stage's launches and readbacks, and its bookkeeping after the wait
+    void delete_enqueue(uint64_t begin, uint64_t count);
+    void delete_finish(uint64_t count, dcg_phase_stats *st);
+    void add_enqueue(uint64_t begin, uint64_t count);
+    void add_finish(uint64_t count, dcg_phase_stats *st);
+    // both stages of a snapshot with one wait
+    void snapshot_step(uint64_t db, uint64_t dc, uint64_t ab, uint64_t ac, dcg_phase_stats *dst,
+                       dcg_phase_stats *ast);
+    int del_repair_slot = -1, add_repair_slot = -1;   // timing slots of the stages' repairs
     uint32_t coop_blocks(const void *kernel);
     void autotune(dcg_tune_result *res);
     void repair_stage(int stage);   // run_stage, or tune_stage while tuning is armed

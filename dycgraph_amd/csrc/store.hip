@@ -1883,9 +1883,12 @@ void Engine::mutate_add(const dcg_edge *recs, uint64_t count, dcg_phase_stats *s
                        d_gstart2.p, big_in, d_counters.p + C_NBIG_IN, reloc_in, d_tail.p, d_pool.p, d_counters.p, 1);
     time_end(slot);
     DCG_HIP(hipGetLastError());
-    DCG_HIP(hipMemcpyAsync(h_counters, d_counters.p, C_NUM * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+    // the add stage's counters have their own pinned mirror: a snapshot step
+    // reads both stages' counters after one wait
+    DCG_HIP(hipMemcpyAsync(h_counters + kCounterSlots, d_counters.p, C_NUM * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, stream));
     DCG_HIP(hipMemcpyAsync(h_tail, d_tail.p, T_NUM * sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
-    mut_slot[0] = slot;
+    add_slot = slot;
 }
 
 // After the add mutation's readbacks landed.  A store that ran out of slots
@@ -1894,22 +1897,23 @@ void Engine::mutate_add(const dcg_edge *recs, uint64_t count, dcg_phase_stats *s
 void Engine::finish_add(uint64_t count, dcg_phase_stats *st) {
     if (!count) return;
     const uint64_t *tl = h_tail;
-    const int slot = mut_slot[0];
+    const unsigned long long *hc = h_counters + kCounterSlots;
+    const int slot = add_slot;
     pool_len = tl[T_RETIRED];
     pool_taken = 0;
     for (int c = 0; c < kClasses; ++c) pool_taken += tl[T_TAKE + c];
     reused_blocks += pool_taken;
-    timed[slot].bytes = 4.0 * (double)(h_counters[C_SLOTS_READ] + h_counters[C_SLOTS_WRITTEN]) +
-                        16.0 * (double)(h_counters[C_GROUPS_OUT] + h_counters[C_GROUPS_IN]);
+    timed[slot].bytes = 4.0 * (double)(hc[C_SLOTS_READ] + hc[C_SLOTS_WRITTEN]) +
+                        16.0 * (double)(hc[C_GROUPS_OUT] + hc[C_GROUPS_IN]);
     if (tl[T_OVERFLOW]) throw Error(DCG_ERR_CAPACITY, "chunk store slot space exhausted (raise store_slack)");
     used_slots = tl[T_TAIL];
     if (st) {
-        st->touched_sources = h_counters[C_GROUPS_OUT];
-        st->changed_sources = h_counters[C_GROUPS_OUT];
-        st->invalid_additions = h_counters[C_INVALID];
-        st->relocated_rows = h_counters[C_RELOCATED];
+        st->touched_sources = hc[C_GROUPS_OUT];
+        st->changed_sources = hc[C_GROUPS_OUT];
+        st->invalid_additions = hc[C_INVALID];
+        st->relocated_rows = hc[C_RELOCATED];
     }
-    num_edges += count - h_counters[C_INVALID];
+    num_edges += count - hc[C_INVALID];
 }
 
 // ---------------------------------------------------------------------- export
@@ -2010,7 +2014,7 @@ __global__ void k_fold_reads(StoreView s, const unsigned long long *nlist_p, flo
     wg_sum_flush(tot, x, dst);
 }
 
-void Engine::fold_enqueue() {
+void Engine::fold_enqueue(int stage) {
     if (cfg.placement == DCG_PLACE_DEVICE) return;
     fill<unsigned long long>(stream, d_counters.p + C_CACHED_READS, 2, 0ull);
     float *reads = cache.d_reads.n ? cache.d_reads.p : nullptr;
@@ -2020,23 +2024,24 @@ void Engine::fold_enqueue() {
                        reads, rows, d_counters.p);
     fill<unsigned long long>(stream, d_hot_count.p, 1, 0ull);
     DCG_HIP(hipGetLastError());
-    DCG_HIP(hipMemcpyAsync(h_fold, d_counters.p + C_CACHED_READS, 2 * sizeof(unsigned long long),
+    DCG_HIP(hipMemcpyAsync(h_fold + 2 * stage, d_counters.p + C_CACHED_READS, 2 * sizeof(unsigned long long),
                            hipMemcpyDeviceToHost, stream));
 }
 
 // After the fold's readback landed (and, for a device store, after the
 // phase's edges_scanned is known: every read is an HBM read).
-void Engine::fold_finish(dcg_phase_stats *st) {
+void Engine::fold_finish(dcg_phase_stats *st, int stage) {
     if (cfg.placement == DCG_PLACE_DEVICE) {
         if (st) st->cached_edge_reads += st->edges_scanned;
         cum_cached_reads += st ? st->edges_scanned : 0;
         return;
     }
-    cum_cached_reads += h_fold[0];
-    cum_host_reads += h_fold[1];
+    const unsigned long long *f = h_fold + 2 * stage;
+    cum_cached_reads += f[0];
+    cum_host_reads += f[1];
     if (st) {
-        st->cached_edge_reads += h_fold[0];
-        st->host_edge_reads += h_fold[1];
+        st->cached_edge_reads += f[0];
+        st->host_edge_reads += f[1];
     }
 }
 

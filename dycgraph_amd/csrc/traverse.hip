@@ -440,28 +440,33 @@ static double stage_bytes(const Ctl *c, int lo, int hi) {
 // Delete stage of one snapshot (framework.cuh:4811-4957, re-ordered: the
 // physical deletion runs first; the invalidation then scans post-delete rows,
 // which only drops deleted edges that the seed step already examined).
-void Engine::delete_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
+// Enqueued in one call and finished after the stream's wait in another, so a
+// snapshot step can enqueue both stages before it waits once.
+void Engine::delete_enqueue(uint64_t begin, uint64_t count) {
     const dcg_edge *recs = batch_records(kDels, begin, count);
     mlp_release_training(*this);   // the learned policy trains beside the delete mutation
-    mutate_delete(recs, count, st);
+    mutate_delete(recs, count, nullptr);
     if (!count) return;
     reserve_stamps(*this, 2);
     h_step[0] = StepArgs{recs, count, {take_stamps(*this), take_stamps(*this), 0u, 0u}};
     DCG_HIP(hipMemcpyAsync(d_step.p, h_step, sizeof(StepArgs), hipMemcpyHostToDevice, stream));
     const int rs = time_begin(DCG_K_REPAIR_DELETE, -1);
+    del_repair_slot = rs;
     if (i16_armed) i16_capture();   // the repair in two parts around the snapshot's captures
     else repair_stage(0);
     time_end(rs);
     mark_batches_used();
-    fold_enqueue();
-    Ctl *c = d_ctl.p;
+    fold_enqueue(0);
     DCG_HIP(hipGetLastError());
-    DCG_HIP(hipMemcpyAsync(h_ctl, c, 4 * sizeof(Ctl), hipMemcpyDeviceToHost, stream));
-    sync();   // one wait per stage: mutation counters, repair control blocks, read fold
+    DCG_HIP(hipMemcpyAsync(h_ctl, d_ctl.p, 4 * sizeof(Ctl), hipMemcpyDeviceToHost, stream));
+}
+
+void Engine::delete_finish(uint64_t count, dcg_phase_stats *st) {
+    if (!count) return;
     finish_delete(count, st);
     for (int i = 0; i < 4; ++i)
         check_launch(h_ctl[i], "the delete stage");
-    timed[rs].bytes = stage_bytes(h_ctl, 1, 4);
+    timed[del_repair_slot].bytes = stage_bytes(h_ctl, 1, 4);
     if (graph_exec[0]) collect_graph_timers(0);
     if (st) {
         st->affected = h_ctl[1].aff_count;
@@ -470,31 +475,41 @@ void Engine::delete_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
         st->far_entries = h_ctl[3].far_scanned;
         st->edges_scanned = h_ctl[1].edges + h_ctl[2].edges + h_ctl[3].edges;
     }
-    fold_finish(st);
+    fold_finish(st, 0);
+}
+
+void Engine::delete_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
+    delete_enqueue(begin, count);
+    if (!count) return;
+    sync();   // one wait per stage: mutation counters, repair control blocks, read fold
+    delete_finish(count, st);
 }
 
 // Add stage (framework.cuh:3525-3756): append, seed from the added edges, close.
-void Engine::add_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
+void Engine::add_enqueue(uint64_t begin, uint64_t count) {
     const dcg_edge *recs = batch_records(kAdds, begin, count);
-    mutate_add(recs, count, st);
+    mutate_add(recs, count, nullptr);
     if (!count) return;
     reserve_stamps(*this, 2);
     h_step[1] = StepArgs{recs, count, {take_stamps(*this), take_stamps(*this), 0u, 0u}};
     DCG_HIP(hipMemcpyAsync(d_step.p + 1, h_step + 1, sizeof(StepArgs), hipMemcpyHostToDevice, stream));
     const int rs = time_begin(DCG_K_REPAIR_ADD, -1);
+    add_repair_slot = rs;
     if (cx.on) coexec_add_stage();   // the closure shared with host threads
     else repair_stage(1);
     time_end(rs);
     mark_batches_used();
-    fold_enqueue();
-    Ctl *c = d_ctl.p;
+    fold_enqueue(1);
     DCG_HIP(hipGetLastError());
-    DCG_HIP(hipMemcpyAsync(h_ctl + 4, c + 4, 2 * sizeof(Ctl), hipMemcpyDeviceToHost, stream));
-    sync();   // one wait per stage: mutation counters, repair control blocks, read fold
+    DCG_HIP(hipMemcpyAsync(h_ctl + 4, d_ctl.p + 4, 2 * sizeof(Ctl), hipMemcpyDeviceToHost, stream));
+}
+
+void Engine::add_finish(uint64_t count, dcg_phase_stats *st) {
+    if (!count) return;
     finish_add(count, st);
     check_launch(h_ctl[4], "the add stage");
     check_launch(h_ctl[5], "the add stage");
-    timed[rs].bytes = stage_bytes(h_ctl, 5, 6);
+    timed[add_repair_slot].bytes = stage_bytes(h_ctl, 5, 6);
     if (graph_exec[1]) collect_graph_timers(1);
     if (st && cx.on) {
         st->coexec_rounds = cx.rounds;
@@ -508,7 +523,33 @@ void Engine::add_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
         st->far_entries = h_ctl[5].far_scanned;
         st->edges_scanned = h_ctl[5].edges;
     }
-    fold_finish(st);
+    fold_finish(st, 1);
+}
+
+void Engine::add_phase(uint64_t begin, uint64_t count, dcg_phase_stats *st) {
+    add_enqueue(begin, count);
+    if (!count) return;
+    sync();   // one wait per stage: mutation counters, repair control blocks, read fold
+    add_finish(count, st);
+}
+
+// One snapshot step: the delete stage and the add stage enqueued back to back
+// and finished after a single wait (the two stages use separate control
+// blocks, step arguments and host mirrors).  The co-executed add stage and the
+// repair-snapshot capture wait on the host inside a stage; with either on the
+// stages run one after the other.
+void Engine::snapshot_step(uint64_t db, uint64_t dc, uint64_t ab, uint64_t ac, dcg_phase_stats *dst,
+                           dcg_phase_stats *ast) {
+    if (cx.on || i16_armed) {
+        delete_phase(db, dc, dst);
+        add_phase(ab, ac, ast);
+        return;
+    }
+    delete_enqueue(db, dc);
+    add_enqueue(ab, ac);
+    sync();
+    delete_finish(dc, dst);
+    add_finish(ac, ast);
 }
 
 // The repair launches of a stage, reading the batch and the round stamps from
@@ -720,7 +761,7 @@ void Engine::tune_stage(int stage) {
     };
     if (ordered) axis(tune.bucket_quarters, {4, 16, 64, 256}, true);
     axis(tune.blocks_per_cu, {1, 2}, false);
-    axis(tune.grid_shift, {1, 2, 3}, false);
+    axis(tune.grid_shift, {1, 2, 3, 5, 8}, false);
     axis(tune.solo_max, {0, 512, 2048, 8192}, false);
     axis(tune.big_row, {256, 1024, 4096}, false);
     axis(tune.piece, {128, 512, 2048}, false);

@@ -143,6 +143,7 @@ def lib() -> C.CDLL:
         "dcg_prefetch_updates": (I, [P, U64, U64, U64, U64]),
         "dcg_del_edge": (I, [P, U64, U64, C.POINTER(PhaseStats)]),
         "dcg_add_edge": (I, [P, U64, U64, C.POINTER(PhaseStats)]),
+        "dcg_snapshot": (I, [P, U64, U64, U64, U64, C.POINTER(PhaseStats), C.POINTER(PhaseStats)]),
         "dcg_update_pagerank": (I, [P, U64, U64, U64, U64, C.POINTER(PhaseStats)]),
         "dcg_cache_maintain": (I, [P, C.POINTER(CacheStats)]),
         "dcg_cache_prefetch": (I, [P, U64, U64, U64, U64, C.POINTER(CacheStats)]),
@@ -492,6 +493,14 @@ class Engine:
         st = PhaseStats()
         self._check(lib().dcg_add_edge(self._h, begin, count, C.byref(st)), "dcg_add_edge")
         return st.as_dict()
+
+    def snapshot(self, del_begin, del_count, add_begin, add_count):
+        """Delete stage then add stage of one snapshot with one host wait; the two
+        stages' statistics (the step's wall time is on the delete record)."""
+        ds, ads = PhaseStats(), PhaseStats()
+        self._check(lib().dcg_snapshot(self._h, del_begin, del_count, add_begin, add_count, C.byref(ds),
+                                       C.byref(ads)), "dcg_snapshot")
+        return ds.as_dict(), ads.as_dict()
 
     def update_pagerank(self, add_begin, add_count, del_begin, del_count):
         st = PhaseStats()

@@ -211,6 +211,14 @@ int dcg_del_edge(dcg_engine *eng, uint64_t begin, uint64_t count, dcg_phase_stat
 /* Add stage: mutation, seeding, closure.  Replaces Engine::add_edge /
  * update_tree_add (framework.cuh:3525-3756). */
 int dcg_add_edge(dcg_engine *eng, uint64_t begin, uint64_t count, dcg_phase_stats *stats);
+/* One snapshot step: the delete stage of [del_begin, del_begin+del_count) and
+ * the add stage of [add_begin, add_begin+add_count), enqueued back to back with
+ * one host wait at the end.  Same results and statistics as dcg_del_edge then
+ * dcg_add_edge (the pair the reference's sample loop calls per snapshot,
+ * samples/hybrid_sssp/hybrid_sssp.cu:446 and :494); an error of either stage
+ * is reported after both ran.  Either stats pointer may be null. */
+int dcg_snapshot(dcg_engine *eng, uint64_t del_begin, uint64_t del_count, uint64_t add_begin, uint64_t add_count,
+                 dcg_phase_stats *del_stats, dcg_phase_stats *add_stats);
 /* PageRank batch (delete+add mutation between the two amend passes, then the
  * residual solve).  Replaces Engine::UpdatePageRank (framework.cuh:2692-2736). */
 int dcg_update_pagerank(dcg_engine *eng, uint64_t add_begin, uint64_t add_count,
