@@ -152,7 +152,9 @@ def engine_config(cfg, n, policy, sources, programs, placement=None):
                           cache_bytes=cfg["cache_bytes"], pr_base=0.15 / n, pr_epsilon=1e-7 / n, pr_max_rounds=100)
 
 
-def run_gpu(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, policy, queries=None, keep_initial=False):
+def run_gpu(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, policy, queries=None, keep_initial=None):
+    """keep_initial: query indices whose (values, parents) are gathered after the
+    warm-up snapshots, where the timed snapshots (and the CPU baseline) start."""
     import dycgraph_amd as d
     pr = cfg["app"] == "pr"
     sources, programs = queries if queries is not None else queries_of(cfg, n, src)
@@ -163,8 +165,6 @@ def run_gpu(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, policy, queries
     t0 = time.time()
     eng.load_graph(n, src, dst)
     eng.start()
-    # the converged state before any snapshot: the CPU baseline starts from it
-    initial = [(eng.gather_value(q), eng.gather_parent(q)) for q in range(eng.queries)] if keep_initial else None
     eng.load_updates(adds, dels)
     eng.sync()
     log(f"load+start {time.time() - t0:.1f}s")
@@ -196,6 +196,8 @@ def run_gpu(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, policy, queries
     for s in range(warmup):
         snapshot(s)
     eng.sync()
+    # the state at the first timed snapshot: the CPU baseline starts from it
+    initial = {q: (eng.gather_value(q), eng.gather_parent(q)) for q in keep_initial} if keep_initial else None
     tune = eng.last_tune() if cfg.get("tune") and not pr and warmup else None
     eng.reset_kernel_stats()
     t_all = time.perf_counter()
@@ -284,13 +286,16 @@ def host_threads():
     return max(1, cores)
 
 
-def cpu_baseline(cfg, n, src, dst, adds, dels, ap, dp, queries, initial, budget_s=20.0):
+def cpu_baseline(cfg, n, src, dst, adds, dels, ap, dp, queries, initial, start, budget_s=20.0):
     """The reference's CPU path on all host cores (oracle/cpu_path.c: grouped row
     mutation by a worker pool, parallel invalidation, pull repair and push
-    closures; the same queries over one store) on the same graph and stream.  It
-    starts from the state the GPU engine converged to before the first snapshot
-    (the same values a from-scratch convergence gives) and applies snapshots from
-    the first one, timed one by one, until ~budget_s of CPU work."""
+    closures; the queries over one store) on the same graph and stream, timed on
+    the same snapshots as the GPU: the store is brought to the first timed
+    snapshot by the warm-up snapshots' mutations alone (untimed), the GPU's state
+    at that point is installed, and snapshots from there are timed one by one
+    until ~budget_s of CPU work.  `initial` maps the query indices that run to
+    their state; with fewer queries than the GPU ran (the grid), the rate is
+    scaled by the query count (the repair dominates and is per query)."""
     import oracle
     codes = {"sssp": oracle.APP_SSSP, "bfs": oracle.APP_BFS, "cc": oracle.APP_CC}
     sources, programs = queries
@@ -299,17 +304,17 @@ def cpu_baseline(cfg, n, src, dst, adds, dels, ap, dp, queries, initial, budget_
     wmax = 1000 if "grid" in cfg else 100
     t0 = time.time()
     cpu = oracle.CpuPath(n, src, dst, threads=threads)
-    for q, (s, p) in enumerate(zip(sources, programs)):
-        k = cpu.add_query(codes[p], s, oracle.W_SEEDED, 11, wmax)
-        if initial is not None:
-            cpu.set_state(k, initial[q][0], initial[q][1])
-        else:
-            cpu.converge(k)
-    build_s = time.time() - t0
     aoff = np.concatenate([[0], np.cumsum(ap)]).astype(np.int64)
     doff = np.concatenate([[0], np.cumsum(dp)]).astype(np.int64)
-    done, spent, upd, times = 0, 0.0, 0, []
-    while done < len(ap) and (done == 0 or spent < budget_s):
+    for s in range(start):
+        cpu.mutate_only(dels[doff[s]:doff[s + 1]], adds[aoff[s]:aoff[s + 1]])
+    sample = sorted(initial)
+    for q in sample:
+        k = cpu.add_query(codes[programs[q]], sources[q], oracle.W_SEEDED, 11, wmax)
+        cpu.set_state(k, initial[q][0], initial[q][1])
+    build_s = time.time() - t0
+    done, spent, upd, times = start, 0.0, 0, []
+    while done < len(ap) and (done == start or spent < budget_s):
         t = time.perf_counter()
         cpu.delete_phase(dels[doff[done]:doff[done + 1]])
         cpu.add_phase(adds[aoff[done]:aoff[done + 1]])
@@ -318,14 +323,46 @@ def cpu_baseline(cfg, n, src, dst, adds, dels, ap, dp, queries, initial, budget_
         spent += dt
         upd += int(ap[done] + dp[done])
         done += 1
-        log(f"cpu snapshot {done}: {dt * 1e3:.1f} ms ({threads} threads)")
+        log(f"cpu snapshot {done - 1}: {dt * 1e3:.1f} ms ({threads} threads, {len(sample)} queries)")
     cpu.close()
-    return dict(value=upd / spent if spent else 0.0, unit="updates/s", cores=threads, kind="port",
-                nproc=os.cpu_count(),
-                sample=f"the same graph and stream: snapshots 1..{done} of this run's stream "
-                       f"({upd} updates; per-snapshot ms {times}), from the GPU's converged state before "
-                       f"snapshot 1; oracle/cpu_path.c on {threads} threads ({'+'.join(programs)} over one "
-                       f"store; store built in {build_s:.1f}s, not timed)")
+    rate = upd / spent if spent else 0.0
+    scale = len(sample) / len(sources)
+    names = "+".join(programs[q] for q in sample)
+    text = (f"the same graph and stream: snapshots {start}..{done - 1} (the GPU's first timed snapshots; "
+            f"{upd} updates; per-snapshot ms {times}), from the GPU's state at snapshot {start} over a store "
+            f"brought there by the earlier snapshots' mutations (untimed; store built in {build_s:.1f}s); "
+            f"oracle/cpu_path.c on {threads} threads, queries {names} over one store")
+    if scale < 1:
+        text += (f"; {len(sample)} of the {len(sources)} queries ran: value = their rate x {len(sample)}/"
+                 f"{len(sources)} (measured {rate:.1f} updates/s for the sample)")
+    return dict(value=rate * scale, unit="updates/s", cores=threads, kind="port", nproc=os.cpu_count(),
+                sample=text)
+
+
+def cpu_single_thread(cfg, n, src, dst, adds, dels, ap, dp, queries, initial, start, budget_s=5.0):
+    """The single-threaded incremental port (oracle.c orc_incr_*) of the one SSSP
+    query on the same snapshots as cpu_baseline."""
+    import oracle
+    sources, _ = queries
+    store = oracle.Store(n, src, dst)
+    aoff = np.concatenate([[0], np.cumsum(ap)]).astype(np.int64)
+    doff = np.concatenate([[0], np.cumsum(dp)]).astype(np.int64)
+    for s in range(start):
+        store.apply(dels[doff[s]:doff[s + 1]], adds[aoff[s]:aoff[s + 1]])
+    rp, col = store.csr()
+    inc = oracle.Incr(n, rp, col, oracle.APP_SSSP, sources[0], oracle.W_SEEDED, 11, 100)
+    inc.set_state(initial[0][0], initial[0][1])
+    done, spent, upd = start, 0.0, 0
+    while done < len(ap) and (done == start or spent < budget_s):
+        t = time.perf_counter()
+        inc.delete_phase(dels[doff[done]:doff[done + 1]])
+        inc.add_phase(adds[aoff[done]:aoff[done + 1]])
+        spent += time.perf_counter() - t
+        upd += int(ap[done] + dp[done])
+        done += 1
+    return {"value": round(upd / spent, 1) if spent else 0.0, "unit": "updates/s", "cores": 1, "kind": "port",
+            "sample": f"oracle.c's single-threaded incremental port, snapshots {start}..{done - 1} from the same "
+                      f"state as cpu_baseline"}
 
 
 def main():
@@ -360,9 +397,13 @@ def main():
     host_lps, hbm_lps = d.calibrate_lines()
     log(f"calibration: HBM copy {hbm_gbs:.0f} GB/s, pinned-host zero-copy read {host_gbs:.1f} GB/s, "
         f"random 64-byte lines: host {host_lps / 1e6:.0f} M/s, HBM {hbm_lps / 1e6:.0f} M/s")
-    # the grid's 65 queries would keep the CPU path busy for minutes per snapshot
-    cpu_wanted = not args.no_cpu and cfg["app"] != "pr" and "grid" not in cfg
-    res = run_gpu(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, cfg["policy"], keep_initial=cpu_wanted)
+    cpu_wanted = not args.no_cpu and cfg["app"] != "pr"
+    nq = len(queries_of(cfg, n, src)[0])
+    # the grid's 65 queries would keep the CPU path busy for minutes per
+    # snapshot: it runs 2 SSSP + 2 BFS queries and the CC query
+    cpu_sample = [0, 1, (nq - 1) // 2, (nq - 1) // 2 + 1, nq - 1] if "grid" in cfg else list(range(nq))
+    res = run_gpu(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, cfg["policy"],
+                  keep_initial=cpu_sample if cpu_wanted else None)
     upd = int(ap[warmup:].sum() + dp[warmup:].sum())
     value = upd / res["total_s"]
     lat = np.array(res["lat"])
@@ -468,9 +509,15 @@ def main():
     res["eng"].close()
     cpu = None
     if cpu_wanted:
-        cpu = cpu_baseline(cfg, n, src, dst, adds, dels, ap, dp, res["queries"], res["initial"])
+        cpu = cpu_baseline(cfg, n, src, dst, adds, dels, ap, dp, res["queries"], res["initial"], warmup)
         out["cpu_baseline"] = {k: (round(v, 1) if isinstance(v, float) else v) for k, v in cpu.items()}
         out["gpu_over_cpu"] = round(value / cpu["value"], 1) if cpu["value"] else None
+        if n <= 1 << 20 and "grid" not in cfg:
+            # small graphs: the single-threaded incremental port beside the
+            # all-core path (its per-level thread overhead dominates tiny batches)
+            st1 = cpu_single_thread(cfg, n, src, dst, adds, dels, ap, dp, res["queries"], res["initial"], warmup)
+            out["cpu_single_thread"] = st1
+            out["gpu_over_cpu_single_thread"] = round(value / st1["value"], 2) if st1["value"] else None
     print(json.dumps(out), flush=True)
 
 

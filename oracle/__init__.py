@@ -98,6 +98,7 @@ def lib():
             "orc_cpu_set_state": (None, [P, I, P, P]),
             "orc_cpu_delete_phase": (U64, [P, P, U64]),
             "orc_cpu_add_phase": (U64, [P, P, U64]),
+            "orc_cpu_mutate_only": (None, [P, P, U64, P, U64]),
             "orc_cpu_values": (None, [P, I, P, P]),
             "orc_cpu_csr": (None, [P, P, P]),
         }
@@ -271,6 +272,11 @@ class CpuPath:
     def add_phase(self, adds):
         e = _edges(adds)
         return lib().orc_cpu_add_phase(self._h, _p(e), len(e))
+
+    def mutate_only(self, dels, adds):
+        """One snapshot's mutations without repair (values go stale until set_state)."""
+        d, a = _edges(dels), _edges(adds)
+        lib().orc_cpu_mutate_only(self._h, _p(d), len(d), _p(a), len(a))
 
     def values(self, q):
         v = np.empty(self.n, np.uint32)

@@ -557,6 +557,15 @@ uint64_t orc_cpu_delete_phase(orc_cpu *c, const orc_edge *dels, uint64_t nd) {
     return missing;
 }
 
+/* The mutations of one snapshot (deletes, then additions) without any repair:
+ * brings the store to a later point of the stream whose values the caller then
+ * installs with orc_cpu_set_state. */
+void orc_cpu_mutate_only(orc_cpu *c, const orc_edge *dels, uint64_t nd, const orc_edge *adds, uint64_t na) {
+    const uint64_t missing = mutate(c, dels, nd, 1);
+    c->m -= nd - missing;
+    c->m += mutate(c, adds, na, 0);
+}
+
 uint64_t orc_cpu_add_phase(orc_cpu *c, const orc_edge *adds, uint64_t na) {
     const uint64_t valid = mutate(c, adds, na, 0);
     c->m += valid;

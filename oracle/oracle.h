@@ -126,6 +126,7 @@ void orc_cpu_set_state(orc_cpu *c, int query, const uint32_t *value, const uint3
 /* Every query; return the missing deletes / the valid additions. */
 uint64_t orc_cpu_delete_phase(orc_cpu *c, const orc_edge *dels, uint64_t nd);
 uint64_t orc_cpu_add_phase(orc_cpu *c, const orc_edge *adds, uint64_t na);
+void orc_cpu_mutate_only(orc_cpu *c, const orc_edge *dels, uint64_t nd, const orc_edge *adds, uint64_t na);
 void orc_cpu_values(const orc_cpu *c, int query, uint32_t *value, uint32_t *parent);
 void orc_cpu_csr(const orc_cpu *c, uint64_t *rp, uint32_t *col);
 

@@ -67,3 +67,45 @@ def test_cpu_path_matches_from_scratch_every_stage():
         store.apply(dels=[], adds=adds)
         check(f"add {s}")
     cpu.close()
+
+
+def test_cpu_path_mutate_only_then_installed_state():
+    """A store brought forward by mutations alone, with the oracle's values of
+    that point installed, repairs the next snapshots exactly (how the timed CPU
+    baseline starts at the first timed snapshot)."""
+    n, m = 2000, 25000
+    src, dst = _rmat_like(n, m, 6)
+    rng = np.random.default_rng(2)
+    cpu = oracle.CpuPath(n, src, dst, threads=4)
+    q = cpu.add_query(oracle.APP_SSSP, 0, oracle.W_SEEDED, seed=3, wmax=50)
+    store = oracle.Store(n, src, dst)
+
+    def batch():
+        rpc, colc = store.csr()
+        srcs = np.repeat(np.arange(n), np.diff(rpc).astype(np.int64))
+        pick = rng.choice(len(colc), 500, replace=False)
+        dels = np.stack([srcs[pick], colc[pick]], 1).astype(np.int64)
+        adds = np.stack([rng.integers(0, n, 600), rng.integers(0, n, 600)], 1)
+        return dels, adds
+
+    for _ in range(3):   # untimed part of the stream: mutations only
+        dels, adds = batch()
+        cpu.mutate_only(dels, adds)
+        store.apply(dels=dels, adds=adds)
+    rp, col = store.csr()
+    crp, ccol = cpu.csr()
+    assert _multiset_rows(n, crp, ccol) == _multiset_rows(n, rp, col)
+    want = oracle.sssp(n, rp, col, 0, oracle.W_SEEDED, 3, 50)
+    ref = oracle.Incr(n, rp, col, oracle.APP_SSSP, 0, oracle.W_SEEDED, 3, 50)
+    ref.converge()
+    _, par = ref.values()
+    cpu.set_state(q, want, par)
+    for s in range(3):
+        dels, adds = batch()
+        cpu.delete_phase(dels)
+        cpu.add_phase(adds)
+        store.apply(dels=dels, adds=adds)
+        rp, col = store.csr()
+        got, _ = cpu.values(q)
+        assert np.array_equal(got, oracle.sssp(n, rp, col, 0, oracle.W_SEEDED, 3, 50)), s
+    cpu.close()
