@@ -69,6 +69,21 @@ def host_link(k, host_lps, delete_pass=None):
             "peak_lines_per_s": round(host_lps), "frac": round(rate / host_lps, 4) if host_lps else None}
 
 
+def hbm_lines(k, hbm_lps):
+    """A repair closure against HBM's random-line ceiling: each expanded row
+    reads its descriptor and its adjacency (at least two 64-byte lines at random
+    places) and each adjacency entry the destination's state slot (one line at a
+    random place), so (2 x expansions + edges) is the fewest random lines the
+    launches could have touched; their rate against the ceiling the same process
+    measured."""
+    if not k.get("edges") or not k["ms"] or not hbm_lps:
+        return None
+    lines = 2 * k["expansions"] + k["edges"]
+    rate = lines / (k["ms"] / 1e3)
+    return {"bound": "random 64-byte accesses to HBM", "lines_per_launch": round(lines / max(k["launches"], 1)),
+            "achieved_lines_per_s": round(rate), "peak_lines_per_s": round(hbm_lps), "frac": round(rate / hbm_lps, 4)}
+
+
 def measured_traffic(config, kernel):
     try:
         with open(TRAFFIC_FILE) as f:
@@ -443,7 +458,8 @@ def main():
                      "traffic": measured_traffic(args.config, kname),
                      "launches": k["launches"], "avg_launch_ms": round(k["ms"] / max(k["launches"], 1), 4),
                      "algo_bytes_per_launch": round(k["algo_bytes"] / max(k["launches"], 1), 1),
-                     "host_link": host_link(k, host_lps, ks.get("delete_pass") if kname == "delete_rows" else None)},
+                     "host_link": host_link(k, host_lps, ks.get("delete_pass") if kname == "delete_rows" else None),
+                     "hbm_lines": hbm_lines(k, hbm_lps)},
         "per_snapshot": res["closure"],
         # parameters the stage autotuner chose on the first warm-up snapshot
         # (best_ms / first_ms: the tuned and the default parameters' repair time
