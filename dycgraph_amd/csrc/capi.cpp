@@ -720,6 +720,16 @@ int dcg_set_tuning(dcg_engine *h, uint32_t blocks_per_cu, uint32_t big_row, uint
     });
 }
 
+int dcg_set_closure_grid(dcg_engine *h, uint32_t grid_shift, uint32_t solo_max) {
+    return guarded(h, [&](Engine &e) {
+        if (grid_shift > 16) throw Error(DCG_ERR_ARG, "grid_shift above 16");
+        e.tune.grid_shift = grid_shift;
+        e.tune.solo_max = solo_max;
+        for (auto &g : e.coop_grid) g = 0;
+        e.drop_graphs();
+    });
+}
+
 int dcg_set_cpu_domain(dcg_engine *h, const uint8_t *cpu_owned, size_t n) {
     return guarded(h, [&](Engine &e) {
         if (!e.loaded) throw Error(DCG_ERR_STATE, "set the CPU domain after the graph load");

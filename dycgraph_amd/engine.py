@@ -164,6 +164,7 @@ def lib() -> C.CDLL:
         "dcg_get_kernel_stats": (I, [P, U32, C.POINTER(KernelStats)]),
         "dcg_autotune": (I, [P, C.POINTER(TuneResult)]),
         "dcg_set_tuning": (I, [P, U32, U32, U32]),
+        "dcg_set_closure_grid": (I, [P, U32, U32]),
         "dcg_autotune_stages": (I, [P, U32]),
         "dcg_set_batch_maintenance": (I, [P, I]),
         "dcg_set_cpu_domain": (I, [P, P, C.c_size_t]),
@@ -640,6 +641,11 @@ class Engine:
             self._check(lib().dcg_set_buckets(self._h, bucket_quarters), "dcg_set_buckets")
         if graphs is not None:
             self._check(lib().dcg_set_graphs(self._h, 1 if graphs else 0), "dcg_set_graphs")
+
+    def set_closure_grid(self, grid_shift, solo_max):
+        """Closures span (workgroups per CU * CUs) >> grid_shift workgroups; frontiers
+        of at most solo_max entries run on one workgroup alone."""
+        self._check(lib().dcg_set_closure_grid(self._h, grid_shift, solo_max), "dcg_set_closure_grid")
 
     def reset_kernel_stats(self):
         self._check(lib().dcg_reset_kernel_stats(self._h), "dcg_reset_kernel_stats")

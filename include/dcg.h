@@ -439,6 +439,11 @@ int dcg_set_round_trace(dcg_engine *eng, uint32_t capacity);
 int dcg_round_trace(dcg_engine *eng, dcg_round_rec *out, size_t cap, size_t *written);
 /* Set the tuned parameters directly (0 keeps a value). */
 int dcg_set_tuning(dcg_engine *eng, uint32_t blocks_per_cu, uint32_t big_row, uint32_t piece);
+/* The closure grid the stage autotuner also picks (dcg_tune_result.grid_shift,
+ * .solo_max): persistent closures span (workgroups per CU * CUs) >> grid_shift
+ * workgroups (at least one; grid_shift <= 16), and frontiers of at most
+ * solo_max entries run on one workgroup alone.  Values never depend on it. */
+int dcg_set_closure_grid(dcg_engine *eng, uint32_t grid_shift, uint32_t solo_max);
 /* Bucket width of the push closure (ordered repair, ordered_gpu_repair.cuh:116-160,
  * used by every SSSP/BFS closure here): SSSP queries w_max*quarters/4, BFS
  * queries `quarters` levels; 0 turns the buckets off (plain frontier order);
