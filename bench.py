@@ -84,7 +84,10 @@ def hbm_lines(k, hbm_lps):
             "achieved_lines_per_s": round(rate), "peak_lines_per_s": round(hbm_lps), "frac": round(rate / hbm_lps, 4)}
 
 
-def measured_traffic(config, kernel):
+def measured_traffic(config, kernel, lines_per_launch=None):
+    """HBM bytes per launch from the committed counter runs: the measured bytes
+    per launch, or, for a closure whose counter runs covered other launches than
+    this run's, their bytes per random line times this run's lines per launch."""
     try:
         with open(TRAFFIC_FILE) as f:
             entries = json.load(f)
@@ -92,6 +95,8 @@ def measured_traffic(config, kernel):
         return None
     for e in entries:
         if e.get("config") == config and e.get("kernel") == kernel:
+            if e.get("bytes_per_line") and lines_per_launch:
+                return round(e["bytes_per_line"] * lines_per_launch)
             return e.get("bytes_per_launch")
     return None
 
@@ -455,7 +460,7 @@ def main():
         "roofline": {"kernel": kname, "hip_kernel": KERNEL_SYMBOL.get(kname, kname), "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(args.config, kname),
+                     "traffic": measured_traffic(args.config, kname, (hbm_lines(k, hbm_lps) or {}).get("lines_per_launch")),
                      "launches": k["launches"], "avg_launch_ms": round(k["ms"] / max(k["launches"], 1), 4),
                      "algo_bytes_per_launch": round(k["algo_bytes"] / max(k["launches"], 1), 1),
                      "host_link": host_link(k, host_lps, ks.get("delete_pass") if kname == "delete_rows" else None),
