@@ -946,23 +946,35 @@ __global__ void k_prd_consume(StoreView s, uint32_t n, uint32_t r, uint32_t max_
     }
 }
 
-// Small in-rows: 16 lanes read a row 16 slots at a time and add the shares.
+// A share of an in-row entry: a tombstone keeps its id under the tombstone bit,
+// so the gather is issued for every entry and the tombstone's share dropped
+// after it (the loads of a lane do not wait on each other).
+__device__ __forceinline__ float share_of(const float *delta, uint32_t x) {
+    const float d = delta[x & ~kTomb];
+    return (x & kTomb) ? 0.f : d;
+}
+
+// Small in-rows: 4 lanes per row (most in-rows of a power-law graph are
+// short); a lane first loads its (at most 8) entries, then gathers their shares.
+constexpr uint32_t kRowLanes = 4;
 __global__ __launch_bounds__(256) void k_prd_pull_small(StoreView s, const uint32_t *rows, const Ctl *c,
                                                         const float *delta, float *res) {
     if (*(volatile const uint32_t *)&c->done) return;
     const uint32_t nrows = c->count[0];
-    const uint32_t lane = lane_id(), ql = lane & 15;
-    const uint64_t quarters = (uint64_t)gridDim.x * blockDim.x / 16;
-    for (uint64_t i = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / 16; i < nrows; i += quarters) {
+    const uint32_t lane = lane_id(), ql = lane & (kRowLanes - 1);
+    const uint64_t groups = (uint64_t)gridDim.x * blockDim.x / kRowLanes;
+    constexpr uint32_t kPer = kDenseSmall / kRowLanes;
+    for (uint64_t i = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kRowLanes; i < nrows; i += groups) {
         const uint32_t v = rows[i];
         const RowDesc d = s.in[v];
         const uint32_t *base = row_base(s, d.off);   // a block of at most 64 slots lies in one chunk
+        uint32_t x[kPer];
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) x[k] = ql + kRowLanes * k < d.ext ? base[ql + kRowLanes * k] : kTomb;
         float sum = 0.f;
-        for (uint32_t j = ql; j < d.ext; j += 16) {
-            const uint32_t x = base[j];
-            if (!(x & kTomb)) sum += delta[x];
-        }
-        for (int o = 8; o >= 1; o >>= 1) sum += __shfl_xor(sum, o, 16);
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) sum += share_of(delta, x[k]);
+        for (int o = kRowLanes / 2; o >= 1; o >>= 1) sum += __shfl_xor(sum, o, kRowLanes);
         if (ql == 0) {
             res[v] = sum;
             mark_read(s, s.hot_in, v);
@@ -980,11 +992,22 @@ __global__ __launch_bounds__(256) void k_prd_pull_big(StoreView s, const uint2 *
     for (uint64_t i = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kWave; i < np; i += nwaves) {
         const uint2 pc = pieces[i];
         const RowDesc d = s.in[pc.x];
-        const uint32_t end = d.ext - pc.y < kDensePiece ? d.ext : pc.y + kDensePiece;
+        const uint32_t len = d.ext - pc.y < kDensePiece ? d.ext - pc.y : kDensePiece;
+        // blocks are aligned to their power-of-two capacity and a piece starts at a
+        // multiple of kDensePiece inside one, so a piece never crosses a chunk
+        const uint32_t *base = row_base(s, d.off + pc.y);
         float sum = 0.f;
-        for (uint32_t j = pc.y + lane; j < end; j += kWave) {
-            const uint32_t x = *row_base(s, d.off + j);
-            if (!(x & kTomb)) sum += delta[x];
+        // eight independent entry loads per lane, then their eight gathers
+        constexpr uint32_t kUnroll = 8;
+        for (uint32_t j0 = 0; j0 < len; j0 += kUnroll * kWave) {
+            uint32_t x[kUnroll];
+#pragma unroll
+            for (uint32_t k = 0; k < kUnroll; ++k) {
+                const uint32_t j = j0 + k * kWave + lane;
+                x[k] = j < len ? base[j] : kTomb;
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < kUnroll; ++k) sum += share_of(delta, x[k]);
         }
         for (int o = 32; o >= 1; o >>= 1) sum += __shfl_xor(sum, o, kWave);
         if (lane == 0) {
@@ -1029,7 +1052,7 @@ void Engine::pagerank_dense(Ctl *c, dcg_phase_stats *st) {
     hipLaunchKernelGGL(k_prd_split, dim3(grid_for(n, 1024, 4096)), dim3(1024), 0, stream, v, n, d_prd_small.p,
                        d_prd_big.p, c);
     const uint32_t max_rounds = cfg.pr_max_rounds;   // 1..100, checked at creation
-    const uint32_t cg = grid_for(n, 256, 4096), sg = grid_for((uint64_t)n * 16, 256, 65536);
+    const uint32_t cg = grid_for(n, 256, 4096), sg = grid_for((uint64_t)n * kRowLanes, 256, 65536);
     const uint32_t rg = grid_for(n, 256, 1024);   // residual maximum: ~16 entries per thread at n = 2^22
     const float eps = (float)cfg.pr_epsilon;
     // rounds in chunks of 16 between host checks; launches past convergence return at once
