@@ -79,7 +79,11 @@ __device__ __forceinline__ void mark_read(const StoreView &s, uint32_t *hot, uin
         s.hot_list[atomicAdd(s.hot_count, 1ull)] = v | (hot == s.hot_in ? 0x80000000u : 0u);
 }
 
+// A device-placed store (no host copy) is its own frame array, chunk c in
+// frame c: its slots resolve without the chunk table, one dependent load less
+// on every row access.
 __device__ __forceinline__ const uint32_t *row_base(const StoreView &s, uint64_t off) {
+    if (!s.host) return s.frames + off;
     const uint32_t c = (uint32_t)(off >> kChunkLog2);
     const int32_t f = s.chunk_frame[c];
     return f >= 0 ? s.frames + (((uint64_t)f) << kChunkLog2) + (off & (kChunkEdges - 1))
@@ -88,6 +92,7 @@ __device__ __forceinline__ const uint32_t *row_base(const StoreView &s, uint64_t
 
 // Frame copy of a slot, or null when its chunk is not resident.
 __device__ __forceinline__ uint32_t *frame_w(const StoreView &s, uint64_t off) {
+    if (!s.host) return s.frames + off;
     const int32_t f = s.chunk_frame[(uint32_t)(off >> kChunkLog2)];
     return f >= 0 ? s.frames + (((uint64_t)f) << kChunkLog2) + (off & (kChunkEdges - 1)) : nullptr;
 }
