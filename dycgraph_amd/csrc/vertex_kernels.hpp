@@ -630,6 +630,43 @@ __device__ __forceinline__ uint32_t front_count(const TravArgs &a, uint32_t slot
     return c < a.qcap ? c : (uint32_t)a.qcap;
 }
 
+// One lane's frontier entry of a batch and its row: the entry, the row's
+// extent (0: nothing to expand) and offset, and the value the row's edges carry.
+struct BatchRow {
+    uint32_t g, deg;
+    uint64_t off, val;
+};
+
+// Load entry i's row (i >= cnt: none).  Issued one batch ahead of its use, so a
+// state word may be older than the round's latest improvement of the vertex:
+// such a vertex was improved in this round and is queued for the next one,
+// which expands it with its new value (the same holds for a word read late).
+template <uint32_t MODE>
+__device__ __forceinline__ BatchRow batch_row(const TravArgs &a, uint32_t i, uint32_t cnt, uint32_t g) {
+    BatchRow b{g, 0u, 0ull, 0ull};
+    if (i >= cnt) return b;
+    if (MODE == M_PR || MODE == M_PR_AMEND) {
+        const RowDesc d = a.s.out[g];
+        const float x = MODE == M_PR ? a.delta[i] : a.rank[g] * a.sign;
+        const uint32_t live = row_live(d);
+        if (live && x != 0.f) {
+            b.deg = d.ext;
+            b.off = d.off;
+            b.val = __float_as_uint(0.85f * x / (float)live);
+        }
+    } else {
+        uint32_t q, v;
+        decode_g(a, g, q, v);
+        const RowDesc d = MODE == M_PULL ? a.s.in[v] : a.s.out[v];
+        b.val = MODE == M_PULL ? 0ull : a.vs[g].st;
+        if (d.ext && (MODE == M_PULL || (uint32_t)(b.val >> 32) != kInf)) {
+            b.deg = d.ext;
+            b.off = d.off;
+        }
+    }
+    return b;
+}
+
 template <class PS, uint32_t MODE>
 __device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L, OutLds *O, uint64_t &edges,
                         uint64_t &expansions) {
@@ -640,40 +677,34 @@ __device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L
     const uint32_t stamp = stamp_base(a) + r;
     const uint32_t cnt = front_count(a, cur);
     Stage sn{L.stage_near, 0}, sf{L.stage_far, 0};
-    uint32_t batch = t.wave, spare = 0;   // the first batch is static; then kClaim per atomic
-    for (;;) {
-        const uint32_t b = batch * kWave;
-        if (b >= cnt) break;
-        const uint32_t i = b + lane;
-        const bool has = i < cnt;
-        const uint32_t g = has ? in[i] : 0u;
-        uint32_t q = 0, v = 0;
-        uint32_t deg = 0;
-        uint64_t val = 0;
-        uint64_t off = 0;
-        if (has) {
-            if (MODE == M_PR || MODE == M_PR_AMEND) {
-                v = g;
-                const RowDesc d = a.s.out[v];
-                const float x = MODE == M_PR ? a.delta[i] : a.rank[v] * a.sign;
-                const uint32_t live = row_live(d);
-                if (live && x != 0.f) {
-                    deg = d.ext;
-                    off = d.off;
-                    val = __float_as_uint(0.85f * x / (float)live);
-                    mark_read(a.s, a.s.hot_out, v);
-                }
-            } else {
-                decode_g(a, g, q, v);
-                const RowDesc d = MODE == M_PULL ? a.s.in[v] : a.s.out[v];
-                val = MODE == M_PULL ? 0ull : a.vs[g].st;
-                if (d.ext && (MODE == M_PULL || (uint32_t)(val >> 32) != kInf)) {
-                    deg = d.ext;
-                    off = d.off;
-                    mark_read(a.s, MODE == M_PULL ? a.s.hot_in : a.s.hot_out, v);
-                }
-            }
+    // the first batch of every wave is static; then kClaim per cursor atomic
+    uint32_t spare = 0;
+    auto claim = [&](uint32_t b) -> uint32_t {
+        if (spare) {
+            --spare;
+            return b + 1;
         }
+        uint32_t nb = 0;
+        if (lane == 0) nb = t.nwaves + atomicAdd(&a.ctl->cursor[cur], kClaim);
+        spare = kClaim - 1;
+        return __shfl(nb, 0, kWave);
+    };
+    // Rows are loaded one batch ahead: the next batch is claimed and its entries
+    // read before this batch's scan, its descriptors and state words before
+    // this batch's edges, so their latency overlaps this batch's work.
+    uint32_t batch = t.wave;
+    const uint32_t i0 = batch * kWave + lane;
+    BatchRow row = batch_row<MODE>(a, i0, cnt, i0 < cnt ? in[i0] : 0u);
+    while (batch * kWave < cnt) {
+        const uint32_t nbatch = claim(batch);
+        const uint32_t ni = nbatch * kWave + lane;
+        const uint32_t ng = ni < cnt ? in[ni] : 0u;
+        const uint32_t g = row.g;
+        uint32_t q = 0, v = g;
+        if (MODE != M_PR && MODE != M_PR_AMEND) decode_g(a, g, q, v);
+        uint32_t deg = row.deg;
+        const uint64_t val = row.val, off = row.off;
+        if (deg) mark_read(a.s, MODE == M_PULL ? a.s.hot_in : a.s.hot_out, v);
         expansions += (uint64_t)__popcll(__ballot(deg > 0));
         if (deg > a.big_row) {
             const uint64_t pieces = (deg + a.piece - 1) / a.piece;
@@ -702,6 +733,7 @@ __device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         edges += total;
+        const BatchRow nrow = batch_row<MODE>(a, ni, cnt, ng);   // in flight during this batch's edges
         // entry k of the wave's edges: its row entry and owner slot in L
         auto fetch_x = [&](uint32_t k, uint32_t &x, uint32_t &lo) {
             x = kTomb;   // no edge (a tombstone is skipped the same way)
@@ -740,15 +772,8 @@ __device__ void phase_a(const TravArgs &a, uint32_t r, const Team &t, WaveLds &L
             lo2 = lo3;
         }
         __builtin_amdgcn_wave_barrier();
-        if (spare) {
-            ++batch;
-            --spare;
-        } else {
-            uint32_t nb = 0;
-            if (lane == 0) nb = t.nwaves + atomicAdd(&a.ctl->cursor[cur], kClaim);
-            batch = __shfl(nb, 0, kWave);
-            spare = kClaim - 1;
-        }
+        batch = nbatch;
+        row = nrow;
     }
     stage_close<MODE>(a, sn, sf, out, &a.ctl->count[nxt]);
 }
