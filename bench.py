@@ -40,7 +40,7 @@ HBM_PEAK_GBS = 8000.0
 _PS = "dcg::ProgramSet<dcg::apps::SSSP, dcg::apps::BFS, dcg::apps::CC>"
 KERNEL_SYMBOL = {"push": f"k_persistent<{_PS}, 0u, 0u>", "pull": f"k_persistent<{_PS}, 1u, 0u>",
                  "invalidate": f"k_persistent<{_PS}, 2u, 0u>",
-                 "pr_solve": f"k_persistent<{_PS}, 3u, 0u> (push) / k_prd_consume + k_prd_pull_small + "
+                 "pr_solve": f"k_persistent<{_PS}, 3u, 0u> (push) / k_prd_consume + k_prd_pull_rows + "
                              "k_prd_pull_big + k_prd_rmax (dense sweeps)",
                  "pr_amend": f"k_persistent<{_PS}, 4u, 0u>", "delete_rows": "k_del_rows", "delete_big": "k_del_big",
                  "append_rows": "k_add_rows + k_add_big",

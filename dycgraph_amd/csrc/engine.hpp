@@ -358,6 +358,7 @@ struct Engine {
     bool dense_pr(uint64_t frontier) const { return frontier * 32 > (uint64_t)n; }
     DevBuf<uint32_t> d_iota;
     DevBuf<uint32_t> d_prd_small;        // dense sweeps: in-rows of at most 64 slots
+    DevBuf<uint32_t> d_prd_medium;       // dense sweeps: in-rows of 65..256 slots
     DevBuf<uint2> d_prd_big;             // dense sweeps: (row, first slot) pieces of larger in-rows
 
     // cache_policy.hip
@@ -391,7 +392,7 @@ struct Engine {
         work = d_temp.bytes() + d_k64a.bytes() + d_k64b.bytes() + d_k64c.bytes() + d_k32a.bytes() +
                d_v32a.bytes() + d_k32c.bytes() + d_v32c.bytes() + d_gstart.bytes() + d_gstart2.bytes() +
                d_uniq.bytes() + d_used.bytes() + d_big_groups.bytes() + d_flags.bytes() + d_counters.bytes() +
-               d_iota.bytes() + d_prd_small.bytes() + d_prd_big.bytes() + mlp_bytes();
+               d_iota.bytes() + d_prd_small.bytes() + d_prd_medium.bytes() + d_prd_big.bytes() + mlp_bytes();
         for (const auto &kind : slots)
             for (const BatchSlot &b : kind) work += b.buf.bytes();
     }

@@ -894,12 +894,14 @@ __global__ void k_iota(uint32_t *q, uint32_t n) {
 // round starts only while the previous round's largest |residual| is above
 // eps and the round cap is not reached (Ctl::done), so launches past the end
 // return at once.
-constexpr uint32_t kDenseSmall = 64;     // in-rows up to this many slots: a quarter-wave each
+constexpr uint32_t kDenseSmall = 64;     // in-rows up to this many slots: 4 lanes each
+constexpr uint32_t kDenseMedium = 256;   // in-rows up to this many slots: 16 lanes each
 constexpr uint32_t kDensePiece = 2048;   // larger in-rows: one wave per piece of this many slots
 
-// In-rows by size: small rows listed, larger rows cut into pieces; the total
-// slot count prices a round (edges).
-__global__ __launch_bounds__(1024) void k_prd_split(StoreView s, uint32_t n, uint32_t *small, uint2 *big, Ctl *c) {
+// In-rows by size: small and medium rows listed (count[0], count[2]), larger
+// rows cut into pieces (count[1]); the total slot count prices a round (edges).
+__global__ __launch_bounds__(1024) void k_prd_split(StoreView s, uint32_t n, uint32_t *small, uint32_t *medium,
+                                                    uint2 *big, Ctl *c) {
     // list positions with one atomic per workgroup and list (one per wave on the
     // two list counters serialised into ~0.9 ms at n = 2^22)
     __shared__ uint32_t wcnt[16], wbase;
@@ -912,9 +914,12 @@ __global__ __launch_bounds__(1024) void k_prd_split(StoreView s, uint32_t n, uin
         const uint32_t ext = v < n ? s.in[v].ext : 0u;
         slots += ext;
         const bool sm = ext && ext <= kDenseSmall;
-        const uint32_t np = ext > kDenseSmall ? (ext + kDensePiece - 1) / kDensePiece : 0u;
+        const bool md = ext > kDenseSmall && ext <= kDenseMedium;
+        const uint32_t np = ext > kDenseMedium ? (ext + kDensePiece - 1) / kDensePiece : 0u;
         const uint32_t ps = wg_reserve(sm ? 1u : 0u, &c->count[0], wcnt, wbase);
         if (sm) small[ps] = v;
+        const uint32_t pm = wg_reserve(md ? 1u : 0u, &c->count[2], wcnt, wbase);
+        if (md) medium[pm] = v;
         const uint32_t pb = wg_reserve(np, &c->count[1], wcnt, wbase);
         for (uint32_t p = 0; p < np; ++p) big[pb + p] = make_uint2(v, p * kDensePiece);
     }
@@ -954,33 +959,36 @@ __device__ __forceinline__ float share_of(const float *delta, uint32_t x) {
     return (x & kTomb) ? 0.f : d;
 }
 
-// Small in-rows: 4 lanes per row (most in-rows of a power-law graph are
-// short); a lane first loads its (at most 8) entries, then gathers their shares.
-constexpr uint32_t kRowLanes = 4;
-__global__ __launch_bounds__(256) void k_prd_pull_small(StoreView s, const uint32_t *rows, const Ctl *c,
-                                                        const float *delta, float *res) {
+// Listed in-rows of at most LANES * PER slots: LANES lanes per row, each of
+// which first loads its (at most PER) entries and then gathers their shares.
+// Most in-rows of a power-law graph are short: rows of at most 64 slots take 4
+// lanes, rows of 65..256 slots take 16 (a block of at most 256 slots lies in
+// one chunk).
+template <uint32_t LANES, uint32_t PER>
+__global__ __launch_bounds__(256) void k_prd_pull_rows(StoreView s, const uint32_t *rows, const uint32_t *count,
+                                                       const Ctl *c, const float *delta, float *res) {
     if (*(volatile const uint32_t *)&c->done) return;
-    const uint32_t nrows = c->count[0];
-    const uint32_t lane = lane_id(), ql = lane & (kRowLanes - 1);
-    const uint64_t groups = (uint64_t)gridDim.x * blockDim.x / kRowLanes;
-    constexpr uint32_t kPer = kDenseSmall / kRowLanes;
-    for (uint64_t i = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kRowLanes; i < nrows; i += groups) {
+    const uint32_t nrows = *count;
+    const uint32_t lane = lane_id(), ql = lane & (LANES - 1);
+    const uint64_t groups = (uint64_t)gridDim.x * blockDim.x / LANES;
+    for (uint64_t i = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / LANES; i < nrows; i += groups) {
         const uint32_t v = rows[i];
         const RowDesc d = s.in[v];
-        const uint32_t *base = row_base(s, d.off);   // a block of at most 64 slots lies in one chunk
-        uint32_t x[kPer];
+        const uint32_t *base = row_base(s, d.off);
+        uint32_t x[PER];
 #pragma unroll
-        for (uint32_t k = 0; k < kPer; ++k) x[k] = ql + kRowLanes * k < d.ext ? base[ql + kRowLanes * k] : kTomb;
+        for (uint32_t k = 0; k < PER; ++k) x[k] = ql + LANES * k < d.ext ? base[ql + LANES * k] : kTomb;
         float sum = 0.f;
 #pragma unroll
-        for (uint32_t k = 0; k < kPer; ++k) sum += share_of(delta, x[k]);
-        for (int o = kRowLanes / 2; o >= 1; o >>= 1) sum += __shfl_xor(sum, o, kRowLanes);
+        for (uint32_t k = 0; k < PER; ++k) sum += share_of(delta, x[k]);
+        for (int o = LANES / 2; o >= 1; o >>= 1) sum += __shfl_xor(sum, o, LANES);
         if (ql == 0) {
             res[v] = sum;
             mark_read(s, s.hot_in, v);
         }
     }
 }
+static_assert(kDenseSmall == 4 * 16 && kDenseMedium == 16 * 16, "row classes match the lanes and loads");
 
 // Pieces of larger in-rows: one wave per piece, one atomic per piece.
 __global__ __launch_bounds__(256) void k_prd_pull_big(StoreView s, const uint2 *pieces, const Ctl *c,
@@ -1042,6 +1050,7 @@ __global__ __launch_bounds__(256) void k_prd_rmax(const float *res, uint32_t n, 
 void Engine::pagerank_dense(Ctl *c, dcg_phase_stats *st) {
     const uint64_t slots_bound = total_slots / kDensePiece + n + 1;
     d_prd_small.ensure(n ? n : 1);
+    d_prd_medium.ensure(n ? n : 1);
     d_prd_big.ensure(slots_bound);
     hipLaunchKernelGGL(k_ctl_setup, dim3(1), dim3(64), 0, stream, c, (const uint32_t *)nullptr, 0u, 0u,
                        (const uint32_t *)nullptr);
@@ -1050,9 +1059,10 @@ void Engine::pagerank_dense(Ctl *c, dcg_phase_stats *st) {
     fill<uint32_t>(stream, c->rmax, 2, 0u);
     const StoreView v = view();
     hipLaunchKernelGGL(k_prd_split, dim3(grid_for(n, 1024, 4096)), dim3(1024), 0, stream, v, n, d_prd_small.p,
-                       d_prd_big.p, c);
+                       d_prd_medium.p, d_prd_big.p, c);
     const uint32_t max_rounds = cfg.pr_max_rounds;   // 1..100, checked at creation
-    const uint32_t cg = grid_for(n, 256, 4096), sg = grid_for((uint64_t)n * kRowLanes, 256, 65536);
+    const uint32_t cg = grid_for(n, 256, 4096), sg = grid_for((uint64_t)n * 4, 256, 65536);
+    const uint32_t mg = grid_for(n, 256, 8192);   // medium rows: 16 lanes each, at most n / 65 of them
     const uint32_t rg = grid_for(n, 256, 1024);   // residual maximum: ~16 entries per thread at n = 2^22
     const float eps = (float)cfg.pr_epsilon;
     // rounds in chunks of 16 between host checks; launches past convergence return at once
@@ -1061,8 +1071,10 @@ void Engine::pagerank_dense(Ctl *c, dcg_phase_stats *st) {
         for (uint32_t r = r0; r < r0 + 16; ++r) {
             hipLaunchKernelGGL(k_prd_consume, dim3(cg), dim3(256), 0, stream, v, n, r, max_rounds, eps, d_rank.p,
                                d_res.p, d_delta.p, d_queued.p, c);
-            hipLaunchKernelGGL(k_prd_pull_small, dim3(sg), dim3(256), 0, stream, v, d_prd_small.p, c, d_delta.p,
-                               d_res.p);
+            hipLaunchKernelGGL((k_prd_pull_rows<4, 16>), dim3(sg), dim3(256), 0, stream, v, d_prd_small.p,
+                               &c->count[0], c, d_delta.p, d_res.p);
+            hipLaunchKernelGGL((k_prd_pull_rows<16, 16>), dim3(mg), dim3(256), 0, stream, v, d_prd_medium.p,
+                               &c->count[2], c, d_delta.p, d_res.p);
             hipLaunchKernelGGL(k_prd_pull_big, dim3(8192), dim3(256), 0, stream, v, d_prd_big.p, c, d_delta.p,
                                d_res.p);
             hipLaunchKernelGGL(k_prd_rmax, dim3(rg), dim3(256), 0, stream, d_res.p, n, r, c);
