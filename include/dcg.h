@@ -442,7 +442,9 @@ int dcg_set_tuning(dcg_engine *eng, uint32_t blocks_per_cu, uint32_t big_row, ui
 /* The closure grid the stage autotuner also picks (dcg_tune_result.grid_shift,
  * .solo_max): persistent closures span (workgroups per CU * CUs) >> grid_shift
  * workgroups (at least one; grid_shift <= 16), and frontiers of at most
- * solo_max entries run on one workgroup alone.  Values never depend on it. */
+ * solo_max entries run on one workgroup alone.  Values never depend on it.
+ * Replaces the reference's fixed launch sizing (KernelSizing,
+ * include/framework/traversal.h:80-90) as dcg_set_tuning does. */
 int dcg_set_closure_grid(dcg_engine *eng, uint32_t grid_shift, uint32_t solo_max);
 /* Bucket width of the push closure (ordered repair, ordered_gpu_repair.cuh:116-160,
  * used by every SSSP/BFS closure here): SSSP queries w_max*quarters/4, BFS
