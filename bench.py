@@ -217,7 +217,13 @@ def run_gpu(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, policy, queries
         snapshot(s)
     eng.sync()
     # the state at the first timed snapshot: the CPU baseline starts from it
-    initial = {q: (eng.gather_value(q), eng.gather_parent(q)) for q in keep_initial} if keep_initial else None
+    # (PageRank: ranks and residuals)
+    if keep_initial and pr:
+        initial = {0: (eng.gather_value(), eng.gather_buffer())}
+    elif keep_initial:
+        initial = {q: (eng.gather_value(q), eng.gather_parent(q)) for q in keep_initial}
+    else:
+        initial = None
     tune = eng.last_tune() if cfg.get("tune") and not pr and warmup else None
     eng.reset_kernel_stats()
     t_all = time.perf_counter()
@@ -359,6 +365,43 @@ def cpu_baseline(cfg, n, src, dst, adds, dels, ap, dp, queries, initial, start, 
                 sample=text)
 
 
+def cpu_baseline_pr(cfg, n, src, dst, adds, dels, ap, dp, initial, start, budget_s=20.0):
+    """PageRank on all host cores (oracle/cpu_path.c orc_cpu_pr_*: the amends
+    around the batch's mutation, then dense Jacobi rounds, the engine's
+    large-batch path) on the same graph and snapshots as the GPU, from the GPU's
+    ranks and residuals at the first timed snapshot over a store brought there
+    by the earlier snapshots' mutations (untimed)."""
+    import oracle
+    threads = host_threads()
+    t0 = time.time()
+    cpu = oracle.CpuPath(n, src, dst, threads=threads)
+    aoff = np.concatenate([[0], np.cumsum(ap)]).astype(np.int64)
+    doff = np.concatenate([[0], np.cumsum(dp)]).astype(np.int64)
+    for s in range(start):
+        cpu.mutate_only(dels[doff[s]:doff[s + 1]], adds[aoff[s]:aoff[s + 1]])
+    cpu.pr_init(0.15 / n, 1e-7 / n, 100)
+    cpu.pr_set_state(initial[0][0], initial[0][1])
+    build_s = time.time() - t0
+    done, spent, upd, times, rounds = start, 0.0, 0, [], []
+    while done < len(ap) and (done == start or spent < budget_s):
+        t = time.perf_counter()
+        rounds.append(cpu.pr_batch(dels[doff[done]:doff[done + 1]], adds[aoff[done]:aoff[done + 1]]))
+        dt = time.perf_counter() - t
+        times.append(round(dt * 1e3, 1))
+        spent += dt
+        upd += int(ap[done] + dp[done])
+        done += 1
+        log(f"cpu snapshot {done - 1}: {dt * 1e3:.1f} ms ({threads} threads, {rounds[-1]} rounds)")
+    cpu.close()
+    return dict(value=upd / spent if spent else 0.0, unit="updates/s", cores=threads, kind="port",
+                nproc=os.cpu_count(),
+                sample=f"the same graph and stream: snapshots {start}..{done - 1} (the GPU's first timed snapshots; "
+                       f"{upd} updates; per-snapshot ms {times}; dense rounds {rounds}), from the GPU's ranks and "
+                       f"residuals at snapshot {start} over a store brought there by the earlier snapshots' "
+                       f"mutations (untimed; store built in {build_s:.1f}s); oracle/cpu_path.c PageRank on "
+                       f"{threads} threads (amends, mutation, dense Jacobi rounds to eps)")
+
+
 def cpu_single_thread(cfg, n, src, dst, adds, dels, ap, dp, queries, initial, start, budget_s=5.0):
     """The single-threaded incremental port (oracle.c orc_incr_*) of the one SSSP
     query on the same snapshots as cpu_baseline."""
@@ -417,7 +460,7 @@ def main():
     host_lps, hbm_lps = d.calibrate_lines()
     log(f"calibration: HBM copy {hbm_gbs:.0f} GB/s, pinned-host zero-copy read {host_gbs:.1f} GB/s, "
         f"random 64-byte lines: host {host_lps / 1e6:.0f} M/s, HBM {hbm_lps / 1e6:.0f} M/s")
-    cpu_wanted = not args.no_cpu and cfg["app"] != "pr"
+    cpu_wanted = not args.no_cpu
     nq = len(queries_of(cfg, n, src)[0])
     # the grid's 65 queries would keep the CPU path busy for minutes per
     # snapshot: it runs 2 SSSP + 2 BFS queries and the CC query
@@ -530,10 +573,13 @@ def main():
     res["eng"].close()
     cpu = None
     if cpu_wanted:
-        cpu = cpu_baseline(cfg, n, src, dst, adds, dels, ap, dp, res["queries"], res["initial"], warmup)
+        if cfg["app"] == "pr":
+            cpu = cpu_baseline_pr(cfg, n, src, dst, adds, dels, ap, dp, res["initial"], warmup)
+        else:
+            cpu = cpu_baseline(cfg, n, src, dst, adds, dels, ap, dp, res["queries"], res["initial"], warmup)
         out["cpu_baseline"] = {k: (round(v, 1) if isinstance(v, float) else v) for k, v in cpu.items()}
         out["gpu_over_cpu"] = round(value / cpu["value"], 1) if cpu["value"] else None
-        if n <= 1 << 20 and "grid" not in cfg:
+        if n <= 1 << 20 and "grid" not in cfg and cfg["app"] != "pr":
             # small graphs: the single-threaded incremental port beside the
             # all-core path (its per-level thread overhead dominates tiny batches)
             st1 = cpu_single_thread(cfg, n, src, dst, adds, dels, ap, dp, res["queries"], res["initial"], warmup)

@@ -99,6 +99,10 @@ def lib():
             "orc_cpu_delete_phase": (U64, [P, P, U64]),
             "orc_cpu_add_phase": (U64, [P, P, U64]),
             "orc_cpu_mutate_only": (None, [P, P, U64, P, U64]),
+            "orc_cpu_pr_init": (None, [P, C.c_double, C.c_double, U32]),
+            "orc_cpu_pr_set_state": (None, [P, P, P]),
+            "orc_cpu_pr_values": (None, [P, P, P]),
+            "orc_cpu_pr_batch": (U32, [P, P, U64, P, U64]),
             "orc_cpu_values": (None, [P, I, P, P]),
             "orc_cpu_csr": (None, [P, P, P]),
         }
@@ -277,6 +281,26 @@ class CpuPath:
         """One snapshot's mutations without repair (values go stale until set_state)."""
         d, a = _edges(dels), _edges(adds)
         lib().orc_cpu_mutate_only(self._h, _p(d), len(d), _p(a), len(a))
+
+    def pr_init(self, base, eps, max_rounds):
+        """PageRank state: rank 0 and residual `base` everywhere (pr_batch of no
+        records then runs the static solve)."""
+        lib().orc_cpu_pr_init(self._h, base, eps, max_rounds)
+
+    def pr_set_state(self, rank, res):
+        lib().orc_cpu_pr_set_state(self._h, _p(np.ascontiguousarray(rank, np.float32)),
+                                   _p(np.ascontiguousarray(res, np.float32)))
+
+    def pr_batch(self, dels, adds):
+        """Amend(-1), mutation, Amend(+1), dense rounds; returns the rounds run."""
+        d, a = _edges(dels), _edges(adds)
+        return lib().orc_cpu_pr_batch(self._h, _p(d), len(d), _p(a), len(a))
+
+    def pr_values(self):
+        r = np.empty(self.n, np.float32)
+        s = np.empty(self.n, np.float32)
+        lib().orc_cpu_pr_values(self._h, _p(r), _p(s))
+        return r, s
 
     def values(self, q):
         v = np.empty(self.n, np.uint32)

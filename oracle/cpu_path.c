@@ -70,6 +70,10 @@ struct orc_cpu {
     uint32_t *front, *next;
     uint64_t *keys, *keys_tmp;   /* mutation: records by row */
     size_t keys_cap;
+    /* PageRank (orc_cpu_pr_*): FP32 ranks, residuals and per-round shares */
+    float *rank, *res, *share;
+    float pr_eps;
+    uint32_t pr_rounds;
 };
 
 static inline uint32_t cap_of(const crow *r) { return r->cap & ~HEAP_BIT; }
@@ -182,6 +186,7 @@ void orc_cpu_destroy(orc_cpu *c) {
     for (int k = 0; k < c->nq; ++k) { free(c->q[k].st); free(c->q[k].stamp); }
     free(c->q); free(c->out_slab); free(c->in_slab); free(c->out); free(c->in);
     free(c->aff_flag); free(c->aff); free(c->front); free(c->next); free(c->keys); free(c->keys_tmp);
+    free(c->rank); free(c->res); free(c->share);
     free(c);
 }
 
@@ -595,3 +600,96 @@ uint64_t orc_cpu_add_phase(orc_cpu *c, const orc_edge *adds, uint64_t na) {
     }
     return valid;
 }
+
+/* ------------------------------------------------------------------ PageRank
+ * The engine's large-batch PageRank path (bench.py's CPU baseline for the
+ * PageRank workload), on all cores: the signed-residual amend of
+ * pr_residual.cuh around the batch's mutation -- Amend(-1) over every touched
+ * source's old out-row, the mutation, Amend(+1) over its new out-row (each a
+ * share 0.85 * rank / out-degree, added with an atomic per entry) -- then dense
+ * Jacobi rounds: every residual is consumed into its rank (pr_residual.cuh:
+ * 53-69) and each vertex pulls its new residual from its in-row, until the
+ * largest |residual| is at most eps or max_rounds rounds ran.  FP32 ranks and
+ * residuals, as on the GPU. */
+void orc_cpu_pr_init(orc_cpu *c, double base, double eps, uint32_t max_rounds) {
+    const size_t nn = c->n ? c->n : 1;
+    free(c->rank); free(c->res); free(c->share);
+    c->rank = (float *)calloc(nn, sizeof(float));
+    c->res = (float *)malloc(nn * sizeof(float));
+    c->share = (float *)calloc(nn, sizeof(float));
+    for (uint32_t v = 0; v < c->n; ++v) c->res[v] = (float)base;   /* rank 0, residual base: the static start */
+    c->pr_eps = (float)eps;
+    c->pr_rounds = max_rounds;
+}
+
+void orc_cpu_pr_set_state(orc_cpu *c, const float *rank, const float *res) {
+    memcpy(c->rank, rank, (size_t)c->n * sizeof(float));
+    memcpy(c->res, res, (size_t)c->n * sizeof(float));
+}
+
+void orc_cpu_pr_values(const orc_cpu *c, float *rank, float *res) {
+    if (rank) memcpy(rank, c->rank, (size_t)c->n * sizeof(float));
+    if (res) memcpy(res, c->res, (size_t)c->n * sizeof(float));
+}
+
+static void pr_amend(orc_cpu *c, size_t nt, float sign) {
+#pragma omp parallel for schedule(dynamic, 64) if (nt > 64)
+    for (size_t i = 0; i < nt; ++i) {
+        const uint32_t u = c->aff[i];
+        const crow *r = &c->out[u];
+        if (!r->deg || c->rank[u] == 0.f) continue;
+        const float x = 0.85f * c->rank[u] * sign / (float)r->deg;
+        for (uint32_t j = 0; j < r->deg; ++j) {
+#pragma omp atomic
+            c->res[r->d[j]] += x;
+        }
+    }
+}
+
+/* One batch; returns the rounds run (the static solve: a batch without records). */
+uint32_t orc_cpu_pr_batch(orc_cpu *c, const orc_edge *dels, uint64_t nd, const orc_edge *adds, uint64_t na) {
+    const uint32_t n = c->n;
+    /* touched sources, each once */
+    size_t nt = 0;
+    for (int k = 0; k < 2; ++k) {
+        const orc_edge *e = k ? adds : dels;
+        const uint64_t ne = k ? na : nd;
+        for (uint64_t i = 0; i < ne; ++i) {
+            const uint32_t u = e[i].u;
+            if (u >= n || e[i].v >= n || c->aff_flag[u]) continue;
+            c->aff_flag[u] = 1;
+            c->aff[nt++] = u;
+        }
+    }
+    for (size_t i = 0; i < nt; ++i) c->aff_flag[c->aff[i]] = 0;
+    pr_amend(c, nt, -1.f);
+    const uint64_t missing = mutate(c, dels, nd, 1);
+    c->m -= nd - missing;
+    c->m += mutate(c, adds, na, 0);
+    pr_amend(c, nt, 1.f);
+    uint32_t r = 0;
+    float rmax = 0.f;
+    for (; r < c->pr_rounds; ++r) {
+        if (r > 0 && !(rmax > c->pr_eps)) break;
+#pragma omp parallel for schedule(static)
+        for (uint32_t v = 0; v < n; ++v) {
+            const float x = c->res[v];
+            c->rank[v] += x;
+            c->res[v] = 0.f;
+            const uint32_t d = c->out[v].deg;
+            c->share[v] = d ? 0.85f * x / (float)d : 0.f;
+        }
+        rmax = 0.f;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(max : rmax)
+        for (uint32_t v = 0; v < n; ++v) {
+            const crow *row = &c->in[v];
+            float s = 0.f;
+            for (uint32_t j = 0; j < row->deg; ++j) s += c->share[row->d[j]];
+            c->res[v] = s;
+            const float a = s < 0.f ? -s : s;
+            if (a > rmax) rmax = a;
+        }
+    }
+    return r;
+}
+

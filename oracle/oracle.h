@@ -127,6 +127,10 @@ void orc_cpu_set_state(orc_cpu *c, int query, const uint32_t *value, const uint3
 uint64_t orc_cpu_delete_phase(orc_cpu *c, const orc_edge *dels, uint64_t nd);
 uint64_t orc_cpu_add_phase(orc_cpu *c, const orc_edge *adds, uint64_t na);
 void orc_cpu_mutate_only(orc_cpu *c, const orc_edge *dels, uint64_t nd, const orc_edge *adds, uint64_t na);
+void orc_cpu_pr_init(orc_cpu *c, double base, double eps, uint32_t max_rounds);
+void orc_cpu_pr_set_state(orc_cpu *c, const float *rank, const float *res);
+void orc_cpu_pr_values(const orc_cpu *c, float *rank, float *res);
+uint32_t orc_cpu_pr_batch(orc_cpu *c, const orc_edge *dels, uint64_t nd, const orc_edge *adds, uint64_t na);
 void orc_cpu_values(const orc_cpu *c, int query, uint32_t *value, uint32_t *parent);
 void orc_cpu_csr(const orc_cpu *c, uint64_t *rp, uint32_t *col);
 

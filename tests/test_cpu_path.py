@@ -109,3 +109,37 @@ def test_cpu_path_mutate_only_then_installed_state():
         got, _ = cpu.values(q)
         assert np.array_equal(got, oracle.sssp(n, rp, col, 0, oracle.W_SEEDED, 3, 50)), s
     cpu.close()
+
+
+def test_cpu_path_pagerank_within_tolerance():
+    """The all-core PageRank path (amends around the mutation, dense rounds):
+    the static solve and three batches each within the FP64 solve's tolerance
+    at the probability scale (max-abs 1e-6 and the checker's L1 bound)."""
+    n, m = 3000, 30000
+    src, dst = _rmat_like(n, m, 11)
+    base, eps = 0.15 / n, 1e-7 / n
+    cpu = oracle.CpuPath(n, src, dst, threads=4)
+    cpu.pr_init(base, eps, 400)
+    store = oracle.Store(n, src, dst)
+    rng = np.random.default_rng(5)
+
+    def check(stage):
+        rp, col = store.csr()
+        rank, res = cpu.pr_values()
+        chk = oracle.check_pagerank(n, rp, col, rank, res, base=base, eps=eps, max_abs=1e-6)
+        assert chk["ok"] and chk["max_error"] <= 1e-6 and chk["l1"] <= chk["bound"], (stage, chk)
+        assert float(np.max(np.abs(res))) <= eps, stage
+
+    rounds = cpu.pr_batch([], [])
+    assert 0 < rounds < 400
+    check("static")
+    for s in range(3):
+        rpc, colc = store.csr()
+        srcs = np.repeat(np.arange(n), np.diff(rpc).astype(np.int64))
+        pick = rng.choice(len(colc), 300, replace=False)
+        dels = np.stack([srcs[pick], colc[pick]], 1).astype(np.int64)
+        adds = np.stack([rng.integers(0, n, 300), rng.integers(0, n, 300)], 1)
+        assert cpu.pr_batch(dels, adds) < 400
+        store.apply(dels=dels, adds=adds)
+        check(f"batch {s}")
+    cpu.close()
