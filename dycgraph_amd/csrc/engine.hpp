@@ -163,6 +163,7 @@ struct Engine {
     hipGraphExec_t graph_exec[2] = {nullptr, nullptr};   // captured repair launches of each stage
     bool capturing = false;
     void run_stage(int stage);
+    bool build_stage_graph(int stage);
     // part 0: the whole stage; for the delete stage, part 1 ends after the
     // reset of the affected set and part 2 runs the rest (pull, push, CC check)
     void enqueue_repair(int stage, int part = 0);

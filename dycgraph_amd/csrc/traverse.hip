@@ -622,12 +622,10 @@ void Engine::enqueue_repair(int stage, int part) {
 // Run a stage's repair launches: replay its captured graph (captured at the
 // first call; a capture the runtime rejects turns graphs off for the engine and
 // the launches are issued directly), or issue them directly.
-void Engine::run_stage(int stage) {
-    mlp_join(*this);   // before the stage's closures (and outside any capture)
-    if (!tune.graphs) {
-        enqueue_repair(stage);
-        return;
-    }
+// Capture the stage's repair launches as a graph unless one is ready; false
+// when capture failed (graphs are then off and the launches go direct).
+bool Engine::build_stage_graph(int stage) {
+    if (!tune.graphs) return false;
     if (!graph_exec[stage]) {
         hipGraph_t g = nullptr;
         bool ok = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
@@ -648,9 +646,17 @@ void Engine::run_stage(int stage) {
             (void)hipGetLastError();
             graph_exec[stage] = nullptr;
             tune.graphs = 0;
-            enqueue_repair(stage);
-            return;
+            return false;
         }
+    }
+    return true;
+}
+
+void Engine::run_stage(int stage) {
+    mlp_join(*this);   // before the stage's closures (and outside any capture)
+    if (!build_stage_graph(stage)) {
+        enqueue_repair(stage);
+        return;
     }
     DCG_HIP(hipGraphLaunch(graph_exec[stage], stream));
 }
@@ -692,12 +698,13 @@ void Engine::repair_stage(int stage) {
 
 // Stage autotuner: the repair of this stage, run per candidate from the same
 // post-mutation state (values, CC parents), each run with fresh round stamps
-// and its own captured graph, timed with HIP events.  A candidate's time is the
-// minimum over its runs (as many as fit ~5 ms, at most 5); coordinate descent
-// keeps a candidate only if it beats the best time so far by 3%, so timing
-// noise does not move the parameters.  Every run reaches the same values (the
-// fixed point does not depend on the parameters), so the state the last run
-// leaves is the stage's result.
+// and fresh round stamps, timed with HIP events around the replay of the stage's
+// graph (captured for each parameter set before the clock starts).  A
+// candidate's time is the minimum over its runs (as many as fit ~5 ms, at most
+// 5); coordinate descent keeps a candidate only if it beats the best time so far
+// by 3%, so timing noise does not move the parameters.  Every run reaches the
+// same values (the fixed point does not depend on the parameters), so the state
+// the last run leaves is the stage's result.
 void Engine::tune_stage(int stage) {
     const uint64_t qn = (uint64_t)n * Q;
     DevBuf<VSlot> vs0;   // state words, stamps and parents after the mutation
@@ -716,8 +723,10 @@ void Engine::tune_stage(int stage) {
         h_step[stage].stamp[0] = take_stamps(*this);
         h_step[stage].stamp[1] = take_stamps(*this);
         DCG_HIP(hipMemcpyAsync(d_step.p + stage, h_step + stage, sizeof(StepArgs), hipMemcpyHostToDevice, stream));
-        for (auto &g : coop_grid) g = 0;
-        drop_graphs();
+        // the graph of the current parameters is captured before the clock
+        // starts: a candidate's time is its launches' device time only
+        mlp_join(*this);
+        build_stage_graph(stage);
         DCG_HIP(hipEventRecord(e0, stream));
         run_stage(stage);
         DCG_HIP(hipEventRecord(e1, stream));
@@ -742,29 +751,65 @@ void Engine::tune_stage(int stage) {
         for (int i = 1; i < reps; ++i) t = std::min(t, run());
         return t;
     };
+    // a parameter change needs new grid sizes and a new graph
+    auto params_changed = [&](bool buckets) {
+        if (buckets) update_buckets();
+        for (auto &g : coop_grid) g = 0;
+        drop_graphs();
+    };
     // one coordinate at a time: try its candidates, keep the best, move on
     auto axis = [&](uint32_t &param, std::initializer_list<uint32_t> values, bool buckets) {
         for (uint32_t v : values) {
             if (v == param) continue;
             const uint32_t old = param;
             param = v;
-            if (buckets) update_buckets();
+            params_changed(buckets);
             const double t = timed_runs();
             ++evals;
             if (t < 0.97 * best) {
                 best = t;
             } else {
                 param = old;
-                if (buckets) update_buckets();
+                params_changed(buckets);
             }
         }
     };
+    const Tune start = tune;
     if (ordered) axis(tune.bucket_quarters, {4, 16, 64, 256}, true);
     axis(tune.blocks_per_cu, {1, 2}, false);
     axis(tune.grid_shift, {1, 2, 3, 5, 8}, false);
     axis(tune.solo_max, {0, 512, 2048, 8192}, false);
     axis(tune.big_row, {256, 1024, 4096}, false);
     axis(tune.piece, {128, 512, 2048}, false);
+    // Coordinate descent compares single runs on long stages, where one run
+    // can differ from the next by more than the 3% margin: the choice is kept
+    // only if it still beats the stage's starting parameters in three
+    // interleaved runs each (best of three).
+    const bool changed = start.bucket_quarters != tune.bucket_quarters || start.blocks_per_cu != tune.blocks_per_cu ||
+                         start.grid_shift != tune.grid_shift || start.solo_max != tune.solo_max ||
+                         start.big_row != tune.big_row || start.piece != tune.piece;
+    if (changed) {
+        const Tune chosen = tune;
+        Tune from = start;
+        from.graphs = tune.graphs;   // graphs stay off if a capture failed meanwhile
+        double t_start = 1e300, t_chosen = 1e300;
+        for (int i = 0; i < 3; ++i) {
+            tune = from;
+            params_changed(ordered);
+            t_start = std::min(t_start, run());
+            tune = chosen;
+            params_changed(ordered);
+            t_chosen = std::min(t_chosen, run());
+        }
+        ++evals;
+        if (t_chosen < 0.97 * t_start) {
+            best = std::min(best, t_chosen);
+        } else {
+            tune = from;
+            params_changed(ordered);
+            best = std::min(first, t_start);
+        }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     for (auto &g : coop_grid) g = 0;
