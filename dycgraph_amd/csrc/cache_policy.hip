@@ -244,7 +244,8 @@ void Engine::cache_maintain(dcg_cache_stats *st) {
     if (st) *st = dcg_cache_stats{};
     snapshots++;
     if (cfg.placement == DCG_PLACE_DEVICE || cache.frames == 0) {
-        fill<float>(stream, cache.d_reads.p, 2 * cache.chunks, 0.f);
+        // a device store tallies no reads (its kernels get no tally array)
+        if (cfg.placement != DCG_PLACE_DEVICE) fill<float>(stream, cache.d_reads.p, 2 * cache.chunks, 0.f);
         fill_common(*this, st);
         if (st) st->ms = now_ms() - t0;
         return;

@@ -715,6 +715,7 @@ int dcg_set_tuning(dcg_engine *h, uint32_t blocks_per_cu, uint32_t big_row, uint
         if (blocks_per_cu) e.tune.blocks_per_cu = blocks_per_cu;
         if (big_row) e.tune.big_row = big_row;
         if (piece) e.tune.piece = piece;
+        e.forget_stage_tunes();
         for (auto &g : e.coop_grid) g = 0;
         e.drop_graphs();
     });
@@ -725,6 +726,7 @@ int dcg_set_closure_grid(dcg_engine *h, uint32_t grid_shift, uint32_t solo_max) 
         if (grid_shift > 16) throw Error(DCG_ERR_ARG, "grid_shift above 16");
         e.tune.grid_shift = grid_shift;
         e.tune.solo_max = solo_max;
+        e.forget_stage_tunes();
         for (auto &g : e.coop_grid) g = 0;
         e.drop_graphs();
     });

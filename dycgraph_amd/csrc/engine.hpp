@@ -95,6 +95,12 @@ struct Engine {
         uint32_t solo_max = 2048;         // frontiers this small run on one workgroup (no grid barriers)
         uint32_t grid_shift = 0;          // persistent grids span (workgroups per CU * CUs) >> grid_shift
     } tune;
+    // A stage the stage tuner has tuned runs its repair closures with its own
+    // grid, solo threshold and row split (bucket widths stay shared: they live
+    // in device memory); everything else uses `tune`.
+    Tune stage_tune[2];
+    bool stage_tuned[2] = {false, false};
+    void forget_stage_tunes() { stage_tuned[0] = stage_tuned[1] = false; }
     bool large_batches = false;           // every batch through the device-wide sort path (no one-workgroup sort)
     uint32_t tune_arm = 0;                // repair stages still to be tuned (tune_stage)
     dcg_tune_result last_tune{};          // outcome of the last tuned stage

@@ -1743,10 +1743,17 @@ void Engine::mutate_delete(const dcg_edge *recs, uint64_t count, dcg_phase_stats
     // go to the first launch of each kernel, so sums and averages hold.  The
     // in-row pass runs on stream2 beside the out-row pass: the two touch
     // disjoint rows and have their own occurrence counters and large-row lists
-    // (shared counters and read tallies are atomics).
+    // (shared counters and read tallies are atomics).  A batch that the
+    // one-workgroup sort took runs both passes on the engine stream: their
+    // kernels take microseconds, less than the fork's and join's cross-stream
+    // waits.
+    const bool fork = count > kSmallSort || large_batches;
+    hipStream_t s_in = fork ? stream2 : stream;
     const int pass = time_begin(DCG_K_DELETE_PASS, -1);
-    DCG_HIP(hipEventRecord(fork_ev, stream));
-    DCG_HIP(hipStreamWaitEvent(stream2, fork_ev, 0));
+    if (fork) {
+        DCG_HIP(hipEventRecord(fork_ev, stream));
+        DCG_HIP(hipStreamWaitEvent(stream2, fork_ev, 0));
+    }
     const int w0 = time_begin(DCG_K_DELETE_ROWS, -1);
     hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream, v, d_out.p, d_sorted_out.p, d_hot_out.p, kout,
                        count, d_gstart.p, d_counters.p + C_GROUPS_OUT, used_out, d_counters.p, 1, big_out,
@@ -1757,18 +1764,20 @@ void Engine::mutate_delete(const dcg_edge *recs, uint64_t count, dcg_phase_stats
                        kout, count, big_out, d_counters.p + C_NBIG, used_out, d_counters.p, 1, d_fbase_out.p,
                        d_fence_out.p);
     time_end(b0);
-    const int w1 = time_begin(DCG_K_DELETE_ROWS, -1, 0, stream2);
-    hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream2, v, d_in.p, d_sorted_in.p, d_hot_in.p, kin,
+    const int w1 = time_begin(DCG_K_DELETE_ROWS, -1, 0, s_in);
+    hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, s_in, v, d_in.p, d_sorted_in.p, d_hot_in.p, kin,
                        count, d_gstart2.p, d_counters.p + C_GROUPS_IN, used_in, d_counters.p, 0, big_in,
                        d_counters.p + C_NBIG_IN, d_fbase_in.p, d_fence_in.p, policy_reads());
     time_end(w1);
-    const int b1 = time_begin(DCG_K_DELETE_BIG, -1, 0, stream2);
-    hipLaunchKernelGGL(k_del_big, dim3(kDelBigGrid), dim3(1024), 0, stream2, v, d_in.p, d_sorted_in.p, d_hot_in.p,
+    const int b1 = time_begin(DCG_K_DELETE_BIG, -1, 0, s_in);
+    hipLaunchKernelGGL(k_del_big, dim3(kDelBigGrid), dim3(1024), 0, s_in, v, d_in.p, d_sorted_in.p, d_hot_in.p,
                        kin, count, big_in, d_counters.p + C_NBIG_IN, used_in, d_counters.p, 0, d_fbase_in.p,
                        d_fence_in.p);
     time_end(b1);
-    DCG_HIP(hipEventRecord(join_ev, stream2));
-    DCG_HIP(hipStreamWaitEvent(stream, join_ev, 0));
+    if (fork) {
+        DCG_HIP(hipEventRecord(join_ev, stream2));
+        DCG_HIP(hipStreamWaitEvent(stream, join_ev, 0));
+    }
     time_end(pass);
     DCG_HIP(hipGetLastError());
     DCG_HIP(hipMemcpyAsync(h_counters, d_counters.p, C_NUM * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));

@@ -320,6 +320,7 @@ void Engine::autotune(dcg_tune_result *res) {
     };
     const dcg_kernel_stats keep = kstat[DCG_K_STATIC];
     kstat[DCG_K_STATIC] = dcg_kernel_stats{};
+    forget_stage_tunes();
     double best = measure();
     const double first = best;
     uint32_t evals = 1;
@@ -652,8 +653,37 @@ bool Engine::build_stage_graph(int stage) {
     return true;
 }
 
+// The stage's own closure parameters (when the stage tuner set them) for the
+// scope's launches and captures; `tune` is restored at the end of the scope.
+struct StageTuneScope {
+    Engine &e;
+    Engine::Tune saved;
+    bool on, grid;
+    StageTuneScope(Engine &en, int stage) : e(en), saved(en.tune), on(en.stage_tuned[stage]), grid(false) {
+        if (!on) return;
+        const Engine::Tune &s = e.stage_tune[stage];
+        grid = s.blocks_per_cu != e.tune.blocks_per_cu || s.grid_shift != e.tune.grid_shift;
+        e.tune.blocks_per_cu = s.blocks_per_cu;
+        e.tune.grid_shift = s.grid_shift;
+        e.tune.solo_max = s.solo_max;
+        e.tune.big_row = s.big_row;
+        e.tune.piece = s.piece;
+        if (grid)
+            for (auto &g : e.coop_grid) g = 0;
+    }
+    ~StageTuneScope() {
+        if (!on) return;
+        const uint32_t graphs = e.tune.graphs;   // a failed capture turned graphs off
+        e.tune = saved;
+        e.tune.graphs = graphs;
+        if (grid)
+            for (auto &g : e.coop_grid) g = 0;
+    }
+};
+
 void Engine::run_stage(int stage) {
     mlp_join(*this);   // before the stage's closures (and outside any capture)
+    StageTuneScope scope(*this, stage);
     if (!build_stage_graph(stage)) {
         enqueue_repair(stage);
         return;
@@ -706,6 +736,20 @@ void Engine::repair_stage(int stage) {
 // same values (the fixed point does not depend on the parameters), so the state
 // the last run leaves is the stage's result.
 void Engine::tune_stage(int stage) {
+    // candidates are set in `tune` itself: the stage's own parameters (if any)
+    // are the starting point, and the choice becomes the stage's parameters
+    const Tune global = tune;
+    if (stage_tuned[stage]) {
+        const Tune &s = stage_tune[stage];
+        tune.blocks_per_cu = s.blocks_per_cu;
+        tune.grid_shift = s.grid_shift;
+        tune.solo_max = s.solo_max;
+        tune.big_row = s.big_row;
+        tune.piece = s.piece;
+        for (auto &g : coop_grid) g = 0;
+        drop_graphs();
+    }
+    stage_tuned[stage] = false;
     const uint64_t qn = (uint64_t)n * Q;
     DevBuf<VSlot> vs0;   // state words, stamps and parents after the mutation
     vs0.alloc(qn);
@@ -812,18 +856,27 @@ void Engine::tune_stage(int stage) {
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    stage_tune[stage] = tune;
+    stage_tuned[stage] = true;
+    // the other closures keep the engine-wide parameters; the bucket width
+    // (device memory) and a graphs-off fallback stay as the tuner left them
+    const uint32_t quarters = tune.bucket_quarters, graphs = tune.graphs;
+    tune = global;
+    tune.bucket_quarters = quarters;
+    tune.graphs = graphs;
     for (auto &g : coop_grid) g = 0;
     drop_graphs();   // the next stage captures its graph with the chosen parameters
+    const Tune &chosen = stage_tune[stage];
     last_tune = dcg_tune_result{};
-    last_tune.blocks_per_cu = tune.blocks_per_cu;
-    last_tune.big_row = tune.big_row;
-    last_tune.piece = tune.piece;
+    last_tune.blocks_per_cu = chosen.blocks_per_cu;
+    last_tune.big_row = chosen.big_row;
+    last_tune.piece = chosen.piece;
     last_tune.evaluations = evals;
     last_tune.best_ms = best;
-    last_tune.bucket_quarters = ordered ? tune.bucket_quarters : 0u;
-    last_tune.solo_max = tune.solo_max;
+    last_tune.bucket_quarters = ordered ? chosen.bucket_quarters : 0u;
+    last_tune.solo_max = chosen.solo_max;
     last_tune.first_ms = first;
-    last_tune.grid_shift = tune.grid_shift;
+    last_tune.grid_shift = chosen.grid_shift;
 }
 
 void Engine::drop_graphs() {
