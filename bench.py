@@ -210,7 +210,8 @@ def run_gpu(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, policy, queries
             if policy == d.CACHE_LEARNED:
                 eng.cache_prefetch(ab, na, db, nd)
             phases = list(eng.snapshot(db, nd, ab, na))
-        eng.cache_maintain()
+        if cfg["placement"] == 1:   # a device-resident store has no chunk cache to maintain
+            eng.cache_maintain()
         return phases
 
     for s in range(warmup):
@@ -228,7 +229,8 @@ def run_gpu(cfg, n, src, dst, adds, dels, ap, dp, warmup, steps, policy, queries
         initial = {q: (eng.gather_value(q), eng.gather_parent(q)) for q in keep_initial}
     else:
         initial = None
-    tune = eng.last_tune() if cfg.get("tune") and not pr and warmup else None
+    tune = ({"delete_stage": eng.stage_tune(0), "add_stage": eng.stage_tune(1)}
+            if cfg.get("tune") and not pr and warmup else None)
     eng.reset_kernel_stats()
     t_all = time.perf_counter()
     for s in range(warmup, warmup + steps):
@@ -513,9 +515,9 @@ def main():
                      "host_link": host_link(k, host_lps, ks.get("delete_pass") if kname == "delete_rows" else None),
                      "hbm_lines": hbm_lines(k, hbm_lps)},
         "per_snapshot": res["closure"],
-        # parameters the stage autotuner chose on the last warm-up snapshot
-        # (best_ms / first_ms: the tuned and the default parameters' repair time
-        # of that snapshot's add stage)
+        # parameters the stage autotuner chose on the last warm-up snapshot, per
+        # stage (best_ms / first_ms: the tuned and the starting parameters'
+        # repair time of that snapshot's delete / add stage)
         "autotune": res.get("tune"),
         # per timed snapshot: each record's time, launches and algorithmic rate
         # (stage records span all of a stage's repair launches; ingest is the

@@ -700,6 +700,14 @@ int dcg_autotune_stages(dcg_engine *h, uint32_t stages) {
     });
 }
 
+int dcg_autotune_stage_last(dcg_engine *h, uint32_t stage, dcg_tune_result *out) {
+    return guarded(h, [&](Engine &e) {
+        if (stage > 1) throw Error(DCG_ERR_ARG, "stage is 0 (delete) or 1 (add)");
+        if (!out) throw Error(DCG_ERR_ARG, "null result");
+        *out = e.stage_result[stage];
+    });
+}
+
 int dcg_autotune_last(dcg_engine *h, dcg_tune_result *out) {
     return guarded(h, [&](Engine &e) {
         if (!out) throw Error(DCG_ERR_ARG, "null result");

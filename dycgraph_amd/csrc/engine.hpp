@@ -100,7 +100,11 @@ struct Engine {
     // in device memory); everything else uses `tune`.
     Tune stage_tune[2];
     bool stage_tuned[2] = {false, false};
-    void forget_stage_tunes() { stage_tuned[0] = stage_tuned[1] = false; }
+    dcg_tune_result stage_result[2]{};   // the last tuning of each stage
+    void forget_stage_tunes() {
+        stage_tuned[0] = stage_tuned[1] = false;
+        stage_result[0] = stage_result[1] = dcg_tune_result{};
+    }
     bool large_batches = false;           // every batch through the device-wide sort path (no one-workgroup sort)
     uint32_t tune_arm = 0;                // repair stages still to be tuned (tune_stage)
     dcg_tune_result last_tune{};          // outcome of the last tuned stage

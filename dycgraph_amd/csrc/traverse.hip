@@ -823,8 +823,8 @@ void Engine::tune_stage(int stage) {
     axis(tune.blocks_per_cu, {1, 2}, false);
     axis(tune.grid_shift, {1, 2, 3, 5, 8}, false);
     axis(tune.solo_max, {0, 512, 2048, 8192}, false);
-    axis(tune.big_row, {256, 1024, 4096}, false);
-    axis(tune.piece, {128, 512, 2048}, false);
+    axis(tune.big_row, {64, 128, 256, 1024, 4096}, false);
+    axis(tune.piece, {64, 128, 512, 2048}, false);
     // Coordinate descent compares single runs on long stages, where one run
     // can differ from the next by more than the 3% margin: the choice is kept
     // only if it still beats the stage's starting parameters in three
@@ -877,6 +877,7 @@ void Engine::tune_stage(int stage) {
     last_tune.solo_max = chosen.solo_max;
     last_tune.first_ms = first;
     last_tune.grid_shift = chosen.grid_shift;
+    stage_result[stage] = last_tune;
 }
 
 void Engine::drop_graphs() {

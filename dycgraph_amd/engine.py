@@ -172,6 +172,7 @@ def lib() -> C.CDLL:
         "dcg_set_round_trace": (I, [P, U32]),
         "dcg_round_trace": (I, [P, P, C.c_size_t, C.POINTER(C.c_size_t)]),
         "dcg_autotune_last": (I, [P, C.POINTER(TuneResult)]),
+        "dcg_autotune_stage_last": (I, [P, U32, C.POINTER(TuneResult)]),
         "dcg_set_buckets": (I, [P, U32]),
         "dcg_set_graphs": (I, [P, I]),
         "dcg_reset_kernel_stats": (I, [P]),
@@ -633,6 +634,12 @@ class Engine:
     def last_tune(self):
         st = TuneResult()
         self._check(lib().dcg_autotune_last(self._h, C.byref(st)), "dcg_autotune_last")
+        return st.as_dict()
+
+    def stage_tune(self, stage):
+        """The last tuning of stage 0 (delete) or 1 (add)."""
+        st = TuneResult()
+        self._check(lib().dcg_autotune_stage_last(self._h, stage, C.byref(st)), "dcg_autotune_stage_last")
         return st.as_dict()
 
     def set_tuning(self, blocks_per_cu=0, big_row=0, piece=0, bucket_quarters=None, graphs=None):

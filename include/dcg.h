@@ -393,9 +393,16 @@ int dcg_autotune(dcg_engine *eng, dcg_tune_result *out);
  * CU, grid size, solo-frontier threshold, large-row threshold and piece size; each run
  * timed with HIP events).  Values are those of any single run.  The stage's
  * timing records include the extra runs, so tune outside a measured window.
- * dcg_autotune_last reports the last tuned stage (evaluations 0 if none). */
+ * Each stage keeps its own choice of grid, solo threshold and row split (the
+ * bucket width is shared).  dcg_autotune_last reports the last tuned stage,
+ * dcg_autotune_stage_last the last tuning of stage 0 (delete) or 1 (add);
+ * evaluations 0 if none (also after dcg_set_tuning / dcg_set_closure_grid /
+ * dcg_autotune, which set the parameters of both stages).  No reference
+ * counterpart: the reference's launch parameters are fixed
+ * (traversal.h:80-90). */
 int dcg_autotune_stages(dcg_engine *eng, uint32_t stages);
 int dcg_autotune_last(dcg_engine *eng, dcg_tune_result *out);
+int dcg_autotune_stage_last(dcg_engine *eng, uint32_t stage, dcg_tune_result *out);
 /* CPU/GPU co-execution of the insertion closure (the reference's owner runtime:
  * --sssp_cpu_partition_capacity / --sssp_cpu_domain_map, framework.cuh:923-1155,
  * 5198-5517, dual_domain_event_runtime.h:81-268).  cpu_owned[v] != 0 gives vertex

@@ -503,6 +503,11 @@ def test_stage_autotune_exact(graph, placement):
             assert t["evaluations"] >= 8, t
             assert t["best_ms"] <= t["first_ms"], t
             assert t["blocks_per_cu"] in (1, 2) and t["solo_max"] in (0, 512, 2048, 8192), t
+            # each stage reports its own choice; the add stage's is the last one
+            assert eng.stage_tune(1) == t, (eng.stage_tune(1), t)
+            td = eng.stage_tune(0)
+            assert td["evaluations"] >= 8 and td["best_ms"] <= td["first_ms"], td
+            assert td["piece"] in (64, 128, 512, 2048) and td["big_row"] in (64, 128, 256, 1024, 4096), td
     eng.close()
 
 
