@@ -128,15 +128,21 @@ const dcg_edge *Engine::batch_records(int kind, uint64_t begin, uint64_t count) 
         else pick = a.stamp <= b.stamp ? &a : &b;
         fill_slot(kind, *pick, begin, count, stream);
     }
+    pick->read = true;
     return pick->buf.p;
 }
 
 // After a call enqueued the readers of its batch slots: their end, which a later
-// copy into the slot waits for.
+// copy into the slot waits for.  Only the slots this call read get the event (an
+// event between launches costs a few microseconds of queue time); the others
+// keep the one of their last readers.
 void Engine::mark_batches_used() {
     for (auto &kind : slots)
         for (BatchSlot &b : kind)
-            if (b.count && !b.pending) DCG_HIP(hipEventRecord(b.used, stream));
+            if (b.read) {
+                DCG_HIP(hipEventRecord(b.used, stream));
+                b.read = false;
+            }
 }
 
 void Engine::prefetch_records(int kind, uint64_t begin, uint64_t count) {

@@ -257,6 +257,7 @@ struct Engine {
         bool pending = false;                // copied on copy_stream; `ready` not yet waited for
         hipEvent_t ready = nullptr;          // end of its copy on copy_stream
         hipEvent_t used = nullptr;           // end of the last work that read it
+        bool read = false;                   // handed out since the last mark_batches_used
         uint64_t stamp = 0;                  // fill order (the older slot is refilled first)
     };
     enum { kDels = 0, kAdds = 1 };

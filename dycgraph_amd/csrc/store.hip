@@ -1749,31 +1749,37 @@ void Engine::mutate_delete(const dcg_edge *recs, uint64_t count, dcg_phase_stats
     // waits.
     const bool fork = count > kSmallSort || large_batches;
     hipStream_t s_in = fork ? stream2 : stream;
+    // such a batch also times its pass as one record: an event between two
+    // dependent launches costs ~5 us of gap, more than those kernels take
+    auto tb = [&](int kernel, hipStream_t on) { return fork ? time_begin(kernel, -1, 0, on) : -1; };
+    auto te = [&](int slot) {
+        if (slot >= 0) time_end(slot);
+    };
     const int pass = time_begin(DCG_K_DELETE_PASS, -1);
     if (fork) {
         DCG_HIP(hipEventRecord(fork_ev, stream));
         DCG_HIP(hipStreamWaitEvent(stream2, fork_ev, 0));
     }
-    const int w0 = time_begin(DCG_K_DELETE_ROWS, -1);
+    const int w0 = tb(DCG_K_DELETE_ROWS, stream);
     hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, stream, v, d_out.p, d_sorted_out.p, d_hot_out.p, kout,
                        count, d_gstart.p, d_counters.p + C_GROUPS_OUT, used_out, d_counters.p, 1, big_out,
                        d_counters.p + C_NBIG, d_fbase_out.p, d_fence_out.p, policy_reads());
-    time_end(w0);
-    const int b0 = time_begin(DCG_K_DELETE_BIG, -1);
+    te(w0);
+    const int b0 = tb(DCG_K_DELETE_BIG, stream);
     hipLaunchKernelGGL(k_del_big, dim3(kDelBigGrid), dim3(1024), 0, stream, v, d_out.p, d_sorted_out.p, d_hot_out.p,
                        kout, count, big_out, d_counters.p + C_NBIG, used_out, d_counters.p, 1, d_fbase_out.p,
                        d_fence_out.p);
-    time_end(b0);
-    const int w1 = time_begin(DCG_K_DELETE_ROWS, -1, 0, s_in);
+    te(b0);
+    const int w1 = tb(DCG_K_DELETE_ROWS, s_in);
     hipLaunchKernelGGL(k_del_rows, dim3(wg), dim3(256), 0, s_in, v, d_in.p, d_sorted_in.p, d_hot_in.p, kin,
                        count, d_gstart2.p, d_counters.p + C_GROUPS_IN, used_in, d_counters.p, 0, big_in,
                        d_counters.p + C_NBIG_IN, d_fbase_in.p, d_fence_in.p, policy_reads());
-    time_end(w1);
-    const int b1 = time_begin(DCG_K_DELETE_BIG, -1, 0, s_in);
+    te(w1);
+    const int b1 = tb(DCG_K_DELETE_BIG, s_in);
     hipLaunchKernelGGL(k_del_big, dim3(kDelBigGrid), dim3(1024), 0, s_in, v, d_in.p, d_sorted_in.p, d_hot_in.p,
                        kin, count, big_in, d_counters.p + C_NBIG_IN, used_in, d_counters.p, 0, d_fbase_in.p,
                        d_fence_in.p);
-    time_end(b1);
+    te(b1);
     if (fork) {
         DCG_HIP(hipEventRecord(join_ev, stream2));
         DCG_HIP(hipStreamWaitEvent(stream, join_ev, 0));
@@ -1790,14 +1796,18 @@ void Engine::mutate_delete(const dcg_edge *recs, uint64_t count, dcg_phase_stats
 
 void Engine::finish_delete(uint64_t count, dcg_phase_stats *st) {
     if (!count) return;
-    timed[mut_slot[0]].bytes =
+    const double wb =
         4.0 * (double)(h_counters[C_W_READ] + h_counters[C_W_WRITTEN]) + 16.0 * (double)h_counters[C_W_ROWS];
-    timed[mut_slot[1]].bytes = 0.0;
-    timed[mut_slot[0]].host_lines = h_counters[C_W_HOST_LINES];
-    timed[mut_slot[2]].bytes = 4.0 * (double)(h_counters[C_SLOTS_READ] + h_counters[C_SLOTS_WRITTEN]) +
-                               16.0 * (double)h_counters[C_B_ROWS];
-    timed[mut_slot[3]].bytes = 0.0;
-    timed[mut_slot[4]].bytes = timed[mut_slot[0]].bytes + timed[mut_slot[2]].bytes;
+    const double bb = 4.0 * (double)(h_counters[C_SLOTS_READ] + h_counters[C_SLOTS_WRITTEN]) +
+                      16.0 * (double)h_counters[C_B_ROWS];
+    if (mut_slot[0] >= 0) {   // per-kernel records (a batch past the one-workgroup sort)
+        timed[mut_slot[0]].bytes = wb;
+        timed[mut_slot[1]].bytes = 0.0;
+        timed[mut_slot[0]].host_lines = h_counters[C_W_HOST_LINES];
+        timed[mut_slot[2]].bytes = bb;
+        timed[mut_slot[3]].bytes = 0.0;
+    }
+    timed[mut_slot[4]].bytes = wb + bb;
     timed[mut_slot[4]].host_lines = h_counters[C_W_HOST_LINES];
     if (st) {
         st->touched_sources = h_counters[C_GROUPS_OUT];
