@@ -179,6 +179,16 @@ void Engine::collect_graph_timers(int stage) {
 }
 
 // Fold recorded launches into kstat (call after the stream has synchronised).
+// Stage calls fold their timing records lazily: an elapsed-time query per
+// record is host time on the snapshot's critical path.  Records priced from
+// their control blocks (closures launched outside graphs) are folded at once,
+// before the next stage reuses the blocks; the statistics calls fold the rest.
+void Engine::collect_timing_lazy() {
+    bool priced = false;
+    for (const Timed &t : timed) priced |= t.ctl >= 0;
+    if (priced || timed.size() >= 256) collect_timing();
+}
+
 void Engine::collect_timing() {
     if (timed.empty()) return;
     // a copy still running on copy_stream (a prefetch) is folded by a later call
@@ -467,7 +477,7 @@ int dcg_del_edge(dcg_engine *h, uint64_t begin, uint64_t count, dcg_phase_stats 
         if (st) *st = dcg_phase_stats{};
         const double t0 = now_ms();
         e.delete_phase(begin, count, st);
-        e.collect_timing();
+        e.collect_timing_lazy();
         if (st) {
             st->requests = count;
             st->ms = now_ms() - t0;
@@ -483,7 +493,7 @@ int dcg_add_edge(dcg_engine *h, uint64_t begin, uint64_t count, dcg_phase_stats 
         if (st) *st = dcg_phase_stats{};
         const double t0 = now_ms();
         e.add_phase(begin, count, st);
-        e.collect_timing();
+        e.collect_timing_lazy();
         if (st) {
             st->requests = count;
             st->ms = now_ms() - t0;
@@ -502,7 +512,7 @@ int dcg_snapshot(dcg_engine *h, uint64_t del_begin, uint64_t del_count, uint64_t
         if (ast) *ast = dcg_phase_stats{};
         const double t0 = now_ms();
         e.snapshot_step(del_begin, del_count, add_begin, add_count, dst, ast);
-        e.collect_timing();
+        e.collect_timing_lazy();
         const double ms = now_ms() - t0;
         if (dst) {
             dst->requests = del_count;
@@ -525,7 +535,7 @@ int dcg_update_pagerank(dcg_engine *h, uint64_t add_begin, uint64_t add_count, u
         if (st) *st = dcg_phase_stats{};
         const double t0 = now_ms();
         e.pagerank_batch(add_begin, add_count, del_begin, del_count, st);
-        e.collect_timing();
+        e.collect_timing_lazy();
         e.fold_reads(st);
         if (st) st->ms = now_ms() - t0;
     });

@@ -311,6 +311,7 @@ struct Engine {
     int time_begin(int kernel, int ctl, double bytes = 0, hipStream_t on = nullptr);   // on: default the engine stream
     void time_end(int slot);
     void collect_timing();
+    void collect_timing_lazy();
     // Closures inside a captured stage graph: each replay stamps the device wall
     // clock at a closure's first workgroup start and last workgroup end in its
     // Ctl block; after the stage's wait the host copy (h_ctl) gives its time and
