@@ -620,9 +620,6 @@ void Engine::enqueue_repair(int stage, int part) {
     }
 }
 
-// Run a stage's repair launches: replay its captured graph (captured at the
-// first call; a capture the runtime rejects turns graphs off for the engine and
-// the launches are issued directly), or issue them directly.
 // Capture the stage's repair launches as a graph unless one is ready; false
 // when capture failed (graphs are then off and the launches go direct).
 bool Engine::build_stage_graph(int stage) {
@@ -681,6 +678,9 @@ struct StageTuneScope {
     }
 };
 
+// Run a stage's repair launches: replay its captured graph (captured at the
+// first call; a capture the runtime rejects turns graphs off for the engine and
+// the launches are issued directly), or issue them directly.
 void Engine::run_stage(int stage) {
     mlp_join(*this);   // before the stage's closures (and outside any capture)
     StageTuneScope scope(*this, stage);
