@@ -488,6 +488,9 @@ def test_stage_autotune_exact(graph, placement):
                 assert chk["invalid_parents"] == 0 and chk["relaxable_edges"] == 0, (stage, q, chk)
 
     assert eng.last_tune()["evaluations"] == 0
+    assert eng.stage_tune(0)["evaluations"] == 0 and eng.stage_tune(1)["evaluations"] == 0
+    with pytest.raises(d.DcgError):
+        eng.stage_tune(2)   # stages are 0 (delete) and 1 (add)
     eng.autotune_stages(3)   # both stages of snapshot 0, the delete stage of snapshot 1
     off = 0
     for s, k in enumerate(sizes):
@@ -508,6 +511,9 @@ def test_stage_autotune_exact(graph, placement):
             td = eng.stage_tune(0)
             assert td["evaluations"] >= 8 and td["best_ms"] <= td["first_ms"], td
             assert td["piece"] in (64, 128, 512, 2048) and td["big_row"] in (64, 128, 256, 1024, 4096), td
+    # explicit closure parameters replace both stages' tuned ones
+    eng.set_closure_grid(0, 2048)
+    assert eng.stage_tune(0)["evaluations"] == 0 and eng.stage_tune(1)["evaluations"] == 0
     eng.close()
 
 
